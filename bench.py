@@ -1,0 +1,238 @@
+"""TopoSZp benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bs 64] [--shape 512 512 512]
+
+One step = the whole topology pipeline over one 512^3 float32 field resident in HBM:
+critical points + per-bin sort positions + sort-position stream + topology stream
+(compress), then the topology stream -> field + types and the sort-position stream
+-> sort positions (decompress).  value = 4 N bytes of original data per step / step
+time (the reference's GB/s convention, example/test_cuda_compress.cu:95-96).
+
+Also reported: per-stage kernel times (HIP events on the launch stream), the roofline
+of the dominant kernel, plain SZp throughput on the same field, and the reference's
+own OpenMP CPU path timed on this host on a bounded slab of the same field.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "toposzp-cuda_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+CPU_SCRIPT = r"""
+import ctypes as C, json, os, sys, time
+import numpy as np
+vp, sz, i32, f32 = C.c_void_p, C.c_size_t, C.c_int, C.c_float
+R = C.CDLL(sys.argv[1])
+libc = C.CDLL("libc.so.6"); libc.free.argtypes = [vp]
+d1, d2, d3, eb, bs = int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]), float(sys.argv[6]), int(sys.argv[7])
+data = np.fromfile(sys.argv[2], dtype=np.float32)
+n = data.size
+P = lambda a: a.ctypes.data_as(vp)
+R.szp_find_critical_points_3d.argtypes = [vp, C.POINTER(sz), i32, i32, i32, f32]; R.szp_find_critical_points_3d.restype = vp
+R.szp_sort_critical_points_3d.argtypes = [vp, sz, vp, i32, i32]
+R.szp_compress_sort_positions_3d.argtypes = [vp, sz, C.POINTER(sz), i32]; R.szp_compress_sort_positions_3d.restype = vp
+R.szp_float_compress_topology_3d.argtypes = [vp, C.POINTER(sz), f32, sz, i32, vp, i32, i32, i32, i32]
+R.szp_float_compress_topology_3d.restype = vp
+R.szp_float_decompress_openmp_threadblock_randomaccess_topology_preserved.argtypes = [C.POINTER(vp), sz, f32, i32, vp, C.POINTER(vp)]
+R.szp_decompress_sort_positions.argtypes = [vp, sz, i32]; R.szp_decompress_sort_positions.restype = vp
+t0 = time.perf_counter()
+cnt = sz()
+cps = R.szp_find_critical_points_3d(P(data), C.byref(cnt), d1, d2, d3, eb)
+R.szp_sort_critical_points_3d(cps, cnt.value, P(data), d2, d3)
+spn = sz()
+sps = R.szp_compress_sort_positions_3d(cps, cnt.value, C.byref(spn), bs)
+tn = sz()
+topo = R.szp_float_compress_topology_3d(P(data), C.byref(tn), eb, n, bs, cps, cnt.value, d1, d2, d3)
+t1 = time.perf_counter()
+out, fn = vp(), vp()
+R.szp_float_decompress_openmp_threadblock_randomaccess_topology_preserved(C.byref(out), n, eb, bs, topo, C.byref(fn))
+ext = C.c_uint64.from_address(sps).value
+spd = R.szp_decompress_sort_positions(sps + 8, ext, bs)
+t2 = time.perf_counter()
+print(json.dumps({"compress_s": t1 - t0, "decompress_s": t2 - t1, "n": n, "cps": cnt.value,
+                  "topo_bytes": tn.value, "sp_bytes": spn.value}))
+"""
+
+
+def cpu_baseline(field_np, eb, bs, threads, slab):
+    """The reference's OpenMP CPU path (oracle/_ref/libszp_ref.so, built from the
+    reference sources) on a bounded slab, in a child process."""
+    import numpy as np
+
+    lib = os.path.join(ROOT, "oracle", "_ref", "libszp_ref.so")
+    if not os.path.exists(lib):
+        return None, "oracle/_ref/libszp_ref.so not built"
+    sub = np.ascontiguousarray(field_np[:slab])
+    path = os.path.join("/tmp", f"tszp_cpu_slab_{os.getpid()}.f32")
+    sub.tofile(path)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    try:
+        r = subprocess.run([sys.executable, "-c", CPU_SCRIPT, lib, path, *map(str, sub.shape), repr(eb),
+                            str(bs)], capture_output=True, text=True, timeout=300, env=env)
+    finally:
+        os.unlink(path)
+    if r.returncode != 0:
+        return None, f"reference CPU run failed rc={r.returncode}: {r.stderr[-300:]}"
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["shape"] = list(sub.shape)
+    return res, None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--bs", type=int, default=64)
+    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 512])
+    ap.add_argument("--seed", type=int, default=20251203)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
+    ap.add_argument("--cpu-slab", type=int, default=64, help="planes of the field for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the sharded design is "
+              f"described in DESIGN.md and is not measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import toposzp_hip as tz
+
+    assert torch.cuda.is_available(), "no GPU"
+    dev = torch.device("cuda:0")
+    shape = tuple(args.shape)
+    n = shape[0] * shape[1] * shape[2]
+    bs = args.bs
+    log(f"generating fbm field {shape}")
+    f = tz.gen_fbm(shape, seed=args.seed, octaves=6, noise=0.05, device=dev)
+    eb, rng = tz.rel_to_abs(f, args.rel)
+    log(f"eb={eb:.6g} (rel {args.rel} of range {rng:.6g}), bs={bs}")
+    buf = tz.TopoBuffers(n, bs, 1, dev)
+
+    def step():
+        info, _ = tz.topo_compress(f, eb, bs, T=1, buf=buf)
+        tz.topo_decompress(buf, n, eb, bs, T=1)
+        if info.ext_count:
+            tz.sortpos_decompress(buf, info.ext_count, bs, T=1)
+        return info
+
+    for i in range(args.warmup):
+        info = step()
+        log(f"warmup {i + 1}/{args.warmup}: cps={info.cp_count} ext={info.ext_count} "
+            f"topo={info.topo_bytes} sp={info.sortpos_bytes}")
+    torch.cuda.synchronize()
+    tz.prof_reset()
+    tz.prof_enable(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        info = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tz.prof_enable(False)
+    prof = tz.prof_read()
+    # a second, unprofiled timing of the same steps (event records add a little work)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        info = step()
+    torch.cuda.synchronize()
+    elapsed = min(elapsed, time.perf_counter() - t0)
+    ms_step = 1e3 * elapsed / args.steps
+    gbs = 4.0 * n / (elapsed / args.steps) / 1e9
+    log(f"topology step {ms_step:.3f} ms -> {gbs:.1f} GB/s")
+
+    # stage breakdown per step
+    stages = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in prof.items()}
+    comp_payload = info.topo_bytes - 8
+    sp_payload = max(0, info.sortpos_bytes - 16)
+    # algorithmic bytes per launch of each stage's kernel(s)
+    algo = {
+        "codec_topo": 4 * n + comp_payload,                  # read field, write topology payload
+        "codec_int": 4 * info.ext_count + sp_payload,        # read sort positions, write stream
+        "decode": (comp_payload + 5 * n) + (sp_payload + 4 * info.ext_count),  # both decoders
+        "sort": 4 * 16 * info.cp_count,                      # 4 passes x (key+payload) x (read+write)
+        "rank": 12 * info.cp_count,
+    }
+    dom = max((k for k in algo), key=lambda k: stages.get(k, (0, 0))[0])
+    dom_ms = stages[dom][0] / max(1.0, stages[dom][1]) if stages[dom][1] else stages[dom][0]
+    # per launch: average duration of the stage's launches
+    launches = prof[dom][1] / args.steps
+    per_launch_ms = prof[dom][0] / max(1, prof[dom][1])
+    per_launch_bytes = algo[dom] / max(1.0, launches)
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms}
+    e2e_bytes = (4 * n + info.topo_bytes + info.sortpos_bytes) + (info.topo_bytes + info.sortpos_bytes + 5 * n
+                                                                   + 4 * info.ext_count)
+    e2e_frac = e2e_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS
+
+    # plain SZp on the same field
+    nbytes, pout, pblk = tz.plain_compress(f, eb, bs)
+    pdec = torch.empty(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        nbytes, _, _ = tz.plain_compress(f, eb, bs, out=pout, blk=pblk)
+        tz.plain_decompress(pout, pblk, n, eb, bs, out=pdec)
+    torch.cuda.synchronize()
+    plain_gbs = 4.0 * n / ((time.perf_counter() - t0) / args.steps) / 1e9
+    log(f"plain SZp compress+decompress {plain_gbs:.1f} GB/s (ratio {4 * n / nbytes:.2f})")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"reference CPU baseline on a {args.cpu_slab}-plane slab with {threads} threads")
+        field_np = f.cpu().numpy()
+        res, err = cpu_baseline(field_np, eb, bs, threads, args.cpu_slab)
+        if res:
+            t = res["compress_s"] + res["decompress_s"]
+            cpu = {"value": 4.0 * res["n"] / t / 1e9, "unit": "GB/s", "cores": threads, "kind": "reference",
+                   "sample": f"{res['shape']} slab of the same field, reference OpenMP compress "
+                             f"({res['compress_s']:.3f} s) + decompress ({res['decompress_s']:.3f} s) "
+                             f"library calls, no post-process"}
+        else:
+            cpu = {"value": None, "unit": "GB/s", "cores": threads, "kind": "reference", "sample": err}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "topo compress+decompress GB/s & %HBM peak (512^3 f32)",
+        "value": gbs,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated fBm: 6 octaves value noise + 0.05 N(0,1))",
+        "config": {"workload": f"topology compress+decompress {shape[0]}x{shape[1]}x{shape[2]} f32, "
+                               f"rel {args.rel}, blockSize {bs}",
+                   "shape": list(shape), "block_size": bs, "error_bound": f"rel:{args.rel}", "abs_eb": eb,
+                   "nchunks": 1, "cp_count": info.cp_count, "ext_count": info.ext_count,
+                   "compression_ratio": 4 * n / (info.topo_bytes + info.sortpos_bytes)},
+        "roofline": roofline,
+        "end_to_end_hbm_frac": e2e_frac,
+        "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
+        "plain_gbs": plain_gbs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

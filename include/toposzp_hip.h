@@ -1,0 +1,107 @@
+/*
+ * toposzp_hip.h -- native (device-pointer) API of libtoposzp_hip.so for MI355X.
+ *
+ * The reference has no counterpart: its CUDA entry points take host pointers and
+ * copy every buffer over PCIe (SZp/src/szp_cuda_topology.cu:93-121).  These entry
+ * points keep fields, streams and metadata resident in HBM and run on a caller's
+ * HIP stream; the drop-in host-pointer API (szp_cuda_*.h) is built on them.
+ *
+ * Conventions
+ *   - d_* arguments are device pointers, 4-byte aligned.  Stream buffers must stay
+ *     readable up to the next multiple of 4 bytes past their end (size them with
+ *     the *_bound functions, which include that padding).
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Calls are asynchronous
+ *     except where a result size has to reach the host (compress calls return
+ *     after their sizes are known).
+ *   - Every call returns 0 on success or a negative TSZP_E* code; the message is
+ *     in tszp_last_error() (per host thread).
+ *   - Block-offset side-car: d_*_blk[b] is the byte offset of block b from the
+ *     first byte after the chunk-offset header, for b = 0..blocks, entry [blocks]
+ *     being the payload size.  The compressors write it; the decompressors read it,
+ *     so decoding needs no serial walk.
+ */
+#ifndef TOPOSZP_HIP_H
+#define TOPOSZP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSZP_OK 0
+#define TSZP_EINVAL (-1)   /* invalid argument */
+#define TSZP_EHIP (-2)     /* HIP runtime error */
+#define TSZP_ECAP (-3)     /* output buffer too small */
+#define TSZP_EDEVICE (-4)  /* a kernel reported a corrupt stream or a look-back timeout */
+
+typedef struct {
+  uint64_t topo_bytes;     /* [u64 off[T]][blocks], the reference CPU topology stream */
+  uint64_t sortpos_bytes;  /* [u64 E][u64 off[T]][blocks], the CPU sort-position stream; 0 if E == 0 */
+  uint64_t cp_count;       /* critical points */
+  uint64_t ext_count;      /* maxima + minima (E) */
+} tszp_topo_info;
+
+const char *tszp_last_error(void);
+int tszp_version(void);
+
+/* worst-case stream sizes (including header and read padding) */
+size_t tszp_topo_bound(size_t n, int block_size, unsigned nchunks);
+size_t tszp_sortpos_bound(size_t n_extrema, int block_size, unsigned nchunks);
+size_t tszp_plain_bound(size_t n, int block_size, unsigned nchunks);
+size_t tszp_blocks(size_t n, int block_size);
+
+/* relative -> absolute error bound exactly as SZp/include/szp_errbound.h:34-51:
+ * eb = rel * (max - min) in float, FLT_EPSILON when the range is not positive. */
+int tszp_rel_to_abs(const float *d_field, size_t n, float rel, float *eb, float *range, void *stream);
+
+/* Full topology compression of a 2D (dims = rows, cols) or 3D (d1, d2, d3) field:
+ * critical points, per-bin sort positions, the sort-position stream and the
+ * topology stream, byte-identical to the reference CPU path with nchunks threads
+ * (SZp/src/szp_float.cc:28-1318; SZp/src/szp_topology_3d.cc:30-632).
+ * d_sp_values (nullable, >= ext_count ints) receives the extrema sort positions. */
+int tszp_topo_compress(const float *d_field, int ndim, const uint32_t *dims, float eb, int block_size,
+                       unsigned nchunks, uint8_t *d_topo, size_t topo_cap, uint32_t *d_topo_blk,
+                       uint8_t *d_sortpos, size_t sortpos_cap, uint32_t *d_sortpos_blk,
+                       int32_t *d_sp_values, tszp_topo_info *info, void *stream);
+
+/* Topology stream -> (float)q * eb and the per-point types (0..3, uint8). */
+int tszp_topo_decompress(const uint8_t *d_topo, const uint32_t *d_topo_blk, unsigned nchunks, size_t n,
+                         float eb, int block_size, float *d_out, uint8_t *d_types, void *stream);
+
+/* Sort-position stream (with its u64 count header) -> int sort positions. */
+int tszp_sortpos_decompress(const uint8_t *d_sortpos, const uint32_t *d_sortpos_blk, unsigned nchunks,
+                            size_t n_extrema, int block_size, int32_t *d_out, void *stream);
+
+/* Plain random-access SZp (SZp/src/szp_float.cc:2061-2292, szpd_float.cc:431-524):
+ * [f32 eb big-endian][u64 off[T]][blocks]. */
+int tszp_plain_compress(const float *d_in, size_t n, float eb, int block_size, unsigned nchunks,
+                        uint8_t *d_out, size_t cap, uint32_t *d_blk, uint64_t *out_bytes, void *stream);
+int tszp_plain_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
+                          float eb, int block_size, float *d_out, void *stream);
+
+/* Per-point critical-point types of a field (0 regular / boundary, 1 max, 2 min,
+ * 3 saddle); *cp_count (nullable, host) gets the number of critical points. */
+int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t *d_types,
+                  uint64_t *cp_count, void *stream);
+
+/* Kernel timing with HIP events on the launch stream (off by default).  Kinds:
+ * 0 topology codec (classify + pack), 1 int codec (sort positions), 2 plain codec,
+ * 3 decoders, 4 radix sort, 5 rank.  tszp_prof_read returns the summed
+ * milliseconds and the number of timed launches since tszp_prof_reset. */
+void tszp_prof_enable(int on);
+void tszp_prof_reset(void);
+int tszp_prof_read(int kind, double *ms, uint64_t *launches);
+
+/* Synthetic fields from a seed (device generated). */
+int tszp_gen_blobs(float *d_field, int ndim, const uint32_t *dims, uint64_t seed, int nblobs,
+                   float sigma_min, float sigma_max, float noise, int waves, void *stream);
+int tszp_gen_fbm(float *d_field, const uint32_t *dims, uint64_t seed, int octaves, float noise,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,216 @@
+"""GPU parity: the HIP library against the reference's golden vectors and the CPU oracle.
+
+Bit-exact comparisons of critical-point lists, sort positions, streams, quantized
+reconstructions and type arrays; every call goes through the C-ABI of
+libtoposzp_hip.so (drop-in host API and native device API).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import fields
+import oracle
+import toposzp_hip as tz
+
+pytestmark = pytest.mark.gpu
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "g*.npz")))
+TOPO = [g for g in GOLD if os.path.basename(g).startswith(("g2d", "g3d"))]
+PLAIN = [g for g in GOLD if os.path.basename(g).startswith("gplain")]
+
+
+def _load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _eq(a, b, msg=""):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, f"{msg}: shape {a.shape} vs {b.shape}"
+    if a.dtype == np.float32:
+        a, b = a.view(np.uint32), b.view(np.uint32)
+    if not np.array_equal(a, b):
+        bad = np.flatnonzero(a.ravel() != b.ravel())
+        raise AssertionError(f"{msg}: {bad.size} mismatches, first at {bad[:8]}: "
+                             f"{a.ravel()[bad[:4]]} vs {b.ravel()[bad[:4]]}")
+
+
+# ------------------------------------------------------------------------------------------
+# drop-in (host pointers) against the reference's own outputs
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("path", TOPO, ids=os.path.basename)
+def test_dropin_matches_reference_golden(path):
+    _torch()
+    g = _load(path)
+    data, eb, bs = g["data"], float(g["eb"]), int(g["bs"])
+    n = data.size
+    cps = tz.find_critical_points(data, eb)
+    _eq(cps.view(np.int32), g["cps_found"].view(np.int32), "find_critical_points")
+    srt = tz.sort_critical_points(cps, data)
+    _eq(srt.view(np.int32), g["cps_sorted"].view(np.int32), "sort positions")
+    sp_stream = tz.compress_sort_positions(srt, bs)
+    _eq(sp_stream, g["sortpos_T1"][8:], "sort-position stream")  # no u64 count in the CUDA form
+    topo = tz.compress_topology(data, eb, bs, srt)
+    _eq(topo, g["topo_T1"], "topology stream")
+    dec, fn = tz.decompress_topology(g["topo_T1"], n, eb, bs)
+    _eq(dec, g["topo_dec_T1"], "decompressed field")
+    _eq(fn, g["topo_fn_T1"], "FN types")
+    ext = srt["sort_position"][(srt["type"] == 1) | (srt["type"] == 2)]
+    if ext.size:
+        _eq(tz.decompress_sort_positions(g["sortpos_T1"][8:], ext.size, bs), ext, "sort-pos decode")
+    for key in g:
+        if key.startswith("topo_T") and key != "topo_T1":
+            d2, f2 = tz.decompress_topology(g[key], n, eb, bs)
+            _eq(d2, g["topo_dec_T1"], f"{key} decode")
+            _eq(f2, g["topo_fn_T1"], f"{key} types")
+
+
+@pytest.mark.parametrize("path", PLAIN, ids=os.path.basename)
+def test_dropin_plain_matches_reference_golden(path):
+    _torch()
+    g = _load(path)
+    data, eb, bs = g["data"], float(g["eb"]), int(g["bs"])
+    s = tz.compress_plain(data, eb, bs)
+    _eq(s, g["plain_T1"], "plain stream")
+    for key in g:
+        if key.startswith("plain_T"):
+            dec = tz.decompress_plain(g[key][4:], data.size, eb, bs)
+            _eq(dec, g["plain_dec_T1"], f"{key} decode")
+
+
+# ------------------------------------------------------------------------------------------
+# native device API against the oracle
+# ------------------------------------------------------------------------------------------
+def _oracle_topo(data, eb, bs, T):
+    cps = oracle.find_cps_3d(data, eb) if data.ndim == 3 else oracle.find_cps_2d(data, eb)
+    oracle.sort_positions(cps, data)
+    types = oracle.mark_types(cps, data.shape)
+    sp = oracle.extrema_sort_positions(cps)
+    adaptive = data.ndim == 2
+    topo = oracle.topo_compress(data, eb, bs, types, T=T, adaptive=adaptive)
+    sps = oracle.sortpos_compress(sp, bs, T=T, adaptive=adaptive) if sp.size else np.zeros(0, np.uint8)
+    return cps, types, sp, topo, sps
+
+
+CASES = [
+    ("blobs2d", lambda: fields.blobs2d(300, 517, nblobs=24, seed=21), 1e-3, 64, 1),
+    ("blobs2d_T16", lambda: fields.blobs2d(211, 333, nblobs=24, seed=22), 2e-3, 64, 16),
+    ("blobs2d_bs8_T4", lambda: fields.blobs2d(97, 129, seed=23), 1e-3, 8, 4),
+    ("ties2d_bs16", lambda: fields.ties2d(150, 151, seed=24), 1e-3, 16, 1),
+    ("huge2d_bs128", lambda: fields.huge2d(77, 91, seed=25), 1e-3, 128, 2),
+    ("ragged2d_bs64", lambda: fields.blobs2d(5, 13, seed=3, noise=0.05), 1e-3, 64, 1),
+    ("bs1_2d", lambda: fields.blobs2d(31, 33, seed=26), 1e-3, 1, 1),
+    ("bs7_2d", lambda: fields.blobs2d(41, 43, seed=27), 1e-3, 7, 3),
+    ("bs257_2d", lambda: fields.blobs2d(60, 70, seed=28, noise=0.02), 5e-4, 257, 1),
+    ("fbm3d", lambda: fields.fbm3d(40, 52, 61, seed=29), 3e-3, 64, 1),
+    ("fbm3d_bs128_T8", lambda: fields.fbm3d(33, 40, 45, seed=30), 3e-3, 128, 8),
+    ("blobs3d_bs256", lambda: fields.blobs3d(30, 31, 37, seed=31, noise=1e-3), 1e-4, 256, 1),
+    ("tiny3d", lambda: fields.blobs3d(3, 3, 3, seed=32, noise=0.1), 1e-3, 64, 1),
+    ("const2d", lambda: np.full((40, 50), 1.25, np.float32), 1e-3, 64, 1),
+]
+
+
+@pytest.mark.parametrize("name,make,eb,bs,T", CASES, ids=[c[0] for c in CASES])
+def test_native_topology_matches_oracle(name, make, eb, bs, T):
+    torch = _torch()
+    data = make()
+    n = data.size
+    cps, types, sp, topo, sps = _oracle_topo(data, eb, bs, T)
+    f = torch.from_numpy(data).cuda()
+    info, buf = tz.topo_compress(f, eb, bs, T=T)
+    torch.cuda.synchronize()
+    assert info.cp_count == cps.size
+    assert info.ext_count == sp.size
+    _eq(buf.topo[:info.topo_bytes].cpu().numpy(), topo, "topology stream")
+    if sp.size:
+        _eq(buf.sp_values[:sp.size].cpu().numpy(), sp, "sort positions")
+        _eq(buf.sp[:info.sortpos_bytes].cpu().numpy(), sps, "sort-position stream")
+    else:
+        assert info.sortpos_bytes == 0
+    # decode with the side-car
+    out, ty = tz.topo_decompress(buf, n, eb, bs, T=T)
+    dec, fn = oracle.topo_decompress(topo[:], n, eb, bs) if T == 1 else oracle.topo_decompress(
+        oracle.topo_compress(data, eb, bs, types, T=1), n, eb, bs)
+    _eq(out.cpu().numpy(), dec, "decoded field")
+    _eq(ty.cpu().numpy().astype(np.int32), fn, "decoded types")
+    _eq(ty.cpu().numpy(), types, "decoded types vs classification")
+    if sp.size:
+        _eq(tz.sortpos_decompress(buf, sp.size, bs, T=T).cpu().numpy(), sp, "sort-position decode")
+    # the GPU classifier agrees with the oracle's
+    t2, cnt = tz.classify(f)
+    assert cnt == cps.size
+    _eq(t2.cpu().numpy(), oracle.classify(data).ravel(), "classify")
+
+
+@pytest.mark.parametrize("bs,T", [(64, 1), (128, 1), (32, 5), (1025, 1), (2, 2), (100, 16)])
+def test_native_plain_matches_oracle(bs, T):
+    torch = _torch()
+    data = fields.blobs2d(123, 457, seed=bs).ravel()
+    eb = 1e-3
+    ref = oracle.plain_compress(data, eb, bs, T=T, adaptive=True)
+    f = torch.from_numpy(data).cuda()
+    nbytes, out, blk = tz.plain_compress(f, eb, bs, T=T)
+    torch.cuda.synchronize()
+    _eq(out[:nbytes].cpu().numpy(), ref, "plain stream")
+    dec = tz.plain_decompress(out, blk, data.size, eb, bs, T=T)
+    ref1 = oracle.plain_compress(data, eb, bs, T=1)
+    _eq(dec.cpu().numpy(), oracle.plain_decompress(ref1[4:], data.size, eb, bs), "plain decode")
+
+
+def test_rel_error_bound_matches_reference_rule():
+    torch = _torch()
+    data = fields.fbm3d(20, 30, 40, seed=5)
+    want = oracle.rel_to_abs(data, 1e-3)
+    got = tz.rel_to_abs(torch.from_numpy(data).cuda(), 1e-3)
+    assert got == want
+
+
+def test_dropin_plain_round_trip_matches_oracle():
+    _torch()
+    data = fields.huge2d(64, 97, seed=7).ravel()
+    for bs in (16, 64, 1024):
+        s = tz.compress_plain(data, 0.5, bs)
+        _eq(s, oracle.plain_compress(data, 0.5, bs), f"plain bs={bs}")
+        _eq(tz.decompress_plain(s[4:], data.size, 0.5, bs), oracle.plain_decompress(s[4:], data.size, 0.5, bs))
+
+
+def test_dropin_3d_pipeline_matches_oracle():
+    _torch()
+    data = fields.fbm3d(24, 30, 36, seed=11)
+    eb, bs = 4e-3, 64
+    cps = tz.find_critical_points(data, eb)
+    ocps = oracle.find_cps_3d(data, eb)
+    _eq(cps.view(np.int32), ocps.view(np.int32), "3D critical points")
+    srt = tz.sort_critical_points(cps, data)
+    oracle.sort_positions(ocps, data)
+    _eq(srt.view(np.int32), ocps.view(np.int32), "3D sort positions")
+    types = oracle.mark_types(ocps, data.shape)
+    _eq(tz.compress_topology(data, eb, bs, srt), oracle.topo_compress(data, eb, bs, types, adaptive=False))
+    sp = oracle.extrema_sort_positions(ocps)
+    _eq(tz.compress_sort_positions(srt, bs), oracle.sortpos_compress(sp, bs, adaptive=False)[8:])
+
+
+def test_large_field_matches_oracle():
+    """A 3D field of 8.4 M points (the oracle takes a few seconds)."""
+    torch = _torch()
+    f = tz.gen_fbm((128, 256, 256), seed=2024)
+    data = f.cpu().numpy()
+    eb, _ = tz.rel_to_abs(f, 1e-3)
+    bs = 64
+    info, buf = tz.topo_compress(f, eb, bs)
+    cps, types, sp, topo, sps = _oracle_topo(data, eb, bs, 1)
+    assert info.cp_count == cps.size and info.ext_count == sp.size
+    _eq(buf.topo[:info.topo_bytes].cpu().numpy(), topo, "topology stream")
+    _eq(buf.sp[:info.sortpos_bytes].cpu().numpy(), sps, "sort-position stream")
+    out, ty = tz.topo_decompress(buf, data.size, eb, bs)
+    dec, fn = oracle.topo_decompress(topo, data.size, eb, bs)
+    _eq(out.cpu().numpy(), dec)
+    _eq(ty.cpu().numpy(), types)

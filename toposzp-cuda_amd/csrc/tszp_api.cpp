@@ -1,0 +1,1054 @@
+// tszp_api.cpp -- host side of libtoposzp_hip.so: the native device-pointer API
+// (include/toposzp_hip.h) and the reference-compatible host-pointer drop-in
+// (include/szp_cuda_*.h), both driving the HIP kernels of this directory.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/szp_cuda_compress.h"
+#include "../../include/szp_cuda_decompress.h"
+#include "../../include/szp_cuda_topology.h"
+#include "../../include/szp_cuda_topology_3d.h"
+#include "../../include/toposzp_hip.h"
+#include "tszp_kernels.h"
+
+using namespace tszp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(TSZP_EHIP, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+// ---- per-device context: a grow-only workspace, a stream for drop-in calls ------------
+enum Slot {
+  kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
+  kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
+  kWsCount
+};
+
+struct Ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  void* ptr[kWsCount] = {};
+  size_t cap[kWsCount] = {};
+  CompressResult* host_res = nullptr;  // pinned
+
+  int get(int slot, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 4;
+    if (cap[slot] < bytes) {
+      if (ptr[slot]) {
+        hipError_t e = hipFree(ptr[slot]);
+        ptr[slot] = nullptr;
+        cap[slot] = 0;
+        if (e != hipSuccess) return fail(TSZP_EHIP, "hipFree: %s", hipGetErrorString(e));
+      }
+      size_t want = bytes + bytes / 4 + 256;
+      hipError_t e = hipMalloc(&ptr[slot], want);
+      if (e != hipSuccess) return fail(TSZP_EHIP, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+      cap[slot] = want;
+    }
+    *out = ptr[slot];
+    return TSZP_OK;
+  }
+};
+
+std::mutex g_ctx_mu;
+std::vector<Ctx*> g_ctx;
+
+// Optional per-kernel timing: HIP events recorded on the launch stream around the
+// kernels of each stage, accumulated until tszp_prof_reset().
+enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfCount };
+int g_prof = 0;
+double g_prof_ms[kProfCount];
+uint64_t g_prof_n[kProfCount];
+
+struct ProfScope {
+  hipEvent_t a = nullptr, b = nullptr;
+  int kind;
+  hipStream_t st;
+  ProfScope(int k, hipStream_t s) : kind(k), st(s) {
+    if (!g_prof) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, st);
+  }
+  void stop() {
+    if (a) hipEventRecord(b, st);
+  }
+  // call after the stream has been synchronized
+  void collect() {
+    if (!a) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+      g_prof_ms[kind] += ms;
+      g_prof_n[kind]++;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    a = b = nullptr;
+  }
+  ~ProfScope() { collect(); }
+};
+
+int get_ctx(Ctx** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(TSZP_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if ((int)g_ctx.size() <= dev) g_ctx.resize(dev + 1, nullptr);
+  if (!g_ctx[dev]) {
+    Ctx* c = new Ctx();
+    c->device = dev;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(TSZP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    e = hipHostMalloc((void**)&c->host_res, sizeof(CompressResult), hipHostMallocDefault);
+    if (e != hipSuccess) { delete c; return fail(TSZP_EHIP, "hipHostMalloc: %s", hipGetErrorString(e)); }
+    g_ctx[dev] = c;
+  }
+  *out = g_ctx[dev];
+  return TSZP_OK;
+}
+
+#define WS(slot, bytes, ptrvar)                                     \
+  do {                                                              \
+    void* p_;                                                       \
+    int r_ = c->get((slot), (bytes), &p_);                          \
+    if (r_) return r_;                                              \
+    ptrvar = reinterpret_cast<decltype(ptrvar)>(p_);                \
+  } while (0)
+
+FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  const uint64_t num = 1ull << (31 + l);
+  FastDiv f;
+  f.mul = (uint32_t)((num + d - 1) / d);
+  f.shift = 31 + l;
+  return f;
+}
+
+Grid make_grid(int ndim, const uint32_t* dims) {
+  Grid g{};
+  g.ndim = ndim;
+  g.d0 = dims[0];
+  g.d1 = ndim >= 2 ? dims[1] : 1;
+  g.d2 = ndim == 3 ? dims[2] : 1;
+  g.plane = g.d1 * g.d2;
+  g.div_row = make_fastdiv(ndim == 2 ? g.d1 : g.d2);
+  g.div_plane = make_fastdiv(g.plane ? g.plane : 1);
+  return g;
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+size_t block_worst(int bs, bool topo) {
+  const size_t n = (size_t)bs - 1;
+  return 5 + (n + 7) / 8 + 4 * n + (topo ? (2 * (size_t)bs + 7) / 8 : 0);
+}
+
+int check_bs(int bs, bool topo) {
+  if (topo ? (bs < 1 || bs > 257) : (bs < 2 || bs > 1025))
+    return fail(TSZP_EINVAL, "blockSize %d out of range [%d, %d]", bs, topo ? 1 : 2, topo ? 257 : 1025);
+  return TSZP_OK;
+}
+
+// zero the look-back words + tile counter + result of one codec launch
+int codec_prepare(Ctx* c, uint32_t ntiles, uint64_t** st_bytes, uint64_t** st_cnt, uint32_t** counter,
+                  CompressResult** res, hipStream_t st) {
+  uint64_t* s;
+  const size_t words = 2 * (size_t)ntiles + 2;
+  WS(kWsStatus, words * 8, s);
+  CompressResult* r;
+  WS(kWsResult, sizeof(CompressResult), r);
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(s), 0u, words * 2, st));
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(r), 0u, sizeof(CompressResult) / 4, st));
+  *st_bytes = s;
+  *st_cnt = s + ntiles;
+  *counter = reinterpret_cast<uint32_t*>(s + 2 * ntiles);
+  *res = r;
+  return TSZP_OK;
+}
+
+int read_result(Ctx* c, CompressResult* d_res, hipStream_t st, CompressResult* out) {
+  HIPCHK(hipMemcpyAsync(c->host_res, d_res, sizeof(CompressResult), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = *c->host_res;
+  if (out->error)
+    return fail(TSZP_EDEVICE, "device error bits 0x%x (%s%s%s)", out->error,
+                (out->error & TSZP_ERR_LOOKBACK) ? "look-back timeout " : "",
+                (out->error & TSZP_ERR_STREAM) ? "corrupt stream " : "",
+                (out->error & TSZP_ERR_OVERFLOW) ? "capacity " : "");
+  return TSZP_OK;
+}
+
+// One codec pass (compress) over `n` values; writes blocks at d_stream + hdr and
+// the block side-car.  *payload_bytes receives the payload size.
+int run_codec(Ctx* c, int mode, const void* d_in, uint32_t n, int bs, float eb, double inv, const Grid* grid,
+              const uint8_t* d_tmap, uint8_t* d_stream, size_t cap, size_t hdr, uint32_t* d_blk,
+              uint32_t* cp_key, uint32_t* cp_pay, uint32_t cp_cap, hipStream_t st, CompressResult* out) {
+  int S, EPL;
+  uint32_t bpt;
+  block_geometry((uint32_t)bs, &S, &EPL, &bpt);
+  const uint32_t nb = cdiv(n, bs);
+  const uint32_t ntiles = cdiv(nb, bpt);
+  if (((uintptr_t)(d_stream + hdr)) & 3u) return fail(TSZP_EINVAL, "stream payload must be 4-byte aligned");
+  if (cap < hdr + 4) return fail(TSZP_ECAP, "stream capacity %zu too small", cap);
+  CompressArgs a{};
+  uint64_t *stb, *stc;
+  uint32_t* counter;
+  CompressResult* res;
+  int r = codec_prepare(c, ntiles, &stb, &stc, &counter, &res, st);
+  if (r) return r;
+  a.in = d_in;
+  a.n = n;
+  a.bs = (uint32_t)bs;
+  a.nb = nb;
+  a.ntiles = ntiles;
+  a.eb = eb;
+  a.inv = inv;
+  if (grid) a.grid = *grid;
+  a.tmap = d_tmap;
+  a.payload = d_stream + hdr;
+  uint64_t pc = cap - hdr - 4;  // keep 4 readable bytes past the end
+  a.cap = pc > 0xffffffffull ? 0xffffffffull : pc;
+  a.blk = d_blk;
+  a.st_bytes = stb;
+  a.st_cnt = stc;
+  a.tile_counter = counter;
+  a.cp_key = cp_key;
+  a.cp_pay = cp_pay;
+  a.cp_cap = cp_cap;
+  a.res = res;
+  ProfScope prof(mode == kModeInt ? kProfCodecInt : mode == kModePlain ? kProfCodecPlain : kProfCodecTopo, st);
+  HIPCHK(launch_compress(mode, a, st));
+  prof.stop();
+  r = read_result(c, res, st, out);
+  prof.collect();
+  return r;
+}
+
+SortWork sort_work(Ctx* c, uint32_t count, bool hi, int* rc) {
+  SortWork w{};
+  *rc = TSZP_OK;
+  auto get = [&](int slot, size_t bytes) -> void* {
+    void* p = nullptr;
+    if (*rc == TSZP_OK) *rc = c->get(slot, bytes, &p);
+    return p;
+  };
+  w.keys_alt = (uint32_t*)get(kWsKeyAlt, (size_t)count * 4);
+  w.pay_alt = (uint32_t*)get(kWsPayAlt, (size_t)count * 4);
+  w.hi_alt = hi ? (uint32_t*)get(kWsHiAlt, (size_t)count * 4) : nullptr;
+  w.hist = (uint32_t*)get(kWsHist, sort_hist_words(count) * 4);
+  const size_t stw = sort_status_words(count) + 16 / 2 + 1;
+  uint64_t* s = (uint64_t*)get(kWsSortSt, stw * 8 + 64);
+  if (*rc) return w;
+  w.st = s;
+  w.counters = reinterpret_cast<uint32_t*>(s + sort_status_words(count));
+  w.err = w.counters + 15;
+  return w;
+}
+
+// sort (keys[, hi], pay) and write dense in-bin ranks to out[pay]
+int sort_and_rank(Ctx* c, uint32_t* keys, uint32_t* hi, uint32_t* pay, uint32_t count, float eb, double inv,
+                  int32_t* out, uint32_t out_cap, hipStream_t st) {
+  if (!count) return TSZP_OK;
+  int rc;
+  SortWork w = sort_work(c, count, hi != nullptr, &rc);
+  if (rc) return rc;
+  const size_t stw = sort_status_words(count) + 8;
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(w.st), 0u, stw * 2, st));
+  ProfScope ps(kProfSort, st);
+  HIPCHK(radix_sort_pairs(keys, hi, pay, count, w, st));
+  ps.stop();
+  ProfScope pr(kProfRank, st);
+  HIPCHK(launch_rank(keys, hi, pay, count, eb, inv, out, out_cap, w, st));
+  pr.stop();
+  uint32_t herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, w.err, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (herr) return fail(TSZP_EDEVICE, "sort: device error bits 0x%x", herr);
+  return TSZP_OK;
+}
+
+// ---- host helpers for streams that arrive without a side-car -------------------------
+inline uint64_t rd64(const unsigned char* p) {
+  uint64_t v;
+  memcpy(&v, p, 8);
+  return v;
+}
+
+inline size_t fixed_len(size_t n, unsigned bc) {
+  const unsigned rem = bc & 7u;
+  return (size_t)(bc >> 3) * n + (rem ? (rem * n + 7) / 8 : 0);
+}
+
+// Walk `nb` consecutive blocks from `p` (chunks are concatenated in block order, so
+// the walk needs no partition rule).  topo: blocks carry types and always a bit
+// count byte; otherwise a one-element block has no bit-count byte for the reader.
+// Returns false when a bit count is impossible or the walk passes `lim` bytes.
+bool walk_blocks(const unsigned char* p, size_t n, int bs, bool topo, size_t lim, uint32_t* blk, size_t* total) {
+  const size_t nb = (n + bs - 1) / bs;
+  size_t pos = 0;
+  for (size_t b = 0; b < nb; b++) {
+    const size_t i = b * (size_t)bs;
+    const size_t cur = (i + bs > n) ? n - i : (size_t)bs;
+    if (blk) blk[b] = (uint32_t)pos;
+    if (pos + 4 > lim) return false;
+    pos += 4;
+    if (topo || cur > 1) {
+      if (pos >= lim) return false;
+      const unsigned bc = p[pos++];
+      if (bc > 32) return false;
+      if (bc) pos += (cur - 1 + 7) / 8 + fixed_len(cur - 1, bc);
+    }
+    if (topo) pos += (2 * cur + 7) / 8;
+    if (pos > lim) return false;
+  }
+  if (blk) blk[nb] = (uint32_t)pos;
+  *total = pos;
+  return true;
+}
+
+// Chunk count of a stream whose offset table starts at cmp: the first T in
+// 1..128 whose offsets are non-decreasing and bounded and whose blocks walk within
+// `lim` bytes (SZp/include/szp_detect_threads.h:161-309; the offsets of T > 1 must
+// moreover fall on block boundaries).
+unsigned detect_chunks(const unsigned char* cmp, size_t n, int bs, bool topo, size_t lim) {
+  if (rd64(cmp) != 0) return 1;
+  const size_t nb = (n + bs - 1) / bs;
+  std::vector<uint32_t> blk(nb + 1);
+  for (unsigned t = 1; t <= 128; t++) {
+    bool ok = true;
+    for (unsigned k = 1; k < t && ok; k++) {
+      const uint64_t a = rd64(cmp + 8 * k), b = rd64(cmp + 8 * (k - 1));
+      if (a > lim || a < b) ok = false;
+    }
+    if (!ok) break;
+    size_t total;
+    if (!walk_blocks(cmp + 8 * (size_t)t, n, bs, topo, lim, blk.data(), &total)) continue;
+    for (unsigned k = 1; k < t && ok; k++) {
+      const uint64_t o = rd64(cmp + 8 * k);
+      if (o > total) { ok = false; break; }
+      // the offset must coincide with a block start or the end
+      size_t lo = 0, hi = nb;
+      while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (blk[mid] < o) lo = mid + 1; else hi = mid;
+      }
+      if (blk[lo] != o) ok = false;
+    }
+    if (ok) return t;
+  }
+  return 1;
+}
+
+// Upload a host stream (payload after a T-chunk table) and its block offsets.
+int upload_stream(Ctx* c, const unsigned char* cmp, size_t n, int bs, bool topo, size_t lim, unsigned* T_out,
+                  uint8_t** d_payload, uint32_t** d_blk, hipStream_t st) {
+  const unsigned T = detect_chunks(cmp, n, bs, topo, lim);
+  const size_t nb = (n + bs - 1) / bs;
+  std::vector<uint32_t> blk(nb + 1);
+  size_t total = 0;
+  if (!walk_blocks(cmp + 8 * (size_t)T, n, bs, topo, lim, blk.data(), &total))
+    return fail(TSZP_EINVAL, "corrupt stream (block walk failed)");
+  uint8_t* dp;
+  uint32_t* db;
+  WS(kWsStream, total + 8, dp);
+  WS(kWsBlk, (nb + 1) * 4, db);
+  HIPCHK(hipMemcpyAsync(dp, cmp + 8 * (size_t)T, total, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(db, blk.data(), (nb + 1) * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vector dies at return
+  *T_out = T;
+  *d_payload = dp;
+  *d_blk = db;
+  return TSZP_OK;
+}
+
+int decode(Ctx* c, int mode, const uint8_t* payload, const uint32_t* blk, size_t n, int bs, float eb, void* out,
+           uint8_t* t8, int32_t* t32, hipStream_t st) {
+  uint32_t* err;
+  WS(kWsMinMax, 16, err);
+  HIPCHK(launch_fill_u32(err, 0u, 1, st));
+  DecodeArgs a{};
+  a.payload = payload;
+  a.blk = blk;
+  a.n = (uint32_t)n;
+  a.bs = (uint32_t)bs;
+  a.nb = cdiv(n, bs);
+  a.eb = eb;
+  a.out = out;
+  a.types8 = t8;
+  a.types32 = t32;
+  a.err = err;
+  ProfScope prof(kProfDecode, st);
+  HIPCHK(launch_decode(mode, a, st));
+  prof.stop();
+  uint32_t herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (herr) return fail(TSZP_EDEVICE, "decode: device error bits 0x%x", herr);
+  return TSZP_OK;
+}
+
+int check_n(size_t n) {
+  if (n == 0 || n >= 0x7fffffffull) return fail(TSZP_EINVAL, "element count %zu out of range [1, 2^31)", n);
+  return TSZP_OK;
+}
+
+}  // namespace
+
+// =========================================================================================
+// native API
+// =========================================================================================
+extern "C" {
+
+const char* tszp_last_error(void) { return g_err.c_str(); }
+
+void tszp_prof_enable(int on) { g_prof = on; }
+void tszp_prof_reset(void) {
+  for (int k = 0; k < kProfCount; k++) g_prof_ms[k] = 0, g_prof_n[k] = 0;
+}
+int tszp_prof_read(int kind, double* ms, uint64_t* launches) {
+  if (kind < 0 || kind >= kProfCount) return TSZP_EINVAL;
+  *ms = g_prof_ms[kind];
+  *launches = g_prof_n[kind];
+  return TSZP_OK;
+}
+int tszp_version(void) {
+  g_err.clear(); return 1; }
+
+size_t tszp_blocks(size_t n, int bs) {
+  g_err.clear(); return bs > 0 ? (n + bs - 1) / bs : 0; }
+size_t tszp_topo_bound(size_t n, int bs, unsigned T) {
+  g_err.clear();
+  return bs > 0 ? 8 * (size_t)T + tszp_blocks(n, bs) * block_worst(bs, true) + 8 : 0;
+}
+size_t tszp_sortpos_bound(size_t e, int bs, unsigned T) {
+  g_err.clear();
+  return bs > 0 ? 8 + 8 * (size_t)T + tszp_blocks(e, bs) * block_worst(bs, false) + 8 : 0;
+}
+size_t tszp_plain_bound(size_t n, int bs, unsigned T) {
+  g_err.clear();
+  return bs > 0 ? 4 + 8 * (size_t)T + tszp_blocks(n, bs) * block_worst(bs, false) + 8 : 0;
+}
+
+int tszp_rel_to_abs(const float* d_field, size_t n, float rel, float* eb, float* range, void* stream) {
+  g_err.clear();
+  if (!d_field || n == 0 || !(rel > 0) || !eb) return fail(TSZP_EINVAL, "rel_to_abs: bad arguments");
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t* mm;
+  WS(kWsMinMax, 16, mm);
+  HIPCHK(launch_minmax(d_field, n, mm, st));
+  uint32_t h[2];
+  HIPCHK(hipMemcpyAsync(h, mm, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  auto val = [](uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+  };
+  const float lo = val(h[0]), hi = val(h[1]);
+  float rg = hi - lo;
+  if (rg <= 0) rg = FLT_EPSILON;
+  *eb = rel * rg;
+  if (range) *range = rg;
+  return TSZP_OK;
+}
+
+int tszp_topo_compress(const float* d_field, int ndim, const uint32_t* dims, float eb, int bs, unsigned T,
+                       uint8_t* d_topo, size_t topo_cap, uint32_t* d_topo_blk, uint8_t* d_sp, size_t sp_cap,
+                       uint32_t* d_sp_blk, int32_t* d_sp_values, tszp_topo_info* info, void* stream) {
+  g_err.clear();
+  if (!d_field || !dims || !d_topo || !d_topo_blk || !info || (ndim != 2 && ndim != 3))
+    return fail(TSZP_EINVAL, "topo_compress: bad arguments");
+  if (!(eb > 0)) return fail(TSZP_EINVAL, "topo_compress: error bound must be positive");
+  if (T < 1 || T > 4096) return fail(TSZP_EINVAL, "topo_compress: nchunks %u out of range", T);
+  int r = check_bs(bs, true);
+  if (r) return r;
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) {
+    if (dims[k] == 0) return fail(TSZP_EINVAL, "topo_compress: zero dimension");
+    n *= dims[k];
+  }
+  if ((r = check_n(n))) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const Grid g = make_grid(ndim, dims);
+  const double inv = 1.0 / (double)eb;
+
+  uint32_t *cpk, *cpp;
+  WS(kWsCpKey, n * 4, cpk);
+  WS(kWsCpPay, n * 4, cpp);
+  CompressResult res;
+  r = run_codec(c, kModeTopoClassify, d_field, (uint32_t)n, bs, eb, inv, &g, nullptr, d_topo, topo_cap, 8 * (size_t)T,
+                d_topo_blk, cpk, cpp, (uint32_t)n, st, &res);
+  if (r) return r;
+  const uint32_t nb = cdiv(n, bs);
+  HIPCHK(launch_write_header(d_topo, 0, T, ndim == 2, nb, d_topo_blk, 0, eb, st));
+  info->topo_bytes = 8 * (uint64_t)T + res.payload_bytes;
+  info->cp_count = res.cp_count;
+  info->ext_count = res.ext_count;
+  info->sortpos_bytes = 0;
+
+  const uint32_t E = (uint32_t)res.ext_count;
+  if (E == 0) return TSZP_OK;
+  if (!d_sp || !d_sp_blk) return fail(TSZP_EINVAL, "topo_compress: sort-position buffers missing");
+  int32_t* sp = d_sp_values;
+  if (!sp) WS(kWsSp, (size_t)E * 4, sp);
+  r = sort_and_rank(c, cpk, nullptr, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st);
+  if (r) return r;
+  CompressResult r2;
+  r = run_codec(c, kModeInt, sp, E, bs, 0.f, 0.0, nullptr, nullptr, d_sp, sp_cap, 8 + 8 * (size_t)T, d_sp_blk,
+                nullptr, nullptr, 0, st, &r2);
+  if (r) return r;
+  HIPCHK(launch_write_header(d_sp, 1, T, ndim == 2, cdiv(E, bs), d_sp_blk, E, 0.f, st));
+  HIPCHK(hipStreamSynchronize(st));
+  info->sortpos_bytes = 8 + 8 * (uint64_t)T + r2.payload_bytes;
+  return TSZP_OK;
+}
+
+int tszp_topo_decompress(const uint8_t* d_topo, const uint32_t* d_blk, unsigned T, size_t n, float eb, int bs,
+                         float* d_out, uint8_t* d_types, void* stream) {
+  g_err.clear();
+  if (!d_topo || !d_blk || !d_out || T < 1) return fail(TSZP_EINVAL, "topo_decompress: bad arguments");
+  int r = check_bs(bs, true);
+  if (r || (r = check_n(n))) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return decode(c, kModeTopoMap, d_topo + 8 * (size_t)T, d_blk, n, bs, eb, d_out, d_types, nullptr,
+                (hipStream_t)stream);
+}
+
+int tszp_sortpos_decompress(const uint8_t* d_sp, const uint32_t* d_blk, unsigned T, size_t e, int bs,
+                            int32_t* d_out, void* stream) {
+  g_err.clear();
+  if (!d_sp || !d_blk || !d_out || T < 1) return fail(TSZP_EINVAL, "sortpos_decompress: bad arguments");
+  int r = check_bs(bs, true);
+  if (r || (r = check_n(e))) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return decode(c, kModeInt, d_sp + 8 + 8 * (size_t)T, d_blk, e, bs, 0.f, d_out, nullptr, nullptr,
+                (hipStream_t)stream);
+}
+
+int tszp_plain_compress(const float* d_in, size_t n, float eb, int bs, unsigned T, uint8_t* d_out, size_t cap,
+                        uint32_t* d_blk, uint64_t* out_bytes, void* stream) {
+  g_err.clear();
+  if (!d_in || !d_out || !d_blk || !out_bytes || T < 1 || T > 4096)
+    return fail(TSZP_EINVAL, "plain_compress: bad arguments");
+  if (!(eb > 0)) return fail(TSZP_EINVAL, "plain_compress: error bound must be positive");
+  int r = check_bs(bs, false);
+  if (r || (r = check_n(n))) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  // SZp/src/szp_float.cc:2125: the reciprocal is an int / float division, then widened
+  volatile float finv = 1 / eb;
+  CompressResult res;
+  r = run_codec(c, kModePlain, d_in, (uint32_t)n, bs, eb, (double)finv, nullptr, nullptr, d_out, cap,
+                4 + 8 * (size_t)T, d_blk, nullptr, nullptr, 0, st, &res);
+  if (r) return r;
+  HIPCHK(launch_write_header(d_out, 2, T, 1, cdiv(n, bs), d_blk, 0, eb, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out_bytes = 4 + 8 * (uint64_t)T + res.payload_bytes;
+  return TSZP_OK;
+}
+
+int tszp_plain_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, float eb, int bs,
+                          float* d_out, void* stream) {
+  g_err.clear();
+  if (!d_stream || !d_blk || !d_out || T < 1) return fail(TSZP_EINVAL, "plain_decompress: bad arguments");
+  int r = check_bs(bs, false);
+  if (r || (r = check_n(n))) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return decode(c, kModePlain, d_stream + 4 + 8 * (size_t)T, d_blk, n, bs, eb, d_out, nullptr, nullptr,
+                (hipStream_t)stream);
+}
+
+int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t* d_types, uint64_t* cp_count,
+                  void* stream) {
+  g_err.clear();
+  if (!d_field || !dims || (ndim != 2 && ndim != 3)) return fail(TSZP_EINVAL, "classify: bad arguments");
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t nt = classify_tiles((uint32_t)n);
+  uint64_t *stb, *stc;
+  uint32_t* counter;
+  CompressResult* res;
+  if ((r = codec_prepare(c, nt, &stb, &stc, &counter, &res, st))) return r;
+  ClassifyArgs a{};
+  a.f = d_field;
+  a.n = (uint32_t)n;
+  a.grid = make_grid(ndim, dims);
+  a.eb = 1.f;
+  a.inv = 1.0;
+  a.ntiles = nt;
+  a.st_cnt = stb;
+  a.tile_counter = counter;
+  a.types = d_types;
+  a.res = res;
+  HIPCHK(launch_classify(a, st));
+  CompressResult out;
+  if ((r = read_result(c, res, st, &out))) return r;
+  if (cp_count) *cp_count = out.cp_count;
+  return TSZP_OK;
+}
+
+int tszp_gen_blobs(float* d, int ndim, const uint32_t* dims, uint64_t seed, int nblobs, float smin, float smax,
+                   float noise, int waves, void* stream) {
+  g_err.clear();
+  if (!d || !dims || (ndim != 2 && ndim != 3) || nblobs < 0) return fail(TSZP_EINVAL, "gen_blobs: bad arguments");
+  HIPCHK(launch_gen_blobs(d, ndim, dims, seed, nblobs, smin, smax, noise, waves, (hipStream_t)stream));
+  return TSZP_OK;
+}
+
+int tszp_gen_fbm(float* d, const uint32_t* dims, uint64_t seed, int octaves, float noise, void* stream) {
+  g_err.clear();
+  if (!d || !dims) return fail(TSZP_EINVAL, "gen_fbm: bad arguments");
+  HIPCHK(launch_gen_fbm(d, dims, seed, octaves, noise, (hipStream_t)stream));
+  return TSZP_OK;
+}
+
+}  // extern "C"
+
+// =========================================================================================
+// drop-in host-pointer API (reference names; NULL / zero size on error, message in
+// tszp_last_error() -- the reference exits the process instead)
+// =========================================================================================
+namespace {
+
+template <typename Rec>
+int find_cps_impl(const float* data, int ndim, const uint32_t* dims, float eb, Rec** out, size_t* count) {
+  *out = nullptr;
+  *count = 0;
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  float* df;
+  WS(kWsIn, n * 4, df);
+  HIPCHK(hipMemcpyAsync(df, data, n * 4, hipMemcpyHostToDevice, st));
+  size_t interior = 1;
+  for (int k = 0; k < ndim; k++) interior *= (dims[k] > 2 ? dims[k] - 2 : 0);
+  const int rec = (int)(sizeof(Rec) / 4);
+  int32_t* dc;
+  WS(kWsCps, interior * sizeof(Rec) + 4, dc);
+  const uint32_t nt = classify_tiles((uint32_t)n);
+  uint64_t *stb, *stc;
+  uint32_t* counter;
+  CompressResult* res;
+  if ((r = codec_prepare(c, nt, &stb, &stc, &counter, &res, st))) return r;
+  ClassifyArgs a{};
+  a.f = df;
+  a.n = (uint32_t)n;
+  a.grid = make_grid(ndim, dims);
+  a.eb = eb;
+  a.inv = 1.0 / (double)eb;
+  a.ntiles = nt;
+  a.st_cnt = stb;
+  a.tile_counter = counter;
+  a.cps = dc;
+  a.rec = rec;
+  a.cp_cap = (uint32_t)interior;
+  a.res = res;
+  HIPCHK(launch_classify(a, st));
+  CompressResult o;
+  if ((r = read_result(c, res, st, &o))) return r;
+  if (o.cp_count == 0) return TSZP_OK;
+  Rec* h = (Rec*)malloc(o.cp_count * sizeof(Rec));
+  if (!h) return fail(TSZP_EINVAL, "malloc failed");
+  hipError_t e = hipMemcpy(h, dc, o.cp_count * sizeof(Rec), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    free(h);
+    return fail(TSZP_EHIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  *count = o.cp_count;
+  return TSZP_OK;
+}
+
+// sort positions of a caller's record array, grouped by the stored bins
+template <typename Rec>
+int sort_cps_impl(Rec* cps, size_t count, const float* data, uint32_t row, uint64_t plane) {
+  if (count == 0 || count >= 0x7fffffffull) return count ? fail(TSZP_EINVAL, "too many points") : TSZP_OK;
+  const int rec = (int)(sizeof(Rec) / 4);
+  // the largest flat index tells how much of the field has to travel
+  // (SZp/src/szp_cuda_topology.cu:202-206)
+  uint64_t max_flat = 0;
+  for (size_t i = 0; i < count; i++) {
+    const int32_t* p = reinterpret_cast<const int32_t*>(&cps[i]);
+    const int64_t f = rec == 5 ? (int64_t)p[0] * row + p[1] : (int64_t)p[0] * (int64_t)plane + (int64_t)p[1] * row + p[2];
+    if (f < 0) return fail(TSZP_EINVAL, "negative coordinate in critical point %zu", i);
+    if ((uint64_t)f > max_flat) max_flat = (uint64_t)f;
+  }
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  float* df;
+  int32_t* dc;
+  uint32_t *k, *hi, *pay, *err;
+  int32_t* rank;
+  WS(kWsIn, (max_flat + 1) * 4, df);
+  WS(kWsCps, count * sizeof(Rec), dc);
+  WS(kWsCpKey, count * 4, k);
+  WS(kWsHiKey, count * 4, hi);
+  WS(kWsCpPay, count * 4, pay);
+  WS(kWsSp, count * 4, rank);
+  WS(kWsMinMax, 16, err);
+  HIPCHK(hipMemcpyAsync(df, data, (max_flat + 1) * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dc, cps, count * sizeof(Rec), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_fill_u32(err, 0u, 1, st));
+  HIPCHK(launch_cp_keys(dc, (uint32_t)count, rec, df, max_flat + 1, row, plane, k, hi, pay, err, st));
+  if ((r = sort_and_rank(c, k, hi, pay, (uint32_t)count, 0.f, 0.0, rank, (uint32_t)count, st))) return r;
+  std::vector<int32_t> h(count);
+  HIPCHK(hipMemcpyAsync(h.data(), rank, count * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < count; i++) cps[i].sort_position = h[i];
+  return TSZP_OK;
+}
+
+// int codec of the extrema sort positions in record order -> [u64 0][blocks]
+template <typename Rec>
+unsigned char* compress_sp_impl(const Rec* cps, size_t count, size_t* outSize, int bs) {
+  *outSize = 0;
+  if (!cps || count == 0 || check_bs(bs, true)) return nullptr;
+  std::vector<int32_t> sp;
+  sp.reserve(count);
+  for (size_t i = 0; i < count; i++)
+    if (cps[i].type == 1 || cps[i].type == 2) sp.push_back(cps[i].sort_position);
+  if (sp.empty() || check_n(sp.size())) return nullptr;
+  Ctx* c;
+  if (get_ctx(&c)) return nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  const size_t e = sp.size();
+  int32_t* din;
+  uint8_t* dout;
+  uint32_t* dblk;
+  const size_t cap = tszp_sortpos_bound(e, bs, 1);
+  if (c->get(kWsSp, e * 4, (void**)&din) || c->get(kWsOut, cap, (void**)&dout) ||
+      c->get(kWsBlk, (tszp_blocks(e, bs) + 1) * 4, (void**)&dblk))
+    return nullptr;
+  if (hipMemcpyAsync(din, sp.data(), e * 4, hipMemcpyHostToDevice, st) != hipSuccess) return nullptr;
+  CompressResult res;
+  // header is the single zero chunk offset; the payload follows at byte 8
+  if (run_codec(c, kModeInt, din, (uint32_t)e, bs, 0.f, 0.0, nullptr, nullptr, dout, cap, 8, dblk, nullptr,
+                nullptr, 0, st, &res))
+    return nullptr;
+  const size_t total = 8 + res.payload_bytes;
+  unsigned char* h = (unsigned char*)malloc(total);
+  if (!h) return nullptr;
+  memset(h, 0, 8);
+  if (hipMemcpy(h + 8, dout + 8, res.payload_bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    free(h);
+    return nullptr;
+  }
+  *outSize = total;
+  return h;
+}
+
+template <typename Rec>
+unsigned char* compress_topo_impl(const float* data, size_t* outSize, float eb, size_t n, int bs, const Rec* cps,
+                                  int count, int ndim, const uint32_t* dims) {
+  *outSize = 0;
+  if (!data || n == 0 || !(eb > 0) || check_bs(bs, true) || check_n(n)) {
+    if (!(eb > 0)) fail(TSZP_EINVAL, "error bound must be positive");
+    return nullptr;
+  }
+  // type map from the caller's list (SZp/src/szp_float.cc:1040-1064)
+  std::vector<uint8_t> types(n, 0);
+  for (int i = 0; cps && i < count; i++) {
+    const int32_t* p = reinterpret_cast<const int32_t*>(&cps[i]);
+    const int t = ndim == 2 ? p[2] : p[3];
+    int64_t f;
+    if (ndim == 2) {
+      if (p[0] < 0 || p[0] >= (int)dims[0] || p[1] < 0 || p[1] >= (int)dims[1]) continue;
+      f = (int64_t)p[0] * dims[1] + p[1];
+    } else {
+      if (p[0] < 0 || p[0] >= (int)dims[0] || p[1] < 0 || p[1] >= (int)dims[1] || p[2] < 0 ||
+          p[2] >= (int)dims[2])
+        continue;
+      f = ((int64_t)p[0] * dims[1] + p[1]) * dims[2] + p[2];
+    }
+    if (t >= 1 && t <= 3 && (size_t)f < n) types[f] = (uint8_t)t;
+  }
+  Ctx* c;
+  if (get_ctx(&c)) return nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  float* din;
+  uint8_t *dt, *dout;
+  uint32_t* dblk;
+  const size_t cap = tszp_topo_bound(n, bs, 1);
+  if (c->get(kWsIn, n * 4, (void**)&din) || c->get(kWsTypes, n, (void**)&dt) || c->get(kWsOut, cap, (void**)&dout) ||
+      c->get(kWsBlk, (tszp_blocks(n, bs) + 1) * 4, (void**)&dblk))
+    return nullptr;
+  if (hipMemcpyAsync(din, data, n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dt, types.data(), n, hipMemcpyHostToDevice, st) != hipSuccess)
+    return nullptr;
+  CompressResult res;
+  if (run_codec(c, kModeTopoMap, din, (uint32_t)n, bs, eb, 1.0 / (double)eb, nullptr, dt, dout, cap, 8, dblk,
+                nullptr, nullptr, 0, st, &res))
+    return nullptr;
+  const size_t total = 8 + res.payload_bytes;
+  unsigned char* h = (unsigned char*)malloc(total);
+  if (!h) return nullptr;
+  memset(h, 0, 8);
+  if (hipMemcpy(h + 8, dout + 8, res.payload_bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    free(h);
+    return nullptr;
+  }
+  *outSize = total;
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+CriticalPoint* szp_cuda_find_critical_points(float* data, size_t* outCount, int rows, int cols, float eb) {
+  g_err.clear();
+  *outCount = 0;
+  if (!data || rows <= 2 || cols <= 2) return nullptr;
+  const uint32_t dims[2] = {(uint32_t)rows, (uint32_t)cols};
+  CriticalPoint* out;
+  if (find_cps_impl(data, 2, dims, eb, &out, outCount)) return nullptr;
+  return out;
+}
+
+CriticalPoint3D* szp_cuda_find_critical_points_3d(float* data, size_t* outCount, int d1, int d2, int d3, float eb) {
+  g_err.clear();
+  *outCount = 0;
+  if (!data || d1 <= 2 || d2 <= 2 || d3 <= 2) return nullptr;
+  const uint32_t dims[3] = {(uint32_t)d1, (uint32_t)d2, (uint32_t)d3};
+  CriticalPoint3D* out;
+  if (find_cps_impl(data, 3, dims, eb, &out, outCount)) return nullptr;
+  return out;
+}
+
+void szp_cuda_sort_critical_points_by_original_data(CriticalPoint* cps, size_t count, float* data, int cols) {
+  g_err.clear();
+  if (!cps || count == 0 || !data || cols <= 0) return;
+  sort_cps_impl(cps, count, data, (uint32_t)cols, 0);
+}
+
+void szp_cuda_sort_critical_points_3d(CriticalPoint3D* cps, size_t count, float* data, int d2, int d3) {
+  g_err.clear();
+  if (!cps || count == 0 || !data || d2 <= 0 || d3 <= 0) return;
+  sort_cps_impl(cps, count, data, (uint32_t)d3, (uint64_t)d2 * d3);
+}
+
+unsigned char* szp_cuda_compress_sort_positions(CriticalPoint* cps, size_t count, size_t* outSize, int bs) {
+  g_err.clear();
+  return compress_sp_impl(cps, count, outSize, bs);
+}
+
+unsigned char* szp_cuda_compress_sort_positions_3d(CriticalPoint3D* cps, size_t count, size_t* outSize, int bs) {
+  g_err.clear();
+  return compress_sp_impl(cps, count, outSize, bs);
+}
+
+unsigned char* szp_cuda_float_compress_randomaccess_topology_preserved(float* data, size_t* outSize, float eb,
+                                                                       size_t n, int bs, CriticalPoint* cps,
+                                                                       int count, int rows, int cols) {
+  g_err.clear();
+  *outSize = 0;
+  if (rows <= 0 || cols <= 0) return nullptr;
+  const uint32_t dims[2] = {(uint32_t)rows, (uint32_t)cols};
+  return compress_topo_impl(data, outSize, eb, n, bs, cps, count, 2, dims);
+}
+
+unsigned char* szp_cuda_float_compress_topology_3d(float* data, size_t* outSize, float eb, size_t n, int bs,
+                                                   CriticalPoint3D* cps, int count, int d1, int d2, int d3) {
+  g_err.clear();
+  *outSize = 0;
+  if (d1 <= 0 || d2 <= 0 || d3 <= 0) return nullptr;
+  const uint32_t dims[3] = {(uint32_t)d1, (uint32_t)d2, (uint32_t)d3};
+  return compress_topo_impl(data, outSize, eb, n, bs, cps, count, 3, dims);
+}
+
+size_t szp_cuda_float_compress_randomaccess_bound(size_t n, int bs) {
+  g_err.clear(); return tszp_plain_bound(n, bs, 1); }
+
+void szp_cuda_float_compress_randomaccess_arg(unsigned char* output, float* data, size_t* outSize, float eb,
+                                              size_t n, int bs) {
+  g_err.clear();
+  *outSize = 0;
+  if (!output || !data || !(eb > 0) || check_bs(bs, false) || check_n(n)) return;
+  Ctx* c;
+  if (get_ctx(&c)) return;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  float* din;
+  uint8_t* dout;
+  uint32_t* dblk;
+  const size_t cap = tszp_plain_bound(n, bs, 1);
+  if (c->get(kWsIn, n * 4, (void**)&din) || c->get(kWsOut, cap, (void**)&dout) ||
+      c->get(kWsBlk, (tszp_blocks(n, bs) + 1) * 4, (void**)&dblk))
+    return;
+  if (hipMemcpyAsync(din, data, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return;
+  volatile float finv = 1 / eb;
+  CompressResult res;
+  if (run_codec(c, kModePlain, din, (uint32_t)n, bs, eb, (double)finv, nullptr, nullptr, dout, cap, 12, dblk,
+                nullptr, nullptr, 0, st, &res))
+    return;
+  uint32_t u;
+  memcpy(&u, &eb, 4);
+  output[0] = (unsigned char)(u >> 24);
+  output[1] = (unsigned char)(u >> 16);
+  output[2] = (unsigned char)(u >> 8);
+  output[3] = (unsigned char)u;
+  memset(output + 4, 0, 8);
+  if (hipMemcpy(output + 12, dout + 12, res.payload_bytes, hipMemcpyDeviceToHost) != hipSuccess) return;
+  *outSize = 12 + res.payload_bytes;
+}
+
+unsigned char* szp_cuda_float_compress_randomaccess(float* data, size_t* outSize, float eb, size_t n, int bs) {
+  g_err.clear();
+  *outSize = 0;
+  if (!data || check_bs(bs, false) || check_n(n)) return nullptr;
+  unsigned char* out = (unsigned char*)malloc(tszp_plain_bound(n, bs, 1));
+  if (!out) return nullptr;
+  szp_cuda_float_compress_randomaccess_arg(out, data, outSize, eb, n, bs);
+  if (*outSize == 0) {
+    free(out);
+    return nullptr;
+  }
+  unsigned char* shrunk = (unsigned char*)realloc(out, *outSize);
+  return shrunk ? shrunk : out;
+}
+
+void szp_cuda_float_decompress_randomaccess_arg(float* newData, size_t n, float eb, int bs, unsigned char* cmp) {
+  g_err.clear();
+  if (!newData || !cmp || check_bs(bs, false) || check_n(n)) return;
+  Ctx* c;
+  if (get_ctx(&c)) return;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  unsigned T;
+  uint8_t* dp;
+  uint32_t* db;
+  float* dout;
+  if (upload_stream(c, cmp, n, bs, false, 4 * n + 4, &T, &dp, &db, st)) return;
+  if (c->get(kWsOut2, n * 4, (void**)&dout)) return;
+  if (decode(c, kModePlain, dp, db, n, bs, eb, dout, nullptr, nullptr, st)) return;
+  if (hipMemcpy(newData, dout, n * 4, hipMemcpyDeviceToHost) != hipSuccess) fail(TSZP_EHIP, "hipMemcpy");
+}
+
+float* szp_cuda_float_decompress_randomaccess(size_t n, float eb, int bs, unsigned char* cmp) {
+  g_err.clear();
+  if (!cmp || check_n(n)) return nullptr;
+  float* out = (float*)malloc(n * 4);
+  if (!out) return nullptr;
+  g_err.clear();
+  szp_cuda_float_decompress_randomaccess_arg(out, n, eb, bs, cmp);
+  if (!g_err.empty()) {
+    free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+void szp_cuda_float_decompress_randomaccess_topology_preserved(float** newData, size_t n, float eb, int bs,
+                                                               unsigned char* cmp, int** FN) {
+  g_err.clear();
+  *newData = nullptr;
+  *FN = nullptr;
+  if (!cmp || n == 0 || check_bs(bs, true) || check_n(n)) return;
+  Ctx* c;
+  if (get_ctx(&c)) return;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  unsigned T;
+  uint8_t* dp;
+  uint32_t* db;
+  float* dout;
+  int32_t* dfn;
+  if (upload_stream(c, cmp, n, bs, true, 8 * n + 1024, &T, &dp, &db, st)) return;
+  if (c->get(kWsOut2, n * 4, (void**)&dout) || c->get(kWsTypes, n * 4, (void**)&dfn)) return;
+  if (decode(c, kModeTopoMap, dp, db, n, bs, eb, dout, nullptr, dfn, st)) return;
+  float* h = (float*)malloc(n * 4);
+  int* f = (int*)malloc(n * 4);
+  if (!h || !f || hipMemcpy(h, dout, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(f, dfn, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    free(h);
+    free(f);
+    fail(TSZP_EHIP, "copy back failed");
+    return;
+  }
+  *newData = h;
+  *FN = f;
+}
+
+int* szp_cuda_decompress_sort_positions(unsigned char* cmp, size_t count, int bs) {
+  g_err.clear();
+  if (!cmp || count == 0 || check_bs(bs, true) || check_n(count)) return nullptr;
+  Ctx* c;
+  if (get_ctx(&c)) return nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  unsigned T;
+  uint8_t* dp;
+  uint32_t* db;
+  int32_t* dout;
+  if (upload_stream(c, cmp, count, bs, false, 4 * count + 4, &T, &dp, &db, st)) return nullptr;
+  if (c->get(kWsOut2, count * 4, (void**)&dout)) return nullptr;
+  if (decode(c, kModeInt, dp, db, count, bs, 0.f, dout, nullptr, nullptr, st)) return nullptr;
+  int* h = (int*)malloc(count * 4);
+  if (!h) return nullptr;
+  if (hipMemcpy(h, dout, count * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    free(h);
+    return nullptr;
+  }
+  return h;
+}
+
+}  // extern "C"

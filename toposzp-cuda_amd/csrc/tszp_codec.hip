@@ -1,0 +1,505 @@
+// tszp_codec.hip -- SZp block codec kernels for gfx950.
+//
+// One kernel compresses a field in a single pass over HBM:
+//   quantize (+ critical-point classification on the fly) -> per-block delta,
+//   bit count and byte size -> tile scan -> decoupled look-back for the tile's
+//   global byte offset (and critical-point counts) -> bit packing in LDS ->
+//   coalesced copy of the tile's bytes to the stream.
+// The byte stream is the reference CPU writer's (SZp/src/szp_float.cc:998-1318 for
+// topology, :2061-2292 plain, :521-996 sort positions); see the block layout in
+// put_block() of oracle/oracle.c.  A side-car of u32 block offsets is written next
+// to the stream so that decoding is fully parallel.
+//
+// Work decomposition: a workgroup (4 waves) owns a tile of BPT consecutive SZp
+// blocks.  Within a wave, one "step" covers G = 64 / S blocks, each block spread
+// over S lanes with EPL consecutive elements per lane (S * EPL >= blockSize).
+#include "tszp_device.hpp"
+#include "tszp_kernels.h"
+
+namespace tszp {
+
+template <int S, int EPL>
+struct Geom {
+  static constexpr int G = kWave / S;                        // blocks per wave step
+  static constexpr int SPW = EPL >= 16 ? 1 : 16 / EPL;       // steps per wave
+  static constexpr int BPT = kWavesPerBlock * SPW * G;       // blocks per tile
+  static constexpr int BSMAX = S * EPL;                      // largest block size
+  static constexpr int worst_block(bool topo) {
+    return 5 + (BSMAX - 1 + 7) / 8 + 4 * (BSMAX - 1) + (topo ? (2 * BSMAX + 7) / 8 : 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// element access per mode
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ int quantize(const CompressArgs& a, uint32_t e, float* vout) {
+  if constexpr (MODE == kModeInt) {
+    *vout = 0.f;
+    return reinterpret_cast<const int*>(a.in)[e];
+  } else {
+    float v = reinterpret_cast<const float*>(a.in)[e];
+    *vout = v;
+    return cvt_i32((double)v * a.inv);
+  }
+}
+
+// critical-point type of element e (0 regular, 1 max, 2 min, 3 saddle);
+// SZp/src/szp_float.cc:73-90 (4 neighbours) and SZp/src/szp_topology_3d.cc:79-98 (6)
+__device__ __forceinline__ int classify_at(const float* __restrict__ f, const Grid& g, uint32_t e, float c) {
+  if (g.ndim == 2) {
+    uint32_t i = fdiv(e, g.div_row);
+    uint32_t j = e - i * g.d1;
+    if (i == 0 || j == 0 || i + 1 >= g.d0 || j + 1 >= g.d1) return 0;
+    float up = f[e - g.d1], dn = f[e + g.d1], lf = f[e - 1], rt = f[e + 1];
+    if (c > up && c > dn && c > lf && c > rt) return 1;
+    if (c < up && c < dn && c < lf && c < rt) return 2;
+    if ((c < up && c < dn && c > lf && c > rt) || (c > up && c > dn && c < lf && c < rt)) return 3;
+    return 0;
+  } else if (g.ndim == 3) {
+    uint32_t x = fdiv(e, g.div_plane);
+    uint32_t r = e - x * g.plane;
+    uint32_t y = fdiv(r, g.div_row);
+    uint32_t z = r - y * g.d2;
+    if (x == 0 || y == 0 || z == 0 || x + 1 >= g.d0 || y + 1 >= g.d1 || z + 1 >= g.d2) return 0;
+    float xm = f[e - g.plane], xp = f[e + g.plane];
+    float ym = f[e - g.d2], yp = f[e + g.d2];
+    float zm = f[e - 1], zp = f[e + 1];
+    if (c > xm && c > xp && c > ym && c > yp && c > zm && c > zp) return 1;
+    if (c < xm && c < xp && c < ym && c < yp && c < zm && c < zp) return 2;
+    int hi = (c > xm && c > xp) + (c > ym && c > yp) + (c > zm && c > zp);
+    int lo = (c < xm && c < xp) + (c < ym && c < yp) + (c < zm && c < zp);
+    return (hi >= 1 && lo >= 1) ? 3 : 0;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// compression
+// ---------------------------------------------------------------------------
+template <int MODE, int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
+  using Gm = Geom<S, EPL>;
+  constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr bool CLASSIFY = (MODE == kModeTopoClassify);
+  constexpr int STAGE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 2;
+
+  __shared__ uint32_t stage[STAGE_WORDS];
+  __shared__ uint32_t blk_off[BPT + 1];
+  __shared__ uint32_t step_base[kWavesPerBlock * SPW];  // cp | ext << 16 (tile-local prefix)
+  __shared__ uint64_t sh_prefix[2];
+  __shared__ uint32_t sh_tile;
+
+  const int tid = threadIdx.x;
+  const int w = tid / kWave, lane = tid & (kWave - 1);
+  const int g = lane / S, ls = lane & (S - 1);
+  const uint32_t bs = a.bs;
+
+  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
+  __syncthreads();
+  const uint32_t tile = sh_tile;
+
+  // ---- phase A: quantize, classify, block sizes -------------------------------
+  int q[SPW][EPL];
+  uint32_t tbits = 0;  // 2-bit types of the lane's SPW*EPL elements (classify/map modes)
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t b = tile * BPT + lb;
+    const uint32_t base = b * bs;
+    const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
+    unsigned step_cp = 0, step_ext = 0;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      int qq = 0, t = 0;
+      if (j < cur) {
+        float v;
+        qq = quantize<MODE>(a, base + j, &v);
+        if constexpr (CLASSIFY) t = classify_at(reinterpret_cast<const float*>(a.in), a.grid, base + j, v);
+        if constexpr (MODE == kModeTopoMap) t = a.tmap[base + j] & 3;
+      }
+      q[s][r] = qq;
+      if constexpr (TOPO) {
+        tbits |= (uint32_t)t << (2 * (s * EPL + r));
+        step_cp += t != 0;
+        step_ext += (t == 1 || t == 2);
+      }
+    }
+    const int nxt = __shfl_down(q[s][0], 1, kWave);
+    unsigned m = 0;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      const int qn = r + 1 < EPL ? q[s][r + 1] : nxt;
+      const unsigned d = (unsigned)qn - (unsigned)q[s][r];
+      const unsigned mag = ((int)d < 0) ? 0u - d : d;
+      if (j + 1 < cur && mag > m) m = mag;
+    }
+    m = seg_max<S>(m);
+    const unsigned bc = bitcount(m);
+    if (ls == 0 && lb < (uint32_t)BPT) blk_off[lb] = cur ? block_bytes(cur, bc, TOPO) : 0u;
+    if constexpr (CLASSIFY) {
+      // per-step totals of the whole wave (packed cp | ext << 16); the per-lane
+      // counts are recomputed from tbits in phase E
+      unsigned tot = seg_sum<kWave>(step_cp | (step_ext << 16));
+      if (lane == 0) step_base[w * SPW + s] = tot;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase B: tile scans + look-back (wave 0) ---------------------------------
+  if (w == 0) {
+    // exclusive scan of block sizes, BPT / 64 consecutive entries per lane
+    constexpr int PER = (BPT + kWave - 1) / kWave;
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = lane * PER + k;
+      loc[k] = i < BPT ? blk_off[i] : 0u;
+      sum += loc[k];
+    }
+    uint32_t incl = seg_incl_scan<kWave>(sum, lane);
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = lane * PER + k;
+      if (i < BPT) blk_off[i] = run;
+      run += loc[k];
+    }
+    const uint32_t tile_bytes = __shfl(incl, kWave - 1, kWave);
+    if (lane == 0) blk_off[BPT] = tile_bytes;
+
+    uint64_t cnt_agg = 0;
+    if constexpr (CLASSIFY) {
+      // exclusive prefix over the (wave, step) totals in element order
+      constexpr int NS = kWavesPerBlock * SPW;  // <= 64
+      uint32_t v = lane < NS ? step_base[lane] : 0u;
+      uint32_t inc = seg_incl_scan<kWave>(v, lane);
+      if (lane < NS) step_base[lane] = inc - v;
+      uint32_t tot = __shfl(inc, kWave - 1, kWave);
+      cnt_agg = (uint64_t)(tot & 0xffffu) | ((uint64_t)(tot >> 16) << 31);
+    }
+    uint64_t pb = lookback(a.st_bytes, tile, tile_bytes, &a.res->error);
+    uint64_t pc = 0;
+    if constexpr (CLASSIFY) pc = lookback(a.st_cnt, tile, cnt_agg, &a.res->error);
+    if (lane == 0) {
+      sh_prefix[0] = pb;
+      sh_prefix[1] = pc;
+      if (tile == a.ntiles - 1) {
+        a.res->payload_bytes = pb + tile_bytes;
+        if constexpr (CLASSIFY) {
+          uint64_t tc = pc + cnt_agg;
+          a.res->cp_count = tc & 0x7fffffffull;
+          a.res->ext_count = (tc >> 31) & 0x7fffffffull;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- phase C: pack every block into the LDS stage -----------------------------
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t b = tile * BPT + lb;
+    const uint32_t base = b * bs;
+    const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
+    const int nxt = __shfl_down(q[s][0], 1, kWave);
+    unsigned mags[EPL];
+    bool negs[EPL];
+    unsigned m = 0;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      const int qn = r + 1 < EPL ? q[s][r + 1] : nxt;
+      const unsigned d = (unsigned)qn - (unsigned)q[s][r];
+      negs[r] = (int)d < 0;
+      mags[r] = negs[r] ? 0u - d : d;
+      if (j + 1 >= cur) { mags[r] = 0; negs[r] = false; }
+      if (mags[r] > m) m = mags[r];
+    }
+    m = seg_max<S>(m);
+    if (cur == 0) continue;  // uniform per segment; no shuffles below
+    const unsigned bc = bitcount(m);
+    const unsigned off = blk_off[lb];
+    const unsigned n = cur - 1u;
+    if (ls == 0) {
+      lds_or_bytes(stage, off, (uint32_t)q[s][0], 4);
+      lds_or_bytes(stage, off + 4, bc, 1);
+    }
+    const unsigned sB = bc ? (n + 7u) >> 3 : 0u;
+    const unsigned nfull = bc >> 3, rem = bc & 7u;
+    const unsigned fbase = off + 5u + sB;
+    const unsigned rbase = fbase + nfull * n;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      if (bc && j < n) {
+        if (negs[r]) lds_or_bits_msb(stage, off + 5u, j, 1u, 1u);
+        if (nfull) lds_or_bytes(stage, fbase + j * nfull, mags[r] >> rem, nfull);
+        if (rem) lds_or_bits_msb(stage, rbase, j * rem, mags[r], rem);
+      }
+      if constexpr (TOPO) {
+        if (j < cur) {
+          const unsigned t = (tbits >> (2 * (s * EPL + r))) & 3u;
+          const unsigned tbase = off + 5u + (bc ? sB + mag_bytes(n, bc) : 0u);
+          if (t) lds_or_bits_msb(stage, tbase, 2u * j, t, 2u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- phase D: stage -> stream, block offsets side-car --------------------------
+  const uint64_t g0 = sh_prefix[0];
+  const uint32_t len = blk_off[BPT];
+  if (g0 + len > a.cap) {
+    if (tid == 0) atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
+  } else if (len) {
+    uint8_t* dst = a.payload;
+    const uint64_t a0 = (g0 + 3) & ~3ull, e0 = g0 + len;
+    const uint64_t a1 = e0 & ~3ull;
+    if (a0 >= e0 || a1 <= a0) {
+      for (uint32_t i = tid; i < len; i += kBlockThreads) dst[g0 + i] = (uint8_t)lds_byte(stage, i);
+    } else {
+      const uint32_t head = (uint32_t)(a0 - g0), tail = (uint32_t)(e0 - a1);
+      if (tid < (int)head) dst[g0 + tid] = (uint8_t)lds_byte(stage, tid);
+      if (tid < (int)tail) dst[a1 + tid] = (uint8_t)lds_byte(stage, (uint32_t)(a1 - g0) + tid);
+      const uint32_t nw = (uint32_t)((a1 - a0) >> 2);
+      uint32_t* dw = reinterpret_cast<uint32_t*>(dst + a0);
+      for (uint32_t i = tid; i < nw; i += kBlockThreads) {
+        const uint32_t sb = head + 4 * i;
+        dw[i] = lds_byte(stage, sb) | (lds_byte(stage, sb + 1) << 8) | (lds_byte(stage, sb + 2) << 16) |
+                (lds_byte(stage, sb + 3) << 24);
+      }
+    }
+  }
+  for (int lb = tid; lb < BPT; lb += kBlockThreads) {
+    const uint32_t b = tile * BPT + lb;
+    if (b < a.nb) a.blk[b] = (uint32_t)(g0 + blk_off[lb]);
+  }
+  if (tile == a.ntiles - 1 && tid == 0) a.blk[a.nb] = (uint32_t)(g0 + len);
+
+  // ---- phase E: critical-point records in row-major order ------------------------
+  if constexpr (CLASSIFY) {
+    if (a.cp_key) {
+      const uint64_t pc = sh_prefix[1];
+      const uint32_t cp0 = (uint32_t)(pc & 0x7fffffffull), ext0 = (uint32_t)((pc >> 31) & 0x7fffffffull);
+      for (int s = 0; s < SPW; s++) {
+        unsigned c = 0, x = 0;
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const unsigned t = (tbits >> (2 * (s * EPL + r))) & 3u;
+          c += t != 0;
+          x += (t == 1 || t == 2);
+        }
+        const unsigned packed = c | (x << 16);
+        const unsigned incl = seg_incl_scan<kWave>(packed, lane);
+        const unsigned excl = incl - packed;
+        const unsigned sb = step_base[w * SPW + s];
+        uint32_t pcp = cp0 + (sb & 0xffffu) + (excl & 0xffffu);
+        uint32_t pex = ext0 + (sb >> 16) + (excl >> 16);
+        if (!c) continue;
+        const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+        const uint32_t base = (tile * BPT + lb) * bs;
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const unsigned t = (tbits >> (2 * (s * EPL + r))) & 3u;
+          if (!t) continue;
+          const uint32_t e = base + (uint32_t)(ls * EPL + r);
+          if (pcp < a.cp_cap) {
+            a.cp_key[pcp] = okey(reinterpret_cast<const float*>(a.in)[e]);
+            a.cp_pay[pcp] = (t == 3) ? 0xffffffffu : pex;
+          } else {
+            atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
+          }
+          pcp++;
+          if (t != 3) pex++;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decompression (block offsets from the side-car)
+// ---------------------------------------------------------------------------
+template <int MODE, int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
+  using Gm = Geom<S, EPL>;
+  constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr int TILE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 2;
+  __shared__ uint32_t tw[TILE_WORDS];
+
+  const int tid = threadIdx.x;
+  const int w = tid / kWave, lane = tid & (kWave - 1);
+  const int g = lane / S, ls = lane & (S - 1);
+  const uint32_t bs = a.bs;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t b0 = tile * BPT;
+  const uint32_t b1 = min(a.nb, b0 + (uint32_t)BPT);
+
+  const uint32_t t0 = a.blk[b0], t1 = a.blk[b1];
+  const uint32_t wbase = t0 & ~3u;
+  const uint32_t nwords = ((t1 + 3u) & ~3u) / 4u - wbase / 4u;
+  if (t1 < t0 || nwords > (uint32_t)TILE_WORDS) {
+    if (tid == 0) atomicOr(a.err, TSZP_ERR_STREAM);
+    return;
+  }
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.payload + wbase);
+  for (uint32_t i = tid; i < nwords; i += kBlockThreads) tw[i] = src[i];
+  __syncthreads();
+
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t b = b0 + lb;
+    const uint32_t base = b * bs;
+    const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
+    const uint32_t o = cur ? a.blk[b] - wbase : 0u;
+    int anchor = 0;
+    unsigned bc = 0;
+    if (cur) {
+      anchor = (int)(lds_byte(tw, o) | (lds_byte(tw, o + 1) << 8) | (lds_byte(tw, o + 2) << 16) |
+                     (lds_byte(tw, o + 3) << 24));
+      // the topology decoder always reads bc (SZp/src/szpd_float.cc:735); the
+      // others only for blocks of more than one element (:487, :958)
+      if (TOPO || cur > 1) bc = lds_byte(tw, o + 4);
+      if (bc > 32) {
+        if (ls == 0) atomicOr(a.err, TSZP_ERR_STREAM);
+        bc = 0;
+      }
+    }
+    const unsigned n = cur ? cur - 1u : 0u;
+    const unsigned sB = bc ? (n + 7u) >> 3 : 0u;
+    const unsigned nfull = bc >> 3, rem = bc & 7u;
+    const unsigned fbase = o + 5u + sB;
+    const unsigned rbase = fbase + nfull * n;
+    int dl[EPL];
+    int lane_sum = 0;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      int d = 0;
+      if (bc && j >= 1 && j < cur) {
+        const unsigned k = j - 1u;  // delta k leads into element j
+        const unsigned neg = (lds_byte(tw, o + 5u + (k >> 3)) >> (7u - (k & 7u))) & 1u;
+        unsigned mag = 0;
+        for (unsigned kk = 0; kk < nfull; kk++) mag |= lds_byte(tw, fbase + k * nfull + kk) << (8u * kk);
+        mag <<= rem;
+        if (rem) {
+          const unsigned pos = k * rem, by = rbase + (pos >> 3), sh = pos & 7u;
+          unsigned win = lds_byte(tw, by) << 8;
+          if (sh + rem > 8u) win |= lds_byte(tw, by + 1);
+          mag |= (win >> (16u - rem - sh)) & ((1u << rem) - 1u);
+        }
+        d = (int)(neg ? 0u - mag : mag);
+      }
+      lane_sum = (int)((unsigned)lane_sum + (unsigned)d);
+      dl[r] = lane_sum;
+    }
+    const int incl = seg_incl_scan<S>(lane_sum, lane);
+    const int excl = (int)((unsigned)incl - (unsigned)lane_sum);
+    if (cur == 0) continue;
+    unsigned tbase = 0;
+    if constexpr (TOPO) tbase = o + 5u + (bc ? sB + mag_bytes(n, bc) : 0u);
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      if (j >= cur) continue;
+      const int qv = (int)((unsigned)anchor + (unsigned)excl + (unsigned)dl[r]);
+      const uint32_t e = base + j;
+      if constexpr (MODE == kModeInt) {
+        reinterpret_cast<int*>(a.out)[e] = qv;
+      } else {
+        reinterpret_cast<float*>(a.out)[e] = __fmul_rn(__int2float_rn(qv), a.eb);
+      }
+      if constexpr (TOPO) {
+        const unsigned t = (lds_byte(tw, tbase + ((2u * j) >> 3)) >> (6u - ((2u * j) & 7u))) & 3u;
+        if (a.types8) a.types8[e] = (uint8_t)t;
+        if (a.types32) a.types32[e] = (int)t;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch on block size
+// ---------------------------------------------------------------------------
+template <int MODE>
+static hipError_t launch_compress_mode(const CompressArgs& a, int S, int EPL, hipStream_t st) {
+#define TSZP_CASE(SS, EE)                                                                  \
+  if (S == SS && EPL == EE) {                                                              \
+    hipLaunchKernelGGL((k_compress<MODE, SS, EE>), dim3(a.ntiles), dim3(kBlockThreads), 0, st, a); \
+    return hipGetLastError();                                                              \
+  }
+  TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
+  TSZP_CASE(32, 1) TSZP_CASE(64, 1) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  if constexpr (MODE == kModePlain || MODE == kModeInt) {
+    TSZP_CASE(64, 16) TSZP_CASE(64, 32)
+  }
+#undef TSZP_CASE
+  return hipErrorInvalidValue;
+}
+
+template <int MODE>
+static hipError_t launch_decode_mode(const DecodeArgs& a, int S, int EPL, uint32_t ntiles, hipStream_t st) {
+#define TSZP_CASE(SS, EE)                                                                 \
+  if (S == SS && EPL == EE) {                                                             \
+    hipLaunchKernelGGL((k_decode<MODE, SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, a); \
+    return hipGetLastError();                                                             \
+  }
+  TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
+  TSZP_CASE(32, 1) TSZP_CASE(64, 1) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  if constexpr (MODE == kModePlain || MODE == kModeInt) {
+    TSZP_CASE(64, 16) TSZP_CASE(64, 32)
+  }
+#undef TSZP_CASE
+  return hipErrorInvalidValue;
+}
+
+void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* bpt) {
+  int s = 1, e = 1;
+  if (bs <= 64) {
+    while ((uint32_t)s < bs) s <<= 1;
+  } else {
+    s = 64;
+    while ((uint32_t)(64 * e) < bs) e <<= 1;
+  }
+  *S = s;
+  *EPL = e;
+  const int spw = e >= 16 ? 1 : 16 / e;
+  *bpt = (uint32_t)(kWavesPerBlock * spw * (kWave / s));
+}
+
+hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st) {
+  int S, EPL;
+  uint32_t bpt;
+  block_geometry(a.bs, &S, &EPL, &bpt);
+  switch (mode) {
+    case kModeTopoClassify: return launch_compress_mode<kModeTopoClassify>(a, S, EPL, st);
+    case kModeTopoMap: return launch_compress_mode<kModeTopoMap>(a, S, EPL, st);
+    case kModePlain: return launch_compress_mode<kModePlain>(a, S, EPL, st);
+    case kModeInt: return launch_compress_mode<kModeInt>(a, S, EPL, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st) {
+  int S, EPL;
+  uint32_t bpt;
+  block_geometry(a.bs, &S, &EPL, &bpt);
+  const uint32_t ntiles = (a.nb + bpt - 1) / bpt;
+  if (ntiles == 0) return hipSuccess;
+  switch (mode) {
+    case kModeTopoClassify:
+    case kModeTopoMap: return launch_decode_mode<kModeTopoMap>(a, S, EPL, ntiles, st);
+    case kModePlain: return launch_decode_mode<kModePlain>(a, S, EPL, ntiles, st);
+    case kModeInt: return launch_decode_mode<kModeInt>(a, S, EPL, ntiles, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace tszp

@@ -1,0 +1,113 @@
+// tszp_kernels.h -- launch interface between the host API (tszp_api.cpp) and the
+// HIP kernels.  Host-only callers see plain structs and launcher functions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tszp_types.h"
+
+namespace tszp {
+
+enum Mode {
+  kModeTopoClassify = 0,  // topology stream, types classified from the field itself
+  kModeTopoMap = 1,       // topology stream, types from a uint8 map (a caller's CP list)
+  kModePlain = 2,         // plain random-access float stream
+  kModeInt = 3,           // int stream (sort positions)
+};
+
+struct CompressArgs {
+  const void* in;         // float[n] or int[n]
+  uint32_t n, bs, nb, ntiles;
+  float eb;
+  double inv;             // reciprocal used for quantization (mode dependent)
+  Grid grid;              // classify mode
+  const uint8_t* tmap;    // map mode
+  uint8_t* payload;       // stream bytes after the chunk-offset header (4-byte aligned)
+  uint64_t cap;           // payload capacity in bytes
+  uint32_t* blk;          // nb + 1 block offsets relative to payload
+  uint64_t* st_bytes;     // look-back status words, ntiles each
+  uint64_t* st_cnt;
+  uint32_t* tile_counter;
+  uint32_t* cp_key;       // classify mode: order-preserving value keys, row-major
+  uint32_t* cp_pay;       // extremum ordinal, or 0xffffffff for saddles
+  uint32_t cp_cap;
+  CompressResult* res;
+};
+
+struct DecodeArgs {
+  const uint8_t* payload; // 4-byte aligned; readable up to the next multiple of 4 past the end
+  const uint32_t* blk;    // nb + 1 block offsets relative to payload
+  uint32_t n, bs, nb;
+  float eb;
+  void* out;              // float[n] or int[n]
+  uint8_t* types8;        // topology: uint8 types (nullable)
+  int32_t* types32;       // topology: int types, the reference FN layout (nullable)
+  uint32_t* err;
+};
+
+void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* blocks_per_tile);
+hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st);
+hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st);
+
+// ---- topology helpers (tszp_topo.hip) ----
+struct ClassifyArgs {
+  const float* f;
+  uint32_t n;
+  Grid grid;
+  float eb;
+  double inv;
+  uint32_t ntiles;
+  uint64_t* st_cnt;
+  uint32_t* tile_counter;
+  uint8_t* types;          // nullable: type per element
+  int32_t* cps;            // nullable: AoS records of `rec` ints: x, y[, z], type, bin, 0
+  int rec;                 // 5 (2D) or 6 (3D)
+  uint32_t cp_cap;
+  CompressResult* res;
+};
+hipError_t launch_classify(const ClassifyArgs& a, hipStream_t st);
+uint32_t classify_tiles(uint32_t n);
+hipError_t launch_cp_keys(const int32_t* cps, uint32_t count, int rec, const float* data, uint64_t data_n,
+                          uint32_t row, uint64_t plane, uint32_t* keys, uint32_t* hi, uint32_t* pay, uint32_t* err,
+                          hipStream_t st);
+
+// ---- helpers (tszp_util.hip) ----
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t st);
+hipError_t launch_minmax(const float* f, size_t n, uint32_t* out2 /* okey min, okey max */,
+                         hipStream_t st);
+hipError_t launch_write_header(uint8_t* stream, int kind, unsigned T, int adaptive, uint32_t nb,
+                               const uint32_t* blk, uint64_t count, float eb, hipStream_t st);
+
+// ---- sort positions (tszp_sort.hip) ----
+constexpr uint32_t kSortTile = 4096;  // items per sort / scan / rank tile
+struct SortWork {
+  uint32_t* keys_alt;      // count
+  uint32_t* hi_alt;        // count (only with a second key word)
+  uint32_t* pay_alt;       // count
+  uint32_t* hist;          // 256 * ceil(count / kSortTile)
+  uint64_t* st;            // look-back words: 8 passes * ceil(256 * tiles / kSortTile) + rank tiles
+  uint32_t* counters;      // 16 tile counters
+  uint32_t* err;
+};
+size_t sort_hist_words(uint32_t count);
+size_t sort_status_words(uint32_t count);
+// Stable LSD radix sort of (key, payload) pairs, 8-bit digits; key_hi (nullable)
+// is a more significant key word.  Results land back in keys / key_hi / pay.
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
+                            const SortWork& wk, hipStream_t st);
+// Dense rank of distinct values inside quantized bins over the sorted keys
+// (SZp/src/szp_float.cc:344-358): rank resets at each new bin and grows at each new
+// value.  Bins come from the float value (by_value: bin = (int)((v + eb) * inv)) or
+// from key_hi (biased bin).  Writes out[pay] = rank for every pay < out_cap
+// (saddles carry pay = 0xffffffff).
+hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint32_t* pay, uint32_t count,
+                       float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st);
+
+// ---- synthetic fields (tszp_gen.hip) ----
+hipError_t launch_gen_blobs(float* f, int ndim, const uint32_t* dims, uint64_t seed, int nblobs,
+                            float smin, float smax, float noise, int waves, hipStream_t st);
+hipError_t launch_gen_fbm(float* f, const uint32_t* dims, uint64_t seed, int octaves, float noise,
+                          hipStream_t st);
+
+}  // namespace tszp

@@ -1,0 +1,353 @@
+"""Python front end of libtoposzp_hip.so (ctypes).
+
+Two layers, mirroring the C-ABI:
+  * the reference-compatible drop-in (szp_cuda_*; numpy arrays in host memory),
+  * the native device API (tszp_*; torch tensors resident on the GPU).
+
+The HIP library is the only implementation: if it is missing or fails to load,
+every call raises.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtoposzp_hip.so")
+
+CP2 = np.dtype([("x", "<i4"), ("y", "<i4"), ("type", "<i4"), ("quantized_bin", "<i4"),
+                ("sort_position", "<i4")])
+CP3 = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("type", "<i4"),
+                ("quantized_bin", "<i4"), ("sort_position", "<i4")])
+
+
+class TszpError(RuntimeError):
+    pass
+
+
+class TopoInfo(C.Structure):
+    _fields_ = [("topo_bytes", C.c_uint64), ("sortpos_bytes", C.c_uint64),
+                ("cp_count", C.c_uint64), ("ext_count", C.c_uint64)]
+
+
+_lib = None
+_libc = C.CDLL("libc.so.6")
+_libc.free.argtypes = [C.c_void_p]
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TszpError(f"{LIB_PATH} is not built; run toposzp-cuda_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i32, u32, f32 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint, C.c_float
+    u64p = C.POINTER(C.c_uint64)
+    L.tszp_last_error.restype = C.c_char_p
+    L.tszp_version.restype = i32
+    for f in ("tszp_topo_bound", "tszp_sortpos_bound", "tszp_plain_bound"):
+        getattr(L, f).argtypes = [sz, i32, u32]
+        getattr(L, f).restype = sz
+    L.tszp_blocks.argtypes = [sz, i32]; L.tszp_blocks.restype = sz
+    L.tszp_rel_to_abs.argtypes = [vp, sz, f32, C.POINTER(f32), C.POINTER(f32), vp]
+    L.tszp_topo_compress.argtypes = [vp, i32, vp, f32, i32, u32, vp, sz, vp, vp, sz, vp, vp,
+                                     C.POINTER(TopoInfo), vp]
+    L.tszp_topo_decompress.argtypes = [vp, vp, u32, sz, f32, i32, vp, vp, vp]
+    L.tszp_sortpos_decompress.argtypes = [vp, vp, u32, sz, i32, vp, vp]
+    L.tszp_plain_compress.argtypes = [vp, sz, f32, i32, u32, vp, sz, vp, u64p, vp]
+    L.tszp_plain_decompress.argtypes = [vp, vp, u32, sz, f32, i32, vp, vp]
+    L.tszp_classify.argtypes = [vp, i32, vp, vp, u64p, vp]
+    L.tszp_gen_blobs.argtypes = [vp, i32, vp, C.c_uint64, i32, f32, f32, f32, i32, vp]
+    L.tszp_gen_fbm.argtypes = [vp, vp, C.c_uint64, i32, f32, vp]
+    for f in ("tszp_rel_to_abs", "tszp_topo_compress", "tszp_topo_decompress",
+              "tszp_sortpos_decompress", "tszp_plain_compress", "tszp_plain_decompress",
+              "tszp_classify", "tszp_gen_blobs", "tszp_gen_fbm"):
+        getattr(L, f).restype = i32
+    # drop-in
+    L.szp_cuda_find_critical_points.argtypes = [vp, C.POINTER(sz), i32, i32, f32]
+    L.szp_cuda_find_critical_points.restype = vp
+    L.szp_cuda_find_critical_points_3d.argtypes = [vp, C.POINTER(sz), i32, i32, i32, f32]
+    L.szp_cuda_find_critical_points_3d.restype = vp
+    L.szp_cuda_sort_critical_points_by_original_data.argtypes = [vp, sz, vp, i32]
+    L.szp_cuda_sort_critical_points_by_original_data.restype = None
+    L.szp_cuda_sort_critical_points_3d.argtypes = [vp, sz, vp, i32, i32]
+    L.szp_cuda_sort_critical_points_3d.restype = None
+    L.szp_cuda_compress_sort_positions.argtypes = [vp, sz, C.POINTER(sz), i32]
+    L.szp_cuda_compress_sort_positions.restype = vp
+    L.szp_cuda_compress_sort_positions_3d.argtypes = [vp, sz, C.POINTER(sz), i32]
+    L.szp_cuda_compress_sort_positions_3d.restype = vp
+    L.szp_cuda_float_compress_randomaccess_topology_preserved.argtypes = [
+        vp, C.POINTER(sz), f32, sz, i32, vp, i32, i32, i32]
+    L.szp_cuda_float_compress_randomaccess_topology_preserved.restype = vp
+    L.szp_cuda_float_compress_topology_3d.argtypes = [vp, C.POINTER(sz), f32, sz, i32, vp, i32,
+                                                      i32, i32, i32]
+    L.szp_cuda_float_compress_topology_3d.restype = vp
+    L.szp_cuda_float_decompress_randomaccess_topology_preserved.argtypes = [
+        C.POINTER(vp), sz, f32, i32, vp, C.POINTER(vp)]
+    L.szp_cuda_float_decompress_randomaccess_topology_preserved.restype = None
+    L.szp_cuda_decompress_sort_positions.argtypes = [vp, sz, i32]
+    L.szp_cuda_decompress_sort_positions.restype = vp
+    L.szp_cuda_float_compress_randomaccess.argtypes = [vp, C.POINTER(sz), f32, sz, i32]
+    L.szp_cuda_float_compress_randomaccess.restype = vp
+    L.szp_cuda_float_compress_randomaccess_arg.argtypes = [vp, vp, C.POINTER(sz), f32, sz, i32]
+    L.szp_cuda_float_compress_randomaccess_arg.restype = None
+    L.szp_cuda_float_compress_randomaccess_bound.argtypes = [sz, i32]
+    L.szp_cuda_float_compress_randomaccess_bound.restype = sz
+    L.szp_cuda_float_decompress_randomaccess.argtypes = [sz, f32, i32, vp]
+    L.szp_cuda_float_decompress_randomaccess.restype = vp
+    L.szp_cuda_float_decompress_randomaccess_arg.argtypes = [vp, sz, f32, i32, vp]
+    L.szp_cuda_float_decompress_randomaccess_arg.restype = None
+    L.tszp_prof_enable.argtypes = [i32]; L.tszp_prof_enable.restype = None
+    L.tszp_prof_reset.argtypes = []; L.tszp_prof_reset.restype = None
+    L.tszp_prof_read.argtypes = [i32, C.POINTER(C.c_double), u64p]; L.tszp_prof_read.restype = i32
+    _lib = L
+    return L
+
+
+PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5}
+
+
+def prof_enable(on=True):
+    lib().tszp_prof_enable(int(on))
+
+
+def prof_reset():
+    lib().tszp_prof_reset()
+
+
+def prof_read():
+    """{kind: (total_ms, launches)} of the event-timed kernels since prof_reset()."""
+    out = {}
+    for k, v in PROF_KINDS.items():
+        ms, n = C.c_double(), C.c_uint64()
+        _check(lib().tszp_prof_read(v, C.byref(ms), C.byref(n)))
+        out[k] = (ms.value, n.value)
+    return out
+
+
+def last_error():
+    return lib().tszp_last_error().decode()
+
+
+def _check(rc):
+    if rc != 0:
+        raise TszpError(f"toposzp_hip error {rc}: {last_error()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _take(ptr, nbytes, dtype):
+    if not ptr:
+        raise TszpError(f"NULL result: {last_error()}")
+    buf = (C.c_ubyte * nbytes).from_address(ptr)
+    out = np.frombuffer(bytes(buf), dtype=dtype).copy()
+    _libc.free(ptr)
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# drop-in (host pointers), reference names
+# ------------------------------------------------------------------------------------------
+def find_critical_points(data, eb):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    cnt = C.c_size_t()
+    if d.ndim == 2:
+        p = lib().szp_cuda_find_critical_points(_p(d), C.byref(cnt), d.shape[0], d.shape[1], eb)
+        dt = CP2
+    else:
+        p = lib().szp_cuda_find_critical_points_3d(_p(d), C.byref(cnt), *d.shape, eb)
+        dt = CP3
+    if not p:
+        if last_error():
+            raise TszpError(last_error())
+        return np.zeros(0, dtype=dt)
+    return _take(p, cnt.value * dt.itemsize, dt)
+
+
+def sort_critical_points(cps, data):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    out = np.ascontiguousarray(cps).copy()
+    if out.size == 0:
+        return out
+    if out.dtype == CP2:
+        lib().szp_cuda_sort_critical_points_by_original_data(_p(out), out.size, _p(d), d.shape[1])
+    else:
+        lib().szp_cuda_sort_critical_points_3d(_p(out), out.size, _p(d), d.shape[1], d.shape[2])
+    return out
+
+
+def compress_sort_positions(cps, bs):
+    c = np.ascontiguousarray(cps)
+    n = C.c_size_t()
+    f = (lib().szp_cuda_compress_sort_positions if c.dtype == CP2
+         else lib().szp_cuda_compress_sort_positions_3d)
+    p = f(_p(c), c.size, C.byref(n), bs)
+    if not p:
+        return np.zeros(0, np.uint8)
+    return _take(p, n.value, np.uint8)
+
+
+def compress_topology(data, eb, bs, cps):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    c = np.ascontiguousarray(cps)
+    n = C.c_size_t()
+    if d.ndim == 2:
+        p = lib().szp_cuda_float_compress_randomaccess_topology_preserved(
+            _p(d), C.byref(n), eb, d.size, bs, _p(c), c.size, d.shape[0], d.shape[1])
+    else:
+        p = lib().szp_cuda_float_compress_topology_3d(_p(d), C.byref(n), eb, d.size, bs, _p(c),
+                                                      c.size, *d.shape)
+    return _take(p, n.value, np.uint8)
+
+
+def decompress_topology(cmp, n, eb, bs):
+    c = np.ascontiguousarray(cmp, dtype=np.uint8)
+    dp, fp = C.c_void_p(), C.c_void_p()
+    lib().szp_cuda_float_decompress_randomaccess_topology_preserved(C.byref(dp), n, eb, bs, _p(c),
+                                                                    C.byref(fp))
+    if not dp.value:
+        raise TszpError(last_error())
+    return _take(dp.value, 4 * n, np.float32), _take(fp.value, 4 * n, np.int32)
+
+
+def decompress_sort_positions(cmp_at_offsets, count, bs):
+    c = np.ascontiguousarray(cmp_at_offsets, dtype=np.uint8)
+    p = lib().szp_cuda_decompress_sort_positions(_p(c), count, bs)
+    return _take(p, 4 * count, np.int32)
+
+
+def compress_plain(data, eb, bs):
+    d = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    n = C.c_size_t()
+    p = lib().szp_cuda_float_compress_randomaccess(_p(d), C.byref(n), eb, d.size, bs)
+    return _take(p, n.value, np.uint8)
+
+
+def decompress_plain(cmp_after_header, n, eb, bs):
+    c = np.ascontiguousarray(cmp_after_header, dtype=np.uint8)
+    p = lib().szp_cuda_float_decompress_randomaccess(n, eb, bs, _p(c))
+    return _take(p, 4 * n, np.float32)
+
+
+# ------------------------------------------------------------------------------------------
+# native device API (torch tensors on the GPU)
+# ------------------------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _dp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dims(shape):
+    return (C.c_uint32 * len(shape))(*shape)
+
+
+def bounds(n, bs, T=1):
+    L = lib()
+    return L.tszp_topo_bound(n, bs, T), L.tszp_sortpos_bound(n, bs, T), L.tszp_plain_bound(n, bs, T)
+
+
+def rel_to_abs(field, rel):
+    eb, rng = C.c_float(), C.c_float()
+    _check(lib().tszp_rel_to_abs(_dp(field), field.numel(), rel, C.byref(eb), C.byref(rng), _stream()))
+    return eb.value, rng.value
+
+
+class TopoBuffers:
+    """Device buffers sized for one field shape / block size / chunk count."""
+
+    def __init__(self, n, bs, T=1, device="cuda"):
+        torch = _torch()
+        tb, sb, _ = bounds(n, bs, T)
+        nb = lib().tszp_blocks(n, bs)
+        self.n, self.bs, self.T = n, bs, T
+        self.topo = torch.empty(tb, dtype=torch.uint8, device=device)
+        self.topo_blk = torch.empty(nb + 1, dtype=torch.int32, device=device)
+        self.sp = torch.empty(sb, dtype=torch.uint8, device=device)
+        self.sp_blk = torch.empty(nb + 1, dtype=torch.int32, device=device)
+        self.sp_values = torch.empty(n, dtype=torch.int32, device=device)
+        self.out = torch.empty(n, dtype=torch.float32, device=device)
+        self.types = torch.empty(n, dtype=torch.uint8, device=device)
+        self.sp_out = torch.empty(n, dtype=torch.int32, device=device)
+
+
+def topo_compress(field, eb, bs, T=1, buf=None):
+    """field: float32 CUDA tensor (2D or 3D).  Returns (info, buffers)."""
+    if buf is None:
+        buf = TopoBuffers(field.numel(), bs, T, field.device)
+    info = TopoInfo()
+    _check(lib().tszp_topo_compress(
+        _dp(field), field.dim(), _dims(field.shape), eb, bs, T, _dp(buf.topo), buf.topo.numel(),
+        _dp(buf.topo_blk), _dp(buf.sp), buf.sp.numel(), _dp(buf.sp_blk), _dp(buf.sp_values),
+        C.byref(info), _stream()))
+    return info, buf
+
+
+def topo_decompress(buf, n, eb, bs, T=1):
+    _check(lib().tszp_topo_decompress(_dp(buf.topo), _dp(buf.topo_blk), T, n, eb, bs,
+                                      _dp(buf.out), _dp(buf.types), _stream()))
+    return buf.out, buf.types
+
+
+def sortpos_decompress(buf, e, bs, T=1):
+    _check(lib().tszp_sortpos_decompress(_dp(buf.sp), _dp(buf.sp_blk), T, e, bs, _dp(buf.sp_out),
+                                         _stream()))
+    return buf.sp_out[:e]
+
+
+def plain_compress(field, eb, bs, T=1, out=None, blk=None):
+    torch = _torch()
+    n = field.numel()
+    if out is None:
+        out = torch.empty(lib().tszp_plain_bound(n, bs, T), dtype=torch.uint8, device=field.device)
+    if blk is None:
+        blk = torch.empty(lib().tszp_blocks(n, bs) + 1, dtype=torch.int32, device=field.device)
+    nbytes = C.c_uint64()
+    _check(lib().tszp_plain_compress(_dp(field), n, eb, bs, T, _dp(out), out.numel(), _dp(blk),
+                                     C.byref(nbytes), _stream()))
+    return nbytes.value, out, blk
+
+
+def plain_decompress(stream, blk, n, eb, bs, T=1, out=None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=stream.device)
+    _check(lib().tszp_plain_decompress(_dp(stream), _dp(blk), T, n, eb, bs, _dp(out), _stream()))
+    return out
+
+
+def classify(field, types=None):
+    torch = _torch()
+    if types is None:
+        types = torch.empty(field.numel(), dtype=torch.uint8, device=field.device)
+    cnt = C.c_uint64()
+    _check(lib().tszp_classify(_dp(field), field.dim(), _dims(field.shape), _dp(types),
+                               C.byref(cnt), _stream()))
+    return types, cnt.value
+
+
+def gen_blobs(shape, seed, nblobs, smin, smax, noise, waves=False, device="cuda"):
+    torch = _torch()
+    f = torch.empty(tuple(shape), dtype=torch.float32, device=device)
+    _check(lib().tszp_gen_blobs(_dp(f), len(shape), _dims(shape), seed, nblobs, smin, smax, noise,
+                                int(waves), _stream()))
+    return f
+
+
+def gen_fbm(shape, seed, octaves=6, noise=0.05, device="cuda"):
+    torch = _torch()
+    f = torch.empty(tuple(shape), dtype=torch.float32, device=device)
+    _check(lib().tszp_gen_fbm(_dp(f), _dims(shape), seed, octaves, noise, _stream()))
+    return f
