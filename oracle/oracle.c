@@ -460,7 +460,7 @@ static size_t write_chunks(unsigned char *hdr, const int *q_all, size_t n, int b
 size_t or_topo_compress(const float *data, size_t n, float eb, int bs,
                         const unsigned char *types, unsigned T, int adaptive, unsigned char *out)
 {
-    if (!(eb > 0) || n == 0 || bs <= 0 || bs > 1024 || T == 0) return 0;
+    if (!(eb > 0) || n == 0 || bs <= 0 || bs > 1025 || T == 0) return 0;
     double inv = 1.0 / eb;
     int *q = (int *)malloc(n * sizeof(int));
     for (size_t i = 0; i < n; i++) q[i] = or_cvt_i32((double)data[i] * inv);
@@ -474,7 +474,7 @@ size_t or_topo_compress(const float *data, size_t n, float eb, int bs,
 size_t or_plain_compress(const float *data, size_t n, float eb, int bs, unsigned T,
                          int adaptive, unsigned char *out)
 {
-    if (!(eb > 0) || n == 0 || bs <= 1 || bs > 1024 || T == 0) return 0;
+    if (!(eb > 0) || n == 0 || bs <= 1 || bs > 1025 || T == 0) return 0;
     float finv = 1 / eb;
     double inv = (double)finv;
     uint32_t bits;
@@ -493,7 +493,7 @@ size_t or_plain_compress(const float *data, size_t n, float eb, int bs, unsigned
 size_t or_compress_sortpos(const int *sp, size_t e, int bs, unsigned T, int adaptive,
                            unsigned char *out)
 {
-    if (!sp || e == 0 || bs <= 0 || bs > 1024 || T == 0) return 0;
+    if (!sp || e == 0 || bs <= 0 || bs > 1025 || T == 0) return 0;
     uint64_t ec = e;
     memcpy(out, &ec, 8);
     return 8 + write_chunks(out + 8, sp, e, bs, NULL, T, adaptive);

@@ -163,11 +163,15 @@ constexpr uint64_t kFlagPre = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+// A status word carries its own payload (flag + value in one 64-bit word), so the
+// look-back needs no ordering with any other memory: relaxed device-scope atomics
+// suffice.  Release/acquire would add an L2 write-back / invalidate per access on
+// this multi-XCD part, which serialises the tiles.
 __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
-  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Called by a full wave of tile `tile`.  Publishes `agg`, walks predecessors 64
