@@ -4,8 +4,9 @@
 
 One step = the whole topology pipeline over one 512^3 float32 field resident in HBM:
 critical points + per-bin sort positions + sort-position stream + topology stream
-(compress), then the topology stream -> field + types and the sort-position stream
--> sort positions (decompress).  value = 4 N bytes of original data per step / step
+(compress), then the topology stream -> field + types, the sort-position stream
+-> sort positions and the topology restoration of the field (decompress, as the
+reference's decompression timing includes its post-process).  value = 4 N bytes of original data per step / step
 time (the reference's GB/s convention, example/test_cuda_compress.cu:95-96).
 
 Also reported: per-stage kernel times (HIP events on the launch stream), the roofline
@@ -101,6 +102,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
     ap.add_argument("--cpu-slab", type=int, default=64, help="planes of the field for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the sharded design is "
@@ -121,11 +123,14 @@ def main():
     log(f"eb={eb:.6g} (rel {args.rel} of range {rng:.6g}), bs={bs}")
     buf = tz.TopoBuffers(n, bs, 1, dev)
 
+    last = {}
+
     def step():
         info, _ = tz.topo_compress(f, eb, bs, T=1, buf=buf)
-        tz.topo_decompress(buf, n, eb, bs, T=1)
-        if info.ext_count:
-            tz.sortpos_decompress(buf, info.ext_count, bs, T=1)
+        out, types = tz.topo_decompress(buf, n, eb, bs, T=1)
+        sp = tz.sortpos_decompress(buf, info.ext_count, bs, T=1) if info.ext_count else None
+        if not args.no_restore:
+            last["restore"] = tz.topo_restore(out, types, shape, eb, sp=sp, n_ext=info.ext_count)
         return info
 
     for i in range(args.warmup):
@@ -164,6 +169,7 @@ def main():
         "decode": (comp_payload + 5 * n) + (sp_payload + 4 * info.ext_count),  # both decoders
         "sort": 4 * 16 * info.cp_count,                      # 4 passes x (key+payload) x (read+write)
         "rank": 12 * info.cp_count,
+        "restore": 0,
     }
     dom = max((k for k in algo), key=lambda k: stages.get(k, (0, 0))[0])
     dom_ms = stages[dom][0] / max(1.0, stages[dom][1]) if stages[dom][1] else stages[dom][0]
@@ -229,6 +235,11 @@ def main():
         "end_to_end_hbm_frac": e2e_frac,
         "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
         "plain_gbs": plain_gbs,
+        "restore": ({"missing": last["restore"].missing, "spurious": last["restore"].spurious,
+                     "changed": last["restore"].changed, "moves": last["restore"].moves,
+                     "pair_moves": last["restore"].pair_moves, "rounds": last["restore"].rounds,
+                     "max_dev_over_eb": last["restore"].max_dev_over_eb}
+                    if "restore" in last else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

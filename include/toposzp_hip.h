@@ -86,9 +86,35 @@ int tszp_plain_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsign
 int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t *d_types,
                   uint64_t *cp_count, void *stream);
 
+/* Topology restoration (no counterpart in the reference library: its post-process is
+ * host code in example/test_cuda_topology.cu:118-326 and test_cuda_topology_3d.cu:77-434).
+ * In place on d_field, which holds the decoded (float)q * eb on entry: every point
+ * stays inside its quantization cell (|y - d| <= 0.45 eb, on the side of the cell that
+ * holds the original value when q != 0) while a colour-class local search and pair
+ * moves make the critical-point type of every interior point equal to d_types.
+ * d_sp (nullable): extrema sort positions in row-major extremum order, used to order
+ * the extrema of a bin.  The counts returned are those of tszp_topo_verify. */
+typedef struct {
+  uint64_t missing;        /* type != 0, restored field regular */
+  uint64_t spurious;       /* type == 0, restored field critical */
+  uint64_t changed;        /* both critical, different types */
+  uint64_t moves;          /* accepted single-point moves */
+  uint64_t pair_moves;     /* accepted pair moves */
+  uint32_t rounds;         /* local-search rounds */
+  float max_dev_over_eb;   /* max |y - d| / eb */
+} tszp_restore_info;
+
+int tszp_topo_restore(float *d_field, const uint8_t *d_types, const int32_t *d_sp, uint64_t n_extrema, int ndim,
+                      const uint32_t *dims, float eb, int max_rounds, tszp_restore_info *info, void *stream);
+
+/* Critical-point mismatches of a field against reference types:
+ * counts[0] missing, counts[1] spurious, counts[2] changed type. */
+int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, const uint32_t *dims,
+                     uint64_t *counts, void *stream);
+
 /* Kernel timing with HIP events on the launch stream (off by default).  Kinds:
  * 0 topology codec (classify + pack), 1 int codec (sort positions), 2 plain codec,
- * 3 decoders, 4 radix sort, 5 rank.  tszp_prof_read returns the summed
+ * 3 decoders, 4 radix sort, 5 rank, 6 topology restoration (whole call).  tszp_prof_read returns the summed
  * milliseconds and the number of timed launches since tszp_prof_reset. */
 void tszp_prof_enable(int on);
 void tszp_prof_reset(void);

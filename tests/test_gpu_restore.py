@@ -1,0 +1,91 @@
+"""GPU topology restoration: after decompression + restore, every interior point must
+have the critical-point type of the original field (checked by the CPU oracle's
+classifier), with every value inside its quantization cell."""
+import numpy as np
+import pytest
+
+import fields
+import oracle
+import toposzp_hip as tz
+
+pytestmark = pytest.mark.gpu
+
+
+def _pipeline(data, eb, bs=64):
+    import torch
+    assert torch.cuda.is_available()
+    f = torch.from_numpy(np.ascontiguousarray(data)).cuda()
+    info, buf = tz.topo_compress(f, eb, bs)
+    out, types = tz.topo_decompress(buf, data.size, eb, bs)
+    sp = tz.sortpos_decompress(buf, info.ext_count, bs) if info.ext_count else None
+    dec = out.clone()
+    rinfo = tz.topo_restore(out, types, data.shape, eb, sp=sp, n_ext=info.ext_count)
+    torch.cuda.synchronize()
+    return dec.cpu().numpy(), out.cpu().numpy().reshape(data.shape), types.cpu().numpy(), rinfo
+
+
+def _check_cells(x, dec, y, eb):
+    x, dec, y = x.ravel().astype(np.float64), dec.ravel().astype(np.float64), y.ravel().astype(np.float64)
+    assert np.all(np.abs(y - dec) <= 0.45 * eb * (1 + 1e-5) + np.abs(dec) * 2.0 ** -22)
+    nz = dec != 0
+    # outside the zero cell the value stays on the original's side of the cell: error < eb
+    assert np.all(np.abs(y - x)[nz] < eb * (1 + 1e-4) + np.abs(x[nz]) * 2.0 ** -21)
+    assert np.all(np.abs(y - x)[~nz] < 1.45 * eb * (1 + 1e-4))
+
+
+NOISY = [
+    ("blobs2d_cfg0", lambda: fields.blobs2d(450, 900, nblobs=64, noise=0.01, seed=1), 1e-3),
+    ("blobs2d_b", lambda: fields.blobs2d(300, 700, nblobs=64, noise=0.01, seed=7), 1e-3),
+    ("ties_coarse", lambda: fields.ties2d(300, 300, seed=5, step=0.001), 5e-3),
+    ("fbm3d", lambda: fields.fbm3d(48, 64, 80, seed=3), 3e-3),
+    ("blobs3d", lambda: fields.blobs3d(40, 60, 70, seed=4, noise=2e-3), 2e-4),
+]
+
+
+@pytest.mark.parametrize("name,make,eb", NOISY, ids=[c[0] for c in NOISY])
+def test_restore_zero_mismatches(name, make, eb):
+    data = make()
+    dec, y, types, info = _pipeline(data, eb)
+    want = oracle.classify(data).ravel()
+    np.testing.assert_array_equal(types, want)
+    got = oracle.classify(y).ravel()
+    raw = oracle.classify(dec.reshape(data.shape)).ravel()
+    print(f"{name}: cps={np.count_nonzero(want)} raw mismatches={np.count_nonzero(raw != want)} "
+          f"after={np.count_nonzero(got != want)} moves={info.moves} pairs={info.pair_moves} "
+          f"rounds={info.rounds} maxdev={info.max_dev_over_eb:.3f}")
+    assert info.mismatches() == np.count_nonzero(got != want)
+    assert np.count_nonzero(got != want) == 0
+    _check_cells(data, dec, y, eb)
+
+
+SMOOTH = [
+    ("smooth2d", lambda: fields.blobs2d(400, 600, nblobs=20, noise=0.0005, seed=2), 1e-3),
+    ("smooth2d_coarse", lambda: fields.blobs2d(400, 600, nblobs=20, noise=0.0, seed=3), 1e-2),
+]
+
+
+@pytest.mark.parametrize("name,make,eb", SMOOTH, ids=[c[0] for c in SMOOTH])
+def test_restore_smooth_fields(name, make, eb):
+    """Smooth fields with coarse bounds have large flat cells; the repair must at least
+    remove almost every mismatch and never leave the cells."""
+    data = make()
+    dec, y, types, info = _pipeline(data, eb)
+    want = oracle.classify(data).ravel()
+    got = oracle.classify(y).ravel()
+    raw = oracle.classify(dec.reshape(data.shape)).ravel()
+    after = np.count_nonzero(got != want)
+    print(f"{name}: cps={np.count_nonzero(want)} raw={np.count_nonzero(raw != want)} after={after}")
+    assert after == info.mismatches()
+    assert after <= max(2, np.count_nonzero(raw != want) // 50)
+    _check_cells(data, dec, y, eb)
+
+
+def test_verify_counts_match_oracle():
+    import torch
+    data = fields.blobs2d(200, 300, seed=9, noise=0.02)
+    types = torch.from_numpy(oracle.classify(data).ravel()).cuda()
+    other = fields.blobs2d(200, 300, seed=10, noise=0.02)
+    counts = tz.topo_verify(torch.from_numpy(other).cuda(), types, data.shape)
+    a, b = oracle.classify(data).ravel(), oracle.classify(other).ravel()
+    assert counts == (int(np.count_nonzero((a != 0) & (b == 0))), int(np.count_nonzero((a == 0) & (b != 0))),
+                      int(np.count_nonzero((a != 0) & (b != 0) & (a != b))))

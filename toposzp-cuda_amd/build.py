@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libtoposzp_hip.so")
 ARCH = os.environ.get("TSZP_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["tszp_codec.hip", "tszp_sort.hip", "tszp_topo.hip", "tszp_util.hip", "tszp_gen.hip",
-           "tszp_api.cpp"]
+           "tszp_post.hip", "tszp_api.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -31,8 +31,9 @@ def hipcc():
 def compile_one(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
-    if src.endswith(".cpp"):
-        cmd = [hipcc(), *FLAGS[:5], "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, src), "-o", out]
+    if src.endswith(".cpp"):  # host code only
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "--cuda-host-only", *FLAGS[:5], "-c",
+               os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

@@ -46,6 +46,7 @@ int fail(int code, const char* fmt, ...) {
 enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
+  kWsPpD, kWsPpBad, kWsPpAct, kWsPpDone, kWsPpLocks, kWsPpExt, kWsPpMisc, kWsPpSt,
   kWsCount
 };
 
@@ -81,7 +82,8 @@ std::vector<Ctx*> g_ctx;
 
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
-enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfCount };
+enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
+                kProfCount };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -92,12 +94,11 @@ struct ProfScope {
   hipStream_t st;
   ProfScope(int k, hipStream_t s) : kind(k), st(s) {
     if (!g_prof) return;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    hipEventRecord(a, st);
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(a, st);
   }
   void stop() {
-    if (a) hipEventRecord(b, st);
+    if (a) (void)hipEventRecord(b, st);
   }
   // call after the stream has been synchronized
   void collect() {
@@ -107,8 +108,8 @@ struct ProfScope {
       g_prof_ms[kind] += ms;
       g_prof_n[kind]++;
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     a = b = nullptr;
   }
   ~ProfScope() { collect(); }
@@ -633,6 +634,144 @@ int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t*
   CompressResult out;
   if ((r = read_result(c, res, st, &out))) return r;
   if (cp_count) *cp_count = out.cp_count;
+  return TSZP_OK;
+}
+
+int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_sp, uint64_t n_ext, int ndim,
+                      const uint32_t* dims, float eb, int max_rounds, tszp_restore_info* info, void* stream) {
+  g_err.clear();
+  if (!d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0))
+    return fail(TSZP_EINVAL, "topo_restore: bad arguments");
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const Grid g = make_grid(ndim, dims);
+  const uint32_t N = (uint32_t)n;
+  float* d;
+  uint32_t *bad, *misc, *locks;
+  uint8_t *act, *done;
+  WS(kWsPpD, n * 4, d);
+  const uint32_t badcap = (uint32_t)(n < (1u << 20) ? n : (n / 8 > (1u << 20) ? n / 8 : (1u << 20)));
+  WS(kWsPpBad, (size_t)badcap * 4, bad);
+  WS(kWsPpAct, n, act);
+  WS(kWsPpDone, badcap, done);
+  WS(kWsPpMisc, 64, misc);  // [0] nbad [1] moves [2] pending [4..11] verify (u64) [12] err [13] ext total
+  const uint32_t tx = (g.d0 + 7) / 8, ty = (g.d1 + 7) / 8, tzn = ndim == 3 ? (g.d2 + 7) / 8 : 1;
+  const size_t ntl = (size_t)tx * ty * tzn;
+  WS(kWsPpLocks, ntl * 4, locks);
+  ProfScope prof(kProfRestore, st);
+  HIPCHK(hipMemcpyAsync(d, d_field, n * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(pp_init(d, d_types, d_field, N, eb, st));
+  if (d_sp && n_ext) {
+    uint32_t* ext;
+    uint64_t* sw;
+    WS(kWsPpExt, n_ext * 4, ext);
+    const uint32_t tiles = pp_ext_tiles(N);
+    WS(kWsPpSt, (size_t)tiles * 8 + 16, sw);
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(sw), 0u, (size_t)tiles * 2 + 4, st));
+    HIPCHK(launch_fill_u32(misc, 0u, 16, st));
+    HIPCHK(pp_ext_scan(d_types, N, ext, (uint32_t)n_ext, sw, reinterpret_cast<uint32_t*>(sw + tiles), misc + 12,
+                       misc + 13, st));
+    HIPCHK(pp_extrema(d, d_types, ext, d_sp, (uint32_t)n_ext, misc + 13, N, d_field, eb, st));
+  }
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(act), 0u, (n + 3) / 4, st));
+  uint32_t h[16];
+  info->rounds = 0;
+  info->moves = 0;
+  info->pair_moves = 0;
+  const int colours = ndim == 2 ? 5 : 7;
+  const int phases = ndim == 2 ? 4 : 8;
+  auto mark = [&](uint32_t* nb_out) -> int {
+    HIPCHK(launch_fill_u32(misc, 0u, 3, st));
+    HIPCHK(pp_mark(g, d_field, d_types, N, misc, bad, badcap, st));
+    HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *nb_out = h[0] < badcap ? h[0] : badcap;
+    return TSZP_OK;
+  };
+  for (int outer = 0; outer < 4; outer++) {
+    uint32_t nb = 0;
+    bool progress = false;
+    for (int round = 0; round < max_rounds; round++) {
+      if ((r = mark(&nb))) return r;
+      if (!nb) break;
+      info->rounds++;
+      HIPCHK(pp_active(g, bad, nb, act, st));
+      for (int col = 0; col < colours; col++) HIPCHK(pp_move(g, d, d_types, d_field, act, N, eb, col, misc + 1, st));
+      HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      info->moves += h[1];
+      if (!h[1]) break;
+      progress = true;
+    }
+    if ((r = mark(&nb))) return r;
+    if (!nb) break;
+    // pair moves over the remaining mismatches
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(done), 0u, (nb + 3) / 4, st));
+    uint32_t pm = 0;
+    for (int ph = 0; ph < phases; ph++) {
+      for (int it = 0; it < 64; it++) {
+        HIPCHK(launch_fill_u32(locks, 0u, ntl, st));
+        HIPCHK(launch_fill_u32(misc + 1, 0u, 2, st));
+        HIPCHK(pp_pair(g, d, d_types, d_field, bad, nb, done, locks, tx, ty, ph, eb, misc + 1, misc + 2, st));
+        HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        pm += h[1];
+        if (!h[2]) break;
+      }
+    }
+    info->pair_moves += pm;
+    if (!pm && !progress) break;
+  }
+  // verification against the transmitted types
+  unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + 4);
+  HIPCHK(launch_fill_u32(misc + 4, 0u, 8, st));
+  HIPCHK(pp_verify(g, d_field, d_types, d, N, vo, st));
+  prof.stop();
+  unsigned long long hv[4];
+  HIPCHK(hipMemcpyAsync(hv, vo, 32, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h + 12, misc + 12, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (d_sp && n_ext && h[13] != n_ext)
+    return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[13],
+                (unsigned long long)n_ext);
+  info->missing = hv[0];
+  info->spurious = hv[1];
+  info->changed = hv[2];
+  uint32_t dv = (uint32_t)hv[3];
+  float dev;
+  memcpy(&dev, &dv, 4);
+  info->max_dev_over_eb = dev / eb;
+  return TSZP_OK;
+}
+
+int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, const uint32_t* dims, uint64_t* counts,
+                     void* stream) {
+  g_err.clear();
+  if (!d_field || !d_types || !dims || !counts || (ndim != 2 && ndim != 3))
+    return fail(TSZP_EINVAL, "topo_verify: bad arguments");
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t* misc;
+  WS(kWsPpMisc, 64, misc);
+  unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + 4);
+  HIPCHK(launch_fill_u32(misc + 4, 0u, 8, st));
+  HIPCHK(pp_verify(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, vo, st));
+  unsigned long long hv[4];
+  HIPCHK(hipMemcpyAsync(hv, vo, 32, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int k = 0; k < 3; k++) counts[k] = hv[k];
   return TSZP_OK;
 }
 

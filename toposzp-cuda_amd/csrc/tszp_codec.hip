@@ -33,45 +33,46 @@ struct Geom {
 // element access per mode
 // ---------------------------------------------------------------------------
 template <int MODE>
-__device__ __forceinline__ int quantize(const CompressArgs& a, uint32_t e, float* vout) {
+__device__ __forceinline__ int quantize_raw(const CompressArgs& a, uint32_t raw) {
   if constexpr (MODE == kModeInt) {
-    *vout = 0.f;
-    return reinterpret_cast<const int*>(a.in)[e];
+    return (int)raw;
   } else {
-    float v = reinterpret_cast<const float*>(a.in)[e];
-    *vout = v;
-    return cvt_i32((double)v * a.inv);
+    return cvt_i32((double)__uint_as_float(raw) * a.inv);
   }
 }
 
-// critical-point type of element e (0 regular, 1 max, 2 min, 3 saddle);
-// SZp/src/szp_float.cc:73-90 (4 neighbours) and SZp/src/szp_topology_3d.cc:79-98 (6)
-__device__ __forceinline__ int classify_at(const float* __restrict__ f, const Grid& g, uint32_t e, float c) {
+// Interior test and neighbour offsets of element e; the caller loads the
+// neighbours (many at once) and calls type_of().
+__device__ __forceinline__ bool interior_at(const Grid& g, uint32_t e) {
   if (g.ndim == 2) {
-    uint32_t i = fdiv(e, g.div_row);
-    uint32_t j = e - i * g.d1;
-    if (i == 0 || j == 0 || i + 1 >= g.d0 || j + 1 >= g.d1) return 0;
-    float up = f[e - g.d1], dn = f[e + g.d1], lf = f[e - 1], rt = f[e + 1];
+    const uint32_t i = fdiv(e, g.div_row), j = e - i * g.d1;
+    return i != 0 && j != 0 && i + 1 < g.d0 && j + 1 < g.d1;
+  }
+  if (g.ndim == 3) {
+    const uint32_t x = fdiv(e, g.div_plane), r = e - x * g.plane;
+    const uint32_t y = fdiv(r, g.div_row), z = r - y * g.d2;
+    return x != 0 && y != 0 && z != 0 && x + 1 < g.d0 && y + 1 < g.d1 && z + 1 < g.d2;
+  }
+  return false;
+}
+
+// critical-point type from the centre and its neighbours (0 regular, 1 max, 2 min,
+// 3 saddle); SZp/src/szp_float.cc:73-90 (4 neighbours: up, down, left, right) and
+// SZp/src/szp_topology_3d.cc:79-98 (6 neighbours: -x, +x, -y, +y, -z, +z)
+__device__ __forceinline__ int type_of(int ndim, float c, const float* nb) {
+  if (ndim == 2) {
+    const float up = nb[0], dn = nb[1], lf = nb[2], rt = nb[3];
     if (c > up && c > dn && c > lf && c > rt) return 1;
     if (c < up && c < dn && c < lf && c < rt) return 2;
     if ((c < up && c < dn && c > lf && c > rt) || (c > up && c > dn && c < lf && c < rt)) return 3;
     return 0;
-  } else if (g.ndim == 3) {
-    uint32_t x = fdiv(e, g.div_plane);
-    uint32_t r = e - x * g.plane;
-    uint32_t y = fdiv(r, g.div_row);
-    uint32_t z = r - y * g.d2;
-    if (x == 0 || y == 0 || z == 0 || x + 1 >= g.d0 || y + 1 >= g.d1 || z + 1 >= g.d2) return 0;
-    float xm = f[e - g.plane], xp = f[e + g.plane];
-    float ym = f[e - g.d2], yp = f[e + g.d2];
-    float zm = f[e - 1], zp = f[e + 1];
-    if (c > xm && c > xp && c > ym && c > yp && c > zm && c > zp) return 1;
-    if (c < xm && c < xp && c < ym && c < yp && c < zm && c < zp) return 2;
-    int hi = (c > xm && c > xp) + (c > ym && c > yp) + (c > zm && c > zp);
-    int lo = (c < xm && c < xp) + (c < ym && c < yp) + (c < zm && c < zp);
-    return (hi >= 1 && lo >= 1) ? 3 : 0;
   }
-  return 0;
+  const float xm = nb[0], xp = nb[1], ym = nb[2], yp = nb[3], zm = nb[4], zp = nb[5];
+  if (c > xm && c > xp && c > ym && c > yp && c > zm && c > zp) return 1;
+  if (c < xm && c < xp && c < ym && c < yp && c < zm && c < zp) return 2;
+  const int hi = (c > xm && c > xp) + (c > ym && c > yp) + (c > zm && c > zp);
+  const int lo = (c < xm && c < xp) + (c < ym && c < yp) + (c < zm && c < zp);
+  return (hi >= 1 && lo >= 1) ? 3 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -101,48 +102,105 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   __syncthreads();
   const uint32_t tile = sh_tile;
 
-  // ---- phase A: quantize, classify, block sizes -------------------------------
-  int q[SPW][EPL];
-  uint32_t tbits = 0;  // 2-bit types of the lane's SPW*EPL elements (classify/map modes)
+  // ---- phase A1: every element load of the tile in flight at once ---------------
+  uint32_t raw[SPW][EPL];
+  uint32_t tbits = 0;  // 2-bit types of the lane's SPW*EPL elements (topology modes)
+  const uint32_t* in32 = reinterpret_cast<const uint32_t*>(a.in);
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t b = tile * BPT + (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t base = b * bs;
+    const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
+#pragma unroll
+    for (int r = 0; r < EPL; r++) {
+      const uint32_t j = (uint32_t)(ls * EPL + r);
+      raw[s][r] = j < cur ? in32[base + j] : 0u;
+      if constexpr (MODE == kModeTopoMap) {
+        const uint32_t t = j < cur ? (a.tmap[base + j] & 3u) : 0u;
+        tbits |= t << (2 * (s * EPL + r));
+      }
+    }
+  }
+
+  // ---- phase A2: classification, neighbour loads of 4 steps in flight together ---
+  if constexpr (CLASSIFY) {
+    const float* f = reinterpret_cast<const float*>(a.in);
+    const Grid& gr = a.grid;
+    const int nn = gr.ndim == 3 ? 6 : 4;
+    const uint32_t o1 = gr.ndim == 3 ? gr.plane : gr.d1;  // first axis stride
+    const uint32_t o2 = gr.ndim == 3 ? gr.d2 : 1u;        // second axis stride
+    constexpr int GS = SPW < 4 ? SPW : 4;
+#pragma unroll
+    for (int s0 = 0; s0 < SPW; s0 += GS) {
+      float nbv[GS][EPL][6];
+      bool inner[GS][EPL];
+#pragma unroll
+      for (int k = 0; k < GS; k++) {
+        const int s = s0 + k;
+        const uint32_t b = tile * BPT + (uint32_t)((w * SPW + s) * G + g);
+        const uint32_t base = b * bs;
+        const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const uint32_t j = (uint32_t)(ls * EPL + r);
+          const uint32_t e = base + j;
+          inner[k][r] = j < cur && interior_at(gr, e);
+          if (inner[k][r]) {
+            nbv[k][r][0] = f[e - o1];
+            nbv[k][r][1] = f[e + o1];
+            nbv[k][r][2] = f[e - o2];
+            nbv[k][r][3] = f[e + o2];
+            if (nn == 6) {
+              nbv[k][r][4] = f[e - 1];
+              nbv[k][r][5] = f[e + 1];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < GS; k++) {
+#pragma unroll
+        for (int r = 0; r < EPL; r++) {
+          const int t = inner[k][r] ? type_of(gr.ndim, __uint_as_float(raw[s0 + k][r]), nbv[k][r]) : 0;
+          tbits |= (uint32_t)t << (2 * ((s0 + k) * EPL + r));
+        }
+      }
+    }
+  }
+
+  // ---- phase A3: quantize, block bit counts and sizes ---------------------------
+#pragma unroll
   for (int s = 0; s < SPW; s++) {
     const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
     const uint32_t b = tile * BPT + lb;
     const uint32_t base = b * bs;
     const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
-    unsigned step_cp = 0, step_ext = 0;
+    int q[EPL];
 #pragma unroll
-    for (int r = 0; r < EPL; r++) {
-      const uint32_t j = (uint32_t)(ls * EPL + r);
-      int qq = 0, t = 0;
-      if (j < cur) {
-        float v;
-        qq = quantize<MODE>(a, base + j, &v);
-        if constexpr (CLASSIFY) t = classify_at(reinterpret_cast<const float*>(a.in), a.grid, base + j, v);
-        if constexpr (MODE == kModeTopoMap) t = a.tmap[base + j] & 3;
-      }
-      q[s][r] = qq;
-      if constexpr (TOPO) {
-        tbits |= (uint32_t)t << (2 * (s * EPL + r));
-        step_cp += t != 0;
-        step_ext += (t == 1 || t == 2);
-      }
-    }
-    const int nxt = __shfl_down(q[s][0], 1, kWave);
+    for (int r = 0; r < EPL; r++) q[r] = (uint32_t)(ls * EPL + r) < cur ? quantize_raw<MODE>(a, raw[s][r]) : 0;
+    const int nxt = __shfl_down(q[0], 1, kWave);
     unsigned m = 0;
 #pragma unroll
     for (int r = 0; r < EPL; r++) {
       const uint32_t j = (uint32_t)(ls * EPL + r);
-      const int qn = r + 1 < EPL ? q[s][r + 1] : nxt;
-      const unsigned d = (unsigned)qn - (unsigned)q[s][r];
+      const int qn = r + 1 < EPL ? q[r + 1] : nxt;
+      const unsigned d = (unsigned)qn - (unsigned)q[r];
       const unsigned mag = ((int)d < 0) ? 0u - d : d;
       if (j + 1 < cur && mag > m) m = mag;
     }
     m = seg_max<S>(m);
     const unsigned bc = bitcount(m);
-    if (ls == 0 && lb < (uint32_t)BPT) blk_off[lb] = cur ? block_bytes(cur, bc, TOPO) : 0u;
+    if (ls == 0) blk_off[lb] = cur ? block_bytes(cur, bc, TOPO) : 0u;
     if constexpr (CLASSIFY) {
       // per-step totals of the whole wave (packed cp | ext << 16); the per-lane
       // counts are recomputed from tbits in phase E
+      unsigned step_cp = 0, step_ext = 0;
+#pragma unroll
+      for (int r = 0; r < EPL; r++) {
+        const unsigned t = (tbits >> (2 * (s * EPL + r))) & 3u;
+        step_cp += t != 0;
+        step_ext += (t == 1 || t == 2);
+      }
       unsigned tot = seg_sum<kWave>(step_cp | (step_ext << 16));
       if (lane == 0) step_base[w * SPW + s] = tot;
     }
@@ -201,20 +259,24 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   __syncthreads();
 
   // ---- phase C: pack every block into the LDS stage -----------------------------
+#pragma unroll
   for (int s = 0; s < SPW; s++) {
     const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
     const uint32_t b = tile * BPT + lb;
     const uint32_t base = b * bs;
     const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
-    const int nxt = __shfl_down(q[s][0], 1, kWave);
+    int q[EPL];
+#pragma unroll
+    for (int r = 0; r < EPL; r++) q[r] = (uint32_t)(ls * EPL + r) < cur ? quantize_raw<MODE>(a, raw[s][r]) : 0;
+    const int nxt = __shfl_down(q[0], 1, kWave);
     unsigned mags[EPL];
     bool negs[EPL];
     unsigned m = 0;
 #pragma unroll
     for (int r = 0; r < EPL; r++) {
       const uint32_t j = (uint32_t)(ls * EPL + r);
-      const int qn = r + 1 < EPL ? q[s][r + 1] : nxt;
-      const unsigned d = (unsigned)qn - (unsigned)q[s][r];
+      const int qn = r + 1 < EPL ? q[r + 1] : nxt;
+      const unsigned d = (unsigned)qn - (unsigned)q[r];
       negs[r] = (int)d < 0;
       mags[r] = negs[r] ? 0u - d : d;
       if (j + 1 >= cur) { mags[r] = 0; negs[r] = false; }
@@ -226,7 +288,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
     const unsigned off = blk_off[lb];
     const unsigned n = cur - 1u;
     if (ls == 0) {
-      lds_or_bytes(stage, off, (uint32_t)q[s][0], 4);
+      lds_or_bytes(stage, off, (uint32_t)q[0], 4);
       lds_or_bytes(stage, off + 4, bc, 1);
     }
     const unsigned sB = bc ? (n + 7u) >> 3 : 0u;
@@ -287,6 +349,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
     if (a.cp_key) {
       const uint64_t pc = sh_prefix[1];
       const uint32_t cp0 = (uint32_t)(pc & 0x7fffffffull), ext0 = (uint32_t)((pc >> 31) & 0x7fffffffull);
+#pragma unroll
       for (int s = 0; s < SPW; s++) {
         unsigned c = 0, x = 0;
 #pragma unroll
@@ -302,15 +365,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
         uint32_t pcp = cp0 + (sb & 0xffffu) + (excl & 0xffffu);
         uint32_t pex = ext0 + (sb >> 16) + (excl >> 16);
         if (!c) continue;
-        const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
-        const uint32_t base = (tile * BPT + lb) * bs;
 #pragma unroll
         for (int r = 0; r < EPL; r++) {
           const unsigned t = (tbits >> (2 * (s * EPL + r))) & 3u;
           if (!t) continue;
-          const uint32_t e = base + (uint32_t)(ls * EPL + r);
           if (pcp < a.cp_cap) {
-            a.cp_key[pcp] = okey(reinterpret_cast<const float*>(a.in)[e]);
+            a.cp_key[pcp] = okey(__uint_as_float(raw[s][r]));
             a.cp_pay[pcp] = (t == 3) ? 0xffffffffu : pex;
           } else {
             atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
@@ -350,7 +410,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
     return;
   }
   const uint32_t* src = reinterpret_cast<const uint32_t*>(a.payload + wbase);
-  for (uint32_t i = tid; i < nwords; i += kBlockThreads) tw[i] = src[i];
+  // eight loads in flight per thread before the LDS stores
+  for (uint32_t i0 = 0; i0 < nwords; i0 += 8 * kBlockThreads) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t i = i0 + k * kBlockThreads + tid;
+      v[k] = i < nwords ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t i = i0 + k * kBlockThreads + tid;
+      if (i < nwords) tw[i] = v[k];
+    }
+  }
   __syncthreads();
 
   for (int s = 0; s < SPW; s++) {

@@ -30,6 +30,15 @@ class TopoInfo(C.Structure):
                 ("cp_count", C.c_uint64), ("ext_count", C.c_uint64)]
 
 
+class RestoreInfo(C.Structure):
+    _fields_ = [("missing", C.c_uint64), ("spurious", C.c_uint64), ("changed", C.c_uint64),
+                ("moves", C.c_uint64), ("pair_moves", C.c_uint64), ("rounds", C.c_uint32),
+                ("max_dev_over_eb", C.c_float)]
+
+    def mismatches(self):
+        return self.missing + self.spurious + self.changed
+
+
 _lib = None
 _libc = C.CDLL("libc.so.6")
 _libc.free.argtypes = [C.c_void_p]
@@ -99,6 +108,10 @@ def lib():
     L.szp_cuda_float_decompress_randomaccess.restype = vp
     L.szp_cuda_float_decompress_randomaccess_arg.argtypes = [vp, sz, f32, i32, vp]
     L.szp_cuda_float_decompress_randomaccess_arg.restype = None
+    L.tszp_topo_restore.argtypes = [vp, vp, vp, C.c_uint64, i32, vp, f32, i32, C.POINTER(RestoreInfo), vp]
+    L.tszp_topo_restore.restype = i32
+    L.tszp_topo_verify.argtypes = [vp, vp, i32, vp, u64p, vp]
+    L.tszp_topo_verify.restype = i32
     L.tszp_prof_enable.argtypes = [i32]; L.tszp_prof_enable.restype = None
     L.tszp_prof_reset.argtypes = []; L.tszp_prof_reset.restype = None
     L.tszp_prof_read.argtypes = [i32, C.POINTER(C.c_double), u64p]; L.tszp_prof_read.restype = i32
@@ -106,7 +119,8 @@ def lib():
     return L
 
 
-PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5}
+PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5,
+              "restore": 6}
 
 
 def prof_enable(on=True):
@@ -351,3 +365,18 @@ def gen_fbm(shape, seed, octaves=6, noise=0.05, device="cuda"):
     f = torch.empty(tuple(shape), dtype=torch.float32, device=device)
     _check(lib().tszp_gen_fbm(_dp(f), _dims(shape), seed, octaves, noise, _stream()))
     return f
+
+
+def topo_restore(field, types, shape, eb, sp=None, n_ext=0, max_rounds=16):
+    """Restores the critical points of a decoded field in place (field: float32 CUDA
+    tensor holding (float)q * eb; types: uint8 CUDA tensor)."""
+    info = RestoreInfo()
+    _check(lib().tszp_topo_restore(_dp(field), _dp(types), _dp(sp) if sp is not None else None, n_ext,
+                                   len(shape), _dims(shape), eb, max_rounds, C.byref(info), _stream()))
+    return info
+
+
+def topo_verify(field, types, shape):
+    out = (C.c_uint64 * 3)()
+    _check(lib().tszp_topo_verify(_dp(field), _dp(types), len(shape), _dims(shape), out, _stream()))
+    return tuple(out)
