@@ -237,7 +237,7 @@ def main():
         "plain_gbs": plain_gbs,
         "restore": ({"missing": last["restore"].missing, "spurious": last["restore"].spurious,
                      "changed": last["restore"].changed, "moves": last["restore"].moves,
-                     "pair_moves": last["restore"].pair_moves, "rounds": last["restore"].rounds,
+                     "windows": last["restore"].windows, "rounds": last["restore"].rounds,
                      "max_dev_over_eb": last["restore"].max_dev_over_eb}
                     if "restore" in last else None),
         "cpu_baseline": cpu,

@@ -88,18 +88,23 @@ int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t 
 
 /* Topology restoration (no counterpart in the reference library: its post-process is
  * host code in example/test_cuda_topology.cu:118-326 and test_cuda_topology_3d.cu:77-434).
- * In place on d_field, which holds the decoded (float)q * eb on entry: every point
- * stays inside its quantization cell (|y - d| <= 0.45 eb, on the side of the cell that
- * holds the original value when q != 0) while a colour-class local search and pair
- * moves make the critical-point type of every interior point equal to d_types.
- * d_sp (nullable): extrema sort positions in row-major extremum order, used to order
- * the extrema of a bin.  The counts returned are those of tszp_topo_verify. */
+ * In place on d_field, which holds the decoded (float)q * eb on entry.  Every point
+ * stays in the interval of its cell -- [d, d + 0.9 eb] for d > 0, [d - 0.9 eb, d] for
+ * d < 0 (error < eb), [-eb/1024, eb/1024] for d == 0 (error < eb (1 + 1/1024)) -- while
+ * a colour-class local search and an exact solve of small plateau windows make the
+ * critical-point type of every interior point equal to d_types.  d_sp (nullable):
+ * extrema sort positions in row-major extremum order; with them the extrema of one
+ * non-zero cell keep the order of their sort positions.  d_types must be 4-byte
+ * aligned.  The counts returned are those of tszp_topo_verify on the result. */
 typedef struct {
   uint64_t missing;        /* type != 0, restored field regular */
   uint64_t spurious;       /* type == 0, restored field critical */
   uint64_t changed;        /* both critical, different types */
   uint64_t moves;          /* accepted single-point moves */
-  uint64_t pair_moves;     /* accepted pair moves */
+  uint64_t windows;        /* plateau windows solved exactly */
+  uint64_t windows_reordered; /* of which moved an anchored extremum freely (the order of
+                                 its sort position may break: a boundary value whose
+                                 sort position belongs to the neighbouring bin) */
   uint32_t rounds;         /* local-search rounds */
   float max_dev_over_eb;   /* max |y - d| / eb */
 } tszp_restore_info;

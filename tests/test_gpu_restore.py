@@ -25,12 +25,43 @@ def _pipeline(data, eb, bs=64):
 
 
 def _check_cells(x, dec, y, eb):
+    """Every restored value stays in the interval of its cell: [d, d + 0.9 eb] above
+    zero, [d - 0.9 eb, d] below, [-eb/1024, eb/1024] in the zero cell."""
     x, dec, y = x.ravel().astype(np.float64), dec.ravel().astype(np.float64), y.ravel().astype(np.float64)
-    assert np.all(np.abs(y - dec) <= 0.45 * eb * (1 + 1e-5) + np.abs(dec) * 2.0 ** -22)
-    nz = dec != 0
-    # outside the zero cell the value stays on the original's side of the cell: error < eb
-    assert np.all(np.abs(y - x)[nz] < eb * (1 + 1e-4) + np.abs(x[nz]) * 2.0 ** -21)
-    assert np.all(np.abs(y - x)[~nz] < 1.45 * eb * (1 + 1e-4))
+    slack = np.abs(dec) * 2.0 ** -22
+    pos, neg, zero = dec > 0, dec < 0, dec == 0
+    assert np.all(y[pos] >= dec[pos]) and np.all(y[pos] <= dec[pos] + 0.9 * eb * (1 + 1e-5) + slack[pos])
+    assert np.all(y[neg] <= dec[neg]) and np.all(y[neg] >= dec[neg] - 0.9 * eb * (1 + 1e-5) - slack[neg])
+    assert np.all(np.abs(y[zero]) <= eb / 1024 * (1 + 1e-5))
+    # error bound: below eb outside the zero cell, below eb (1 + 1/1024) inside it
+    err = np.abs(y - x)
+    assert np.all(err[~zero] < eb * (1 + 1e-4) + np.abs(x[~zero]) * 2.0 ** -21)
+    assert np.all(err[zero] < eb * (1 + 1 / 1024) * (1 + 1e-4))
+
+
+def _check_extrema_order(x, dec, y, types, eb, info, slack=1):
+    """Extrema of one reference bin (outside the zero cell) keep the order of the
+    original values: sorted by x, the restored values never decrease.  The only
+    exceptions allowed are around windows that had to move an anchored extremum (an
+    extremum on a cell boundary whose sort position belongs to the neighbouring bin, or
+    a window cut out of a large plateau), and there must be no more such windows than
+    boundary extrema, or `slack`."""
+    x, dec, y, types = x.ravel(), dec.ravel(), y.ravel(), types.ravel()
+    ext = ((types == 1) | (types == 2)) & (dec != 0)
+    xe, de, ye = x[ext], dec[ext], y[ext]
+    inv = 1.0 / np.float64(np.float32(eb))
+    binv = ((xe + np.float32(eb)).astype(np.float64) * inv).astype(np.int64)
+    q = np.rint(de.astype(np.float64) * inv).astype(np.int64)
+    boundary = int(np.count_nonzero(binv != q + 1))
+    order = np.lexsort((xe, binv, de))
+    same = (de[order][1:] == de[order][:-1]) & (binv[order][1:] == binv[order][:-1])
+    dy = ye[order][1:] - ye[order][:-1]
+    dx = xe[order][1:] - xe[order][:-1]
+    bad = int(np.count_nonzero(same & ((dy < 0) | ((dx == 0) & (dy != 0)))))
+    print(f"  extrema order: pairs={int(np.count_nonzero(same))} violations={bad} "
+          f"reordered windows={info.windows_reordered} boundary extrema={boundary}")
+    assert info.windows_reordered <= max(boundary, slack)
+    assert bad <= 8 * info.windows_reordered
 
 
 NOISY = [
@@ -51,11 +82,12 @@ def test_restore_zero_mismatches(name, make, eb):
     got = oracle.classify(y).ravel()
     raw = oracle.classify(dec.reshape(data.shape)).ravel()
     print(f"{name}: cps={np.count_nonzero(want)} raw mismatches={np.count_nonzero(raw != want)} "
-          f"after={np.count_nonzero(got != want)} moves={info.moves} pairs={info.pair_moves} "
+          f"after={np.count_nonzero(got != want)} moves={info.moves} windows={info.windows} "
           f"rounds={info.rounds} maxdev={info.max_dev_over_eb:.3f}")
     assert info.mismatches() == np.count_nonzero(got != want)
     assert np.count_nonzero(got != want) == 0
     _check_cells(data, dec, y, eb)
+    _check_extrema_order(data, dec, y, types, eb, info)
 
 
 SMOOTH = [
@@ -78,6 +110,7 @@ def test_restore_smooth_fields(name, make, eb):
     assert after == info.mismatches()
     assert after <= max(2, np.count_nonzero(raw != want) // 50)
     _check_cells(data, dec, y, eb)
+    _check_extrema_order(data, dec, y, types, eb, info, slack=8)
 
 
 def test_verify_counts_match_oracle():

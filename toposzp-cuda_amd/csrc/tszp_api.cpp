@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/szp_cuda_compress.h"
@@ -46,7 +47,7 @@ int fail(int code, const char* fmt, ...) {
 enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
-  kWsPpD, kWsPpBad, kWsPpAct, kWsPpDone, kWsPpLocks, kWsPpExt, kWsPpMisc, kWsPpSt,
+  kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsCount
 };
 
@@ -640,12 +641,14 @@ int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t*
 int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_sp, uint64_t n_ext, int ndim,
                       const uint32_t* dims, float eb, int max_rounds, tszp_restore_info* info, void* stream) {
   g_err.clear();
-  if (!d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0))
+  if (!d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0) || max_rounds < 0)
     return fail(TSZP_EINVAL, "topo_restore: bad arguments");
+  if (reinterpret_cast<uintptr_t>(d_types) & 3u) return fail(TSZP_EINVAL, "topo_restore: types must be 4-byte aligned");
   size_t n = 1;
   for (int k = 0; k < ndim; k++) n *= dims[k];
   int r = check_n(n);
   if (r) return r;
+  if (n_ext > n) return fail(TSZP_EINVAL, "topo_restore: %llu extrema for %zu points", (unsigned long long)n_ext, n);
   Ctx* c;
   if ((r = get_ctx(&c))) return r;
   std::lock_guard<std::mutex> lk(c->mu);
@@ -653,97 +656,122 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
   const Grid g = make_grid(ndim, dims);
   const uint32_t N = (uint32_t)n;
   float* d;
-  uint32_t *bad, *misc, *locks;
-  uint8_t *act, *done;
+  uint32_t *bad, *bad2, *act, *bits, *misc;
+  uint64_t* status;
   WS(kWsPpD, n * 4, d);
-  const uint32_t badcap = (uint32_t)(n < (1u << 20) ? n : (n / 8 > (1u << 20) ? n / 8 : (1u << 20)));
-  WS(kWsPpBad, (size_t)badcap * 4, bad);
-  WS(kWsPpAct, n, act);
-  WS(kWsPpDone, badcap, done);
-  WS(kWsPpMisc, 64, misc);  // [0] nbad [1] moves [2] pending [4..11] verify (u64) [12] err [13] ext total
-  const uint32_t tx = (g.d0 + 7) / 8, ty = (g.d1 + 7) / 8, tzn = ndim == 3 ? (g.d2 + 7) / 8 : 1;
-  const size_t ntl = (size_t)tx * ty * tzn;
-  WS(kWsPpLocks, ntl * 4, locks);
-  ProfScope prof(kProfRestore, st);
-  HIPCHK(hipMemcpyAsync(d, d_field, n * 4, hipMemcpyDeviceToDevice, st));
-  HIPCHK(pp_init(d, d_types, d_field, N, eb, st));
-  if (d_sp && n_ext) {
-    uint32_t* ext;
-    uint64_t* sw;
-    WS(kWsPpExt, n_ext * 4, ext);
-    const uint32_t tiles = pp_ext_tiles(N);
-    WS(kWsPpSt, (size_t)tiles * 8 + 16, sw);
-    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(sw), 0u, (size_t)tiles * 2 + 4, st));
-    HIPCHK(launch_fill_u32(misc, 0u, 16, st));
-    HIPCHK(pp_ext_scan(d_types, N, ext, (uint32_t)n_ext, sw, reinterpret_cast<uint32_t*>(sw + tiles), misc + 12,
-                       misc + 13, st));
-    HIPCHK(pp_extrema(d, d_types, ext, d_sp, (uint32_t)n_ext, misc + 13, N, d_field, eb, st));
-  }
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(act), 0u, (n + 3) / 4, st));
-  uint32_t h[16];
-  info->rounds = 0;
-  info->moves = 0;
-  info->pair_moves = 0;
-  const int colours = ndim == 2 ? 5 : 7;
-  const int phases = ndim == 2 ? 4 : 8;
-  auto mark = [&](uint32_t* nb_out) -> int {
-    HIPCHK(launch_fill_u32(misc, 0u, 3, st));
-    HIPCHK(pp_mark(g, d_field, d_types, N, misc, bad, badcap, st));
-    HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
+  // mismatch lists: a quarter of the points (more are listed on the next full pass)
+  const uint32_t cap = N / 4 + 4096;
+  WS(kWsPpBad, (size_t)cap * 4, bad);
+  WS(kWsPpBad2, (size_t)cap * 4, bad2);
+  const size_t bit_words = n / 32 + 4;
+  WS(kWsPpBits, bit_words * 4, bits);
+  WS(kWsPpMisc, kPpMiscWords * 4, misc);
+  const uint32_t itiles = pp_init_tiles(N);
+  WS(kWsPpSt, (size_t)itiles * 8 + 64, status);
+  uint32_t h[kPpMiscWords];
+  auto read_misc = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    *nb_out = h[0] < badcap ? h[0] : badcap;
     return TSZP_OK;
   };
-  for (int outer = 0; outer < 4; outer++) {
-    uint32_t nb = 0;
-    bool progress = false;
-    for (int round = 0; round < max_rounds; round++) {
-      if ((r = mark(&nb))) return r;
-      if (!nb) break;
-      info->rounds++;
-      HIPCHK(pp_active(g, bad, nb, act, st));
-      for (int col = 0; col < colours; col++) HIPCHK(pp_move(g, d, d_types, d_field, act, N, eb, col, misc + 1, st));
-      HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      info->moves += h[1];
-      if (!h[1]) break;
-      progress = true;
-    }
-    if ((r = mark(&nb))) return r;
-    if (!nb) break;
-    // pair moves over the remaining mismatches
-    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(done), 0u, (nb + 3) / 4, st));
-    uint32_t pm = 0;
-    for (int ph = 0; ph < phases; ph++) {
-      for (int it = 0; it < 64; it++) {
-        HIPCHK(launch_fill_u32(locks, 0u, ntl, st));
-        HIPCHK(launch_fill_u32(misc + 1, 0u, 2, st));
-        HIPCHK(pp_pair(g, d, d_types, d_field, bad, nb, done, locks, tx, ty, ph, eb, misc + 1, misc + 2, st));
-        HIPCHK(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        pm += h[1];
-        if (!h[2]) break;
-      }
-    }
-    info->pair_moves += pm;
-    if (!pm && !progress) break;
-  }
-  // verification against the transmitted types
-  unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + 4);
-  HIPCHK(launch_fill_u32(misc + 4, 0u, 8, st));
-  HIPCHK(pp_verify(g, d_field, d_types, d, N, vo, st));
-  prof.stop();
-  unsigned long long hv[4];
-  HIPCHK(hipMemcpyAsync(hv, vo, 32, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(h + 12, misc + 12, 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  if (d_sp && n_ext && h[13] != n_ext)
-    return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[13],
+  ProfScope prof(kProfRestore, st);
+  HIPCHK(hipMemcpyAsync(d, d_field, n * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, (size_t)itiles * 2, st));
+  const bool use_sp = d_sp != nullptr && n_ext > 0;
+  if (use_sp) HIPCHK(pp_spmax(d_sp, (uint32_t)n_ext, misc, st));
+  HIPCHK(pp_init(d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext, N, eb, d_field, status, misc, st));
+  // full marking pass: fresh list of mismatches and the verification counts
+  auto full_mark = [&](uint32_t* nb_out) -> int {
+    HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
+    HIPCHK(launch_fill_u32(misc + kPpVerify, 0u, 8, st));
+    HIPCHK(pp_mark(g, d_field, d_types, d, N, bad, cap, misc, st));
+    int rr = read_misc();
+    if (rr) return rr;
+    *nb_out = h[kPpBad];
+    return TSZP_OK;
+  };
+  uint32_t nb = 0;
+  if ((r = full_mark(&nb))) return r;
+  if (use_sp && h[kPpExt] != n_ext)
+    return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[kPpExt],
                 (unsigned long long)n_ext);
+  if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
+  const int colours = ndim == 2 ? 5 : 7;
+  uint64_t moves = 0, solved = 0, freed = 0;
+  uint32_t rounds = 0;
+  bool dirty = false;  // the field changed after the last full pass
+  for (int outer = 0; outer < 4 && nb; outer++) {
+    // local search, re-checking only around the previous mismatches
+    bool complete = nb <= cap;
+    for (int round = 0; round < max_rounds && nb; round++) {
+      const uint32_t nl = nb < cap ? nb : cap;
+      // a closed neighbourhood holds at most one point of each colour class, so every
+      // colour list has room for one entry per listed mismatch
+      WS(kWsPpAct, (size_t)nl * colours * 4 + 16, act);
+      HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
+      HIPCHK(launch_fill_u32(misc + kPpMoves, 0u, 1, st));
+      HIPCHK(launch_fill_u32(misc + kPpActC, 0u, colours, st));
+      HIPCHK(pp_active(g, bad, nl, d, d_types, bits, act, nl, use_sp, misc, st));
+      for (int col = 0; col < colours; col++)
+        HIPCHK(pp_move(g, act, nl, nl, col, d, d_types, d_field, eb, misc, st));
+      rounds++;
+      dirty = true;
+      if (complete) {
+        HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
+        HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
+        HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, misc, st));
+        if ((r = read_misc())) return r;
+        std::swap(bad, bad2);
+        nb = h[kPpBad];
+      } else {
+        if ((r = full_mark(&nb))) return r;
+        dirty = false;
+      }
+      complete = nb <= cap;
+      moves += h[kPpMoves];
+      if (!h[kPpMoves]) break;
+    }
+    if (!nb) break;
+    if (dirty) {
+      if ((r = full_mark(&nb))) return r;
+      dirty = false;
+      if (!nb) break;
+    }
+    // exact window solve around what is left
+    uint32_t tx, ty, tz;
+    pp_tile_counts(g, &tx, &ty, &tz);
+    const size_t ntiles = (size_t)tx * ty * tz;
+    const uint32_t nl = nb < cap ? nb : cap;
+    const uint32_t tcap = (uint32_t)(nl < ntiles ? nl : ntiles);
+    uint32_t *tbits, *tiles;
+    WS(kWsPpTBits, (ntiles / 32 + 4) * 4, tbits);
+    WS(kWsPpTiles, (size_t)tcap * 4 + 16, tiles);
+    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
+    HIPCHK(launch_fill_u32(tbits, 0u, ntiles / 32 + 4, st));
+    HIPCHK(launch_fill_u32(misc + kPpTiles, 0u, 1, st));
+    HIPCHK(launch_fill_u32(misc + kPpSolved, 0u, 2, st));
+    HIPCHK(launch_fill_u32(misc + kPpFreed, 0u, 1, st));
+    HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
+    for (int ph = 0; ph < (1 << ndim); ph++)
+      HIPCHK(pp_solve(g, tiles, tcap, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+    if ((r = full_mark(&nb))) return r;
+    solved += h[kPpSolved];
+    freed += h[kPpFreed];
+    if (!h[kPpSolved] && outer > 0) break;
+  }
+  if (dirty && (r = full_mark(&nb))) return r;
+  prof.stop();
+  if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
+  const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);
   info->missing = hv[0];
   info->spurious = hv[1];
   info->changed = hv[2];
-  uint32_t dv = (uint32_t)hv[3];
+  info->moves = moves;
+  info->windows = solved;
+  info->windows_reordered = freed;
+  info->rounds = rounds;
+  const uint32_t dv = (uint32_t)hv[3];
   float dev;
   memcpy(&dev, &dv, 4);
   info->max_dev_over_eb = dev / eb;
@@ -764,12 +792,11 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   std::lock_guard<std::mutex> lk(c->mu);
   hipStream_t st = (hipStream_t)stream;
   uint32_t* misc;
-  WS(kWsPpMisc, 64, misc);
-  unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + 4);
-  HIPCHK(launch_fill_u32(misc + 4, 0u, 8, st));
-  HIPCHK(pp_verify(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, vo, st));
-  unsigned long long hv[4];
-  HIPCHK(hipMemcpyAsync(hv, vo, 32, hipMemcpyDeviceToHost, st));
+  WS(kWsPpMisc, kPpMiscWords * 4, misc);
+  HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
+  HIPCHK(pp_mark(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, nullptr, 0, misc, st));
+  uint64_t hv[4];
+  HIPCHK(hipMemcpyAsync(hv, misc + kPpVerify, 32, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int k = 0; k < 3; k++) counts[k] = hv[k];
   return TSZP_OK;

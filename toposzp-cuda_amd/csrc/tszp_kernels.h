@@ -105,22 +105,30 @@ hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint3
                        float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st);
 
 // ---- topology restoration (tszp_post.hip) ----
-hipError_t pp_init(const float* d, const uint8_t* t, float* y, uint32_t n, float eb, hipStream_t st);
-uint32_t pp_ext_tiles(uint32_t n);
-hipError_t pp_ext_scan(const uint8_t* t, uint32_t n, uint32_t* ext_pos, uint32_t cap, uint64_t* st_words,
-                       uint32_t* counter, uint32_t* err, uint32_t* total, hipStream_t st);
-hipError_t pp_extrema(const float* d, const uint8_t* t, const uint32_t* ext_pos, const int32_t* sp, uint32_t n_ext,
-                      const uint32_t* n_found, uint32_t n, float* y, float eb, hipStream_t st);
-hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, uint32_t n, uint32_t* nbad, uint32_t* badlist,
-                   uint32_t badcap, hipStream_t st);
-hipError_t pp_active(const Grid& g, const uint32_t* badlist, uint32_t nbad, uint8_t* act, hipStream_t st);
-hipError_t pp_move(const Grid& g, const float* d, const uint8_t* t, float* y, uint8_t* act, uint32_t n, float eb,
-                   int colour, uint32_t* moves, hipStream_t st);
-hipError_t pp_pair(const Grid& g, const float* d, const uint8_t* t, float* y, const uint32_t* badlist, uint32_t nbad,
-                   uint8_t* done, uint32_t* locks, uint32_t tiles_x, uint32_t tiles_y, int phase, float eb,
-                   uint32_t* moves, uint32_t* pending, hipStream_t st);
-hipError_t pp_verify(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n,
-                     unsigned long long* out, hipStream_t st);
+// Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.
+constexpr int kPpMiscWords = 32;
+enum : int {
+  kPpBad = 0, kPpMoves = 1, kPpAct = 2, kPpTiles = 3, kPpVerify = 4, kPpErr = 12, kPpExt = 13, kPpSpMax = 14,
+  kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18, kPpActC = 20  // 7 colour-list counts
+};
+uint32_t pp_init_tiles(uint32_t n);
+hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
+hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
+                   float* y, uint64_t* status, uint32_t* misc, hipStream_t st);
+hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
+                   uint32_t cap, uint32_t* misc, hipStream_t st);
+hipError_t pp_active(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
+                     uint32_t* bits, uint32_t* act, uint32_t cap, int anchor, uint32_t* misc, hipStream_t st);
+hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
+                   const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st);
+hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
+                     uint32_t* bits, uint32_t* bad, uint32_t cap, uint32_t* misc, hipStream_t st);
+void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
+hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t ty, uint32_t tz, uint32_t* pbits,
+                    uint32_t* tbits, uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);
+hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t max_tiles, uint32_t cap, uint32_t ty, uint32_t tz,
+                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
+                    int anchor, uint32_t* misc, hipStream_t st);
 
 // ---- synthetic fields (tszp_gen.hip) ----
 hipError_t launch_gen_blobs(float* f, int ndim, const uint32_t* dims, uint64_t seed, int nblobs,

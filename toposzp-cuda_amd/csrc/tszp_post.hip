@@ -4,501 +4,1121 @@
 // stencils (example/test_cuda_topology.cu:118-326, example/test_cuda_topology_3d.cu:
 // 77-434) that leave 2-15 % of them lost.  Here the decoded field d = (float)q * eb is
 // turned into y with classify(y) == the transmitted types at every interior point,
-// as a constraint repair that stays inside each quantization cell:
+// as a constraint repair that keeps every value inside its quantization cell:
 //
-//   * allowed interval of a point: [d, d + 0.45 eb] when d > 0, [d - 0.45 eb, d] when
-//     d < 0, [-0.45 eb, 0.45 eb] when d == 0.  Points of different cells therefore
-//     keep their order (q is monotone in the original value), and |y - x| < eb
-//     except in the zero cell (< 1.45 eb);
-//   * start: every point at the middle of its interval (equal values inside a
-//     plateau make its regular points regular), maxima / minima pushed to the upper
-//     / lower part of their interval, ordered by their sort positions;
+//   * the interval of a point: [d, d + 0.9 eb] when d > 0, [d - 0.9 eb, d] when d < 0
+//     (both on the side of d that holds the original value, so |y - x| < eb), and
+//     [-eb/1024, eb/1024] in the zero cell, whose sign truncation lost
+//     (|y - x| < eb (1 + 1/1024)).  Intervals of different cells never overlap, so a
+//     comparison between points of different cells is the original's: only the order
+//     inside a plateau (a connected set of points with one decoded value) is free,
+//     and the original field proves that a solution exists;
+//   * start: every point at the middle of its interval; the extrema of a non-zero cell
+//     at lo + (hi - lo) (0.05 + 0.9 (sp + 1) / (S + 2)), S the largest sort position,
+//     so that the extrema of one cell keep the order of their sort positions (ties
+//     stay ties).  These extrema are anchors: the local search does not move them;
 //   * local search: points within one step of a mismatch move to the value that
 //     minimises the mismatches of their closed neighbourhood.  Points of one colour
-//     class are at least three apart ((i + 2j) mod 5 in 2D, (x + 2y + 3z) mod 7 in
-//     3D), so a whole class moves in parallel with exact gains, and every accepted
-//     move strictly lowers the total;
-//   * pair moves for what remains: a mismatched point and one neighbour of its
-//     plateau move together over a value grid, one point per 8-wide tile of one
-//     parity class at a time (tile locks).
-// A verification kernel counts missing / spurious / changed critical points.
+//     class are at least three steps apart ((x + 2y) mod 5 in 2D, (x + 2y + 3z) mod 7
+//     in 3D), so a class moves in parallel with exact gains, and every accepted move
+//     strictly lowers the total.  After a round only points within two steps of a
+//     previous mismatch can have changed, and only those are re-checked;
+//   * exact window solve for what remains: the plateau around a mismatch (up to
+//     kWinMax points, breadth first) is re-ordered by a depth-first search over the
+//     weak orders of its free points relative to the fixed values around them, which
+//     checks every constraint as soon as the points it reads are placed; a found order
+//     becomes values spaced evenly between the fixed ones.  Windows of one 16-wide tile
+//     run in order in one thread, tiles of one parity class in parallel, so the result
+//     does not depend on scheduling.
+// The full marking pass also yields the verification counts (missing / spurious /
+// changed critical points) and the largest |y - d|.
 #include "tszp_device.hpp"
 #include "tszp_kernels.h"
 
 namespace tszp {
 
 // ---------------------------------------------------------------------------------
-// geometry helpers
+// geometry
 // ---------------------------------------------------------------------------------
-struct Coord {
-  int c[3];
-};
-
-__device__ __forceinline__ Coord coord_of(const Grid& g, uint32_t e) {
-  Coord k;
-  if (g.ndim == 2) {
-    const uint32_t i = fdiv(e, g.div_row);
-    k.c[0] = (int)i;
-    k.c[1] = (int)(e - i * g.d1);
-    k.c[2] = 0;
+template <int ND>
+__device__ __forceinline__ void coords_of(const Grid& g, uint32_t e, int (&c)[ND]) {
+  if constexpr (ND == 2) {
+    const uint32_t x = fdiv(e, g.div_row);
+    c[0] = (int)x;
+    c[1] = (int)(e - x * g.d1);
   } else {
     const uint32_t x = fdiv(e, g.div_plane), r = e - x * g.plane;
     const uint32_t y = fdiv(r, g.div_row);
-    k.c[0] = (int)x;
-    k.c[1] = (int)y;
-    k.c[2] = (int)(r - y * g.d2);
+    c[0] = (int)x;
+    c[1] = (int)y;
+    c[2] = (int)(r - y * g.d2);
   }
-  return k;
 }
 
-__device__ __forceinline__ bool inside(const Grid& g, int x, int y, int z) {
-  return x >= 0 && y >= 0 && z >= 0 && x < (int)g.d0 && y < (int)g.d1 && z < (int)(g.ndim == 3 ? g.d2 : 1u);
-}
-__device__ __forceinline__ bool interior3(const Grid& g, int x, int y, int z) {
-  if (g.ndim == 2) return x > 0 && y > 0 && x + 1 < (int)g.d0 && y + 1 < (int)g.d1;
-  return x > 0 && y > 0 && z > 0 && x + 1 < (int)g.d0 && y + 1 < (int)g.d1 && z + 1 < (int)g.d2;
-}
-__device__ __forceinline__ uint32_t flat_of(const Grid& g, int x, int y, int z) {
-  return g.ndim == 2 ? (uint32_t)x * g.d1 + (uint32_t)y : (uint32_t)x * g.plane + (uint32_t)y * g.d2 + (uint32_t)z;
+template <int ND>
+__device__ __forceinline__ int dim_of(const Grid& g, int a) {
+  return a == 0 ? (int)g.d0 : (a == 1 ? (int)g.d1 : (int)g.d2);
 }
 
-// neighbour direction k (2D: up, down, left, right; 3D: -x, +x, -y, +y, -z, +z)
-__device__ __forceinline__ void dir_of(int ndim, int k, int* dx, int* dy, int* dz) {
-  *dx = *dy = *dz = 0;
-  const int s = (k & 1) ? 1 : -1;
-  const int axis = k >> 1;
-  if (axis == 0) *dx = s;
-  else if (axis == 1) *dy = s;
-  else *dz = s;
-  (void)ndim;
+template <int ND>
+__device__ __forceinline__ uint32_t stride_of(const Grid& g, int a) {
+  if constexpr (ND == 2) return a == 0 ? g.d1 : 1u;
+  else return a == 0 ? g.plane : (a == 1 ? g.d2 : 1u);
 }
 
-// classification from centre + neighbour values in dir_of order
-__device__ __forceinline__ int classify_vals(int ndim, float c, const float* nb) {
-  if (ndim == 2) {
-    const float up = nb[0], dn = nb[1], lf = nb[2], rt = nb[3];
-    if (c > up && c > dn && c > lf && c > rt) return 1;
-    if (c < up && c < dn && c < lf && c < rt) return 2;
-    if ((c < up && c < dn && c > lf && c > rt) || (c > up && c > dn && c < lf && c < rt)) return 3;
-    return 0;
+template <int ND>
+__device__ __forceinline__ bool interior(const Grid& g, const int (&c)[ND]) {
+  bool r = true;
+#pragma unroll
+  for (int a = 0; a < ND; a++) r = r && c[a] >= 1 && c[a] + 2 <= dim_of<ND>(g, a);
+  return r;
+}
+
+template <int ND>
+__device__ __forceinline__ bool inside(const Grid& g, const int (&c)[ND]) {
+  bool r = true;
+#pragma unroll
+  for (int a = 0; a < ND; a++) r = r && c[a] >= 0 && c[a] < dim_of<ND>(g, a);
+  return r;
+}
+
+template <int ND>
+__device__ __forceinline__ int colour_of(const int (&c)[ND]) {
+  if constexpr (ND == 2) return (c[0] + 2 * c[1]) % 5;
+  else return (c[0] + 2 * c[1] + 3 * c[2]) % 7;
+}
+
+// classification from the centre value and its 2*ND neighbours in axis order
+// (axis a: v[2a] at -1, v[2a+1] at +1).  2D: max / min over the 4 neighbours, saddle
+// when one axis is higher on both sides and the other lower (example/
+// test_cuda_topology.cu:56-68); 3D: saddle when at least one axis is higher and one
+// lower (SZp/src/szp_topology_3d.cc:79-98) -- the same rule for two axes.
+template <int ND>
+__device__ __forceinline__ int classify_vals(float c, const float (&v)[2 * ND]) {
+  bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    const bool hi = c > v[2 * a] && c > v[2 * a + 1];
+    const bool lo = c < v[2 * a] && c < v[2 * a + 1];
+    all_gt = all_gt && hi;
+    all_lt = all_lt && lo;
+    any_hi = any_hi || hi;
+    any_lo = any_lo || lo;
   }
-  const float xm = nb[0], xp = nb[1], ym = nb[2], yp = nb[3], zm = nb[4], zp = nb[5];
-  if (c > xm && c > xp && c > ym && c > yp && c > zm && c > zp) return 1;
-  if (c < xm && c < xp && c < ym && c < yp && c < zm && c < zp) return 2;
-  const int hi = (c > xm && c > xp) + (c > ym && c > yp) + (c > zm && c > zp);
-  const int lo = (c < xm && c < xp) + (c < ym && c < yp) + (c < zm && c < zp);
-  return (hi >= 1 && lo >= 1) ? 3 : 0;
+  return all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
 }
 
-// allowed interval of a decoded value
-__device__ __forceinline__ void interval_of(float d, float eb, float* lo, float* hi) {
-  const float w = 0.45f * eb;
+template <int ND>
+__device__ __forceinline__ int classify_at(const Grid& g, const float* __restrict__ y, uint32_t e,
+                                           const int (&c)[ND]) {
+  if (!interior<ND>(g, c)) return 0;
+  float v[2 * ND];
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    const uint32_t s = stride_of<ND>(g, a);
+    v[2 * a] = y[e - s];
+    v[2 * a + 1] = y[e + s];
+  }
+  return classify_vals<ND>(y[e], v);
+}
+
+// interval of the restored value of a point with decoded value d
+__device__ __forceinline__ void cell_of(float d, float eb, float* lo, float* hi) {
   if (d > 0.f) {
     *lo = d;
-    *hi = __fadd_rn(d, w);
+    *hi = d + 0.9f * eb;
   } else if (d < 0.f) {
-    *lo = __fsub_rn(d, w);
+    *lo = d - 0.9f * eb;
     *hi = d;
   } else {
-    *lo = -w;
-    *hi = w;
+    *lo = -eb * (1.f / 1024.f);
+    *hi = eb * (1.f / 1024.f);
   }
 }
 
-// classification of point p = (x,y,z) with y[ovr] replaced by v (reads global memory)
-__device__ int class_at(const Grid& g, const float* __restrict__ yv, int x, int y, int z, uint32_t ovr_a, float va,
-                        uint32_t ovr_b, float vb) {
-  if (!interior3(g, x, y, z)) return 0;
-  const int K = 2 * g.ndim;
-  const uint32_t e = flat_of(g, x, y, z);
-  float nb[6];
-  for (int k = 0; k < K; k++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, k, &dx, &dy, &dz);
-    const uint32_t f = flat_of(g, x + dx, y + dy, z + dz);
-    nb[k] = f == ovr_a ? va : (f == ovr_b ? vb : yv[f]);
+__device__ __forceinline__ bool anchored(int t, float d) { return (t == 1 || t == 2) && d != 0.f; }
+
+__device__ __forceinline__ bool claim_bit(uint32_t* bits, uint32_t e) {
+  const uint32_t b = 1u << (e & 31u);
+  return (atomicOr(&bits[e >> 5], b) & b) == 0u;
+}
+
+// append val to list for every lane whose flag is set (one atomic per wave)
+__device__ __forceinline__ void wave_append(bool flag, uint32_t val, uint32_t* __restrict__ list, uint32_t cap,
+                                            uint32_t* count) {
+  const uint64_t m = __ballot(flag);
+  if (!m) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (flag) {
+    const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt(lane));
+    if (pos < cap) list[pos] = val;
   }
-  const float c = e == ovr_a ? va : (e == ovr_b ? vb : yv[e]);
-  return classify_vals(g.ndim, c, nb);
+}
+
+__device__ __forceinline__ void wave_count(bool flag, uint32_t* count) {
+  const uint64_t m = __ballot(flag);
+  if (m && (threadIdx.x & (kWave - 1)) == __ffsll((unsigned long long)m) - 1)
+    atomicAdd(count, (uint32_t)__popcll(m));
+}
+
+// Block-level list staging: lanes append to an LDS buffer (one LDS atomic per wave);
+// lds_flush moves it to the global list with one global atomic per block, so that
+// millions of appends never queue on one global counter.
+__device__ __forceinline__ void lds_append(bool flag, uint32_t val, uint32_t* lbuf, uint32_t* lcnt) {
+  const uint64_t m = __ballot(flag);
+  if (!m) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(lcnt, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (flag) lbuf[base + (uint32_t)__popcll(m & lanemask_lt(lane))] = val;
+}
+
+// whole block; leaves the LDS list empty
+__device__ __forceinline__ void lds_flush(const uint32_t* lbuf, uint32_t* lcnt, uint32_t* sh_base,
+                                          uint32_t* __restrict__ list, uint32_t cap, uint32_t* gcount) {
+  __syncthreads();
+  const uint32_t cnt = *lcnt;
+  if (threadIdx.x == 0 && cnt) *sh_base = atomicAdd(gcount, cnt);
+  __syncthreads();
+  if (cnt) {
+    const uint32_t base = *sh_base;
+    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x)
+      if (base + j < cap) list[base + j] = lbuf[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *lcnt = 0;
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------
-// initial values
+// largest sort position
 // ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlockThreads) void k_pp_spmax(const int32_t* __restrict__ sp, uint32_t m,
+                                                           uint32_t* __restrict__ misc) {
+  __shared__ uint32_t red[kWavesPerBlock];
+  uint32_t v = 0;
+  for (uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x; i < m; i += gridDim.x * kBlockThreads) {
+    const int32_t s = sp[i];
+    v = s > (int32_t)v ? (uint32_t)s : v;
+  }
+  v = seg_max<kWave>(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = red[0];
+#pragma unroll
+    for (int k = 1; k < kWavesPerBlock; k++) r = red[k] > r ? red[k] : r;
+    atomicMax(&misc[kPpSpMax], r);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// initial values.  A tile of kInitTile points ranks its extrema in row-major order
+// (look-back over tiles) so that the k-th extremum takes sort position sp[k].
+// ---------------------------------------------------------------------------------
+constexpr int kInitPer = 64;
+constexpr uint32_t kInitTile = (uint32_t)kBlockThreads * kInitPer;
+
 __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restrict__ d, const uint8_t* __restrict__ t,
-                                                          float* __restrict__ y, uint32_t n, float eb) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  float lo, hi;
-  interval_of(d[e], eb, &lo, &hi);
-  const float base = 0.5f * (lo + hi);
-  const int tt = t[e];
-  float v = base;
-  if (tt == 1) v = base + 0.7f * (hi - base);
-  if (tt == 2) v = base - 0.7f * (base - lo);
-  y[e] = v;
-}
-
-// extrema ordered by their sort positions inside the upper / lower part of their
-// interval (a larger sort position -> a larger value, as in the original field)
-__global__ __launch_bounds__(kBlockThreads) void k_pp_extrema(const float* __restrict__ d,
-                                                             const uint8_t* __restrict__ t,
-                                                             const uint32_t* __restrict__ ext_pos,
-                                                             const int32_t* __restrict__ sp, uint32_t n_ext,
-                                                             const uint32_t* __restrict__ n_found, uint32_t n,
-                                                             float* __restrict__ y, float eb) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_ext || k >= *n_found) return;
-  const uint32_t e = ext_pos[k];
-  if (e >= n) return;
-  float lo, hi;
-  interval_of(d[e], eb, &lo, &hi);
-  const float base = 0.5f * (lo + hi);
-  const float r = 1.f - 1.f / (1.f + (float)max(sp[k], 0) * (1.f / 64.f));  // 0 <= r < 1, increasing
-  if (t[e] == 1) y[e] = base + (hi - base) * (0.55f + 0.4f * r);
-  else if (t[e] == 2) y[e] = base - (base - lo) * (0.95f - 0.4f * r);
-}
-
-// positions of the extrema in row-major order (compaction by a look-back scan)
-constexpr int kPpSteps = 16;
-constexpr uint32_t kPpTile = kBlockThreads * kPpSteps;
-
-__global__ __launch_bounds__(kBlockThreads) void k_pp_ext_scan(const uint8_t* __restrict__ t, uint32_t n,
-                                                              uint32_t* __restrict__ ext_pos, uint32_t cap,
-                                                              uint64_t* st, uint32_t* counter, uint32_t* err,
-                                                              uint32_t* total) {
-  __shared__ uint32_t step_tot[kWavesPerBlock * kPpSteps];
-  __shared__ uint32_t sh_tile;
-  __shared__ uint64_t sh_pre;
-  const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
-  if (tid == 0) sh_tile = atomicAdd(counter, 1u);
+                                                          const int32_t* __restrict__ sp, uint32_t n_ext, uint32_t n,
+                                                          float eb, float* __restrict__ y, uint64_t* status,
+                                                          uint32_t* misc) {
+  __shared__ uint32_t tw[kInitTile / 4];
+  __shared__ uint64_t msk[kBlockThreads];
+  __shared__ uint32_t exl[kBlockThreads];
+  __shared__ uint32_t red[kWavesPerBlock];
+  __shared__ uint32_t sh_tile, sh_base;
+  const int tid = threadIdx.x;
+  if (tid == 0) sh_tile = atomicAdd(&misc[kPpCounter], 1u);
   __syncthreads();
   const uint32_t tile = sh_tile;
-  const uint32_t base = tile * kPpTile + w * (kWave * kPpSteps);
-  uint32_t bits = 0;
+  const uint32_t t0 = tile * kInitTile;
+  // stage the tile's types (4-byte aligned words, coalesced)
+  const uint32_t* tg = reinterpret_cast<const uint32_t*>(t);
+#pragma unroll 4
+  for (int j = 0; j < (int)(kInitTile / 4 / kBlockThreads); j++) {
+    const uint32_t w = j * kBlockThreads + tid;
+    const uint32_t e = t0 + 4 * w;
+    uint32_t v = 0;
+    if (e + 3 < n) {
+      v = tg[e >> 2];
+    } else {
 #pragma unroll
-  for (int s = 0; s < kPpSteps; s++) {
-    const uint32_t e = base + s * kWave + lane;
-    const int tt = e < n ? t[e] : 0;
-    const bool x = tt == 1 || tt == 2;
-    bits |= (uint32_t)x << s;
-    const uint64_t m = __ballot(x);
-    if (lane == 0) step_tot[w * kPpSteps + s] = (uint32_t)__popcll(m);
+      for (int b = 0; b < 4; b++)
+        if (e + b < n) v |= (uint32_t)t[e + b] << (8 * b);
+    }
+    tw[w] = v;
   }
   __syncthreads();
-  if (w == 0) {
-    const uint32_t v = step_tot[lane];
-    const uint32_t incl = seg_incl_scan<kWave>(v, lane);
-    step_tot[lane] = incl - v;
-    const uint32_t tot = __shfl(incl, kWave - 1, kWave);
-    const uint64_t ex = lookback(st, tile, tot, err);
-    if (lane == 0) {
-      sh_pre = ex;
-      if (tile == gridDim.x - 1) *total = (uint32_t)(ex + tot);
+  uint64_t m = 0;
+#pragma unroll
+  for (int k = 0; k < kInitPer / 4; k++) {
+    const uint32_t v = tw[tid * (kInitPer / 4) + k];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const uint32_t tb = (v >> (8 * b)) & 0xffu;
+      m |= (uint64_t)(tb == 1u || tb == 2u) << (4 * k + b);
+    }
+  }
+  msk[tid] = m;
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), red, &tot);
+  exl[tid] = ex;
+  if (tid < kWave) {
+    const uint64_t pre = lookback(status, tile, tot, &misc[kPpErr]);
+    if (tid == 0) {
+      sh_base = (uint32_t)pre;
+      if (tile == gridDim.x - 1) misc[kPpExt] = (uint32_t)(pre + tot);
     }
   }
   __syncthreads();
-  const uint64_t lt = lanemask_lt(lane);
+  const uint32_t base = sh_base;
+  const float sden = (float)misc[kPpSpMax] + 2.f;
+  for (int j = 0; j < kInitPer; j++) {
+    const uint32_t i = (uint32_t)j * kBlockThreads + tid;
+    const uint32_t e = t0 + i;
+    if (e >= n) break;
+    const uint32_t tb = (tw[i >> 2] >> (8 * (i & 3u))) & 0xffu;
+    const float dd = d[e];
+    float lo, hi;
+    cell_of(dd, eb, &lo, &hi);
+    const float mid = dd == 0.f ? 0.f : 0.5f * (lo + hi);
+    float v = mid;
+    if (tb == 1u || tb == 2u) {
+      const uint32_t o = i >> 6, b = i & 63u;
+      const uint32_t k = base + exl[o] + (uint32_t)__popcll(msk[o] & ((1ull << b) - 1ull));
+      int32_t s = (sp != nullptr && k < n_ext) ? sp[k] : 0;
+      s = s < 0 ? 0 : s;
+      if (sp != nullptr && dd != 0.f) {
+        v = lo + (hi - lo) * (0.05f + 0.9f * (((float)s + 1.f) / sden));
+      } else {
+        const float r = (float)s / ((float)s + 64.f);
+        v = tb == 1u ? mid + (hi - mid) * (0.55f + 0.4f * r) : mid - (mid - lo) * (0.95f - 0.4f * r);
+      }
+    }
+    y[e] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// full marking pass: list of mismatched points + verification counts
+// ---------------------------------------------------------------------------------
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_mark(Grid g, const float* __restrict__ y,
+                                                          const uint8_t* __restrict__ t, const float* __restrict__ d,
+                                                          uint32_t n, uint32_t* __restrict__ bad, uint32_t cap,
+                                                          uint32_t* __restrict__ misc) {
+  constexpr uint32_t kStage = 4096;
+  __shared__ uint32_t red[4][kWavesPerBlock];
+  __shared__ uint32_t lbuf[kStage];
+  __shared__ uint32_t lcnt, lbase;
+  if (threadIdx.x == 0) lcnt = 0;
+  __syncthreads();
+  uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
+  for (uint32_t b0 = blockIdx.x * kBlockThreads; b0 < n; b0 += gridDim.x * kBlockThreads) {
+    const uint32_t e = b0 + threadIdx.x;
+    bool isbad = false;
+    if (e < n) {
+      int c[ND];
+      coords_of<ND>(g, e, c);
+      const int k = classify_at<ND>(g, y, e, c);
+      const int tt = t[e];
+      miss += (tt != 0 && k == 0);
+      spur += (tt == 0 && k != 0);
+      chg += (tt != 0 && k != 0 && k != tt);
+      isbad = k != tt;
+      if (d != nullptr) {
+        const uint32_t dv = __float_as_uint(fabsf(y[e] - d[e]));
+        dev = dv > dev ? dv : dev;
+      }
+    }
+    if (bad != nullptr) {
+      lds_append(isbad, e, lbuf, &lcnt);
+      __syncthreads();
+      if (lcnt > kStage - kBlockThreads) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+    }
+  }
+  if (bad != nullptr) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+  miss = seg_sum<kWave>(miss);
+  spur = seg_sum<kWave>(spur);
+  chg = seg_sum<kWave>(chg);
+  dev = seg_max<kWave>(dev);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[0][w] = miss;
+    red[1][w] = spur;
+    red[2][w] = chg;
+    red[3][w] = dev;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
-  for (int s = 0; s < kPpSteps; s++) {
-    const bool x = (bits >> s) & 1u;
-    const uint64_t m = __ballot(x);
-    if (!x) continue;
-    const uint32_t pos = (uint32_t)sh_pre + step_tot[w * kPpSteps + s] + (uint32_t)__popcll(m & lt);
-    if (pos < cap) ext_pos[pos] = base + s * kWave + lane;
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+      s2 += red[2][k];
+      s3 = red[3][k] > s3 ? red[3][k] : s3;
+    }
+    unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + kPpVerify);
+    if (s0) atomicAdd(&vo[0], (unsigned long long)s0);
+    if (s1) atomicAdd(&vo[1], (unsigned long long)s1);
+    if (s2) atomicAdd(&vo[2], (unsigned long long)s2);
+    if (s3) atomicMax(&vo[3], (unsigned long long)s3);
   }
 }
 
 // ---------------------------------------------------------------------------------
-// marking: mismatches and the active set (points within one step of a mismatch)
+// active set: the non-anchored points of the closed neighbourhoods of the mismatches
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlockThreads) void k_pp_mark(Grid g, const float* __restrict__ yv,
-                                                          const uint8_t* __restrict__ t, uint32_t n,
-                                                          uint32_t* __restrict__ nbad,
-                                                          uint32_t* __restrict__ badlist, uint32_t badcap) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const Coord k = coord_of(g, e);
-  const int c = class_at(g, yv, k.c[0], k.c[1], k.c[2], 0xffffffffu, 0.f, 0xffffffffu, 0.f);
-  if (c != (int)t[e]) {
-    const uint32_t slot = atomicAdd(nbad, 1u);
-    if (slot < badcap) badlist[slot] = e;
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_active(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                            const float* __restrict__ d,
+                                                            const uint8_t* __restrict__ t, uint32_t* bits,
+                                                            uint32_t* __restrict__ act, uint32_t cap, int anchor,
+                                                            uint32_t* misc) {
+  // one list per colour class (act + colour * cap), filled in two passes: block-local
+  // offsets from LDS counters, then one global atomic per colour per block
+  constexpr int NC = ND == 2 ? 5 : 7;
+  __shared__ uint32_t lcnt[NC], lbase[NC];
+  if (threadIdx.x < NC) lcnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  const bool valid = i < nbad;
+  const uint32_t e = valid ? bad[i] : 0u;
+  int c[ND];
+  coords_of<ND>(g, e, c);
+  uint32_t pk[2 * ND + 1], off[2 * ND + 1];
+  int col[2 * ND + 1];
+#pragma unroll
+  for (int k = 0; k <= 2 * ND; k++) {
+    col[k] = -1;
+    pk[k] = e;
+    off[k] = 0;
+    if (!valid) continue;
+    int q[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) q[a] = c[a];
+    if (k > 0) {
+      const int a = (k - 1) >> 1;
+      const int s = ((k - 1) & 1) ? 1 : -1;
+      q[a] += s;
+      pk[k] = s > 0 ? e + stride_of<ND>(g, a) : e - stride_of<ND>(g, a);
+    }
+    if (k == 0 || inside<ND>(g, q)) {
+      const uint32_t p = pk[k];
+      if (!(anchor && anchored(t[p], d[p])) && claim_bit(bits, p)) {
+        col[k] = colour_of<ND>(q);
+        off[k] = atomicAdd(&lcnt[col[k]], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const uint32_t n = lcnt[threadIdx.x];
+    lbase[threadIdx.x] = n ? atomicAdd(&misc[kPpActC + threadIdx.x], n) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k <= 2 * ND; k++) {
+    if (col[k] < 0) continue;
+    const uint32_t pos = lbase[col[k]] + off[k];
+    if (pos < cap) act[(size_t)col[k] * cap + pos] = pk[k];
   }
 }
 
-// active points of one colour: every point within one step of a mismatched point
-__global__ __launch_bounds__(kBlockThreads) void k_pp_active(Grid g, const uint32_t* __restrict__ badlist,
-                                                            uint32_t nbad, uint8_t* __restrict__ act) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nbad) return;
-  const uint32_t e = badlist[i];
-  const Coord k = coord_of(g, e);
-  act[e] = 1;
-  for (int d = 0; d < 2 * g.ndim; d++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, d, &dx, &dy, &dz);
-    const int x = k.c[0] + dx, y = k.c[1] + dy, z = k.c[2] + dz;
-    if (inside(g, x, y, z)) act[flat_of(g, x, y, z)] = 1;
+// ---------------------------------------------------------------------------------
+// single-point moves of one colour class
+// ---------------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ void sort_small(float (&v)[M]) {
+#pragma unroll
+  for (int a = 0; a < M; a++)
+#pragma unroll
+    for (int b = 0; b + 1 < M - a; b++) {
+      const float x = v[b], z = v[b + 1];
+      v[b] = fminf(x, z);
+      v[b + 1] = fmaxf(x, z);
+    }
+}
+
+template <int ND>
+__device__ bool move_point(const Grid& g, uint32_t p, const int (&c)[ND], const float* __restrict__ d,
+                           const uint8_t* __restrict__ t, float* __restrict__ y, float eb) {
+  constexpr int K = 2 * ND;
+  const bool intp = interior<ND>(g, c);
+  const int tp = t[p];
+  float yq[K], nn[K][K];
+  int tq[K];
+  bool inq[K], intq[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int a = k >> 1;
+    const uint32_t s = stride_of<ND>(g, a);
+    int q[ND];
+#pragma unroll
+    for (int b = 0; b < ND; b++) q[b] = c[b];
+    q[a] += (k & 1) ? 1 : -1;
+    inq[k] = inside<ND>(g, q);
+    intq[k] = inq[k] && interior<ND>(g, q);
+    const uint32_t qe = (k & 1) ? p + s : p - s;
+    yq[k] = inq[k] ? y[qe] : 0.f;
+    tq[k] = inq[k] ? (int)t[qe] : 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      nn[k][j] = 0.f;
+      if (intq[k] && j != (k ^ 1)) {
+        const uint32_t sj = stride_of<ND>(g, j >> 1);
+        nn[k][j] = y[(j & 1) ? qe + sj : qe - sj];
+      }
+    }
   }
-}
-
-__device__ __forceinline__ int colour_of(const Grid& g, const Coord& k) {
-  return g.ndim == 2 ? (k.c[0] + 2 * k.c[1]) % 5 : (k.c[0] + 2 * k.c[1] + 3 * k.c[2]) % 7;
-}
-
-// mismatches of the closed neighbourhood of (x,y,z) when y[e] = v
-__device__ int nbhd_bad(const Grid& g, const float* __restrict__ yv, const uint8_t* __restrict__ t, const Coord& k,
-                        uint32_t e, float v, uint32_t e2, float v2) {
-  int bad = class_at(g, yv, k.c[0], k.c[1], k.c[2], e, v, e2, v2) != (int)t[e];
-  for (int d = 0; d < 2 * g.ndim; d++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, d, &dx, &dy, &dz);
-    const int x = k.c[0] + dx, y = k.c[1] + dy, z = k.c[2] + dz;
-    if (!inside(g, x, y, z)) continue;
-    const uint32_t f = flat_of(g, x, y, z);
-    bad += class_at(g, yv, x, y, z, e, v, e2, v2) != (int)t[f];
-  }
-  return bad;
-}
-
-// single-point moves of one colour class over the active set
-__global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const float* __restrict__ d,
-                                                          const uint8_t* __restrict__ t, float* __restrict__ yv,
-                                                          uint8_t* __restrict__ act, uint32_t n, float eb, int colour,
-                                                          uint32_t* __restrict__ moves) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n || !act[e]) return;
-  const Coord k = coord_of(g, e);
-  if (colour_of(g, k) != colour) return;
-  act[e] = 0;
-  const float cur = yv[e];
-  const int v0 = nbhd_bad(g, yv, t, k, e, cur, 0xffffffffu, 0.f);
-  if (v0 == 0) return;
+  auto nbad = [&](float v) -> int {
+    int b = intp ? (classify_vals<ND>(v, yq) != tp) : 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (intq[k]) {
+        float w[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) w[j] = (j == (k ^ 1)) ? v : nn[k][j];
+        b += classify_vals<ND>(yq[k], w) != tq[k];
+      }
+    }
+    return b;
+  };
+  const float cur = y[p];
+  const int v0 = nbad(cur);
+  if (v0 == 0) return false;
   float lo, hi;
-  interval_of(d[e], eb, &lo, &hi);
-  // candidate values: the interval ends, every plateau neighbour's value, midpoints
-  float vals[8];
-  int nv = 0;
-  vals[nv++] = lo;
-  vals[nv++] = hi;
-  for (int q = 0; q < 2 * g.ndim; q++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, q, &dx, &dy, &dz);
-    const int x = k.c[0] + dx, y = k.c[1] + dy, z = k.c[2] + dz;
-    if (!inside(g, x, y, z)) continue;
-    const float v = yv[flat_of(g, x, y, z)];
-    if (v >= lo && v <= hi) vals[nv++] = v;
-  }
-  for (int a = 1; a < nv; a++) {  // insertion sort
-    const float v = vals[a];
-    int b = a - 1;
-    while (b >= 0 && vals[b] > v) {
-      vals[b + 1] = vals[b];
-      b--;
-    }
-    vals[b + 1] = v;
-  }
+  cell_of(d[p], eb, &lo, &hi);
+  float vals[K + 2];
+  vals[0] = lo;
+  vals[1] = hi;
+#pragma unroll
+  for (int k = 0; k < K; k++) vals[k + 2] = (inq[k] && yq[k] >= lo && yq[k] <= hi) ? yq[k] : INFINITY;
+  sort_small<K + 2>(vals);
   int best = v0;
   float bestv = cur;
-  for (int a = 0; a < nv; a++) {
-    if (a > 0 && vals[a] == vals[a - 1]) continue;
-    const float c1 = vals[a];
-    int s = nbhd_bad(g, yv, t, k, e, c1, 0xffffffffu, 0.f);
-    if (s < best) {
-      best = s;
-      bestv = c1;
+#pragma unroll
+  for (int a = 0; a < K + 2; a++) {
+    const float va = vals[a];
+    if (!(va <= hi)) continue;
+    if (a == 0 || va != vals[a - 1]) {
+      const int s = nbad(va);
+      if (s < best) {
+        best = s;
+        bestv = va;
+      }
     }
-    if (a + 1 < nv && vals[a + 1] > vals[a]) {
-      const float m = 0.5f * (vals[a] + vals[a + 1]);
-      if (m > vals[a] && m < vals[a + 1]) {
-        s = nbhd_bad(g, yv, t, k, e, m, 0xffffffffu, 0.f);
-        if (s < best) {
-          best = s;
-          bestv = m;
+    if (a + 1 < K + 2) {
+      const float vb = vals[a + 1];
+      if (vb <= hi && vb > va) {
+        const float mv = 0.5f * (va + vb);
+        if (mv > va && mv < vb) {
+          const int s = nbad(mv);
+          if (s < best) {
+            best = s;
+            bestv = mv;
+          }
         }
       }
     }
   }
   if (best < v0) {
-    yv[e] = bestv;
-    atomicAdd(moves, 1u);
+    y[p] = bestv;
+    return true;
   }
+  return false;
 }
 
-// pair moves: a mismatched point with one neighbour of its plateau, over a 9 x 9 grid
-// of values.  One point per 8-wide tile of the given parity class at a time.
-constexpr int kPairGrid = 9;
-
-__device__ int union_bad(const Grid& g, const float* __restrict__ yv, const uint8_t* __restrict__ t, const Coord& a,
-                         uint32_t ea, float va, const Coord& b, uint32_t eb_, float vb) {
-  // mismatches over the closed neighbourhoods of a and b (points counted once)
-  int bad = nbhd_bad(g, yv, t, a, ea, va, eb_, vb);  // includes b, a neighbour of a
-  for (int d = 0; d < 2 * g.ndim; d++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, d, &dx, &dy, &dz);
-    const int x = b.c[0] + dx, y = b.c[1] + dy, z = b.c[2] + dz;
-    if (!inside(g, x, y, z)) continue;
-    const uint32_t f = flat_of(g, x, y, z);
-    if (f == ea) continue;
-    // skip points already counted as neighbours of a
-    const int md = abs(x - a.c[0]) + abs(y - a.c[1]) + abs(z - a.c[2]);
-    if (md <= 1) continue;
-    bad += class_at(g, yv, x, y, z, ea, va, eb_, vb) != (int)t[f];
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_t* __restrict__ act, uint32_t cap,
+                                                          int colour, const float* __restrict__ d,
+                                                          const uint8_t* __restrict__ t, float* __restrict__ y,
+                                                          float eb, uint32_t* misc) {
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  uint32_t cnt = misc[kPpActC + colour];
+  cnt = cnt < cap ? cnt : cap;
+  bool moved = false;
+  if (i < cnt) {
+    const uint32_t p = act[(size_t)colour * cap + i];
+    int c[ND];
+    coords_of<ND>(g, p, c);
+    moved = move_point<ND>(g, p, c, d, t, y, eb);
   }
-  return bad;
+  wave_count(moved, &misc[kPpMoves]);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_pp_pair(Grid g, const float* __restrict__ d,
-                                                          const uint8_t* __restrict__ t, float* __restrict__ yv,
-                                                          const uint32_t* __restrict__ badlist, uint32_t nbad,
-                                                          uint8_t* __restrict__ done, uint32_t* __restrict__ locks,
-                                                          uint32_t tiles_x, uint32_t tiles_y, int phase, float eb,
-                                                          uint32_t* __restrict__ moves, uint32_t* __restrict__ pending) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nbad || done[i]) return;
-  const uint32_t e = badlist[i];
-  const Coord k = coord_of(g, e);
-  const int tx = k.c[0] >> 3, ty = k.c[1] >> 3, tz = k.c[2] >> 3;
-  const int ph = (tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2);
-  if (ph != phase) return;
-  const uint32_t tile = ((uint32_t)tz * tiles_y + (uint32_t)ty) * tiles_x + (uint32_t)tx;
-  if (atomicCAS(&locks[tile], 0u, 1u) != 0u) {
-    atomicAdd(pending, 1u);  // another point of this tile goes first
-    return;
-  }
-  done[i] = 1;
-  if (class_at(g, yv, k.c[0], k.c[1], k.c[2], 0xffffffffu, 0.f, 0xffffffffu, 0.f) == (int)t[e]) return;
-  const float de = d[e];
-  float lo, hi;
-  interval_of(de, eb, &lo, &hi);
-  for (int q = 0; q < 2 * g.ndim; q++) {
-    int dx, dy, dz;
-    dir_of(g.ndim, q, &dx, &dy, &dz);
-    Coord kb;
-    kb.c[0] = k.c[0] + dx;
-    kb.c[1] = k.c[1] + dy;
-    kb.c[2] = k.c[2] + dz;
-    if (!inside(g, kb.c[0], kb.c[1], kb.c[2])) continue;
-    const uint32_t f = flat_of(g, kb.c[0], kb.c[1], kb.c[2]);
-    if (d[f] != de) continue;  // same plateau only
-    const float ya = yv[e], yb = yv[f];
-    const int base = union_bad(g, yv, t, k, e, ya, kb, f, yb);
-    int best = base;
-    float ba = ya, bb = yb;
-    for (int u = 0; u < kPairGrid; u++) {
-      const float va = lo + (hi - lo) * ((float)u + 0.5f) * (1.f / kPairGrid);
-      for (int v = 0; v < kPairGrid; v++) {
-        const float vb = lo + (hi - lo) * ((float)v + 0.5f) * (1.f / kPairGrid);
-        const int s = union_bad(g, yv, t, k, e, va, kb, f, vb);
-        if (s < best) {
-          best = s;
-          ba = va;
-          bb = vb;
-        }
+// ---------------------------------------------------------------------------------
+// re-marking: every point within two steps of a previous mismatch
+// ---------------------------------------------------------------------------------
+__constant__ const int8_t kRing2D[13][2] = {{0, 0},  {-1, 0}, {1, 0},   {0, -1}, {0, 1},  {-2, 0}, {2, 0},
+                                            {0, -2}, {0, 2},  {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+__constant__ const int8_t kRing3D[25][3] = {
+    {0, 0, 0},  {-1, 0, 0},  {1, 0, 0},  {0, -1, 0},  {0, 1, 0},   {0, 0, -1},  {0, 0, 1},
+    {-2, 0, 0}, {2, 0, 0},   {0, -2, 0}, {0, 2, 0},   {0, 0, -2},  {0, 0, 2},   {-1, -1, 0},
+    {-1, 1, 0}, {1, -1, 0},  {1, 1, 0},  {-1, 0, -1}, {-1, 0, 1},  {1, 0, -1},  {1, 0, 1},
+    {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
+
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint32_t* __restrict__ old,
+                                                            uint32_t nold, const float* __restrict__ y,
+                                                            const uint8_t* __restrict__ t, uint32_t* bits,
+                                                            uint32_t* __restrict__ bad, uint32_t cap,
+                                                            uint32_t* misc) {
+  constexpr int R = ND == 2 ? 13 : 25;
+  __shared__ uint32_t lbuf[kBlockThreads * R];
+  __shared__ uint32_t lcnt, lbase;
+  if (threadIdx.x == 0) lcnt = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  const bool valid = i < nold;
+  const uint32_t e = valid ? old[i] : 0u;
+  int c[ND];
+  coords_of<ND>(g, e, c);
+#pragma unroll 1
+  for (int r = 0; r < R; r++) {
+    bool isbad = false;
+    uint32_t p = 0;
+    if (valid) {
+      int q[ND];
+      int64_t off = 0;
+#pragma unroll
+      for (int a = 0; a < ND; a++) {
+        const int o = ND == 2 ? kRing2D[r][a] : kRing3D[r][a];
+        q[a] = c[a] + o;
+        off += (int64_t)o * (int64_t)stride_of<ND>(g, a);
+      }
+      if (inside<ND>(g, q)) {
+        p = (uint32_t)((int64_t)e + off);
+        if (claim_bit(bits, p)) isbad = classify_at<ND>(g, y, p, q) != (int)t[p];
       }
     }
-    if (best < base) {
-      yv[e] = ba;
-      yv[f] = bb;
-      atomicAdd(moves, 1u);
-      break;
+    lds_append(isbad, p, lbuf, &lcnt);
+  }
+  lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+}
+
+// ---------------------------------------------------------------------------------
+// exact window solve
+// ---------------------------------------------------------------------------------
+constexpr int kTileShift = 4;  // 16-wide tiles: windows of one parity class never interact
+constexpr int kWinMax = 8;
+constexpr int kSolveBudget = 1 << 14;
+
+template <int ND>
+__device__ __forceinline__ uint32_t tile_of(const int (&c)[ND], uint32_t ty, uint32_t tz) {
+  uint32_t id = 0;
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    const uint32_t tc = (uint32_t)c[a] >> kTileShift;
+    id = a == 0 ? tc : id * (a == 1 ? ty : tz) + tc;
+  }
+  return id;
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                           uint32_t ty, uint32_t tz, uint32_t* pbits,
+                                                           uint32_t* tbits, uint32_t* __restrict__ tiles,
+                                                           uint32_t cap, uint32_t* misc) {
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  bool take = false;
+  uint32_t id = 0;
+  if (i < nbad) {
+    const uint32_t e = bad[i];
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    atomicOr(&pbits[e >> 5], 1u << (e & 31u));
+    id = tile_of<ND>(c, ty, tz);
+    take = claim_bit(tbits, id);
+  }
+  wave_append(take, id, tiles, cap, &misc[kPpTiles]);
+}
+
+// Search state of one window.  Elements: the window's plateau points, then the plateau
+// points next to them.  Every interior element is a constraint; an attempt frees some
+// window points (every other element keeps its value) and checks the constraints that
+// read a free point.
+template <int ND>
+struct Window {
+  static constexpr int K = 2 * ND;
+  static constexpr int EM = kWinMax * (K + 1);
+  static constexpr int kGt = 125, kLt = 126, kEq = 127;  // fixed relations (centre vs neighbour)
+  uint32_t el[EM];
+  float val[EM];
+  int8_t cnb[EM][K];  // neighbour codes of an interior element: element index or kGt / kLt / kEq
+  int8_t typ[EM];     // target type, -1 for a boundary element
+  int8_t anch[EM];    // anchored extremum
+  int8_t cls[EM];
+  int8_t fidx[EM];  // free index or -1
+  int8_t felem[kWinMax];
+  int8_t cself[EM];   // constraints grouped by the depth at which they are checked
+  int8_t cstart[kWinMax + 1];
+  int8_t choice[kWinMax];
+  int8_t newcls[kWinMax];
+  float cv[EM + kWinMax];
+  int8_t canchor[EM + kWinMax];
+  float oldv[kWinMax];
+  float flo[kWinMax], fhi[kWinMax];  // open bounds of the free points' values
+  int ne, nw, nf, nc, ncls;
+};
+
+template <int ND>
+__device__ __forceinline__ int find_el(const Window<ND>& w, uint32_t e) {
+  for (int i = 0; i < w.ne; i++)
+    if (w.el[i] == e) return i;
+  return -1;
+}
+
+template <int ND>
+__device__ __forceinline__ bool rel_ok(const Window<ND>& w, int ci) {
+  const int s = w.cself[ci];
+  const int cs = w.cls[s];
+  bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    int r[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int code = w.cnb[s][2 * a + h];
+      if (code == Window<ND>::kGt) r[h] = 1;
+      else if (code == Window<ND>::kLt) r[h] = -1;
+      else if (code == Window<ND>::kEq) r[h] = 0;
+      else r[h] = cs > w.cls[code] ? 1 : (cs < w.cls[code] ? -1 : 0);
+    }
+    const bool hi = r[0] > 0 && r[1] > 0, lo = r[0] < 0 && r[1] < 0;
+    all_gt = all_gt && hi;
+    all_lt = all_lt && lo;
+    any_hi = any_hi || hi;
+    any_lo = any_lo || lo;
+  }
+  const int k = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
+  return k == w.typ[s];
+}
+
+// values of the classes (classes holding a fixed element keep its value, runs of free
+// classes are spaced evenly between their fixed neighbours); false when floats run
+// out or a free point leaves its bounds
+template <int ND>
+__device__ bool realize(Window<ND>& w, float lo, float hi) {
+  for (int k = 0; k < w.ncls; k++) w.canchor[k] = 0;
+  for (int i = 0; i < w.ne; i++)
+    if (w.fidx[i] < 0) {
+      w.cv[w.cls[i]] = w.val[i];
+      w.canchor[w.cls[i]] = 1;
+    }
+  int k = 0;
+  while (k < w.ncls) {
+    if (w.canchor[k]) {
+      k++;
+      continue;
+    }
+    int r = 0;
+    while (k + r < w.ncls && !w.canchor[k + r]) r++;
+    const float L = k > 0 ? w.cv[k - 1] : lo;
+    const float U = k + r < w.ncls ? w.cv[k + r] : hi;
+    for (int j = 0; j < r; j++) w.cv[k + j] = L + (U - L) * ((float)(j + 1) / (float)(r + 1));
+    k += r;
+  }
+  for (int j = 0; j + 1 < w.ncls; j++)
+    if (!(w.cv[j] < w.cv[j + 1])) return false;
+  for (int f = 0; f < w.nf; f++) {
+    const float v = w.cv[w.cls[w.felem[f]]];
+    if (!(v > w.flo[f] && v < w.fhi[f])) return false;
+  }
+  return true;
+}
+
+// breadth-first window over the plateau of b and the neighbour codes of its elements
+template <int ND>
+__device__ void build_window(Window<ND>& w, const Grid& g, const float* __restrict__ d,
+                             const uint8_t* __restrict__ t, const float* __restrict__ y, uint32_t b, bool anchor) {
+  constexpr int K = 2 * ND;
+  const float db = d[b];
+  w.ne = 0;
+  w.el[w.ne++] = b;
+  int head = 0;
+  while (head < w.ne && w.ne < kWinMax) {
+    const uint32_t e = w.el[head++];
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    for (int k = 0; k < K && w.ne < kWinMax; k++) {
+      int q[ND];
+      for (int a = 0; a < ND; a++) q[a] = c[a];
+      const int a = k >> 1;
+      q[a] += (k & 1) ? 1 : -1;
+      if (!inside<ND>(g, q)) continue;
+      const uint32_t s = stride_of<ND>(g, a);
+      const uint32_t qe = (k & 1) ? e + s : e - s;
+      if (d[qe] == db && find_el<ND>(w, qe) < 0) w.el[w.ne++] = qe;
+    }
+  }
+  w.nw = w.ne;
+  for (int i = 0; i < w.nw; i++) {
+    int c[ND];
+    coords_of<ND>(g, w.el[i], c);
+    for (int k = 0; k < K; k++) {
+      int q[ND];
+      for (int a = 0; a < ND; a++) q[a] = c[a];
+      const int a = k >> 1;
+      q[a] += (k & 1) ? 1 : -1;
+      if (!inside<ND>(g, q)) continue;
+      const uint32_t s = stride_of<ND>(g, a);
+      const uint32_t qe = (k & 1) ? w.el[i] + s : w.el[i] - s;
+      if (d[qe] == db && find_el<ND>(w, qe) < 0 && w.ne < Window<ND>::EM) w.el[w.ne++] = qe;
+    }
+  }
+  for (int i = 0; i < w.ne; i++) {
+    const uint32_t e = w.el[i];
+    w.val[i] = y[e];
+    const int tt = t[e];
+    w.anch[i] = anchor && anchored(tt, db);
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    if (!interior<ND>(g, c)) {
+      w.typ[i] = -1;
+      continue;
+    }
+    w.typ[i] = (int8_t)tt;
+    for (int k = 0; k < K; k++) {
+      const uint32_t s = stride_of<ND>(g, k >> 1);
+      const uint32_t qe = (k & 1) ? e + s : e - s;
+      const int j = find_el<ND>(w, qe);
+      if (j >= 0) {
+        w.cnb[i][k] = (int8_t)j;
+      } else {
+        // not an element: another cell (the order of the cells) or a point that keeps
+        // its value next to an element that keeps its value
+        const float dq = d[qe];
+        const float r = dq != db ? db - dq : y[e] - y[qe];
+        w.cnb[i][k] = r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
+      }
     }
   }
 }
 
-// ---------------------------------------------------------------------------------
-// verification: missing / spurious / changed critical points and the largest
-// deviation from the decoded field
-// ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlockThreads) void k_pp_verify(Grid g, const float* __restrict__ yv,
-                                                            const uint8_t* __restrict__ t,
-                                                            const float* __restrict__ d, uint32_t n,
-                                                            unsigned long long* __restrict__ out /* 4 */) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t miss = 0, spur = 0, chg = 0;
-  uint32_t dev = 0;
-  if (e < n) {
-    const Coord k = coord_of(g, e);
-    const int c = class_at(g, yv, k.c[0], k.c[1], k.c[2], 0xffffffffu, 0.f, 0xffffffffu, 0.f);
-    const int tt = t[e];
-    miss = (tt != 0 && c == 0);
-    spur = (tt == 0 && c != 0);
-    chg = (tt != 0 && c != 0 && c != tt);
-    if (d) dev = __float_as_uint(fabsf(yv[e] - d[e]));
+// One attempt on a built window.  mode 0: the anchored extrema keep their values;
+// 1: window element `pick` (an anchored extremum) moves too, within (v - delta, v + delta),
+// which keeps it between the extrema of the neighbouring sort positions; 2: `pick`
+// moves freely; 3: every window point moves.
+template <int ND>
+__device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* __restrict__ y, float lo,
+                        float hi, int mode, int pick, float delta) {
+  w.nf = 0;
+  for (int i = 0; i < w.ne; i++) {
+    const bool fr = i < w.nw && (!w.anch[i] || mode == 3 || i == pick);
+    w.fidx[i] = fr ? (int8_t)w.nf : (int8_t)-1;
+    w.cls[i] = -1;
+    if (fr) {
+      w.flo[w.nf] = -INFINITY;
+      w.fhi[w.nf] = INFINITY;
+      if (mode == 1 && i == pick) {
+        w.flo[w.nf] = w.val[i] - delta;
+        w.fhi[w.nf] = w.val[i] + delta;
+      }
+      w.felem[w.nf++] = (int8_t)i;
+    }
   }
-  miss = seg_sum<kWave>(miss);
-  spur = seg_sum<kWave>(spur);
-  chg = seg_sum<kWave>(chg);
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t w = __shfl_xor(dev, o, kWave);
-    dev = w > dev ? w : dev;
+  if (w.nf == 0) return false;
+  // classes of the fixed elements: distinct values in increasing order
+  w.ncls = 0;
+  for (int i = 0; i < w.ne; i++) {
+    if (w.fidx[i] >= 0) continue;
+    int rank = 0;
+    bool dup = false;
+    for (int j = 0; j < w.ne; j++) {
+      if (w.fidx[j] >= 0 || !(w.val[j] < w.val[i])) continue;
+      bool first = true;  // count each smaller distinct value once
+      for (int h = 0; h < j; h++)
+        if (w.fidx[h] < 0 && w.val[h] == w.val[j]) first = false;
+      rank += first;
+    }
+    for (int h = 0; h < i; h++)
+      if (w.fidx[h] < 0 && w.val[h] == w.val[i]) dup = true;
+    w.cls[i] = (int8_t)rank;
+    if (!dup) w.ncls++;
   }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    if (miss) atomicAdd(&out[0], (unsigned long long)miss);
-    if (spur) atomicAdd(&out[1], (unsigned long long)spur);
-    if (chg) atomicAdd(&out[2], (unsigned long long)chg);
-    if (dev) atomicMax(&out[3], (unsigned long long)dev);
+  // constraints that read a free point, checked once the last of them is placed
+  // (grouped by that depth)
+  w.nc = 0;
+  for (int f = 0; f < w.nf; f++) {
+    w.cstart[f] = (int8_t)w.nc;
+    for (int i = 0; i < w.ne; i++) {
+      if (w.typ[i] < 0) continue;
+      int dep = w.fidx[i];
+      for (int k = 0; k < 2 * ND; k++) {
+        const int code = w.cnb[i][k];
+        if (code < Window<ND>::kGt && w.fidx[code] > dep) dep = w.fidx[code];
+      }
+      if (dep == f) w.cself[w.nc++] = (int8_t)i;
+    }
   }
+  w.cstart[w.nf] = (int8_t)w.nc;
+  // depth-first search over the weak orders of the free points
+  const int budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
+  int depth = 0, nodes = 0;
+  w.choice[0] = -1;
+  while (depth >= 0) {
+    const int fe = w.felem[depth];
+    if (w.choice[depth] >= 0) {  // undo the previous choice at this depth
+      const int pos = w.cls[fe];
+      w.cls[fe] = -1;
+      if (w.newcls[depth]) {
+        for (int i = 0; i < w.ne; i++)
+          if (w.cls[i] > pos) w.cls[i]--;
+        w.ncls--;
+      }
+    }
+    const int ch = ++w.choice[depth];
+    if (ch > 2 * w.ncls) {
+      w.choice[depth] = -1;
+      depth--;
+      continue;
+    }
+    if (++nodes > budget) break;
+    const int pos = ch >> 1;
+    if ((ch & 1) == 0) {  // a new class at position pos
+      for (int i = 0; i < w.ne; i++)
+        if (w.cls[i] >= pos) w.cls[i]++;
+      w.cls[fe] = (int8_t)pos;
+      w.ncls++;
+      w.newcls[depth] = 1;
+    } else {  // join class pos
+      w.cls[fe] = (int8_t)pos;
+      w.newcls[depth] = 0;
+    }
+    bool ok = true;
+    for (int ci = w.cstart[depth]; ci < w.cstart[depth + 1] && ok; ci++) ok = rel_ok<ND>(w, ci);
+    if (!ok) continue;
+    if (depth + 1 < w.nf) {
+      depth++;
+      w.choice[depth] = -1;
+      continue;
+    }
+    // a complete order: turn it into values and check them on the field
+    if (!realize<ND>(w, lo, hi)) continue;
+    for (int f = 0; f < w.nf; f++) {
+      const uint32_t e = w.el[w.felem[f]];
+      w.oldv[f] = y[e];
+      y[e] = w.cv[w.cls[w.felem[f]]];
+    }
+    bool good = true;
+    for (int ci = 0; ci < w.nc && good; ci++) {
+      const uint32_t e = w.el[w.cself[ci]];
+      int c[ND];
+      coords_of<ND>(g, e, c);
+      good = classify_at<ND>(g, y, e, c) == (int)t[e];
+    }
+    if (good) return true;
+    for (int f = 0; f < w.nf; f++) y[w.el[w.felem[f]]] = w.oldv[f];
+  }
+  return false;
+}
+
+// Solve the window of mismatch b; returns the mode that succeeded or -1.  An extremum
+// whose original value lies on a cell boundary can carry the sort position of the
+// neighbouring bin (the bin rounds x + eb in float, the cell truncates x / eb in
+// double), which makes the anchored values of its window contradictory; modes 1-3
+// free extrema, the most extreme anchored value first.
+template <int ND>
+__device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
+                            float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac) {
+  float lo, hi;
+  cell_of(d[b], eb, &lo, &hi);
+  build_window<ND>(w, g, d, t, y, b, anchor);
+  if (attempt<ND>(w, g, t, y, lo, hi, 0, -1, 0.f)) return 0;
+  int na = 0;
+  for (int i = 0; i < w.nw; i++) na += w.anch[i];
+  if (na == 0) return -1;
+  const float delta = delta_frac * (hi - lo);
+  for (int i = 0; i < w.nw; i++)
+    if (w.anch[i] && attempt<ND>(w, g, t, y, lo, hi, 1, i, delta)) return 1;
+  // single extremum moved freely, most extreme anchored value first
+  const float mid = 0.5f * (lo + hi);
+  uint32_t tried = 0;
+  for (int r = 0; r < na; r++) {
+    int pick = -1;
+    float far = -1.f;
+    for (int i = 0; i < w.nw; i++)
+      if (w.anch[i] && !((tried >> i) & 1u) && fabsf(w.val[i] - mid) > far) {
+        far = fabsf(w.val[i] - mid);
+        pick = i;
+      }
+    if (pick < 0) break;
+    tried |= 1u << pick;
+    if (attempt<ND>(w, g, t, y, lo, hi, 2, pick, 0.f)) return 2;
+  }
+  if (na > 1 && attempt<ND>(w, g, t, y, lo, hi, 3, -1, 0.f)) return 3;
+  return -1;
+}
+
+template <int ND>
+__global__ __launch_bounds__(64) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
+                                                 uint32_t ty, uint32_t tz, int phase,
+                                                 const uint32_t* __restrict__ pbits, const float* __restrict__ d,
+                                                 const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
+                                                 int anchor, uint32_t* misc) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = misc[kPpTiles];
+  cnt = cnt < cap ? cnt : cap;
+  if (i >= cnt) return;
+  uint32_t id = tiles[i];
+  int tc[ND];
+  if constexpr (ND == 2) {
+    tc[1] = (int)(id % ty);
+    tc[0] = (int)(id / ty);
+  } else {
+    tc[2] = (int)(id % tz);
+    id /= tz;
+    tc[1] = (int)(id % ty);
+    tc[0] = (int)(id / ty);
+  }
+  int ph = 0;
+#pragma unroll
+  for (int a = 0; a < ND; a++) ph |= (tc[a] & 1) << a;
+  if (ph != phase) return;
+  Window<ND> w;
+  uint32_t solved = 0, failed = 0, freed = 0;
+  // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
+  const float delta_frac = 0.99f * 0.9f / ((float)misc[kPpSpMax] + 2.f);
+  constexpr int kT = 1 << kTileShift;
+  const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
+  const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
+  int z0 = 0, z1 = 1;
+  if constexpr (ND == 3) {
+    z0 = tc[2] * kT;
+    z1 = min(z0 + kT, (int)g.d2);
+  }
+  const int ya = ND == 3 ? y0 : 0, yb = ND == 3 ? y1 : 1;
+  for (int x = x0; x < x1; x++) {
+    for (int yy = ya; yy < yb; yy++) {
+      // one row segment of the tile along the fastest axis, in flat order
+      uint32_t e0;
+      int len;
+      if constexpr (ND == 2) {
+        e0 = (uint32_t)x * g.d1 + (uint32_t)y0;
+        len = y1 - y0;
+      } else {
+        e0 = (uint32_t)x * g.plane + (uint32_t)yy * g.d2 + (uint32_t)z0;
+        len = z1 - z0;
+      }
+      const uint64_t wv = (uint64_t)pbits[e0 >> 5] | ((uint64_t)pbits[(e0 >> 5) + 1] << 32);
+      uint32_t bits = (uint32_t)(wv >> (e0 & 31u)) & (uint32_t)((1ull << len) - 1ull);
+      while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        const uint32_t e = e0 + (uint32_t)j;
+        int c[ND];
+        coords_of<ND>(g, e, c);
+        if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;
+        const int mode = solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac);
+        solved += mode >= 0;
+        failed += mode < 0;
+        freed += mode >= 2;
+      }
+    }
+  }
+  if (solved) atomicAdd(&misc[kPpSolved], solved);
+  if (failed) atomicAdd(&misc[kPpFailed], failed);
+  if (freed) atomicAdd(&misc[kPpFreed], freed);
 }
 
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
 static inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kBlockThreads - 1) / kBlockThreads); }
+static inline unsigned grid_stride_blocks(uint64_t n) {
+  const unsigned b = blocks_for(n);
+  return b < 4096u ? (b ? b : 1u) : 4096u;
+}
 
-hipError_t pp_init(const float* d, const uint8_t* t, float* y, uint32_t n, float eb, hipStream_t st) {
-  hipLaunchKernelGGL(k_pp_init, dim3(blocks_for(n)), dim3(kBlockThreads), 0, st, d, t, y, n, eb);
+uint32_t pp_init_tiles(uint32_t n) { return (n + kInitTile - 1) / kInitTile; }
+
+hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st) {
+  if (!m) return hipSuccess;
+  hipLaunchKernelGGL(k_pp_spmax, dim3(grid_stride_blocks(m)), dim3(kBlockThreads), 0, st, sp, m, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_ext_scan(const uint8_t* t, uint32_t n, uint32_t* ext_pos, uint32_t cap, uint64_t* st_words,
-                       uint32_t* counter, uint32_t* err, uint32_t* total, hipStream_t st) {
-  const uint32_t tiles = (n + kPpTile - 1) / kPpTile;
-  hipLaunchKernelGGL(k_pp_ext_scan, dim3(tiles), dim3(kBlockThreads), 0, st, t, n, ext_pos, cap, st_words, counter,
-                     err, total);
-  return hipGetLastError();
-}
-uint32_t pp_ext_tiles(uint32_t n) { return (n + kPpTile - 1) / kPpTile; }
-
-hipError_t pp_extrema(const float* d, const uint8_t* t, const uint32_t* ext_pos, const int32_t* sp, uint32_t n_ext,
-                      const uint32_t* n_found, uint32_t n, float* y, float eb, hipStream_t st) {
-  if (!n_ext) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_extrema, dim3(blocks_for(n_ext)), dim3(kBlockThreads), 0, st, d, t, ext_pos, sp, n_ext,
-                     n_found, n, y, eb);
+hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
+                   float* y, uint64_t* status, uint32_t* misc, hipStream_t st) {
+  hipLaunchKernelGGL(k_pp_init, dim3(pp_init_tiles(n)), dim3(kBlockThreads), 0, st, d, t, sp, n_ext, n, eb, y, status,
+                     misc);
   return hipGetLastError();
 }
 
-hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, uint32_t n, uint32_t* nbad, uint32_t* badlist,
-                   uint32_t badcap, hipStream_t st) {
-  hipLaunchKernelGGL(k_pp_mark, dim3(blocks_for(n)), dim3(kBlockThreads), 0, st, g, y, t, n, nbad, badlist, badcap);
+hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
+                   uint32_t cap, uint32_t* misc, hipStream_t st) {
+  const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
+  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark<2>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
+  else hipLaunchKernelGGL(k_pp_mark<3>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_active(const Grid& g, const uint32_t* badlist, uint32_t nbad, uint8_t* act, hipStream_t st) {
+hipError_t pp_active(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
+                     uint32_t* bits, uint32_t* act, uint32_t cap, int anchor, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_active, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, g, badlist, nbad, act);
+  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
+  if (g.ndim == 2)
+    hipLaunchKernelGGL(k_pp_active<2>, grid, blk, 0, st, g, bad, nbad, d, t, bits, act, cap, anchor, misc);
+  else hipLaunchKernelGGL(k_pp_active<3>, grid, blk, 0, st, g, bad, nbad, d, t, bits, act, cap, anchor, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_move(const Grid& g, const float* d, const uint8_t* t, float* y, uint8_t* act, uint32_t n, float eb,
-                   int colour, uint32_t* moves, hipStream_t st) {
-  hipLaunchKernelGGL(k_pp_move, dim3(blocks_for(n)), dim3(kBlockThreads), 0, st, g, d, t, y, act, n, eb, colour,
-                     moves);
+hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
+                   const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st) {
+  const uint32_t m = max_act < cap ? max_act : cap;
+  if (!m) return hipSuccess;
+  const dim3 grid(blocks_for(m)), blk(kBlockThreads);
+  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+  else hipLaunchKernelGGL(k_pp_move<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_pair(const Grid& g, const float* d, const uint8_t* t, float* y, const uint32_t* badlist, uint32_t nbad,
-                   uint8_t* done, uint32_t* locks, uint32_t tiles_x, uint32_t tiles_y, int phase, float eb,
-                   uint32_t* moves, uint32_t* pending, hipStream_t st) {
+hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
+                     uint32_t* bits, uint32_t* bad, uint32_t cap, uint32_t* misc, hipStream_t st) {
+  if (!nold) return hipSuccess;
+  const dim3 grid(blocks_for(nold)), blk(kBlockThreads);
+  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_remark<2>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, misc);
+  else hipLaunchKernelGGL(k_pp_remark<3>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, misc);
+  return hipGetLastError();
+}
+
+void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz) {
+  const uint32_t s = 1u << kTileShift;
+  *tx = (g.d0 + s - 1) / s;
+  *ty = (g.d1 + s - 1) / s;
+  *tz = g.ndim == 3 ? (g.d2 + s - 1) / s : 1u;
+}
+
+hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t ty, uint32_t tz, uint32_t* pbits,
+                    uint32_t* tbits, uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_pair, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, g, d, t, y, badlist, nbad, done,
-                     locks, tiles_x, tiles_y, phase, eb, moves, pending);
+  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
+  if (g.ndim == 2)
+    hipLaunchKernelGGL(k_pp_tiles<2>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
+  else hipLaunchKernelGGL(k_pp_tiles<3>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_verify(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n,
-                     unsigned long long* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pp_verify, dim3(blocks_for(n)), dim3(kBlockThreads), 0, st, g, y, t, d, n, out);
+hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t max_tiles, uint32_t cap, uint32_t ty, uint32_t tz,
+                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
+                    int anchor, uint32_t* misc, hipStream_t st) {
+  const uint32_t m = max_tiles < cap ? max_tiles : cap;
+  if (!m) return hipSuccess;
+  const dim3 grid((m + 63) / 64), blk(64);
+  if (g.ndim == 2)
+    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
+                       misc);
+  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
+                          misc);
   return hipGetLastError();
 }
 

@@ -32,7 +32,7 @@ class TopoInfo(C.Structure):
 
 class RestoreInfo(C.Structure):
     _fields_ = [("missing", C.c_uint64), ("spurious", C.c_uint64), ("changed", C.c_uint64),
-                ("moves", C.c_uint64), ("pair_moves", C.c_uint64), ("rounds", C.c_uint32),
+                ("moves", C.c_uint64), ("windows", C.c_uint64), ("windows_reordered", C.c_uint64), ("rounds", C.c_uint32),
                 ("max_dev_over_eb", C.c_float)]
 
     def mismatches(self):
