@@ -717,7 +717,9 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
         HIPCHK(pp_move(g, act, nl, nl, col, d, d_types, d_field, eb, misc, st));
       rounds++;
       dirty = true;
-      if (complete) {
+      // re-check around the listed mismatches, or the whole field when that is cheaper
+      const uint64_t ring = ndim == 2 ? 13 : 25;
+      if (complete && (uint64_t)nl * ring < n / 4) {
         HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
         HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
         HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, misc, st));
@@ -745,17 +747,25 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
     const uint32_t nl = nb < cap ? nb : cap;
     const uint32_t tcap = (uint32_t)(nl < ntiles ? nl : ntiles);
     uint32_t *tbits, *tiles;
+    const int phases = 1 << ndim;
     WS(kWsPpTBits, (ntiles / 32 + 4) * 4, tbits);
-    WS(kWsPpTiles, (size_t)tcap * 4 + 16, tiles);
+    WS(kWsPpTiles, (size_t)tcap * phases * 4 + 16, tiles);
     HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
     HIPCHK(launch_fill_u32(tbits, 0u, ntiles / 32 + 4, st));
-    HIPCHK(launch_fill_u32(misc + kPpTiles, 0u, 1, st));
     HIPCHK(launch_fill_u32(misc + kPpSolved, 0u, 2, st));
     HIPCHK(launch_fill_u32(misc + kPpFreed, 0u, 1, st));
+    HIPCHK(launch_fill_u32(misc + kPpTilesP, 0u, 8, st));
+    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 2, st));
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
-    for (int ph = 0; ph < (1 << ndim); ph++)
-      HIPCHK(pp_solve(g, tiles, tcap, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+    if ((r = read_misc())) return r;
+    uint32_t tile_count[8];
+    for (int ph = 0; ph < phases; ph++) tile_count[ph] = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+    for (int ph = 0; ph < phases; ph++)
+      HIPCHK(pp_solve(g, tiles, tile_count[ph], tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
     if ((r = full_mark(&nb))) return r;
+    if (getenv("TSZP_RESTORE_STATS"))
+      fprintf(stderr, "[restore] outer %d: listed %u solved %u failed %u freed %u nodes %u max nodes/tile %u left %u\n",
+              outer, nl, h[kPpSolved], h[kPpFailed], h[kPpFreed], h[kPpStatNodes], h[kPpStatMaxNodes], nb);
     solved += h[kPpSolved];
     freed += h[kPpFreed];
     if (!h[kPpSolved] && outer > 0) break;

@@ -106,10 +106,13 @@ hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint3
 
 // ---- topology restoration (tszp_post.hip) ----
 // Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.
-constexpr int kPpMiscWords = 32;
+constexpr int kPpMiscWords = 64;
 enum : int {
   kPpBad = 0, kPpMoves = 1, kPpAct = 2, kPpTiles = 3, kPpVerify = 4, kPpErr = 12, kPpExt = 13, kPpSpMax = 14,
-  kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18, kPpActC = 20  // 7 colour-list counts
+  kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18,
+  kPpActC = 20,      // 7 colour-list counts
+  kPpTilesP = 32,    // 8 parity-phase tile-list counts
+  kPpStatNodes = 40, kPpStatMaxNodes = 41
 };
 uint32_t pp_init_tiles(uint32_t n);
 hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);

@@ -628,24 +628,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(Grid g, const uint32
                                                            uint32_t ty, uint32_t tz, uint32_t* pbits,
                                                            uint32_t* tbits, uint32_t* __restrict__ tiles,
                                                            uint32_t cap, uint32_t* misc) {
+  // one tile list per parity phase: tiles + phase * cap
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   bool take = false;
   uint32_t id = 0;
+  int ph = 0;
   if (i < nbad) {
     const uint32_t e = bad[i];
     int c[ND];
     coords_of<ND>(g, e, c);
     atomicOr(&pbits[e >> 5], 1u << (e & 31u));
     id = tile_of<ND>(c, ty, tz);
+#pragma unroll
+    for (int a = 0; a < ND; a++) ph |= (int)(((uint32_t)c[a] >> kTileShift) & 1u) << a;
     take = claim_bit(tbits, id);
   }
-  wave_append(take, id, tiles, cap, &misc[kPpTiles]);
+#pragma unroll
+  for (int p = 0; p < (1 << ND); p++)
+    wave_append(take && ph == p, id, tiles + (size_t)p * cap, cap, &misc[kPpTilesP + p]);
 }
 
-// Search state of one window.  Elements: the window's plateau points, then the plateau
-// points next to them.  Every interior element is a constraint; an attempt frees some
-// window points (every other element keeps its value) and checks the constraints that
-// read a free point.
+// Search state of one window (LDS, one per wavefront).  Elements: the window's plateau
+// points, then the plateau points next to them.  Every interior element is a
+// constraint; an attempt frees some window points (every other element keeps its
+// value) and checks the constraints that read a free point.
 template <int ND>
 struct Window {
   static constexpr int K = 2 * ND;
@@ -659,15 +665,18 @@ struct Window {
   int8_t cls[EM];
   int8_t fidx[EM];  // free index or -1
   int8_t felem[kWinMax];
-  int8_t cself[EM];   // constraints grouped by the depth at which they are checked
+  int8_t cself[EM];  // constraints grouped by the depth at which they are checked
   int8_t cstart[kWinMax + 1];
   int8_t choice[kWinMax];
   int8_t newcls[kWinMax];
+  int8_t picks[kWinMax];
   float cv[EM + kWinMax];
   int8_t canchor[EM + kWinMax];
   float oldv[kWinMax];
   float flo[kWinMax], fhi[kWinMax];  // open bounds of the free points' values
-  int ne, nw, nf, nc, ncls;
+  uint32_t cand[kWave];
+  int ne, nw, nf, nc, ncls, depth, nodes, budget, npick, found;
+  float lo, hi, db;
 };
 
 template <int ND>
@@ -707,7 +716,7 @@ __device__ __forceinline__ bool rel_ok(const Window<ND>& w, int ci) {
 // classes are spaced evenly between their fixed neighbours); false when floats run
 // out or a free point leaves its bounds
 template <int ND>
-__device__ bool realize(Window<ND>& w, float lo, float hi) {
+__device__ bool realize(Window<ND>& w) {
   for (int k = 0; k < w.ncls; k++) w.canchor[k] = 0;
   for (int i = 0; i < w.ne; i++)
     if (w.fidx[i] < 0) {
@@ -722,8 +731,8 @@ __device__ bool realize(Window<ND>& w, float lo, float hi) {
     }
     int r = 0;
     while (k + r < w.ncls && !w.canchor[k + r]) r++;
-    const float L = k > 0 ? w.cv[k - 1] : lo;
-    const float U = k + r < w.ncls ? w.cv[k + r] : hi;
+    const float L = k > 0 ? w.cv[k - 1] : w.lo;
+    const float U = k + r < w.ncls ? w.cv[k + r] : w.hi;
     for (int j = 0; j < r; j++) w.cv[k + j] = L + (U - L) * ((float)(j + 1) / (float)(r + 1));
     k += r;
   }
@@ -736,81 +745,102 @@ __device__ bool realize(Window<ND>& w, float lo, float hi) {
   return true;
 }
 
-// breadth-first window over the plateau of b and the neighbour codes of its elements
+// neighbour k of element point e, if it lies in the plateau of value db
 template <int ND>
-__device__ void build_window(Window<ND>& w, const Grid& g, const float* __restrict__ d,
-                             const uint8_t* __restrict__ t, const float* __restrict__ y, uint32_t b, bool anchor) {
-  constexpr int K = 2 * ND;
-  const float db = d[b];
-  w.ne = 0;
-  w.el[w.ne++] = b;
-  int head = 0;
-  while (head < w.ne && w.ne < kWinMax) {
-    const uint32_t e = w.el[head++];
-    int c[ND];
-    coords_of<ND>(g, e, c);
-    for (int k = 0; k < K && w.ne < kWinMax; k++) {
-      int q[ND];
-      for (int a = 0; a < ND; a++) q[a] = c[a];
-      const int a = k >> 1;
-      q[a] += (k & 1) ? 1 : -1;
-      if (!inside<ND>(g, q)) continue;
-      const uint32_t s = stride_of<ND>(g, a);
-      const uint32_t qe = (k & 1) ? e + s : e - s;
-      if (d[qe] == db && find_el<ND>(w, qe) < 0) w.el[w.ne++] = qe;
-    }
-  }
-  w.nw = w.ne;
-  for (int i = 0; i < w.nw; i++) {
-    int c[ND];
-    coords_of<ND>(g, w.el[i], c);
-    for (int k = 0; k < K; k++) {
-      int q[ND];
-      for (int a = 0; a < ND; a++) q[a] = c[a];
-      const int a = k >> 1;
-      q[a] += (k & 1) ? 1 : -1;
-      if (!inside<ND>(g, q)) continue;
-      const uint32_t s = stride_of<ND>(g, a);
-      const uint32_t qe = (k & 1) ? w.el[i] + s : w.el[i] - s;
-      if (d[qe] == db && find_el<ND>(w, qe) < 0 && w.ne < Window<ND>::EM) w.el[w.ne++] = qe;
-    }
-  }
-  for (int i = 0; i < w.ne; i++) {
-    const uint32_t e = w.el[i];
-    w.val[i] = y[e];
-    const int tt = t[e];
-    w.anch[i] = anchor && anchored(tt, db);
-    int c[ND];
-    coords_of<ND>(g, e, c);
-    if (!interior<ND>(g, c)) {
-      w.typ[i] = -1;
-      continue;
-    }
-    w.typ[i] = (int8_t)tt;
-    for (int k = 0; k < K; k++) {
-      const uint32_t s = stride_of<ND>(g, k >> 1);
-      const uint32_t qe = (k & 1) ? e + s : e - s;
-      const int j = find_el<ND>(w, qe);
-      if (j >= 0) {
-        w.cnb[i][k] = (int8_t)j;
-      } else {
-        // not an element: another cell (the order of the cells) or a point that keeps
-        // its value next to an element that keeps its value
-        const float dq = d[qe];
-        const float r = dq != db ? db - dq : y[e] - y[qe];
-        w.cnb[i][k] = r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
-      }
-    }
-  }
+__device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, const float* __restrict__ d, uint32_t e, int k,
+                                                float db) {
+  int c[ND];
+  coords_of<ND>(g, e, c);
+  const int a = k >> 1;
+  c[a] += (k & 1) ? 1 : -1;
+  if (!inside<ND>(g, c)) return 0xffffffffu;
+  const uint32_t s = stride_of<ND>(g, a);
+  const uint32_t qe = (k & 1) ? e + s : e - s;
+  return d[qe] == db ? qe : 0xffffffffu;
 }
 
-// One attempt on a built window.  mode 0: the anchored extrema keep their values;
-// 1: window element `pick` (an anchored extremum) moves too, within (v - delta, v + delta),
-// which keeps it between the extrema of the neighbouring sort positions; 2: `pick`
-// moves freely; 3: every window point moves.
+// Builds the window of mismatch b (whole wavefront: the lanes load neighbours in
+// parallel, lane 0 keeps the lists in order).
 template <int ND>
-__device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* __restrict__ y, float lo,
-                        float hi, int mode, int pick, float delta) {
+__device__ void build_window(Window<ND>& w, const Grid& g, const float* __restrict__ d,
+                             const uint8_t* __restrict__ t, const float* __restrict__ y, float eb, uint32_t b,
+                             bool anchor, int lane) {
+  constexpr int K = 2 * ND;
+  if (lane == 0) {
+    w.ne = 1;
+    w.el[0] = b;
+    w.db = d[b];
+    cell_of(w.db, eb, &w.lo, &w.hi);
+  }
+  __syncthreads();
+  const float db = w.db;
+  // breadth-first levels over the plateau, up to kWinMax points
+  int head = 0;
+  while (true) {
+    const int ne0 = w.ne;
+    if (head >= ne0 || ne0 >= kWinMax) break;
+    const int ncand = (ne0 - head) * K;
+    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, d, w.el[head + lane / K], lane % K, db) : 0xffffffffu;
+    __syncthreads();
+    if (lane == 0)
+      for (int j = 0; j < ncand && w.ne < kWinMax; j++) {
+        const uint32_t q = w.cand[j];
+        if (q != 0xffffffffu && find_el<ND>(w, q) < 0) w.el[w.ne++] = q;
+      }
+    __syncthreads();
+    head = ne0;
+  }
+  if (lane == 0) w.nw = w.ne;
+  __syncthreads();
+  // the plateau points next to the window keep their values
+  {
+    const int ncand = w.nw * K;
+    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, d, w.el[lane / K], lane % K, db) : 0xffffffffu;
+    __syncthreads();
+    if (lane == 0)
+      for (int j = 0; j < ncand; j++) {
+        const uint32_t q = w.cand[j];
+        if (q != 0xffffffffu && find_el<ND>(w, q) < 0 && w.ne < Window<ND>::EM) w.el[w.ne++] = q;
+      }
+    __syncthreads();
+  }
+  const int ne = w.ne;
+  if (lane < ne) {
+    const uint32_t e = w.el[lane];
+    w.val[lane] = y[e];
+    const int tt = t[e];
+    w.anch[lane] = anchor && anchored(tt, db);
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    w.typ[lane] = interior<ND>(g, c) ? (int8_t)tt : (int8_t)-1;
+  }
+  __syncthreads();
+  for (int j = lane; j < ne * K; j += kWave) {
+    const int i = j / K, k = j % K;
+    if (w.typ[i] < 0) continue;
+    const uint32_t e = w.el[i];
+    const uint32_t s = stride_of<ND>(g, k >> 1);
+    const uint32_t qe = (k & 1) ? e + s : e - s;
+    const int jj = find_el<ND>(w, qe);
+    if (jj >= 0) {
+      w.cnb[i][k] = (int8_t)jj;
+    } else {
+      // not an element: another cell (the order of the cells) or a point that keeps its
+      // value next to an element that keeps its value
+      const float dq = d[qe];
+      const float r = dq != db ? db - dq : w.val[i] - y[qe];
+      w.cnb[i][k] = r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
+    }
+  }
+  __syncthreads();
+}
+
+// Sets up an attempt (lane 0).  mode 0: the anchored extrema keep their values; 1:
+// window element `pick` (an anchored extremum) moves too, within (v - delta, v + delta),
+// which keeps it between the extrema of the neighbouring sort positions; 2: `pick`
+// moves freely; 3: every window point moves.  Returns false when nothing is free.
+template <int ND>
+__device__ bool setup_attempt(Window<ND>& w, int mode, int pick, float delta) {
   w.nf = 0;
   for (int i = 0; i < w.ne; i++) {
     const bool fr = i < w.nw && (!w.anch[i] || mode == 3 || i == pick);
@@ -827,26 +857,25 @@ __device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict_
     }
   }
   if (w.nf == 0) return false;
-  // classes of the fixed elements: distinct values in increasing order
-  w.ncls = 0;
+  // classes of the fixed elements: distinct values in increasing order (insertion sort
+  // of their indices in cand[])
+  int nfix = 0;
   for (int i = 0; i < w.ne; i++) {
     if (w.fidx[i] >= 0) continue;
-    int rank = 0;
-    bool dup = false;
-    for (int j = 0; j < w.ne; j++) {
-      if (w.fidx[j] >= 0 || !(w.val[j] < w.val[i])) continue;
-      bool first = true;  // count each smaller distinct value once
-      for (int h = 0; h < j; h++)
-        if (w.fidx[h] < 0 && w.val[h] == w.val[j]) first = false;
-      rank += first;
+    int j = nfix++;
+    while (j > 0 && w.val[w.cand[j - 1]] > w.val[i]) {
+      w.cand[j] = w.cand[j - 1];
+      j--;
     }
-    for (int h = 0; h < i; h++)
-      if (w.fidx[h] < 0 && w.val[h] == w.val[i]) dup = true;
-    w.cls[i] = (int8_t)rank;
-    if (!dup) w.ncls++;
+    w.cand[j] = (uint32_t)i;
   }
-  // constraints that read a free point, checked once the last of them is placed
-  // (grouped by that depth)
+  w.ncls = 0;
+  for (int j = 0; j < nfix; j++) {
+    if (j > 0 && w.val[w.cand[j]] != w.val[w.cand[j - 1]]) w.ncls++;
+    w.cls[w.cand[j]] = (int8_t)w.ncls;
+  }
+  if (nfix) w.ncls++;
+  // constraints that read a free point, grouped by the depth of their last free point
   w.nc = 0;
   for (int f = 0; f < w.nf; f++) {
     w.cstart[f] = (int8_t)w.nc;
@@ -861,10 +890,20 @@ __device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict_
     }
   }
   w.cstart[w.nf] = (int8_t)w.nc;
-  // depth-first search over the weak orders of the free points
-  const int budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
-  int depth = 0, nodes = 0;
+  w.depth = 0;
   w.choice[0] = -1;
+  w.nodes = 0;
+  w.budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
+  return true;
+}
+
+// Depth-first search over the weak orders of the free points (lane 0), resumed where
+// the previous call stopped.  Returns true with the values of the next complete order
+// in cv[], false when the orders or the budget are exhausted.
+template <int ND>
+__device__ bool next_order(Window<ND>& w) {
+  int depth = w.depth, nodes = w.nodes;
+  bool found = false;
   while (depth >= 0) {
     const int fe = w.felem[depth];
     if (w.choice[depth] >= 0) {  // undo the previous choice at this depth
@@ -882,7 +921,10 @@ __device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict_
       depth--;
       continue;
     }
-    if (++nodes > budget) break;
+    if (++nodes > w.budget) {
+      depth = -1;
+      break;
+    }
     const int pos = ch >> 1;
     if ((ch & 1) == 0) {  // a new class at position pos
       for (int i = 0; i < w.ne; i++)
@@ -902,24 +944,51 @@ __device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict_
       w.choice[depth] = -1;
       continue;
     }
-    // a complete order: turn it into values and check them on the field
-    if (!realize<ND>(w, lo, hi)) continue;
-    for (int f = 0; f < w.nf; f++) {
-      const uint32_t e = w.el[w.felem[f]];
-      w.oldv[f] = y[e];
-      y[e] = w.cv[w.cls[w.felem[f]]];
+    if (realize<ND>(w)) {
+      found = true;
+      break;
     }
+  }
+  w.depth = depth;
+  w.nodes = nodes;
+  return found;
+}
+
+// One attempt (whole wavefront): search orders on lane 0, write each found order to
+// the field and check its constraints in parallel; keep the first that holds.
+template <int ND>
+__device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* __restrict__ y,
+                            int mode, int pick, float delta, int lane, uint32_t* nodes_out) {
+  if (lane == 0) w.found = setup_attempt<ND>(w, mode, pick, delta) ? 1 : 0;
+  __syncthreads();
+  if (!w.found) return false;
+  while (true) {
+    if (lane == 0) w.found = next_order<ND>(w) ? 1 : 0;
+    __syncthreads();
+    if (!w.found) break;
+    const int nf = w.nf;
+    if (lane < nf) {
+      const uint32_t e = w.el[w.felem[lane]];
+      w.oldv[lane] = y[e];
+      y[e] = w.cv[w.cls[w.felem[lane]]];
+    }
+    __syncthreads();
     bool good = true;
-    for (int ci = 0; ci < w.nc && good; ci++) {
+    for (int ci = lane; ci < w.nc; ci += kWave) {
       const uint32_t e = w.el[w.cself[ci]];
       int c[ND];
       coords_of<ND>(g, e, c);
-      good = classify_at<ND>(g, y, e, c) == (int)t[e];
+      good = good && classify_at<ND>(g, y, e, c) == (int)t[e];
     }
-    if (good) return true;
-    for (int f = 0; f < w.nf; f++) y[w.el[w.felem[f]]] = w.oldv[f];
+    const bool all_good = !__any(!good);
+    if (all_good) break;
+    if (lane < nf) y[w.el[w.felem[lane]]] = w.oldv[lane];
+    __syncthreads();
   }
-  return false;
+  if (lane == 0) *nodes_out += (uint32_t)w.nodes;
+  const bool ok = w.found != 0;
+  __syncthreads();
+  return ok;
 }
 
 // Solve the window of mismatch b; returns the mode that succeeded or -1.  An extremum
@@ -929,47 +998,54 @@ __device__ bool attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict_
 // free extrema, the most extreme anchored value first.
 template <int ND>
 __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
-                            float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac) {
-  float lo, hi;
-  cell_of(d[b], eb, &lo, &hi);
-  build_window<ND>(w, g, d, t, y, b, anchor);
-  if (attempt<ND>(w, g, t, y, lo, hi, 0, -1, 0.f)) return 0;
-  int na = 0;
-  for (int i = 0; i < w.nw; i++) na += w.anch[i];
-  if (na == 0) return -1;
-  const float delta = delta_frac * (hi - lo);
-  for (int i = 0; i < w.nw; i++)
-    if (w.anch[i] && attempt<ND>(w, g, t, y, lo, hi, 1, i, delta)) return 1;
-  // single extremum moved freely, most extreme anchored value first
-  const float mid = 0.5f * (lo + hi);
-  uint32_t tried = 0;
-  for (int r = 0; r < na; r++) {
-    int pick = -1;
-    float far = -1.f;
-    for (int i = 0; i < w.nw; i++)
-      if (w.anch[i] && !((tried >> i) & 1u) && fabsf(w.val[i] - mid) > far) {
-        far = fabsf(w.val[i] - mid);
-        pick = i;
+                            float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
+                            uint32_t* nodes) {
+  build_window<ND>(w, g, d, t, y, eb, b, anchor, lane);
+  if (run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes)) return 0;
+  if (lane == 0) {
+    // anchored window points, the most extreme value first
+    const float mid = 0.5f * (w.lo + w.hi);
+    w.npick = 0;
+    for (int i = 0; i < w.nw; i++) {
+      if (!w.anch[i]) continue;
+      int j = w.npick++;
+      while (j > 0 && fabsf(w.val[w.picks[j - 1]] - mid) < fabsf(w.val[i] - mid)) {
+        w.picks[j] = w.picks[j - 1];
+        j--;
       }
-    if (pick < 0) break;
-    tried |= 1u << pick;
-    if (attempt<ND>(w, g, t, y, lo, hi, 2, pick, 0.f)) return 2;
+      w.picks[j] = (int8_t)i;
+    }
   }
-  if (na > 1 && attempt<ND>(w, g, t, y, lo, hi, 3, -1, 0.f)) return 3;
+  __syncthreads();
+  const int na = w.npick;
+  if (na == 0) return -1;
+  const float delta = delta_frac * (w.hi - w.lo);
+  for (int r = 0; r < na; r++)
+    if (run_attempt<ND>(w, g, t, y, 1, w.picks[r], delta, lane, nodes)) return 1;
+  for (int r = 0; r < na; r++)
+    if (run_attempt<ND>(w, g, t, y, 2, w.picks[r], 0.f, lane, nodes)) return 2;
+  if (na > 1 && run_attempt<ND>(w, g, t, y, 3, -1, 0.f, lane, nodes)) return 3;
   return -1;
 }
 
+// One wavefront per tile of the given parity phase: the tile's listed mismatches are
+// solved in flat order.
 template <int ND>
-__global__ __launch_bounds__(64) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                 uint32_t ty, uint32_t tz, int phase,
-                                                 const uint32_t* __restrict__ pbits, const float* __restrict__ d,
-                                                 const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
-                                                 int anchor, uint32_t* misc) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t cnt = misc[kPpTiles];
+__global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
+                                                    uint32_t ty, uint32_t tz, int phase,
+                                                    const uint32_t* __restrict__ pbits, const float* __restrict__ d,
+                                                    const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
+                                                    int anchor, uint32_t* misc) {
+  constexpr int kT = 1 << kTileShift;
+  constexpr int kTilePts = ND == 2 ? kT * kT : kT * kT * kT;
+  __shared__ Window<ND> w;
+  __shared__ uint32_t blist[kTilePts];
+  __shared__ uint32_t sh_nb;
+  const int lane = threadIdx.x;
+  uint32_t cnt = misc[kPpTilesP + phase];
   cnt = cnt < cap ? cnt : cap;
-  if (i >= cnt) return;
-  uint32_t id = tiles[i];
+  if (blockIdx.x >= cnt) return;
+  uint32_t id = tiles[(size_t)phase * cap + blockIdx.x];
   int tc[ND];
   if constexpr (ND == 2) {
     tc[1] = (int)(id % ty);
@@ -980,15 +1056,6 @@ __global__ __launch_bounds__(64) void k_pp_solve(Grid g, const uint32_t* __restr
     tc[1] = (int)(id % ty);
     tc[0] = (int)(id / ty);
   }
-  int ph = 0;
-#pragma unroll
-  for (int a = 0; a < ND; a++) ph |= (tc[a] & 1) << a;
-  if (ph != phase) return;
-  Window<ND> w;
-  uint32_t solved = 0, failed = 0, freed = 0;
-  // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
-  const float delta_frac = 0.99f * 0.9f / ((float)misc[kPpSpMax] + 2.f);
-  constexpr int kT = 1 << kTileShift;
   const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
   const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
   int z0 = 0, z1 = 1;
@@ -996,38 +1063,63 @@ __global__ __launch_bounds__(64) void k_pp_solve(Grid g, const uint32_t* __restr
     z0 = tc[2] * kT;
     z1 = min(z0 + kT, (int)g.d2);
   }
-  const int ya = ND == 3 ? y0 : 0, yb = ND == 3 ? y1 : 1;
-  for (int x = x0; x < x1; x++) {
-    for (int yy = ya; yy < yb; yy++) {
-      // one row segment of the tile along the fastest axis, in flat order
-      uint32_t e0;
-      int len;
+  // the tile's listed mismatches in flat order: row segments along the fastest axis
+  constexpr int kSegs = ND == 2 ? kT : kT * kT;
+  uint32_t nb = 0;
+  for (int s0 = 0; s0 < kSegs; s0 += kWave) {
+    const int sgi = s0 + lane;
+    uint32_t bits = 0, e0 = 0;
+    if (sgi < kSegs) {
+      int x, len;
       if constexpr (ND == 2) {
+        x = x0 + sgi;
         e0 = (uint32_t)x * g.d1 + (uint32_t)y0;
         len = y1 - y0;
       } else {
+        x = x0 + sgi / kT;
+        const int yy = y0 + sgi % kT;
         e0 = (uint32_t)x * g.plane + (uint32_t)yy * g.d2 + (uint32_t)z0;
         len = z1 - z0;
+        if (yy >= y1) x = x1;
       }
-      const uint64_t wv = (uint64_t)pbits[e0 >> 5] | ((uint64_t)pbits[(e0 >> 5) + 1] << 32);
-      uint32_t bits = (uint32_t)(wv >> (e0 & 31u)) & (uint32_t)((1ull << len) - 1ull);
-      while (bits) {
-        const int j = __ffs(bits) - 1;
-        bits &= bits - 1u;
-        const uint32_t e = e0 + (uint32_t)j;
-        int c[ND];
-        coords_of<ND>(g, e, c);
-        if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;
-        const int mode = solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac);
-        solved += mode >= 0;
-        failed += mode < 0;
-        freed += mode >= 2;
+      if (x < x1) {
+        const uint64_t wv = (uint64_t)pbits[e0 >> 5] | ((uint64_t)pbits[(e0 >> 5) + 1] << 32);
+        bits = (uint32_t)(wv >> (e0 & 31u)) & (uint32_t)((1ull << len) - 1ull);
       }
     }
+    const uint32_t c = (uint32_t)__popc(bits);
+    const uint32_t incl = seg_incl_scan<kWave>(c, lane);
+    uint32_t pos = nb + incl - c;
+    while (bits) {
+      const int j = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      blist[pos++] = e0 + (uint32_t)j;
+    }
+    nb += __shfl(incl, kWave - 1, kWave);
   }
-  if (solved) atomicAdd(&misc[kPpSolved], solved);
-  if (failed) atomicAdd(&misc[kPpFailed], failed);
-  if (freed) atomicAdd(&misc[kPpFreed], freed);
+  if (lane == 0) sh_nb = nb;
+  __syncthreads();
+  nb = sh_nb;
+  uint32_t solved = 0, failed = 0, freed = 0, nodes = 0;
+  // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
+  const float delta_frac = 0.99f * 0.9f / ((float)misc[kPpSpMax] + 2.f);
+  for (uint32_t j = 0; j < nb; j++) {
+    const uint32_t e = blist[j];
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
+    const int mode = solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes);
+    solved += mode >= 0;
+    failed += mode < 0;
+    freed += mode >= 2;
+  }
+  if (lane == 0) {
+    if (solved) atomicAdd(&misc[kPpSolved], solved);
+    if (failed) atomicAdd(&misc[kPpFailed], failed);
+    if (freed) atomicAdd(&misc[kPpFreed], freed);
+    atomicAdd(&misc[kPpStatNodes], nodes);
+    atomicMax(&misc[kPpStatMaxNodes], nodes);
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -1113,7 +1205,7 @@ hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t max_tiles, ui
                     int anchor, uint32_t* misc, hipStream_t st) {
   const uint32_t m = max_tiles < cap ? max_tiles : cap;
   if (!m) return hipSuccess;
-  const dim3 grid((m + 63) / 64), blk(64);
+  const dim3 grid(m), blk(kWave);
   if (g.ndim == 2)
     hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
                        misc);
