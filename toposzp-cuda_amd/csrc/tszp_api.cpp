@@ -47,7 +47,7 @@ int fail(int code, const char* fmt, ...) {
 enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
-  kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
+  kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpTDone, kWsPpMisc, kWsPpSt,
   kWsCount
 };
 
@@ -693,14 +693,32 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
   };
   uint32_t nb = 0;
   if ((r = full_mark(&nb))) return r;
+  const bool stats = getenv("TSZP_RESTORE_STATS") != nullptr;
+  if (stats) fprintf(stderr, "[restore] after init: %u mismatches\n", nb);
   if (use_sp && h[kPpExt] != n_ext)
     return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[kPpExt],
                 (unsigned long long)n_ext);
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
+  bool dirty = false;  // the field changed after the last full pass
+  uint32_t tiny = 0;
+  // plateaus of two or three points: exact in one pass; what stays listed is re-checked
+  if (nb && nb <= cap) {
+    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
+    HIPCHK(launch_fill_u32(misc + kPpTiny, 0u, 1, st));
+    HIPCHK(pp_tiny(g, bad, nb, d, d_types, d_field, eb, use_sp, bits, misc, st));
+    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
+    HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
+    HIPCHK(pp_remark(g, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
+    if ((r = read_misc())) return r;
+    std::swap(bad, bad2);
+    if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);
+    nb = h[kPpBad];
+    tiny = h[kPpTiny];
+    dirty = true;
+  }
   const int colours = ndim == 2 ? 5 : 7;
   uint64_t moves = 0, solved = 0, freed = 0;
   uint32_t rounds = 0;
-  bool dirty = false;  // the field changed after the last full pass
   for (int outer = 0; outer < 4 && nb; outer++) {
     // local search, re-checking only around the previous mismatches
     bool complete = nb <= cap;
@@ -722,7 +740,7 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
       if (complete && (uint64_t)nl * ring < n / 4) {
         HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
         HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
-        HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, misc, st));
+        HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, 2, misc, st));
         if ((r = read_misc())) return r;
         std::swap(bad, bad2);
         nb = h[kPpBad];
@@ -732,6 +750,7 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
       }
       complete = nb <= cap;
       moves += h[kPpMoves];
+      if (stats) fprintf(stderr, "[restore] round %u: listed %u moved %u -> %u mismatches\n", rounds, nl, h[kPpMoves], nb);
       if (!h[kPpMoves]) break;
     }
     if (!nb) break;
@@ -758,12 +777,15 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
     HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 2, st));
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
-    uint32_t tile_count[8];
-    for (int ph = 0; ph < phases; ph++) tile_count[ph] = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
-    for (int ph = 0; ph < phases; ph++)
-      HIPCHK(pp_solve(g, tiles, tile_count[ph], tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+    uint32_t total = 0;
+    for (int ph = 0; ph < phases; ph++) total += h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+    uint32_t* tdone;
+    WS(kWsPpTDone, ntiles * 4 + 16, tdone);
+    HIPCHK(launch_fill_u32(tdone, 0u, ntiles, st));
+    HIPCHK(launch_fill_u32(misc + kPpSolveCounter, 0u, 1, st));
+    HIPCHK(pp_solve(g, tiles, total, tcap, tx, ty, tz, tbits, tdone, bits, d, d_types, d_field, eb, use_sp, misc, st));
     if ((r = full_mark(&nb))) return r;
-    if (getenv("TSZP_RESTORE_STATS"))
+    if (stats)
       fprintf(stderr, "[restore] outer %d: listed %u solved %u failed %u freed %u nodes %u max nodes/tile %u left %u\n",
               outer, nl, h[kPpSolved], h[kPpFailed], h[kPpFreed], h[kPpStatNodes], h[kPpStatMaxNodes], nb);
     solved += h[kPpSolved];
@@ -778,7 +800,7 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
   info->spurious = hv[1];
   info->changed = hv[2];
   info->moves = moves;
-  info->windows = solved;
+  info->windows = solved + tiny;
   info->windows_reordered = freed;
   info->rounds = rounds;
   const uint32_t dv = (uint32_t)hv[3];

@@ -570,10 +570,12 @@ template <int ND>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint32_t* __restrict__ old,
                                                             uint32_t nold, const float* __restrict__ y,
                                                             const uint8_t* __restrict__ t, uint32_t* bits,
-                                                            uint32_t* __restrict__ bad, uint32_t cap,
+                                                            uint32_t* __restrict__ bad, uint32_t cap, int rings,
                                                             uint32_t* misc) {
-  constexpr int R = ND == 2 ? 13 : 25;
-  __shared__ uint32_t lbuf[kBlockThreads * R];
+  // rings: 1 re-checks the listed points only, otherwise everything within two steps
+  constexpr int RM = ND == 2 ? 13 : 25;
+  const int R = rings == 1 ? 1 : RM;
+  __shared__ uint32_t lbuf[kBlockThreads * RM];
   __shared__ uint32_t lcnt, lbase;
   if (threadIdx.x == 0) lcnt = 0;
   __syncthreads();
@@ -603,6 +605,177 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint3
     lds_append(isbad, p, lbuf, &lcnt);
   }
   lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+}
+
+// ---------------------------------------------------------------------------------
+// exact solve of plateaus of two or three points.  Every other neighbour of such a
+// plateau lies in another cell, so only the order inside the plateau decides the
+// types of its points, and no other point's type depends on it: the 3 (two points)
+// or 13 (three points) weak orders are checked in registers and the first that gives
+// every point its type is written.
+// ---------------------------------------------------------------------------------
+__constant__ const int8_t kWeak3[13][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
+                                           {1, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                           {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+__constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                          const float* __restrict__ d, const uint8_t* __restrict__ t,
+                                                          float* __restrict__ y, float eb, int anchor,
+                                                          uint32_t* bits, uint32_t* misc) {
+  constexpr int K = 2 * ND;
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  bool done = false;
+  if (i < nbad) {
+    const uint32_t b = bad[i];
+    const float db = d[b];
+    // the plateau of b, given up beyond three points
+    uint32_t P[3] = {b, b, b};
+    int np = 1;
+    bool big = false;
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+      if (u >= np || big) continue;
+      int c[ND];
+      coords_of<ND>(g, P[u], c);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        int q[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) q[a] = c[a];
+        q[k >> 1] += (k & 1) ? 1 : -1;
+        if (!inside<ND>(g, q)) continue;
+        const uint32_t s = stride_of<ND>(g, k >> 1);
+        const uint32_t qe = (k & 1) ? P[u] + s : P[u] - s;
+        if (d[qe] != db) continue;
+        const bool seen = qe == P[0] || (np > 1 && qe == P[1]) || (np > 2 && qe == P[2]);
+        if (seen) continue;
+        if (np == 3) {
+          big = true;
+        } else {
+          if (np == 1) P[1] = qe;
+          else P[2] = qe;
+          np++;
+        }
+      }
+    }
+    const uint32_t owner = min(P[0], min(P[1], P[2]));
+    if (!big && np >= 2 && claim_bit(bits, owner)) {
+      // per plateau point: type, interior flag, anchor value, and for each neighbour
+      // either the plateau index or the fixed sign of (point - neighbour)
+      int tp[3], rel[3][K];
+      bool intr[3], anc[3];
+      float yv[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        tp[j] = 0;
+        intr[j] = false;
+        anc[j] = false;
+        yv[j] = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; k++) rel[j][k] = 0;
+        if (j >= np) continue;
+        const uint32_t e = P[j];
+        int c[ND];
+        coords_of<ND>(g, e, c);
+        tp[j] = t[e];
+        intr[j] = interior<ND>(g, c);
+        anc[j] = anchor && anchored(tp[j], db);
+        yv[j] = y[e];
+        if (!intr[j]) continue;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const uint32_t s = stride_of<ND>(g, k >> 1);
+          const uint32_t qe = (k & 1) ? e + s : e - s;
+          // 10 + index for a plateau neighbour, else the sign of db - d[q]
+          if (qe == P[0]) rel[j][k] = 10;
+          else if (np > 1 && qe == P[1]) rel[j][k] = 11;
+          else if (np > 2 && qe == P[2]) rel[j][k] = 12;
+          else rel[j][k] = db > d[qe] ? 1 : -1;
+        }
+      }
+      float lo, hi;
+      cell_of(db, eb, &lo, &hi);
+      const int nord = np == 2 ? 3 : 13;
+      for (int o = 0; o < nord && !done; o++) {
+        int rk[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) rk[j] = j < np ? (np == 2 ? kWeak2[o][j] : kWeak3[o][j]) : 0;
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          if (j >= np || !intr[j]) continue;
+          bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+          for (int a = 0; a < ND; a++) {
+            int r2[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              const int code = rel[j][2 * a + h];
+              if (code >= 10) {
+                const int other = code == 10 ? rk[0] : (code == 11 ? rk[1] : rk[2]);
+                r2[h] = rk[j] > other ? 1 : (rk[j] < other ? -1 : 0);
+              } else {
+                r2[h] = code;
+              }
+            }
+            const bool hi2 = r2[0] > 0 && r2[1] > 0, lo2 = r2[0] < 0 && r2[1] < 0;
+            all_gt = all_gt && hi2;
+            all_lt = all_lt && lo2;
+            any_hi = any_hi || hi2;
+            any_lo = any_lo || lo2;
+          }
+          const int k = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
+          ok = ok && k == tp[j];
+        }
+        if (!ok) continue;
+        // values: classes holding an anchor keep its value, the others are spaced
+        // evenly between their neighbours (or the interval ends)
+        int nr = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          if (j < np) nr = max(nr, rk[j] + 1);
+        float cv[3] = {0.f, 0.f, 0.f};
+        bool ca[3] = {false, false, false};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          if (j >= np || !anc[j]) continue;
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            if (k != rk[j]) continue;
+            if (ca[k] && cv[k] != yv[j]) ok = false;
+            ca[k] = true;
+            cv[k] = yv[j];
+          }
+        }
+        if (!ok) continue;
+        float fv[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          fv[k] = cv[k];
+          if (k >= nr || ca[k]) continue;
+          int pa = -1, na = nr;  // nearest anchored classes below and above
+#pragma unroll
+          for (int m = 0; m < 3; m++) {
+            if (m < nr && ca[m] && m < k) pa = m;
+            if (m < nr && ca[m] && m > k && na == nr) na = m;
+          }
+          const float L = pa >= 0 ? (pa == 0 ? cv[0] : (pa == 1 ? cv[1] : cv[2])) : lo;
+          const float U = na < nr ? (na == 0 ? cv[0] : (na == 1 ? cv[1] : cv[2])) : hi;
+          fv[k] = L + (U - L) * ((float)(k - pa) / (float)(na - pa));
+        }
+#pragma unroll
+        for (int k = 0; k + 1 < 3; k++) ok = ok && (k + 1 >= nr || fv[k] < fv[k + 1]);
+        if (!ok) continue;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
+        done = true;
+      }
+    }
+  }
+  wave_count(done, &misc[kPpTiny]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1028,11 +1201,15 @@ __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restric
   return -1;
 }
 
-// One wavefront per tile of the given parity phase: the tile's listed mismatches are
-// solved in flat order.
+// One wavefront per listed tile, all parity phases in one launch.  Tiles are taken in
+// phase order from a counter; a tile first waits until every listed neighbour tile of
+// a lower phase is done, which reproduces the phase-by-phase order (tiles of one
+// phase never share a window) while the phases overlap.  Inside a tile the listed
+// mismatches are solved in flat order.
 template <int ND>
 __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                    uint32_t ty, uint32_t tz, int phase,
+                                                    uint32_t tx, uint32_t ty, uint32_t tz,
+                                                    const uint32_t* __restrict__ tbits, uint32_t* tdone,
                                                     const uint32_t* __restrict__ pbits, const float* __restrict__ d,
                                                     const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
                                                     int anchor, uint32_t* misc) {
@@ -1040,12 +1217,21 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
   constexpr int kTilePts = ND == 2 ? kT * kT : kT * kT * kT;
   __shared__ Window<ND> w;
   __shared__ uint32_t blist[kTilePts];
-  __shared__ uint32_t sh_nb;
+  __shared__ uint32_t sh_nb, sh_item;
   const int lane = threadIdx.x;
-  uint32_t cnt = misc[kPpTilesP + phase];
-  cnt = cnt < cap ? cnt : cap;
-  if (blockIdx.x >= cnt) return;
-  uint32_t id = tiles[(size_t)phase * cap + blockIdx.x];
+  if (lane == 0) sh_item = atomicAdd(&misc[kPpSolveCounter], 1u);
+  __syncthreads();
+  uint32_t item = sh_item;
+  int phase = 0;
+  for (; phase < (1 << ND); phase++) {
+    uint32_t c = misc[kPpTilesP + phase];
+    c = c < cap ? c : cap;
+    if (item < c) break;
+    item -= c;
+  }
+  if (phase >= (1 << ND)) return;
+  const uint32_t tid_ = tiles[(size_t)phase * cap + item];
+  uint32_t id = tid_;
   int tc[ND];
   if constexpr (ND == 2) {
     tc[1] = (int)(id % ty);
@@ -1055,6 +1241,41 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     id /= tz;
     tc[1] = (int)(id % ty);
     tc[0] = (int)(id / ty);
+  }
+  // wait for the listed neighbour tiles of lower phases
+  {
+    constexpr int kNbr = ND == 2 ? 9 : 27;
+    bool ok = true;
+    if (lane < kNbr) {
+      int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+      int nc[ND];
+      bool valid = lane != kNbr / 2;
+      int nph = 0;
+#pragma unroll
+      for (int a = 0; a < ND; a++) {
+        nc[a] = tc[a] + o[a];
+        const int lim = a == 0 ? (int)tx : (a == 1 ? (int)ty : (int)tz);
+        valid = valid && nc[a] >= 0 && nc[a] < lim;
+        nph |= (nc[a] & 1) << a;
+      }
+      if (valid && nph < phase) {
+        const uint32_t nid = ND == 2 ? (uint32_t)nc[0] * ty + (uint32_t)nc[1]
+                                     : ((uint32_t)nc[0] * ty + (uint32_t)nc[1]) * tz + (uint32_t)nc[2];
+        if ((tbits[nid >> 5] >> (nid & 31u)) & 1u) {
+          uint32_t spins = 0;
+          while (__hip_atomic_load(&tdone[nid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+            if (++spins > kSpinLimit) {
+              ok = false;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+          }
+        }
+      }
+    }
+    if (!ok) atomicOr(&misc[kPpErr], TSZP_ERR_LOOKBACK);
+    __threadfence();  // acquire: see the neighbours' values
+    __syncthreads();
   }
   const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
   const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
@@ -1113,7 +1334,10 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     failed += mode < 0;
     freed += mode >= 2;
   }
+  __threadfence();  // release: the tile's values before its done flag
+  __syncthreads();
   if (lane == 0) {
+    __hip_atomic_store(&tdone[tid_], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (solved) atomicAdd(&misc[kPpSolved], solved);
     if (failed) atomicAdd(&misc[kPpFailed], failed);
     if (freed) atomicAdd(&misc[kPpFreed], freed);
@@ -1175,11 +1399,21 @@ hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_
 }
 
 hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
-                     uint32_t* bits, uint32_t* bad, uint32_t cap, uint32_t* misc, hipStream_t st) {
+                     uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st) {
   if (!nold) return hipSuccess;
   const dim3 grid(blocks_for(nold)), blk(kBlockThreads);
-  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_remark<2>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, misc);
-  else hipLaunchKernelGGL(k_pp_remark<3>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, misc);
+  if (g.ndim == 2)
+    hipLaunchKernelGGL(k_pp_remark<2>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
+  else hipLaunchKernelGGL(k_pp_remark<3>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
+  return hipGetLastError();
+}
+
+hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
+                   float eb, int anchor, uint32_t* bits, uint32_t* misc, hipStream_t st) {
+  if (!nbad) return hipSuccess;
+  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
+  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_tiny<2>, grid, blk, 0, st, g, bad, nbad, d, t, y, eb, anchor, bits, misc);
+  else hipLaunchKernelGGL(k_pp_tiny<3>, grid, blk, 0, st, g, bad, nbad, d, t, y, eb, anchor, bits, misc);
   return hipGetLastError();
 }
 
@@ -1200,17 +1434,16 @@ hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t max_tiles, uint32_t cap, uint32_t ty, uint32_t tz,
-                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
-                    int anchor, uint32_t* misc, hipStream_t st) {
-  const uint32_t m = max_tiles < cap ? max_tiles : cap;
-  if (!m) return hipSuccess;
-  const dim3 grid(m), blk(kWave);
+hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
+                    uint32_t tz, const uint32_t* tbits, uint32_t* tdone, const uint32_t* pbits, const float* d,
+                    const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st) {
+  if (!total) return hipSuccess;
+  const dim3 grid(total), blk(kWave);
   if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
-                       misc);
-  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
-                          misc);
+    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, tdone, pbits, d, t, y, eb,
+                       anchor, misc);
+  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, tdone, pbits, d, t, y,
+                          eb, anchor, misc);
   return hipGetLastError();
 }
 
