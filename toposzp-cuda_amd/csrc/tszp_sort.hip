@@ -76,7 +76,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_scan_excl(uint32_t* a, uint32
 }
 
 // ---- stable scatter: rank = (scanned tile base for the digit) + (earlier items of
-//      the tile with the same digit); items are visited in index order (wave, step, lane)
+//      the tile with the same digit); items are visited in index order (wave, step,
+//      lane).  The tile is first reordered by digit in LDS, so that the global writes
+//      go out as contiguous runs of one digit instead of single scattered words.
+template <bool HI>
 __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ hin,
                                                           const uint32_t* __restrict__ pin,
@@ -86,23 +89,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
                                                           uint32_t ntiles) {
   constexpr int NS = kWavesPerBlock * kItemsPerThread;  // 64 (wave, step) groups
   __shared__ uint16_t cnt[NS][256];
-  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t gbase[256], lstart[256];
+  __shared__ uint32_t lds4[kWavesPerBlock];
+  __shared__ uint32_t skey[kSortTile], spay[kSortTile];
+  __shared__ uint32_t shi[HI ? kSortTile : 1];
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
   uint32_t* cz = reinterpret_cast<uint32_t*>(&cnt[0][0]);
   for (int i = tid; i < NS * 256 / 2; i += kBlockThreads) cz[i] = 0;
   gbase[tid] = hist[(size_t)tid * ntiles + blockIdx.x];
   __syncthreads();
 
-  const uint32_t base = blockIdx.x * kSortTile + w * (kWave * kItemsPerThread);
+  const uint32_t tile0 = blockIdx.x * kSortTile;
+  const uint32_t base = tile0 + w * (kWave * kItemsPerThread);
   const uint64_t lt = lanemask_lt(lane);
+  uint32_t kv[kItemsPerThread], pv[kItemsPerThread], hv[kItemsPerThread];
   uint32_t digit[kItemsPerThread];
   uint8_t rnk[kItemsPerThread];
 #pragma unroll
   for (int s = 0; s < kItemsPerThread; s++) {
     const uint32_t idx = base + s * kWave + lane;
     const bool valid = idx < count;
-    uint32_t d = 0;
-    if (valid) d = ((digit_from_hi ? hin[idx] : kin[idx]) >> shift) & 255u;
+    kv[s] = valid ? kin[idx] : 0u;
+    pv[s] = valid ? pin[idx] : 0u;
+    hv[s] = (HI && valid) ? hin[idx] : 0u;
+    const uint32_t d = ((digit_from_hi ? hv[s] : kv[s]) >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -116,25 +126,36 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
     rnk[s] = (uint8_t)r;
   }
   __syncthreads();
-  {  // exclusive prefix over the (wave, step) groups for each digit
-    uint32_t run = 0;
-    for (int k = 0; k < NS; k++) {
-      const uint32_t c = cnt[k][tid];
-      cnt[k][tid] = (uint16_t)run;
-      run += c;
-    }
+  uint32_t run = 0;
+  for (int k = 0; k < NS; k++) {  // exclusive prefix over the (wave, step) groups of digit tid
+    const uint32_t c = cnt[k][tid];
+    cnt[k][tid] = (uint16_t)run;
+    run += c;
   }
+  uint32_t total;
+  lstart[tid] = block_excl_scan(run, lds4, &total);  // tile-local start of digit tid
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < kItemsPerThread; s++) {
     const uint32_t idx = base + s * kWave + lane;
     if (idx >= count) continue;
     const uint32_t d = digit[s];
-    const uint32_t pos = gbase[d] + cnt[w * kItemsPerThread + s][d] + rnk[s];
+    const uint32_t loc = lstart[d] + cnt[w * kItemsPerThread + s][d] + rnk[s];
+    skey[loc] = kv[s];
+    spay[loc] = pv[s];
+    if (HI) shi[loc] = hv[s];
+  }
+  __syncthreads();
+  const uint32_t n_tile = count - tile0 < kSortTile ? count - tile0 : kSortTile;
+  for (uint32_t i = tid; i < n_tile; i += kBlockThreads) {
+    const uint32_t kk = skey[i];
+    const uint32_t hh = HI ? shi[i] : 0u;
+    const uint32_t d = ((digit_from_hi ? hh : kk) >> shift) & 255u;
+    const uint32_t pos = gbase[d] + (i - lstart[d]);
     if (pos >= count) continue;  // cannot happen for a consistent histogram
-    kout[pos] = kin[idx];
-    pout[pos] = pin[idx];
-    if (hin) hout[pos] = hin[idx];
+    kout[pos] = kk;
+    pout[pos] = spay[i];
+    if (HI) hout[pos] = hh;
   }
 }
 
@@ -154,8 +175,12 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uin
       hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlockThreads), 0, st, dw, count, shift, wk.hist, ntiles);
       hipLaunchKernelGGL(k_scan_excl, dim3(stiles), dim3(kBlockThreads), 0, st, wk.hist, hwords,
                          wk.st + (size_t)pass * stiles, wk.counters + pass, wk.err);
-      hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1, count,
-                         shift, wd, wk.hist, ntiles);
+      if (h0)
+        hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1, count,
+                           shift, wd, wk.hist, ntiles);
+      else
+        hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1,
+                           count, shift, wd, wk.hist, ntiles);
       std::swap(k0, k1);
       std::swap(h0, h1);
       std::swap(p0, p1);
