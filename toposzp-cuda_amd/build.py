@@ -43,6 +43,8 @@ def compile_one(src):
 
 def build(jobs=8, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
+    if os.path.exists(LIB):  # a failed build must not leave an older library behind
+        os.remove(LIB)
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(compile_one, SOURCES))
     for _, msgs in results:

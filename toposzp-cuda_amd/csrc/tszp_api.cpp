@@ -47,7 +47,7 @@ int fail(int code, const char* fmt, ...) {
 enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
-  kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpTDone, kWsPpMisc, kWsPpSt,
+  kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsCount
 };
 
@@ -682,24 +682,28 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
   if (use_sp) HIPCHK(pp_spmax(d_sp, (uint32_t)n_ext, misc, st));
   HIPCHK(pp_init(d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext, N, eb, d_field, status, misc, st));
   // full marking pass: fresh list of mismatches and the verification counts
-  auto full_mark = [&](uint32_t* nb_out) -> int {
+  // The list `bad` is exact when it holds every current mismatch; the verification
+  // counts in misc are current when the last full pass saw the present field.
+  bool exact = false, verified = false;
+  auto full_mark = [&](uint32_t* nb_out, bool final_pass) -> int {
     HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
     HIPCHK(launch_fill_u32(misc + kPpVerify, 0u, 8, st));
-    HIPCHK(pp_mark(g, d_field, d_types, d, N, bad, cap, misc, st));
+    HIPCHK(pp_mark(g, d_field, d_types, final_pass ? d : nullptr, N, bad, cap, misc, st));
     int rr = read_misc();
     if (rr) return rr;
     *nb_out = h[kPpBad];
+    exact = *nb_out <= cap;
+    verified = final_pass;
     return TSZP_OK;
   };
   uint32_t nb = 0;
-  if ((r = full_mark(&nb))) return r;
+  if ((r = full_mark(&nb, false))) return r;
   const bool stats = getenv("TSZP_RESTORE_STATS") != nullptr;
   if (stats) fprintf(stderr, "[restore] after init: %u mismatches\n", nb);
   if (use_sp && h[kPpExt] != n_ext)
     return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[kPpExt],
                 (unsigned long long)n_ext);
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
-  bool dirty = false;  // the field changed after the last full pass
   uint32_t tiny = 0;
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
   if (nb && nb <= cap) {
@@ -714,7 +718,8 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
     if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);
     nb = h[kPpBad];
     tiny = h[kPpTiny];
-    dirty = true;
+    exact = nb <= cap;  // only the plateau points changed, all of them listed or fixed
+    verified = false;
   }
   const int colours = ndim == 2 ? 5 : 7;
   uint64_t moves = 0, solved = 0, freed = 0;
@@ -734,7 +739,7 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
       for (int col = 0; col < colours; col++)
         HIPCHK(pp_move(g, act, nl, nl, col, d, d_types, d_field, eb, misc, st));
       rounds++;
-      dirty = true;
+      verified = false;
       // re-check around the listed mismatches, or the whole field when that is cheaper
       const uint64_t ring = ndim == 2 ? 13 : 25;
       if (complete && (uint64_t)nl * ring < n / 4) {
@@ -744,9 +749,9 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
         if ((r = read_misc())) return r;
         std::swap(bad, bad2);
         nb = h[kPpBad];
+        exact = nb <= cap;
       } else {
-        if ((r = full_mark(&nb))) return r;
-        dirty = false;
+        if ((r = full_mark(&nb, false))) return r;
       }
       complete = nb <= cap;
       moves += h[kPpMoves];
@@ -754,9 +759,8 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
       if (!h[kPpMoves]) break;
     }
     if (!nb) break;
-    if (dirty) {
-      if ((r = full_mark(&nb))) return r;
-      dirty = false;
+    if (!exact) {
+      if ((r = full_mark(&nb, false))) return r;
       if (!nb) break;
     }
     // exact window solve around what is left
@@ -774,25 +778,27 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
     HIPCHK(launch_fill_u32(misc + kPpSolved, 0u, 2, st));
     HIPCHK(launch_fill_u32(misc + kPpFreed, 0u, 1, st));
     HIPCHK(launch_fill_u32(misc + kPpTilesP, 0u, 8, st));
-    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 2, st));
+    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 9, st));
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
-    for (int ph = 0; ph < phases; ph++) total += h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
-    uint32_t* tdone;
-    WS(kWsPpTDone, ntiles * 4 + 16, tdone);
-    HIPCHK(launch_fill_u32(tdone, 0u, ntiles, st));
-    HIPCHK(launch_fill_u32(misc + kPpSolveCounter, 0u, 1, st));
-    HIPCHK(pp_solve(g, tiles, total, tcap, tx, ty, tz, tbits, tdone, bits, d, d_types, d_field, eb, use_sp, misc, st));
-    if ((r = full_mark(&nb))) return r;
+    for (int ph = 0; ph < phases; ph++) {
+      const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+      total += cnt;
+      HIPCHK(pp_solve(g, tiles, cnt, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+    }
+    if ((r = full_mark(&nb, true))) return r;
     if (stats)
-      fprintf(stderr, "[restore] outer %d: listed %u solved %u failed %u freed %u nodes %u max nodes/tile %u left %u\n",
-              outer, nl, h[kPpSolved], h[kPpFailed], h[kPpFreed], h[kPpStatNodes], h[kPpStatMaxNodes], nb);
+      fprintf(stderr, "[restore] outer %d: listed %u tiles %u solved %u failed %u freed %u nodes %u max nodes/tile %u "
+              "max windows/tile %u max tile us %.1f leaves %u verify fails %u left %u\n",
+              outer, nl, total, h[kPpSolved], h[kPpFailed], h[kPpFreed], h[kPpStatNodes], h[kPpStatMaxNodes],
+              h[kPpStatMaxWin], h[kPpStatMaxTicks] * 0.01, h[kPpStatLeaves],
+              h[kPpStatLeaves + 1], nb);
     solved += h[kPpSolved];
     freed += h[kPpFreed];
     if (!h[kPpSolved] && outer > 0) break;
   }
-  if (dirty && (r = full_mark(&nb))) return r;
+  if (!verified && (r = full_mark(&nb, true))) return r;
   prof.stop();
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
   const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);

@@ -821,7 +821,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(Grid g, const uint32
     wave_append(take && ph == p, id, tiles + (size_t)p * cap, cap, &misc[kPpTilesP + p]);
 }
 
-// Search state of one window (LDS, one per wavefront).  Elements: the window's plateau
+// Window of one mismatch (LDS, one per wavefront).  Elements: the window's plateau
 // points, then the plateau points next to them.  Every interior element is a
 // constraint; an attempt frees some window points (every other element keeps its
 // value) and checks the constraints that read a free point.
@@ -835,20 +835,11 @@ struct Window {
   int8_t cnb[EM][K];  // neighbour codes of an interior element: element index or kGt / kLt / kEq
   int8_t typ[EM];     // target type, -1 for a boundary element
   int8_t anch[EM];    // anchored extremum
-  int8_t cls[EM];
-  int8_t fidx[EM];  // free index or -1
-  int8_t felem[kWinMax];
-  int8_t cself[EM];  // constraints grouped by the depth at which they are checked
-  int8_t cstart[kWinMax + 1];
-  int8_t choice[kWinMax];
-  int8_t newcls[kWinMax];
   int8_t picks[kWinMax];
-  float cv[EM + kWinMax];
-  int8_t canchor[EM + kWinMax];
-  float oldv[kWinMax];
-  float flo[kWinMax], fhi[kWinMax];  // open bounds of the free points' values
+  float cv[EM];      // realisation: value of each class
+  int8_t canchor[EM];
   uint32_t cand[kWave];
-  int ne, nw, nf, nc, ncls, depth, nodes, budget, npick, found;
+  int ne, nw, npick;
   float lo, hi, db;
 };
 
@@ -857,65 +848,6 @@ __device__ __forceinline__ int find_el(const Window<ND>& w, uint32_t e) {
   for (int i = 0; i < w.ne; i++)
     if (w.el[i] == e) return i;
   return -1;
-}
-
-template <int ND>
-__device__ __forceinline__ bool rel_ok(const Window<ND>& w, int ci) {
-  const int s = w.cself[ci];
-  const int cs = w.cls[s];
-  bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
-#pragma unroll
-  for (int a = 0; a < ND; a++) {
-    int r[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int code = w.cnb[s][2 * a + h];
-      if (code == Window<ND>::kGt) r[h] = 1;
-      else if (code == Window<ND>::kLt) r[h] = -1;
-      else if (code == Window<ND>::kEq) r[h] = 0;
-      else r[h] = cs > w.cls[code] ? 1 : (cs < w.cls[code] ? -1 : 0);
-    }
-    const bool hi = r[0] > 0 && r[1] > 0, lo = r[0] < 0 && r[1] < 0;
-    all_gt = all_gt && hi;
-    all_lt = all_lt && lo;
-    any_hi = any_hi || hi;
-    any_lo = any_lo || lo;
-  }
-  const int k = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
-  return k == w.typ[s];
-}
-
-// values of the classes (classes holding a fixed element keep its value, runs of free
-// classes are spaced evenly between their fixed neighbours); false when floats run
-// out or a free point leaves its bounds
-template <int ND>
-__device__ bool realize(Window<ND>& w) {
-  for (int k = 0; k < w.ncls; k++) w.canchor[k] = 0;
-  for (int i = 0; i < w.ne; i++)
-    if (w.fidx[i] < 0) {
-      w.cv[w.cls[i]] = w.val[i];
-      w.canchor[w.cls[i]] = 1;
-    }
-  int k = 0;
-  while (k < w.ncls) {
-    if (w.canchor[k]) {
-      k++;
-      continue;
-    }
-    int r = 0;
-    while (k + r < w.ncls && !w.canchor[k + r]) r++;
-    const float L = k > 0 ? w.cv[k - 1] : w.lo;
-    const float U = k + r < w.ncls ? w.cv[k + r] : w.hi;
-    for (int j = 0; j < r; j++) w.cv[k + j] = L + (U - L) * ((float)(j + 1) / (float)(r + 1));
-    k += r;
-  }
-  for (int j = 0; j + 1 < w.ncls; j++)
-    if (!(w.cv[j] < w.cv[j + 1])) return false;
-  for (int f = 0; f < w.nf; f++) {
-    const float v = w.cv[w.cls[w.felem[f]]];
-    if (!(v > w.flo[f] && v < w.fhi[f])) return false;
-  }
-  return true;
 }
 
 // neighbour k of element point e, if it lies in the plateau of value db
@@ -1008,160 +940,209 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
   __syncthreads();
 }
 
-// Sets up an attempt (lane 0).  mode 0: the anchored extrema keep their values; 1:
+// One attempt (whole wavefront).  mode 0: the anchored extrema keep their values; 1:
 // window element `pick` (an anchored extremum) moves too, within (v - delta, v + delta),
 // which keeps it between the extrema of the neighbouring sort positions; 2: `pick`
-// moves freely; 3: every window point moves.  Returns false when nothing is free.
-template <int ND>
-__device__ bool setup_attempt(Window<ND>& w, int mode, int pick, float delta) {
-  w.nf = 0;
-  for (int i = 0; i < w.ne; i++) {
-    const bool fr = i < w.nw && (!w.anch[i] || mode == 3 || i == pick);
-    w.fidx[i] = fr ? (int8_t)w.nf : (int8_t)-1;
-    w.cls[i] = -1;
-    if (fr) {
-      w.flo[w.nf] = -INFINITY;
-      w.fhi[w.nf] = INFINITY;
-      if (mode == 1 && i == pick) {
-        w.flo[w.nf] = w.val[i] - delta;
-        w.fhi[w.nf] = w.val[i] + delta;
-      }
-      w.felem[w.nf++] = (int8_t)i;
-    }
-  }
-  if (w.nf == 0) return false;
-  // classes of the fixed elements: distinct values in increasing order (insertion sort
-  // of their indices in cand[])
-  int nfix = 0;
-  for (int i = 0; i < w.ne; i++) {
-    if (w.fidx[i] >= 0) continue;
-    int j = nfix++;
-    while (j > 0 && w.val[w.cand[j - 1]] > w.val[i]) {
-      w.cand[j] = w.cand[j - 1];
-      j--;
-    }
-    w.cand[j] = (uint32_t)i;
-  }
-  w.ncls = 0;
-  for (int j = 0; j < nfix; j++) {
-    if (j > 0 && w.val[w.cand[j]] != w.val[w.cand[j - 1]]) w.ncls++;
-    w.cls[w.cand[j]] = (int8_t)w.ncls;
-  }
-  if (nfix) w.ncls++;
-  // constraints that read a free point, grouped by the depth of their last free point
-  w.nc = 0;
-  for (int f = 0; f < w.nf; f++) {
-    w.cstart[f] = (int8_t)w.nc;
-    for (int i = 0; i < w.ne; i++) {
-      if (w.typ[i] < 0) continue;
-      int dep = w.fidx[i];
-      for (int k = 0; k < 2 * ND; k++) {
-        const int code = w.cnb[i][k];
-        if (code < Window<ND>::kGt && w.fidx[code] > dep) dep = w.fidx[code];
-      }
-      if (dep == f) w.cself[w.nc++] = (int8_t)i;
-    }
-  }
-  w.cstart[w.nf] = (int8_t)w.nc;
-  w.depth = 0;
-  w.choice[0] = -1;
-  w.nodes = 0;
-  w.budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
-  return true;
-}
-
-// Depth-first search over the weak orders of the free points (lane 0), resumed where
-// the previous call stopped.  Returns true with the values of the next complete order
-// in cv[], false when the orders or the budget are exhausted.
-template <int ND>
-__device__ bool next_order(Window<ND>& w) {
-  int depth = w.depth, nodes = w.nodes;
-  bool found = false;
-  while (depth >= 0) {
-    const int fe = w.felem[depth];
-    if (w.choice[depth] >= 0) {  // undo the previous choice at this depth
-      const int pos = w.cls[fe];
-      w.cls[fe] = -1;
-      if (w.newcls[depth]) {
-        for (int i = 0; i < w.ne; i++)
-          if (w.cls[i] > pos) w.cls[i]--;
-        w.ncls--;
-      }
-    }
-    const int ch = ++w.choice[depth];
-    if (ch > 2 * w.ncls) {
-      w.choice[depth] = -1;
-      depth--;
-      continue;
-    }
-    if (++nodes > w.budget) {
-      depth = -1;
-      break;
-    }
-    const int pos = ch >> 1;
-    if ((ch & 1) == 0) {  // a new class at position pos
-      for (int i = 0; i < w.ne; i++)
-        if (w.cls[i] >= pos) w.cls[i]++;
-      w.cls[fe] = (int8_t)pos;
-      w.ncls++;
-      w.newcls[depth] = 1;
-    } else {  // join class pos
-      w.cls[fe] = (int8_t)pos;
-      w.newcls[depth] = 0;
-    }
-    bool ok = true;
-    for (int ci = w.cstart[depth]; ci < w.cstart[depth + 1] && ok; ci++) ok = rel_ok<ND>(w, ci);
-    if (!ok) continue;
-    if (depth + 1 < w.nf) {
-      depth++;
-      w.choice[depth] = -1;
-      continue;
-    }
-    if (realize<ND>(w)) {
-      found = true;
-      break;
-    }
-  }
-  w.depth = depth;
-  w.nodes = nodes;
-  return found;
-}
-
-// One attempt (whole wavefront): search orders on lane 0, write each found order to
-// the field and check its constraints in parallel; keep the first that holds.
+// moves freely; 3: every window point moves.
+//
+// Depth-first search over the weak orders of the free points: lane i holds the class
+// (position in the order) of element i, so inserting a class is one vector compare-
+// and-increment, and the constraints that become decidable at a depth are checked by
+// their own lanes with K lane shuffles.  A complete order becomes values (classes of
+// fixed elements keep their value, runs of free classes are spaced evenly between
+// them), is written to the field and verified there by the constraint lanes; the first
+// order that holds is kept.  All control flow is uniform over the wavefront.
 template <int ND>
 __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* __restrict__ y,
-                            int mode, int pick, float delta, int lane, uint32_t* nodes_out) {
-  if (lane == 0) w.found = setup_attempt<ND>(w, mode, pick, delta) ? 1 : 0;
-  __syncthreads();
-  if (!w.found) return false;
+                            int mode, int pick, float delta, int lane, uint32_t* nodes_out, uint32_t* stats) {
+  constexpr int K = 2 * ND;
+  const int ne = w.ne, nw = w.nw;
+  const bool is_el = lane < ne;
+  const float my_val = is_el ? w.val[lane] : 0.f;
+  const bool my_free = is_el && lane < nw && (!w.anch[lane] || mode == 3 || lane == pick);
+  const uint64_t fmask = __ballot(my_free);
+  const int nf = __popcll(fmask);
+  if (nf == 0) return false;
+  const int my_fidx = my_free ? __popcll(fmask & lanemask_lt(lane)) : -1;
+  int felem_reg = 0;  // lane f: element index of free point f
+  {
+    uint64_t m = fmask;
+    for (int f = 0; m; f++) {
+      const int i = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      if (lane == f) felem_reg = i;
+    }
+  }
+  // classes of the fixed elements: rank among their distinct values
+  const bool my_fixed = is_el && !my_free;
+  const uint64_t xmask = __ballot(my_fixed);
+  bool my_first = my_fixed;
+  for (int j = 0; j < ne; j++) {
+    const float vj = __shfl(my_val, j, kWave);
+    if (((xmask >> j) & 1ull) && j < lane && vj == my_val) my_first = false;
+  }
+  const uint64_t first_mask = __ballot(my_first);
+  int my_cls = -1;
+  {
+    int rank = 0;
+    for (int j = 0; j < ne; j++) {
+      const float vj = __shfl(my_val, j, kWave);
+      if (((first_mask >> j) & 1ull) && vj < my_val) rank++;
+    }
+    if (my_fixed) my_cls = rank;
+  }
+  int ncls = __popcll(first_mask);
+  const float my_flo = (mode == 1 && lane == pick) ? my_val - delta : -INFINITY;
+  const float my_fhi = (mode == 1 && lane == pick) ? my_val + delta : INFINITY;
+  // constraints: every interior element, checked at the depth of its last free point
+  const bool my_con = is_el && w.typ[lane] >= 0;
+  const int my_typ = my_con ? w.typ[lane] : 0;
+  int code[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) code[k] = my_con ? w.cnb[lane][k] : Window<ND>::kGt;
+  int my_dep = my_con ? my_fidx : -1;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int src = code[k] < Window<ND>::kGt ? code[k] : lane;
+    const int fj = __shfl(my_fidx, src, kWave);
+    if (my_con && code[k] < Window<ND>::kGt && fj > my_dep) my_dep = fj;
+  }
+  const int budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
+  int depth = 0, nodes = 0;
+  uint32_t leaves = 0, vfails = 0;
+  int choice_reg = -1, newcls_reg = 0;  // lane d: choice and new-class flag of depth d
+  bool found = false;
+  float my_fval = 0.f, my_old = 0.f;
   while (true) {
-    if (lane == 0) w.found = next_order<ND>(w) ? 1 : 0;
-    __syncthreads();
-    if (!w.found) break;
-    const int nf = w.nf;
-    if (lane < nf) {
-      const uint32_t e = w.el[w.felem[lane]];
-      w.oldv[lane] = y[e];
-      y[e] = w.cv[w.cls[w.felem[lane]]];
+    bool leaf = false;
+    while (depth >= 0) {
+      const int fe = __shfl(felem_reg, depth, kWave);
+      int ch = __shfl(choice_reg, depth, kWave);
+      if (ch >= 0) {  // undo the previous choice at this depth
+        const int pos = __shfl(my_cls, fe, kWave);
+        const int nc = __shfl(newcls_reg, depth, kWave);
+        if (lane == fe) my_cls = -1;
+        if (nc) {
+          if (my_cls > pos) my_cls--;
+          ncls--;
+        }
+      }
+      ch++;
+      if (lane == depth) choice_reg = ch;
+      if (ch > 2 * ncls) {
+        if (lane == depth) choice_reg = -1;
+        depth--;
+        continue;
+      }
+      if (++nodes > budget) {
+        depth = -1;
+        break;
+      }
+      const int pos = ch >> 1;
+      if ((ch & 1) == 0) {  // a new class at position pos
+        if (my_cls >= pos) my_cls++;
+        if (lane == fe) my_cls = pos;
+        ncls++;
+        if (lane == depth) newcls_reg = 1;
+      } else {  // join class pos
+        if (lane == fe) my_cls = pos;
+        if (lane == depth) newcls_reg = 0;
+      }
+      bool okl = true;
+      {
+        int r[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int src = code[k] < Window<ND>::kGt ? code[k] : lane;
+          const int oc = __shfl(my_cls, src, kWave);
+          r[k] = code[k] == Window<ND>::kGt ? 1
+                 : code[k] == Window<ND>::kLt ? -1
+                 : code[k] == Window<ND>::kEq ? 0
+                 : (my_cls > oc ? 1 : (my_cls < oc ? -1 : 0));
+        }
+        bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+          const bool hi = r[2 * a] > 0 && r[2 * a + 1] > 0, lo = r[2 * a] < 0 && r[2 * a + 1] < 0;
+          all_gt = all_gt && hi;
+          all_lt = all_lt && lo;
+          any_hi = any_hi || hi;
+          any_lo = any_lo || lo;
+        }
+        const int kk = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
+        if (my_dep == depth) okl = kk == my_typ;
+      }
+      if (__any(!okl)) continue;
+      if (depth + 1 < nf) {
+        depth++;
+        if (lane == depth) choice_reg = -1;
+        continue;
+      }
+      // a complete order: values of the classes.  Lane k holds class k; classes with a
+      // fixed element keep its value, the others are spaced evenly between the nearest
+      // fixed classes (or the interval ends).
+      leaves++;
+      if (lane < ncls) w.canchor[lane] = 0;
+      __syncthreads();
+      if (my_fixed) {
+        w.cv[my_cls] = my_val;
+        w.canchor[my_cls] = 1;
+      }
+      __syncthreads();
+      const bool is_cls = lane < ncls;
+      const bool anc_k = is_cls && w.canchor[lane];
+      float v_k = anc_k ? w.cv[lane] : 0.f;
+      const uint64_t am = __ballot(anc_k);
+      const uint64_t below = am & lanemask_lt(lane);
+      const uint64_t above = lane < kWave - 1 ? am & (~0ull << (lane + 1)) : 0ull;
+      const int pa = below ? 63 - __clzll((long long)below) : -1;
+      const int na = above ? __ffsll((unsigned long long)above) - 1 : ncls;
+      const float vpa = __shfl(v_k, pa >= 0 ? pa : 0, kWave);
+      const float vna = __shfl(v_k, na < ncls ? na : 0, kWave);
+      if (is_cls && !anc_k) {
+        const float L = pa >= 0 ? vpa : w.lo;
+        const float U = na < ncls ? vna : w.hi;
+        v_k = L + (U - L) * ((float)(lane - pa) / (float)(na - pa));
+      }
+      const float v_next = __shfl(v_k, lane + 1 < kWave ? lane + 1 : lane, kWave);
+      bool vok = !(is_cls && lane + 1 < ncls) || v_k < v_next;
+      my_fval = __shfl(v_k, my_cls >= 0 ? my_cls : 0, kWave);
+      if (my_free) vok = vok && my_fval > my_flo && my_fval < my_fhi;
+      __syncthreads();
+      if (!__any(!vok)) {
+        leaf = true;
+        break;
+      }
+    }
+    if (!leaf) break;
+    // write the free points and check every constraint on the field
+    if (my_free) {
+      my_old = y[w.el[lane]];
+      y[w.el[lane]] = my_fval;
     }
     __syncthreads();
     bool good = true;
-    for (int ci = lane; ci < w.nc; ci += kWave) {
-      const uint32_t e = w.el[w.cself[ci]];
+    if (my_dep >= 0) {
+      const uint32_t e = w.el[lane];
       int c[ND];
       coords_of<ND>(g, e, c);
-      good = good && classify_at<ND>(g, y, e, c) == (int)t[e];
+      good = classify_at<ND>(g, y, e, c) == (int)t[e];
     }
-    const bool all_good = !__any(!good);
-    if (all_good) break;
-    if (lane < nf) y[w.el[w.felem[lane]]] = w.oldv[lane];
+    if (!__any(!good)) {
+      found = true;
+      break;
+    }
+    if (my_free) y[w.el[lane]] = my_old;
+    vfails++;
     __syncthreads();
   }
-  if (lane == 0) *nodes_out += (uint32_t)w.nodes;
-  const bool ok = w.found != 0;
+  if (lane == 0) {
+    *nodes_out += (uint32_t)nodes;
+    atomicAdd(&stats[0], leaves);
+    atomicAdd(&stats[1], vfails);
+  }
   __syncthreads();
-  return ok;
+  return found;
 }
 
 // Solve the window of mismatch b; returns the mode that succeeded or -1.  An extremum
@@ -1172,9 +1153,9 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
 template <int ND>
 __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
                             float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
-                            uint32_t* nodes) {
+                            uint32_t* nodes, uint32_t* stats) {
   build_window<ND>(w, g, d, t, y, eb, b, anchor, lane);
-  if (run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes)) return 0;
+  if (run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats)) return 0;
   if (lane == 0) {
     // anchored window points, the most extreme value first
     const float mid = 0.5f * (w.lo + w.hi);
@@ -1194,22 +1175,19 @@ __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restric
   if (na == 0) return -1;
   const float delta = delta_frac * (w.hi - w.lo);
   for (int r = 0; r < na; r++)
-    if (run_attempt<ND>(w, g, t, y, 1, w.picks[r], delta, lane, nodes)) return 1;
+    if (run_attempt<ND>(w, g, t, y, 1, w.picks[r], delta, lane, nodes, stats)) return 1;
   for (int r = 0; r < na; r++)
-    if (run_attempt<ND>(w, g, t, y, 2, w.picks[r], 0.f, lane, nodes)) return 2;
-  if (na > 1 && run_attempt<ND>(w, g, t, y, 3, -1, 0.f, lane, nodes)) return 3;
+    if (run_attempt<ND>(w, g, t, y, 2, w.picks[r], 0.f, lane, nodes, stats)) return 2;
+  if (na > 1 && run_attempt<ND>(w, g, t, y, 3, -1, 0.f, lane, nodes, stats)) return 3;
   return -1;
 }
 
-// One wavefront per listed tile, all parity phases in one launch.  Tiles are taken in
-// phase order from a counter; a tile first waits until every listed neighbour tile of
-// a lower phase is done, which reproduces the phase-by-phase order (tiles of one
-// phase never share a window) while the phases overlap.  Inside a tile the listed
-// mismatches are solved in flat order.
+// One wavefront per listed tile of one parity phase (a launch per phase: tiles of one
+// phase never share a window, and the launch boundary makes one phase's values visible
+// to the next).  Inside a tile the listed mismatches are solved in flat order.
 template <int ND>
 __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                    uint32_t tx, uint32_t ty, uint32_t tz,
-                                                    const uint32_t* __restrict__ tbits, uint32_t* tdone,
+                                                    uint32_t ty, uint32_t tz, int phase,
                                                     const uint32_t* __restrict__ pbits, const float* __restrict__ d,
                                                     const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
                                                     int anchor, uint32_t* misc) {
@@ -1217,21 +1195,10 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
   constexpr int kTilePts = ND == 2 ? kT * kT : kT * kT * kT;
   __shared__ Window<ND> w;
   __shared__ uint32_t blist[kTilePts];
-  __shared__ uint32_t sh_nb, sh_item;
+  __shared__ uint32_t sh_nb;
   const int lane = threadIdx.x;
-  if (lane == 0) sh_item = atomicAdd(&misc[kPpSolveCounter], 1u);
-  __syncthreads();
-  uint32_t item = sh_item;
-  int phase = 0;
-  for (; phase < (1 << ND); phase++) {
-    uint32_t c = misc[kPpTilesP + phase];
-    c = c < cap ? c : cap;
-    if (item < c) break;
-    item -= c;
-  }
-  if (phase >= (1 << ND)) return;
-  const uint32_t tid_ = tiles[(size_t)phase * cap + item];
-  uint32_t id = tid_;
+  const uint64_t t_work = wall_clock64();
+  uint32_t id = tiles[(size_t)phase * cap + blockIdx.x];
   int tc[ND];
   if constexpr (ND == 2) {
     tc[1] = (int)(id % ty);
@@ -1241,41 +1208,6 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     id /= tz;
     tc[1] = (int)(id % ty);
     tc[0] = (int)(id / ty);
-  }
-  // wait for the listed neighbour tiles of lower phases
-  {
-    constexpr int kNbr = ND == 2 ? 9 : 27;
-    bool ok = true;
-    if (lane < kNbr) {
-      int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
-      int nc[ND];
-      bool valid = lane != kNbr / 2;
-      int nph = 0;
-#pragma unroll
-      for (int a = 0; a < ND; a++) {
-        nc[a] = tc[a] + o[a];
-        const int lim = a == 0 ? (int)tx : (a == 1 ? (int)ty : (int)tz);
-        valid = valid && nc[a] >= 0 && nc[a] < lim;
-        nph |= (nc[a] & 1) << a;
-      }
-      if (valid && nph < phase) {
-        const uint32_t nid = ND == 2 ? (uint32_t)nc[0] * ty + (uint32_t)nc[1]
-                                     : ((uint32_t)nc[0] * ty + (uint32_t)nc[1]) * tz + (uint32_t)nc[2];
-        if ((tbits[nid >> 5] >> (nid & 31u)) & 1u) {
-          uint32_t spins = 0;
-          while (__hip_atomic_load(&tdone[nid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-            if (++spins > kSpinLimit) {
-              ok = false;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-          }
-        }
-      }
-    }
-    if (!ok) atomicOr(&misc[kPpErr], TSZP_ERR_LOOKBACK);
-    __threadfence();  // acquire: see the neighbours' values
-    __syncthreads();
   }
   const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
   const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
@@ -1329,20 +1261,20 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     int c[ND];
     coords_of<ND>(g, e, c);
     if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
-    const int mode = solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes);
+    const int mode =
+        solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, misc + kPpStatLeaves);
     solved += mode >= 0;
     failed += mode < 0;
     freed += mode >= 2;
   }
-  __threadfence();  // release: the tile's values before its done flag
-  __syncthreads();
   if (lane == 0) {
-    __hip_atomic_store(&tdone[tid_], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (solved) atomicAdd(&misc[kPpSolved], solved);
     if (failed) atomicAdd(&misc[kPpFailed], failed);
     if (freed) atomicAdd(&misc[kPpFreed], freed);
     atomicAdd(&misc[kPpStatNodes], nodes);
     atomicMax(&misc[kPpStatMaxNodes], nodes);
+    atomicMax(&misc[kPpStatMaxWin], solved + failed);
+    atomicMax(&misc[kPpStatMaxTicks], (uint32_t)(wall_clock64() - t_work));
   }
 }
 
@@ -1434,16 +1366,16 @@ hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
-                    uint32_t tz, const uint32_t* tbits, uint32_t* tdone, const uint32_t* pbits, const float* d,
-                    const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st) {
-  if (!total) return hipSuccess;
-  const dim3 grid(total), blk(kWave);
+hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, uint32_t ty, uint32_t tz,
+                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
+                    int anchor, uint32_t* misc, hipStream_t st) {
+  if (!ntiles) return hipSuccess;
+  const dim3 grid(ntiles), blk(kWave);
   if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, tdone, pbits, d, t, y, eb,
-                       anchor, misc);
-  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, tdone, pbits, d, t, y,
-                          eb, anchor, misc);
+    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
+                       misc);
+  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
+                          misc);
   return hipGetLastError();
 }
 
