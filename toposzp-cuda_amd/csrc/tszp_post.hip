@@ -1010,15 +1010,16 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
   const int budget = mode == 0 ? kSolveBudget : kSolveBudget / 4;
   int depth = 0, nodes = 0;
   uint32_t leaves = 0, vfails = 0;
-  int choice_reg = -1, newcls_reg = 0;  // lane d: choice and new-class flag of depth d
+  // lane d: iteration count, new-class flag and starting choice of depth d
+  int choice_reg = -1, newcls_reg = 0, home_reg = 0;
   bool found = false;
   float my_fval = 0.f, my_old = 0.f;
   while (true) {
     bool leaf = false;
     while (depth >= 0) {
       const int fe = __shfl(felem_reg, depth, kWave);
-      int ch = __shfl(choice_reg, depth, kWave);
-      if (ch >= 0) {  // undo the previous choice at this depth
+      int it = __shfl(choice_reg, depth, kWave);
+      if (it >= 0) {  // undo the previous choice at this depth
         const int pos = __shfl(my_cls, fe, kWave);
         const int nc = __shfl(newcls_reg, depth, kWave);
         if (lane == fe) my_cls = -1;
@@ -1026,10 +1027,24 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
           if (my_cls > pos) my_cls--;
           ncls--;
         }
+      } else {
+        // starting choice: the class of an equal current value, else a new class just
+        // above every placed element of smaller current value
+        const float vf = __shfl(my_val, fe, kWave);
+        int above = my_cls >= 0 && my_val < vf ? my_cls + 1 : 0;
+        int eqc = my_cls >= 0 && my_val == vf ? my_cls : -1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          above = max(above, __shfl_xor(above, o, kWave));
+          eqc = max(eqc, __shfl_xor(eqc, o, kWave));
+        }
+        const int hm = eqc >= 0 ? 2 * eqc + 1 : 2 * above;
+        if (lane == depth) home_reg = hm;
       }
-      ch++;
-      if (lane == depth) choice_reg = ch;
-      if (ch > 2 * ncls) {
+      it++;
+      if (lane == depth) choice_reg = it;
+      const int maxch = 2 * ncls;
+      if (it > maxch) {
         if (lane == depth) choice_reg = -1;
         depth--;
         continue;
@@ -1037,6 +1052,14 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
       if (++nodes > budget) {
         depth = -1;
         break;
+      }
+      // choices in the order home, home + 1, home - 1, home + 2, ... over [0, 2 ncls]
+      int ch;
+      {
+        const int home = __shfl(home_reg, depth, kWave);
+        const int nlo = home, nhi = maxch - home, mm = min(nlo, nhi);
+        if (it <= 2 * mm) ch = (it & 1) ? home + ((it + 1) >> 1) : home - (it >> 1);
+        else ch = nhi > nlo ? home + (it - mm) : home - (it - mm);
       }
       const int pos = ch >> 1;
       if ((ch & 1) == 0) {  // a new class at position pos
@@ -1048,29 +1071,55 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
         if (lane == fe) my_cls = pos;
         if (lane == depth) newcls_reg = 0;
       }
+      // every placed constraint: fully decided ones must give their type; partly
+      // decided maxima, minima and saddles are refuted as soon as possible
       bool okl = true;
       {
         int r[K];
+        bool kn[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
           const int src = code[k] < Window<ND>::kGt ? code[k] : lane;
           const int oc = __shfl(my_cls, src, kWave);
+          kn[k] = code[k] >= Window<ND>::kGt || oc >= 0;
           r[k] = code[k] == Window<ND>::kGt ? 1
                  : code[k] == Window<ND>::kLt ? -1
                  : code[k] == Window<ND>::kEq ? 0
                  : (my_cls > oc ? 1 : (my_cls < oc ? -1 : 0));
         }
-        bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+        if (my_con && my_cls >= 0) {
+          bool all_kn = true, all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+          bool can_hi[ND], can_lo[ND];
 #pragma unroll
-        for (int a = 0; a < ND; a++) {
-          const bool hi = r[2 * a] > 0 && r[2 * a + 1] > 0, lo = r[2 * a] < 0 && r[2 * a + 1] < 0;
-          all_gt = all_gt && hi;
-          all_lt = all_lt && lo;
-          any_hi = any_hi || hi;
-          any_lo = any_lo || lo;
+          for (int a = 0; a < ND; a++) {
+            const int r0 = r[2 * a], r1 = r[2 * a + 1];
+            const bool k0 = kn[2 * a], k1 = kn[2 * a + 1];
+            all_kn = all_kn && k0 && k1;
+            const bool hi = r0 > 0 && r1 > 0, lo = r0 < 0 && r1 < 0;
+            all_gt = all_gt && hi;
+            all_lt = all_lt && lo;
+            any_hi = any_hi || hi;
+            any_lo = any_lo || lo;
+            can_hi[a] = (!k0 || r0 > 0) && (!k1 || r1 > 0);
+            can_lo[a] = (!k0 || r0 < 0) && (!k1 || r1 < 0);
+          }
+          if (all_kn) {
+            const int kk = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
+            okl = kk == my_typ;
+          } else if (my_typ == 1 || my_typ == 2) {
+            bool cand = true;
+#pragma unroll
+            for (int a = 0; a < ND; a++) cand = cand && (my_typ == 1 ? can_hi[a] : can_lo[a]);
+            okl = cand;
+          } else if (my_typ == 3) {
+            bool cand = false;
+#pragma unroll
+            for (int a = 0; a < ND; a++)
+#pragma unroll
+              for (int b2 = 0; b2 < ND; b2++) cand = cand || (a != b2 && can_hi[a] && can_lo[b2]);
+            okl = cand;
+          }
         }
-        const int kk = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
-        if (my_dep == depth) okl = kk == my_typ;
       }
       if (__any(!okl)) continue;
       if (depth + 1 < nf) {
