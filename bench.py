@@ -124,13 +124,14 @@ def main():
     buf = tz.TopoBuffers(n, bs, 1, dev)
 
     last = {}
+    restored = torch.empty(n, dtype=torch.float32, device=dev)
 
     def step():
         info, _ = tz.topo_compress(f, eb, bs, T=1, buf=buf)
         out, types = tz.topo_decompress(buf, n, eb, bs, T=1)
         sp = tz.sortpos_decompress(buf, info.ext_count, bs, T=1) if info.ext_count else None
         if not args.no_restore:
-            last["restore"] = tz.topo_restore(out, types, shape, eb, sp=sp, n_ext=info.ext_count)
+            last["restore"] = tz.topo_restore(out, types, shape, eb, sp=sp, n_ext=info.ext_count, out=restored)
         return info
 
     for i in range(args.warmup):

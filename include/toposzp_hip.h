@@ -88,7 +88,8 @@ int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t 
 
 /* Topology restoration (no counterpart in the reference library: its post-process is
  * host code in example/test_cuda_topology.cu:118-326 and test_cuda_topology_3d.cu:77-434).
- * In place on d_field, which holds the decoded (float)q * eb on entry.  Every point
+ * d_decoded holds the decoded (float)q * eb; the result goes to d_field, which may be
+ * the same buffer (then a copy of the decoded values is kept internally).  Every point
  * stays in the interval of its cell -- [d, d + 0.9 eb] for d > 0, [d - 0.9 eb, d] for
  * d < 0 (error < eb), [-eb/1024, eb/1024] for d == 0 (error < eb (1 + 1/1024)) -- while
  * a colour-class local search and an exact solve of small plateau windows make the
@@ -109,8 +110,9 @@ typedef struct {
   float max_dev_over_eb;   /* max |y - d| / eb */
 } tszp_restore_info;
 
-int tszp_topo_restore(float *d_field, const uint8_t *d_types, const int32_t *d_sp, uint64_t n_extrema, int ndim,
-                      const uint32_t *dims, float eb, int max_rounds, tszp_restore_info *info, void *stream);
+int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_types, const int32_t *d_sp,
+                      uint64_t n_extrema, int ndim, const uint32_t *dims, float eb, int max_rounds,
+                      tszp_restore_info *info, void *stream);
 
 /* Critical-point mismatches of a field against reference types:
  * counts[0] missing, counts[1] spurious, counts[2] changed type. */

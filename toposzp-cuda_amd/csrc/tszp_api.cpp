@@ -638,10 +638,12 @@ int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t*
   return TSZP_OK;
 }
 
-int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_sp, uint64_t n_ext, int ndim,
-                      const uint32_t* dims, float eb, int max_rounds, tszp_restore_info* info, void* stream) {
+int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_types, const int32_t* d_sp,
+                      uint64_t n_ext, int ndim, const uint32_t* dims, float eb, int max_rounds,
+                      tszp_restore_info* info, void* stream) {
   g_err.clear();
-  if (!d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0) || max_rounds < 0)
+  if (!d_decoded || !d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0) ||
+      max_rounds < 0)
     return fail(TSZP_EINVAL, "topo_restore: bad arguments");
   if (reinterpret_cast<uintptr_t>(d_types) & 3u) return fail(TSZP_EINVAL, "topo_restore: types must be 4-byte aligned");
   size_t n = 1;
@@ -655,10 +657,11 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
   hipStream_t st = (hipStream_t)stream;
   const Grid g = make_grid(ndim, dims);
   const uint32_t N = (uint32_t)n;
-  float* d;
+  const float* d = d_decoded;  // the decoded values define every point's interval
   uint32_t *bad, *bad2, *act, *bits, *misc;
   uint64_t* status;
-  WS(kWsPpD, n * 4, d);
+  float* dcopy = nullptr;
+  if (d_decoded == d_field) WS(kWsPpD, n * 4, dcopy);  // in place: keep a copy
   // mismatch lists: a quarter of the points (more are listed on the next full pass)
   const uint32_t cap = N / 4 + 4096;
   WS(kWsPpBad, (size_t)cap * 4, bad);
@@ -675,7 +678,10 @@ int tszp_topo_restore(float* d_field, const uint8_t* d_types, const int32_t* d_s
     return TSZP_OK;
   };
   ProfScope prof(kProfRestore, st);
-  HIPCHK(hipMemcpyAsync(d, d_field, n * 4, hipMemcpyDeviceToDevice, st));
+  if (dcopy) {
+    HIPCHK(hipMemcpyAsync(dcopy, d_decoded, n * 4, hipMemcpyDeviceToDevice, st));
+    d = dcopy;
+  }
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
   HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, (size_t)itiles * 2, st));
   const bool use_sp = d_sp != nullptr && n_ext > 0;

@@ -374,6 +374,121 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark(Grid g, const float* 
   }
 }
 
+// Same pass, 4 consecutive points per thread along the fastest axis (its length a
+// multiple of 4, buffers 16-byte aligned): one float4 per row of the stencil.
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float* __restrict__ y,
+                                                           const uint8_t* __restrict__ t,
+                                                           const float* __restrict__ d, uint32_t n4,
+                                                           uint32_t* __restrict__ bad, uint32_t cap,
+                                                           uint32_t* __restrict__ misc) {
+  constexpr uint32_t kStage = 4096;
+  constexpr int FA = ND - 1;  // fastest axis
+  __shared__ uint32_t red[4][kWavesPerBlock];
+  __shared__ uint32_t lbuf[kStage];
+  __shared__ uint32_t lcnt, lbase;
+  if (threadIdx.x == 0) lcnt = 0;
+  __syncthreads();
+  const float4* y4 = reinterpret_cast<const float4*>(y);
+  const float4* d4 = reinterpret_cast<const float4*>(d);
+  const uint32_t* t4 = reinterpret_cast<const uint32_t*>(t);
+  const int dimz = dim_of<ND>(g, FA);
+  uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
+  for (uint32_t b0 = blockIdx.x * kBlockThreads; b0 < n4; b0 += gridDim.x * kBlockThreads) {
+    const uint32_t gi = b0 + threadIdx.x;
+    bool flag[4] = {false, false, false, false};
+    const uint32_t e0 = 4 * gi;
+    if (gi < n4) {
+      int c[ND];
+      coords_of<ND>(g, e0, c);
+      bool rowin = true;
+#pragma unroll
+      for (int a = 0; a < FA; a++) rowin = rowin && c[a] >= 1 && c[a] + 2 <= dim_of<ND>(g, a);
+      const float4 v = y4[gi];
+      const uint32_t tw = t4[gi];
+      float4 lo4[ND - 1], hi4[ND - 1];
+      float zm = 0.f, zp = 0.f;
+      if (rowin) {
+#pragma unroll
+        for (int a = 0; a < FA; a++) {
+          const uint32_t s4 = stride_of<ND>(g, a) >> 2;
+          lo4[a] = y4[gi - s4];
+          hi4[a] = y4[gi + s4];
+        }
+        if (c[FA] > 0) zm = y[e0 - 1];
+        if (c[FA] + 4 < dimz) zp = y[e0 + 4];
+      }
+      const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int z = c[FA] + i;
+        int k = 0;
+        if (rowin && z >= 1 && z + 2 <= dimz) {
+          float nb[2 * ND];
+#pragma unroll
+          for (int a = 0; a < FA; a++) {
+            const float l[4] = {lo4[a].x, lo4[a].y, lo4[a].z, lo4[a].w};
+            const float h[4] = {hi4[a].x, hi4[a].y, hi4[a].z, hi4[a].w};
+            nb[2 * a] = l[i];
+            nb[2 * a + 1] = h[i];
+          }
+          nb[2 * FA] = i == 0 ? zm : vv[i - 1];
+          nb[2 * FA + 1] = i == 3 ? zp : vv[i + 1];
+          k = classify_vals<ND>(vv[i], nb);
+        }
+        const int tt = (int)((tw >> (8 * i)) & 0xffu);
+        miss += (tt != 0 && k == 0);
+        spur += (tt == 0 && k != 0);
+        chg += (tt != 0 && k != 0 && k != tt);
+        flag[i] = k != tt;
+      }
+      if (d != nullptr) {
+        const float4 dd = d4[gi];
+        const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint32_t u = __float_as_uint(fabsf(vv[i] - dv[i]));
+          dev = u > dev ? u : dev;
+        }
+      }
+    }
+    if (bad != nullptr) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) lds_append(flag[i], e0 + i, lbuf, &lcnt);
+      __syncthreads();
+      if (lcnt > kStage - 4 * kBlockThreads) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+    }
+  }
+  if (bad != nullptr) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+  miss = seg_sum<kWave>(miss);
+  spur = seg_sum<kWave>(spur);
+  chg = seg_sum<kWave>(chg);
+  dev = seg_max<kWave>(dev);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[0][w] = miss;
+    red[1][w] = spur;
+    red[2][w] = chg;
+    red[3][w] = dev;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+      s2 += red[2][k];
+      s3 = red[3][k] > s3 ? red[3][k] : s3;
+    }
+    unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + kPpVerify);
+    if (s0) atomicAdd(&vo[0], (unsigned long long)s0);
+    if (s1) atomicAdd(&vo[1], (unsigned long long)s1);
+    if (s2) atomicAdd(&vo[2], (unsigned long long)s2);
+    if (s3) atomicMax(&vo[3], (unsigned long long)s3);
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // active set: the non-anchored points of the closed neighbourhoods of the mismatches
 // ---------------------------------------------------------------------------------
@@ -1353,6 +1468,15 @@ hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t
 
 hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
                    uint32_t cap, uint32_t* misc, hipStream_t st) {
+  const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
+  const bool vec = fast % 4 == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0;
+  if (vec) {
+    const dim3 grid(grid_stride_blocks(n / 4)), blk(kBlockThreads);
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
+    else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
+    return hipGetLastError();
+  }
   const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
   if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark<2>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   else hipLaunchKernelGGL(k_pp_mark<3>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);

@@ -108,7 +108,7 @@ def lib():
     L.szp_cuda_float_decompress_randomaccess.restype = vp
     L.szp_cuda_float_decompress_randomaccess_arg.argtypes = [vp, sz, f32, i32, vp]
     L.szp_cuda_float_decompress_randomaccess_arg.restype = None
-    L.tszp_topo_restore.argtypes = [vp, vp, vp, C.c_uint64, i32, vp, f32, i32, C.POINTER(RestoreInfo), vp]
+    L.tszp_topo_restore.argtypes = [vp, vp, vp, vp, C.c_uint64, i32, vp, f32, i32, C.POINTER(RestoreInfo), vp]
     L.tszp_topo_restore.restype = i32
     L.tszp_topo_verify.argtypes = [vp, vp, i32, vp, u64p, vp]
     L.tszp_topo_verify.restype = i32
@@ -367,11 +367,13 @@ def gen_fbm(shape, seed, octaves=6, noise=0.05, device="cuda"):
     return f
 
 
-def topo_restore(field, types, shape, eb, sp=None, n_ext=0, max_rounds=16):
-    """Restores the critical points of a decoded field in place (field: float32 CUDA
-    tensor holding (float)q * eb; types: uint8 CUDA tensor)."""
+def topo_restore(field, types, shape, eb, sp=None, n_ext=0, max_rounds=16, out=None):
+    """Restores the critical points of a decoded field (float32 CUDA tensor holding
+    (float)q * eb; types: uint8 CUDA tensor).  The result goes to `out` (a float32
+    tensor like `field`), or in place into `field` when out is None."""
     info = RestoreInfo()
-    _check(lib().tszp_topo_restore(_dp(field), _dp(types), _dp(sp) if sp is not None else None, n_ext,
+    dst = field if out is None else out
+    _check(lib().tszp_topo_restore(_dp(field), _dp(dst), _dp(types), _dp(sp) if sp is not None else None, n_ext,
                                    len(shape), _dims(shape), eb, max_rounds, C.byref(info), _stream()))
     return info
 
