@@ -784,7 +784,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     HIPCHK(launch_fill_u32(misc + kPpSolved, 0u, 2, st));
     HIPCHK(launch_fill_u32(misc + kPpFreed, 0u, 1, st));
     HIPCHK(launch_fill_u32(misc + kPpTilesP, 0u, 8, st));
-    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 9, st));
+    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 12, st));
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
@@ -796,10 +796,11 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     if ((r = full_mark(&nb, true))) return r;
     if (stats)
       fprintf(stderr, "[restore] outer %d: listed %u tiles %u solved %u failed %u freed %u nodes %u max nodes/tile %u "
-              "max windows/tile %u max tile us %.1f leaves %u verify fails %u left %u\n",
+              "max windows/tile %u max tile us %.1f leaves %u verify fails %u max build us %.1f max attempt us %.1f max search us %.1f left %u\n",
               outer, nl, total, h[kPpSolved], h[kPpFailed], h[kPpFreed], h[kPpStatNodes], h[kPpStatMaxNodes],
               h[kPpStatMaxWin], h[kPpStatMaxTicks] * 0.01, h[kPpStatLeaves],
-              h[kPpStatLeaves + 1], nb);
+              h[kPpStatLeaves + 1], h[kPpStatLeaves + 2] * 0.01, h[kPpStatLeaves + 3] * 0.01,
+              h[kPpStatLeaves + 4] * 0.01, nb);
     solved += h[kPpSolved];
     freed += h[kPpFreed];
     if (!h[kPpSolved] && outer > 0) break;

@@ -896,6 +896,27 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(Grid g, const uint32_
 // ---------------------------------------------------------------------------------
 // exact window solve
 // ---------------------------------------------------------------------------------
+// wave maximum of a value in [0, 127], one ballot per bit (no lane shuffles)
+__device__ __forceinline__ int wave_max7(int v) {
+  int res = 0;
+  bool alive = true;
+#pragma unroll
+  for (int b = 6; b >= 0; b--) {
+    const bool bit = (v >> b) & 1;
+    const uint64_t m = __ballot(alive && bit);
+    if (m) {
+      res |= 1 << b;
+      alive = alive && bit;
+    }
+  }
+  return res;
+}
+
+// value of v in lane `src` (wave-uniform), through a scalar register
+__device__ __forceinline__ int rdl(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ float rdlf(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 constexpr int kTileShift = 4;  // 16-wide tiles: windows of one parity class never interact
 constexpr int kWinMax = 8;
 constexpr int kSolveBudget = 1 << 14;
@@ -1093,7 +1114,7 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
   const uint64_t xmask = __ballot(my_fixed);
   bool my_first = my_fixed;
   for (int j = 0; j < ne; j++) {
-    const float vj = __shfl(my_val, j, kWave);
+    const float vj = rdlf(my_val, j);
     if (((xmask >> j) & 1ull) && j < lane && vj == my_val) my_first = false;
   }
   const uint64_t first_mask = __ballot(my_first);
@@ -1101,7 +1122,7 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
   {
     int rank = 0;
     for (int j = 0; j < ne; j++) {
-      const float vj = __shfl(my_val, j, kWave);
+      const float vj = rdlf(my_val, j);
       if (((first_mask >> j) & 1ull) && vj < my_val) rank++;
     }
     if (my_fixed) my_cls = rank;
@@ -1131,12 +1152,13 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
   float my_fval = 0.f, my_old = 0.f;
   while (true) {
     bool leaf = false;
+    const uint64_t tdfs = wall_clock64();
     while (depth >= 0) {
-      const int fe = __shfl(felem_reg, depth, kWave);
-      int it = __shfl(choice_reg, depth, kWave);
+      const int fe = rdl(felem_reg, depth);
+      int it = rdl(choice_reg, depth);
       if (it >= 0) {  // undo the previous choice at this depth
-        const int pos = __shfl(my_cls, fe, kWave);
-        const int nc = __shfl(newcls_reg, depth, kWave);
+        const int pos = rdl(my_cls, fe);
+        const int nc = rdl(newcls_reg, depth);
         if (lane == fe) my_cls = -1;
         if (nc) {
           if (my_cls > pos) my_cls--;
@@ -1145,14 +1167,9 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
       } else {
         // starting choice: the class of an equal current value, else a new class just
         // above every placed element of smaller current value
-        const float vf = __shfl(my_val, fe, kWave);
-        int above = my_cls >= 0 && my_val < vf ? my_cls + 1 : 0;
-        int eqc = my_cls >= 0 && my_val == vf ? my_cls : -1;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          above = max(above, __shfl_xor(above, o, kWave));
-          eqc = max(eqc, __shfl_xor(eqc, o, kWave));
-        }
+        const float vf = rdlf(my_val, fe);
+        const int above = wave_max7(my_cls >= 0 && my_val < vf ? my_cls + 1 : 0);
+        const int eqc = wave_max7(my_cls >= 0 && my_val == vf ? my_cls + 1 : 0) - 1;
         const int hm = eqc >= 0 ? 2 * eqc + 1 : 2 * above;
         if (lane == depth) home_reg = hm;
       }
@@ -1171,7 +1188,7 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
       // choices in the order home, home + 1, home - 1, home + 2, ... over [0, 2 ncls]
       int ch;
       {
-        const int home = __shfl(home_reg, depth, kWave);
+        const int home = rdl(home_reg, depth);
         const int nlo = home, nhi = maxch - home, mm = min(nlo, nhi);
         if (it <= 2 * mm) ch = (it & 1) ? home + ((it + 1) >> 1) : home - (it >> 1);
         else ch = nhi > nlo ? home + (it - mm) : home - (it - mm);
@@ -1278,6 +1295,7 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
         break;
       }
     }
+    if (lane == 0) stats[6] += (uint32_t)(wall_clock64() - tdfs);
     if (!leaf) break;
     // write the free points and check every constraint on the field
     if (my_free) {
@@ -1302,8 +1320,8 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
   }
   if (lane == 0) {
     *nodes_out += (uint32_t)nodes;
-    atomicAdd(&stats[0], leaves);
-    atomicAdd(&stats[1], vfails);
+    stats[0] += leaves;
+    stats[1] += vfails;
   }
   __syncthreads();
   return found;
@@ -1318,8 +1336,13 @@ template <int ND>
 __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
                             float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
                             uint32_t* nodes, uint32_t* stats) {
+  const uint64_t t0 = wall_clock64();
   build_window<ND>(w, g, d, t, y, eb, b, anchor, lane);
-  if (run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats)) return 0;
+  const uint64_t t1 = wall_clock64();
+  stats[4] += (uint32_t)(t1 - t0);
+  const bool ok0 = run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats);
+  stats[5] += (uint32_t)(wall_clock64() - t1);
+  if (ok0) return 0;
   if (lane == 0) {
     // anchored window points, the most extreme value first
     const float mid = 0.5f * (w.lo + w.hi);
@@ -1418,6 +1441,7 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
   __syncthreads();
   nb = sh_nb;
   uint32_t solved = 0, failed = 0, freed = 0, nodes = 0;
+  uint32_t tst[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // [0] leaves [1] verify fails [4] build ticks [5] attempt ticks
   // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
   const float delta_frac = 0.99f * 0.9f / ((float)misc[kPpSpMax] + 2.f);
   for (uint32_t j = 0; j < nb; j++) {
@@ -1426,7 +1450,7 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     coords_of<ND>(g, e, c);
     if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
     const int mode =
-        solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, misc + kPpStatLeaves);
+        solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
     solved += mode >= 0;
     failed += mode < 0;
     freed += mode >= 2;
@@ -1439,6 +1463,11 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     atomicMax(&misc[kPpStatMaxNodes], nodes);
     atomicMax(&misc[kPpStatMaxWin], solved + failed);
     atomicMax(&misc[kPpStatMaxTicks], (uint32_t)(wall_clock64() - t_work));
+    atomicAdd(&misc[kPpStatLeaves], tst[0]);
+    atomicAdd(&misc[kPpStatLeaves + 1], tst[1]);
+    atomicMax(&misc[kPpStatLeaves + 2], tst[4]);
+    atomicMax(&misc[kPpStatLeaves + 3], tst[5]);
+    atomicMax(&misc[kPpStatLeaves + 4], tst[6]);
   }
 }
 
