@@ -90,6 +90,21 @@ def cpu_baseline(field_np, eb, bs, threads, slab):
     return res, None
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE summary
+    (already scaled by the copy-kernel calibration there), averaged over its dispatches;
+    None when the summary does not list it."""
+    path = os.path.join(ROOT, "profiles", "topo_512cube_bs64_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = []
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and kernel in cells[0] and cells[4].isdigit():
+            vals.append(int(cells[4]))
+    return sum(vals) / len(vals) if vals else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +118,8 @@ def main():
     ap.add_argument("--cpu-slab", type=int, default=64, help="planes of the field for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the sharded design is "
@@ -134,6 +151,14 @@ def main():
             last["restore"] = tz.topo_restore(out, types, shape, eb, sp=sp, n_ext=info.ext_count, out=restored)
         return info
 
+    if args.profile_pass:
+        info = step()
+        x = torch.empty_like(f)
+        torch.mul(f, 1.0, out=x)  # elementwise: reads and writes 4N bytes
+        torch.cuda.synchronize()
+        log(f"profile pass: cps={info.cp_count} topo={info.topo_bytes} sp={info.sortpos_bytes}")
+        return
+
     for i in range(args.warmup):
         info = step()
         log(f"warmup {i + 1}/{args.warmup}: cps={info.cp_count} ext={info.ext_count} "
@@ -163,25 +188,28 @@ def main():
     stages = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in prof.items()}
     comp_payload = info.topo_bytes - 8
     sp_payload = max(0, info.sortpos_bytes - 16)
-    # algorithmic bytes per launch of each stage's kernel(s)
-    algo = {
-        "codec_topo": 4 * n + comp_payload,                  # read field, write topology payload
-        "codec_int": 4 * info.ext_count + sp_payload,        # read sort positions, write stream
-        "decode": (comp_payload + 5 * n) + (sp_payload + 4 * info.ext_count),  # both decoders
-        "sort": 4 * 16 * info.cp_count,                      # 4 passes x (key+payload) x (read+write)
-        "rank": 12 * info.cp_count,
-        "restore": 0,
+    # Single kernels timed per launch by the library, with their algorithmic bytes per
+    # launch (the minimum HBM traffic of what each launch must read and write).
+    kernels = {
+        # read the field, write the topology payload (the 8 B/CP sort keys it also emits
+        # are an intermediate of this design and not counted)
+        "codec_topo": ("k_compress<0, 64, 1>", 4 * n + comp_payload),
+        # one LSD pass: read and write a 4 B key + 4 B payload per critical point
+        "scatter": ("k_scatter<false>", 16 * info.cp_count),
+        # read sorted keys + payloads, write the sort position of each extremum
+        "rank": ("k_rank", 8 * info.cp_count + 4 * info.ext_count),
+        # read the sort positions, write their stream
+        "codec_int": ("k_compress<3, 64, 1>", 4 * info.ext_count + sp_payload),
     }
-    dom = max((k for k in algo), key=lambda k: stages.get(k, (0, 0))[0])
-    dom_ms = stages[dom][0] / max(1.0, stages[dom][1]) if stages[dom][1] else stages[dom][0]
-    # per launch: average duration of the stage's launches
-    launches = prof[dom][1] / args.steps
+    dom = max(kernels, key=lambda k: stages.get(k, (0, 0))[0])
     per_launch_ms = prof[dom][0] / max(1, prof[dom][1])
-    per_launch_bytes = algo[dom] / max(1.0, launches)
+    per_launch_bytes = kernels[dom][1]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms}
+    traffic = pmc_traffic(kernels[dom][0])
+    roofline = {"bound": "hbm", "kernel": kernels[dom][0], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
+                "launches_per_step": prof[dom][1] / args.steps}
     e2e_bytes = (4 * n + info.topo_bytes + info.sortpos_bytes) + (info.topo_bytes + info.sortpos_bytes + 5 * n
                                                                    + 4 * info.ext_count)
     e2e_frac = e2e_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS

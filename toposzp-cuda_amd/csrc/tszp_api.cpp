@@ -84,7 +84,7 @@ std::vector<Ctx*> g_ctx;
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
 enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
-                kProfCount };
+                kProfScatter, kProfSolve, kProfCount };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -114,6 +114,40 @@ struct ProfScope {
     a = b = nullptr;
   }
   ~ProfScope() { collect(); }
+};
+
+// Times individual launches of one kind: event pairs are recorded around each launch
+// and read when the timer goes out of scope (after the caller has synchronized).
+struct EventTimer : LaunchTimer {
+  int kind;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  explicit EventTimer(int k) : kind(k) {}
+  void begin(hipStream_t st) override {
+    if (!g_prof) return;
+    hipEvent_t a = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    (void)hipEventRecord(a, st);
+    ev.emplace_back(a, nullptr);
+  }
+  void end(hipStream_t st) override {
+    if (!g_prof || ev.empty() || ev.back().second) return;
+    hipEvent_t b = nullptr;
+    if (hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(b, st);
+    ev.back().second = b;
+  }
+  ~EventTimer() override {
+    for (auto& p : ev) {
+      float ms = 0.f;
+      if (p.second && hipEventSynchronize(p.second) == hipSuccess &&
+          hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) {
+        g_prof_ms[kind] += ms;
+        g_prof_n[kind]++;
+      }
+      (void)hipEventDestroy(p.first);
+      if (p.second) (void)hipEventDestroy(p.second);
+    }
+  }
 };
 
 int get_ctx(Ctx** out) {
@@ -283,8 +317,9 @@ int sort_and_rank(Ctx* c, uint32_t* keys, uint32_t* hi, uint32_t* pay, uint32_t 
   if (rc) return rc;
   const size_t stw = sort_status_words(count) + 8;
   HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(w.st), 0u, stw * 2, st));
+  EventTimer scatter_timer(kProfScatter);
   ProfScope ps(kProfSort, st);
-  HIPCHK(radix_sort_pairs(keys, hi, pay, count, w, st));
+  HIPCHK(radix_sort_pairs(keys, hi, pay, count, w, st, &scatter_timer));
   ps.stop();
   ProfScope pr(kProfRank, st);
   HIPCHK(launch_rank(keys, hi, pay, count, eb, inv, out, out_cap, w, st));
@@ -788,10 +823,14 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
+    EventTimer solve_timer(kProfSolve);
     for (int ph = 0; ph < phases; ph++) {
       const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
       total += cnt;
+      if (!cnt) continue;
+      solve_timer.begin(st);
       HIPCHK(pp_solve(g, tiles, cnt, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+      solve_timer.end(st);
     }
     if ((r = full_mark(&nb, true))) return r;
     if (stats)

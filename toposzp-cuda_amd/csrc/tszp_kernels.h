@@ -92,10 +92,18 @@ struct SortWork {
 };
 size_t sort_hist_words(uint32_t count);
 size_t sort_status_words(uint32_t count);
+// Optional per-launch timing: begin/end are called around a launch on its stream.
+struct LaunchTimer {
+  virtual void begin(hipStream_t st) = 0;
+  virtual void end(hipStream_t st) = 0;
+  virtual ~LaunchTimer() = default;
+};
+
 // Stable LSD radix sort of (key, payload) pairs, 8-bit digits; key_hi (nullable)
 // is a more significant key word.  Results land back in keys / key_hi / pay.
+// scatter_timer (nullable) times every scatter launch.
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
-                            const SortWork& wk, hipStream_t st);
+                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer = nullptr);
 // Dense rank of distinct values inside quantized bins over the sorted keys
 // (SZp/src/szp_float.cc:344-358): rank resets at each new bin and grows at each new
 // value.  Bins come from the float value (by_value: bin = (int)((v + eb) * inv)) or

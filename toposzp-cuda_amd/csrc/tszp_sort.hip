@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
-                            const SortWork& wk, hipStream_t st) {
+                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer) {
   if (count <= 1) return hipSuccess;
   const uint32_t ntiles = ceil_div(count, kSortTile);
   const uint32_t hwords = 256u * ntiles;
@@ -175,12 +175,14 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uin
       hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlockThreads), 0, st, dw, count, shift, wk.hist, ntiles);
       hipLaunchKernelGGL(k_scan_excl, dim3(stiles), dim3(kBlockThreads), 0, st, wk.hist, hwords,
                          wk.st + (size_t)pass * stiles, wk.counters + pass, wk.err);
+      if (scatter_timer) scatter_timer->begin(st);
       if (h0)
         hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1, count,
                            shift, wd, wk.hist, ntiles);
       else
         hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1,
                            count, shift, wd, wk.hist, ntiles);
+      if (scatter_timer) scatter_timer->end(st);
       std::swap(k0, k1);
       std::swap(h0, h1);
       std::swap(p0, p1);
