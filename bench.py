@@ -194,10 +194,10 @@ def main():
         # read the field, write the topology payload (the 8 B/CP sort keys it also emits
         # are an intermediate of this design and not counted)
         "codec_topo": ("k_compress<0, 64, 1>", 4 * n + comp_payload),
-        # one LSD pass: read and write a 4 B key + 4 B payload per critical point
-        "scatter": ("k_scatter<false>", 16 * info.cp_count),
-        # read sorted keys + payloads, write the sort position of each extremum
-        "rank": ("k_rank", 8 * info.cp_count + 4 * info.ext_count),
+        # bucket the critical points' value keys by bin: read and write 4 B each
+        "scatter": ("k_bin_scatter", 8 * info.cp_count),
+        # read each critical point's key and payload, write each extremum's sort position
+        "query": ("k_bin_query", 8 * info.cp_count + 4 * info.ext_count),
         # read the sort positions, write their stream
         "codec_int": ("k_compress<3, 64, 1>", 4 * info.ext_count + sp_payload),
     }

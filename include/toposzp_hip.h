@@ -121,9 +121,12 @@ int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, con
 
 /* Kernel timing with HIP events on the launch stream (off by default).  Kinds:
  * 0 topology codec (classify + pack), 1 int codec (sort positions), 2 plain codec,
- * 3 decoders, 4 radix sort, 5 rank, 6 topology restoration (whole call), 7 each radix
- * scatter launch, 8 each window-solve launch.  tszp_prof_read returns the summed
- * milliseconds and the number of timed launches since tszp_prof_reset. */
+ * 3 decoders, 4 sort-position bucketing (bins, scatter, bitmaps; or the radix sort),
+ * 5 rank queries (and the radix sort of the bins next to zero; or the rank scan),
+ * 6 topology restoration (whole call), 7 each scatter launch (bin scatter or radix
+ * pass), 8 each window-solve launch, 9 each bin rank-query launch.  tszp_prof_read
+ * returns the summed milliseconds and the number of timed launches since
+ * tszp_prof_reset. */
 void tszp_prof_enable(int on);
 void tszp_prof_reset(void);
 int tszp_prof_read(int kind, double *ms, uint64_t *launches);

@@ -214,3 +214,66 @@ def test_large_field_matches_oracle():
     dec, fn = oracle.topo_decompress(topo, data.size, eb, bs)
     _eq(out.cpu().numpy(), dec)
     _eq(ty.cpu().numpy(), types)
+
+
+# ------------------------------------------------------------------------------------------
+# sort positions: per-bin bitmap path against the radix path and the oracle
+# ------------------------------------------------------------------------------------------
+def _centred(data, scale=1.0):
+    return ((data - np.float32(np.median(data))) * np.float32(scale)).astype(np.float32)
+
+
+SP_CASES = [
+    # values around zero: the bins next to zero are wide (radix on their buckets) and
+    # their neighbours need several bitmap chunks
+    ("zero_centred_3d", lambda: _centred(fields.fbm3d(40, 44, 48, seed=41)), 1e-2),
+    ("zero_centred_2d", lambda: _centred(fields.blobs2d(257, 263, seed=42, noise=0.05)), 5e-3),
+    # every value within a few error bounds of zero: all bins wide
+    ("near_zero_2d", lambda: _centred(fields.blobs2d(120, 130, seed=43, noise=0.05), 1e-3), 1e-3),
+    # more bins than the bitmap path takes: radix path
+    ("many_bins_2d", lambda: fields.blobs2d(200, 210, seed=44, noise=0.05), 2e-5),
+    # one-signed fields, far from zero
+    ("negative_3d", lambda: fields.fbm3d(30, 32, 34, seed=45) - np.float32(5.0), 2e-3),
+    ("offset_2d", lambda: fields.blobs2d(150, 160, seed=46) + np.float32(300.0), 1e-3),
+]
+
+
+@pytest.mark.parametrize("name,make,eb", SP_CASES, ids=[c[0] for c in SP_CASES])
+def test_sort_positions_bin_path(name, make, eb, monkeypatch):
+    torch = _torch()
+    data = np.ascontiguousarray(make(), dtype=np.float32)
+    f = torch.from_numpy(data).cuda()
+    bs = 64
+    info, buf = tz.topo_compress(f, eb, bs)
+    torch.cuda.synchronize()
+    sp_bins = buf.sp_values[:info.ext_count].cpu().numpy().copy()
+    stream_bins = buf.sp[:info.sortpos_bytes].cpu().numpy().copy()
+    monkeypatch.setenv("TSZP_SORTPOS", "radix")
+    info2, buf2 = tz.topo_compress(f, eb, bs)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("TSZP_SORTPOS")
+    assert info2.ext_count == info.ext_count and info.ext_count > 0
+    _eq(sp_bins, buf2.sp_values[:info2.ext_count].cpu().numpy(), "bin path vs radix path")
+    _eq(stream_bins, buf2.sp[:info2.sortpos_bytes].cpu().numpy(), "sort-position streams")
+    cps, types, sp, topo, sps = _oracle_topo(data, eb, bs, 1)
+    _eq(sp_bins, sp, "sort positions vs oracle")
+
+
+def test_sort_positions_bin_path_large():
+    """8.4 M points, field centred on zero (wide, multi-chunk and narrow bins together);
+    the radix path is the reference."""
+    torch = _torch()
+    f = tz.gen_fbm((128, 256, 256), seed=77)
+    f -= f.median()
+    eb, _ = tz.rel_to_abs(f, 1e-3)
+    info, buf = tz.topo_compress(f, eb, 64)
+    torch.cuda.synchronize()
+    a = buf.sp_values[:info.ext_count].clone()
+    os.environ["TSZP_SORTPOS"] = "radix"
+    try:
+        info2, buf2 = tz.topo_compress(f, eb, 64)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["TSZP_SORTPOS"]
+    assert info2.ext_count == info.ext_count
+    assert torch.equal(a, buf2.sp_values[:info.ext_count])

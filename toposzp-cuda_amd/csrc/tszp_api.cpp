@@ -48,6 +48,7 @@ enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
+  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits,
   kWsCount
 };
 
@@ -84,7 +85,7 @@ std::vector<Ctx*> g_ctx;
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
 enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
-                kProfScatter, kProfSolve, kProfCount };
+                kProfScatter, kProfSolve, kProfQuery, kProfCount };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -331,6 +332,86 @@ int sort_and_rank(Ctx* c, uint32_t* keys, uint32_t* hi, uint32_t* pay, uint32_t 
   return TSZP_OK;
 }
 
+// TSZP_SORTPOS=radix selects the full radix sort for sort positions by value (the
+// two paths give identical ranks; the switch exists to compare them).
+bool force_radix_sortpos() {
+  const char* e = getenv("TSZP_SORTPOS");
+  return e && strcmp(e, "radix") == 0;
+}
+
+// Sort positions by value through per-bin bitmaps (tszp_binrank.hip): writes
+// out[pay] = dense in-bin rank, like sort_and_rank with by-value bins.  Sets
+// *done = false, having written nothing, when the bins do not fit (more than kBinMax
+// bins, or bins that are not monotone in the value); the caller then uses the radix
+// path.  The few bins whose key span exceeds kBinWideBits are ranked by the radix
+// sort on their buckets.
+int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32_t count, float eb, double inv,
+                       int32_t* out, uint32_t out_cap, hipStream_t st, bool* done) {
+  *done = false;
+  if (!count || force_radix_sortpos()) return TSZP_OK;
+  BinWork w{};
+  uint32_t* meta;
+  WS(kWsBinMeta, bin_meta_words() * 4 + 64, meta);
+  bin_work_carve(meta, &w);
+  uint32_t h[16];
+  ProfScope ps(kProfSort, st);
+  HIPCHK(bin_range(keys, count, eb, inv, w, st));
+  HIPCHK(hipMemcpyAsync(h, w.mm, 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t bmin = (int64_t)(int32_t)(h[2] ^ 0x80000000u), bmax = (int64_t)(int32_t)(~h[3] ^ 0x80000000u);
+  if (bmax < bmin || bmax - bmin + 1 > (int64_t)kBinMax) return TSZP_OK;
+  const uint32_t B = (uint32_t)(bmax - bmin + 1);
+  uint32_t nslice, slice;
+  bin_slices(count, &nslice, &slice);
+  HIPCHK(bin_count(keys, count, eb, inv, (int)bmin, B, w, st));
+  HIPCHK(bin_plan(B, nslice, w, st));
+  HIPCHK(hipMemcpyAsync(h, w.info, 64, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h[8]) return fail(TSZP_EDEVICE, "sort positions: device error bits 0x%x", h[8]);
+  if (h[6] != count) return TSZP_OK;  // some key outside [bmin, bmax]: bins not monotone
+  *done = true;
+  const uint32_t nwords = h[0], nchunks = h[1], wide_s = h[4], wide_e = h[5];
+  if (getenv("TSZP_SORTPOS_STATS"))
+    fprintf(stderr, "[sortpos] keys %u bins %u slices %u bitmap words %u chunks %u wide bins %d..%d keys %u\n",
+            count, B, nslice, nwords, nchunks, h[2] == 0xffffffffu ? 0 : (int)(h[2] + bmin),
+            h[2] == 0xffffffffu ? -1 : (int)(h[3] + bmin), wide_e - wide_s);
+  WS(kWsBinKey, (size_t)count * 4, w.bkey);
+  WS(kWsBinPay, (size_t)count * 4, w.bpay);
+  uint2* gw;
+  WS(kWsBinBits, (size_t)nwords * 8, gw);
+  {
+    EventTimer t(kProfScatter);
+    t.begin(st);
+    HIPCHK(bin_scatter(keys, pay, count, eb, inv, (int)bmin, B, w, st));
+    t.end(st);
+  }
+  HIPCHK(bin_bitmap(nchunks, w, gw, st));
+  ps.stop();
+  ProfScope pr(kProfRank, st);
+  {
+    EventTimer t(kProfQuery);
+    t.begin(st);
+    HIPCHK(bin_query(keys, pay, count, eb, inv, (int)bmin, B, w, gw, out, out_cap, st));
+    t.end(st);
+  }
+  if (wide_e > wide_s) {
+    const uint32_t wn = wide_e - wide_s;
+    int rc;
+    SortWork sw = sort_work(c, wn, false, &rc);
+    if (rc) return rc;
+    const size_t stw = sort_status_words(wn) + 8;
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(sw.st), 0u, stw * 2, st));
+    HIPCHK(radix_sort_pairs(w.bkey + wide_s, nullptr, w.bpay + wide_s, wn, sw, st));
+    HIPCHK(launch_rank(w.bkey + wide_s, nullptr, w.bpay + wide_s, wn, eb, inv, out, out_cap, sw, st));
+    uint32_t herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, sw.err, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (herr) return fail(TSZP_EDEVICE, "sort: device error bits 0x%x", herr);
+  }
+  pr.stop();
+  return TSZP_OK;
+}
+
 // ---- host helpers for streams that arrive without a side-car -------------------------
 inline uint64_t rd64(const unsigned char* p) {
   uint64_t v;
@@ -563,7 +644,9 @@ int tszp_topo_compress(const float* d_field, int ndim, const uint32_t* dims, flo
   if (!d_sp || !d_sp_blk) return fail(TSZP_EINVAL, "topo_compress: sort-position buffers missing");
   int32_t* sp = d_sp_values;
   if (!sp) WS(kWsSp, (size_t)E * 4, sp);
-  r = sort_and_rank(c, cpk, nullptr, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st);
+  bool by_bins;
+  r = bin_sort_positions(c, cpk, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st, &by_bins);
+  if (!r && !by_bins) r = sort_and_rank(c, cpk, nullptr, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st);
   if (r) return r;
   CompressResult r2;
   r = run_codec(c, kModeInt, sp, E, bs, 0.f, 0.0, nullptr, nullptr, d_sp, sp_cap, 8 + 8 * (size_t)T, d_sp_blk,
@@ -770,6 +853,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     bool complete = nb <= cap;
     for (int round = 0; round < max_rounds && nb; round++) {
       const uint32_t nl = nb < cap ? nb : cap;
+      const uint32_t nb_before = nb;
       // a closed neighbourhood holds at most one point of each colour class, so every
       // colour list has room for one entry per listed mismatch
       WS(kWsPpAct, (size_t)nl * colours * 4 + 16, act);
@@ -797,7 +881,9 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       complete = nb <= cap;
       moves += h[kPpMoves];
       if (stats) fprintf(stderr, "[restore] round %u: listed %u moved %u -> %u mismatches\n", rounds, nl, h[kPpMoves], nb);
-      if (!h[kPpMoves]) break;
+      // stop once a round removes less than a quarter of the mismatches: what is left
+      // is mostly plateaus that only the window solver can order
+      if (!h[kPpMoves] || (uint64_t)nb * 4 > (uint64_t)nb_before * 3) break;
     }
     if (!nb) break;
     if (!exact) {

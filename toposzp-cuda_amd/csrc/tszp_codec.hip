@@ -509,7 +509,7 @@ static hipError_t launch_compress_mode(const CompressArgs& a, int S, int EPL, hi
     return hipGetLastError();                                                              \
   }
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
-  TSZP_CASE(32, 1) TSZP_CASE(64, 1) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
   if constexpr (MODE == kModePlain || MODE == kModeInt) {
     TSZP_CASE(64, 16) TSZP_CASE(64, 32)
   }
@@ -525,7 +525,7 @@ static hipError_t launch_decode_mode(const DecodeArgs& a, int S, int EPL, uint32
     return hipGetLastError();                                                             \
   }
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
-  TSZP_CASE(32, 1) TSZP_CASE(64, 1) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
   if constexpr (MODE == kModePlain || MODE == kModeInt) {
     TSZP_CASE(64, 16) TSZP_CASE(64, 32)
   }
@@ -535,8 +535,13 @@ static hipError_t launch_decode_mode(const DecodeArgs& a, int S, int EPL, uint32
 
 void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* bpt) {
   int s = 1, e = 1;
-  if (bs <= 64) {
+  if (bs <= 16) {
     while ((uint32_t)s < bs) s <<= 1;
+  } else if (bs <= 64) {
+    // four consecutive elements per lane: the per-block wave operations (maximum,
+    // scans) are shared by four elements
+    e = 4;
+    while ((uint32_t)(4 * s) < bs) s <<= 1;
   } else {
     s = 64;
     while ((uint32_t)(64 * e) < bs) e <<= 1;

@@ -92,6 +92,11 @@ struct SortWork {
 };
 size_t sort_hist_words(uint32_t count);
 size_t sort_status_words(uint32_t count);
+// In-place exclusive scan of a[0..count) in one launch (decoupled look-back);
+// status: scan_status_words(count) zeroed u64 words, counter: one zeroed u32.
+size_t scan_status_words(uint32_t count);
+hipError_t launch_scan_excl(uint32_t* a, uint32_t count, uint64_t* status, uint32_t* counter, uint32_t* err,
+                            hipStream_t st);
 // Optional per-launch timing: begin/end are called around a launch on its stream.
 struct LaunchTimer {
   virtual void begin(hipStream_t st) = 0;
@@ -111,6 +116,47 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uin
 // (saddles carry pay = 0xffffffff).
 hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint32_t* pay, uint32_t count,
                        float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st);
+
+// ---- sort positions by per-bin bitmaps (tszp_binrank.hip) ----
+// Ranks by value (keys are order-preserving float keys, bins come from the values)
+// without a full sort: bucket the keys by bin, mark each bin's distinct keys in a
+// bitmap over the bin's key span, and read each rank as a prefix popcount.  Bins
+// whose span exceeds kBinWideBits (the few bins next to zero) are ranked by the radix
+// sort above, on their buckets only.  Keys and payloads are read as uint4: both arrays
+// must be 16-byte aligned and readable up to the next multiple of 4 entries.
+constexpr uint32_t kBinMax = 4096;             // bins per call (more: radix path)
+constexpr uint32_t kBinSlices = 512;           // key slices (one workgroup each) for count and scatter
+constexpr uint32_t kBinChunkWords = 4096;      // bitmap words per workgroup
+constexpr uint32_t kBinMaxChunks = 32;         // chunks per bin
+constexpr uint64_t kBinWideBits = (uint64_t)kBinChunkWords * 32 * kBinMaxChunks;
+struct BinWork {
+  uint32_t* mm;       // 4: min key, ~max key, min bin ^ 2^31, ~(max bin ^ 2^31)
+  uint32_t* part;     // range partials: min and max key per workgroup
+  uint32_t* klo;      // kBinMax + 1: lowest key of each bin's span
+  uint32_t* boff;     // kBinMax + 1: bucket offsets
+  uint4* bt;          // kBinMax + 1: per bin {klo, word offset, first chunk, wide}
+  uint32_t* cbin;     // kBinMax * kBinMaxChunks: bin of each chunk
+  uint32_t* ctot;     // kBinMax * kBinMaxChunks: distinct keys per chunk
+  uint32_t* info;     // 16: words, chunks, wide bins first/last, their bucket range, scan error
+  uint64_t* scan_st;  // scan status words for the histogram
+  uint32_t* scan_ctr; // scan tile counter
+  uint32_t* hist;     // kBinMax * kBinSlices + 1: per (bin, slice) counts, bin-major, then scanned
+  uint32_t* bkey;     // count: keys bucketed by bin
+  uint32_t* bpay;     // count: payloads (written for the wide-bin range only)
+};
+size_t bin_meta_words();
+void bin_work_carve(uint32_t* meta, BinWork* w);  // meta: bin_meta_words() u32, 16-byte aligned
+// slices for `count` keys: *nslice workgroups of *slice keys each
+void bin_slices(uint32_t count, uint32_t* nslice, uint32_t* slice);
+hipError_t bin_range(const uint32_t* key, uint32_t count, float eb, double inv, const BinWork& w, hipStream_t st);
+hipError_t bin_count(const uint32_t* key, uint32_t count, float eb, double inv, int bmin, uint32_t B,
+                     const BinWork& w, hipStream_t st);
+hipError_t bin_plan(uint32_t B, uint32_t nslice, const BinWork& w, hipStream_t st);
+hipError_t bin_scatter(const uint32_t* key, const uint32_t* pay, uint32_t count, float eb, double inv, int bmin,
+                       uint32_t B, const BinWork& w, hipStream_t st);
+hipError_t bin_bitmap(uint32_t nchunks, const BinWork& w, uint2* gw, hipStream_t st);
+hipError_t bin_query(const uint32_t* key, const uint32_t* pay, uint32_t count, float eb, double inv, int bmin,
+                     uint32_t B, const BinWork& w, const uint2* gw, int32_t* out, uint32_t out_cap, hipStream_t st);
 
 // ---- topology restoration (tszp_post.hip) ----
 // Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.

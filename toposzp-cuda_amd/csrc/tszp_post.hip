@@ -280,29 +280,77 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restri
   __syncthreads();
   const uint32_t base = sh_base;
   const float sden = (float)misc[kPpSpMax] + 2.f;
-  for (int j = 0; j < kInitPer; j++) {
-    const uint32_t i = (uint32_t)j * kBlockThreads + tid;
-    const uint32_t e = t0 + i;
-    if (e >= n) break;
-    const uint32_t tb = (tw[i >> 2] >> (8 * (i & 3u))) & 0xffu;
-    const float dd = d[e];
-    float lo, hi;
-    cell_of(dd, eb, &lo, &hi);
-    const float mid = dd == 0.f ? 0.f : 0.5f * (lo + hi);
-    float v = mid;
-    if (tb == 1u || tb == 2u) {
-      const uint32_t o = i >> 6, b = i & 63u;
-      const uint32_t k = base + exl[o] + (uint32_t)__popcll(msk[o] & ((1ull << b) - 1ull));
-      int32_t s = (sp != nullptr && k < n_ext) ? sp[k] : 0;
-      s = s < 0 ? 0 : s;
-      if (sp != nullptr && dd != 0.f) {
-        v = lo + (hi - lo) * (0.05f + 0.9f * (((float)s + 1.f) / sden));
+  // float4 groups of 4 points (the tile start is a multiple of 4); all loads of a batch
+  // are issued before the values are computed
+  constexpr int kGroups = kInitTile / 4 / kBlockThreads;  // 16
+  constexpr int kBatch = 4;
+  const float4* d4 = reinterpret_cast<const float4*>(d);
+  float4* y4 = reinterpret_cast<float4*>(y);
+  const bool vec = ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(y)) & 15u) == 0;
+#pragma unroll
+  for (int j0 = 0; j0 < kGroups; j0 += kBatch) {
+    float dv[kBatch][4];
+    int32_t sv[kBatch][4];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const uint32_t gidx = (uint32_t)(j0 + u) * kBlockThreads + tid;  // group in the tile
+      const uint32_t e0 = t0 + 4 * gidx;
+      if (vec && e0 + 3 < n) {
+        const float4 q = d4[e0 >> 2];
+        dv[u][0] = q.x;
+        dv[u][1] = q.y;
+        dv[u][2] = q.z;
+        dv[u][3] = q.w;
       } else {
-        const float r = (float)s / ((float)s + 64.f);
-        v = tb == 1u ? mid + (hi - mid) * (0.55f + 0.4f * r) : mid - (mid - lo) * (0.95f - 0.4f * r);
+#pragma unroll
+        for (int c = 0; c < 4; c++) dv[u][c] = e0 + c < n ? d[e0 + c] : 0.f;
+      }
+      const uint32_t tv = tw[gidx];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const uint32_t i = 4 * gidx + c;
+        const uint32_t tb = (tv >> (8 * c)) & 0xffu;
+        sv[u][c] = 0;
+        if ((tb == 1u || tb == 2u) && sp != nullptr) {
+          const uint32_t o = i >> 6, b = i & 63u;
+          const uint32_t k = base + exl[o] + (uint32_t)__popcll(msk[o] & ((1ull << b) - 1ull));
+          sv[u][c] = k < n_ext ? sp[k] : 0;
+        }
       }
     }
-    y[e] = v;
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const uint32_t gidx = (uint32_t)(j0 + u) * kBlockThreads + tid;
+      const uint32_t e0 = t0 + 4 * gidx;
+      const uint32_t tv = tw[gidx];
+      float out[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const uint32_t tb = (tv >> (8 * c)) & 0xffu;
+        const float dd = dv[u][c];
+        float lo, hi;
+        cell_of(dd, eb, &lo, &hi);
+        const float mid = dd == 0.f ? 0.f : 0.5f * (lo + hi);
+        float v = mid;
+        if (tb == 1u || tb == 2u) {
+          const int32_t s = sv[u][c] < 0 ? 0 : sv[u][c];
+          if (sp != nullptr && dd != 0.f) {
+            v = lo + (hi - lo) * (0.05f + 0.9f * (((float)s + 1.f) / sden));
+          } else {
+            const float r = (float)s / ((float)s + 64.f);
+            v = tb == 1u ? mid + (hi - mid) * (0.55f + 0.4f * r) : mid - (mid - lo) * (0.95f - 0.4f * r);
+          }
+        }
+        out[c] = v;
+      }
+      if (vec && e0 + 3 < n) {
+        y4[e0 >> 2] = make_float4(out[0], out[1], out[2], out[3]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (e0 + c < n) y[e0 + c] = out[c];
+      }
+    }
   }
 }
 
@@ -375,97 +423,133 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark(Grid g, const float* 
 }
 
 // Same pass, 4 consecutive points per thread along the fastest axis (its length a
-// multiple of 4, buffers 16-byte aligned): one float4 per row of the stencil.
+// multiple of 4, buffers 16-byte aligned): one float4 per row of the stencil.  Each
+// wave stages its mismatches in its own LDS list (its fill level is wave-uniform, so
+// no LDS atomics and no workgroup barriers) and flushes it with one global atomic.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kMarkWaveStage = 1024;  // staged entries per wave
+constexpr int kMarkGroups = 2;            // float4 groups per thread and loop step
+
+__device__ __forceinline__ void wave_stage_flush(const uint32_t* lw, uint32_t& wcnt, uint32_t* __restrict__ list,
+                                                 uint32_t cap, uint32_t* gcount) {
+  const int lane = threadIdx.x & (kWave - 1);
+  wave_sync();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(gcount, wcnt);
+  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(base, 0, kWave));
+  for (uint32_t j = lane; j < wcnt; j += kWave)
+    if (base + j < cap) list[base + j] = lw[j];
+  wave_sync();
+  wcnt = 0;
+}
+
 template <int ND>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float* __restrict__ y,
                                                            const uint8_t* __restrict__ t,
                                                            const float* __restrict__ d, uint32_t n4,
                                                            uint32_t* __restrict__ bad, uint32_t cap,
                                                            uint32_t* __restrict__ misc) {
-  constexpr uint32_t kStage = 4096;
   constexpr int FA = ND - 1;  // fastest axis
+  constexpr int U = kMarkGroups;
   __shared__ uint32_t red[4][kWavesPerBlock];
-  __shared__ uint32_t lbuf[kStage];
-  __shared__ uint32_t lcnt, lbase;
-  if (threadIdx.x == 0) lcnt = 0;
-  __syncthreads();
+  __shared__ uint32_t lbuf[kWavesPerBlock][kMarkWaveStage];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  uint32_t* lw = lbuf[w];
+  uint32_t wcnt = 0;  // wave-uniform fill level of lw
   const float4* y4 = reinterpret_cast<const float4*>(y);
   const float4* d4 = reinterpret_cast<const float4*>(d);
   const uint32_t* t4 = reinterpret_cast<const uint32_t*>(t);
   const int dimz = dim_of<ND>(g, FA);
   uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
-  for (uint32_t b0 = blockIdx.x * kBlockThreads; b0 < n4; b0 += gridDim.x * kBlockThreads) {
-    const uint32_t gi = b0 + threadIdx.x;
-    bool flag[4] = {false, false, false, false};
-    const uint32_t e0 = 4 * gi;
-    if (gi < n4) {
-      int c[ND];
-      coords_of<ND>(g, e0, c);
-      bool rowin = true;
+  // the loop bound is uniform per workgroup, so every lane takes part in the ballots
+  for (uint32_t b0 = blockIdx.x * kBlockThreads * U; b0 < n4; b0 += gridDim.x * kBlockThreads * U) {
+    float4 v[U], lo4[U][ND - 1], hi4[U][ND - 1];
+    float zm[U], zp[U];
+    uint32_t tw[U];
+    int c[U][ND];
+    bool rowin[U], in[U];
 #pragma unroll
-      for (int a = 0; a < FA; a++) rowin = rowin && c[a] >= 1 && c[a] + 2 <= dim_of<ND>(g, a);
-      const float4 v = y4[gi];
-      const uint32_t tw = t4[gi];
-      float4 lo4[ND - 1], hi4[ND - 1];
-      float zm = 0.f, zp = 0.f;
-      if (rowin) {
+    for (int u = 0; u < U; u++) {
+      const uint32_t gi = b0 + u * kBlockThreads + threadIdx.x;
+      in[u] = gi < n4;
+      const uint32_t e0 = 4 * (in[u] ? gi : 0u);
+      coords_of<ND>(g, e0, c[u]);
+      rowin[u] = in[u];
+#pragma unroll
+      for (int a = 0; a < FA; a++) rowin[u] = rowin[u] && c[u][a] >= 1 && c[u][a] + 2 <= dim_of<ND>(g, a);
+      v[u] = in[u] ? y4[gi] : make_float4(0.f, 0.f, 0.f, 0.f);
+      tw[u] = in[u] ? t4[gi] : 0u;
+      zm[u] = zp[u] = 0.f;
+      if (rowin[u]) {
 #pragma unroll
         for (int a = 0; a < FA; a++) {
           const uint32_t s4 = stride_of<ND>(g, a) >> 2;
-          lo4[a] = y4[gi - s4];
-          hi4[a] = y4[gi + s4];
+          lo4[u][a] = y4[gi - s4];
+          hi4[u][a] = y4[gi + s4];
         }
-        if (c[FA] > 0) zm = y[e0 - 1];
-        if (c[FA] + 4 < dimz) zp = y[e0 + 4];
+        if (c[u][FA] > 0) zm[u] = y[e0 - 1];
+        if (c[u][FA] + 4 < dimz) zp[u] = y[e0 + 4];
       }
-      const float vv[4] = {v.x, v.y, v.z, v.w};
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t e0 = 4 * (b0 + u * kBlockThreads + threadIdx.x);
+      const float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+      bool flag[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int z = c[FA] + i;
+        const int z = c[u][FA] + i;
         int k = 0;
-        if (rowin && z >= 1 && z + 2 <= dimz) {
+        if (rowin[u] && z >= 1 && z + 2 <= dimz) {
           float nb[2 * ND];
 #pragma unroll
           for (int a = 0; a < FA; a++) {
-            const float l[4] = {lo4[a].x, lo4[a].y, lo4[a].z, lo4[a].w};
-            const float h[4] = {hi4[a].x, hi4[a].y, hi4[a].z, hi4[a].w};
+            const float l[4] = {lo4[u][a].x, lo4[u][a].y, lo4[u][a].z, lo4[u][a].w};
+            const float h[4] = {hi4[u][a].x, hi4[u][a].y, hi4[u][a].z, hi4[u][a].w};
             nb[2 * a] = l[i];
             nb[2 * a + 1] = h[i];
           }
-          nb[2 * FA] = i == 0 ? zm : vv[i - 1];
-          nb[2 * FA + 1] = i == 3 ? zp : vv[i + 1];
+          nb[2 * FA] = i == 0 ? zm[u] : vv[i - 1];
+          nb[2 * FA + 1] = i == 3 ? zp[u] : vv[i + 1];
           k = classify_vals<ND>(vv[i], nb);
         }
-        const int tt = (int)((tw >> (8 * i)) & 0xffu);
-        miss += (tt != 0 && k == 0);
-        spur += (tt == 0 && k != 0);
-        chg += (tt != 0 && k != 0 && k != tt);
-        flag[i] = k != tt;
+        const int tt = (int)((tw[u] >> (8 * i)) & 0xffu);
+        miss += (in[u] && tt != 0 && k == 0);
+        spur += (in[u] && tt == 0 && k != 0);
+        chg += (in[u] && tt != 0 && k != 0 && k != tt);
+        flag[i] = in[u] && k != tt;
       }
-      if (d != nullptr) {
-        const float4 dd = d4[gi];
+      if (d != nullptr && in[u]) {
+        const float4 dd = d4[b0 + u * kBlockThreads + threadIdx.x];
         const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          const uint32_t u = __float_as_uint(fabsf(vv[i] - dv[i]));
-          dev = u > dev ? u : dev;
+          const uint32_t q = __float_as_uint(fabsf(vv[i] - dv[i]));
+          dev = q > dev ? q : dev;
+        }
+      }
+      if (bad != nullptr) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint64_t m = __ballot(flag[i]);
+          if (flag[i]) lw[wcnt + (uint32_t)__popcll(m & lanemask_lt(lane))] = e0 + i;
+          wcnt += (uint32_t)__popcll(m);
         }
       }
     }
-    if (bad != nullptr) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) lds_append(flag[i], e0 + i, lbuf, &lcnt);
-      __syncthreads();
-      if (lcnt > kStage - 4 * kBlockThreads) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
-    }
+    if (bad != nullptr && wcnt > kMarkWaveStage - 4 * U * kWave) wave_stage_flush(lw, wcnt, bad, cap, &misc[kPpBad]);
   }
-  if (bad != nullptr) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+  if (bad != nullptr && wcnt) wave_stage_flush(lw, wcnt, bad, cap, &misc[kPpBad]);
   miss = seg_sum<kWave>(miss);
   spur = seg_sum<kWave>(spur);
   chg = seg_sum<kWave>(chg);
   dev = seg_max<kWave>(dev);
-  const int w = threadIdx.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) {
+  if (lane == 0) {
     red[0][w] = miss;
     red[1][w] = spur;
     red[2][w] = chg;
@@ -1501,7 +1585,7 @@ hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float*
   const bool vec = fast % 4 == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
                    (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0;
   if (vec) {
-    const dim3 grid(grid_stride_blocks(n / 4)), blk(kBlockThreads);
+    const dim3 grid(grid_stride_blocks((n / 4 + kMarkGroups - 1) / kMarkGroups)), blk(kBlockThreads);
     if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
     else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
     return hipGetLastError();

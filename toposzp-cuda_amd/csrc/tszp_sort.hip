@@ -159,6 +159,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
   }
 }
 
+size_t scan_status_words(uint32_t count) { return ceil_div(count, kSortTile); }
+
+hipError_t launch_scan_excl(uint32_t* a, uint32_t count, uint64_t* status, uint32_t* counter, uint32_t* err,
+                            hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_excl, dim3(ceil_div(count, kSortTile)), dim3(kBlockThreads), 0, st, a, count, status,
+                     counter, err);
+  return hipGetLastError();
+}
+
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
                             const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer) {
   if (count <= 1) return hipSuccess;

@@ -120,7 +120,7 @@ def lib():
 
 
 PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5,
-              "restore": 6, "scatter": 7, "solve": 8}
+              "restore": 6, "scatter": 7, "solve": 8, "query": 9}
 
 
 def prof_enable(on=True):
