@@ -115,6 +115,13 @@ CASES = [
     ("blobs3d_bs256", lambda: fields.blobs3d(30, 31, 37, seed=31, noise=1e-3), 1e-4, 256, 1),
     ("tiny3d", lambda: fields.blobs3d(3, 3, 3, seed=32, noise=0.1), 1e-3, 64, 1),
     ("const2d", lambda: np.full((40, 50), 1.25, np.float32), 1e-3, 64, 1),
+    # fastest axis a multiple of 4 and block size 32 or 64: 16-byte stencil path
+    ("vec2d_bs64", lambda: fields.blobs2d(96, 260, nblobs=24, seed=33), 1e-3, 64, 1),
+    ("vec2d_bs32_T5", lambda: fields.blobs2d(77, 132, seed=34, noise=0.03), 1e-3, 32, 5),
+    ("vec3d_bs64", lambda: fields.fbm3d(24, 36, 40, seed=35), 3e-3, 64, 1),
+    ("vec3d_bs32_T3", lambda: fields.fbm3d(17, 20, 28, seed=36), 2e-3, 32, 3),
+    ("vec3d_ragged_tail", lambda: fields.fbm3d(5, 7, 12, seed=37), 2e-3, 64, 1),
+    ("vec2d_ties", lambda: fields.ties2d(64, 128, seed=38), 1e-3, 64, 2),
 ]
 
 
@@ -277,3 +284,30 @@ def test_sort_positions_bin_path_large():
         del os.environ["TSZP_SORTPOS"]
     assert info2.ext_count == info.ext_count
     assert torch.equal(a, buf2.sp_values[:info.ext_count])
+
+
+@pytest.mark.parametrize("shape,bs", [((128, 256, 256), 64), ((40, 96, 100), 32), ((1024, 1028), 64)])
+def test_codec_fast_path_matches_generic(shape, bs):
+    """The 16-byte-stencil codec against the generic kernel on the same field: identical
+    topology, sort-position and plain streams."""
+    torch = _torch()
+    f = tz.gen_fbm(shape, seed=91) if len(shape) == 3 else torch.from_numpy(
+        fields.blobs2d(*shape, nblobs=40, seed=92, noise=0.05)).cuda()
+    eb, _ = tz.rel_to_abs(f, 1e-3)
+    outs = []
+    for generic in (False, True):
+        if generic:
+            os.environ["TSZP_CODEC_GENERIC"] = "1"
+        try:
+            info, buf = tz.topo_compress(f, eb, bs)
+            nbytes, pout, _ = tz.plain_compress(f, eb, bs)
+            torch.cuda.synchronize()
+            outs.append((info.topo_bytes, info.sortpos_bytes, buf.topo[:info.topo_bytes].cpu().numpy().copy(),
+                         buf.sp[:info.sortpos_bytes].cpu().numpy().copy(), pout[:nbytes].cpu().numpy().copy()))
+        finally:
+            os.environ.pop("TSZP_CODEC_GENERIC", None)
+    (a, b) = outs
+    assert a[0] == b[0] and a[1] == b[1]
+    _eq(a[2], b[2], "topology stream")
+    _eq(a[3], b[3], "sort-position stream")
+    _eq(a[4], b[4], "plain stream")

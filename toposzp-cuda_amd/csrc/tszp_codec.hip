@@ -13,6 +13,8 @@
 // Work decomposition: a workgroup (4 waves) owns a tile of BPT consecutive SZp
 // blocks.  Within a wave, one "step" covers G = 64 / S blocks, each block spread
 // over S lanes with EPL consecutive elements per lane (S * EPL >= blockSize).
+#include <cstdlib>
+
 #include "tszp_device.hpp"
 #include "tszp_kernels.h"
 
@@ -499,6 +501,399 @@ __global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// compression, fast path for block sizes 32 and 64: S = bs / 4 lanes per block and
+// four consecutive elements per lane, loaded as one 16-byte vector; with the fastest
+// axis a multiple of 4 long, the classification stencil rows are 16-byte loads too and
+// the neighbours along the fastest axis come from the adjacent lanes.  Each lane packs
+// its four elements as whole runs: one nibble of sign bits, one run of full magnitude
+// bytes, one run of remainder bits and one byte of types, so a lane issues a handful
+// of LDS ORs per four elements.  Same stream bytes as k_compress.
+// ---------------------------------------------------------------------------
+// OR a little-endian run of up to 16 bytes (w0..w3) at byte address a of the stage.
+__device__ __forceinline__ void lds_or_run16(uint32_t* words, unsigned a, uint32_t w0, uint32_t w1, uint32_t w2,
+                                             uint32_t w3) {
+  const unsigned sh = a & 3u, k = a >> 2;
+  uint32_t o0 = w0, o1 = w1, o2 = w2, o3 = w3, o4 = 0;
+  if (sh) {
+    o0 = w0 << (8 * sh);
+    o1 = __builtin_amdgcn_alignbyte(w1, w0, 4 - sh);
+    o2 = __builtin_amdgcn_alignbyte(w2, w1, 4 - sh);
+    o3 = __builtin_amdgcn_alignbyte(w3, w2, 4 - sh);
+    o4 = w3 >> (32 - 8 * sh);
+  }
+  if (o0) atomicOr(&words[k], o0);
+  if (o1) atomicOr(&words[k + 1], o1);
+  if (o2) atomicOr(&words[k + 2], o2);
+  if (o3) atomicOr(&words[k + 3], o3);
+  if (o4) atomicOr(&words[k + 4], o4);
+}
+
+// critical-point type from the centre and the two neighbours along each axis
+template <int ND>
+__device__ __forceinline__ int type_minmax(float c, const float (&lo)[ND], const float (&hi)[ND]) {
+  bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    const bool h = c > fmaxf(lo[k], hi[k]);
+    const bool l = c < fminf(lo[k], hi[k]);
+    all_gt = all_gt && h;
+    all_lt = all_lt && l;
+    any_hi = any_hi || h;
+    any_lo = any_lo || l;
+  }
+  return all_gt ? 1 : all_lt ? 2 : (any_hi && any_lo) ? 3 : 0;
+}
+
+template <int MODE, int S, int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
+  constexpr int EPL = 4, BS = 4 * S;
+  using Gm = Geom<S, EPL>;
+  constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr bool CLASSIFY = (MODE == kModeTopoClassify);
+  constexpr int STAGE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 8;  // runs may spill 16 zero bytes
+
+  __shared__ uint32_t stage[STAGE_WORDS];
+  __shared__ uint32_t blk_off[BPT + 1];
+  __shared__ uint32_t step_base[kWavesPerBlock * SPW];  // cp | ext << 16 (tile-local prefix)
+  __shared__ uint64_t sh_prefix[2];
+  __shared__ uint32_t sh_tile;
+
+  const int tid = threadIdx.x;
+  const int w = tid / kWave, lane = tid & (kWave - 1);
+  const int g = lane / S, ls = lane & (S - 1);
+  const uint32_t j0 = 4u * (uint32_t)ls;  // first element of the lane within its block
+
+  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
+  __syncthreads();
+  const uint32_t tile = sh_tile;
+
+  auto block_of = [&](int s) { return tile * BPT + (uint32_t)((w * SPW + s) * G + g); };
+  auto cur_of = [&](uint32_t b) { return b < a.nb ? min((uint32_t)BS, a.n - b * BS) : 0u; };
+
+  // ---- A1: the lane's 4 elements of every step as 16-byte loads ---------------------
+  uint32_t raw[SPW][4];
+  uint32_t tbits = 0;
+  const uint32_t* in32 = reinterpret_cast<const uint32_t*>(a.in);
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t b = block_of(s), cur = cur_of(b);
+    const uint32_t e0 = b * BS + j0;
+    if (j0 + 3 < cur) {
+      const uint4 v = reinterpret_cast<const uint4*>(a.in)[e0 >> 2];
+      raw[s][0] = v.x;
+      raw[s][1] = v.y;
+      raw[s][2] = v.z;
+      raw[s][3] = v.w;
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; r++) raw[s][r] = j0 + r < cur ? in32[e0 + r] : 0u;
+    }
+    if constexpr (MODE == kModeTopoMap) {
+      uint32_t tw = 0;
+      if (j0 + 3 < cur) {
+        tw = reinterpret_cast<const uint32_t*>(a.tmap)[e0 >> 2];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) tw |= (j0 + r < cur ? (uint32_t)a.tmap[e0 + r] : 0u) << (8 * r);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) tbits |= ((tw >> (8 * r)) & 3u) << (2 * (s * 4 + r));
+    }
+  }
+
+  // ---- A2: classification ------------------------------------------------------------
+  if constexpr (CLASSIFY) {
+    const Grid& gr = a.grid;
+    const uint32_t dz = ND == 3 ? gr.d2 : gr.d1;          // fastest axis length (multiple of 4)
+    const uint32_t s1 = ND == 3 ? gr.plane : gr.d1;      // stride of the slowest axis
+    const uint4* f4 = reinterpret_cast<const uint4*>(a.in);
+#pragma unroll
+    for (int s = 0; s < SPW; s++) {
+      const uint32_t b = block_of(s), cur = cur_of(b);
+      const uint32_t e0 = b * BS + j0;
+      const bool act = j0 < cur;  // n is a multiple of 4: all four elements or none
+      uint32_t z;
+      bool rowok;
+      if constexpr (ND == 3) {
+        const uint32_t x = fdiv(e0, gr.div_plane), rr = e0 - x * gr.plane;
+        const uint32_t y = fdiv(rr, gr.div_row);
+        z = rr - y * gr.d2;
+        rowok = act && x >= 1 && x + 1 < gr.d0 && y >= 1 && y + 1 < gr.d1;
+      } else {
+        const uint32_t i = fdiv(e0, gr.div_row);
+        z = e0 - i * gr.d1;
+        rowok = act && i >= 1 && i + 1 < gr.d0;
+      }
+      // neighbours along the fastest axis: the adjacent lanes hold elements e0 - 1 and
+      // e0 + 4 (blocks of a step are consecutive); the wave's end lanes load them
+      uint32_t zm = __shfl_up(raw[s][3], 1, kWave), zp = __shfl_down(raw[s][0], 1, kWave);
+      if (lane == 0 && rowok && z >= 1) zm = in32[e0 - 1];
+      if (lane == kWave - 1 && rowok && z + 4 < dz) zp = in32[e0 + 4];
+      uint4 nlo[ND - 1], nhi[ND - 1];
+      if (rowok) {
+        nlo[0] = f4[(e0 - s1) >> 2];
+        nhi[0] = f4[(e0 + s1) >> 2];
+        if constexpr (ND == 3) {
+          nlo[1] = f4[(e0 - gr.d2) >> 2];
+          nhi[1] = f4[(e0 + gr.d2) >> 2];
+        }
+      }
+      if (rowok) {
+        const float c[4] = {__uint_as_float(raw[s][0]), __uint_as_float(raw[s][1]), __uint_as_float(raw[s][2]),
+                            __uint_as_float(raw[s][3])};
+        const float zl[4] = {__uint_as_float(zm), c[0], c[1], c[2]};
+        const float zh[4] = {c[1], c[2], c[3], __uint_as_float(zp)};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (z + r < 1 || z + r + 1 >= dz) continue;
+          float lo[ND], hi[ND];
+#pragma unroll
+          for (int k = 0; k < ND - 1; k++) {
+            const uint32_t l4[4] = {nlo[k].x, nlo[k].y, nlo[k].z, nlo[k].w};
+            const uint32_t h4[4] = {nhi[k].x, nhi[k].y, nhi[k].z, nhi[k].w};
+            lo[k] = __uint_as_float(l4[r]);
+            hi[k] = __uint_as_float(h4[r]);
+          }
+          lo[ND - 1] = zl[r];
+          hi[ND - 1] = zh[r];
+          tbits |= (uint32_t)type_minmax<ND>(c[r], lo, hi) << (2 * (s * 4 + r));
+        }
+      }
+    }
+  }
+
+  // ---- A3: quantize once; block bit counts, sizes and delta magnitudes ---------------
+  uint32_t mg[SPW][4];
+  uint32_t nmask = 0;  // sign bits, bit s*4+r
+  uint32_t bcs[SPW];
+  int q0s[SPW];
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t cur = cur_of(tile * BPT + lb);
+    int q[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) q[r] = j0 + r < cur ? quantize_raw<MODE>(a, raw[s][r]) : 0;
+    const int nxt = __shfl_down(q[0], 1, kWave);
+    unsigned m = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int qn = r < 3 ? q[r + 1] : nxt;
+      const unsigned d = (unsigned)qn - (unsigned)q[r];
+      const bool neg = (int)d < 0;
+      const bool ok = j0 + r + 1 < cur;
+      const unsigned mag = ok ? (neg ? 0u - d : d) : 0u;
+      mg[s][r] = mag;
+      nmask |= (ok && neg ? 1u : 0u) << (s * 4 + r);
+      m = mag > m ? mag : m;
+    }
+    m = seg_max<S>(m);
+    const unsigned bc = bitcount(m);
+    bcs[s] = bc;
+    q0s[s] = q[0];
+    if (ls == 0) blk_off[lb] = cur ? block_bytes(cur, bc, TOPO) : 0u;
+    if constexpr (CLASSIFY) {
+      unsigned step_cp = 0, step_ext = 0;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const unsigned t = (tbits >> (2 * (s * 4 + r))) & 3u;
+        step_cp += t != 0;
+        step_ext += (t == 1 || t == 2);
+      }
+      const unsigned tot = seg_sum<kWave>(step_cp | (step_ext << 16));
+      if (lane == 0) step_base[w * SPW + s] = tot;
+    }
+  }
+  __syncthreads();
+
+  // ---- B: tile scans + look-back (wave 0) ---------------------------------------------
+  if (w == 0) {
+    constexpr int PER = (BPT + kWave - 1) / kWave;
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = lane * PER + k;
+      loc[k] = i < BPT ? blk_off[i] : 0u;
+      sum += loc[k];
+    }
+    const uint32_t incl = seg_incl_scan<kWave>(sum, lane);
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = lane * PER + k;
+      if (i < BPT) blk_off[i] = run;
+      run += loc[k];
+    }
+    const uint32_t tile_bytes = __shfl(incl, kWave - 1, kWave);
+    if (lane == 0) blk_off[BPT] = tile_bytes;
+    uint64_t cnt_agg = 0;
+    if constexpr (CLASSIFY) {
+      constexpr int NS = kWavesPerBlock * SPW;
+      const uint32_t v = lane < NS ? step_base[lane] : 0u;
+      const uint32_t inc = seg_incl_scan<kWave>(v, lane);
+      if (lane < NS) step_base[lane] = inc - v;
+      const uint32_t tot = __shfl(inc, kWave - 1, kWave);
+      cnt_agg = (uint64_t)(tot & 0xffffu) | ((uint64_t)(tot >> 16) << 31);
+    }
+    const uint64_t pb = lookback(a.st_bytes, tile, tile_bytes, &a.res->error);
+    uint64_t pc = 0;
+    if constexpr (CLASSIFY) pc = lookback(a.st_cnt, tile, cnt_agg, &a.res->error);
+    if (lane == 0) {
+      sh_prefix[0] = pb;
+      sh_prefix[1] = pc;
+      if (tile == a.ntiles - 1) {
+        a.res->payload_bytes = pb + tile_bytes;
+        if constexpr (CLASSIFY) {
+          const uint64_t tc = pc + cnt_agg;
+          a.res->cp_count = tc & 0x7fffffffull;
+          a.res->ext_count = (tc >> 31) & 0x7fffffffull;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- C: pack the lane's runs into the LDS stage ---------------------------------------
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
+    const uint32_t cur = cur_of(tile * BPT + lb);
+    const unsigned bc = bcs[s];
+    // sign nibble of elements j0..j0+3 (MSB first); the even lane of a pair writes the byte
+    const unsigned nib = ((nmask >> (s * 4)) & 1u) << 3 | ((nmask >> (s * 4 + 1)) & 1u) << 2 |
+                         ((nmask >> (s * 4 + 2)) & 1u) << 1 | ((nmask >> (s * 4 + 3)) & 1u);
+    const unsigned nib_next = __shfl_down(nib, 1, kWave);
+    if (cur == 0) continue;  // uniform per block
+    const unsigned off = blk_off[lb];
+    const unsigned n = cur - 1u;
+    if (ls == 0) {
+      lds_or_bytes(stage, off, (uint32_t)q0s[s], 4);
+      lds_or_bytes(stage, off + 4, bc, 1);
+    }
+    const unsigned sB = bc ? (n + 7u) >> 3 : 0u;
+    if (bc) {
+      if ((ls & 1) == 0 && (unsigned)(ls >> 1) < sB) {
+        const unsigned byte = (nib << 4) | (ls + 1 < S ? nib_next : 0u);
+        if (byte) lds_or_bytes(stage, off + 5u + (ls >> 1), byte, 1);
+      }
+      const unsigned nfull = bc >> 3, rem = bc & 7u;
+      const unsigned fbase = off + 5u + sB;
+      if (nfull) {
+        const uint32_t v0 = mg[s][0] >> rem, v1 = mg[s][1] >> rem, v2 = mg[s][2] >> rem, v3 = mg[s][3] >> rem;
+        uint32_t w0, w1 = 0, w2 = 0, w3 = 0;
+        if (nfull == 1) {
+          w0 = v0 | v1 << 8 | v2 << 16 | v3 << 24;
+        } else if (nfull == 2) {
+          w0 = v0 | v1 << 16;
+          w1 = v2 | v3 << 16;
+        } else if (nfull == 3) {
+          w0 = v0 | v1 << 24;
+          w1 = v1 >> 8 | v2 << 16;
+          w2 = v2 >> 16 | v3 << 8;
+        } else {
+          w0 = v0;
+          w1 = v1;
+          w2 = v2;
+          w3 = v3;
+        }
+        if (j0 < n) lds_or_run16(stage, fbase + j0 * nfull, w0, w1, w2, w3);
+      }
+      if (rem) {
+        const uint32_t rm = (1u << rem) - 1u;
+        const uint32_t R = ((((mg[s][0] & rm) << rem | (mg[s][1] & rm)) << rem | (mg[s][2] & rm)) << rem) |
+                           (mg[s][3] & rm);
+        const unsigned P = j0 * rem;                   // bit offset of the run
+        const unsigned B = fbase + nfull * n + (P >> 3), sh = P & 7u;
+        const uint32_t be = R << (32u - sh - 4u * rem);  // big-endian window at byte B
+        if (j0 < n && be) {
+          const uint32_t le = __builtin_bswap32(be);
+          lds_or_run16(stage, B, le, 0u, 0u, 0u);
+        }
+      }
+    }
+    if constexpr (TOPO) {
+      const unsigned tb = (tbits >> (8 * s)) & 0xffu;  // types of elements j0..j0+3, 2 bits each
+      // MSB-first: element j0 in the top two bits of the lane's byte
+      const unsigned byte = (tb & 3u) << 6 | ((tb >> 2) & 3u) << 4 | ((tb >> 4) & 3u) << 2 | ((tb >> 6) & 3u);
+      const unsigned tbase = off + 5u + (bc ? sB + mag_bytes(n, bc) : 0u);
+      if (byte && j0 < cur) lds_or_bytes(stage, tbase + ls, byte, 1);
+    }
+  }
+  __syncthreads();
+
+  // ---- D: stage -> stream, block offsets side-car ----------------------------------------
+  const uint64_t g0 = sh_prefix[0];
+  const uint32_t len = blk_off[BPT];
+  if (g0 + len > a.cap) {
+    if (tid == 0) atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
+  } else if (len) {
+    uint8_t* dst = a.payload;
+    const uint64_t a0 = (g0 + 3) & ~3ull, e0 = g0 + len;
+    const uint64_t a1 = e0 & ~3ull;
+    if (a0 >= e0 || a1 <= a0) {
+      for (uint32_t i = tid; i < len; i += kBlockThreads) dst[g0 + i] = (uint8_t)lds_byte(stage, i);
+    } else {
+      const uint32_t head = (uint32_t)(a0 - g0), tail = (uint32_t)(e0 - a1);
+      if (tid < (int)head) dst[g0 + tid] = (uint8_t)lds_byte(stage, tid);
+      if (tid < (int)tail) dst[a1 + tid] = (uint8_t)lds_byte(stage, (uint32_t)(a1 - g0) + tid);
+      const uint32_t nw = (uint32_t)((a1 - a0) >> 2);
+      uint32_t* dw = reinterpret_cast<uint32_t*>(dst + a0);
+      for (uint32_t i = tid; i < nw; i += kBlockThreads) {
+        // output word i = stage bytes head + 4i .. head + 4i + 3
+        const uint32_t lo = stage[i], hi = stage[i + 1];
+        dw[i] = head ? __builtin_amdgcn_alignbyte(hi, lo, head) : lo;
+      }
+    }
+  }
+  for (int lb = tid; lb < BPT; lb += kBlockThreads) {
+    const uint32_t b = tile * BPT + lb;
+    if (b < a.nb) a.blk[b] = (uint32_t)(g0 + blk_off[lb]);
+  }
+  if (tile == a.ntiles - 1 && tid == 0) a.blk[a.nb] = (uint32_t)(g0 + len);
+
+  // ---- E: critical-point records in row-major order ---------------------------------------
+  if constexpr (CLASSIFY) {
+    if (a.cp_key) {
+      const uint64_t pc = sh_prefix[1];
+      const uint32_t cp0 = (uint32_t)(pc & 0x7fffffffull), ext0 = (uint32_t)((pc >> 31) & 0x7fffffffull);
+#pragma unroll
+      for (int s = 0; s < SPW; s++) {
+        unsigned c = 0, x = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const unsigned t = (tbits >> (2 * (s * 4 + r))) & 3u;
+          c += t != 0;
+          x += (t == 1 || t == 2);
+        }
+        const unsigned packed = c | (x << 16);
+        const unsigned incl = seg_incl_scan<kWave>(packed, lane);
+        const unsigned excl = incl - packed;
+        const unsigned sb = step_base[w * SPW + s];
+        uint32_t pcp = cp0 + (sb & 0xffffu) + (excl & 0xffffu);
+        uint32_t pex = ext0 + (sb >> 16) + (excl >> 16);
+        if (!c) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const unsigned t = (tbits >> (2 * (s * 4 + r))) & 3u;
+          if (!t) continue;
+          if (pcp < a.cp_cap) {
+            a.cp_key[pcp] = okey(__uint_as_float(raw[s][r]));
+            a.cp_pay[pcp] = (t == 3) ? 0xffffffffu : pex;
+          } else {
+            atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
+          }
+          pcp++;
+          if (t != 3) pex++;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side dispatch on block size
 // ---------------------------------------------------------------------------
 template <int MODE>
@@ -552,10 +947,47 @@ void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* bpt) {
   *bpt = (uint32_t)(kWavesPerBlock * spw * (kWave / s));
 }
 
+// The fast path (k_compress_v4) takes block sizes 32 and 64 with a 16-byte aligned
+// input; classification also needs the fastest axis a multiple of 4 long, the type map
+// 4-byte alignment.
+template <int MODE, int S>
+static hipError_t launch_v4(const CompressArgs& a, hipStream_t st) {
+  if constexpr (MODE == kModeTopoClassify) {
+    if (a.grid.ndim == 3) {
+      hipLaunchKernelGGL((k_compress_v4<MODE, S, 3>), dim3(a.ntiles), dim3(kBlockThreads), 0, st, a);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_compress_v4<MODE, S, 2>), dim3(a.ntiles), dim3(kBlockThreads), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int MODE>
+static bool try_v4(const CompressArgs& a, hipStream_t st, hipError_t* e) {
+  if ((a.bs != 32 && a.bs != 64) || (reinterpret_cast<uintptr_t>(a.in) & 15u)) return false;
+  if (MODE == kModeTopoClassify) {
+    const Grid& g = a.grid;
+    if (g.ndim != 2 && g.ndim != 3) return false;
+    if ((g.ndim == 3 ? g.d2 : g.d1) % 4u) return false;
+  }
+  if (MODE == kModeTopoMap && (reinterpret_cast<uintptr_t>(a.tmap) & 3u)) return false;
+  *e = a.bs == 64 ? launch_v4<MODE, 16>(a, st) : launch_v4<MODE, 8>(a, st);
+  return true;
+}
+
 hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st) {
   int S, EPL;
   uint32_t bpt;
   block_geometry(a.bs, &S, &EPL, &bpt);
+  hipError_t e = hipSuccess;
+  if (!getenv("TSZP_CODEC_GENERIC")) {
+    switch (mode) {
+      case kModeTopoClassify: if (try_v4<kModeTopoClassify>(a, st, &e)) return e; break;
+      case kModeTopoMap: if (try_v4<kModeTopoMap>(a, st, &e)) return e; break;
+      case kModePlain: if (try_v4<kModePlain>(a, st, &e)) return e; break;
+      case kModeInt: if (try_v4<kModeInt>(a, st, &e)) return e; break;
+    }
+  }
   switch (mode) {
     case kModeTopoClassify: return launch_compress_mode<kModeTopoClassify>(a, S, EPL, st);
     case kModeTopoMap: return launch_compress_mode<kModeTopoMap>(a, S, EPL, st);
