@@ -48,7 +48,7 @@ enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
-  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits,
+  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap,
   kWsCount
 };
 
@@ -809,10 +809,14 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   // The list `bad` is exact when it holds every current mismatch; the verification
   // counts in misc are current when the last full pass saw the present field.
   bool exact = false, verified = false;
-  auto full_mark = [&](uint32_t* nb_out, bool final_pass) -> int {
+  uint32_t* part;
+  WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + pp_tiny_parts(cap) + 64) * 4, part);
+  const bool vec_mark = (ndim == 2 ? dims[1] : dims[2]) % 4 == 0;
+  uint8_t* badmap = nullptr;
+  auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint8_t* bmap = nullptr) -> int {
     HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
     HIPCHK(launch_fill_u32(misc + kPpVerify, 0u, 8, st));
-    HIPCHK(pp_mark(g, d_field, d_types, final_pass ? d : nullptr, N, bad, cap, misc, st));
+    HIPCHK(pp_mark(g, d_field, d_types, final_pass ? d : nullptr, N, bad, cap, part, bmap, misc, st));
     int rr = read_misc();
     if (rr) return rr;
     *nb_out = h[kPpBad];
@@ -821,7 +825,12 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     return TSZP_OK;
   };
   uint32_t nb = 0;
-  if ((r = full_mark(&nb, false))) return r;
+  WS(kWsPpBadmap, n + 16, badmap);
+  if (vec_mark) {
+    if ((r = full_mark(&nb, false, badmap))) return r;
+  } else {
+    if ((r = full_mark(&nb, false))) return r;
+  }
   const bool stats = getenv("TSZP_RESTORE_STATS") != nullptr;
   if (stats) fprintf(stderr, "[restore] after init: %u mismatches\n", nb);
   if (use_sp && h[kPpExt] != n_ext)
@@ -831,9 +840,17 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   uint32_t tiny = 0;
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
   if (nb && nb <= cap) {
-    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
     HIPCHK(launch_fill_u32(misc + kPpTiny, 0u, 1, st));
-    HIPCHK(pp_tiny(g, bad, nb, d, d_types, d_field, eb, use_sp, bits, misc, st));
+    uint4* plist;
+    uint16_t* rel;
+    WS(kWsPpPlist, (size_t)nb * 16, plist);
+    WS(kWsPpRel, n * 2 + 16, rel);
+    if (!vec_mark) {  // the mismatch map from the list
+      HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(badmap), 0u, (n + 3) / 4, st));
+      HIPCHK(pp_badmap(bad, nb, badmap, st));
+    }
+    HIPCHK(pp_rel(g, d, N, rel, st));
+    HIPCHK(pp_tiny(g, bad, nb, d, rel, badmap, d_types, d_field, eb, use_sp, plist, part, misc, st));
     HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
     HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
     HIPCHK(pp_remark(g, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
@@ -964,7 +981,9 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   uint32_t* misc;
   WS(kWsPpMisc, kPpMiscWords * 4, misc);
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
-  HIPCHK(pp_mark(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, nullptr, 0, misc, st));
+  uint32_t* part;
+  WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + 64) * 4, part);
+  HIPCHK(pp_mark(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, nullptr, 0, part, nullptr, misc, st));
   uint64_t hv[4];
   HIPCHK(hipMemcpyAsync(hv, misc + kPpVerify, 32, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

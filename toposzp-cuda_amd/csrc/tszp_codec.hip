@@ -628,17 +628,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
       }
       // neighbours along the fastest axis: the adjacent lanes hold elements e0 - 1 and
       // e0 + 4 (blocks of a step are consecutive); the wave's end lanes load them
-      uint32_t zm = __shfl_up(raw[s][3], 1, kWave), zp = __shfl_down(raw[s][0], 1, kWave);
+      uint32_t zm = dpp_mov<kDppWaveShr1>(raw[s][3]), zp = dpp_mov<kDppWaveShl1>(raw[s][0]);
       if (lane == 0 && rowok && z >= 1) zm = in32[e0 - 1];
       if (lane == kWave - 1 && rowok && z + 4 < dz) zp = in32[e0 + 4];
+      // unconditional 16-byte loads; lanes without a stencil read vector 0 (unused)
       uint4 nlo[ND - 1], nhi[ND - 1];
-      if (rowok) {
-        nlo[0] = f4[(e0 - s1) >> 2];
-        nhi[0] = f4[(e0 + s1) >> 2];
-        if constexpr (ND == 3) {
-          nlo[1] = f4[(e0 - gr.d2) >> 2];
-          nhi[1] = f4[(e0 + gr.d2) >> 2];
-        }
+      nlo[0] = f4[rowok ? (e0 - s1) >> 2 : 0u];
+      nhi[0] = f4[rowok ? (e0 + s1) >> 2 : 0u];
+      if constexpr (ND == 3) {
+        nlo[1] = f4[rowok ? (e0 - gr.d2) >> 2 : 0u];
+        nhi[1] = f4[rowok ? (e0 + gr.d2) >> 2 : 0u];
       }
       if (rowok) {
         const float c[4] = {__uint_as_float(raw[s][0]), __uint_as_float(raw[s][1]), __uint_as_float(raw[s][2]),
@@ -676,7 +675,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
     int q[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) q[r] = j0 + r < cur ? quantize_raw<MODE>(a, raw[s][r]) : 0;
-    const int nxt = __shfl_down(q[0], 1, kWave);
+    const int nxt = (int)dpp_mov<kDppWaveShl1>((uint32_t)q[0]);
     unsigned m = 0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -689,7 +688,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
       nmask |= (ok && neg ? 1u : 0u) << (s * 4 + r);
       m = mag > m ? mag : m;
     }
-    m = seg_max<S>(m);
+    m = seg_max_dpp<S>(m);
     const unsigned bc = bitcount(m);
     bcs[s] = bc;
     q0s[s] = q[0];
@@ -702,7 +701,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
         step_cp += t != 0;
         step_ext += (t == 1 || t == 2);
       }
-      const unsigned tot = seg_sum<kWave>(step_cp | (step_ext << 16));
+      const unsigned tot = wave_sum_dpp(step_cp | (step_ext << 16));
       if (lane == 0) step_base[w * SPW + s] = tot;
     }
   }
@@ -719,7 +718,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
       loc[k] = i < BPT ? blk_off[i] : 0u;
       sum += loc[k];
     }
-    const uint32_t incl = seg_incl_scan<kWave>(sum, lane);
+    const uint32_t incl = wave_incl_scan_dpp(sum, lane);
     uint32_t run = incl - sum;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
@@ -727,15 +726,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
       if (i < BPT) blk_off[i] = run;
       run += loc[k];
     }
-    const uint32_t tile_bytes = __shfl(incl, kWave - 1, kWave);
+    const uint32_t tile_bytes = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
     if (lane == 0) blk_off[BPT] = tile_bytes;
     uint64_t cnt_agg = 0;
     if constexpr (CLASSIFY) {
       constexpr int NS = kWavesPerBlock * SPW;
       const uint32_t v = lane < NS ? step_base[lane] : 0u;
-      const uint32_t inc = seg_incl_scan<kWave>(v, lane);
+      const uint32_t inc = wave_incl_scan_dpp(v, lane);
       if (lane < NS) step_base[lane] = inc - v;
-      const uint32_t tot = __shfl(inc, kWave - 1, kWave);
+      const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
       cnt_agg = (uint64_t)(tot & 0xffffu) | ((uint64_t)(tot >> 16) << 31);
     }
     const uint64_t pb = lookback(a.st_bytes, tile, tile_bytes, &a.res->error);
@@ -765,7 +764,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
     // sign nibble of elements j0..j0+3 (MSB first); the even lane of a pair writes the byte
     const unsigned nib = ((nmask >> (s * 4)) & 1u) << 3 | ((nmask >> (s * 4 + 1)) & 1u) << 2 |
                          ((nmask >> (s * 4 + 2)) & 1u) << 1 | ((nmask >> (s * 4 + 3)) & 1u);
-    const unsigned nib_next = __shfl_down(nib, 1, kWave);
+    const unsigned nib_next = dpp_mov<kDppWaveShl1>(nib);
     if (cur == 0) continue;  // uniform per block
     const unsigned off = blk_off[lb];
     const unsigned n = cur - 1u;
@@ -869,7 +868,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
           x += (t == 1 || t == 2);
         }
         const unsigned packed = c | (x << 16);
-        const unsigned incl = seg_incl_scan<kWave>(packed, lane);
+        const unsigned incl = wave_incl_scan_dpp(packed, lane);
         const unsigned excl = incl - packed;
         const unsigned sb = step_base[w * SPW + s];
         uint32_t pcp = cp0 + (sb & 0xffffu) + (excl & 0xffffu);

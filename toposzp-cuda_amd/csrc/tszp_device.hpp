@@ -120,6 +120,57 @@ __device__ __forceinline__ T seg_incl_scan(T v, int lane) {
   return v;
 }
 
+// DPP lane moves (GFX9 encodings; lane layouts checked on gfx950).  The result of
+// lane i is v of the source lane; lanes without a source (or inactive ones) read 0.
+// Call with the whole wave active.
+constexpr int kDppQuad1032 = 0xB1, kDppQuad2301 = 0x4E, kDppRowHalfMirror = 0x141, kDppRowMirror = 0x140;
+constexpr int kDppWaveShr1 = 0x138;  // lane i <- lane i - 1
+constexpr int kDppWaveShl1 = 0x130;  // lane i <- lane i + 1
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+}
+
+// maximum / sum over aligned segments of S <= 16 lanes, in every lane of the segment
+template <int S>
+__device__ __forceinline__ uint32_t seg_max_dpp(uint32_t v) {
+  static_assert(S >= 1 && S <= 16 && (S & (S - 1)) == 0, "segment of 1..16 lanes");
+  if constexpr (S >= 2) v = max(v, dpp_mov<kDppQuad1032>(v));
+  if constexpr (S >= 4) v = max(v, dpp_mov<kDppQuad2301>(v));
+  if constexpr (S >= 8) v = max(v, dpp_mov<kDppRowHalfMirror>(v));
+  if constexpr (S >= 16) v = max(v, dpp_mov<kDppRowMirror>(v));
+  return v;
+}
+template <int S>
+__device__ __forceinline__ uint32_t seg_sum_dpp(uint32_t v) {
+  static_assert(S >= 1 && S <= 16 && (S & (S - 1)) == 0, "segment of 1..16 lanes");
+  if constexpr (S >= 2) v += dpp_mov<kDppQuad1032>(v);
+  if constexpr (S >= 4) v += dpp_mov<kDppQuad2301>(v);
+  if constexpr (S >= 8) v += dpp_mov<kDppRowHalfMirror>(v);
+  if constexpr (S >= 16) v += dpp_mov<kDppRowMirror>(v);
+  return v;
+}
+// sum over the wave (wave-uniform)
+__device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t v) {
+  v = seg_sum_dpp<16>(v);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) + (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+         (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+// inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v, int lane) {
+  v += dpp_mov<kDppRowShr1>(v);
+  v += dpp_mov<kDppRowShr2>(v);
+  v += dpp_mov<kDppRowShr4>(v);
+  v += dpp_mov<kDppRowShr8>(v);
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const int row = lane >> 4;
+  return v + (row >= 1 ? r0 : 0u) + (row >= 2 ? r1 : 0u) + (row >= 3 ? r2 : 0u);
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }

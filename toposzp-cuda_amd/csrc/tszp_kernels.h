@@ -173,16 +173,27 @@ uint32_t pp_init_tiles(uint32_t n);
 hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
 hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
                    float* y, uint64_t* status, uint32_t* misc, hipStream_t st);
+// full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
+// badmap (nullable) receives one byte per point, 1 where mismatched (fastest axis a
+// multiple of 4 only)
 hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
-                   uint32_t cap, uint32_t* misc, hipStream_t st);
+                   uint32_t cap, uint32_t* part, uint8_t* badmap, uint32_t* misc, hipStream_t st);
+uint32_t pp_mark_parts();
 hipError_t pp_active(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
                      uint32_t* bits, uint32_t* act, uint32_t cap, int anchor, uint32_t* misc, hipStream_t st);
 hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
                    const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st);
 hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st);
-hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
-                   float eb, int anchor, uint32_t* bits, uint32_t* misc, hipStream_t st);
+// relation masks of the decoded values (bit k: same value at neighbour k, bit 8+k:
+// smaller value at neighbour k; neighbours inside the field only)
+hipError_t pp_rel(const Grid& g, const float* d, uint32_t n, uint16_t* rel, hipStream_t st);
+hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
+                   const uint8_t* badmap, const uint8_t* t, float* y, float eb, int anchor, uint4* plist,
+                   uint32_t* part, uint32_t* misc, hipStream_t st);
+uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
+// mismatch map (one byte per point) from a list; the map must be zeroed first
+hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
 hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t ty, uint32_t tz, uint32_t* pbits,
                     uint32_t* tbits, uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);

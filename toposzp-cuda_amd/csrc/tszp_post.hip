@@ -38,6 +38,27 @@
 namespace tszp {
 
 // ---------------------------------------------------------------------------------
+// XCD-aware split of a streaming pass.  Workgroups are dispatched round-robin over the
+// 8 XCDs (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Giving every
+// XCD one contiguous eighth of the items, swept in order by its workgroups, keeps the
+// stencil neighbours one row or one plane away in the L2 that loaded them.  Items are
+// handled in chunks of `chunk`; the grid is a multiple of 8.
+// ---------------------------------------------------------------------------------
+constexpr uint32_t kXcds = 8;
+struct XcdSweep {
+  uint32_t begin, end, step;
+};
+__device__ __forceinline__ XcdSweep xcd_sweep(uint32_t n, uint32_t chunk) {
+  const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+  const uint32_t nch = (n + chunk - 1) / chunk, chx = (nch + kXcds - 1) / kXcds;  // chunks per XCD
+  XcdSweep r;
+  r.begin = (x * chx + j) * chunk;
+  r.end = min((x + 1) * chx * chunk, n);
+  r.step = per * chunk;
+  return r;
+}
+
+// ---------------------------------------------------------------------------------
 // geometry
 // ---------------------------------------------------------------------------------
 template <int ND>
@@ -448,16 +469,22 @@ __device__ __forceinline__ void wave_stage_flush(const uint32_t* lw, uint32_t& w
   wcnt = 0;
 }
 
+// Verification counts go to per-workgroup partials (part[4 * block + k]) that
+// k_pp_reduce_verify adds up: thousands of same-address atomics would serialise.
+// badmap4 (nullable) receives one byte per point, 1 where the point is mismatched.
 template <int ND>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float* __restrict__ y,
                                                            const uint8_t* __restrict__ t,
                                                            const float* __restrict__ d, uint32_t n4,
                                                            uint32_t* __restrict__ bad, uint32_t cap,
+                                                           uint32_t* __restrict__ part,
+                                                           uint32_t* __restrict__ badmap4,
                                                            uint32_t* __restrict__ misc) {
   constexpr int FA = ND - 1;  // fastest axis
   constexpr int U = kMarkGroups;
   __shared__ uint32_t red[4][kWavesPerBlock];
   __shared__ uint32_t lbuf[kWavesPerBlock][kMarkWaveStage];
+  __shared__ uint32_t wc[kWavesPerBlock], sh_base;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   uint32_t* lw = lbuf[w];
   uint32_t wcnt = 0;  // wave-uniform fill level of lw
@@ -466,8 +493,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
   const uint32_t* t4 = reinterpret_cast<const uint32_t*>(t);
   const int dimz = dim_of<ND>(g, FA);
   uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
+  const XcdSweep sw = xcd_sweep(n4, kBlockThreads * U);
   // the loop bound is uniform per workgroup, so every lane takes part in the ballots
-  for (uint32_t b0 = blockIdx.x * kBlockThreads * U; b0 < n4; b0 += gridDim.x * kBlockThreads * U) {
+  for (uint32_t b0 = sw.begin; b0 < sw.end; b0 += sw.step) {
     float4 v[U], lo4[U][ND - 1], hi4[U][ND - 1];
     float zm[U], zp[U];
     uint32_t tw[U];
@@ -476,7 +504,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint32_t gi = b0 + u * kBlockThreads + threadIdx.x;
-      in[u] = gi < n4;
+      in[u] = gi < sw.end;
       const uint32_t e0 = 4 * (in[u] ? gi : 0u);
       coords_of<ND>(g, e0, c[u]);
       rowin[u] = in[u];
@@ -524,6 +552,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
         chg += (in[u] && tt != 0 && k != 0 && k != tt);
         flag[i] = in[u] && k != tt;
       }
+      if (badmap4 != nullptr && in[u])
+        badmap4[b0 + u * kBlockThreads + threadIdx.x] =
+            (flag[0] ? 1u : 0u) | (flag[1] ? 1u : 0u) << 8 | (flag[2] ? 1u : 0u) << 16 | (flag[3] ? 1u : 0u) << 24;
       if (d != nullptr && in[u]) {
         const float4 dd = d4[b0 + u * kBlockThreads + threadIdx.x];
         const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
@@ -544,7 +575,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
     }
     if (bad != nullptr && wcnt > kMarkWaveStage - 4 * U * kWave) wave_stage_flush(lw, wcnt, bad, cap, &misc[kPpBad]);
   }
-  if (bad != nullptr && wcnt) wave_stage_flush(lw, wcnt, bad, cap, &misc[kPpBad]);
+  // what is left in the four wave lists goes out with one global atomic per workgroup
+  if (bad != nullptr) {
+    if (lane == 0) wc[w] = wcnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+#pragma unroll
+      for (int k = 0; k < kWavesPerBlock; k++) tot += wc[k];
+      sh_base = tot ? atomicAdd(&misc[kPpBad], tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = sh_base;
+    for (int k = 0; k < w; k++) base += wc[k];
+    for (uint32_t j = lane; j < wcnt; j += kWave)
+      if (base + j < cap) bad[base + j] = lw[j];
+  }
   miss = seg_sum<kWave>(miss);
   spur = seg_sum<kWave>(spur);
   chg = seg_sum<kWave>(chg);
@@ -565,11 +611,68 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
       s2 += red[2][k];
       s3 = red[3][k] > s3 ? red[3][k] : s3;
     }
+    part[4 * blockIdx.x] = s0;
+    part[4 * blockIdx.x + 1] = s1;
+    part[4 * blockIdx.x + 2] = s2;
+    part[4 * blockIdx.x + 3] = s3;
+  }
+}
+
+// adds up the per-workgroup verification partials (one workgroup)
+__global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_verify(const uint32_t* __restrict__ part, uint32_t nb,
+                                                                   uint32_t* __restrict__ misc) {
+  __shared__ unsigned long long red[4][kWavesPerBlock];
+  unsigned long long s0 = 0, s1 = 0, s2 = 0;
+  uint32_t s3 = 0;
+  for (uint32_t k = threadIdx.x; k < nb; k += kBlockThreads) {
+    s0 += part[4 * k];
+    s1 += part[4 * k + 1];
+    s2 += part[4 * k + 2];
+    s3 = part[4 * k + 3] > s3 ? part[4 * k + 3] : s3;
+  }
+  s0 = seg_sum<kWave>(s0);
+  s1 = seg_sum<kWave>(s1);
+  s2 = seg_sum<kWave>(s2);
+  s3 = seg_max<kWave>(s3);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[0][w] = s0;
+    red[1][w] = s1;
+    red[2][w] = s2;
+    red[3][w] = s3;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      t0 += red[0][k];
+      t1 += red[1][k];
+      t2 += red[2][k];
+      t3 = red[3][k] > t3 ? red[3][k] : t3;
+    }
     unsigned long long* vo = reinterpret_cast<unsigned long long*>(misc + kPpVerify);
-    if (s0) atomicAdd(&vo[0], (unsigned long long)s0);
-    if (s1) atomicAdd(&vo[1], (unsigned long long)s1);
-    if (s2) atomicAdd(&vo[2], (unsigned long long)s2);
-    if (s3) atomicMax(&vo[3], (unsigned long long)s3);
+    vo[0] += t0;
+    vo[1] += t1;
+    vo[2] += t2;
+    vo[3] = t3 > vo[3] ? t3 : vo[3];
+  }
+}
+
+// sum of n per-workgroup counts into misc[idx] (one workgroup)
+__global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_sum(const uint32_t* __restrict__ part, uint32_t n,
+                                                                uint32_t* __restrict__ misc, int idx) {
+  __shared__ uint32_t red[kWavesPerBlock];
+  uint32_t v = 0;
+  for (uint32_t k = threadIdx.x; k < n; k += kBlockThreads) v += part[k];
+  v = seg_sum<kWave>(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerBlock; k++) t += red[k];
+    misc[idx] += t;
   }
 }
 
@@ -813,168 +916,270 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint3
 // or 13 (three points) weak orders are checked in registers and the first that gives
 // every point its type is written.
 // ---------------------------------------------------------------------------------
+
+
+// ---------------------------------------------------------------------------------
+// relation masks of the decoded values: for every point and each of its 2*ND
+// neighbours (axis a: bit 2a at -1, bit 2a+1 at +1), bit k says the neighbour is inside
+// the field with the same decoded value, bit 8+k that it is inside with a smaller one.
+// The plateau passes read one 16-bit mask per point instead of its neighbours' values.
+// ---------------------------------------------------------------------------------
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_rel(Grid g, const float* __restrict__ d, uint32_t n,
+                                                         uint16_t* __restrict__ rel) {
+  for (uint32_t e = blockIdx.x * kBlockThreads + threadIdx.x; e < n; e += gridDim.x * kBlockThreads) {
+    int c[ND];
+    coords_of<ND>(g, e, c);
+    const float v = d[e];
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * ND; k++) {
+      const int a = k >> 1;
+      const bool ok = (k & 1) ? c[a] + 1 < dim_of<ND>(g, a) : c[a] >= 1;
+      if (!ok) continue;
+      const uint32_t s = stride_of<ND>(g, a);
+      const float q = d[(k & 1) ? e + s : e - s];
+      r |= (q == v ? 1u : 0u) << k;
+      r |= (v > q ? 1u : 0u) << (8 + k);
+    }
+    rel[e] = (uint16_t)r;
+  }
+}
+
+// the same with 4 consecutive points per thread along the fastest axis (its length a
+// multiple of 4, buffers aligned): one float4 per stencil row
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_rel4(Grid g, const float* __restrict__ d, uint32_t n4,
+                                                          uint2* __restrict__ rel4) {
+  constexpr int FA = ND - 1;
+  const float4* d4 = reinterpret_cast<const float4*>(d);
+  const int dz = dim_of<ND>(g, FA);
+  const XcdSweep sw = xcd_sweep(n4, kBlockThreads);
+  for (uint32_t gi = sw.begin + threadIdx.x; gi < sw.end; gi += sw.step) {
+    const uint32_t e0 = 4 * gi;
+    int c[ND];
+    coords_of<ND>(g, e0, c);
+    const float4 v4 = d4[gi];
+    const float v[4] = {v4.x, v4.y, v4.z, v4.w};
+    float lo[FA][4], hi[FA][4];
+    bool hlo[FA], hhi[FA];
+#pragma unroll
+    for (int a = 0; a < FA; a++) {
+      const uint32_t s4 = stride_of<ND>(g, a) >> 2;
+      hlo[a] = c[a] >= 1;
+      hhi[a] = c[a] + 1 < dim_of<ND>(g, a);
+      // unconditional 16-byte loads (a missing neighbour reads the centre, unused)
+      const float4 l = d4[hlo[a] ? gi - s4 : gi], h = d4[hhi[a] ? gi + s4 : gi];
+      lo[a][0] = l.x; lo[a][1] = l.y; lo[a][2] = l.z; lo[a][3] = l.w;
+      hi[a][0] = h.x; hi[a][1] = h.y; hi[a][2] = h.z; hi[a][3] = h.w;
+    }
+    const int z = c[FA];
+    const float zm = d[z >= 1 ? e0 - 1 : e0], zp = d[z + 4 < dz ? e0 + 4 : e0];
+    uint32_t r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int a = 0; a < FA; a++) {
+        if (hlo[a]) m |= (lo[a][i] == v[i] ? 1u : 0u) << (2 * a) | (v[i] > lo[a][i] ? 1u : 0u) << (8 + 2 * a);
+        if (hhi[a]) m |= (hi[a][i] == v[i] ? 1u : 0u) << (2 * a + 1) | (v[i] > hi[a][i] ? 1u : 0u) << (9 + 2 * a);
+      }
+      const bool hzl = z + i >= 1, hzh = z + i + 1 < dz;
+      const float ql = i == 0 ? zm : v[i - 1], qh = i == 3 ? zp : v[i + 1];
+      if (hzl) m |= (ql == v[i] ? 1u : 0u) << (2 * FA) | (v[i] > ql ? 1u : 0u) << (8 + 2 * FA);
+      if (hzh) m |= (qh == v[i] ? 1u : 0u) << (2 * FA + 1) | (v[i] > qh ? 1u : 0u) << (9 + 2 * FA);
+      r[i] = m;
+    }
+    rel4[gi] = make_uint2(r[0] | r[1] << 16, r[2] | r[3] << 16);
+  }
+}
+
+// Pass 1: the plateau of every listed point, given up beyond three points; one thread
+// per plateau (the first to claim its smallest point) lists it as {P0, P1, P2, np}.
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(Grid g, const uint32_t* __restrict__ bad,
+                                                               uint32_t nbad, const uint16_t* __restrict__ rel,
+                                                               const uint8_t* __restrict__ badmap,
+                                                               uint4* __restrict__ plist) {
+  constexpr int K = 2 * ND;
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i >= nbad) return;
+  const uint32_t b = bad[i];
+  uint32_t P[3] = {b, b, b};
+  int np = 1;
+  bool big = false;
+#pragma unroll
+  for (int u = 0; u < 3; u++) {
+    if (u >= np || big) continue;
+    const uint32_t eq = rel[P[u]] & ((1u << K) - 1u);  // same-value neighbours
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (!((eq >> k) & 1u)) continue;
+      const uint32_t s = stride_of<ND>(g, k >> 1);
+      const uint32_t qe = (k & 1) ? P[u] + s : P[u] - s;
+      const bool seen = qe == P[0] || (np > 1 && qe == P[1]) || (np > 2 && qe == P[2]);
+      if (seen) continue;
+      if (np == 3) {
+        big = true;
+      } else {
+        if (np == 1) P[1] = qe;
+        else P[2] = qe;
+        np++;
+      }
+    }
+  }
+  // the plateau goes to the thread of its smallest mismatched point
+  bool take = !big && np >= 2;
+  if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
+  if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
+  plist[i] = take ? make_uint4(P[0], P[1], P[2], (uint32_t)np) : make_uint4(0, 0, 0, 0);
+}
+
+// Pass 2, one thread per listed plateau.  Every other neighbour of a plateau of two or
+// three points lies in another cell, so only the order inside the plateau decides the
+// types of its points, and no other point's type depends on it.  Each point's type is
+// tabulated for every relation (<, =, >) to each of the other plateau points (a grid
+// has no triangles, so a point sees at most two of them); the 3 (two points) or 13
+// (three points) weak orders are then table look-ups, and the first that gives every
+// point its type is written.
 __constant__ const int8_t kWeak3[13][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
                                            {1, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 2, 1}, {1, 0, 2},
                                            {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 __constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
 
 template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
-                                                          const float* __restrict__ d, const uint8_t* __restrict__ t,
-                                                          float* __restrict__ y, float eb, int anchor,
-                                                          uint32_t* bits, uint32_t* misc) {
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(Grid g, const uint4* __restrict__ plist,
+                                                                const float* __restrict__ d,
+                                                                const uint16_t* __restrict__ rel,
+                                                                const uint8_t* __restrict__ t, float* __restrict__ y,
+                                                                float eb, int anchor, uint32_t count,
+                                                                uint32_t* __restrict__ part) {
   constexpr int K = 2 * ND;
+  __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   bool done = false;
-  if (i < nbad) {
-    const uint32_t b = bad[i];
-    const float db = d[b];
-    // the plateau of b, given up beyond three points
-    uint32_t P[3] = {b, b, b};
-    int np = 1;
-    bool big = false;
+  const uint4 ent = i < count ? plist[i] : make_uint4(0, 0, 0, 0);
+  if (ent.w >= 2) {
+    const uint32_t P[3] = {ent.x, ent.y, ent.z};
+    const int np = (int)ent.w;
+    const float db = d[P[0]];
+    int tp[3];
+    bool intr[3], anc[3];
+    float yv[3];
+    uint32_t ttab[3];  // type of point j for relations (ra, rb) to its two other points: 2 bits each
 #pragma unroll
-    for (int u = 0; u < 3; u++) {
-      if (u >= np || big) continue;
+    for (int j = 0; j < 3; j++) {
+      tp[j] = 0;
+      intr[j] = false;
+      anc[j] = false;
+      yv[j] = 0.f;
+      ttab[j] = 0;
+      if (j >= np) continue;
+      const uint32_t e = P[j];
       int c[ND];
-      coords_of<ND>(g, P[u], c);
+      coords_of<ND>(g, e, c);
+      tp[j] = t[e];
+      intr[j] = interior<ND>(g, c);
+      anc[j] = anchor && anchored(tp[j], db);
+      yv[j] = y[e];
+      if (!intr[j]) continue;
+      // the other two plateau indices of j, in increasing order
+      const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
+      int code[K];  // +-1: fixed sign of (point - neighbour); 2: other point oa; 3: other point ob
+      const uint32_t rm = rel[e];
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        int q[ND];
-#pragma unroll
-        for (int a = 0; a < ND; a++) q[a] = c[a];
-        q[k >> 1] += (k & 1) ? 1 : -1;
-        if (!inside<ND>(g, q)) continue;
         const uint32_t s = stride_of<ND>(g, k >> 1);
-        const uint32_t qe = (k & 1) ? P[u] + s : P[u] - s;
-        if (d[qe] != db) continue;
-        const bool seen = qe == P[0] || (np > 1 && qe == P[1]) || (np > 2 && qe == P[2]);
-        if (seen) continue;
-        if (np == 3) {
-          big = true;
-        } else {
-          if (np == 1) P[1] = qe;
-          else P[2] = qe;
-          np++;
-        }
+        const uint32_t qe = (k & 1) ? e + s : e - s;
+        // a same-value neighbour of a plateau of at most three points is one of them
+        code[k] = (oa < np && qe == P[oa]) ? 2 : (ob < np && qe == P[ob]) ? 3 : (((rm >> (8 + k)) & 1u) ? 1 : -1);
       }
-    }
-    const uint32_t owner = min(P[0], min(P[1], P[2]));
-    if (!big && np >= 2 && claim_bit(bits, owner)) {
-      // per plateau point: type, interior flag, anchor value, and for each neighbour
-      // either the plateau index or the fixed sign of (point - neighbour)
-      int tp[3], rel[3][K];
-      bool intr[3], anc[3];
-      float yv[3];
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        tp[j] = 0;
-        intr[j] = false;
-        anc[j] = false;
-        yv[j] = 0.f;
+      for (int ra = -1; ra <= 1; ra++) {
 #pragma unroll
-        for (int k = 0; k < K; k++) rel[j][k] = 0;
-        if (j >= np) continue;
-        const uint32_t e = P[j];
-        int c[ND];
-        coords_of<ND>(g, e, c);
-        tp[j] = t[e];
-        intr[j] = interior<ND>(g, c);
-        anc[j] = anchor && anchored(tp[j], db);
-        yv[j] = y[e];
-        if (!intr[j]) continue;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-          const uint32_t s = stride_of<ND>(g, k >> 1);
-          const uint32_t qe = (k & 1) ? e + s : e - s;
-          // 10 + index for a plateau neighbour, else the sign of db - d[q]
-          if (qe == P[0]) rel[j][k] = 10;
-          else if (np > 1 && qe == P[1]) rel[j][k] = 11;
-          else if (np > 2 && qe == P[2]) rel[j][k] = 12;
-          else rel[j][k] = db > d[qe] ? 1 : -1;
-        }
-      }
-      float lo, hi;
-      cell_of(db, eb, &lo, &hi);
-      const int nord = np == 2 ? 3 : 13;
-      for (int o = 0; o < nord && !done; o++) {
-        int rk[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) rk[j] = j < np ? (np == 2 ? kWeak2[o][j] : kWeak3[o][j]) : 0;
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          if (j >= np || !intr[j]) continue;
+        for (int rb = -1; rb <= 1; rb++) {
           bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
 #pragma unroll
           for (int a = 0; a < ND; a++) {
-            int r2[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-              const int code = rel[j][2 * a + h];
-              if (code >= 10) {
-                const int other = code == 10 ? rk[0] : (code == 11 ? rk[1] : rk[2]);
-                r2[h] = rk[j] > other ? 1 : (rk[j] < other ? -1 : 0);
-              } else {
-                r2[h] = code;
-              }
-            }
-            const bool hi2 = r2[0] > 0 && r2[1] > 0, lo2 = r2[0] < 0 && r2[1] < 0;
+            const int c0 = code[2 * a], c1 = code[2 * a + 1];
+            const int r0 = c0 == 2 ? ra : c0 == 3 ? rb : c0;
+            const int r1 = c1 == 2 ? ra : c1 == 3 ? rb : c1;
+            const bool hi2 = r0 > 0 && r1 > 0, lo2 = r0 < 0 && r1 < 0;
             all_gt = all_gt && hi2;
             all_lt = all_lt && lo2;
             any_hi = any_hi || hi2;
             any_lo = any_lo || lo2;
           }
-          const int k = all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
-          ok = ok && k == tp[j];
+          const uint32_t k = all_gt ? 1u : (all_lt ? 2u : ((any_hi && any_lo) ? 3u : 0u));
+          ttab[j] |= k << (2 * (3 * (ra + 1) + (rb + 1)));
         }
-        if (!ok) continue;
-        // values: classes holding an anchor keep its value, the others are spaced
-        // evenly between their neighbours (or the interval ends)
-        int nr = 0;
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          if (j < np) nr = max(nr, rk[j] + 1);
-        float cv[3] = {0.f, 0.f, 0.f};
-        bool ca[3] = {false, false, false};
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          if (j >= np || !anc[j]) continue;
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            if (k != rk[j]) continue;
-            if (ca[k] && cv[k] != yv[j]) ok = false;
-            ca[k] = true;
-            cv[k] = yv[j];
-          }
-        }
-        if (!ok) continue;
-        float fv[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          fv[k] = cv[k];
-          if (k >= nr || ca[k]) continue;
-          int pa = -1, na = nr;  // nearest anchored classes below and above
-#pragma unroll
-          for (int m = 0; m < 3; m++) {
-            if (m < nr && ca[m] && m < k) pa = m;
-            if (m < nr && ca[m] && m > k && na == nr) na = m;
-          }
-          const float L = pa >= 0 ? (pa == 0 ? cv[0] : (pa == 1 ? cv[1] : cv[2])) : lo;
-          const float U = na < nr ? (na == 0 ? cv[0] : (na == 1 ? cv[1] : cv[2])) : hi;
-          fv[k] = L + (U - L) * ((float)(k - pa) / (float)(na - pa));
-        }
-#pragma unroll
-        for (int k = 0; k + 1 < 3; k++) ok = ok && (k + 1 >= nr || fv[k] < fv[k + 1]);
-        if (!ok) continue;
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
-        done = true;
       }
     }
+    float lo, hi;
+    cell_of(db, eb, &lo, &hi);
+    const int nord = np == 2 ? 3 : 13;
+    for (int o = 0; o < nord && !done; o++) {
+      int rk[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) rk[j] = j < np ? (np == 2 ? kWeak2[o][j] : kWeak3[o][j]) : 0;
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        if (j >= np || !intr[j]) continue;
+        const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
+        const int ra = rk[j] > rk[oa] ? 1 : (rk[j] < rk[oa] ? -1 : 0);
+        const int rb = ob < np ? (rk[j] > rk[ob] ? 1 : (rk[j] < rk[ob] ? -1 : 0)) : 0;
+        ok = ok && (int)((ttab[j] >> (2 * (3 * (ra + 1) + (rb + 1)))) & 3u) == tp[j];
+      }
+      if (!ok) continue;
+      // values: classes holding an anchor keep its value, the others are spaced
+      // evenly between their neighbours (or the interval ends)
+      int nr = 0;
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        if (j < np) nr = max(nr, rk[j] + 1);
+      float cv[3] = {0.f, 0.f, 0.f};
+      bool ca[3] = {false, false, false};
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        if (j >= np || !anc[j]) continue;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (k != rk[j]) continue;
+          if (ca[k] && cv[k] != yv[j]) ok = false;
+          ca[k] = true;
+          cv[k] = yv[j];
+        }
+      }
+      if (!ok) continue;
+      float fv[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        fv[k] = cv[k];
+        if (k >= nr || ca[k]) continue;
+        int pa = -1, na = nr;  // nearest anchored classes below and above
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          if (m < nr && ca[m] && m < k) pa = m;
+          if (m < nr && ca[m] && m > k && na == nr) na = m;
+        }
+        const float L = pa >= 0 ? (pa == 0 ? cv[0] : (pa == 1 ? cv[1] : cv[2])) : lo;
+        const float U = na < nr ? (na == 0 ? cv[0] : (na == 1 ? cv[1] : cv[2])) : hi;
+        fv[k] = L + (U - L) * ((float)(k - pa) / (float)(na - pa));
+      }
+#pragma unroll
+      for (int k = 0; k + 1 < 3; k++) ok = ok && (k + 1 >= nr || fv[k] < fv[k + 1]);
+      if (!ok) continue;
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
+      done = true;
+    }
   }
-  wave_count(done, &misc[kPpTiny]);
+  const uint32_t nd = (uint32_t)__popcll(__ballot(done));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nd;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // ---------------------------------------------------------------------------------
@@ -1579,17 +1784,25 @@ hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t
   return hipGetLastError();
 }
 
+constexpr unsigned kMarkBlocks = 2048;
+uint32_t pp_mark_parts() { return kMarkBlocks; }
+
 hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
-                   uint32_t cap, uint32_t* misc, hipStream_t st) {
+                   uint32_t cap, uint32_t* part, uint8_t* badmap, uint32_t* misc, hipStream_t st) {
   const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
   const bool vec = fast % 4 == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0;
+                   (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(badmap) & 3) == 0;
   if (vec) {
-    const dim3 grid(grid_stride_blocks((n / 4 + kMarkGroups - 1) / kMarkGroups)), blk(kBlockThreads);
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
-    else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, misc);
+    unsigned nb = kMarkBlocks;  // a multiple of the XCD count
+    const dim3 grid(nb), blk(kBlockThreads);
+    uint32_t* bm4 = reinterpret_cast<uint32_t*>(badmap);
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
+    else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
+    hipLaunchKernelGGL(k_pp_reduce_verify, dim3(1), blk, 0, st, part, nb, misc);
     return hipGetLastError();
   }
+  if (badmap) return hipErrorInvalidValue;  // the mismatch map needs the vector layout
   const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
   if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark<2>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   else hipLaunchKernelGGL(k_pp_mark<3>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
@@ -1626,12 +1839,50 @@ hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const fl
   return hipGetLastError();
 }
 
-hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
-                   float eb, int anchor, uint32_t* bits, uint32_t* misc, hipStream_t st) {
+hipError_t pp_rel(const Grid& g, const float* d, uint32_t n, uint16_t* rel, hipStream_t st) {
+  const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
+  const bool vec = fast % 4 == 0 && ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(rel)) & 15) == 0;
+  if (vec) {
+    const dim3 grid(2048), blk(kBlockThreads);  // a multiple of the XCD count
+    uint2* r4 = reinterpret_cast<uint2*>(rel);
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_rel4<2>, grid, blk, 0, st, g, d, n / 4, r4);
+    else hipLaunchKernelGGL(k_pp_rel4<3>, grid, blk, 0, st, g, d, n / 4, r4);
+  } else {
+    const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_rel<2>, grid, blk, 0, st, g, d, n, rel);
+    else hipLaunchKernelGGL(k_pp_rel<3>, grid, blk, 0, st, g, d, n, rel);
+  }
+  return hipGetLastError();
+}
+
+hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
+                   const uint8_t* badmap, const uint8_t* t, float* y, float eb, int anchor, uint4* plist,
+                   uint32_t* part, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_tiny<2>, grid, blk, 0, st, g, bad, nbad, d, t, y, eb, anchor, bits, misc);
-  else hipLaunchKernelGGL(k_pp_tiny<3>, grid, blk, 0, st, g, bad, nbad, d, t, y, eb, anchor, bits, misc);
+  if (g.ndim == 2) {
+    hipLaunchKernelGGL(k_pp_tiny_find<2>, grid, blk, 0, st, g, bad, nbad, rel, badmap, plist);
+    hipLaunchKernelGGL(k_pp_tiny_solve<2>, grid, blk, 0, st, g, plist, d, rel, t, y, eb, anchor, nbad, part);
+  } else {
+    hipLaunchKernelGGL(k_pp_tiny_find<3>, grid, blk, 0, st, g, bad, nbad, rel, badmap, plist);
+    hipLaunchKernelGGL(k_pp_tiny_solve<3>, grid, blk, 0, st, g, plist, d, rel, t, y, eb, anchor, nbad, part);
+  }
+  hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
+  return hipGetLastError();
+}
+
+uint32_t pp_tiny_parts(uint32_t nbad) { return blocks_for(nbad); }
+
+// mismatch map from a list: badmap must be zero; one byte store per listed point
+__global__ __launch_bounds__(kBlockThreads) void k_pp_badmap(const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                            uint8_t* __restrict__ badmap) {
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i < nbad) badmap[bad[i]] = 1;
+}
+
+hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st) {
+  if (!nbad) return hipSuccess;
+  hipLaunchKernelGGL(k_pp_badmap, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, bad, nbad, badmap);
   return hipGetLastError();
 }
 
