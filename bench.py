@@ -90,6 +90,27 @@ def cpu_baseline(field_np, eb, bs, threads, slab):
     return res, None
 
 
+def codec_kernel(mode, bs, ndim, fast):
+    """Name of the compression kernel the library launches for this block size: the
+    16-byte path for block sizes 32 and 64 (classification also needs the fastest axis a
+    multiple of 4), else the generic kernel with its lane geometry."""
+    if bs in (32, 64) and (mode != 0 or fast % 4 == 0):
+        return f"k_compress_v4<{mode}, {bs // 4}, {ndim if mode == 0 else 2}>"
+    if bs <= 16:
+        s, e = 1, 1
+        while s < bs:
+            s <<= 1
+    elif bs <= 64:
+        s, e = 1, 4
+        while 4 * s < bs:
+            s <<= 1
+    else:
+        s, e = 64, 1
+        while 64 * e < bs:
+            e <<= 1
+    return f"k_compress<{mode}, {s}, {e}>"
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE summary
     (already scaled by the copy-kernel calibration there), averaged over its dispatches;
@@ -190,16 +211,17 @@ def main():
     sp_payload = max(0, info.sortpos_bytes - 16)
     # Single kernels timed per launch by the library, with their algorithmic bytes per
     # launch (the minimum HBM traffic of what each launch must read and write).
+    ndim = len(shape)
     kernels = {
         # read the field, write the topology payload (the 8 B/CP sort keys it also emits
         # are an intermediate of this design and not counted)
-        "codec_topo": ("k_compress<0, 64, 1>", 4 * n + comp_payload),
+        "codec_topo": (codec_kernel(0, bs, ndim, shape[-1]), 4 * n + comp_payload),
         # bucket the critical points' value keys by bin: read and write 4 B each
         "scatter": ("k_bin_scatter", 8 * info.cp_count),
         # read each critical point's key and payload, write each extremum's sort position
         "query": ("k_bin_query", 8 * info.cp_count + 4 * info.ext_count),
         # read the sort positions, write their stream
-        "codec_int": ("k_compress<3, 64, 1>", 4 * info.ext_count + sp_payload),
+        "codec_int": (codec_kernel(3, bs, 2, 4), 4 * info.ext_count + sp_payload),
     }
     dom = max(kernels, key=lambda k: stages.get(k, (0, 0))[0])
     per_launch_ms = prof[dom][0] / max(1, prof[dom][1])

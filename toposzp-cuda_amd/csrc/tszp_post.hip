@@ -45,6 +45,20 @@ namespace tszp {
 // handled in chunks of `chunk`; the grid is a multiple of 8.
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kXcds = 8;
+
+// non-temporal 16- and 8-byte accesses for data streamed once (keeps L2 for reused rows)
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned nt_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(uint2* p, uint2 v) {
+  nt_u2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_u2*>(p));
+}
 struct XcdSweep {
   uint32_t begin, end, step;
 };
@@ -454,7 +468,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 constexpr uint32_t kMarkWaveStage = 1024;  // staged entries per wave
-constexpr int kMarkGroups = 2;            // float4 groups per thread and loop step
+constexpr int kMarkGroups = 1;            // float4 groups per thread and loop step
 
 __device__ __forceinline__ void wave_stage_flush(const uint32_t* lw, uint32_t& wcnt, uint32_t* __restrict__ list,
                                                  uint32_t cap, uint32_t* gcount) {
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
 #pragma unroll
       for (int a = 0; a < FA; a++) rowin[u] = rowin[u] && c[u][a] >= 1 && c[u][a] + 2 <= dim_of<ND>(g, a);
       v[u] = in[u] ? y4[gi] : make_float4(0.f, 0.f, 0.f, 0.f);
-      tw[u] = in[u] ? t4[gi] : 0u;
+      tw[u] = in[u] ? __builtin_nontemporal_load(&t4[gi]) : 0u;  // streamed once: keep L2 for y
       zm[u] = zp[u] = 0.f;
       if (rowin[u]) {
 #pragma unroll
@@ -553,10 +567,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
         flag[i] = in[u] && k != tt;
       }
       if (badmap4 != nullptr && in[u])
-        badmap4[b0 + u * kBlockThreads + threadIdx.x] =
-            (flag[0] ? 1u : 0u) | (flag[1] ? 1u : 0u) << 8 | (flag[2] ? 1u : 0u) << 16 | (flag[3] ? 1u : 0u) << 24;
+        __builtin_nontemporal_store(
+            (flag[0] ? 1u : 0u) | (flag[1] ? 1u : 0u) << 8 | (flag[2] ? 1u : 0u) << 16 | (flag[3] ? 1u : 0u) << 24,
+            &badmap4[b0 + u * kBlockThreads + threadIdx.x]);
       if (d != nullptr && in[u]) {
-        const float4 dd = d4[b0 + u * kBlockThreads + threadIdx.x];
+        const float4 dd = ld_nt(&d4[b0 + u * kBlockThreads + threadIdx.x]);
         const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -990,7 +1005,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_rel4(Grid g, const float* 
       if (hzh) m |= (qh == v[i] ? 1u : 0u) << (2 * FA + 1) | (v[i] > qh ? 1u : 0u) << (9 + 2 * FA);
       r[i] = m;
     }
-    rel4[gi] = make_uint2(r[0] | r[1] << 16, r[2] | r[3] << 16);
+    st_nt(&rel4[gi], make_uint2(r[0] | r[1] << 16, r[2] | r[3] << 16));
   }
 }
 
