@@ -119,6 +119,20 @@ int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_t
 int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, const uint32_t *dims,
                      uint64_t *counts, void *stream);
 
+/* .SZp container written by the reference's CPU compress driver and read by its
+ * decompress driver: [topology stream][sort-position stream][u64 topology bytes]
+ * [u64 sort-position bytes] (example/testfloat_compress_fastmode1_topology.cc:105-139,
+ * example/testfloat_decompress_fastmode1_topology.cc:722-816).  The topology stream is
+ * the native [u64 off[T]][blocks], the sort-position stream the native
+ * [u64 E][u64 off[T]][blocks].  Buffers may be host or device memory.
+ * tszp_szp_pack needs cap >= topo_bytes + sp_bytes + 16 and returns after the copies.
+ * tszp_szp_unpack validates the footer as the reference does (otherwise the whole
+ * buffer is one topology stream) and reads E from the sort-position stream. */
+int tszp_szp_pack(const uint8_t *topo, size_t topo_bytes, const uint8_t *sp, size_t sp_bytes, uint8_t *out,
+                  size_t cap, size_t *out_bytes, void *stream);
+int tszp_szp_unpack(const uint8_t *buf, size_t len, size_t *topo_bytes, size_t *sp_offset, size_t *sp_bytes,
+                    uint64_t *n_extrema);
+
 /* Kernel timing with HIP events on the launch stream (off by default).  Kinds:
  * 0 topology codec (classify + pack), 1 int codec (sort positions), 2 plain codec,
  * 3 decoders, 4 sort-position bucketing (bins, scatter, bitmaps; or the radix sort),

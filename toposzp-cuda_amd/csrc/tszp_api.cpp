@@ -991,6 +991,54 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   return TSZP_OK;
 }
 
+// .SZp container of the reference's CPU drivers: [topology stream][sort-position
+// stream][u64 topology bytes][u64 sort-position bytes]
+// (example/testfloat_compress_fastmode1_topology.cc:105-139).  Pointers may be host or
+// device memory (unified addressing); the call returns when the copies are done.
+int tszp_szp_pack(const uint8_t* topo, size_t topo_bytes, const uint8_t* sp, size_t sp_bytes, uint8_t* out,
+                  size_t cap, size_t* out_bytes, void* stream) {
+  g_err.clear();
+  if (!topo || !out || !out_bytes || (sp_bytes && !sp)) return fail(TSZP_EINVAL, "szp_pack: bad arguments");
+  const size_t total = topo_bytes + sp_bytes + 2 * sizeof(uint64_t);
+  if (total < topo_bytes) return fail(TSZP_EINVAL, "szp_pack: size overflow");
+  if (cap < total) return fail(TSZP_ECAP, "szp_pack: %zu bytes needed, capacity %zu", total, cap);
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t footer[2] = {(uint64_t)topo_bytes, (uint64_t)sp_bytes};
+  HIPCHK(hipMemcpyAsync(out, topo, topo_bytes, hipMemcpyDefault, st));
+  if (sp_bytes) HIPCHK(hipMemcpyAsync(out + topo_bytes, sp, sp_bytes, hipMemcpyDefault, st));
+  HIPCHK(hipMemcpyAsync(out + topo_bytes + sp_bytes, footer, sizeof(footer), hipMemcpyDefault, st));
+  HIPCHK(hipStreamSynchronize(st));  // the footer source lives on this stack
+  *out_bytes = total;
+  return TSZP_OK;
+}
+
+// Reads the footer as the reference's decompress driver does
+// (example/testfloat_decompress_fastmode1_topology.cc:722-746): sizes that do not fit
+// the buffer mean an old single-stream file, all topology.  *n_extrema is the u64 count
+// at the head of the sort-position stream (0 without one).
+int tszp_szp_unpack(const uint8_t* buf, size_t len, size_t* topo_bytes, size_t* sp_offset, size_t* sp_bytes,
+                    uint64_t* n_extrema) {
+  g_err.clear();
+  if (!buf || !topo_bytes || !sp_offset || !sp_bytes || !n_extrema) return fail(TSZP_EINVAL, "szp_unpack: bad arguments");
+  uint64_t f[2] = {0, 0};
+  size_t tb = len, sb = 0;
+  if (len >= sizeof(f)) {
+    HIPCHK(hipMemcpy(f, buf + len - sizeof(f), sizeof(f), hipMemcpyDefault));
+    const uint64_t calc = f[0] + f[1] + sizeof(f);
+    if (f[0] > 0 && calc >= f[0] && calc <= (uint64_t)len + 16 && f[0] < len) {
+      tb = (size_t)f[0];
+      sb = (size_t)f[1];
+    }
+  }
+  uint64_t e = 0;
+  if (sb >= sizeof(uint64_t) && tb + sizeof(uint64_t) <= len) HIPCHK(hipMemcpy(&e, buf + tb, sizeof(e), hipMemcpyDefault));
+  *topo_bytes = tb;
+  *sp_offset = tb;
+  *sp_bytes = sb;
+  *n_extrema = e;
+  return TSZP_OK;
+}
+
 int tszp_gen_blobs(float* d, int ndim, const uint32_t* dims, uint64_t seed, int nblobs, float smin, float smax,
                    float noise, int waves, void* stream) {
   g_err.clear();

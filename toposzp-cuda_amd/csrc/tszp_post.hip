@@ -859,6 +859,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_
                                                           int colour, const float* __restrict__ d,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
                                                           float eb, uint32_t* misc) {
+  __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   uint32_t cnt = misc[kPpActC + colour];
   cnt = cnt < cap ? cnt : cap;
@@ -869,7 +870,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_
     coords_of<ND>(g, p, c);
     moved = move_point<ND>(g, p, c, d, t, y, eb);
   }
-  wave_count(moved, &misc[kPpMoves]);
+  // one atomic per workgroup (same-address atomics serialise)
+  const uint32_t nm = (uint32_t)__popcll(__ballot(moved));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = red[0] + red[1] + red[2] + red[3];
+    if (tot) atomicAdd(&misc[kPpMoves], tot);
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -1200,22 +1208,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(Grid g, const u
 // ---------------------------------------------------------------------------------
 // exact window solve
 // ---------------------------------------------------------------------------------
-// wave maximum of a value in [0, 127], one ballot per bit (no lane shuffles)
+// wave maximum of a non-negative value (whole wave active)
 __device__ __forceinline__ int wave_max7(int v) {
-  int res = 0;
-  bool alive = true;
-#pragma unroll
-  for (int b = 6; b >= 0; b--) {
-    const bool bit = (v >> b) & 1;
-    const uint64_t m = __ballot(alive && bit);
-    if (m) {
-      res |= 1 << b;
-      alive = alive && bit;
-    }
-  }
-  return res;
+  // DPP maximum over each 16-lane row, then over the four rows
+  const uint32_t r = seg_max_dpp<16>((uint32_t)v);
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)r, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)r, 16);
+  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)r, 32), e = (uint32_t)__builtin_amdgcn_readlane((int)r, 48);
+  return (int)max(max(a, b), max(c, e));
 }
-
 // value of v in lane `src` (wave-uniform), through a scalar register
 __device__ __forceinline__ int rdl(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
 __device__ __forceinline__ float rdlf(float v, int src) {

@@ -112,6 +112,10 @@ def lib():
     L.tszp_topo_restore.restype = i32
     L.tszp_topo_verify.argtypes = [vp, vp, i32, vp, u64p, vp]
     L.tszp_topo_verify.restype = i32
+    L.tszp_szp_pack.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz), vp]
+    L.tszp_szp_pack.restype = i32
+    L.tszp_szp_unpack.argtypes = [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), u64p]
+    L.tszp_szp_unpack.restype = i32
     L.tszp_prof_enable.argtypes = [i32]; L.tszp_prof_enable.restype = None
     L.tszp_prof_reset.argtypes = []; L.tszp_prof_reset.restype = None
     L.tszp_prof_read.argtypes = [i32, C.POINTER(C.c_double), u64p]; L.tszp_prof_read.restype = i32
@@ -382,3 +386,36 @@ def topo_verify(field, types, shape):
     out = (C.c_uint64 * 3)()
     _check(lib().tszp_topo_verify(_dp(field), _dp(types), len(shape), _dims(shape), out, _stream()))
     return tuple(out)
+
+
+# ------------------------------------------------------------------------------------------
+# .SZp container (example/testfloat_compress_fastmode1_topology.cc:105-139)
+# ------------------------------------------------------------------------------------------
+def _addr(a):
+    """Address of a numpy array (host) or a torch tensor (device or host)."""
+    return _p(a) if isinstance(a, np.ndarray) else _dp(a)
+
+
+def szp_pack(topo, topo_bytes, sp, sp_bytes, out=None):
+    """[topology stream][sort-position stream][u64 topology bytes][u64 sort-position bytes].
+    topo / sp: numpy arrays or torch tensors (host or device); out defaults to a numpy
+    array when topo is one, else a uint8 tensor on topo's device."""
+    total = topo_bytes + sp_bytes + 16
+    if out is None:
+        if isinstance(topo, np.ndarray):
+            out = np.zeros(total, np.uint8)
+        else:
+            out = _torch().empty(total, dtype=_torch().uint8, device=topo.device)
+    nb = C.c_size_t()
+    _check(lib().tszp_szp_pack(_addr(topo), topo_bytes, _addr(sp) if sp_bytes else None, sp_bytes, _addr(out),
+                               out.nbytes if isinstance(out, np.ndarray) else out.numel(), C.byref(nb), _stream()))
+    return out, nb.value
+
+
+def szp_unpack(buf, length=None):
+    """(topology bytes, sort-position offset, sort-position bytes, extrema count) of a
+    container, read as the reference's decompress driver reads the footer."""
+    n = length if length is not None else (buf.nbytes if isinstance(buf, np.ndarray) else buf.numel())
+    tb, so, sb, e = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
+    _check(lib().tszp_szp_unpack(_addr(buf), n, C.byref(tb), C.byref(so), C.byref(sb), C.byref(e)))
+    return tb.value, so.value, sb.value, e.value
