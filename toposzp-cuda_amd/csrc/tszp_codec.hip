@@ -545,7 +545,7 @@ __device__ __forceinline__ int type_minmax(float c, const float (&lo)[ND], const
 }
 
 template <int MODE, int S, int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_compress_v4(CompressArgs a) {
+__global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a) {
   constexpr int EPL = 4, BS = 4 * S;
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
