@@ -787,8 +787,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   const size_t bit_words = n / 32 + 4;
   WS(kWsPpBits, bit_words * 4, bits);
   WS(kWsPpMisc, kPpMiscWords * 4, misc);
-  const uint32_t itiles = pp_init_tiles(N);
-  WS(kWsPpSt, (size_t)itiles * 8 + 64, status);
+  WS(kWsPpSt, pp_init_work_words(N) * 4 + 64, status);
   uint32_t h[kPpMiscWords];
   auto read_misc = [&]() -> int {
     HIPCHK(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -801,10 +800,10 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     d = dcopy;
   }
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, (size_t)itiles * 2, st));
   const bool use_sp = d_sp != nullptr && n_ext > 0;
   if (use_sp) HIPCHK(pp_spmax(d_sp, (uint32_t)n_ext, misc, st));
-  HIPCHK(pp_init(d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext, N, eb, d_field, status, misc, st));
+  HIPCHK(pp_init(d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext, N, eb, d_field,
+                 reinterpret_cast<uint32_t*>(status), misc, st));
   // full marking pass: fresh list of mismatches and the verification counts
   // The list `bad` is exact when it holds every current mismatch; the verification
   // counts in misc are current when the last full pass saw the present field.

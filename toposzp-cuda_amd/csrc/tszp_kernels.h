@@ -171,8 +171,10 @@ enum : int {
 };
 uint32_t pp_init_tiles(uint32_t n);
 hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
+// start values; work: pp_init_work_words(n) u32 (8-byte aligned), no initialisation needed
+size_t pp_init_work_words(uint32_t n);
 hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
-                   float* y, uint64_t* status, uint32_t* misc, hipStream_t st);
+                   float* y, uint32_t* work, uint32_t* misc, hipStream_t st);
 // full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
 // badmap (nullable) receives one byte per point, 1 where mismatched (fastest axis a
 // multiple of 4 only)

@@ -260,19 +260,49 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_spmax(const int32_t* __res
 constexpr int kInitPer = 64;
 constexpr uint32_t kInitTile = (uint32_t)kBlockThreads * kInitPer;
 
+// extrema per tile of kInitTile points (pass 1; an exclusive scan of these counts gives
+// every tile the rank of its first extremum, so the start pass needs no look-back)
+__global__ __launch_bounds__(kBlockThreads) void k_pp_ext_count(const uint8_t* __restrict__ t, uint32_t n,
+                                                               uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t red[kWavesPerBlock];
+  const uint32_t t0 = blockIdx.x * kInitTile;
+  const uint32_t* tg = reinterpret_cast<const uint32_t*>(t);
+  uint32_t c = 0;
+#pragma unroll 4
+  for (int j = 0; j < (int)(kInitTile / 4 / kBlockThreads); j++) {
+    const uint32_t e = t0 + 4 * (j * kBlockThreads + threadIdx.x);
+    uint32_t v = 0;
+    if (e + 3 < n) {
+      v = tg[e >> 2];
+    } else {
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if (e + b < n) v |= (uint32_t)t[e + b] << (8 * b);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const uint32_t tb = (v >> (8 * b)) & 0xffu;
+      c += (tb == 1u || tb == 2u) ? 1u : 0u;
+    }
+  }
+  c = seg_sum<kWave>(c);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// tile_base[k]: extrema in the tiles before tile k (the scanned counts)
 __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restrict__ d, const uint8_t* __restrict__ t,
                                                           const int32_t* __restrict__ sp, uint32_t n_ext, uint32_t n,
-                                                          float eb, float* __restrict__ y, uint64_t* status,
+                                                          float eb, float* __restrict__ y,
+                                                          const uint32_t* __restrict__ tile_base,
                                                           uint32_t* misc) {
   __shared__ uint32_t tw[kInitTile / 4];
   __shared__ uint64_t msk[kBlockThreads];
   __shared__ uint32_t exl[kBlockThreads];
   __shared__ uint32_t red[kWavesPerBlock];
-  __shared__ uint32_t sh_tile, sh_base;
   const int tid = threadIdx.x;
-  if (tid == 0) sh_tile = atomicAdd(&misc[kPpCounter], 1u);
-  __syncthreads();
-  const uint32_t tile = sh_tile;
+  const uint32_t tile = blockIdx.x;
   const uint32_t t0 = tile * kInitTile;
   // stage the tile's types (4-byte aligned words, coalesced)
   const uint32_t* tg = reinterpret_cast<const uint32_t*>(t);
@@ -305,15 +335,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restri
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), red, &tot);
   exl[tid] = ex;
-  if (tid < kWave) {
-    const uint64_t pre = lookback(status, tile, tot, &misc[kPpErr]);
-    if (tid == 0) {
-      sh_base = (uint32_t)pre;
-      if (tile == gridDim.x - 1) misc[kPpExt] = (uint32_t)(pre + tot);
-    }
-  }
+  const uint32_t base = tile_base[tile];
+  if (tid == 0 && tile == gridDim.x - 1) misc[kPpExt] = base + tot;
   __syncthreads();
-  const uint32_t base = sh_base;
   const float sden = (float)misc[kPpSpMax] + 2.f;
   // float4 groups of 4 points (the tile start is a multiple of 4); all loads of a batch
   // are issued before the values are computed
@@ -1792,10 +1816,28 @@ hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t s
   return hipGetLastError();
 }
 
+// work: pp_init_tiles(n) + 1 counts, then scan_status_words(tiles + 1) zeroed u64 status
+// words and a zeroed u32 counter and error word (pp_init_work_words in total)
+size_t pp_init_work_words(uint32_t n) {
+  const uint32_t nt = pp_init_tiles(n) + 1;
+  return nt + 1 + 2 * scan_status_words(nt) + 2;
+}
+
 hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
-                   float* y, uint64_t* status, uint32_t* misc, hipStream_t st) {
-  hipLaunchKernelGGL(k_pp_init, dim3(pp_init_tiles(n)), dim3(kBlockThreads), 0, st, d, t, sp, n_ext, n, eb, y, status,
-                     misc);
+                   float* y, uint32_t* work, uint32_t* misc, hipStream_t st) {
+  const uint32_t tiles = pp_init_tiles(n);
+  uint32_t* cnt = work;
+  uint64_t* status = reinterpret_cast<uint64_t*>(work + ((tiles + 2) & ~1u));  // 8-byte aligned
+  const size_t sw = scan_status_words(tiles + 1);
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(status + sw);
+  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 2 * sw + 2, st);
+  if (e != hipSuccess) return e;
+  e = launch_fill_u32(cnt + tiles, 0u, 1, st);  // the scan turns it into the total
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pp_ext_count, dim3(tiles), dim3(kBlockThreads), 0, st, t, n, cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan_excl(cnt, tiles + 1, status, ctr, ctr + 1, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pp_init, dim3(tiles), dim3(kBlockThreads), 0, st, d, t, sp, n_ext, n, eb, y, cnt, misc);
   return hipGetLastError();
 }
 
