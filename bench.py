@@ -248,6 +248,18 @@ def main():
     plain_gbs = 4.0 * n / ((time.perf_counter() - t0) / args.steps) / 1e9
     log(f"plain SZp compress+decompress {plain_gbs:.1f} GB/s (ratio {4 * n / nbytes:.2f})")
 
+    # side figure: block offsets of the topology stream parsed on the device, as for a
+    # stream read from a file without its side-car
+    info, _ = tz.topo_compress(f, eb, bs, T=1, buf=buf)
+    tz.parse_blocks(buf.topo[:info.topo_bytes], n, bs, "topo")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        tz.parse_blocks(buf.topo[:info.topo_bytes], n, bs, "topo")
+    torch.cuda.synchronize()
+    parse_ms = 1e3 * (time.perf_counter() - t0) / 3
+    log(f"device block-offset parse of the topology stream {parse_ms:.3f} ms")
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -286,6 +298,7 @@ def main():
         "end_to_end_hbm_frac": e2e_frac,
         "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
         "plain_gbs": plain_gbs,
+        "topo_stream_parse_ms": parse_ms,
         "restore": ({"missing": last["restore"].missing, "spurious": last["restore"].spurious,
                      "changed": last["restore"].changed, "moves": last["restore"].moves,
                      "windows": last["restore"].windows, "rounds": last["restore"].rounds,

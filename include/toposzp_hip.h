@@ -119,6 +119,19 @@ int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_t
 int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, const uint32_t *dims,
                      uint64_t *counts, void *stream);
 
+/* Block offsets of a device-resident stream that has no side-car (a stream read from a
+ * file written by the reference), parsed on the device: replaces the host walk of
+ * SZp/src/szp_cuda_decompress.cu:391-477 and detects the chunk count T as
+ * SZp/include/szp_detect_threads.h:229-309 (first T whose offsets are non-decreasing,
+ * whose blocks walk within the buffer, and whose chunk offsets fall on block starts).
+ * kind: 0 topology stream [u64 off[T]][blocks], 1 sort-position stream
+ * [u64 E][u64 off[T]][blocks] (n = E), 2 plain stream [f32 eb][u64 off[T]][blocks].
+ * len: readable bytes of d_stream.  d_blk receives tszp_blocks(n, bs) + 1 offsets
+ * relative to the payload, ready for the decoders above; *T the chunk count.
+ * Block sizes whose largest block exceeds 2048 bytes are not supported (TSZP_EINVAL). */
+int tszp_parse_blocks(const uint8_t *d_stream, size_t len, size_t n, int block_size, int kind, uint32_t *d_blk,
+                      unsigned *nchunks, void *stream);
+
 /* .SZp container written by the reference's CPU compress driver and read by its
  * decompress driver: [topology stream][sort-position stream][u64 topology bytes]
  * [u64 sort-position bytes] (example/testfloat_compress_fastmode1_topology.cc:105-139,

@@ -990,6 +990,104 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   return TSZP_OK;
 }
 
+// Block offsets of a device-resident stream without a side-car.  kind: 0 topology
+// ([u64 off[T]][blocks]), 1 sort positions ([u64 E][u64 off[T]][blocks]), 2 plain
+// ([f32 eb][u64 off[T]][blocks]).  The chunk count is chosen by the rule of
+// detect_chunks (first T whose offsets are non-decreasing and whose blocks walk), with
+// the walk on the device; the host only reads the head of the stream to list the
+// candidates.  d_blk receives tszp_blocks(n, bs) + 1 offsets relative to the payload.
+int tszp_parse_blocks(const uint8_t* d_stream, size_t len, size_t n, int bs, int kind, uint32_t* d_blk,
+                      unsigned* T_out, void* stream) {
+  g_err.clear();
+  if (!d_stream || !d_blk || !T_out || kind < 0 || kind > 2 || bs < 1 || n == 0)
+    return fail(TSZP_EINVAL, "parse_blocks: bad arguments");
+  int r = check_n(n);
+  if (r) return r;
+  const bool topo = kind == 0;
+  const size_t nb = (n + bs - 1) / bs;
+  const size_t skip = kind == 1 ? 8 : kind == 2 ? 4 : 0;  // count / eb before the chunk table
+  if (len <= skip + 8) return fail(TSZP_EINVAL, "parse_blocks: stream too short");
+  const uint8_t* d_tab = d_stream + skip;
+  const size_t avail = len - skip;
+  // sizes of full blocks by bit count
+  ParseArgs a{};
+  a.topo = topo;
+  a.has_bc = topo || bs > 1;
+  uint32_t maxb = 0, minb = 0xffffffffu;
+  for (unsigned bc = 0; bc <= 32; bc++) {
+    const size_t m = (size_t)bs - 1;
+    size_t z = 4 + (a.has_bc ? 1 : 0) + (bc ? (m + 7) / 8 + fixed_len(m, bc) : 0) + (topo ? (2 * (size_t)bs + 7) / 8 : 0);
+    if (z > kParseMaxBlock) return fail(TSZP_EINVAL, "parse_blocks: block size %d too large for the device parse", bs);
+    a.sz[bc] = (uint16_t)z;
+    maxb = z > maxb ? (uint32_t)z : maxb;
+    minb = z < minb ? (uint32_t)z : minb;
+  }
+  a.sz[33] = (uint16_t)minb;  // an impossible bit count: keep walking, the final walk rejects it
+  a.maxb = maxb;
+  a.rounds = 1;
+  while (((uint64_t)1 << a.rounds) < kParseSeg / minb + 2) a.rounds++;
+  // candidates: the head of the chunk table, and enough of the payload for a first
+  // look at the blocks of each candidate
+  const size_t hb = std::min<size_t>(avail, 8 * 129 + 32 * (size_t)kParseMaxBlock);
+  std::vector<unsigned char> head(hb);
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(head.data(), d_tab, hb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const uint64_t lim = avail;
+  std::vector<unsigned> cand;
+  if (rd64(head.data()) != 0) {
+    cand.push_back(1);
+  } else {
+    for (unsigned t = 1; t <= 128 && 8 * (size_t)t + 8 <= hb; t++) {
+      bool ok = true;
+      for (unsigned k = 1; k < t && ok; k++) {
+        const uint64_t x = rd64(head.data() + 8 * k), y = rd64(head.data() + 8 * (k - 1));
+        if (x > lim || x < y) ok = false;
+      }
+      if (!ok) break;
+      cand.push_back(t);
+    }
+    if (cand.empty()) cand.push_back(1);
+  }
+  uint32_t* err;
+  WS(kWsMinMax, 16, err);
+  for (unsigned t : cand) {
+    if (avail <= 8 * (size_t)t) break;
+    {  // the first blocks must have possible bit counts (cheap rejection of wrong T)
+      size_t pos = 8 * (size_t)t;
+      bool bad = false;
+      for (size_t b = 0; b < std::min<size_t>(nb, 32) && !bad; b++) {
+        const size_t cur = std::min<size_t>((size_t)bs, n - b * (size_t)bs);
+        if (!(topo || cur > 1)) break;
+        if (pos + 4 >= hb) break;
+        const unsigned bc = head[pos + 4];
+        if (bc > 32) bad = true;
+        pos += 5 + (bc ? (cur - 1 + 7) / 8 + fixed_len(cur - 1, bc) : 0) + (topo ? (2 * cur + 7) / 8 : 0);
+      }
+      if (bad) continue;
+    }
+    a.p = d_tab + 8 * (size_t)t;
+    a.len = avail - 8 * (size_t)t;
+    if (a.len > 0xffffffffull) return fail(TSZP_EINVAL, "parse_blocks: payload beyond 4 GiB");
+    void* work;
+    WS(kWsStream2, parse_work_bytes(a.len, a.maxb), work);
+    HIPCHK(launch_fill_u32(err, 0u, 2, st));
+    HIPCHK(parse_blocks(a, n, (uint32_t)bs, (uint32_t)nb, d_blk, work, err, st));
+    HIPCHK(parse_check_offsets(d_tab, t, d_blk, (uint32_t)nb, err, st));
+    uint32_t h[2];
+    HIPCHK(hipMemcpyAsync(h, err, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!h[0] && h[1]) {
+      *T_out = t;
+      return TSZP_OK;
+    }
+  }
+  return fail(TSZP_EINVAL, "parse_blocks: no chunk count gives a consistent block sequence (corrupt stream?)");
+}
+
 // .SZp container of the reference's CPU drivers: [topology stream][sort-position
 // stream][u64 topology bytes][u64 sort-position bytes]
 // (example/testfloat_compress_fastmode1_topology.cc:105-139).  Pointers may be host or

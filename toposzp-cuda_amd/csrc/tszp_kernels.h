@@ -158,6 +158,26 @@ hipError_t bin_bitmap(uint32_t nchunks, const BinWork& w, uint2* gw, hipStream_t
 hipError_t bin_query(const uint32_t* key, const uint32_t* pay, uint32_t count, float eb, double inv, int bmin,
                      uint32_t B, const BinWork& w, const uint2* gw, int32_t* out, uint32_t out_cap, hipStream_t st);
 
+// ---- block offsets of streams without a side-car (tszp_parse.hip) ----
+constexpr uint32_t kParseSeg = 8192;       // payload bytes per segment
+constexpr uint32_t kParseMaxBlock = 2048;  // largest block the device parse handles
+constexpr uint32_t kParseGroup = 64;       // tables per composition group
+struct ParseArgs {
+  const uint8_t* p;  // payload (after the chunk table)
+  uint64_t len;      // payload bytes readable
+  uint16_t sz[34];   // size of a full block by bit count; [33]: an impossible bit count
+  uint32_t maxb;     // largest full-block size (entry offsets into a segment)
+  uint32_t rounds;   // pointer-doubling rounds: 2^rounds >= kParseSeg / smallest size + 1
+  int has_bc;        // full blocks carry a bit-count byte
+  int topo;          // blocks carry 2-bit types
+};
+size_t parse_work_bytes(uint64_t len, uint32_t maxb);
+// err: 2 zeroed u32 (error bits, "last block reached")
+hipError_t parse_blocks(const ParseArgs& a, uint64_t n, uint32_t bs, uint32_t nb, uint32_t* blk, void* work,
+                        uint32_t* err, hipStream_t st);
+hipError_t parse_check_offsets(const uint8_t* head, uint32_t T, const uint32_t* blk, uint32_t nb, uint32_t* err,
+                               hipStream_t st);
+
 // ---- topology restoration (tszp_post.hip) ----
 // Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.
 constexpr int kPpMiscWords = 64;

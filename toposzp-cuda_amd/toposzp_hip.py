@@ -112,6 +112,8 @@ def lib():
     L.tszp_topo_restore.restype = i32
     L.tszp_topo_verify.argtypes = [vp, vp, i32, vp, u64p, vp]
     L.tszp_topo_verify.restype = i32
+    L.tszp_parse_blocks.argtypes = [vp, sz, sz, i32, i32, vp, C.POINTER(u32), vp]
+    L.tszp_parse_blocks.restype = i32
     L.tszp_szp_pack.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz), vp]
     L.tszp_szp_pack.restype = i32
     L.tszp_szp_unpack.argtypes = [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), u64p]
@@ -419,3 +421,20 @@ def szp_unpack(buf, length=None):
     tb, so, sb, e = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
     _check(lib().tszp_szp_unpack(_addr(buf), n, C.byref(tb), C.byref(so), C.byref(sb), C.byref(e)))
     return tb.value, so.value, sb.value, e.value
+
+
+# ------------------------------------------------------------------------------------------
+# streams without a side-car, parsed on the device
+# ------------------------------------------------------------------------------------------
+PARSE_KINDS = {"topo": 0, "sortpos": 1, "plain": 2}
+
+
+def parse_blocks(stream, n, bs, kind, length=None):
+    """Block offsets (int32 tensor, tszp_blocks(n, bs) + 1) and chunk count of a device
+    stream written without a side-car (e.g. read from a file)."""
+    torch = _torch()
+    blk = torch.empty(lib().tszp_blocks(n, bs) + 1, dtype=torch.int32, device=stream.device)
+    T = C.c_uint32()
+    _check(lib().tszp_parse_blocks(_dp(stream), length if length is not None else stream.numel(), n, bs,
+                                   PARSE_KINDS[kind], _dp(blk), C.byref(T), _stream()))
+    return blk, T.value
