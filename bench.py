@@ -248,6 +248,17 @@ def main():
     plain_gbs = 4.0 * n / ((time.perf_counter() - t0) / args.steps) / 1e9
     log(f"plain SZp compress+decompress {plain_gbs:.1f} GB/s (ratio {4 * n / nbytes:.2f})")
 
+    # side figure: topology quality of the restored field against the original
+    # (the CUDA driver's report), one GPU pass
+    tz.quality_metrics(f, restored, shape, eb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    quality = tz.quality_metrics(f, restored, shape, eb)
+    quality_ms = 1e3 * (time.perf_counter() - t0)
+    log(f"quality metrics {quality_ms:.3f} ms: preserved {quality['preserved']} of {quality['orig_cp'][0]}, "
+        f"lost {quality['lost'][0]}, changed {quality['changed'][0]}, new {quality['new_cps']}, "
+        f"max err {quality['max_err'] / eb:.3f} eb, order violations {quality['violations'][0]} of {quality['pairs'][0]}")
+
     # side figure: block offsets of the topology stream parsed on the device, as for a
     # stream read from a file without its side-car
     info, _ = tz.topo_compress(f, eb, bs, T=1, buf=buf)
@@ -299,6 +310,10 @@ def main():
         "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
         "plain_gbs": plain_gbs,
         "topo_stream_parse_ms": parse_ms,
+        "quality_ms": quality_ms,
+        "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
+                                            "over_1eb", "pairs", "violations", "saddles_above",
+                                            "saddles_kept_above")},
         "restore": ({"missing": last["restore"].missing, "spurious": last["restore"].spurious,
                      "changed": last["restore"].changed, "moves": last["restore"].moves,
                      "windows": last["restore"].windows, "rounds": last["restore"].rounds,

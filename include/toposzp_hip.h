@@ -119,6 +119,31 @@ int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_t
 int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, const uint32_t *dims,
                      uint64_t *counts, void *stream);
 
+/* Topology quality of a decompressed field y against the original x, as the
+ * reference's CUDA driver reports it (example/test_cuda_topology.cu:540-868), for 2D
+ * and 3D fields, in one GPU pass.  Critical points follow the codec's max / min /
+ * saddle rule on interior points. */
+typedef struct {
+  uint64_t orig_cp[4];          /* original field: total, maxima, minima, saddles */
+  uint64_t dec_cp[4];           /* decompressed field: the same */
+  uint64_t preserved;           /* original CPs with the same type in y */
+  uint64_t new_cps;             /* CPs of y at regular points of x */
+  uint64_t lost[4];             /* original CPs that became regular: total, by type 1..3 */
+  uint64_t changed[4];          /* original CPs of another type in y: total, by type */
+  uint64_t over_1eb, over_2eb;  /* points with |x - y| > 1.01 eb, > 2 eb (in double) */
+  uint64_t saddles;             /* saddles of x */
+  uint64_t saddles_above[6];    /* persistence proxy min |x_c - x_nb| > 0, .5, 1, 2, 5, 10 eb */
+  uint64_t saddles_kept_above[6]; /* of those, saddles that are saddles in y */
+  uint64_t pairs[4];            /* adjacent interior pairs with x_a != x_b: all, >= 1 CP,
+                                   both CPs, >= 1 extremum (forward neighbour per axis) */
+  uint64_t violations[4];       /* of those, pairs whose order in y differs */
+  double max_err;               /* max |x - y| */
+  double pers_min, pers_max, pers_sum; /* persistence proxy over the saddles of x */
+} tszp_quality;
+
+int tszp_quality_metrics(const float *d_x, const float *d_y, int ndim, const uint32_t *dims, float eb,
+                         tszp_quality *q, void *stream);
+
 /* Block offsets of a device-resident stream that has no side-car (a stream read from a
  * file written by the reference), parsed on the device: replaces the host walk of
  * SZp/src/szp_cuda_decompress.cu:391-477 and detects the chunk count T as

@@ -205,3 +205,71 @@ def detect_T_topo(cmp, n, bs):
 def check_bitcount_table(thr, lo, hi):
     t = np.ascontiguousarray(thr, dtype=np.uint32)
     return int(lib().or_check_bitcount_table(_p(t), lo, hi))
+
+
+def quality_metrics(x, y, eb):
+    """Topology quality of y against x, restating the CUDA driver's report
+    (example/test_cuda_topology.cu:540-868: steps 5, 6, 8, 9 and 10), written for 2D and
+    3D: critical points by the codec rule (classify above), preserved / lost / changed /
+    new; pointwise error (double) against 1.01 eb and 2 eb; the saddle persistence proxy
+    min |x_c - x_nb| over the stencil (:687-703) with thresholds 0, .5, 1, 2, 5, 10 eb
+    (:716-735); local order of each interior point with its forward neighbour along
+    every axis when that neighbour is interior too (:778-826)."""
+    x = _f32(x)
+    y = _f32(y)
+    tx = classify(x).astype(np.int64)
+    ty = classify(y).astype(np.int64)
+    q = {}
+    q["orig_cp"] = [int((tx != 0).sum())] + [int((tx == k).sum()) for k in (1, 2, 3)]
+    q["dec_cp"] = [int((ty != 0).sum())] + [int((ty == k).sum()) for k in (1, 2, 3)]
+    cp = tx != 0
+    q["preserved"] = int((cp & (ty == tx)).sum())
+    q["new_cps"] = int(((tx == 0) & (ty != 0)).sum())
+    lost = cp & (ty == 0)
+    chg = cp & (ty != 0) & (ty != tx)
+    q["lost"] = [int(lost.sum())] + [int((lost & (tx == k)).sum()) for k in (1, 2, 3)]
+    q["changed"] = [int(chg.sum())] + [int((chg & (tx == k)).sum()) for k in (1, 2, 3)]
+    err = np.abs(x.astype(np.float64) - y.astype(np.float64))
+    q["over_1eb"] = int((err > float(eb) * 1.01).sum())
+    q["over_2eb"] = int((err > float(eb) * 2.0).sum())
+    q["max_err"] = float(err.max())
+    # persistence proxy of the saddles (all interior by the rule)
+    sad = np.argwhere(tx == 3)
+    xd = x.astype(np.float64)
+    pers = np.full(len(sad), np.inf)
+    for a in range(x.ndim):
+        for s in (-1, 1):
+            nb = sad.copy()
+            nb[:, a] += s
+            pers = np.minimum(pers, np.abs(xd[tuple(sad.T)] - xd[tuple(nb.T)]))
+    kept = ty[tuple(sad.T)] == 3 if len(sad) else np.zeros(0, bool)
+    q["saddles"] = int(len(sad))
+    thr = [0.0, 0.5, 1.0, 2.0, 5.0, 10.0]
+    q["saddles_above"] = [int((pers > t * float(eb)).sum()) for t in thr]
+    q["saddles_kept_above"] = [int(((pers > t * float(eb)) & kept).sum()) for t in thr]
+    q["pers_min"] = float(pers.min()) if len(sad) else 1e300
+    q["pers_max"] = float(pers.max()) if len(sad) else 0.0
+    q["pers_sum"] = float(pers.sum()) if len(sad) else 0.0
+    # local order: interior a with forward neighbour b along each axis, b interior
+    pairs = [0, 0, 0, 0]
+    viol = [0, 0, 0, 0]
+    inner = tuple(slice(1, d - 1) for d in x.shape)
+    for a in range(x.ndim):
+        sa = list(inner)
+        sb = list(inner)
+        sa[a] = slice(1, x.shape[a] - 2)
+        sb[a] = slice(2, x.shape[a] - 1)
+        xa, xb, ya, yb = x[tuple(sa)], x[tuple(sb)], y[tuple(sa)], y[tuple(sb)]
+        ta, tb = tx[tuple(sa)], tx[tuple(sb)]
+        strict = xa != xb
+        oo = np.where(xa < xb, -1, 1)
+        od = np.where(ya < yb, -1, np.where(ya > yb, 1, 0))
+        v = strict & (oo != od)
+        groups = [np.ones_like(strict), (ta != 0) | (tb != 0), (ta != 0) & (tb != 0),
+                  (ta == 1) | (ta == 2) | (tb == 1) | (tb == 2)]
+        for k, gmask in enumerate(groups):
+            pairs[k] += int((strict & gmask).sum())
+            viol[k] += int((v & gmask).sum())
+    q["pairs"] = pairs
+    q["violations"] = viol
+    return q

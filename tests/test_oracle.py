@@ -204,3 +204,41 @@ def test_bitcount_table_exhaustive():
     with ProcessPoolExecutor(8) as ex:
         bad = sum(ex.map(_chunk, [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]))
     assert bad == 0
+
+
+# ------------------------------------------------------------------------------------------
+# quality metrics (example/test_cuda_topology.cu:540-868) on hand-made fields
+# ------------------------------------------------------------------------------------------
+def test_quality_metrics_known_answers():
+    # 5x5: one maximum at (2, 2); the decompressed copy moves it one step right
+    x = np.zeros((5, 5), np.float32)
+    x[2, 2] = 1.0
+    y = x.copy()
+    y[2, 2] = 0.0
+    y[2, 3] = 1.0 + 1e-6  # (2, 3) is interior? cols 5 -> interior columns 1..3: yes
+    q = oracle.quality_metrics(x, y, 0.5)
+    assert q["orig_cp"] == [1, 1, 0, 0]
+    assert q["dec_cp"][0] == 1 and q["dec_cp"][1] == 1
+    assert q["lost"] == [1, 1, 0, 0] and q["preserved"] == 0 and q["new_cps"] == 1
+    # errors 1.0 at (2, 2) and 1.0000010 at (2, 3): both above 1.01 eb, one above 2 eb
+    assert q["over_1eb"] == 2 and q["over_2eb"] == 1 and abs(q["max_err"] - (1.0 + 1e-6)) < 1e-6
+    # local order: interior 3x3, pairs along rows (2 per row, 3 rows) and columns: 12;
+    # strict in x only where the max is involved: (2,1)-(2,2), (2,2)-(2,3), (1,2)-(2,2),
+    # (2,2)-(3,2); in y (2,1)-(2,2) and (1,2)-(2,2), (2,2)-(3,2) become equal (violations),
+    # (2,2)-(2,3) flips
+    assert q["pairs"] == [4, 4, 0, 4] and q["violations"] == [4, 4, 0, 4]
+
+
+def test_quality_metrics_saddle_persistence():
+    # a 2D saddle at (2, 2): higher along rows, lower along columns
+    x = np.zeros((5, 5), np.float32)
+    x[2, 2] = 0.5
+    x[2, 1] = x[2, 3] = 0.2   # row neighbours lower
+    x[1, 2] = x[3, 2] = 0.9   # column neighbours higher
+    q = oracle.quality_metrics(x, x.copy(), 0.1)
+    assert q["saddles"] == 1 and q["orig_cp"][3] == 1
+    assert abs(q["pers_min"] - 0.3) < 1e-6  # min(|0.5-0.2|, |0.5-0.9|)
+    # 0.3 > 0, 0.05, 0.1, 0.2 but not > 0.5 or 1.0
+    assert q["saddles_above"] == [1, 1, 1, 1, 0, 0]
+    assert q["saddles_kept_above"] == [1, 1, 1, 1, 0, 0]
+    assert q["violations"] == [0, 0, 0, 0] and q["lost"][0] == 0

@@ -48,7 +48,7 @@ enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
-  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap,
+  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsCount
 };
 
@@ -987,6 +987,30 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   HIPCHK(hipMemcpyAsync(hv, misc + kPpVerify, 32, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int k = 0; k < 3; k++) counts[k] = hv[k];
+  return TSZP_OK;
+}
+
+int tszp_quality_metrics(const float* d_x, const float* d_y, int ndim, const uint32_t* dims, float eb,
+                         tszp_quality* q, void* stream) {
+  g_err.clear();
+  static_assert(sizeof(tszp_quality) == (kQualCounts + 4) * 8, "tszp_quality layout");
+  if (!d_x || !d_y || !dims || !q || (ndim != 2 && ndim != 3) || !(eb > 0))
+    return fail(TSZP_EINVAL, "quality_metrics: bad arguments");
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t nblocks = 1024;
+  uint64_t* part;
+  WS(kWsQuality, ((size_t)nblocks + 1) * kQualWords * 8, part);
+  uint64_t* out = part + (size_t)nblocks * kQualWords;
+  HIPCHK(launch_quality(make_grid(ndim, dims), d_x, d_y, (uint32_t)n, eb, part, nblocks, out, st));
+  HIPCHK(hipMemcpyAsync(q, out, sizeof(tszp_quality), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return TSZP_OK;
 }
 

@@ -178,6 +178,16 @@ hipError_t parse_blocks(const ParseArgs& a, uint64_t n, uint32_t bs, uint32_t nb
 hipError_t parse_check_offsets(const uint8_t* head, uint32_t T, const uint32_t* blk, uint32_t nb, uint32_t* err,
                                hipStream_t st);
 
+// ---- quality metrics (tszp_quality.hip); word layout of tszp_quality ----
+constexpr int kQualThresholds = 6;
+enum : int {
+  kQOrig = 0, kQDec = 4, kQPreserved = 8, kQNew = 9, kQLost = 10, kQChanged = 14, kQOver1 = 18, kQOver2 = 19,
+  kQSaddles = 20, kQAbove = 21, kQAboveKept = 27, kQPairs = 33, kQViol = 37, kQualCounts = 41,
+  kQualWords = 48  // counts, then 4 doubles (max error, persistence min / max / sum), padding
+};
+hipError_t launch_quality(const Grid& g, const float* x, const float* y, uint32_t n, float eb, uint64_t* part,
+                          uint32_t nblocks, uint64_t* out, hipStream_t st);
+
 // ---- topology restoration (tszp_post.hip) ----
 // Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.
 constexpr int kPpMiscWords = 64;

@@ -44,6 +44,23 @@ _libc = C.CDLL("libc.so.6")
 _libc.free.argtypes = [C.c_void_p]
 
 
+class Quality(C.Structure):
+    """tszp_quality (include/toposzp_hip.h)."""
+    _fields_ = [("orig_cp", C.c_uint64 * 4), ("dec_cp", C.c_uint64 * 4), ("preserved", C.c_uint64),
+                ("new_cps", C.c_uint64), ("lost", C.c_uint64 * 4), ("changed", C.c_uint64 * 4),
+                ("over_1eb", C.c_uint64), ("over_2eb", C.c_uint64), ("saddles", C.c_uint64),
+                ("saddles_above", C.c_uint64 * 6), ("saddles_kept_above", C.c_uint64 * 6),
+                ("pairs", C.c_uint64 * 4), ("violations", C.c_uint64 * 4), ("max_err", C.c_double),
+                ("pers_min", C.c_double), ("pers_max", C.c_double), ("pers_sum", C.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
 def lib():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
@@ -112,6 +129,8 @@ def lib():
     L.tszp_topo_restore.restype = i32
     L.tszp_topo_verify.argtypes = [vp, vp, i32, vp, u64p, vp]
     L.tszp_topo_verify.restype = i32
+    L.tszp_quality_metrics.argtypes = [vp, vp, i32, vp, f32, C.POINTER(Quality), vp]
+    L.tszp_quality_metrics.restype = i32
     L.tszp_parse_blocks.argtypes = [vp, sz, sz, i32, i32, vp, C.POINTER(u32), vp]
     L.tszp_parse_blocks.restype = i32
     L.tszp_szp_pack.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz), vp]
@@ -438,3 +457,10 @@ def parse_blocks(stream, n, bs, kind, length=None):
     _check(lib().tszp_parse_blocks(_dp(stream), length if length is not None else stream.numel(), n, bs,
                                    PARSE_KINDS[kind], _dp(blk), C.byref(T), _stream()))
     return blk, T.value
+
+
+def quality_metrics(x, y, shape, eb):
+    """Topology quality of y against x (device tensors of `shape`), as a dict."""
+    q = Quality()
+    _check(lib().tszp_quality_metrics(_dp(x), _dp(y), len(shape), _dims(shape), eb, C.byref(q), _stream()))
+    return q.as_dict()
