@@ -81,6 +81,48 @@ int tszp_plain_compress(const float *d_in, size_t n, float eb, int block_size, u
 int tszp_plain_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
                           float eb, int block_size, float *d_out, void *stream);
 
+/* Streams partitioned by elements: chunk t of nchunks covers [t * (n / nchunks), ...),
+ * the last chunk up to n; header [eb big-endian, elem_bytes wide][u64 off[nchunks]].
+ * d_blk receives tszp_elem_units(n, block_size, nchunks, ra) + 1 unit offsets relative
+ * to the payload (the side-car the decoders take).  block_size 1 .. 1025, nchunks
+ * 1 .. 4096, elem_bytes 4 (float data) or 8 (double data).
+ *
+ * Threadblock (non-random-access) codec: per chunk one i32 anchor, then blocks of up to
+ * block_size deltas chained across blocks.  Replaces szp_float_openmp_threadblock /
+ * szp_double_openmp_threadblock (SZp/src/szp_float.cc:1445, szp_double.cc:148; CUDA
+ * szp_cuda_{float,double}_compress_threadblock, SZp/src/szp_cuda_compress.cu:752-782)
+ * and their decoders (SZp/src/szpd_float.cc:28, szpd_double.cc:6; CUDA
+ * szp_cuda_{float,double}_decompress_threadblock, szp_cuda_decompress.cu:1032-1060). */
+size_t tszp_elem_units(size_t n, int block_size, unsigned nchunks, int ra);
+/* stream capacity that always suffices */
+size_t tszp_elem_bound(size_t n, int block_size, unsigned nchunks, int elem_bytes, int ra);
+int tszp_threadblock_compress(const void *d_in, int elem_bytes, size_t n, double eb, int block_size,
+                              unsigned nchunks, uint8_t *d_out, size_t cap, uint32_t *d_blk,
+                              uint64_t *out_bytes, void *stream);
+int tszp_threadblock_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
+                                double eb, int block_size, int elem_bytes, void *d_out, void *stream);
+
+/* Double random-access codec: blocks of block_size elements from each chunk start, each
+ * with its own i32 anchor and a bc byte (also for a one-element block).  Replaces
+ * szp_double_openmp_threadblock_randomaccess (SZp/src/szp_double.cc:759) and decodes
+ * what it writes (SZp/src/szpd_double.cc:427 reads it back for nchunks = 1). */
+int tszp_plain_compress_f64(const double *d_in, size_t n, double eb, int block_size, unsigned nchunks,
+                            uint8_t *d_out, size_t cap, uint32_t *d_blk, uint64_t *out_bytes, void *stream);
+int tszp_plain_decompress_f64(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
+                              double eb, int block_size, double *d_out, void *stream);
+
+/* Double-precision 3D topology stream [u64 0][blocks] with the saddle fix
+ * (szp_cuda_double_compress_topology_3d, SZp/src/szp_cuda_topology_3d.cu:1065-1188):
+ * types marked from the critical-point records d_cps (cp_count records of 6 ints:
+ * x, y, z, type, bin, sort position -- CriticalPoint3D), q = C cast of x * (1 / eb),
+ * three Jacobi passes moving broken saddles by one step, then the topology block codec
+ * on q with the marked types.  d_blk receives tszp_blocks(n, block_size) + 1 offsets;
+ * d_q (nullable) the fixed quantized field; *moved (nullable) the saddles moved. */
+int tszp_topo_compress_f64_3d(const double *d_field, const uint32_t *dims, double eb, int block_size,
+                              const int32_t *d_cps, uint64_t cp_count, uint8_t *d_out, size_t cap,
+                              uint32_t *d_blk, uint64_t *out_bytes, int32_t *d_q, uint64_t *moved,
+                              void *stream);
+
 /* Per-point critical-point types of a field (0 regular / boundary, 1 max, 2 min,
  * 3 saddle); *cp_count (nullable, host) gets the number of critical points. */
 int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t *d_types,
@@ -176,7 +218,9 @@ int tszp_szp_unpack(const uint8_t *buf, size_t len, size_t *topo_bytes, size_t *
  * 3 decoders, 4 sort-position bucketing (bins, scatter, bitmaps; or the radix sort),
  * 5 rank queries (and the radix sort of the bins next to zero; or the rank scan),
  * 6 topology restoration (whole call), 7 each scatter launch (bin scatter or radix
- * pass), 8 each window-solve launch, 9 each bin rank-query launch.  tszp_prof_read
+ * pass), 8 each window-solve launch, 9 each bin rank-query launch, 10 threadblock /
+ * double random-access compression, 11 their decoder, 12 double 3D quantization and
+ * saddle fix.  tszp_prof_read
  * returns the summed milliseconds and the number of timed launches since
  * tszp_prof_reset. */
 void tszp_prof_enable(int on);

@@ -669,3 +669,281 @@ int or_block_offsets(const unsigned char *payload, size_t n, int bs, int topo, i
     offs[nb] = pos;
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* element-partitioned writers: the threadblock (chained) codec and the     */
+/* double random-access codec                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Chunk t of T covers elements [t * (n / T), (t + 1) * (n / T)), the last chunk up
+ * to n (SZp/src/szp_float.cc:1489-1504, SZp/src/szp_double.cc:188-199, :790-800). */
+static void elem_chunk(size_t n, unsigned T, unsigned t, size_t *lo, size_t *hi)
+{
+    size_t ce = n / T;
+    *lo = (size_t)t * ce;
+    *hi = (t == T - 1) ? n : (size_t)(t + 1) * ce;
+}
+
+/* [bc][signs][magnitudes] of the m deltas q[j] - q[j - 1], j = 0 .. m - 1 (q[-1] is
+ * read); bc only when m == 0 and `bc_always` is false. */
+static size_t put_run(unsigned char *p, const int *q, size_t m, int bc_always)
+{
+    unsigned char *s = p;
+    if (m == 0 && !bc_always) return 0;
+    unsigned *mags = (unsigned *)malloc((m + 1) * sizeof(unsigned));
+    unsigned char *neg = (unsigned char *)malloc(m + 1);
+    unsigned maxv = 0;
+    for (size_t j = 0; j < m; j++) {
+        unsigned d = (unsigned)q[j] - (unsigned)q[(ptrdiff_t)j - 1];
+        int sd = (int)d;
+        neg[j] = sd < 0;
+        mags[j] = sd < 0 ? 0u - d : d;
+        if (mags[j] > maxv) maxv = mags[j];
+    }
+    unsigned bc = or_bitcount(maxv);
+    *p++ = (unsigned char)bc;
+    if (bc) {
+        size_t sb = (m + 7) / 8;
+        memset(p, 0, sb);
+        for (size_t j = 0; j < m; j++)
+            if (neg[j]) p[j >> 3] |= (unsigned char)(0x80u >> (j & 7));
+        p += sb;
+        unsigned nfull = bc / 8, rem = bc % 8;
+        for (size_t j = 0; j < m; j++) {
+            unsigned v = mags[j] >> rem;
+            for (unsigned k = 0; k < nfull; k++) { *p++ = (unsigned char)v; v >>= 8; }
+        }
+        if (rem) {
+            size_t rb = (rem * m + 7) / 8, pos = 0;
+            memset(p, 0, rb);
+            for (size_t j = 0; j < m; j++) put_bits_msb(p, &pos, mags[j] & ((1u << rem) - 1u), rem);
+            p += rb;
+        }
+    }
+    free(mags);
+    free(neg);
+    return (size_t)(p - s);
+}
+
+/* inverse of put_run: q[j] = q[j - 1] + delta_j; returns bytes consumed */
+static size_t get_run(const unsigned char *p, int *q, size_t m, int bc_always)
+{
+    const unsigned char *s = p;
+    if (m == 0 && !bc_always) return 0;
+    unsigned bc = *p++;
+    int prior = q[-1];
+    if (bc == 0) {
+        for (size_t j = 0; j < m; j++) q[j] = prior;
+        return (size_t)(p - s);
+    }
+    const unsigned char *sg = p;
+    p += (m + 7) / 8;
+    unsigned nfull = bc / 8, rem = bc % 8;
+    const unsigned char *full = p, *rb = p + (size_t)nfull * m;
+    for (size_t j = 0; j < m; j++) {
+        unsigned v = 0;
+        for (unsigned k = 0; k < nfull; k++) v |= (unsigned)full[j * nfull + k] << (8 * k);
+        v <<= rem;
+        if (rem) v |= get_bits_msb(rb, j * rem, rem);
+        int ng = (sg[j >> 3] >> (7 - (j & 7))) & 1;
+        int d = ng ? -(int)v : (int)v;
+        prior = (int)((unsigned)prior + (unsigned)d);
+        q[j] = prior;
+    }
+    p += mag_bytes(m, bc);
+    return (size_t)(p - s);
+}
+
+static void put_be(unsigned char *out, const void *v, int size)
+{
+    const unsigned char *b = (const unsigned char *)v;
+    for (int k = 0; k < size; k++) out[k] = b[size - 1 - k];
+}
+
+/* One chunk of the element-partitioned stream.
+ *   ra = 0: threadblock -- [i32 q[lo]] then blocks of bs deltas from lo + 1, chained
+ *           across blocks (SZp/src/szp_float.cc:1505-1570, szp_double.cc:200-278);
+ *   ra = 1: double random access -- blocks of bs elements from lo, each
+ *           [i32 anchor][bc][signs][magnitudes] over its cur - 1 deltas, the bc byte
+ *           also for a one-element block (SZp/src/szp_double.cc:810-880). */
+static size_t put_elem_chunk(unsigned char *p, const int *q, size_t lo, size_t hi, int bs, int ra)
+{
+    unsigned char *s = p;
+    if (lo >= hi) return 0;
+    if (ra) {
+        for (size_t i = lo; i < hi; i += (size_t)bs) {
+            size_t cur = (i + bs > hi) ? hi - i : (size_t)bs;
+            memcpy(p, &q[i], 4);
+            p += 4;
+            p += put_run(p, q + i + 1, cur - 1, 1);
+        }
+    } else {
+        memcpy(p, &q[lo], 4);
+        p += 4;
+        for (size_t i = lo + 1; i < hi; i += (size_t)bs) {
+            size_t cur = (i + bs > hi) ? hi - i : (size_t)bs;
+            p += put_run(p, q + i, cur, 0);
+        }
+    }
+    return (size_t)(p - s);
+}
+
+static size_t get_elem_chunk(const unsigned char *p, int *q, size_t lo, size_t hi, int bs, int ra)
+{
+    const unsigned char *s = p;
+    if (lo >= hi) return 0;
+    if (ra) {
+        for (size_t i = lo; i < hi; i += (size_t)bs) {
+            size_t cur = (i + bs > hi) ? hi - i : (size_t)bs;
+            memcpy(&q[i], p, 4);
+            p += 4;
+            p += get_run(p, q + i + 1, cur - 1, 1);
+        }
+    } else {
+        memcpy(&q[lo], p, 4);
+        p += 4;
+        for (size_t i = lo + 1; i < hi; i += (size_t)bs) {
+            size_t cur = (i + bs > hi) ? hi - i : (size_t)bs;
+            p += get_run(p, q + i, cur, 0);
+        }
+    }
+    return (size_t)(p - s);
+}
+
+size_t or_elem_bound(size_t n, int bs, unsigned T, int dsize, int ra)
+{
+    size_t nb = (n + bs - 1) / bs + T;
+    return (size_t)dsize + 8 * (size_t)T + 4 * (size_t)T + nb * (4 + 1 + ((size_t)bs + 7) / 8 + 4 * (size_t)bs) +
+           (ra ? 0 : 4 * n);
+}
+
+/* Writers: [eb big-endian (4 or 8 bytes)][u64 off[T]][chunks].  Float: the reciprocal
+ * is an int / float division, widened (SZp/src/szp_float.cc:1489); double: 1 / eb
+ * (SZp/src/szp_double.cc:188, :787).  q = C cast of (double)x * inv. */
+size_t or_elem_compress(const void *data, int dsize, size_t n, double eb, int bs, unsigned T, int ra,
+                        unsigned char *out)
+{
+    if (n == 0 || bs < 1 || T == 0 || (dsize != 4 && dsize != 8) || !(eb > 0)) return 0;
+    double inv;
+    if (dsize == 4) {
+        float ebf = (float)eb;
+        float finv = 1 / ebf;
+        inv = (double)finv;
+        put_be(out, &ebf, 4);
+    } else {
+        inv = 1 / eb;
+        put_be(out, &eb, 8);
+    }
+    int *q = (int *)malloc(n * sizeof(int));
+    for (size_t i = 0; i < n; i++) {
+        double x = dsize == 4 ? (double)((const float *)data)[i] : ((const double *)data)[i];
+        q[i] = or_cvt_i32(x * inv);
+    }
+    unsigned char *hdr = out + dsize, *payload = hdr + 8 * (size_t)T;
+    uint64_t pos = 0;
+    for (unsigned t = 0; t < T; t++) {
+        size_t lo, hi;
+        elem_chunk(n, T, t, &lo, &hi);
+        memcpy(hdr + 8 * (size_t)t, &pos, 8);
+        pos += put_elem_chunk(payload + pos, q, lo, hi, bs, ra);
+    }
+    free(q);
+    return (size_t)dsize + 8 * (size_t)T + pos;
+}
+
+/* Decoders of the writers above (SZp/src/szpd_float.cc:28-135, szpd_double.cc:6-110):
+ * `cmp` starts at the offset table, the chunk count is given.  Float values are
+ * (float)q * eb in float, double values (double)q * eb. */
+int or_elem_decompress(const unsigned char *cmp, size_t n, double eb, int bs, unsigned T, int ra, int dsize,
+                       void *out)
+{
+    if (n == 0 || bs < 1 || T == 0 || (dsize != 4 && dsize != 8)) return -1;
+    int *q = (int *)malloc(n * sizeof(int));
+    const unsigned char *rcp = cmp + 8 * (size_t)T;
+    for (unsigned t = 0; t < T; t++) {
+        size_t lo, hi;
+        elem_chunk(n, T, t, &lo, &hi);
+        get_elem_chunk(rcp + rd64(cmp + 8 * (size_t)t), q, lo, hi, bs, ra);
+    }
+    if (dsize == 4) {
+        float ebf = (float)eb;
+        for (size_t i = 0; i < n; i++) ((float *)out)[i] = (float)q[i] * ebf;
+    } else {
+        for (size_t i = 0; i < n; i++) ((double *)out)[i] = (double)q[i] * eb;
+    }
+    free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* double-precision 3D topology stream (SZp/src/szp_cuda_topology_3d.cu)      */
+/* ------------------------------------------------------------------------ */
+
+/* One Jacobi pass of the saddle fix (:736-826): every interior point typed 3 whose
+ * quantized 6-neighbourhood (read from the pass's snapshot) is no longer a saddle
+ * takes the first of c - 1, c + 1 that is high along one axis and low along another
+ * without reaching a neighbouring maximum or minimum. Returns the points changed. */
+size_t or_fix_saddles_3d_pass(const int *src, int *dst, const unsigned char *types, int d1, int d2, int d3)
+{
+    size_t changed = 0;
+    const size_t pl = (size_t)d2 * d3;
+    for (int x = 1; x + 1 < d1; x++)
+        for (int y = 1; y + 1 < d2; y++)
+            for (int z = 1; z + 1 < d3; z++) {
+                size_t f = (size_t)x * pl + (size_t)y * d3 + z;
+                if (types[f] != 3) continue;
+                const size_t o[6] = {f - pl, f + pl, f - d3, f + d3, f - 1, f + 1};
+                int c = src[f], v[6];
+                for (int k = 0; k < 6; k++) v[k] = src[o[k]];
+                int hi = 0, lo = 0;
+                for (int a = 0; a < 3; a++) {
+                    hi += c > v[2 * a] && c > v[2 * a + 1];
+                    lo += c < v[2 * a] && c < v[2 * a + 1];
+                }
+                if (hi >= 1 && lo >= 1) continue;
+                const int trials[2] = {(int)((unsigned)c - 1u), (int)((unsigned)c + 1u)};
+                for (int k = 0; k < 2; k++) {
+                    int t = trials[k], th = 0, tl = 0, ok = 1;
+                    for (int a = 0; a < 3; a++) {
+                        int mx = v[2 * a] > v[2 * a + 1] ? v[2 * a] : v[2 * a + 1];
+                        int mn = v[2 * a] < v[2 * a + 1] ? v[2 * a] : v[2 * a + 1];
+                        th += t > mx;
+                        tl += t < mn;
+                    }
+                    if (!(th >= 1 && tl >= 1)) continue;
+                    for (int j = 0; j < 6; j++) {
+                        if (types[o[j]] == 1 && v[j] <= t) ok = 0;
+                        if (types[o[j]] == 2 && v[j] >= t) ok = 0;
+                    }
+                    if (ok) {
+                        dst[f] = t;
+                        changed++;
+                        break;
+                    }
+                }
+            }
+    return changed;
+}
+
+/* szp_cuda_double_compress_topology_3d (:1065-1188): q = C cast of x * (1 / eb) in
+ * double, three saddle-fix passes, then the topology writer with one chunk:
+ * [u64 0][blocks]. `types` is the type map marked from the critical-point list.
+ * q_out (nullable) receives the fixed quantized field. */
+size_t or_topo3d_f64_compress(const double *data, int d1, int d2, int d3, double eb, int bs,
+                              const unsigned char *types, unsigned char *out, int *q_out)
+{
+    size_t n = (size_t)d1 * d2 * d3;
+    if (n == 0 || !(eb > 0) || bs < 1 || bs > 1025) return 0;
+    double inv = 1.0 / eb;
+    int *q = (int *)malloc(n * sizeof(int)), *snap = (int *)malloc(n * sizeof(int));
+    for (size_t i = 0; i < n; i++) q[i] = or_cvt_i32(data[i] * inv);
+    for (int pass = 0; pass < 3; pass++) {
+        memcpy(snap, q, n * sizeof(int));
+        or_fix_saddles_3d_pass(snap, q, types, d1, d2, d3);
+    }
+    size_t r = write_chunks(out, q, n, bs, types, 1, 0);
+    if (q_out) memcpy(q_out, q, n * sizeof(int));
+    free(q);
+    free(snap);
+    return r;
+}

@@ -90,6 +90,22 @@ int or_block_offsets(const unsigned char *payload, size_t n, int bs, int topo,
 void or_classify_2d(const float *data, int rows, int cols, unsigned char *types);
 void or_classify_3d(const float *data, int d1, int d2, int d3, unsigned char *types);
 
+/* Element-partitioned streams [eb big-endian][u64 off[T]][chunks], chunk t covering
+ * [t * (n / T), ...): ra = 0 the threadblock codec (SZp/src/szp_float.cc:1445-1590,
+ * SZp/src/szp_double.cc:148-318), ra = 1 the double random-access codec
+ * (SZp/src/szp_double.cc:759-920). dsize 4 = float data, 8 = double data. */
+size_t or_elem_bound(size_t n, int bs, unsigned T, int dsize, int ra);
+size_t or_elem_compress(const void *data, int dsize, size_t n, double eb, int bs, unsigned T, int ra,
+                        unsigned char *out);
+int or_elem_decompress(const unsigned char *cmp, size_t n, double eb, int bs, unsigned T, int ra, int dsize,
+                       void *out);
+
+/* Double 3D topology stream with the saddle fix (SZp/src/szp_cuda_topology_3d.cu:736-826,
+ * :1065-1188). */
+size_t or_fix_saddles_3d_pass(const int *src, int *dst, const unsigned char *types, int d1, int d2, int d3);
+size_t or_topo3d_f64_compress(const double *data, int d1, int d2, int d3, double eb, int bs,
+                              const unsigned char *types, unsigned char *out, int *q_out);
+
 #ifdef __cplusplus
 }
 #endif

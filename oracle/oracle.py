@@ -66,6 +66,16 @@ def lib():
         L.or_classify_2d.argtypes = [vp, i32, i32, vp]; L.or_classify_2d.restype = None
         L.or_classify_3d.argtypes = [vp, i32, i32, i32, vp]; L.or_classify_3d.restype = None
         L.or_chunk_range.argtypes = [sz, u32, i32, u32, vp, vp]; L.or_chunk_range.restype = None
+        f64 = C.c_double
+        L.or_elem_bound.argtypes = [sz, i32, u32, i32, i32]; L.or_elem_bound.restype = sz
+        L.or_elem_compress.argtypes = [vp, i32, sz, f64, i32, u32, i32, vp]
+        L.or_elem_compress.restype = sz
+        L.or_elem_decompress.argtypes = [vp, sz, f64, i32, u32, i32, i32, vp]
+        L.or_elem_decompress.restype = i32
+        L.or_fix_saddles_3d_pass.argtypes = [vp, vp, vp, i32, i32, i32]
+        L.or_fix_saddles_3d_pass.restype = sz
+        L.or_topo3d_f64_compress.argtypes = [vp, i32, i32, i32, f64, i32, vp, vp, vp]
+        L.or_topo3d_f64_compress.restype = sz
         _lib = L
     return _lib
 
@@ -176,6 +186,45 @@ def plain_decompress(cmp_after_header, n, eb, bs):
     out = np.zeros(n, dtype=np.float32)
     lib().or_plain_decompress(_p(c), n, float(eb), bs, _p(out))
     return out
+
+
+def elem_compress(data, eb, bs, T=1, ra=False):
+    """Threadblock (ra=False, float or double data) or double random-access (ra=True)
+    stream: [eb big-endian][u64 off[T]][chunks]."""
+    d = np.ascontiguousarray(data).ravel()
+    if d.dtype not in (np.float32, np.float64):
+        raise TypeError("float32 or float64 data")
+    ds = d.dtype.itemsize
+    out = np.zeros(lib().or_elem_bound(d.size, bs, T, ds, int(ra)), dtype=np.uint8)
+    n = lib().or_elem_compress(_p(d), ds, d.size, float(eb), bs, T, int(ra), _p(out))
+    return out[:n].copy()
+
+
+def elem_decompress(cmp_after_eb, n, eb, bs, T, ra=False, dtype=np.float32):
+    c = np.ascontiguousarray(cmp_after_eb, dtype=np.uint8)
+    out = np.zeros(n, dtype=dtype)
+    if lib().or_elem_decompress(_p(c), n, float(eb), bs, T, int(ra), out.itemsize, _p(out)) != 0:
+        raise ValueError("bad arguments")
+    return out
+
+
+def fix_saddles_pass(q, types):
+    """One Jacobi pass of the double 3D saddle fix; returns (new q, points changed)."""
+    src = np.ascontiguousarray(q, dtype=np.int32)
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    dst = src.copy()
+    k = lib().or_fix_saddles_3d_pass(_p(src), _p(dst), _p(t), *src.shape)
+    return dst, int(k)
+
+
+def topo3d_f64_compress(data, eb, bs, types):
+    """Double 3D topology stream with the saddle fix; returns (stream, fixed q)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    out = np.zeros(lib().or_topo_bound(d.size, bs, 1), dtype=np.uint8)
+    q = np.zeros(d.shape, dtype=np.int32)
+    n = lib().or_topo3d_f64_compress(_p(d), *d.shape, float(eb), bs, _p(t), _p(out), _p(q))
+    return out[:n].copy(), q
 
 
 def block_offsets(payload, n, bs, topo, bc_always=True):

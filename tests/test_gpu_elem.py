@@ -1,0 +1,180 @@
+"""GPU: streams partitioned by elements, against the oracle and the reference's fixtures.
+
+* threadblock codec, float and double data (SZp/src/szp_float.cc:1445-1590,
+  SZp/src/szp_double.cc:148-318, decoders SZp/src/szpd_float.cc:28-135,
+  szpd_double.cc:6-110);
+* double random-access codec (SZp/src/szp_double.cc:759-920);
+* double-precision 3D topology stream with the saddle fix
+  (SZp/src/szp_cuda_topology_3d.cu:736-826, :1065-1188).
+
+Streams are compared byte for byte with the oracle (whose writers are pinned against the
+reference's own outputs in tests/golden), decoded values bit for bit."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import fields
+import oracle
+import toposzp_hip as tz
+
+pytestmark = pytest.mark.gpu
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "g*.npz")))
+ELEM = [g for g in GOLD if os.path.basename(g).startswith(("gtb", "gra"))]
+G3D = [g for g in GOLD if os.path.basename(g).startswith("g3d")]
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _compress(torch, data, eb, bs, T, ra):
+    d = torch.from_numpy(np.ascontiguousarray(data)).cuda()
+    if ra:
+        nbytes, out, blk = tz.plain_compress_f64(d, eb, bs, T)
+    else:
+        nbytes, out, blk = tz.threadblock_compress(d, eb, bs, T)
+    torch.cuda.synchronize()
+    return nbytes, out, blk
+
+
+def _decompress(torch, out, blk, n, eb, bs, T, ra, dtype):
+    if ra:
+        dec = tz.plain_decompress_f64(out, blk, n, eb, bs, T)
+    else:
+        dec = tz.threadblock_decompress(out, blk, n, eb, bs, T,
+                                        dtype=torch.float64 if dtype == np.float64 else torch.float32)
+    torch.cuda.synchronize()
+    return dec.cpu().numpy()
+
+
+def _check(torch, data, eb, bs, T, ra, want=None):
+    print(f"elem n={data.size} {data.dtype} bs={bs} T={T} ra={ra}", flush=True)
+    ref = oracle.elem_compress(data, eb, bs, T=T, ra=ra) if want is None else want
+    nbytes, out, blk = _compress(torch, data, eb, bs, T, ra)
+    got = out[:nbytes].cpu().numpy()
+    assert nbytes == ref.size, (nbytes, ref.size)
+    assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} bytes differ"
+    ds = data.dtype.itemsize
+    dec = _decompress(torch, out, blk, data.size, eb, bs, T, ra, data.dtype)
+    want_dec = oracle.elem_decompress(ref[ds:], data.size, eb, bs, T, ra=ra, dtype=data.dtype)
+    assert np.array_equal(dec.view(np.uint8), want_dec.view(np.uint8))
+    return dec
+
+
+@pytest.mark.parametrize("path", ELEM, ids=os.path.basename)
+def test_elem_matches_reference_fixtures(path):
+    torch = _torch()
+    g = _load(path)
+    data, eb, bs, ra = g["data"], float(g["eb"]), int(g["bs"]), bool(g["ra"])
+    for key in g:
+        if key.startswith("elem_T"):
+            dec = _check(torch, data, eb, bs, int(key[6:]), ra, want=g[key])
+            if key == "elem_T1" and "elem_dec_T1" in g:
+                assert np.array_equal(dec.view(np.uint8), g["elem_dec_T1"].view(np.uint8))
+
+
+SHAPES = [(1, 8, 1), (2, 1, 1), (5, 64, 8), (63, 16, 2), (65, 64, 1), (129, 64, 2), (1000, 7, 3),
+          (4097, 1025, 5), (20011, 32, 64), (100003, 128, 7), (50000, 2, 4096)]
+
+
+@pytest.mark.parametrize("n,bs,T", SHAPES, ids=[f"n{a}_bs{b}_T{c}" for a, b, c in SHAPES])
+@pytest.mark.parametrize("dt,ra", [(np.float32, False), (np.float64, False), (np.float64, True)],
+                         ids=["tb_f32", "tb_f64", "ra_f64"])
+def test_elem_matches_oracle(n, bs, T, dt, ra):
+    torch = _torch()
+    rng = np.random.default_rng(n * 31 + bs + T)
+    x = np.cumsum(rng.normal(0, 1, n)) * 0.01 + 3.0 * np.sin(np.arange(n) * 0.003)
+    _check(torch, x.astype(dt), 1e-3, bs, T, ra)
+
+
+@pytest.mark.parametrize("dt,ra", [(np.float32, False), (np.float64, False), (np.float64, True)],
+                         ids=["tb_f32", "tb_f64", "ra_f64"])
+def test_elem_large_and_wide(dt, ra):
+    """Many tiles chained across one chunk (the segmented look-back), and deltas of
+    every width up to 32 bits (huge values saturate the quantizer to INT_MIN)."""
+    torch = _torch()
+    n = 3_000_017
+    rng = np.random.default_rng(7)
+    x = np.cumsum(rng.normal(0, 1, n)) * 0.05
+    x[::9973] *= 1e4
+    x[5::100003] = 1e12
+    _check(torch, x.astype(dt), 1e-3, 64, 1, ra)
+    _check(torch, x.astype(dt), 1e-3, 256, 3, ra)
+
+
+def test_elem_rejects_bad_arguments():
+    torch = _torch()
+    d = torch.zeros(100, dtype=torch.float64, device="cuda")
+    with pytest.raises(tz.TszpError):
+        tz.threadblock_compress(d, 0.0, 64, 1)
+    with pytest.raises(tz.TszpError):
+        tz.threadblock_compress(d, 1e-3, 2000, 1)
+    with pytest.raises(tz.TszpError):
+        tz.plain_compress_f64(d, 1e-3, 64, 5000)
+
+
+# ------------------------------------------------------------------------------------------
+# double 3D topology with the saddle fix
+# ------------------------------------------------------------------------------------------
+def _cp_records(data, eb):
+    cps = oracle.find_cps_3d(np.ascontiguousarray(data, dtype=np.float32), eb)
+    rec = np.stack([cps["x"], cps["y"], cps["z"], cps["type"], cps["quantized_bin"],
+                    cps["sort_position"]], axis=1).astype(np.int32)
+    types = oracle.mark_types(cps, data.shape)
+    return np.ascontiguousarray(rec), types
+
+
+def _check_topo64(torch, d64, eb, bs, rec, types):
+    print(f"topo64 shape={d64.shape} bs={bs} cps={rec.shape[0]}", flush=True)
+    ref, q_ref = oracle.topo3d_f64_compress(d64, eb, bs, types.reshape(d64.shape))
+    field = torch.from_numpy(np.ascontiguousarray(d64)).cuda()
+    cps = torch.from_numpy(rec).cuda() if rec.shape[0] else None
+    nbytes, out, blk, q, moved = tz.topo_compress_f64_3d(field, eb, bs, cps, want_q=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(q.cpu().numpy(), q_ref)
+    q0 = np.array([oracle.lib().or_cvt_i32(float(v) * (1.0 / eb)) for v in d64.ravel()],
+                  dtype=np.int32).reshape(d64.shape)
+    assert moved == int((q_ref != q0).sum())
+    assert nbytes == ref.size
+    got = out[:nbytes].cpu().numpy()
+    assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} bytes differ"
+    return moved
+
+
+@pytest.mark.parametrize("path", G3D, ids=os.path.basename)
+def test_topo64_fixtures(path):
+    torch = _torch()
+    g = _load(path)
+    data, eb, bs = g["data"], float(g["eb"]), int(g["bs"])
+    rec, types = _cp_records(data, eb)
+    _check_topo64(torch, data.astype(np.float64), eb, bs, rec, types)
+
+
+@pytest.mark.parametrize("shape,eb,bs", [((40, 48, 56), 2e-2, 64), ((33, 27, 45), 5e-2, 16),
+                                         ((64, 64, 64), 1e-1, 128), ((9, 10, 11), 3e-2, 7)])
+def test_topo64_saddles_moved(shape, eb, bs):
+    """Error bounds large enough that many saddles break under quantization; the CP list
+    comes from the float field, the double field carries extra low bits."""
+    torch = _torch()
+    data = fields.fbm3d(*shape, seed=sum(shape))
+    rec, types = _cp_records(data, eb)
+    d64 = data.astype(np.float64) * (1.0 + 2.0 ** -30)
+    moved = _check_topo64(torch, d64, eb, bs, rec, types)
+    assert moved > 0
+
+
+def test_topo64_without_critical_points():
+    torch = _torch()
+    d64 = np.linspace(0, 1, 7 * 8 * 9).reshape(7, 8, 9)
+    rec = np.zeros((0, 6), np.int32)
+    _check_topo64(torch, d64, 1e-3, 64, rec, np.zeros(d64.shape, np.uint8))

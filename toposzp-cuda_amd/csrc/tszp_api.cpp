@@ -49,7 +49,7 @@ enum Slot {
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
-  kWsCount
+  kWsQ, kWsMoves, kWsCount
 };
 
 struct Ctx {
@@ -85,7 +85,7 @@ std::vector<Ctx*> g_ctx;
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
 enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
-                kProfScatter, kProfSolve, kProfQuery, kProfCount };
+                kProfScatter, kProfSolve, kProfQuery, kProfElem, kProfElemDecode, kProfSaddle, kProfCount };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -719,6 +719,204 @@ int tszp_plain_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsign
   std::lock_guard<std::mutex> lk(c->mu);
   return decode(c, kModePlain, d_stream + 4 + 8 * (size_t)T, d_blk, n, bs, eb, d_out, nullptr, nullptr,
                 (hipStream_t)stream);
+}
+
+// ---- element-partitioned streams: threadblock codec (float, double), double RA ---------
+namespace {
+
+int check_elem(size_t n, int bs, unsigned T, int dsize, const char* what) {
+  if (dsize != 4 && dsize != 8) return fail(TSZP_EINVAL, "%s: element size must be 4 or 8", what);
+  if (T < 1 || T > 4096) return fail(TSZP_EINVAL, "%s: chunk count %u out of range [1, 4096]", what, T);
+  if (bs < 1 || bs > 1025) return fail(TSZP_EINVAL, "%s: blockSize %d out of range [1, 1025]", what, bs);
+  return check_n(n);
+}
+
+ElemArgs elem_geometry(size_t n, int bs, unsigned T, bool ra) {
+  ElemArgs a{};
+  a.n = (uint32_t)n;
+  a.bs = (uint32_t)bs;
+  a.T = T;
+  a.ce = (uint32_t)(n / T);
+  a.nu = elem_units(n, (uint32_t)bs, T, ra, &a.ukc);
+  a.div_ukc = make_fastdiv(a.ukc ? a.ukc : 1u);
+  a.ntiles = elem_tiles(a.nu, (uint32_t)bs);
+  return a;
+}
+
+int elem_compress(const void* d_in, int dsize, size_t n, double eb, int bs, unsigned T, bool ra, uint8_t* d_out,
+                  size_t cap, uint32_t* d_blk, uint64_t* out_bytes, hipStream_t st, const char* what) {
+  g_err.clear();
+  if (!d_in || !d_out || !d_blk || !out_bytes) return fail(TSZP_EINVAL, "%s: bad arguments", what);
+  if (!(eb > 0) || (dsize == 4 && !((float)eb > 0))) return fail(TSZP_EINVAL, "%s: error bound must be positive", what);
+  int r = check_elem(n, bs, T, dsize, what);
+  if (r) return r;
+  const size_t hdr = (size_t)dsize + 8 * (size_t)T;
+  if (((uintptr_t)(d_out + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
+  if (cap < hdr + 4) return fail(TSZP_ECAP, "%s: stream capacity %zu too small", what, cap);
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  ElemArgs a = elem_geometry(n, bs, T, ra);
+  uint64_t *stb, *stc;
+  uint32_t* counter;
+  CompressResult* res;
+  if ((r = codec_prepare(c, a.ntiles, &stb, &stc, &counter, &res, st))) return r;
+  a.in = d_in;
+  // float data: the reciprocal is an int / float division, then widened
+  // (SZp/src/szp_float.cc:1489); double data: 1 / eb (SZp/src/szp_double.cc:188, :787)
+  if (dsize == 4) {
+    volatile float finv = 1 / (float)eb;
+    a.inv = (double)finv;
+  } else {
+    a.inv = 1 / eb;
+  }
+  a.payload = d_out + hdr;
+  const uint64_t pc = cap - hdr - 4;  // keep 4 readable bytes past the end
+  a.cap = pc > 0xffffffffull ? 0xffffffffull : pc;
+  a.blk = d_blk;
+  a.st_bytes = stb;
+  a.tile_counter = counter;
+  a.res = res;
+  ProfScope prof(kProfElem, st);
+  HIPCHK(launch_elem_compress(dsize, ra, a, st));
+  HIPCHK(launch_elem_header(d_out, dsize, T, a.ukc, d_blk, eb, st));
+  prof.stop();
+  CompressResult out;
+  r = read_result(c, res, st, &out);
+  prof.collect();
+  if (r) return r;
+  *out_bytes = hdr + out.payload_bytes;
+  return TSZP_OK;
+}
+
+int elem_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, double eb, int bs,
+                    int dsize, bool ra, void* d_out, hipStream_t st, const char* what) {
+  g_err.clear();
+  if (!d_stream || !d_blk || !d_out) return fail(TSZP_EINVAL, "%s: bad arguments", what);
+  int r = check_elem(n, bs, T, dsize, what);
+  if (r) return r;
+  const size_t hdr = (size_t)dsize + 8 * (size_t)T;
+  if (((uintptr_t)(d_stream + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  ElemDecodeArgs a{};
+  a.geo = elem_geometry(n, bs, T, ra);
+  uint64_t *stb, *stc;
+  uint32_t* counter;
+  CompressResult* res;
+  if ((r = codec_prepare(c, a.geo.ntiles, &stb, &stc, &counter, &res, st))) return r;
+  a.payload = d_stream + hdr;
+  a.blk = d_blk;
+  a.eb = dsize == 4 ? (double)(float)eb : eb;
+  a.out = d_out;
+  a.st = stb;
+  a.tile_counter = counter;
+  a.err = &res->error;
+  ProfScope prof(kProfElemDecode, st);
+  HIPCHK(launch_elem_decode(dsize, ra, a, st));
+  prof.stop();
+  CompressResult out;
+  r = read_result(c, res, st, &out);
+  prof.collect();
+  return r;
+}
+
+}  // namespace
+
+size_t tszp_elem_units(size_t n, int block_size, unsigned nchunks, int ra) {
+  if (n == 0 || block_size < 1 || nchunks < 1) return 0;
+  return elem_units(n, (uint32_t)block_size, nchunks, ra != 0, nullptr);
+}
+
+size_t tszp_elem_bound(size_t n, int block_size, unsigned nchunks, int elem_bytes, int ra) {
+  const size_t nu = tszp_elem_units(n, block_size, nchunks, ra);
+  if (!nu) return 0;
+  // anchor, bc byte, signs and 32-bit magnitudes of every unit, 4 readable bytes past the end
+  return (size_t)elem_bytes + 8 * (size_t)nchunks + nu * (5 + ((size_t)block_size + 7) / 8 + 4 * (size_t)block_size) +
+         4;
+}
+
+int tszp_threadblock_compress(const void* d_in, int elem_bytes, size_t n, double eb, int bs, unsigned T,
+                              uint8_t* d_out, size_t cap, uint32_t* d_blk, uint64_t* out_bytes, void* stream) {
+  return elem_compress(d_in, elem_bytes, n, eb, bs, T, false, d_out, cap, d_blk, out_bytes, (hipStream_t)stream,
+                       "threadblock_compress");
+}
+
+int tszp_threadblock_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, double eb,
+                                int bs, int elem_bytes, void* d_out, void* stream) {
+  return elem_decompress(d_stream, d_blk, T, n, eb, bs, elem_bytes, false, d_out, (hipStream_t)stream,
+                         "threadblock_decompress");
+}
+
+int tszp_plain_compress_f64(const double* d_in, size_t n, double eb, int bs, unsigned T, uint8_t* d_out, size_t cap,
+                            uint32_t* d_blk, uint64_t* out_bytes, void* stream) {
+  return elem_compress(d_in, 8, n, eb, bs, T, true, d_out, cap, d_blk, out_bytes, (hipStream_t)stream,
+                       "plain_compress_f64");
+}
+
+int tszp_plain_decompress_f64(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, double eb, int bs,
+                              double* d_out, void* stream) {
+  return elem_decompress(d_stream, d_blk, T, n, eb, bs, 8, true, d_out, (hipStream_t)stream, "plain_decompress_f64");
+}
+
+// szp_cuda_double_compress_topology_3d (SZp/src/szp_cuda_topology_3d.cu:1065-1188)
+int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, double eb, int bs, const int32_t* d_cps,
+                              uint64_t cp_count, uint8_t* d_out, size_t cap, uint32_t* d_blk, uint64_t* out_bytes,
+                              int32_t* d_q, uint64_t* moved, void* stream) {
+  g_err.clear();
+  if (!d_field || !dims || !d_out || !d_blk || !out_bytes || (cp_count && !d_cps))
+    return fail(TSZP_EINVAL, "topo_compress_f64_3d: bad arguments");
+  if (!(eb > 0)) return fail(TSZP_EINVAL, "topo_compress_f64_3d: error bound must be positive");
+  int r = check_bs(bs, true);
+  if (r) return r;
+  size_t n = 1;
+  for (int k = 0; k < 3; k++) {
+    if (dims[k] < 3) return fail(TSZP_EINVAL, "topo_compress_f64_3d: every dimension must be at least 3");
+    n *= dims[k];
+  }
+  if ((r = check_n(n))) return r;
+  if (cp_count > 0x7fffffffull) return fail(TSZP_EINVAL, "topo_compress_f64_3d: too many critical points");
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const Grid g = make_grid(3, dims);
+  uint8_t* types;
+  int32_t* q;
+  int8_t* cd;
+  uint32_t* cnt;
+  WS(kWsTypes, n + 16, types);
+  WS(kWsQ, n * 4 + 16, q);
+  WS(kWsMoves, 2 * ((n + 255) & ~(size_t)255), cd);
+  WS(kWsMinMax, 16, cnt);
+  int8_t* cd2 = cd + ((n + 255) & ~(size_t)255);
+  {
+    ProfScope prof(kProfSaddle, st);
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(types), 0u, (n + 3) / 4, st));
+    HIPCHK(launch_fill_u32(cnt, 0u, 1, st));
+    HIPCHK(launch_mark_cps3(d_cps, (uint32_t)cp_count, g, types, st));
+    HIPCHK(launch_quant_f64(d_field, (uint32_t)n, 1.0 / eb, q, st));
+    // three passes, as the reference: moves of pass k are read by pass k + 1
+    HIPCHK(launch_saddle_pass(g, q, types, nullptr, cd, (uint32_t)n, st));
+    HIPCHK(launch_saddle_pass(g, q, types, cd, cd2, (uint32_t)n, st));
+    HIPCHK(launch_saddle_pass(g, q, types, cd2, cd, (uint32_t)n, st));
+    HIPCHK(launch_saddle_apply(q, cd, (uint32_t)n, cnt, st));
+    prof.stop();
+    prof.collect();
+  }
+  if (d_q) HIPCHK(hipMemcpyAsync(d_q, q, n * 4, hipMemcpyDeviceToDevice, st));
+  CompressResult res;
+  r = run_codec(c, kModeTopoInt, q, (uint32_t)n, bs, (float)eb, 1.0 / eb, nullptr, types, d_out, cap, 8, d_blk,
+                nullptr, nullptr, 0, st, &res);
+  if (r) return r;
+  HIPCHK(launch_write_header(d_out, 0, 1, 0, cdiv(n, bs), d_blk, 0, 0.f, st));
+  uint32_t hm = 0;
+  HIPCHK(hipMemcpyAsync(&hm, cnt, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (moved) *moved = hm;
+  *out_bytes = 8 + res.payload_bytes;
+  return TSZP_OK;
 }
 
 int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t* d_types, uint64_t* cp_count,

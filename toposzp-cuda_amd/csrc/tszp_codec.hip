@@ -36,7 +36,7 @@ struct Geom {
 // ---------------------------------------------------------------------------
 template <int MODE>
 __device__ __forceinline__ int quantize_raw(const CompressArgs& a, uint32_t raw) {
-  if constexpr (MODE == kModeInt) {
+  if constexpr (MODE == kModeInt || MODE == kModeTopoInt) {
     return (int)raw;
   } else {
     return cvt_i32((double)__uint_as_float(raw) * a.inv);
@@ -84,7 +84,7 @@ template <int MODE, int S, int EPL>
 __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
-  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
   constexpr bool CLASSIFY = (MODE == kModeTopoClassify);
   constexpr int STAGE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 2;
 
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
     for (int r = 0; r < EPL; r++) {
       const uint32_t j = (uint32_t)(ls * EPL + r);
       raw[s][r] = j < cur ? in32[base + j] : 0u;
-      if constexpr (MODE == kModeTopoMap) {
+      if constexpr (MODE == kModeTopoMap || MODE == kModeTopoInt) {
         const uint32_t t = j < cur ? (a.tmap[base + j] & 3u) : 0u;
         tbits |= t << (2 * (s * EPL + r));
       }
@@ -392,7 +392,7 @@ template <int MODE, int S, int EPL>
 __global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
-  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
   constexpr int TILE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 2;
   __shared__ uint32_t tw[TILE_WORDS];
 
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   constexpr int EPL = 4, BS = 4 * S;
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
-  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap);
+  constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
   constexpr bool CLASSIFY = (MODE == kModeTopoClassify);
   constexpr int STAGE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 8;  // runs may spill 16 zero bytes
 
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
 #pragma unroll
       for (int r = 0; r < 4; r++) raw[s][r] = j0 + r < cur ? in32[e0 + r] : 0u;
     }
-    if constexpr (MODE == kModeTopoMap) {
+    if constexpr (MODE == kModeTopoMap || MODE == kModeTopoInt) {
       uint32_t tw = 0;
       if (j0 + 3 < cur) {
         tw = reinterpret_cast<const uint32_t*>(a.tmap)[e0 >> 2];
@@ -969,7 +969,7 @@ static bool try_v4(const CompressArgs& a, hipStream_t st, hipError_t* e) {
     if (g.ndim != 2 && g.ndim != 3) return false;
     if ((g.ndim == 3 ? g.d2 : g.d1) % 4u) return false;
   }
-  if (MODE == kModeTopoMap && (reinterpret_cast<uintptr_t>(a.tmap) & 3u)) return false;
+  if ((MODE == kModeTopoMap || MODE == kModeTopoInt) && (reinterpret_cast<uintptr_t>(a.tmap) & 3u)) return false;
   *e = a.bs == 64 ? launch_v4<MODE, 16>(a, st) : launch_v4<MODE, 8>(a, st);
   return true;
 }
@@ -985,6 +985,7 @@ hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st) {
       case kModeTopoMap: if (try_v4<kModeTopoMap>(a, st, &e)) return e; break;
       case kModePlain: if (try_v4<kModePlain>(a, st, &e)) return e; break;
       case kModeInt: if (try_v4<kModeInt>(a, st, &e)) return e; break;
+      case kModeTopoInt: if (try_v4<kModeTopoInt>(a, st, &e)) return e; break;
     }
   }
   switch (mode) {
@@ -992,6 +993,7 @@ hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st) {
     case kModeTopoMap: return launch_compress_mode<kModeTopoMap>(a, S, EPL, st);
     case kModePlain: return launch_compress_mode<kModePlain>(a, S, EPL, st);
     case kModeInt: return launch_compress_mode<kModeInt>(a, S, EPL, st);
+    case kModeTopoInt: return launch_compress_mode<kModeTopoInt>(a, S, EPL, st);
   }
   return hipErrorInvalidValue;
 }

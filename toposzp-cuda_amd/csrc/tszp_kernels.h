@@ -14,6 +14,7 @@ enum Mode {
   kModeTopoMap = 1,       // topology stream, types from a uint8 map (a caller's CP list)
   kModePlain = 2,         // plain random-access float stream
   kModeInt = 3,           // int stream (sort positions)
+  kModeTopoInt = 4,       // topology stream of pre-quantized ints, types from a uint8 map
 };
 
 struct CompressArgs {
@@ -49,6 +50,49 @@ struct DecodeArgs {
 void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* blocks_per_tile);
 hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st);
 hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st);
+
+// ---- element-partitioned streams (tszp_elem.hip): threadblock codec, double RA ----
+struct ElemArgs {
+  const void* in;         // float[n] or double[n]
+  uint32_t n, bs, T;
+  uint32_t ce;            // n / T: elements of every chunk but the last
+  uint32_t ukc;           // units of every chunk but the last
+  uint32_t nu, ntiles;    // units, tiles
+  FastDiv div_ukc;
+  double inv;             // reciprocal used for quantization
+  uint8_t* payload;       // stream bytes after the header (4-byte aligned)
+  uint64_t cap;           // payload capacity in bytes
+  uint32_t* blk;          // nu + 1 unit offsets relative to payload
+  uint64_t* st_bytes;     // look-back status words, ntiles
+  uint32_t* tile_counter;
+  CompressResult* res;
+};
+struct ElemDecodeArgs {
+  ElemArgs geo;           // geometry fields only (n, bs, T, ce, ukc, nu, ntiles, div_ukc)
+  const uint8_t* payload; // 4-byte aligned; readable up to the next multiple of 4 past the end
+  const uint32_t* blk;
+  double eb;
+  void* out;
+  uint64_t* st;           // segmented look-back status words, ntiles
+  uint32_t* tile_counter;
+  uint32_t* err;
+};
+uint32_t elem_units(uint64_t n, uint32_t bs, uint32_t T, bool ra, uint32_t* units_per_chunk);
+uint32_t elem_tiles(uint32_t nu, uint32_t bs);
+hipError_t launch_elem_compress(int dsize, bool ra, const ElemArgs& a, hipStream_t st);
+hipError_t launch_elem_decode(int dsize, bool ra, const ElemDecodeArgs& a, hipStream_t st);
+hipError_t launch_elem_header(uint8_t* stream, int dsize, unsigned T, uint32_t ukc, const uint32_t* blk, double eb,
+                              hipStream_t st);
+
+// ---- double 3D topology: quantization and saddle fix (tszp_saddle.hip) ----
+// types: zero-filled uint8[n], then marked from AoS CP records of 6 ints (x, y, z, type, ..)
+hipError_t launch_mark_cps3(const int32_t* cps, uint32_t count, const Grid& g, uint8_t* types, hipStream_t st);
+hipError_t launch_quant_f64(const double* x, uint32_t n, double inv, int32_t* q, hipStream_t st);
+// one Jacobi pass: cd_out = accumulated move of every saddle (0 elsewhere) given cd_in
+// (nullable = all zero) on top of q
+hipError_t launch_saddle_pass(const Grid& g, const int32_t* q, const uint8_t* types, const int8_t* cd_in,
+                              int8_t* cd_out, uint32_t n, hipStream_t st);
+hipError_t launch_saddle_apply(int32_t* q, const int8_t* cd, uint32_t n, uint32_t* moved, hipStream_t st);
 
 // ---- topology helpers (tszp_topo.hip) ----
 struct ClassifyArgs {

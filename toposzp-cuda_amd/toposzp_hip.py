@@ -137,6 +137,18 @@ def lib():
     L.tszp_szp_pack.restype = i32
     L.tszp_szp_unpack.argtypes = [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), u64p]
     L.tszp_szp_unpack.restype = i32
+    f64 = C.c_double
+    L.tszp_elem_units.argtypes = [sz, i32, u32, i32]; L.tszp_elem_units.restype = sz
+    L.tszp_elem_bound.argtypes = [sz, i32, u32, i32, i32]; L.tszp_elem_bound.restype = sz
+    L.tszp_threadblock_compress.argtypes = [vp, i32, sz, f64, i32, u32, vp, sz, vp, u64p, vp]
+    L.tszp_threadblock_decompress.argtypes = [vp, vp, u32, sz, f64, i32, i32, vp, vp]
+    L.tszp_plain_compress_f64.argtypes = [vp, sz, f64, i32, u32, vp, sz, vp, u64p, vp]
+    L.tszp_plain_decompress_f64.argtypes = [vp, vp, u32, sz, f64, i32, vp, vp]
+    L.tszp_topo_compress_f64_3d.argtypes = [vp, vp, f64, i32, vp, C.c_uint64, vp, sz, vp, u64p, vp,
+                                            u64p, vp]
+    for f in ("tszp_threadblock_compress", "tszp_threadblock_decompress", "tszp_plain_compress_f64",
+              "tszp_plain_decompress_f64", "tszp_topo_compress_f64_3d"):
+        getattr(L, f).restype = i32
     L.tszp_prof_enable.argtypes = [i32]; L.tszp_prof_enable.restype = None
     L.tszp_prof_reset.argtypes = []; L.tszp_prof_reset.restype = None
     L.tszp_prof_read.argtypes = [i32, C.POINTER(C.c_double), u64p]; L.tszp_prof_read.restype = i32
@@ -145,7 +157,8 @@ def lib():
 
 
 PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5,
-              "restore": 6, "scatter": 7, "solve": 8, "query": 9}
+              "restore": 6, "scatter": 7, "solve": 8, "query": 9, "elem": 10, "elem_decode": 11,
+              "saddle": 12}
 
 
 def prof_enable(on=True):
@@ -365,6 +378,72 @@ def plain_decompress(stream, blk, n, eb, bs, T=1, out=None):
         out = torch.empty(n, dtype=torch.float32, device=stream.device)
     _check(lib().tszp_plain_decompress(_dp(stream), _dp(blk), T, n, eb, bs, _dp(out), _stream()))
     return out
+
+
+def elem_buffers(n, bs, T, elem_bytes, ra=False, device="cuda"):
+    """(stream, side-car) device buffers for an element-partitioned stream."""
+    torch = _torch()
+    L = lib()
+    out = torch.empty(L.tszp_elem_bound(n, bs, T, elem_bytes, int(ra)), dtype=torch.uint8, device=device)
+    blk = torch.empty(L.tszp_elem_units(n, bs, T, int(ra)) + 1, dtype=torch.int32, device=device)
+    return out, blk
+
+
+def threadblock_compress(field, eb, bs, T=1, out=None, blk=None):
+    """Threadblock stream of a float32 or float64 CUDA tensor: (bytes, stream, side-car)."""
+    n, ds = field.numel(), field.element_size()
+    if out is None or blk is None:
+        out, blk = elem_buffers(n, bs, T, ds, False, field.device)
+    nbytes = C.c_uint64()
+    _check(lib().tszp_threadblock_compress(_dp(field), ds, n, float(eb), bs, T, _dp(out), out.numel(),
+                                           _dp(blk), C.byref(nbytes), _stream()))
+    return nbytes.value, out, blk
+
+
+def threadblock_decompress(stream, blk, n, eb, bs, T=1, dtype=None, out=None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty(n, dtype=dtype or torch.float32, device=stream.device)
+    _check(lib().tszp_threadblock_decompress(_dp(stream), _dp(blk), T, n, float(eb), bs, out.element_size(),
+                                             _dp(out), _stream()))
+    return out
+
+
+def plain_compress_f64(field, eb, bs, T=1, out=None, blk=None):
+    """Double random-access stream of a float64 CUDA tensor: (bytes, stream, side-car)."""
+    n = field.numel()
+    if out is None or blk is None:
+        out, blk = elem_buffers(n, bs, T, 8, True, field.device)
+    nbytes = C.c_uint64()
+    _check(lib().tszp_plain_compress_f64(_dp(field), n, float(eb), bs, T, _dp(out), out.numel(), _dp(blk),
+                                         C.byref(nbytes), _stream()))
+    return nbytes.value, out, blk
+
+
+def plain_decompress_f64(stream, blk, n, eb, bs, T=1, out=None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=stream.device)
+    _check(lib().tszp_plain_decompress_f64(_dp(stream), _dp(blk), T, n, float(eb), bs, _dp(out), _stream()))
+    return out
+
+
+def topo_compress_f64_3d(field, eb, bs, cps, want_q=False):
+    """Double 3D topology stream with the saddle fix.  field: float64 CUDA tensor
+    (d1, d2, d3); cps: int32 CUDA tensor of CriticalPoint3D records (k x 6).
+    Returns (bytes, stream, side-car, fixed q or None, saddles moved)."""
+    torch = _torch()
+    n = field.numel()
+    L = lib()
+    out = torch.empty(L.tszp_topo_bound(n, bs, 1), dtype=torch.uint8, device=field.device)
+    blk = torch.empty(L.tszp_blocks(n, bs) + 1, dtype=torch.int32, device=field.device)
+    q = torch.empty(field.shape, dtype=torch.int32, device=field.device) if want_q else None
+    nbytes, moved = C.c_uint64(), C.c_uint64()
+    count = cps.shape[0] if cps is not None else 0
+    _check(L.tszp_topo_compress_f64_3d(_dp(field), _dims(field.shape), float(eb), bs,
+                                       _dp(cps) if count else None, count, _dp(out), out.numel(), _dp(blk),
+                                       C.byref(nbytes), _dp(q) if want_q else None, C.byref(moved), _stream()))
+    return nbytes.value, out, blk, q, moved.value
 
 
 def classify(field, types=None):
