@@ -10,8 +10,10 @@ reference's decompression timing includes its post-process).  value = 4 N bytes 
 time (the reference's GB/s convention, example/test_cuda_compress.cu:95-96).
 
 Also reported: per-stage kernel times (HIP events on the launch stream), the roofline
-of the dominant kernel, plain SZp throughput on the same field, and the reference's
-own OpenMP CPU path timed on this host on a bounded slab of the same field.
+of the dominant kernel, plain SZp throughput on the same field, the threadblock codec
+(float and double data), the double random-access codec and the double 3D topology
+path on the same field, and the reference's own OpenMP CPU path timed on this host on
+a bounded slab of the same field.
 """
 import argparse
 import json
@@ -126,6 +128,79 @@ def pmc_traffic(kernel):
     return sum(vals) / len(vals) if vals else None
 
 
+def elem_side(tz, torch, f, shape, eb, bs, steps):
+    """Side figures of the element-partitioned codecs and the double 3D topology path on
+    the same field (as float and as double data): whole-call GB/s of input and the HBM
+    fraction of the compression / decompression kernels from the library's event timers
+    (algorithmic bytes = input + stream)."""
+    n = f.numel()
+    f64 = f.double()
+    out = {}
+    cases = [("threadblock_f32", f, False), ("threadblock_f64", f64, False), ("plain_f64", f64, True)]
+    for name, x, ra in cases:
+        ds = x.element_size()
+        sbuf, blk = tz.elem_buffers(n, bs, 1, ds, ra, x.device)
+        dec = torch.empty_like(x)
+
+        def run():
+            if ra:
+                nb, _, _ = tz.plain_compress_f64(x, eb, bs, 1, out=sbuf, blk=blk)
+                tz.plain_decompress_f64(sbuf, blk, n, eb, bs, 1, out=dec)
+            else:
+                nb, _, _ = tz.threadblock_compress(x, eb, bs, 1, out=sbuf, blk=blk)
+                tz.threadblock_decompress(sbuf, blk, n, eb, bs, 1, out=dec)
+            return nb
+        nb = run()
+        torch.cuda.synchronize()
+        err = float((dec - x).abs().max()) / eb
+        tz.prof_reset()
+        tz.prof_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        tz.prof_enable(False)
+        pr = tz.prof_read()
+        cms = pr["elem"][0] / max(1, pr["elem"][1])
+        dms = pr["elem_decode"][0] / max(1, pr["elem_decode"][1])
+        out[name] = {"gbs": ds * n / el / 1e9, "ratio": ds * n / nb, "max_err_over_eb": err,
+                     "compress_kernel_ms": cms, "decompress_kernel_ms": dms,
+                     "compress_hbm_frac": (ds * n + nb) / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "decompress_hbm_frac": (ds * n + nb) / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"{name}: compress+decompress {out[name]['gbs']:.1f} GB/s, ratio {out[name]['ratio']:.2f}, "
+            f"kernels {cms:.3f} / {dms:.3f} ms ({out[name]['compress_hbm_frac']:.2f} / "
+            f"{out[name]['decompress_hbm_frac']:.2f} of HBM peak), max err {err:.4f} eb")
+        del sbuf, blk, dec
+    # double 3D topology: critical-point records of the field, then the whole call
+    types, _ = tz.classify(f.view(shape))
+    idx = torch.nonzero(types.view(-1)).view(-1)
+    plane = shape[1] * shape[2]
+    rec = torch.stack([idx // plane, (idx % plane) // shape[2], idx % shape[2],
+                       types.view(-1)[idx].long(), torch.zeros_like(idx), torch.zeros_like(idx)], 1)
+    rec = rec.to(torch.int32).contiguous()
+    x = f64.view(shape)
+    nb, _, _, _, moved = tz.topo_compress_f64_3d(x, eb, bs, rec)
+    torch.cuda.synchronize()
+    tz.prof_reset()
+    tz.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nb, _, _, _, moved = tz.topo_compress_f64_3d(x, eb, bs, rec)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    tz.prof_enable(False)
+    pr = tz.prof_read()
+    out["topo_f64_3d"] = {"gbs": 8 * n / el / 1e9, "ms": 1e3 * el, "ratio": 8 * n / nb, "saddles_moved": moved,
+                          "cp_count": int(idx.numel()),
+                          "quantize_saddle_fix_ms": pr["saddle"][0] / max(1, pr["saddle"][1]),
+                          "codec_ms": pr["codec_topo"][0] / max(1, pr["codec_topo"][1])}
+    log(f"topo_f64_3d: {out['topo_f64_3d']['ms']:.3f} ms ({out['topo_f64_3d']['gbs']:.1f} GB/s of doubles), "
+        f"quantize + saddle fix {out['topo_f64_3d']['quantize_saddle_fix_ms']:.3f} ms, "
+        f"codec {out['topo_f64_3d']['codec_ms']:.3f} ms, {moved} saddles moved of {idx.numel()} CPs")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +214,7 @@ def main():
     ap.add_argument("--cpu-slab", type=int, default=64, help="planes of the field for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
+    ap.add_argument("--no-elem", action="store_true", help="leave out the threadblock / double side figures")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -248,6 +324,9 @@ def main():
     plain_gbs = 4.0 * n / ((time.perf_counter() - t0) / args.steps) / 1e9
     log(f"plain SZp compress+decompress {plain_gbs:.1f} GB/s (ratio {4 * n / nbytes:.2f})")
 
+    # side figures: threadblock / double random-access codecs, double 3D topology
+    elem = None if args.no_elem else elem_side(tz, torch, f, shape, eb, bs, 3)
+
     # side figure: topology quality of the restored field against the original
     # (the CUDA driver's report), one GPU pass
     tz.quality_metrics(f, restored, shape, eb)
@@ -310,6 +389,7 @@ def main():
         "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
         "plain_gbs": plain_gbs,
         "topo_stream_parse_ms": parse_ms,
+        "elem_codecs": elem,
         "quality_ms": quality_ms,
         "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
                                             "over_1eb", "pairs", "violations", "saddles_above",

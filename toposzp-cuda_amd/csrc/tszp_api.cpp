@@ -49,7 +49,7 @@ enum Slot {
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
-  kWsQ, kWsMoves, kWsCount
+  kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsCount
 };
 
 struct Ctx {
@@ -885,23 +885,35 @@ int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, doubl
   uint8_t* types;
   int32_t* q;
   int8_t* cd;
-  uint32_t* cnt;
-  WS(kWsTypes, n + 16, types);
+  uint32_t *list, *meta;
+  const size_t nr = (n + 255) & ~(size_t)255;
+  const uint32_t sb = saddle_blocks((uint32_t)n);
+  const size_t scan_words = scan_status_words(sb + 1);
+  // meta: [moved][scan err][scan counter][pad][scan status words (u64)][counts sb + 1]
+  const size_t meta_words = 4 + 2 * scan_words + sb + 1;
+  WS(kWsTypes, nr, types);
   WS(kWsQ, n * 4 + 16, q);
-  WS(kWsMoves, 2 * ((n + 255) & ~(size_t)255), cd);
-  WS(kWsMinMax, 16, cnt);
-  int8_t* cd2 = cd + ((n + 255) & ~(size_t)255);
+  WS(kWsMoves, 2 * nr, cd);
+  WS(kWsSaddleList, n * 4 + 16, list);
+  WS(kWsSaddleMeta, meta_words * 4, meta);
+  int8_t* cd2 = cd + nr;
+  uint32_t* cnt = meta + 4 + 2 * scan_words;
   {
     ProfScope prof(kProfSaddle, st);
-    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(types), 0u, (n + 3) / 4, st));
-    HIPCHK(launch_fill_u32(cnt, 0u, 1, st));
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(types), 0u, nr / 4, st));
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(cd), 0u, 2 * nr / 4, st));
+    HIPCHK(launch_fill_u32(meta, 0u, meta_words, st));
     HIPCHK(launch_mark_cps3(d_cps, (uint32_t)cp_count, g, types, st));
     HIPCHK(launch_quant_f64(d_field, (uint32_t)n, 1.0 / eb, q, st));
+    HIPCHK(launch_saddle_list(g, types, (uint32_t)n, cnt, reinterpret_cast<uint64_t*>(meta + 4), meta + 2, meta + 1,
+                              list, st));
+    const uint32_t* total = cnt + sb;
     // three passes, as the reference: moves of pass k are read by pass k + 1
-    HIPCHK(launch_saddle_pass(g, q, types, nullptr, cd, (uint32_t)n, st));
-    HIPCHK(launch_saddle_pass(g, q, types, cd, cd2, (uint32_t)n, st));
-    HIPCHK(launch_saddle_pass(g, q, types, cd2, cd, (uint32_t)n, st));
-    HIPCHK(launch_saddle_apply(q, cd, (uint32_t)n, cnt, st));
+    int8_t* upd = cd;  // the first map is free again for the last pass
+    HIPCHK(launch_saddle_pass(g, q, types, list, total, nullptr, cd, nullptr, st));
+    HIPCHK(launch_saddle_pass(g, q, types, list, total, cd, cd2, nullptr, st));
+    HIPCHK(launch_saddle_pass(g, q, types, list, total, cd2, nullptr, upd, st));
+    HIPCHK(launch_saddle_apply(q, upd, list, total, meta, st));
     prof.stop();
     prof.collect();
   }
@@ -911,10 +923,11 @@ int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, doubl
                 nullptr, nullptr, 0, st, &res);
   if (r) return r;
   HIPCHK(launch_write_header(d_out, 0, 1, 0, cdiv(n, bs), d_blk, 0, 0.f, st));
-  uint32_t hm = 0;
-  HIPCHK(hipMemcpyAsync(&hm, cnt, 4, hipMemcpyDeviceToHost, st));
+  uint32_t hm[2] = {0u, 0u};
+  HIPCHK(hipMemcpyAsync(hm, meta, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (moved) *moved = hm;
+  if (hm[1]) return fail(TSZP_EDEVICE, "topo_compress_f64_3d: saddle list scan failed");
+  if (moved) *moved = hm[0];
   *out_bytes = 8 + res.payload_bytes;
   return TSZP_OK;
 }

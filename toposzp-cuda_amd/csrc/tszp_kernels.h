@@ -88,11 +88,21 @@ hipError_t launch_elem_header(uint8_t* stream, int dsize, unsigned T, uint32_t u
 // types: zero-filled uint8[n], then marked from AoS CP records of 6 ints (x, y, z, type, ..)
 hipError_t launch_mark_cps3(const int32_t* cps, uint32_t count, const Grid& g, uint8_t* types, hipStream_t st);
 hipError_t launch_quant_f64(const double* x, uint32_t n, double inv, int32_t* q, hipStream_t st);
-// one Jacobi pass: cd_out = accumulated move of every saddle (0 elsewhere) given cd_in
-// (nullable = all zero) on top of q
-hipError_t launch_saddle_pass(const Grid& g, const int32_t* q, const uint8_t* types, const int8_t* cd_in,
-                              int8_t* cd_out, uint32_t n, hipStream_t st);
-hipError_t launch_saddle_apply(int32_t* q, const int8_t* cd, uint32_t n, uint32_t* moved, hipStream_t st);
+// interior saddles of the type map listed in index order: cnt holds saddle_blocks(n) + 1
+// zeroed words (the total lands in cnt[saddle_blocks(n)]), scan_st / scan_ctr zeroed
+// look-back words of launch_scan_excl over that many entries
+uint32_t saddle_blocks(uint32_t n);
+hipError_t launch_saddle_list(const Grid& g, const uint8_t* types, uint32_t n, uint32_t* cnt, uint64_t* scan_st,
+                              uint32_t* scan_ctr, uint32_t* err, uint32_t* list, hipStream_t st);
+// one Jacobi pass over the listed saddles: the accumulated move of saddle list[i] given
+// cd_in (nullable = all zero) on top of q goes to cd_out[list[i]] (nullable; must be zero
+// away from saddles) and to upd[i] (nullable)
+hipError_t launch_saddle_pass(const Grid& g, const int32_t* q, const uint8_t* types, const uint32_t* list,
+                              const uint32_t* count, const int8_t* cd_in, int8_t* cd_out, int8_t* upd,
+                              hipStream_t st);
+// q[list[i]] += upd[i]; *moved += the saddles that moved
+hipError_t launch_saddle_apply(int32_t* q, const int8_t* upd, const uint32_t* list, const uint32_t* count,
+                               uint32_t* moved, hipStream_t st);
 
 // ---- topology helpers (tszp_topo.hip) ----
 struct ClassifyArgs {

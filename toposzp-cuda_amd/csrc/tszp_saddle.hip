@@ -4,10 +4,11 @@
 //   q      <- C cast of x * (1 / eb) in double (quantize_3d_kernel_double, :641-646)
 //   three Jacobi passes of the saddle fix (fix_saddles_3d_kernel, :736-826)
 // The reference snapshots the whole quantized field before each pass (a device copy of
-// 4 bytes per point).  Here q is never written during the passes: each pass reads the
-// moves of the previous pass from a byte map (only saddles move, by at most one step per
-// pass) and writes its own map, so a pass costs the type map plus two bytes per point;
-// the moves are added to q once at the end.
+// 4 bytes per point) and visits every interior point.  Here the interior saddles are
+// listed once (count per workgroup, scan, list), q is never written during the passes,
+// and each pass visits only the listed saddles: it reads the accumulated moves of the
+// previous pass from a byte map (only saddles move, by at most one step per pass) and
+// writes its own map; the moves are added to q once at the end.
 #include "tszp_device.hpp"
 #include "tszp_kernels.h"
 
@@ -75,71 +76,99 @@ __device__ __forceinline__ int saddle_move(const Grid& g, const int32_t* __restr
   return m0;
 }
 
-// four points per thread: one 4-byte load of types, one 4-byte store of moves
-__global__ __launch_bounds__(kBlockThreads) void k_saddle_pass(Grid g, const int32_t* __restrict__ q,
-                                                              const uint8_t* __restrict__ types,
-                                                              const int8_t* __restrict__ cd_in,
-                                                              int8_t* __restrict__ cd_out, uint32_t n) {
-  const uint32_t groups = (n + 3u) / 4u;
-  uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  const uint32_t stride = gridDim.x * kBlockThreads;
-  for (; i < groups; i += stride) {
-    const uint32_t e0 = 4u * i;
-    const bool full = e0 + 4u <= n;
-    uint32_t tw = 0;
-    if (full) {
-      tw = reinterpret_cast<const uint32_t*>(types)[i];
-    } else {
-      for (uint32_t r = 0; e0 + r < n; r++) tw |= (uint32_t)types[e0 + r] << (8 * r);
-    }
-    uint32_t out = 0;
-    if (tw & 0x02020202u) {  // some type is 2 or 3
+constexpr uint32_t kSadPts = 16;                          // points per thread
+constexpr uint32_t kSadBlockPts = kBlockThreads * kSadPts;  // points per workgroup
+
+// the thread's 16 type bytes, bit r of the result set where point e0 + r is an interior
+// saddle
+__device__ __forceinline__ uint32_t saddle_bits(const Grid& g, const uint8_t* __restrict__ types, uint32_t n,
+                                                uint32_t e0) {
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  if (e0 + kSadPts <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(types + e0);
+    w[0] = v.x;
+    w[1] = v.y;
+    w[2] = v.z;
+    w[3] = v.w;
+  } else {
+    for (uint32_t r = 0; e0 + r < n; r++) w[r >> 2] |= (uint32_t)types[e0 + r] << (8 * (r & 3u));
+  }
+  uint32_t bits = 0;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        if (((tw >> (8 * r)) & 0xffu) != 3u) continue;
-        const uint32_t e = e0 + r;
-        const uint32_t x = fdiv(e, g.div_plane), rr = e - x * g.plane;
-        const uint32_t y = fdiv(rr, g.div_row), z = rr - y * g.d2;
-        if (x == 0 || y == 0 || z == 0 || x + 1 >= g.d0 || y + 1 >= g.d1 || z + 1 >= g.d2) continue;
-        const int mv = saddle_move(g, q, types, cd_in, e);
-        out |= (uint32_t)(uint8_t)(int8_t)mv << (8 * r);
-      }
-    }
-    if (full) {
-      reinterpret_cast<uint32_t*>(cd_out)[i] = out;
-    } else {
-      for (uint32_t r = 0; e0 + r < n; r++) cd_out[e0 + r] = (int8_t)(uint8_t)(out >> (8 * r));
-    }
+  for (int r = 0; r < (int)kSadPts; r++) {
+    if (((w[r >> 2] >> (8 * (r & 3))) & 0xffu) != 3u) continue;
+    const uint32_t e = e0 + r;
+    const uint32_t x = fdiv(e, g.div_plane), rr = e - x * g.plane;
+    const uint32_t y = fdiv(rr, g.div_row), z = rr - y * g.d2;
+    if (x == 0 || y == 0 || z == 0 || x + 1 >= g.d0 || y + 1 >= g.d1 || z + 1 >= g.d2) continue;
+    bits |= 1u << r;
+  }
+  return bits;
+}
+
+// interior saddles per workgroup range of kSadBlockPts points
+__global__ __launch_bounds__(kBlockThreads) void k_saddle_count(Grid g, const uint8_t* __restrict__ types, uint32_t n,
+                                                               uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t lds4[kWavesPerBlock];
+  const uint32_t e0 = blockIdx.x * kSadBlockPts + threadIdx.x * kSadPts;
+  const uint32_t c = e0 < n ? (uint32_t)__popc(saddle_bits(g, types, n, e0)) : 0u;
+  uint32_t total;
+  block_excl_scan(c, lds4, &total);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = total;
+}
+
+// list of interior saddles in index order; off: exclusive scan of the counts
+__global__ __launch_bounds__(kBlockThreads) void k_saddle_list(Grid g, const uint8_t* __restrict__ types, uint32_t n,
+                                                              const uint32_t* __restrict__ off,
+                                                              uint32_t* __restrict__ list) {
+  __shared__ uint32_t lds4[kWavesPerBlock];
+  const uint32_t e0 = blockIdx.x * kSadBlockPts + threadIdx.x * kSadPts;
+  uint32_t bits = e0 < n ? saddle_bits(g, types, n, e0) : 0u;
+  uint32_t total;
+  uint32_t pos = off[blockIdx.x] + block_excl_scan((uint32_t)__popc(bits), lds4, &total);
+  while (bits) {
+    const int r = __ffs(bits) - 1;
+    bits &= bits - 1u;
+    list[pos++] = e0 + (uint32_t)r;
   }
 }
 
+// one Jacobi pass over the listed saddles (the count is read on the device); the last
+// pass also writes the moves in list order (upd), so that applying them reads a dense
+// array instead of one scattered byte per saddle
+__global__ __launch_bounds__(kBlockThreads) void k_saddle_pass(Grid g, const int32_t* __restrict__ q,
+                                                              const uint8_t* __restrict__ types,
+                                                              const uint32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ count,
+                                                              const int8_t* __restrict__ cd_in,
+                                                              int8_t* __restrict__ cd_out, int8_t* __restrict__ upd) {
+  const uint32_t m = *count;
+  for (uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x; i < m; i += gridDim.x * kBlockThreads) {
+    const uint32_t e = list[i];
+    const int8_t mv = (int8_t)saddle_move(g, q, types, cd_in, e);
+    if (cd_out) cd_out[e] = mv;
+    if (upd) upd[i] = mv;
+  }
+}
+
+// q += move at the listed saddles that moved
 __global__ __launch_bounds__(kBlockThreads) void k_saddle_apply(int32_t* __restrict__ q,
-                                                               const int8_t* __restrict__ cd, uint32_t n,
+                                                               const int8_t* __restrict__ upd,
+                                                               const uint32_t* __restrict__ list,
+                                                               const uint32_t* __restrict__ count,
                                                                uint32_t* moved) {
-  const uint32_t groups = (n + 3u) / 4u;
-  uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  const uint32_t stride = gridDim.x * kBlockThreads;
-  uint32_t cnt = 0;
-  for (; i < groups; i += stride) {
-    const uint32_t e0 = 4u * i;
-    uint32_t w = 0;
-    if (e0 + 4u <= n) {
-      w = reinterpret_cast<const uint32_t*>(cd)[i];
-    } else {
-      for (uint32_t r = 0; e0 + r < n; r++) w |= (uint32_t)(uint8_t)cd[e0 + r] << (8 * r);
-    }
-    if (!w) continue;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int d = (int)(int8_t)(uint8_t)(w >> (8 * r));
-      if (d) {
-        q[e0 + r] = (int)((uint32_t)q[e0 + r] + (uint32_t)d);
-        cnt++;
-      }
+  const uint32_t m = *count;
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x; i < m; i += gridDim.x * kBlockThreads) {
+    const int d = upd[i];
+    if (d) {
+      const uint32_t e = list[i];
+      q[e] = (int)((uint32_t)q[e] + (uint32_t)d);
+      c++;
     }
   }
-  cnt = seg_sum<kWave>(cnt);
-  if ((threadIdx.x & (kWave - 1)) == 0 && cnt) atomicAdd(moved, cnt);
+  c = seg_sum<kWave>(c);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(moved, c);
 }
 
 static unsigned grid_for(uint64_t items, unsigned cap) {
@@ -160,16 +189,31 @@ hipError_t launch_quant_f64(const double* x, uint32_t n, double inv, int32_t* q,
   return hipGetLastError();
 }
 
-hipError_t launch_saddle_pass(const Grid& g, const int32_t* q, const uint8_t* types, const int8_t* cd_in,
-                              int8_t* cd_out, uint32_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_saddle_pass, dim3(grid_for((n + 3u) / 4u, 8192)), dim3(kBlockThreads), 0, st, g, q, types,
-                     cd_in, cd_out, n);
+uint32_t saddle_blocks(uint32_t n) { return (n + kSadBlockPts - 1) / kSadBlockPts; }
+
+hipError_t launch_saddle_list(const Grid& g, const uint8_t* types, uint32_t n, uint32_t* cnt, uint64_t* scan_st,
+                              uint32_t* scan_ctr, uint32_t* err, uint32_t* list, hipStream_t st) {
+  const uint32_t nb = saddle_blocks(n);
+  hipLaunchKernelGGL(k_saddle_count, dim3(nb), dim3(kBlockThreads), 0, st, g, types, n, cnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // cnt[nb] is zero on entry, so that the scan leaves the total there
+  if ((e = launch_scan_excl(cnt, nb + 1, scan_st, scan_ctr, err, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_saddle_list, dim3(nb), dim3(kBlockThreads), 0, st, g, types, n, cnt, list);
   return hipGetLastError();
 }
 
-hipError_t launch_saddle_apply(int32_t* q, const int8_t* cd, uint32_t n, uint32_t* moved, hipStream_t st) {
-  hipLaunchKernelGGL(k_saddle_apply, dim3(grid_for((n + 3u) / 4u, 8192)), dim3(kBlockThreads), 0, st, q, cd, n,
-                     moved);
+hipError_t launch_saddle_pass(const Grid& g, const int32_t* q, const uint8_t* types, const uint32_t* list,
+                              const uint32_t* count, const int8_t* cd_in, int8_t* cd_out, int8_t* upd,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_saddle_pass, dim3(4096), dim3(kBlockThreads), 0, st, g, q, types, list, count, cd_in, cd_out,
+                     upd);
+  return hipGetLastError();
+}
+
+hipError_t launch_saddle_apply(int32_t* q, const int8_t* upd, const uint32_t* list, const uint32_t* count,
+                               uint32_t* moved, hipStream_t st) {
+  hipLaunchKernelGGL(k_saddle_apply, dim3(4096), dim3(kBlockThreads), 0, st, q, upd, list, count, moved);
   return hipGetLastError();
 }
 
