@@ -1,7 +1,6 @@
 /*
  * szp_cuda_decompress.h -- decompression entry points of libtoposzp_hip.so.
- * Drop-in for the float random-access, topology and sort-position entries of
- * SZp/include/szp_cuda_decompress.cuh:23-72.
+ * Drop-in for SZp/include/szp_cuda_decompress.cuh:23-72.
  *
  * Streams carry no length and no block offsets: the chunk count T is detected from
  * the offset table as the reference does (SZp/include/szp_detect_threads.h) and the
@@ -39,6 +38,32 @@ void szp_cuda_float_decompress_randomaccess_topology_preserved(
  * chunk-offset table ([u64 off[T]][blocks]). */
 int *szp_cuda_decompress_sort_positions(
     unsigned char *cmpBytes, size_t critical_count, int blockSize);
+
+/* Replace szp_cuda_double_decompress_randomaccess[_arg] (.cuh:31-37) and
+ * szp_cuda_{float,double}_decompress_threadblock[_arg] (.cuh:43-57).  cmpBytes points
+ * after the error-bound header (at the chunk-offset table); streams of the CPU writers
+ * with any chunk count, which is detected (candidates as in
+ * SZp/include/szp_detect_threads.h:196-222: the largest T in 2..128 with monotone
+ * offsets whose chunks walk, each ending where the next starts, else 1; the reference
+ * takes the smallest, which small chunks make ambiguous). */
+double *szp_cuda_double_decompress_randomaccess(
+    size_t nbEle, double absErrBound, int blockSize,
+    unsigned char *cmpBytes);
+void szp_cuda_double_decompress_randomaccess_arg(
+    double *newData, size_t nbEle, double absErrBound, int blockSize,
+    unsigned char *cmpBytes);
+float *szp_cuda_float_decompress_threadblock(
+    size_t nbEle, float absErrBound, int blockSize,
+    unsigned char *cmpBytes);
+void szp_cuda_float_decompress_threadblock_arg(
+    float *newData, size_t nbEle, float absErrBound, int blockSize,
+    unsigned char *cmpBytes);
+double *szp_cuda_double_decompress_threadblock(
+    size_t nbEle, double absErrBound, int blockSize,
+    unsigned char *cmpBytes);
+void szp_cuda_double_decompress_threadblock_arg(
+    double *newData, size_t nbEle, double absErrBound, int blockSize,
+    unsigned char *cmpBytes);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 /*
  * szp_cuda_topology_3d.h -- 3D (6-connected) TopoSZp entry points of
- * libtoposzp_hip.so.  Drop-in for SZp/include/szp_cuda_topology_3d.cuh:16-38 (float).
+ * libtoposzp_hip.so.  Drop-in for SZp/include/szp_cuda_topology_3d.cuh:16-38.
  * Flat index = x * d2 * d3 + y * d3 + z.
  */
 #ifndef TOPOSZP_SZP_CUDA_TOPOLOGY_3D_H
@@ -42,6 +42,16 @@ unsigned char *szp_cuda_compress_sort_positions_3d(CriticalPoint3D *critical_poi
  * byte-identical to SZp/src/szp_topology_3d.cc:433-632 run with one thread. */
 unsigned char *szp_cuda_float_compress_topology_3d(
     float *oriData, size_t *outSize, float absErrBound,
+    size_t nbEle, int blockSize,
+    CriticalPoint3D *critical_points, int critical_count,
+    int d1, int d2, int d3);
+
+/* Replaces szp_cuda_double_compress_topology_3d (.cuh:34-38,
+ * SZp/src/szp_cuda_topology_3d.cu:1065-1188): types from the critical points, q = C
+ * cast of x * (1 / eb), three Jacobi passes of the saddle fix, then [u64 0][blocks] of
+ * the topology codec on q.  blockSize in [1, 257], every dimension at least 3. */
+unsigned char *szp_cuda_double_compress_topology_3d(
+    double *oriData, size_t *outSize, double absErrBound,
     size_t nbEle, int blockSize,
     CriticalPoint3D *critical_points, int critical_count,
     int d1, int d2, int d3);

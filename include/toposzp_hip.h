@@ -111,6 +111,11 @@ int tszp_plain_compress_f64(const double *d_in, size_t n, double eb, int block_s
 int tszp_plain_decompress_f64(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
                               double eb, int block_size, double *d_out, void *stream);
 
+/* Quantize and predict: d_out[i] = q[i] - q[i - 1] with q[-1] = 0 and
+ * q = C cast of x * (1 / (double)eb) (the quantization helpers of
+ * SZp/src/szp_cuda_compress.cu:360-377, :664-710; int32 wrap-around differences). */
+int tszp_predict_quantization(const float *d_in, size_t n, float eb, int32_t *d_out, void *stream);
+
 /* Double-precision 3D topology stream [u64 0][blocks] with the saddle fix
  * (szp_cuda_double_compress_topology_3d, SZp/src/szp_cuda_topology_3d.cu:1065-1188):
  * types marked from the critical-point records d_cps (cp_count records of 6 ints:

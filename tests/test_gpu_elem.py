@@ -178,3 +178,130 @@ def test_topo64_without_critical_points():
     d64 = np.linspace(0, 1, 7 * 8 * 9).reshape(7, 8, 9)
     rec = np.zeros((0, 6), np.int32)
     _check_topo64(torch, d64, 1e-3, 64, rec, np.zeros(d64.shape, np.uint8))
+
+
+# ------------------------------------------------------------------------------------------
+# drop-in entry points (host buffers): szp_cuda_{float,double}_{compress,decompress}_
+# threadblock, szp_cuda_double_{compress,decompress}_randomaccess,
+# szp_cuda_double_compress_topology_3d, szp_cuda_float_*_predict_quantization
+# ------------------------------------------------------------------------------------------
+import ctypes as C  # noqa: E402
+
+_libc = C.CDLL("libc.so.6")
+_libc.free.argtypes = [C.c_void_p]
+
+
+def _dropin():
+    L = tz.lib()
+    vp, sz, i32 = C.c_void_p, C.c_size_t, C.c_int
+    for name, f in (("float", C.c_float), ("double", C.c_double)):
+        comp = getattr(L, f"szp_cuda_{name}_compress_threadblock")
+        comp.argtypes, comp.restype = [vp, C.POINTER(sz), f, sz, i32], vp
+        dec = getattr(L, f"szp_cuda_{name}_decompress_threadblock")
+        dec.argtypes, dec.restype = [sz, f, i32, vp], vp
+    L.szp_cuda_double_compress_randomaccess.argtypes = [vp, C.POINTER(sz), C.c_double, sz, i32]
+    L.szp_cuda_double_compress_randomaccess.restype = vp
+    L.szp_cuda_double_decompress_randomaccess.argtypes = [sz, C.c_double, i32, vp]
+    L.szp_cuda_double_decompress_randomaccess.restype = vp
+    L.szp_cuda_double_compress_topology_3d.argtypes = [vp, C.POINTER(sz), C.c_double, sz, i32, vp, i32, i32, i32,
+                                                       i32]
+    L.szp_cuda_double_compress_topology_3d.restype = vp
+    for name in ("direct", "threadblock"):
+        f = getattr(L, f"szp_cuda_float_{name}_predict_quantization")
+        f.argtypes, f.restype = [vp, C.POINTER(sz), C.c_float, sz, i32], vp
+    return L
+
+
+def _take(ptr, nbytes, dtype):
+    assert ptr, tz.last_error()
+    out = np.frombuffer(bytes((C.c_ubyte * nbytes).from_address(ptr)), dtype=dtype).copy()
+    _libc.free(ptr)
+    return out
+
+
+def _dropin_compress(L, data, eb, bs, ra):
+    name = "float" if data.dtype == np.float32 else "double"
+    fn = L.szp_cuda_double_compress_randomaccess if ra else getattr(L, f"szp_cuda_{name}_compress_threadblock")
+    osz = C.c_size_t()
+    p = fn(data.ctypes.data_as(C.c_void_p), C.byref(osz), eb, data.size, bs)
+    return _take(p, osz.value, np.uint8)
+
+
+def _dropin_decompress(L, stream_after_eb, n, eb, bs, ra, dtype):
+    name = "float" if dtype == np.float32 else "double"
+    fn = L.szp_cuda_double_decompress_randomaccess if ra else getattr(L, f"szp_cuda_{name}_decompress_threadblock")
+    buf = np.ascontiguousarray(stream_after_eb)
+    p = fn(n, eb, bs, buf.ctypes.data_as(C.c_void_p))
+    return _take(p, np.dtype(dtype).itemsize * n, dtype)
+
+
+@pytest.mark.parametrize("path", ELEM, ids=os.path.basename)
+def test_dropin_elem_fixtures(path):
+    """The drop-in writes the one-chunk fixture byte for byte and reads every fixture
+    (chunk count detected from the offset table)."""
+    _torch()
+    L = _dropin()
+    g = _load(path)
+    data, eb, bs, ra = g["data"], float(g["eb"]), int(g["bs"]), bool(g["ra"])
+    ds = data.dtype.itemsize
+    got = _dropin_compress(L, data, eb, bs, ra)
+    assert np.array_equal(got, g["elem_T1"])
+    for key in g:
+        if key.startswith("elem_T"):
+            T = int(key[6:])
+            dec = _dropin_decompress(L, g[key][ds:], data.size, eb, bs, ra, data.dtype)
+            want = oracle.elem_decompress(g[key][ds:], data.size, eb, bs, T, ra=ra, dtype=data.dtype)
+            assert np.array_equal(dec.view(np.uint8), want.view(np.uint8)), key
+
+
+# (with more chunks than elements, the leading chunks are empty and their zero offsets
+# read as a valid stream of zeros under a smaller chunk count -- for the reference's
+# detection as for this one -- so detection is tested with every chunk non-empty)
+@pytest.mark.parametrize("n,bs,T", [(1000, 64, 1), (1000, 64, 3), (4097, 1025, 5), (20011, 32, 64), (77, 7, 16),
+                                    (300, 2, 128)])
+@pytest.mark.parametrize("dt,ra", [(np.float32, False), (np.float64, False), (np.float64, True)],
+                         ids=["tb_f32", "tb_f64", "ra_f64"])
+def test_dropin_elem_detects_chunks(n, bs, T, dt, ra):
+    _torch()
+    L = _dropin()
+    rng = np.random.default_rng(n + bs + T)
+    x = (np.cumsum(rng.normal(0, 1, n)) * 0.01).astype(dt)
+    s = oracle.elem_compress(x, 1e-3, bs, T=T, ra=ra)
+    ds = x.dtype.itemsize
+    dec = _dropin_decompress(L, s[ds:], n, 1e-3, bs, ra, dt)
+    want = oracle.elem_decompress(s[ds:], n, 1e-3, bs, T, ra=ra, dtype=dt)
+    assert np.array_equal(dec.view(np.uint8), want.view(np.uint8))
+    if T == 1:
+        assert np.array_equal(_dropin_compress(L, x, 1e-3, bs, ra), s)
+
+
+def test_dropin_topo64():
+    _torch()
+    L = _dropin()
+    data = fields.fbm3d(20, 24, 28, seed=5)
+    eb, bs = 3e-2, 64
+    cps = oracle.find_cps_3d(data, eb)
+    types = oracle.mark_types(cps, data.shape)
+    d64 = np.ascontiguousarray(data.astype(np.float64) * (1.0 + 2.0 ** -30))
+    ref, _ = oracle.topo3d_f64_compress(d64, eb, bs, types.reshape(data.shape))
+    osz = C.c_size_t()
+    p = L.szp_cuda_double_compress_topology_3d(d64.ctypes.data_as(C.c_void_p), C.byref(osz), eb, d64.size, bs,
+                                               cps.ctypes.data_as(C.c_void_p), cps.size, *data.shape)
+    got = _take(p, osz.value, np.uint8)
+    assert np.array_equal(got, ref)
+
+
+def test_dropin_predict_quantization():
+    """q[i] - q[i - 1] with q the C cast of x * (1 / (double)eb): IEEE double multiply and
+    truncation, which numpy computes identically for |x / eb| < 2^31."""
+    _torch()
+    L = _dropin()
+    rng = np.random.default_rng(3)
+    x = (rng.normal(0, 50, 100003)).astype(np.float32)
+    eb = np.float32(1e-3)
+    q = np.trunc(x.astype(np.float64) * (1.0 / np.float64(eb))).astype(np.int64)
+    want = np.diff(q, prepend=0).astype(np.int32)
+    for name in ("direct", "threadblock"):
+        fn = getattr(L, f"szp_cuda_float_{name}_predict_quantization")
+        p = fn(x.ctypes.data_as(C.c_void_p), None, float(eb), x.size, 64)
+        assert np.array_equal(_take(p, 4 * x.size, np.int32), want)

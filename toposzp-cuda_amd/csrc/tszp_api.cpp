@@ -743,24 +743,16 @@ ElemArgs elem_geometry(size_t n, int bs, unsigned T, bool ra) {
   return a;
 }
 
-int elem_compress(const void* d_in, int dsize, size_t n, double eb, int bs, unsigned T, bool ra, uint8_t* d_out,
-                  size_t cap, uint32_t* d_blk, uint64_t* out_bytes, hipStream_t st, const char* what) {
-  g_err.clear();
-  if (!d_in || !d_out || !d_blk || !out_bytes) return fail(TSZP_EINVAL, "%s: bad arguments", what);
-  if (!(eb > 0) || (dsize == 4 && !((float)eb > 0))) return fail(TSZP_EINVAL, "%s: error bound must be positive", what);
-  int r = check_elem(n, bs, T, dsize, what);
-  if (r) return r;
+// the caller holds c->mu; arguments are checked
+int elem_compress_impl(Ctx* c, const void* d_in, int dsize, size_t n, double eb, int bs, unsigned T, bool ra,
+                       uint8_t* d_out, size_t cap, uint32_t* d_blk, uint64_t* out_bytes, hipStream_t st) {
   const size_t hdr = (size_t)dsize + 8 * (size_t)T;
-  if (((uintptr_t)(d_out + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
-  if (cap < hdr + 4) return fail(TSZP_ECAP, "%s: stream capacity %zu too small", what, cap);
-  Ctx* c;
-  if ((r = get_ctx(&c))) return r;
-  std::lock_guard<std::mutex> lk(c->mu);
   ElemArgs a = elem_geometry(n, bs, T, ra);
   uint64_t *stb, *stc;
   uint32_t* counter;
   CompressResult* res;
-  if ((r = codec_prepare(c, a.ntiles, &stb, &stc, &counter, &res, st))) return r;
+  int r = codec_prepare(c, a.ntiles, &stb, &stc, &counter, &res, st);
+  if (r) return r;
   a.in = d_in;
   // float data: the reciprocal is an int / float division, then widened
   // (SZp/src/szp_float.cc:1489); double data: 1 / eb (SZp/src/szp_double.cc:188, :787)
@@ -789,24 +781,33 @@ int elem_compress(const void* d_in, int dsize, size_t n, double eb, int bs, unsi
   return TSZP_OK;
 }
 
-int elem_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, double eb, int bs,
-                    int dsize, bool ra, void* d_out, hipStream_t st, const char* what) {
+int elem_compress(const void* d_in, int dsize, size_t n, double eb, int bs, unsigned T, bool ra, uint8_t* d_out,
+                  size_t cap, uint32_t* d_blk, uint64_t* out_bytes, hipStream_t st, const char* what) {
   g_err.clear();
-  if (!d_stream || !d_blk || !d_out) return fail(TSZP_EINVAL, "%s: bad arguments", what);
+  if (!d_in || !d_out || !d_blk || !out_bytes) return fail(TSZP_EINVAL, "%s: bad arguments", what);
+  if (!(eb > 0) || (dsize == 4 && !((float)eb > 0))) return fail(TSZP_EINVAL, "%s: error bound must be positive", what);
   int r = check_elem(n, bs, T, dsize, what);
   if (r) return r;
   const size_t hdr = (size_t)dsize + 8 * (size_t)T;
-  if (((uintptr_t)(d_stream + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
+  if (((uintptr_t)(d_out + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
+  if (cap < hdr + 4) return fail(TSZP_ECAP, "%s: stream capacity %zu too small", what, cap);
   Ctx* c;
   if ((r = get_ctx(&c))) return r;
   std::lock_guard<std::mutex> lk(c->mu);
+  return elem_compress_impl(c, d_in, dsize, n, eb, bs, T, ra, d_out, cap, d_blk, out_bytes, st);
+}
+
+// the caller holds c->mu; arguments are checked
+int elem_decompress_impl(Ctx* c, const uint8_t* d_payload, const uint32_t* d_blk, unsigned T, size_t n, double eb,
+                         int bs, int dsize, bool ra, void* d_out, hipStream_t st) {
   ElemDecodeArgs a{};
   a.geo = elem_geometry(n, bs, T, ra);
   uint64_t *stb, *stc;
   uint32_t* counter;
   CompressResult* res;
-  if ((r = codec_prepare(c, a.geo.ntiles, &stb, &stc, &counter, &res, st))) return r;
-  a.payload = d_stream + hdr;
+  int r = codec_prepare(c, a.geo.ntiles, &stb, &stc, &counter, &res, st);
+  if (r) return r;
+  a.payload = d_payload;
   a.blk = d_blk;
   a.eb = dsize == 4 ? (double)(float)eb : eb;
   a.out = d_out;
@@ -820,6 +821,141 @@ int elem_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, 
   r = read_result(c, res, st, &out);
   prof.collect();
   return r;
+}
+
+int elem_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsigned T, size_t n, double eb, int bs,
+                    int dsize, bool ra, void* d_out, hipStream_t st, const char* what) {
+  g_err.clear();
+  if (!d_stream || !d_blk || !d_out) return fail(TSZP_EINVAL, "%s: bad arguments", what);
+  int r = check_elem(n, bs, T, dsize, what);
+  if (r) return r;
+  const size_t hdr = (size_t)dsize + 8 * (size_t)T;
+  if (((uintptr_t)(d_stream + hdr)) & 3u) return fail(TSZP_EINVAL, "%s: stream must be 4-byte aligned", what);
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return elem_decompress_impl(c, d_stream + hdr, d_blk, T, n, eb, bs, dsize, ra, d_out, st);
+}
+
+// ---- element-partitioned streams in host memory (the drop-in entry points) ----
+// Walk the units of every chunk of a T-chunk stream (cmp at its offset table): unit
+// offsets relative to the payload into blk, chunks required to be contiguous (chunk t
+// starts where chunk t - 1 ended), every bit count possible, nothing past `lim`
+// payload bytes.  The layout rules are those of the writers
+// (SZp/src/szp_float.cc:1505-1570, szp_double.cc:200-278, :810-880).
+bool walk_elem(const unsigned char* cmp, size_t n, int bs, unsigned T, bool ra, size_t lim, uint32_t* blk,
+               size_t* total) {
+  const unsigned char* p = cmp + 8 * (size_t)T;
+  const size_t ce = n / T;
+  size_t pos = 0, u = 0;
+  auto run = [&](size_t m, bool bc_always) -> bool {
+    if (m == 0 && !bc_always) return true;
+    if (pos >= lim) return false;
+    const unsigned bc = p[pos++];
+    if (bc > 32) return false;
+    if (bc) pos += (m + 7) / 8 + fixed_len((unsigned)m, bc);
+    return pos <= lim;
+  };
+  for (unsigned t = 0; t < T; t++) {
+    if (rd64(cmp + 8 * (size_t)t) != pos) return false;
+    const size_t lo = (size_t)t * ce, hi = t == T - 1 ? n : lo + ce;
+    if (lo >= hi) continue;
+    if (ra) {
+      for (size_t i = lo; i < hi; i += (size_t)bs) {
+        const size_t cur = i + bs > hi ? hi - i : (size_t)bs;
+        blk[u++] = (uint32_t)pos;
+        pos += 4;
+        if (pos > lim || !run(cur - 1, true)) return false;
+      }
+    } else {
+      blk[u++] = (uint32_t)pos;
+      pos += 4;
+      if (pos > lim) return false;
+      for (size_t i = lo + 1; i < hi; i += (size_t)bs) {
+        const size_t cur = i + bs > hi ? hi - i : (size_t)bs;
+        if (i > lo + 1) blk[u++] = (uint32_t)pos;
+        if (!run(cur, false)) return false;
+      }
+    }
+  }
+  blk[u] = (uint32_t)pos;
+  *total = pos;
+  return true;
+}
+
+// Chunk count of a host stream (SZp/include/szp_detect_threads.h:196-222 lists the
+// candidates): none when off[0] != 0; else, among the T in 2..128 whose offsets are
+// non-decreasing and bounded, the largest whose chunks walk, each ending exactly where
+// the next one starts; else 1 if one chunk walks.  (The reference takes the smallest
+// such T; small chunks make that ambiguous, since an offset table of small numbers
+// between zero bytes reads as a run of all-zero blocks under a smaller T.)
+unsigned detect_elem_chunks(const unsigned char* cmp, size_t n, int bs, bool ra, size_t lim,
+                            std::vector<uint32_t>& blk, size_t* total) {
+  if (rd64(cmp) != 0) return 0;
+  unsigned tmax = 1;
+  for (unsigned t = 2; t <= 128; t++) {
+    const uint64_t o = rd64(cmp + 8 * (size_t)(t - 1));
+    if (o < rd64(cmp + 8 * (size_t)(t - 2)) || o > lim) break;
+    tmax = t;
+  }
+  for (unsigned t = tmax; t >= 2; t--) {
+    blk.assign(elem_units(n, (uint32_t)bs, t, ra, nullptr) + 1, 0u);
+    if (walk_elem(cmp, n, bs, t, ra, lim, blk.data(), total)) return t;
+  }
+  blk.assign(elem_units(n, (uint32_t)bs, 1, ra, nullptr) + 1, 0u);
+  return walk_elem(cmp, n, bs, 1, ra, lim, blk.data(), total) ? 1u : 0u;
+}
+
+// host stream (at its offset table) -> device values in the context's workspace
+int elem_decode_host(Ctx* c, const unsigned char* cmp, size_t n, double eb, int bs, int dsize, bool ra, void** d_out,
+                     hipStream_t st) {
+  std::vector<uint32_t> blk;
+  size_t total = 0;
+  const size_t lim = tszp_elem_bound(n, bs, 128, dsize, ra);
+  const unsigned T = detect_elem_chunks(cmp, n, bs, ra, lim, blk, &total);
+  if (!T) return fail(TSZP_EINVAL, "corrupt stream (no chunk count walks)");
+  const size_t hdr = 8 * (size_t)T;
+  uint8_t* dp;
+  uint32_t* db;
+  void* dout;
+  WS(kWsStream, total + 8, dp);
+  WS(kWsBlk, blk.size() * 4, db);
+  WS(kWsOut2, n * (size_t)dsize, dout);
+  HIPCHK(hipMemcpyAsync(dp, cmp + hdr, total, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(db, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, st));
+  int r = elem_decompress_impl(c, dp, db, T, n, eb, bs, dsize, ra, dout, st);  // synchronises
+  if (r) return r;
+  *d_out = dout;
+  return TSZP_OK;
+}
+
+// host values -> host stream [eb big-endian][u64 0][payload] (one chunk, as the CUDA
+// writers); out must hold tszp_elem_bound(n, bs, 1, dsize, ra) bytes
+int elem_compress_host(const void* data, int dsize, size_t n, double eb, int bs, bool ra, unsigned char* out,
+                       size_t* outSize) {
+  *outSize = 0;
+  if (!data || !out) return fail(TSZP_EINVAL, "bad arguments");
+  if (!(eb > 0) || (dsize == 4 && !((float)eb > 0))) return fail(TSZP_EINVAL, "error bound must be positive");
+  int r = check_elem(n, bs, 1, dsize, "compress");
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  const size_t cap = tszp_elem_bound(n, bs, 1, dsize, ra);
+  void* din;
+  uint8_t* dout;
+  uint32_t* dblk;
+  WS(kWsIn, n * (size_t)dsize, din);
+  WS(kWsOut, cap, dout);
+  WS(kWsBlk, (tszp_elem_units(n, bs, 1, ra) + 1) * 4, dblk);
+  HIPCHK(hipMemcpyAsync(din, data, n * (size_t)dsize, hipMemcpyHostToDevice, st));
+  uint64_t nbytes = 0;
+  if ((r = elem_compress_impl(c, din, dsize, n, eb, bs, 1, ra, dout, cap, dblk, &nbytes, st))) return r;
+  HIPCHK(hipMemcpyAsync(out, dout, nbytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *outSize = nbytes;
+  return TSZP_OK;
 }
 
 }  // namespace
@@ -860,13 +996,9 @@ int tszp_plain_decompress_f64(const uint8_t* d_stream, const uint32_t* d_blk, un
   return elem_decompress(d_stream, d_blk, T, n, eb, bs, 8, true, d_out, (hipStream_t)stream, "plain_decompress_f64");
 }
 
-// szp_cuda_double_compress_topology_3d (SZp/src/szp_cuda_topology_3d.cu:1065-1188)
-int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, double eb, int bs, const int32_t* d_cps,
-                              uint64_t cp_count, uint8_t* d_out, size_t cap, uint32_t* d_blk, uint64_t* out_bytes,
-                              int32_t* d_q, uint64_t* moved, void* stream) {
-  g_err.clear();
-  if (!d_field || !dims || !d_out || !d_blk || !out_bytes || (cp_count && !d_cps))
-    return fail(TSZP_EINVAL, "topo_compress_f64_3d: bad arguments");
+namespace {
+
+int check_topo64(const uint32_t* dims, double eb, int bs, uint64_t cp_count, size_t* n_out) {
   if (!(eb > 0)) return fail(TSZP_EINVAL, "topo_compress_f64_3d: error bound must be positive");
   int r = check_bs(bs, true);
   if (r) return r;
@@ -877,10 +1009,15 @@ int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, doubl
   }
   if ((r = check_n(n))) return r;
   if (cp_count > 0x7fffffffull) return fail(TSZP_EINVAL, "topo_compress_f64_3d: too many critical points");
-  Ctx* c;
-  if ((r = get_ctx(&c))) return r;
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipStream_t st = (hipStream_t)stream;
+  *n_out = n;
+  return TSZP_OK;
+}
+
+// the caller holds c->mu; arguments are checked
+int topo64_impl(Ctx* c, const double* d_field, const uint32_t* dims, size_t n, double eb, int bs,
+                const int32_t* d_cps, uint64_t cp_count, uint8_t* d_out, size_t cap, uint32_t* d_blk,
+                uint64_t* out_bytes, int32_t* d_q, uint64_t* moved, hipStream_t st) {
+  int r;
   const Grid g = make_grid(3, dims);
   uint8_t* types;
   int32_t* q;
@@ -929,6 +1066,37 @@ int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, doubl
   if (hm[1]) return fail(TSZP_EDEVICE, "topo_compress_f64_3d: saddle list scan failed");
   if (moved) *moved = hm[0];
   *out_bytes = 8 + res.payload_bytes;
+  return TSZP_OK;
+}
+
+}  // namespace
+
+// szp_cuda_double_compress_topology_3d (SZp/src/szp_cuda_topology_3d.cu:1065-1188)
+int tszp_topo_compress_f64_3d(const double* d_field, const uint32_t* dims, double eb, int bs, const int32_t* d_cps,
+                              uint64_t cp_count, uint8_t* d_out, size_t cap, uint32_t* d_blk, uint64_t* out_bytes,
+                              int32_t* d_q, uint64_t* moved, void* stream) {
+  g_err.clear();
+  if (!d_field || !dims || !d_out || !d_blk || !out_bytes || (cp_count && !d_cps))
+    return fail(TSZP_EINVAL, "topo_compress_f64_3d: bad arguments");
+  size_t n;
+  int r = check_topo64(dims, eb, bs, cp_count, &n);
+  if (r) return r;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return topo64_impl(c, d_field, dims, n, eb, bs, d_cps, cp_count, d_out, cap, d_blk, out_bytes, d_q, moved,
+                     (hipStream_t)stream);
+}
+
+// q[i] - q[i - 1] (q[-1] = 0) with q = C cast of x * (1 / (double)eb)
+// (SZp/src/szp_cuda_compress.cu:360-377, :664-710)
+int tszp_predict_quantization(const float* d_in, size_t n, float eb, int32_t* d_out, void* stream) {
+  g_err.clear();
+  if (!d_in || !d_out) return fail(TSZP_EINVAL, "predict_quantization: bad arguments");
+  if (!(eb > 0)) return fail(TSZP_EINVAL, "predict_quantization: error bound must be positive");
+  int r = check_n(n);
+  if (r) return r;
+  HIPCHK(launch_predict_quant(d_in, (uint32_t)n, 1.0 / (double)eb, d_out, (hipStream_t)stream));
   return TSZP_OK;
 }
 
@@ -1729,6 +1897,195 @@ float* szp_cuda_float_decompress_randomaccess(size_t n, float eb, int bs, unsign
     free(out);
     return nullptr;
   }
+  return out;
+}
+
+// ---- double random access and the threadblock codec from host buffers -------------
+// Streams are the CPU writers' with one chunk: [eb big-endian][u64 0][payload]
+// (SZp/src/szp_float.cc:1445-1590, szp_double.cc:148-318, :759-920).  Decoders take
+// cmp at the offset table (after the eb header), detect the chunk count and walk the
+// unit offsets on the host, as SZp/src/szp_cuda_decompress.cu:480-525, 907-970 do.
+extern "C++" {
+namespace {
+
+template <typename T>
+void elem_compress_arg(unsigned char* output, T* data, size_t* outSize, T eb, size_t n, int bs, bool ra) {
+  g_err.clear();
+  if (!outSize) return;
+  elem_compress_host(data, (int)sizeof(T), n, (double)eb, bs, ra, output, outSize);
+}
+
+template <typename T>
+unsigned char* elem_compress_alloc(T* data, size_t* outSize, T eb, size_t n, int bs, bool ra) {
+  g_err.clear();
+  if (!outSize) return nullptr;
+  *outSize = 0;
+  if (!data || check_elem(n, bs, 1, (int)sizeof(T), "compress")) return nullptr;
+  unsigned char* out = (unsigned char*)malloc(tszp_elem_bound(n, bs, 1, (int)sizeof(T), ra));
+  if (!out) return nullptr;
+  if (elem_compress_host(data, (int)sizeof(T), n, (double)eb, bs, ra, out, outSize)) {
+    free(out);
+    *outSize = 0;
+    return nullptr;
+  }
+  unsigned char* shrunk = (unsigned char*)realloc(out, *outSize);
+  return shrunk ? shrunk : out;
+}
+
+template <typename T>
+void elem_decompress_arg(T* newData, size_t n, T eb, int bs, unsigned char* cmp, bool ra) {
+  g_err.clear();
+  if (!newData || !cmp) {
+    fail(TSZP_EINVAL, "decompress: bad arguments");
+    return;
+  }
+  if (check_elem(n, bs, 1, (int)sizeof(T), "decompress")) return;
+  Ctx* c;
+  if (get_ctx(&c)) return;
+  std::lock_guard<std::mutex> lk(c->mu);
+  void* dout;
+  if (elem_decode_host(c, cmp, n, (double)eb, bs, (int)sizeof(T), ra, &dout, c->stream)) return;
+  if (hipMemcpy(newData, dout, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) fail(TSZP_EHIP, "hipMemcpy");
+}
+
+template <typename T>
+T* elem_decompress_alloc(size_t n, T eb, int bs, unsigned char* cmp, bool ra) {
+  g_err.clear();
+  if (!cmp || check_n(n)) return nullptr;
+  T* out = (T*)malloc(n * sizeof(T));
+  if (!out) return nullptr;
+  elem_decompress_arg<T>(out, n, eb, bs, cmp, ra);
+  if (!g_err.empty()) {
+    free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+}  // namespace
+}  // extern "C++"
+
+unsigned char* szp_cuda_double_compress_randomaccess(double* data, size_t* outSize, double eb, size_t n, int bs) {
+  return elem_compress_alloc<double>(data, outSize, eb, n, bs, true);
+}
+void szp_cuda_double_compress_randomaccess_arg(unsigned char* output, double* data, size_t* outSize, double eb,
+                                               size_t n, int bs) {
+  elem_compress_arg<double>(output, data, outSize, eb, n, bs, true);
+}
+double* szp_cuda_double_decompress_randomaccess(size_t n, double eb, int bs, unsigned char* cmp) {
+  return elem_decompress_alloc<double>(n, eb, bs, cmp, true);
+}
+void szp_cuda_double_decompress_randomaccess_arg(double* newData, size_t n, double eb, int bs, unsigned char* cmp) {
+  elem_decompress_arg<double>(newData, n, eb, bs, cmp, true);
+}
+unsigned char* szp_cuda_float_compress_threadblock(float* data, size_t* outSize, float eb, size_t n, int bs) {
+  return elem_compress_alloc<float>(data, outSize, eb, n, bs, false);
+}
+void szp_cuda_float_compress_threadblock_arg(unsigned char* output, float* data, size_t* outSize, float eb, size_t n,
+                                             int bs) {
+  elem_compress_arg<float>(output, data, outSize, eb, n, bs, false);
+}
+unsigned char* szp_cuda_double_compress_threadblock(double* data, size_t* outSize, double eb, size_t n, int bs) {
+  return elem_compress_alloc<double>(data, outSize, eb, n, bs, false);
+}
+void szp_cuda_double_compress_threadblock_arg(unsigned char* output, double* data, size_t* outSize, double eb,
+                                              size_t n, int bs) {
+  elem_compress_arg<double>(output, data, outSize, eb, n, bs, false);
+}
+float* szp_cuda_float_decompress_threadblock(size_t n, float eb, int bs, unsigned char* cmp) {
+  return elem_decompress_alloc<float>(n, eb, bs, cmp, false);
+}
+void szp_cuda_float_decompress_threadblock_arg(float* newData, size_t n, float eb, int bs, unsigned char* cmp) {
+  elem_decompress_arg<float>(newData, n, eb, bs, cmp, false);
+}
+double* szp_cuda_double_decompress_threadblock(size_t n, double eb, int bs, unsigned char* cmp) {
+  return elem_decompress_alloc<double>(n, eb, bs, cmp, false);
+}
+void szp_cuda_double_decompress_threadblock_arg(double* newData, size_t n, double eb, int bs, unsigned char* cmp) {
+  elem_decompress_arg<double>(newData, n, eb, bs, cmp, false);
+}
+
+// SZp/src/szp_cuda_compress.cu:664-710; *outSize is left alone, as there
+int* szp_cuda_float_direct_predict_quantization(float* data, size_t* outSize, float eb, size_t n, int bs) {
+  (void)outSize;
+  (void)bs;
+  g_err.clear();
+  if (!data || !(eb > 0) || check_n(n)) {
+    if (g_err.empty()) fail(TSZP_EINVAL, "predict_quantization: bad arguments");
+    return nullptr;
+  }
+  Ctx* c;
+  if (get_ctx(&c)) return nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  float* din;
+  int32_t* dout;
+  if (c->get(kWsIn, n * 4, (void**)&din) || c->get(kWsOut2, n * 4, (void**)&dout)) return nullptr;
+  int* out = (int*)malloc(n * sizeof(int));
+  if (!out) return nullptr;
+  if (hipMemcpyAsync(din, data, n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      launch_predict_quant(din, (uint32_t)n, 1.0 / (double)eb, dout, st) != hipSuccess ||
+      hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    fail(TSZP_EHIP, "predict_quantization: device call failed");
+    free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+// SZp/src/szp_cuda_compress.cu:700-709: the same result as the direct variant
+int* szp_cuda_float_threadblock_predict_quantization(float* data, size_t* outSize, float eb, size_t n, int bs) {
+  return szp_cuda_float_direct_predict_quantization(data, outSize, eb, n, bs);
+}
+
+// SZp/src/szp_cuda_topology_3d.cu:1065-1188: [u64 0][blocks], host buffers
+unsigned char* szp_cuda_double_compress_topology_3d(double* data, size_t* outSize, double eb, size_t nbEle, int bs,
+                                                    CriticalPoint3D* cps, int count, int d1, int d2, int d3) {
+  g_err.clear();
+  if (!outSize) return nullptr;
+  *outSize = 0;
+  if (!data || d1 <= 0 || d2 <= 0 || d3 <= 0 || count < 0 || (count > 0 && !cps)) {
+    fail(TSZP_EINVAL, "szp_cuda_double_compress_topology_3d: bad arguments");
+    return nullptr;
+  }
+  const uint32_t dims[3] = {(uint32_t)d1, (uint32_t)d2, (uint32_t)d3};
+  size_t n;
+  if (check_topo64(dims, eb, bs, (uint64_t)count, &n)) return nullptr;
+  if (n != nbEle) {
+    fail(TSZP_EINVAL, "szp_cuda_double_compress_topology_3d: nbEle != d1 * d2 * d3");
+    return nullptr;
+  }
+  Ctx* c;
+  if (get_ctx(&c)) return nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = c->stream;
+  const size_t cap = tszp_topo_bound(n, bs, 1);
+  double* din;
+  int32_t* dc = nullptr;
+  uint8_t* dout;
+  uint32_t* dblk;
+  if (c->get(kWsIn, n * 8, (void**)&din) || c->get(kWsOut, cap, (void**)&dout) ||
+      c->get(kWsBlk, (tszp_blocks(n, bs) + 1) * 4, (void**)&dblk))
+    return nullptr;
+  if (count && c->get(kWsCps, (size_t)count * sizeof(CriticalPoint3D), (void**)&dc)) return nullptr;
+  if (hipMemcpyAsync(din, data, n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (count && hipMemcpyAsync(dc, cps, (size_t)count * sizeof(CriticalPoint3D), hipMemcpyHostToDevice, st) !=
+                    hipSuccess)) {
+    fail(TSZP_EHIP, "hipMemcpyAsync");
+    return nullptr;
+  }
+  uint64_t nbytes = 0;
+  if (topo64_impl(c, din, dims, n, eb, bs, dc, (uint64_t)count, dout, cap, dblk, &nbytes, nullptr, nullptr, st))
+    return nullptr;
+  unsigned char* out = (unsigned char*)malloc(nbytes);
+  if (!out) return nullptr;
+  if (hipMemcpy(out, dout, nbytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    fail(TSZP_EHIP, "hipMemcpy");
+    free(out);
+    return nullptr;
+  }
+  *outSize = nbytes;
   return out;
 }
 

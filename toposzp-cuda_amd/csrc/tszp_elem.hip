@@ -463,6 +463,32 @@ __global__ void k_elem_header(uint8_t* stream, int dsize, unsigned T, uint32_t u
   for (int k = 0; k < 8; k++) o[k] = (uint8_t)(off >> (8 * k));
 }
 
+// q[i] - q[i - 1] with q[-1] = 0 and q = C cast of x * inv: the quantization helpers
+// (SZp/src/szp_cuda_compress.cu:360-377); four points per thread, q of the point before
+// recomputed rather than exchanged
+__global__ __launch_bounds__(kBlockThreads) void k_predict_quant(const float* __restrict__ x, uint32_t n, double inv,
+                                                                int32_t* __restrict__ out) {
+  const uint32_t groups = (n + 3u) / 4u;
+  for (uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x; i < groups; i += gridDim.x * kBlockThreads) {
+    const uint32_t e0 = 4u * i;
+    int prev = e0 ? cvt_i32((double)x[e0 - 1u] * inv) : 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 4u; r++) {
+      if (e0 + r >= n) break;
+      const int q = cvt_i32((double)x[e0 + r] * inv);
+      out[e0 + r] = (int)((uint32_t)q - (uint32_t)prev);
+      prev = q;
+    }
+  }
+}
+
+hipError_t launch_predict_quant(const float* x, uint32_t n, double inv, int32_t* out, hipStream_t st) {
+  uint64_t b = ((uint64_t)n + 4u * kBlockThreads - 1) / (4u * kBlockThreads);
+  if (b > 16384) b = 16384;
+  hipLaunchKernelGGL(k_predict_quant, dim3(b ? (unsigned)b : 1u), dim3(kBlockThreads), 0, st, x, n, inv, out);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // host-side dispatch
 // ---------------------------------------------------------------------------

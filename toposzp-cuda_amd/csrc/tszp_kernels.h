@@ -83,6 +83,8 @@ hipError_t launch_elem_compress(int dsize, bool ra, const ElemArgs& a, hipStream
 hipError_t launch_elem_decode(int dsize, bool ra, const ElemDecodeArgs& a, hipStream_t st);
 hipError_t launch_elem_header(uint8_t* stream, int dsize, unsigned T, uint32_t ukc, const uint32_t* blk, double eb,
                               hipStream_t st);
+// out[i] = q[i] - q[i - 1], q[-1] = 0, q = C cast of x * inv
+hipError_t launch_predict_quant(const float* x, uint32_t n, double inv, int32_t* out, hipStream_t st);
 
 // ---- double 3D topology: quantization and saddle fix (tszp_saddle.hip) ----
 // types: zero-filled uint8[n], then marked from AoS CP records of 6 ints (x, y, z, type, ..)
