@@ -254,6 +254,8 @@ def main():
         torch.mul(f, 1.0, out=x)  # elementwise: reads and writes 4N bytes
         torch.cuda.synchronize()
         log(f"profile pass: cps={info.cp_count} topo={info.topo_bytes} sp={info.sortpos_bytes}")
+        if not args.no_elem:  # one call of each side codec, for per-dispatch counters
+            elem_side(tz, torch, f, shape, eb, bs, 1)
         return
 
     for i in range(args.warmup):

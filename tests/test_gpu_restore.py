@@ -143,3 +143,19 @@ def test_restore_out_of_place_matches_in_place(name, make, eb):
     np.testing.assert_array_equal(sep.cpu().numpy(), out.cpu().numpy())
     assert r1.mismatches() == r2.mismatches() == 0
     assert (r1.moves, r1.windows) == (r2.moves, r2.windows)
+
+
+@pytest.mark.parametrize("make,eb", [(lambda: fields.fbm3d(40, 44, 48, seed=71), 3e-3),
+                                     (lambda: fields.blobs2d(300, 310, seed=72, noise=0.05), 1e-3),
+                                     (lambda: fields.fbm3d(64, 64, 64, seed=73, noise=0.2), 1e-2)],
+                         ids=["fbm3d", "blobs2d", "fbm3d_noisy"])
+def test_solve_one_launch_equals_phases(make, eb, monkeypatch):
+    """The window solve in one dataflow launch (a tile waits for its neighbouring tiles of
+    lower phases) gives the field of one launch per parity phase, bit for bit."""
+    data = make()
+    print(f"solve flow vs phases {data.shape} eb={eb}", flush=True)
+    _, y_flow, _, info_flow = _pipeline(data, eb)
+    monkeypatch.setenv("TSZP_SOLVE_PHASES", "1")
+    _, y_ph, _, info_ph = _pipeline(data, eb)
+    assert np.array_equal(y_flow.view(np.uint32), y_ph.view(np.uint32))
+    assert info_flow.windows == info_ph.windows and info_flow.missing == info_ph.missing == 0

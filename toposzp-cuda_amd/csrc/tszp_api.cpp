@@ -49,7 +49,7 @@ enum Slot {
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
-  kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsCount
+  kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsCount
 };
 
 struct Ctx {
@@ -1304,13 +1304,24 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
+    for (int ph = 0; ph < phases; ph++) total += h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
     EventTimer solve_timer(kProfSolve);
-    for (int ph = 0; ph < phases; ph++) {
-      const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
-      total += cnt;
-      if (!cnt) continue;
+    if (getenv("TSZP_SOLVE_PHASES")) {  // one launch per parity phase
+      for (int ph = 0; ph < phases; ph++) {
+        const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+        if (!cnt) continue;
+        solve_timer.begin(st);
+        HIPCHK(pp_solve(g, tiles, cnt, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+        solve_timer.end(st);
+      }
+    } else if (total) {  // every phase in one launch, ordered per neighbouring tiles
+      uint32_t* done;
+      WS(kWsPpDone, ntiles * 4 + 16, done);
+      HIPCHK(launch_fill_u32(done, 0u, ntiles, st));
+      HIPCHK(launch_fill_u32(misc + kPpFlowCounter, 0u, 1, st));
       solve_timer.begin(st);
-      HIPCHK(pp_solve(g, tiles, cnt, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+      HIPCHK(pp_solve_flow(g, tiles, total, tcap, tx, ty, tz, tbits, done, bits, d, d_types, d_field, eb, use_sp,
+                           misc, st));
       solve_timer.end(st);
     }
     if ((r = full_mark(&nb, true))) return r;

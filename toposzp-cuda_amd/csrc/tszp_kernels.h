@@ -253,7 +253,8 @@ enum : int {
   kPpActC = 20,      // 7 colour-list counts
   kPpTilesP = 32,    // 8 parity-phase tile-list counts
   kPpStatNodes = 40, kPpStatMaxNodes = 41, kPpStatMaxWin = 44, kPpStatMaxTicks = 45, kPpStatMaxWait = 46, kPpStatLeaves = 47, kPpTiny = 42,
-  kPpSolveCounter = 43
+  kPpSolveCounter = 43,
+  kPpFlowCounter = 60  // tile counter of the one-launch solve
 };
 uint32_t pp_init_tiles(uint32_t n);
 hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
@@ -288,6 +289,11 @@ hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t 
 hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, uint32_t ty, uint32_t tz,
                     int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
                     int anchor, uint32_t* misc, hipStream_t st);
+// every phase in one launch (dataflow over neighbouring tiles): `total` listed tiles,
+// tbits the listed-tile bitmap, done one zeroed word per tile, misc[kPpFlowCounter] zeroed
+hipError_t pp_solve_flow(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
+                         uint32_t tz, const uint32_t* tbits, uint32_t* done, const uint32_t* pbits, const float* d,
+                         const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st);
 
 // ---- synthetic fields (tszp_gen.hip) ----
 hipError_t launch_gen_blobs(float* f, int ndim, const uint32_t* dims, uint64_t seed, int nblobs,

@@ -157,6 +157,32 @@ __device__ __forceinline__ int classify_at(const Grid& g, const float* __restric
   return classify_vals<ND>(y[e], v);
 }
 
+// Field accesses of the window solver.  COH: agent-coherent loads and stores (they
+// bypass the XCD-local caches), for the one-launch solve where neighbouring tiles run
+// on other XCDs of the same launch.
+template <bool COH>
+__device__ __forceinline__ float y_ld(const float* y, uint32_t e) {
+  if constexpr (COH) return __hip_atomic_load(const_cast<float*>(y) + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return y[e];
+}
+template <bool COH>
+__device__ __forceinline__ void y_st(float* y, uint32_t e, float v) {
+  if constexpr (COH) __hip_atomic_store(y + e, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else y[e] = v;
+}
+template <int ND, bool COH>
+__device__ __forceinline__ int classify_at_y(const Grid& g, const float* y, uint32_t e, const int (&c)[ND]) {
+  if (!interior<ND>(g, c)) return 0;
+  float v[2 * ND];
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    const uint32_t s = stride_of<ND>(g, a);
+    v[2 * a] = y_ld<COH>(y, e - s);
+    v[2 * a + 1] = y_ld<COH>(y, e + s);
+  }
+  return classify_vals<ND>(y_ld<COH>(y, e), v);
+}
+
 // interval of the restored value of a point with decoded value d
 __device__ __forceinline__ void cell_of(float d, float eb, float* lo, float* hi) {
   if (d > 0.f) {
@@ -1330,9 +1356,9 @@ __device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, const float* __re
 
 // Builds the window of mismatch b (whole wavefront: the lanes load neighbours in
 // parallel, lane 0 keeps the lists in order).
-template <int ND>
+template <int ND, bool COH>
 __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restrict__ d,
-                             const uint8_t* __restrict__ t, const float* __restrict__ y, float eb, uint32_t b,
+                             const uint8_t* __restrict__ t, const float* y, float eb, uint32_t b,
                              bool anchor, int lane) {
   constexpr int K = 2 * ND;
   if (lane == 0) {
@@ -1376,7 +1402,7 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
   const int ne = w.ne;
   if (lane < ne) {
     const uint32_t e = w.el[lane];
-    w.val[lane] = y[e];
+    w.val[lane] = y_ld<COH>(y, e);
     const int tt = t[e];
     w.anch[lane] = anchor && anchored(tt, db);
     int c[ND];
@@ -1397,7 +1423,7 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
       // not an element: another cell (the order of the cells) or a point that keeps its
       // value next to an element that keeps its value
       const float dq = d[qe];
-      const float r = dq != db ? db - dq : w.val[i] - y[qe];
+      const float r = dq != db ? db - dq : w.val[i] - y_ld<COH>(y, qe);
       w.cnb[i][k] = r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
     }
   }
@@ -1416,8 +1442,8 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
 // fixed elements keep their value, runs of free classes are spaced evenly between
 // them), is written to the field and verified there by the constraint lanes; the first
 // order that holds is kept.  All control flow is uniform over the wavefront.
-template <int ND>
-__device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* __restrict__ y,
+template <int ND, bool COH>
+__device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restrict__ t, float* y,
                             int mode, int pick, float delta, int lane, uint32_t* nodes_out, uint32_t* stats) {
   constexpr int K = 2 * ND;
   const int ne = w.ne, nw = w.nw;
@@ -1627,23 +1653,25 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
     if (!leaf) break;
     // write the free points and check every constraint on the field
     if (my_free) {
-      my_old = y[w.el[lane]];
-      y[w.el[lane]] = my_fval;
+      my_old = y_ld<COH>(y, w.el[lane]);
+      y_st<COH>(y, w.el[lane], my_fval);
     }
+    if constexpr (COH) __builtin_amdgcn_s_waitcnt(0);  // the stores land before other lanes read
     __syncthreads();
     bool good = true;
     if (my_dep >= 0) {
       const uint32_t e = w.el[lane];
       int c[ND];
       coords_of<ND>(g, e, c);
-      good = classify_at<ND>(g, y, e, c) == (int)t[e];
+      good = classify_at_y<ND, COH>(g, y, e, c) == (int)t[e];
     }
     if (!__any(!good)) {
       found = true;
       break;
     }
-    if (my_free) y[w.el[lane]] = my_old;
+    if (my_free) y_st<COH>(y, w.el[lane], my_old);
     vfails++;
+    if constexpr (COH) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
   if (lane == 0) {
@@ -1660,15 +1688,15 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
 // neighbouring bin (the bin rounds x + eb in float, the cell truncates x / eb in
 // double), which makes the anchored values of its window contradictory; modes 1-3
 // free extrema, the most extreme anchored value first.
-template <int ND>
+template <int ND, bool COH>
 __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
-                            float* __restrict__ y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
+                            float* y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
                             uint32_t* nodes, uint32_t* stats) {
   const uint64_t t0 = wall_clock64();
-  build_window<ND>(w, g, d, t, y, eb, b, anchor, lane);
+  build_window<ND, COH>(w, g, d, t, y, eb, b, anchor, lane);
   const uint64_t t1 = wall_clock64();
   stats[4] += (uint32_t)(t1 - t0);
-  const bool ok0 = run_attempt<ND>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats);
+  const bool ok0 = run_attempt<ND, COH>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats);
   stats[5] += (uint32_t)(wall_clock64() - t1);
   if (ok0) return 0;
   if (lane == 0) {
@@ -1690,31 +1718,17 @@ __device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restric
   if (na == 0) return -1;
   const float delta = delta_frac * (w.hi - w.lo);
   for (int r = 0; r < na; r++)
-    if (run_attempt<ND>(w, g, t, y, 1, w.picks[r], delta, lane, nodes, stats)) return 1;
+    if (run_attempt<ND, COH>(w, g, t, y, 1, w.picks[r], delta, lane, nodes, stats)) return 1;
   for (int r = 0; r < na; r++)
-    if (run_attempt<ND>(w, g, t, y, 2, w.picks[r], 0.f, lane, nodes, stats)) return 2;
-  if (na > 1 && run_attempt<ND>(w, g, t, y, 3, -1, 0.f, lane, nodes, stats)) return 3;
+    if (run_attempt<ND, COH>(w, g, t, y, 2, w.picks[r], 0.f, lane, nodes, stats)) return 2;
+  if (na > 1 && run_attempt<ND, COH>(w, g, t, y, 3, -1, 0.f, lane, nodes, stats)) return 3;
   return -1;
 }
 
-// One wavefront per listed tile of one parity phase (a launch per phase: tiles of one
-// phase never share a window, and the launch boundary makes one phase's values visible
-// to the next).  Inside a tile the listed mismatches are solved in flat order.
+constexpr int kTileSide = 1 << kTileShift;
+
 template <int ND>
-__global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                    uint32_t ty, uint32_t tz, int phase,
-                                                    const uint32_t* __restrict__ pbits, const float* __restrict__ d,
-                                                    const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
-                                                    int anchor, uint32_t* misc) {
-  constexpr int kT = 1 << kTileShift;
-  constexpr int kTilePts = ND == 2 ? kT * kT : kT * kT * kT;
-  __shared__ Window<ND> w;
-  __shared__ uint32_t blist[kTilePts];
-  __shared__ uint32_t sh_nb;
-  const int lane = threadIdx.x;
-  const uint64_t t_work = wall_clock64();
-  uint32_t id = tiles[(size_t)phase * cap + blockIdx.x];
-  int tc[ND];
+__device__ __forceinline__ void tile_coords(uint32_t id, uint32_t ty, uint32_t tz, int (&tc)[ND]) {
   if constexpr (ND == 2) {
     tc[1] = (int)(id % ty);
     tc[0] = (int)(id / ty);
@@ -1724,6 +1738,39 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     tc[1] = (int)(id % ty);
     tc[0] = (int)(id / ty);
   }
+}
+
+// parity phase of a tile: tiles of one phase are two or more tiles apart, so their
+// windows never meet
+template <int ND>
+__device__ __forceinline__ int tile_phase(const int (&tc)[ND]) {
+  int ph = 0;
+#pragma unroll
+  for (int a = 0; a < ND; a++) ph |= (tc[a] & 1) << a;
+  return ph;
+}
+
+template <int ND>
+struct SolveShared {
+  Window<ND> w;
+  uint32_t blist[ND == 2 ? kTileSide * kTileSide : kTileSide * kTileSide * kTileSide];
+  uint32_t nb;
+  uint32_t tile;
+};
+
+// One wavefront solves the listed mismatches of tile `id` in flat order.
+template <int ND, bool COH>
+__device__ void solve_tile(SolveShared<ND>& sh, const Grid& g, uint32_t id, uint32_t ty, uint32_t tz,
+                           const uint32_t* __restrict__ pbits, const float* __restrict__ d,
+                           const uint8_t* __restrict__ t, float* y, float eb, int anchor,
+                           uint32_t* misc) {
+  constexpr int kT = kTileSide;
+  Window<ND>& w = sh.w;
+  uint32_t* blist = sh.blist;
+  const int lane = threadIdx.x;
+  const uint64_t t_work = wall_clock64();
+  int tc[ND];
+  tile_coords<ND>(id, ty, tz, tc);
   const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
   const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
   int z0 = 0, z1 = 1;
@@ -1765,9 +1812,9 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     }
     nb += __shfl(incl, kWave - 1, kWave);
   }
-  if (lane == 0) sh_nb = nb;
+  if (lane == 0) sh.nb = nb;
   __syncthreads();
-  nb = sh_nb;
+  nb = sh.nb;
   uint32_t solved = 0, failed = 0, freed = 0, nodes = 0;
   uint32_t tst[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // [0] leaves [1] verify fails [4] build ticks [5] attempt ticks
   // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
@@ -1776,9 +1823,9 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     const uint32_t e = blist[j];
     int c[ND];
     coords_of<ND>(g, e, c);
-    if (classify_at<ND>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
+    if (classify_at_y<ND, COH>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
     const int mode =
-        solve_window<ND>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
+        solve_window<ND, COH>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
     solved += mode >= 0;
     failed += mode < 0;
     freed += mode >= 2;
@@ -1797,6 +1844,87 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
     atomicMax(&misc[kPpStatLeaves + 3], tst[5]);
     atomicMax(&misc[kPpStatLeaves + 4], tst[6]);
   }
+}
+
+// One wavefront per listed tile of one parity phase (a launch per phase: the launch
+// boundary makes one phase's values visible to the next).
+template <int ND>
+__global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
+                                                    uint32_t ty, uint32_t tz, int phase,
+                                                    const uint32_t* __restrict__ pbits, const float* __restrict__ d,
+                                                    const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
+                                                    int anchor, uint32_t* misc) {
+  __shared__ SolveShared<ND> sh;
+  solve_tile<ND, false>(sh, g, tiles[(size_t)phase * cap + blockIdx.x], ty, tz, pbits, d, t, y, eb, anchor, misc);
+}
+
+// Every phase in one launch, as a dataflow: tiles are handed out phase by phase from a
+// counter, and a tile starts once each neighbouring listed tile of a lower phase has
+// finished (a done flag per tile, set after the tile's field stores have completed).
+// The field is read and written with agent-coherent accesses, so no XCD-wide cache
+// write-back or invalidation is needed around the flags.  Tiles two or more apart never
+// interact, and every pair of neighbouring tiles runs in phase order, so the values
+// equal those of one launch per phase; the time is the longest chain of dependent tiles
+// instead of the sum of the phases' slowest tiles.  A tile only waits for tiles handed
+// out before it, so the waits end; a bounded spin reports an error instead of hanging.
+template <int ND>
+__global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
+                                                         uint32_t tx, uint32_t ty, uint32_t tz,
+                                                         const uint32_t* __restrict__ tbits, uint32_t* done,
+                                                         const uint32_t* __restrict__ pbits,
+                                                         const float* __restrict__ d, const uint8_t* __restrict__ t,
+                                                         float* __restrict__ y, float eb, int anchor, uint32_t* misc) {
+  __shared__ SolveShared<ND> sh;
+  const int lane = threadIdx.x;
+  if (lane == 0) sh.tile = atomicAdd(&misc[kPpFlowCounter], 1u);
+  __syncthreads();
+  // phase and position of the handed-out index
+  uint32_t i = sh.tile, p = 0;
+  for (; p < (1u << ND); p++) {
+    const uint32_t c = min(misc[kPpTilesP + p], cap);
+    if (i < c) break;
+    i -= c;
+  }
+  if (p == (1u << ND)) return;  // more workgroups than listed tiles
+  const uint32_t id = tiles[(size_t)p * cap + i];
+  int tc[ND];
+  tile_coords<ND>(id, ty, tz, tc);
+  // lane k waits for neighbour k of the 3^ND block (k != centre), if listed and of a
+  // lower phase
+  constexpr int kNb = ND == 2 ? 9 : 27;
+  if (lane < kNb && lane != kNb / 2) {
+    int nc[ND];
+    int k = lane;
+    bool in = true;
+#pragma unroll
+    for (int a = ND - 1; a >= 0; a--) {
+      nc[a] = tc[a] + (k % 3) - 1;
+      k /= 3;
+      const uint32_t lim = a == 0 ? tx : (a == 1 ? ty : tz);
+      in = in && nc[a] >= 0 && (uint32_t)nc[a] < lim;
+    }
+    if (in && (uint32_t)tile_phase<ND>(nc) < p) {
+      uint32_t nid = (uint32_t)nc[0];
+#pragma unroll
+      for (int a = 1; a < ND; a++) nid = nid * (a == 1 ? ty : tz) + (uint32_t)nc[a];
+      if ((tbits[nid >> 5] >> (nid & 31u)) & 1u) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&done[nid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+          if (++spins > kSpinLimit) {
+            atomicOr(&misc[kPpErr], TSZP_ERR_LOOKBACK);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  solve_tile<ND, true>(sh, g, id, ty, tz, pbits, d, t, y, eb, anchor, misc);
+  // every coherent store of the tile has completed before the flag is set
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (lane == 0) __hip_atomic_store(&done[id], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1957,6 +2085,19 @@ hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t 
   if (g.ndim == 2)
     hipLaunchKernelGGL(k_pp_tiles<2>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
   else hipLaunchKernelGGL(k_pp_tiles<3>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
+  return hipGetLastError();
+}
+
+hipError_t pp_solve_flow(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
+                         uint32_t tz, const uint32_t* tbits, uint32_t* done, const uint32_t* pbits, const float* d,
+                         const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st) {
+  if (!total) return hipSuccess;
+  const dim3 grid(total), blk(kWave);
+  if (g.ndim == 2)
+    hipLaunchKernelGGL(k_pp_solve_flow<2>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t, y,
+                       eb, anchor, misc);
+  else hipLaunchKernelGGL(k_pp_solve_flow<3>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t,
+                          y, eb, anchor, misc);
   return hipGetLastError();
 }
 
