@@ -159,3 +159,19 @@ def test_solve_one_launch_equals_phases(make, eb, monkeypatch):
     _, y_ph, _, info_ph = _pipeline(data, eb)
     assert np.array_equal(y_flow.view(np.uint32), y_ph.view(np.uint32))
     assert info_flow.windows == info_ph.windows and info_flow.missing == info_ph.missing == 0
+
+
+@pytest.mark.parametrize("make,eb", [(lambda: fields.fbm3d(40, 44, 48, seed=74), 3e-3),
+                                     (lambda: fields.blobs2d(300, 310, seed=75, noise=0.05), 1e-3),
+                                     (lambda: fields.fbm3d(64, 64, 64, seed=76, noise=0.2), 1e-2)],
+                         ids=["fbm3d", "blobs2d", "fbm3d_noisy"])
+def test_move_segments_equal_serial(make, eb, monkeypatch):
+    """The local moves with sixteen lanes per point give the field of one lane per point,
+    bit for bit (same candidates, the serial loop's choice among equal scores)."""
+    data = make()
+    print(f"move segments vs serial {data.shape} eb={eb}", flush=True)
+    _, y_seg, _, info_seg = _pipeline(data, eb)
+    monkeypatch.setenv("TSZP_MOVE_SERIAL", "1")
+    _, y_ser, _, info_ser = _pipeline(data, eb)
+    assert np.array_equal(y_seg.view(np.uint32), y_ser.view(np.uint32))
+    assert info_seg.moves == info_ser.moves and info_seg.missing == info_ser.missing == 0

@@ -528,14 +528,15 @@ __device__ __forceinline__ void lds_or_run16(uint32_t* words, unsigned a, uint32
   if (o4) atomicOr(&words[k + 4], o4);
 }
 
-// critical-point type from the centre and the two neighbours along each axis
+// critical-point type from the centre and the two neighbours along each axis (plain
+// compares: fmaxf / fminf would add NaN canonicalisations)
 template <int ND>
 __device__ __forceinline__ int type_minmax(float c, const float (&lo)[ND], const float (&hi)[ND]) {
   bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
 #pragma unroll
   for (int k = 0; k < ND; k++) {
-    const bool h = c > fmaxf(lo[k], hi[k]);
-    const bool l = c < fminf(lo[k], hi[k]);
+    const bool h = c > lo[k] && c > hi[k];
+    const bool l = c < lo[k] && c < hi[k];
     all_gt = all_gt && h;
     all_lt = all_lt && l;
     any_hi = any_hi || h;
@@ -646,7 +647,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
         const float zh[4] = {c[1], c[2], c[3], __uint_as_float(zp)};
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          if (z + r < 1 || z + r + 1 >= dz) continue;
+          // every element is classified; the ends of a row are masked afterwards
           float lo[ND], hi[ND];
 #pragma unroll
           for (int k = 0; k < ND - 1; k++) {
@@ -657,7 +658,9 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
           }
           lo[ND - 1] = zl[r];
           hi[ND - 1] = zh[r];
-          tbits |= (uint32_t)type_minmax<ND>(c[r], lo, hi) << (2 * (s * 4 + r));
+          const bool zin = z + r >= 1 && z + r + 1 < dz;
+          const uint32_t ty = zin ? (uint32_t)type_minmax<ND>(c[r], lo, hi) : 0u;
+          tbits |= ty << (2 * (s * 4 + r));
         }
       }
     }

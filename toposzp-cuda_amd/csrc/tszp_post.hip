@@ -32,6 +32,8 @@
 //     does not depend on scheduling.
 // The full marking pass also yields the verification counts (missing / spurious /
 // changed critical points) and the largest |y - d|.
+#include <cstdlib>
+
 #include "tszp_device.hpp"
 #include "tszp_kernels.h"
 
@@ -921,6 +923,140 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_
     moved = move_point<ND>(g, p, c, d, t, y, eb);
   }
   // one atomic per workgroup (same-address atomics serialise)
+  const uint32_t nm = (uint32_t)__popcll(__ballot(moved));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = red[0] + red[1] + red[2] + red[3];
+    if (tot) atomicAdd(&misc[kPpMoves], tot);
+  }
+}
+
+// The same move with the candidates spread over a 16-lane segment: every lane of the
+// segment loads the point's neighbourhood (the same addresses, one request per wave
+// instruction) and sorts the candidate values; lane j scores candidate j of the serial
+// order above (even j: value j/2 of the sorted list, odd j: the midpoint after it), and
+// the lowest score wins, the earliest candidate among equal scores -- the serial
+// loop's choice.  One point's candidates cost one candidate's latency instead of
+// sixteen.
+template <int ND>
+__device__ __forceinline__ bool move_point_seg(const Grid& g, uint32_t p, const int (&c)[ND], const float* __restrict__ d,
+                                               const uint8_t* __restrict__ t, float* __restrict__ y, float eb, int j,
+                                               bool act) {
+  constexpr int K = 2 * ND, M = K + 2;
+  bool intp = false;
+  int tp = 0;
+  float yq[K], nn[K][K];
+  int tq[K];
+  bool inq[K], intq[K];
+  float cur = 0.f;
+  if (act) {
+    intp = interior<ND>(g, c);
+    tp = t[p];
+    cur = y[p];
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int a = k >> 1;
+    const uint32_t s = stride_of<ND>(g, a);
+    int q[ND];
+#pragma unroll
+    for (int b = 0; b < ND; b++) q[b] = c[b];
+    q[a] += (k & 1) ? 1 : -1;
+    inq[k] = act && inside<ND>(g, q);
+    intq[k] = inq[k] && interior<ND>(g, q);
+    const uint32_t qe = (k & 1) ? p + s : p - s;
+    yq[k] = inq[k] ? y[qe] : 0.f;
+    tq[k] = inq[k] ? (int)t[qe] : 0;
+#pragma unroll
+    for (int jj = 0; jj < K; jj++) {
+      nn[k][jj] = 0.f;
+      if (intq[k] && jj != (k ^ 1)) {
+        const uint32_t sj = stride_of<ND>(g, jj >> 1);
+        nn[k][jj] = y[(jj & 1) ? qe + sj : qe - sj];
+      }
+    }
+  }
+  auto nbad = [&](float v) -> int {
+    int b = intp ? (classify_vals<ND>(v, yq) != tp) : 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (intq[k]) {
+        float w[K];
+#pragma unroll
+        for (int jj = 0; jj < K; jj++) w[jj] = (jj == (k ^ 1)) ? v : nn[k][jj];
+        b += classify_vals<ND>(yq[k], w) != tq[k];
+      }
+    }
+    return b;
+  };
+  const int v0 = act ? nbad(cur) : 0;
+  float lo = 0.f, hi = 0.f;
+  if (act) cell_of(d[p], eb, &lo, &hi);
+  float vals[M];
+  vals[0] = lo;
+  vals[1] = hi;
+#pragma unroll
+  for (int k = 0; k < K; k++) vals[k + 2] = (inq[k] && yq[k] >= lo && yq[k] <= hi) ? yq[k] : INFINITY;
+  sort_small<M>(vals);
+  // candidate j of the serial order
+  const int a = j >> 1;
+  float va = INFINITY, vp = INFINITY, vb = INFINITY;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    va = k == a ? vals[k] : va;
+    vp = k + 1 == a ? vals[k] : vp;
+    vb = k == a + 1 ? vals[k] : vb;
+  }
+  bool ok = act && v0 != 0 && a < M && va <= hi;
+  float cand = va;
+  if (ok) {
+    if ((j & 1) == 0) {
+      ok = a == 0 || va != vp;
+    } else {
+      const float mv = 0.5f * (va + vb);
+      ok = a + 1 < M && vb <= hi && vb > va && mv > va && mv < vb;
+      cand = mv;
+    }
+  }
+  const int sc = ok ? nbad(cand) : 0x7fff;
+  // lowest (score, candidate index) over the segment
+  int key = (sc << 4) | j;
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    const int other = __shfl_xor(key, o, kWave);
+    key = other < key ? other : key;
+  }
+  const int best = key >> 4, bj = key & 15;
+  const float bestv = __shfl(cand, (threadIdx.x & (kWave - 1) & ~15) + bj, kWave);
+  if (act && v0 != 0 && best < v0) {
+    if (j == 0) y[p] = bestv;
+    return j == 0;
+  }
+  return false;
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_move_seg(Grid g, const uint32_t* __restrict__ act,
+                                                              uint32_t cap, int colour, const float* __restrict__ d,
+                                                              const uint8_t* __restrict__ t, float* __restrict__ y,
+                                                              float eb, uint32_t* misc) {
+  __shared__ uint32_t red[kWavesPerBlock];
+  const uint32_t gi = blockIdx.x * kBlockThreads + threadIdx.x;
+  const uint32_t i = gi >> 4;
+  const int j = (int)(gi & 15u);
+  uint32_t cnt = misc[kPpActC + colour];
+  cnt = cnt < cap ? cnt : cap;
+  const bool valid = i < cnt;
+  uint32_t p = 0;
+  int c[ND];
+#pragma unroll
+  for (int b = 0; b < ND; b++) c[b] = 0;
+  if (valid) {
+    p = act[(size_t)colour * cap + i];
+    coords_of<ND>(g, p, c);
+  }
+  const bool moved = move_point_seg<ND>(g, p, c, d, t, y, eb, j, valid);
   const uint32_t nm = (uint32_t)__popcll(__ballot(moved));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nm;
   __syncthreads();
@@ -2008,9 +2144,16 @@ hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_
                    const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st) {
   const uint32_t m = max_act < cap ? max_act : cap;
   if (!m) return hipSuccess;
-  const dim3 grid(blocks_for(m)), blk(kBlockThreads);
-  if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
-  else hipLaunchKernelGGL(k_pp_move<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+  const dim3 blk(kBlockThreads);
+  if (getenv("TSZP_MOVE_SERIAL")) {  // one lane per point
+    const dim3 grid(blocks_for(m));
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+    else hipLaunchKernelGGL(k_pp_move<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+  } else {  // sixteen lanes per point
+    const dim3 grid(blocks_for((uint64_t)m * 16));
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move_seg<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+    else hipLaunchKernelGGL(k_pp_move_seg<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+  }
   return hipGetLastError();
 }
 
