@@ -1077,30 +1077,34 @@ __constant__ const int8_t kRing3D[25][3] = {
     {-1, 1, 0}, {1, -1, 0},  {1, 1, 0},  {-1, 0, -1}, {-1, 0, 1},  {1, 0, -1},  {1, 0, 1},
     {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
 
+constexpr int kRemarkChunks = 8;
+
 template <int ND>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint32_t* __restrict__ old,
                                                             uint32_t nold, const float* __restrict__ y,
                                                             const uint8_t* __restrict__ t, uint32_t* bits,
                                                             uint32_t* __restrict__ bad, uint32_t cap, int rings,
                                                             uint32_t* misc) {
-  // rings: 1 re-checks the listed points only, otherwise everything within two steps
+  // rings: 1 re-checks the listed points only, otherwise everything within two steps.
+  // One thread per (listed point, ring offset) and kRemarkChunks of them per thread: the
+  // claims and stencil loads of a point's ring are in flight together instead of one
+  // after the other, and a workgroup appends to the global list once.
   constexpr int RM = ND == 2 ? 13 : 25;
-  const int R = rings == 1 ? 1 : RM;
-  __shared__ uint32_t lbuf[kBlockThreads * RM];
+  const uint32_t R = rings == 1 ? 1u : (uint32_t)RM;
+  __shared__ uint32_t lbuf[kBlockThreads * kRemarkChunks];
   __shared__ uint32_t lcnt, lbase;
   if (threadIdx.x == 0) lcnt = 0;
   __syncthreads();
-  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  const bool valid = i < nold;
-  const uint32_t e = valid ? old[i] : 0u;
-  int c[ND];
-  coords_of<ND>(g, e, c);
-#pragma unroll 1
-  for (int r = 0; r < R; r++) {
+#pragma unroll
+  for (int k = 0; k < kRemarkChunks; k++) {
+    const uint64_t gi = ((uint64_t)blockIdx.x * kRemarkChunks + k) * kBlockThreads + threadIdx.x;
+    const uint32_t i = (uint32_t)(gi / R), r = (uint32_t)(gi % R);
     bool isbad = false;
     uint32_t p = 0;
-    if (valid) {
-      int q[ND];
+    if (i < nold) {
+      const uint32_t e = old[i];
+      int c[ND], q[ND];
+      coords_of<ND>(g, e, c);
       int64_t off = 0;
 #pragma unroll
       for (int a = 0; a < ND; a++) {
@@ -2160,7 +2164,9 @@ hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_
 hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st) {
   if (!nold) return hipSuccess;
-  const dim3 grid(blocks_for(nold)), blk(kBlockThreads);
+  const uint64_t R = rings == 1 ? 1 : (g.ndim == 2 ? 13 : 25);
+  const uint64_t items = (uint64_t)nold * R, per = (uint64_t)kBlockThreads * kRemarkChunks;
+  const dim3 grid((unsigned)((items + per - 1) / per)), blk(kBlockThreads);
   if (g.ndim == 2)
     hipLaunchKernelGGL(k_pp_remark<2>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
   else hipLaunchKernelGGL(k_pp_remark<3>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
