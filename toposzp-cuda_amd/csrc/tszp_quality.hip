@@ -112,8 +112,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_quality(Grid g, const float* 
       cnt[kQPreserved] += td == to;
       cnt[kQLost] += td == 0;
       cnt[kQChanged] += td != 0 && td != to;
-      cnt[kQLost + to] += td == 0;
-      cnt[kQChanged + to] += td != 0 && td != to;
+      // per type, with constant indices (a runtime index would put the counters in
+      // scratch memory)
+#pragma unroll
+      for (int k = 1; k <= 3; k++) {
+        cnt[kQLost + k] += to == k && td == 0;
+        cnt[kQChanged + k] += to == k && td != 0 && td != to;
+      }
     } else {
       cnt[kQNew] += td != 0;
     }
@@ -215,21 +220,41 @@ __global__ __launch_bounds__(kBlockThreads) void k_quality(Grid g, const float* 
   }
 }
 
+// one workgroup per result word: counts summed, error maximum, persistence min / max / sum
 __global__ __launch_bounds__(kBlockThreads) void k_quality_reduce(const uint64_t* __restrict__ part, uint32_t nb,
                                                                  uint64_t* __restrict__ out) {
-  const int k = threadIdx.x;
+  __shared__ uint64_t red[kBlockThreads];
+  __shared__ double dred[kBlockThreads];
+  const int k = blockIdx.x, tid = threadIdx.x;
   if (k < kQualCounts) {
     uint64_t s = 0;
-    for (uint32_t b = 0; b < nb; b++) s += part[(size_t)b * kQualWords + k];
-    out[k] = s;
-  } else if (k < kQualCounts + 4) {
+    for (uint32_t b = tid; b < nb; b += kBlockThreads) s += part[(size_t)b * kQualWords + k];
+    red[tid] = s;
+  } else {
     const int j = k - kQualCounts;
     double v = j == 1 ? 1e300 : 0.0;
-    for (uint32_t b = 0; b < nb; b++) {
+    for (uint32_t b = tid; b < nb; b += kBlockThreads) {
       const double u = reinterpret_cast<const double*>(part + (size_t)b * kQualWords + kQualCounts)[j];
       v = j == 0 || j == 2 ? fmax(v, u) : (j == 1 ? fmin(v, u) : v + u);
     }
-    reinterpret_cast<double*>(out + kQualCounts)[j] = v;
+    dred[tid] = v;
+  }
+  __syncthreads();
+  for (int o = kBlockThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      if (k < kQualCounts) {
+        red[tid] += red[tid + o];
+      } else {
+        const int j = k - kQualCounts;
+        const double a = dred[tid], b = dred[tid + o];
+        dred[tid] = j == 0 || j == 2 ? fmax(a, b) : (j == 1 ? fmin(a, b) : a + b);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (k < kQualCounts) out[k] = red[0];
+    else reinterpret_cast<double*>(out + kQualCounts)[k - kQualCounts] = dred[0];
   }
 }
 
@@ -241,7 +266,7 @@ hipError_t launch_quality(const Grid& g, const float* x, const float* y, uint32_
     hipLaunchKernelGGL(k_quality<3>, dim3(nblocks), dim3(kBlockThreads), 0, st, g, x, y, n, eb, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_quality_reduce, dim3(1), dim3(kBlockThreads), 0, st, part, nblocks, out);
+  hipLaunchKernelGGL(k_quality_reduce, dim3(kQualCounts + 4), dim3(kBlockThreads), 0, st, part, nblocks, out);
   return hipGetLastError();
 }
 
