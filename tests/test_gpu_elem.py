@@ -305,3 +305,29 @@ def test_dropin_predict_quantization():
         fn = getattr(L, f"szp_cuda_float_{name}_predict_quantization")
         p = fn(x.ctypes.data_as(C.c_void_p), None, float(eb), x.size, 64)
         assert np.array_equal(_take(p, 4 * x.size, np.int32), want)
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 4, 5), (5, 3, 4)])
+def test_topo64_smallest_fields(shape):
+    """Fields with a single interior point or plane: the saddle list, the passes and the
+    codec at their smallest sizes."""
+    torch = _torch()
+    rng = np.random.default_rng(sum(shape))
+    data = rng.normal(0, 1, shape).astype(np.float32)
+    eb = 0.5
+    rec, types = _cp_records(data, eb)
+    _check_topo64(torch, data.astype(np.float64), eb, 7, rec, types)
+
+
+def test_dropin_rejects_corrupt_threadblock_stream():
+    """A bit count above 32 in the first block: no chunk count walks, NULL and an error."""
+    _torch()
+    L = _dropin()
+    x = (np.arange(1000) * 0.01).astype(np.float32)
+    s = oracle.elem_compress(x, 1e-3, 64, T=1)
+    bad = s.copy()
+    bad[4 + 8 + 4] = 200  # the first bc byte after the anchor
+    buf = np.ascontiguousarray(bad[4:])
+    p = L.szp_cuda_float_decompress_threadblock(x.size, 1e-3, 64, buf.ctypes.data_as(C.c_void_p))
+    assert not p
+    assert "corrupt" in tz.last_error()
