@@ -5,11 +5,12 @@
 // (SZp/src/szp_cuda_decompress.cu:391-477 does it on the host).  Here it runs in
 // parallel as a composition of per-segment transition tables:
 //   1. the payload is cut into segments of kParseSeg bytes.  For every possible entry
-//      offset e into a segment (0 <= e < largest block size) a workgroup finds, by
-//      pointer doubling over all byte positions of the segment in LDS, where the walk
-//      that starts at e leaves the segment and how many blocks it passes
-//      (k_parse_seg; blocks are taken as full blocks here -- only the last block of the
-//      field is shorter, and nothing after it matters);
+//      offset e into a segment (0 <= e < largest block size) a workgroup finds where
+//      the walk that starts at e leaves the segment and how many blocks it passes, by
+//      one walk per entry over the segment's bytes in LDS (k_parse_seg_walk) or, when
+//      those walks would be long, by pointer doubling over all byte positions
+//      (k_parse_seg); blocks are taken as full blocks here -- only the last block of
+//      the field is shorter, and nothing after it matters;
 //   2. tables of consecutive segments are composed in groups (k_parse_compose), twice,
 //      so that a single thread can walk the top level;
 //   3. the entry offset and the first block index of every segment come back down the
@@ -63,6 +64,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_seg(ParseArgs a, uint2*
     const uint32_t j = J[cur][e];
     // j < kParseSeg cannot happen after enough rounds; report it as a corrupt table
     tab[(size_t)blockIdx.x * a.maxb + e] = make_uint2(j >= kParseSeg ? j - kParseSeg : 0xffffffffu, Cn[cur][e]);
+  }
+}
+
+// The same tables by walking from each entry offset (one thread per entry, the bytes in
+// LDS): maxb walks of at most kParseSeg / (smallest block) steps.  With blocks of tens of
+// bytes that is far less work than doubling over every byte position, and the small LDS
+// footprint lets many segments share a CU; the launcher picks it when the walks are
+// short enough.
+__global__ __launch_bounds__(kBlockThreads) void k_parse_seg_walk(ParseArgs a, uint2* __restrict__ tab) {
+  __shared__ uint8_t bytes[kParseSeg + kParseMaxBlock];
+  __shared__ uint16_t sz[34];
+  const uint64_t s0 = (uint64_t)blockIdx.x * kParseSeg;
+  for (uint32_t i = threadIdx.x; i < kParseSeg + a.maxb; i += kBlockThreads)
+    bytes[i] = s0 + i < a.len ? a.p[s0 + i] : (uint8_t)0;
+  if (threadIdx.x < 34) sz[threadIdx.x] = a.sz[threadIdx.x];
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < a.maxb; e += kBlockThreads) {
+    uint32_t q = e, c = 0;
+    while (q < kParseSeg) {  // every block is at least 4 bytes: at most kParseSeg / 4 steps
+      const uint32_t bc = a.has_bc ? bytes[q + 4] : 0u;
+      q += sz[bc <= 32 ? bc : 33];
+      c++;
+    }
+    tab[(size_t)blockIdx.x * a.maxb + e] = make_uint2(q - kParseSeg, c);
   }
 }
 
@@ -186,7 +211,12 @@ hipError_t parse_blocks(const ParseArgs& a, uint64_t n, uint32_t bs, uint32_t nb
   uint2* e2 = t2 + (size_t)n2 * a.maxb;  // entries of the level-2 tables
   uint2* e1 = e2 + n2;                   // entries of the level-1 tables
   uint2* e0 = e1 + n1;                   // entries of the segments
-  hipLaunchKernelGGL(k_parse_seg, dim3(nseg), dim3(kBlockThreads), 0, st, a, t0);
+  // walks when they stay short (steps per lane), else pointer doubling
+  uint32_t minb = 0xffffu;
+  for (int b = 0; b < 34; b++) minb = a.sz[b] < minb ? a.sz[b] : minb;
+  const uint64_t walk_steps = (uint64_t)a.maxb * (kParseSeg / (minb ? minb : 1u)) / kBlockThreads;
+  if (walk_steps <= 16384) hipLaunchKernelGGL(k_parse_seg_walk, dim3(nseg), dim3(kBlockThreads), 0, st, a, t0);
+  else hipLaunchKernelGGL(k_parse_seg, dim3(nseg), dim3(kBlockThreads), 0, st, a, t0);
   hipLaunchKernelGGL(k_parse_compose, dim3(n1), dim3(kBlockThreads), 0, st, t0, nseg, kParseGroup, a.maxb, t1);
   hipLaunchKernelGGL(k_parse_compose, dim3(n2), dim3(kBlockThreads), 0, st, t1, n1, kParseGroup, a.maxb, t2);
   // top: one thread walks the level-2 tables from the payload start
