@@ -275,7 +275,7 @@ int run_codec(Ctx* c, int mode, const void* d_in, uint32_t n, int bs, float eb, 
   a.blk = d_blk;
   a.st_bytes = stb;
   a.st_cnt = stc;
-  a.tile_counter = counter;
+  a.tile_counter = dyn_counter(counter);
   a.cp_key = cp_key;
   a.cp_pay = cp_pay;
   a.cp_cap = cp_cap;
@@ -767,7 +767,7 @@ int elem_compress_impl(Ctx* c, const void* d_in, int dsize, size_t n, double eb,
   a.cap = pc > 0xffffffffull ? 0xffffffffull : pc;
   a.blk = d_blk;
   a.st_bytes = stb;
-  a.tile_counter = counter;
+  a.tile_counter = dyn_counter(counter);
   a.res = res;
   ProfScope prof(kProfElem, st);
   HIPCHK(launch_elem_compress(dsize, ra, a, st));
@@ -812,7 +812,7 @@ int elem_decompress_impl(Ctx* c, const uint8_t* d_payload, const uint32_t* d_blk
   a.eb = dsize == 4 ? (double)(float)eb : eb;
   a.out = d_out;
   a.st = stb;
-  a.tile_counter = counter;
+  a.tile_counter = dyn_counter(counter);
   a.err = &res->error;
   ProfScope prof(kProfElemDecode, st);
   HIPCHK(launch_elem_decode(dsize, ra, a, st));
@@ -1125,7 +1125,7 @@ int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t*
   a.inv = 1.0;
   a.ntiles = nt;
   a.st_cnt = stb;
-  a.tile_counter = counter;
+  a.tile_counter = dyn_counter(counter);
   a.types = d_types;
   a.res = res;
   HIPCHK(launch_classify(a, st));
@@ -1606,7 +1606,7 @@ int find_cps_impl(const float* data, int ndim, const uint32_t* dims, float eb, R
   a.inv = 1.0 / (double)eb;
   a.ntiles = nt;
   a.st_cnt = stb;
-  a.tile_counter = counter;
+  a.tile_counter = dyn_counter(counter);
   a.cps = dc;
   a.rec = rec;
   a.cp_cap = (uint32_t)interior;

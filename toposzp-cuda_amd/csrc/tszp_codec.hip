@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   const int g = lane / S, ls = lane & (S - 1);
   const uint32_t bs = a.bs;
 
-  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  if (tid == 0) sh_tile = tile_id(a.tile_counter);
   for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
@@ -558,6 +558,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   __shared__ uint32_t blk_off[BPT + 1];
   __shared__ uint32_t step_base[kWavesPerBlock * SPW];  // cp | ext << 16 (tile-local prefix)
   __shared__ uint64_t sh_prefix[2];
+  __shared__ uint64_t sh_cnt_agg;
   __shared__ uint32_t sh_tile;
 
   const int tid = threadIdx.x;
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   const int g = lane / S, ls = lane & (S - 1);
   const uint32_t j0 = 4u * (uint32_t)ls;  // first element of the lane within its block
 
-  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  if (tid == 0) sh_tile = tile_id(a.tile_counter);
   for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   }
   __syncthreads();
 
-  // ---- B: tile scans + look-back (wave 0) ---------------------------------------------
+  // ---- B: tile scans (wave 0); the look-back follows, overlapped with the packing -----
   if (w == 0) {
     constexpr int PER = (BPT + kWave - 1) / kWave;
     uint32_t loc[PER];
@@ -731,15 +732,22 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
     }
     const uint32_t tile_bytes = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
     if (lane == 0) blk_off[BPT] = tile_bytes;
-    uint64_t cnt_agg = 0;
     if constexpr (CLASSIFY) {
       constexpr int NS = kWavesPerBlock * SPW;
       const uint32_t v = lane < NS ? step_base[lane] : 0u;
       const uint32_t inc = wave_incl_scan_dpp(v, lane);
       if (lane < NS) step_base[lane] = inc - v;
       const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
-      cnt_agg = (uint64_t)(tot & 0xffffu) | ((uint64_t)(tot >> 16) << 31);
+      if (lane == 0) sh_cnt_agg = (uint64_t)(tot & 0xffffu) | ((uint64_t)(tot >> 16) << 31);
     }
+  }
+  __syncthreads();
+  // wave 0 finds the tile's stream and CP offsets while the other waves already pack
+  // (the packing uses tile-local offsets only)
+  if (w == 0) {
+    const uint32_t tile_bytes = blk_off[BPT];
+    uint64_t cnt_agg = 0;
+    if constexpr (CLASSIFY) cnt_agg = sh_cnt_agg;
     const uint64_t pb = lookback(a.st_bytes, tile, tile_bytes, &a.res->error);
     uint64_t pc = 0;
     if constexpr (CLASSIFY) pc = lookback(a.st_cnt, tile, cnt_agg, &a.res->error);
@@ -756,7 +764,6 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
       }
     }
   }
-  __syncthreads();
 
   // ---- C: pack the lane's runs into the LDS stage ---------------------------------------
 #pragma unroll

@@ -225,6 +225,16 @@ __device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
   return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Tile of a workgroup in a single-pass look-back kernel.  Workgroup ids are dealt
+// round-robin to the XCDs and every XCD dispatches its share in increasing order, so
+// when a tile runs, the tile before it has been dispatched on its own XCD and the
+// look-back never waits for a workgroup that cannot start.  A dynamic counter (one
+// same-address atomic per workgroup: serialised at ~12 ns each on this part, which
+// caps a 32 K-tile launch near 0.4 ms) is used when the host passes one.
+__device__ __forceinline__ uint32_t tile_id(uint32_t* counter) {
+  return counter ? atomicAdd(counter, 1u) : blockIdx.x;
+}
+
 // Called by a full wave of tile `tile`.  Publishes `agg`, walks predecessors 64
 // at a time, publishes the inclusive prefix and returns the exclusive prefix
 // (wave-uniform).  Values add component-wise, so two 31-bit counters packed

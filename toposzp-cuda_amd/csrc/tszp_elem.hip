@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_elem_compress(ElemArgs a) {
   const int g = lane / S, ls = lane & (S - 1);
   const TF* in = reinterpret_cast<const TF*>(a.in);
 
-  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  if (tid == 0) sh_tile = tile_id(a.tile_counter);
   for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_elem_decode(ElemDecodeArgs a)
   const ElemArgs& ge = a.geo;
 
   if (tid == 0) {
-    sh_tile = atomicAdd(a.tile_counter, 1u);
+    sh_tile = tile_id(a.tile_counter);
     sh_bad = 0;
   }
   __syncthreads();

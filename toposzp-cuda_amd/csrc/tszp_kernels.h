@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <stdint.h>
 
 #include "tszp_types.h"
@@ -50,6 +51,10 @@ struct DecodeArgs {
 void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* blocks_per_tile);
 hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st);
 hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st);
+
+// Tile counter to pass to a look-back kernel: null (tiles = workgroup ids) unless
+// TSZP_TILE_COUNTER is set, which hands tiles out from the counter instead.
+inline uint32_t* dyn_counter(uint32_t* counter) { return getenv("TSZP_TILE_COUNTER") ? counter : nullptr; }
 
 // ---- element-partitioned streams (tszp_elem.hip): threadblock codec, double RA ----
 struct ElemArgs {

@@ -1998,25 +1998,28 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
   solve_tile<ND, false>(sh, g, tiles[(size_t)phase * cap + blockIdx.x], ty, tz, pbits, d, t, y, eb, anchor, misc);
 }
 
-// Every phase in one launch, as a dataflow: tiles are handed out phase by phase from a
-// counter, and a tile starts once each neighbouring listed tile of a lower phase has
-// finished (a done flag per tile, set after the tile's field stores have completed).
+// Every phase in one launch, as a dataflow: workgroup i takes the i-th listed tile in
+// phase order (workgroups start in id order), and a tile starts once each neighbouring
+// listed tile of a lower phase has finished (a done flag per tile, set after the
+// tile's field stores have completed).
 // The field is read and written with agent-coherent accesses, so no XCD-wide cache
 // write-back or invalidation is needed around the flags.  Tiles two or more apart never
 // interact, and every pair of neighbouring tiles runs in phase order, so the values
 // equal those of one launch per phase; the time is the longest chain of dependent tiles
-// instead of the sum of the phases' slowest tiles.  A tile only waits for tiles handed
-// out before it, so the waits end; a bounded spin reports an error instead of hanging.
+// instead of the sum of the phases' slowest tiles.  A tile only waits for tiles of lower
+// ids, which have started before it, so the waits end; a bounded spin reports an error
+// instead of hanging.
 template <int ND>
 __global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
                                                          uint32_t tx, uint32_t ty, uint32_t tz,
                                                          const uint32_t* __restrict__ tbits, uint32_t* done,
                                                          const uint32_t* __restrict__ pbits,
                                                          const float* __restrict__ d, const uint8_t* __restrict__ t,
-                                                         float* __restrict__ y, float eb, int anchor, uint32_t* misc) {
+                                                         float* __restrict__ y, float eb, int anchor, uint32_t* misc,
+                                                         uint32_t* counter) {
   __shared__ SolveShared<ND> sh;
   const int lane = threadIdx.x;
-  if (lane == 0) sh.tile = atomicAdd(&misc[kPpFlowCounter], 1u);
+  if (lane == 0) sh.tile = tile_id(counter);
   __syncthreads();
   // phase and position of the handed-out index
   uint32_t i = sh.tile, p = 0;
@@ -2244,9 +2247,9 @@ hipError_t pp_solve_flow(const Grid& g, const uint32_t* tiles, uint32_t total, u
   const dim3 grid(total), blk(kWave);
   if (g.ndim == 2)
     hipLaunchKernelGGL(k_pp_solve_flow<2>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t, y,
-                       eb, anchor, misc);
+                       eb, anchor, misc, dyn_counter(misc + kPpFlowCounter));
   else hipLaunchKernelGGL(k_pp_solve_flow<3>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t,
-                          y, eb, anchor, misc);
+                          y, eb, anchor, misc, dyn_counter(misc + kPpFlowCounter));
   return hipGetLastError();
 }
 

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_scan_excl(uint32_t* a, uint32
   __shared__ uint32_t sh_tile;
   __shared__ uint64_t sh_pre;
   const int tid = threadIdx.x;
-  if (tid == 0) sh_tile = atomicAdd(counter, 1u);
+  if (tid == 0) sh_tile = tile_id(counter);
   __syncthreads();
   const uint32_t tile = sh_tile;
   const uint32_t base = tile * kSortTile + tid * kItemsPerThread;
@@ -165,7 +165,7 @@ hipError_t launch_scan_excl(uint32_t* a, uint32_t count, uint64_t* status, uint3
                             hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_scan_excl, dim3(ceil_div(count, kSortTile)), dim3(kBlockThreads), 0, st, a, count, status,
-                     counter, err);
+                     dyn_counter(counter), err);
   return hipGetLastError();
 }
 
@@ -184,7 +184,7 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uin
       const uint32_t* dw = wd ? h0 : k0;
       hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlockThreads), 0, st, dw, count, shift, wk.hist, ntiles);
       hipLaunchKernelGGL(k_scan_excl, dim3(stiles), dim3(kBlockThreads), 0, st, wk.hist, hwords,
-                         wk.st + (size_t)pass * stiles, wk.counters + pass, wk.err);
+                         wk.st + (size_t)pass * stiles, dyn_counter(wk.counters + pass), wk.err);
       if (scatter_timer) scatter_timer->begin(st);
       if (h0)
         hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1, count,
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rank(const uint32_t* __restri
   __shared__ uint32_t sh_tile;
   __shared__ uint64_t sh_pre;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
-  if (tid == 0) sh_tile = atomicAdd(counter, 1u);
+  if (tid == 0) sh_tile = tile_id(counter);
   __syncthreads();
   const uint32_t tile = sh_tile;
   const uint32_t base = tile * kSortTile + w * (kWave * kItemsPerThread);
@@ -296,7 +296,7 @@ hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint3
   const uint32_t ntiles = ceil_div(count, kSortTile);
   const uint32_t stiles = ceil_div((uint64_t)256 * ntiles, kSortTile);
   hipLaunchKernelGGL(k_rank, dim3(ntiles), dim3(kBlockThreads), 0, st, keys, key_hi, pay, count, eb, inv, out,
-                     out_cap, wk.st + (size_t)8 * stiles, wk.counters + 8, wk.err);
+                     out_cap, wk.st + (size_t)8 * stiles, dyn_counter(wk.counters + 8), wk.err);
   return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_classify(ClassifyArgs a) {
   __shared__ uint32_t sh_tile;
   __shared__ uint64_t sh_pre;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
-  if (tid == 0) sh_tile = atomicAdd(a.tile_counter, 1u);
+  if (tid == 0) sh_tile = tile_id(a.tile_counter);
   __syncthreads();
   const uint32_t tile = sh_tile;
   const uint32_t base = tile * kClsTile + w * (kWave * kClsSteps);
