@@ -12,8 +12,9 @@ time (the reference's GB/s convention, example/test_cuda_compress.cu:95-96).
 Also reported: per-stage kernel times (HIP events on the launch stream), the roofline
 of the dominant kernel, plain SZp throughput on the same field, the threadblock codec
 (float and double data), the double random-access codec and the double 3D topology
-path on the same field, and the reference's own OpenMP CPU path timed on this host on
-a bounded slab of the same field.
+path on the same field, a batch of 256 mixed-size fields through the batched plain
+codec against a loop of one-field calls, and the reference's own OpenMP CPU path timed
+on this host on a bounded slab of the same field.
 """
 import argparse
 import json
@@ -201,6 +202,86 @@ def elem_side(tz, torch, f, shape, eb, bs, steps):
     return out
 
 
+def batch_side(tz, torch, seed, reps):
+    """Mixed-size workload: 256 float32 fields, half 2D (rows and cols log-uniform in
+    [64, 2048], blobs of sigma 20-200 + 0.01 noise), half 3D (each dim log-uniform in
+    [32, 256], blobs of sigma 5-60 + sine waves + 1e-3 noise), absolute error bound
+    log-uniform in [1e-4, 1e-2] per field, block sizes from the library's autotuner.
+    Batched compress + decompress (one launch per block-size class and stage) against a
+    loop of the one-field device calls on the same fields and block sizes; GB/s of input,
+    latency percentiles over the timed repetitions."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    fields, ebs = [], []
+    for i in range(256):
+        if i % 2 == 0:
+            shape = [int(round(v)) for v in np.exp(rng.uniform(np.log(64), np.log(2048), 2))]
+            f = tz.gen_blobs(shape, seed + i, 64, 20.0, 200.0, 0.01)
+        else:
+            shape = [int(round(v)) for v in np.exp(rng.uniform(np.log(32), np.log(256), 3))]
+            f = tz.gen_blobs(shape, seed + i, 128, 5.0, 60.0, 1e-3, waves=True)
+        fields.append(f.view(-1))
+        ebs.append(float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2)))))
+    total = sum(f.numel() for f in fields)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bss = tz.batch_tune(fields, ebs)
+    tune_ms = 1e3 * (time.perf_counter() - t0)
+    sizes, streams, blks = tz.plain_compress_batch(fields, ebs, bss)
+    outs = tz.plain_decompress_batch(streams, blks, [f.numel() for f in fields], ebs, bss)
+    lstreams = [torch.empty_like(s) for s in streams]
+    lblks = [torch.empty_like(b) for b in blks]
+    louts = [torch.empty_like(o) for o in outs]
+
+    def loop():
+        for i, f in enumerate(fields):
+            tz.plain_compress(f, ebs[i], bss[i], 1, out=lstreams[i], blk=lblks[i])
+            tz.plain_decompress(lstreams[i], lblks[i], f.numel(), ebs[i], bss[i], 1, out=louts[i])
+    loop()
+    torch.cuda.synchronize()
+    same = all(torch.equal(streams[i][:sizes[i]], lstreams[i][:sizes[i]]) and torch.equal(outs[i], louts[i])
+               for i in range(len(fields)))
+    err = max(float((outs[i] - fields[i]).abs().max()) / ebs[i] for i in range(len(fields)))
+    tz.prof_reset()
+    tz.prof_enable(True)
+    bt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        tz.plain_compress_batch(fields, ebs, bss, streams, blks)
+        tz.plain_decompress_batch(streams, blks, [f.numel() for f in fields], ebs, bss, outs)
+        torch.cuda.synchronize()
+        bt.append(time.perf_counter() - t0)
+    tz.prof_enable(False)
+    pr = tz.prof_read()
+    lt = []
+    for _ in range(max(2, reps // 4)):
+        t0 = time.perf_counter()
+        loop()
+        torch.cuda.synchronize()
+        lt.append(time.perf_counter() - t0)
+    cbytes = sum(sizes)
+    bt, lt = np.array(bt), np.array(lt)
+    ck = pr["batch"][0] / max(1, reps)
+    dk = pr["batch_decode"][0] / max(1, reps)
+    res = {"fields": len(fields), "elements": total, "ratio": 4 * total / cbytes,
+           "block_sizes": {str(b): bss.count(b) for b in sorted(set(bss))}, "tune_ms": tune_ms,
+           "batch_gbs": 4 * total / float(np.median(bt)) / 1e9,
+           "batch_p50_ms": 1e3 * float(np.percentile(bt, 50)), "batch_p99_ms": 1e3 * float(np.percentile(bt, 99)),
+           "loop_gbs": 4 * total / float(np.median(lt)) / 1e9,
+           "loop_p50_ms": 1e3 * float(np.percentile(lt, 50)), "loop_p99_ms": 1e3 * float(np.percentile(lt, 99)),
+           "compress_kernels_ms": ck, "decompress_kernels_ms": dk,
+           "compress_hbm_frac": (4 * total + cbytes) / (ck * 1e-3) / 1e9 / HBM_PEAK_GBS if ck else None,
+           "decompress_hbm_frac": (4 * total + cbytes) / (dk * 1e-3) / 1e9 / HBM_PEAK_GBS if dk else None,
+           "identical_to_loop": bool(same), "max_err_over_eb": err}
+    res["speedup_vs_loop"] = res["batch_gbs"] / res["loop_gbs"]
+    log(f"batch of {len(fields)} fields ({total / 1e6:.1f} M elements, ratio {res['ratio']:.2f}, block sizes "
+        f"{res['block_sizes']}): batched {res['batch_gbs']:.1f} GB/s (p50 {res['batch_p50_ms']:.2f} ms, p99 "
+        f"{res['batch_p99_ms']:.2f} ms; kernels {ck:.3f} + {dk:.3f} ms), loop of one-field calls "
+        f"{res['loop_gbs']:.1f} GB/s (p50 {res['loop_p50_ms']:.2f} ms) -> {res['speedup_vs_loop']:.1f}x, "
+        f"identical {same}, max err {err:.4f} eb, tune {tune_ms:.2f} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +296,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
     ap.add_argument("--no-elem", action="store_true", help="leave out the threadblock / double side figures")
+    ap.add_argument("--no-batch", action="store_true", help="leave out the mixed-size batch side figure")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -329,6 +411,9 @@ def main():
     # side figures: threadblock / double random-access codecs, double 3D topology
     elem = None if args.no_elem else elem_side(tz, torch, f, shape, eb, bs, 3)
 
+    # side figure: 256 mixed-size fields, batched against a loop of one-field calls
+    batch = None if args.no_batch else batch_side(tz, torch, args.seed, 12)
+
     # side figure: topology quality of the restored field against the original
     # (the CUDA driver's report), one GPU pass
     tz.quality_metrics(f, restored, shape, eb)
@@ -392,6 +477,7 @@ def main():
         "plain_gbs": plain_gbs,
         "topo_stream_parse_ms": parse_ms,
         "elem_codecs": elem,
+        "batched_plain": batch,
         "quality_ms": quality_ms,
         "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
                                             "over_1eb", "pairs", "violations", "saddles_above",

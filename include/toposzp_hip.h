@@ -81,6 +81,34 @@ int tszp_plain_compress(const float *d_in, size_t n, float eb, int block_size, u
 int tszp_plain_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
                           float eb, int block_size, float *d_out, void *stream);
 
+/* Batches of plain float fields: every field of a batch is compressed (decompressed) by
+ * one launch per block-size class instead of one call per field; each field's stream
+ * and side-car are byte-identical to tszp_plain_compress / tszp_plain_decompress with
+ * nchunks 1: [f32 eb big-endian][u64 0][blocks].  No counterpart in the reference,
+ * whose callers loop over szp_cuda_float_compress_randomaccess
+ * (SZp/src/szp_cuda_compress.cu:718-723) one field at a time.
+ * d_field: compression input / decompression output, n floats; d_stream: 4-byte
+ * aligned, stream_cap >= tszp_plain_bound(n, block_size, 1) for compression; d_blk:
+ * tszp_blocks(n, block_size) + 1 block offsets.  block_size 2 .. 1025; 0 asks
+ * tszp_batch_tune to choose.  Errors name the first failing field (caller's index). */
+typedef struct {
+  float *d_field;
+  uint64_t n;
+  float eb;               /* absolute error bound */
+  int block_size;
+  uint8_t *d_stream;
+  uint64_t stream_cap;
+  uint32_t *d_blk;
+  uint64_t stream_bytes;  /* set by tszp_plain_compress_batch */
+} tszp_batch_field;
+int tszp_plain_compress_batch(tszp_batch_field *fields, uint32_t count, void *stream);
+int tszp_plain_decompress_batch(const tszp_batch_field *fields, uint32_t count, void *stream);
+/* Block-size autotuner: for every field with block_size 0, estimates the stream size at
+ * block sizes 16, 32, 64 and 128 from up to 512 windows of 128 elements spread evenly
+ * over the field (one launch for all fields) and sets block_size to the smallest
+ * estimate (64 on a tie or for fields shorter than 128). */
+int tszp_batch_tune(tszp_batch_field *fields, uint32_t count, void *stream);
+
 /* Streams partitioned by elements: chunk t of nchunks covers [t * (n / nchunks), ...),
  * the last chunk up to n; header [eb big-endian, elem_bytes wide][u64 off[nchunks]].
  * d_blk receives tszp_elem_units(n, block_size, nchunks, ra) + 1 unit offsets relative
@@ -225,7 +253,7 @@ int tszp_szp_unpack(const uint8_t *buf, size_t len, size_t *topo_bytes, size_t *
  * 6 topology restoration (whole call), 7 each scatter launch (bin scatter or radix
  * pass), 8 each window-solve launch, 9 each bin rank-query launch, 10 threadblock /
  * double random-access compression, 11 their decoder, 12 double 3D quantization and
- * saddle fix.  tszp_prof_read
+ * saddle fix, 13 batch compression, 14 batch decompression.  tszp_prof_read
  * returns the summed milliseconds and the number of timed launches since
  * tszp_prof_reset. */
 void tszp_prof_enable(int on);

@@ -626,10 +626,16 @@ void or_sortpos_decompress(const unsigned char *cmp, size_t e, int bs, int *out)
     }
 }
 
-/* :431-524 -- `cmp` is the stream after its 4-byte error-bound header. */
+/* :431-524 -- `cmp` is the stream after its 4-byte error-bound header; the chunk count
+ * comes from the reference's detection (szp_detect_nbThreads_randomaccess), which can
+ * misread some single-chunk streams, so or_plain_decompress_T takes it from the caller. */
 void or_plain_decompress(const unsigned char *cmp, size_t n, float eb, int bs, float *out)
 {
-    unsigned T = or_detect_T_ra(cmp, n, bs, sizeof(float));
+    or_plain_decompress_T(cmp, n, eb, bs, or_detect_T_ra(cmp, n, bs, sizeof(float)), out);
+}
+
+void or_plain_decompress_T(const unsigned char *cmp, size_t n, float eb, int bs, unsigned T, float *out)
+{
     const unsigned char *rcp = cmp + 8 * (size_t)T;
     size_t nb = (n + bs - 1) / bs;
     int q[1024];

@@ -80,6 +80,7 @@ void or_topo_decompress(const unsigned char *cmp, size_t n, float eb, int bs,
                         float *out, int *types);
 void or_sortpos_decompress(const unsigned char *cmp, size_t e, int bs, int *out);
 void or_plain_decompress(const unsigned char *cmp, size_t n, float eb, int bs, float *out);
+void or_plain_decompress_T(const unsigned char *cmp, size_t n, float eb, int bs, unsigned T, float *out);
 
 /* Byte offset (from payload start) of every block of a T = 1 stream: a serial walk. */
 int or_block_offsets(const unsigned char *payload, size_t n, int bs, int topo,

@@ -62,6 +62,7 @@ def lib():
         L.or_topo_decompress.restype = None
         L.or_sortpos_decompress.argtypes = [vp, sz, i32, vp]; L.or_sortpos_decompress.restype = None
         L.or_plain_decompress.argtypes = [vp, sz, f32, i32, vp]; L.or_plain_decompress.restype = None
+        L.or_plain_decompress_T.argtypes = [vp, sz, f32, i32, C.c_uint, vp]; L.or_plain_decompress_T.restype = None
         L.or_block_offsets.argtypes = [vp, sz, i32, i32, i32, vp]; L.or_block_offsets.restype = i32
         L.or_classify_2d.argtypes = [vp, i32, i32, vp]; L.or_classify_2d.restype = None
         L.or_classify_3d.argtypes = [vp, i32, i32, i32, vp]; L.or_classify_3d.restype = None
@@ -181,10 +182,14 @@ def sortpos_decompress(cmp_after_count, e, bs):
     return out
 
 
-def plain_decompress(cmp_after_header, n, eb, bs):
+def plain_decompress(cmp_after_header, n, eb, bs, T=None):
+    """T None: the chunk count from the reference's detection, as its decoder does."""
     c = np.ascontiguousarray(cmp_after_header, dtype=np.uint8)
     out = np.zeros(n, dtype=np.float32)
-    lib().or_plain_decompress(_p(c), n, float(eb), bs, _p(out))
+    if T is None:
+        lib().or_plain_decompress(_p(c), n, float(eb), bs, _p(out))
+    else:
+        lib().or_plain_decompress_T(_p(c), n, float(eb), bs, int(T), _p(out))
     return out
 
 

@@ -264,6 +264,9 @@ CASES = [
     # chunks are uneven, so this incompressible case uses two equal chunks)
     ("gplain_bs128", lambda R: case_plain(R, "gplain_bs128", fields.huge2d(16, 32, seed=12).ravel(), 0.7, 128, Ts=(1, 2))),
     ("gplain_bs32", lambda R: case_plain(R, "gplain_bs32", fields.ties2d(20, 20, seed=13).ravel(), 3e-3, 32)),
+    # a one-chunk stream that the reference's chunk-count detection misreads
+    ("gplain_misdetect", lambda R: case_plain(R, "gplain_misdetect", fields.blobs3d(12, 35, 5, seed=1000).ravel(),
+                                              0.007894032212920654, 2, Ts=(1,))),
     # threadblock (chained) streams: float and double, ragged chunks and blocks
     ("gtb_f32_bs64", lambda R: case_elem(R, "gtb_f32_bs64", fields.blobs2d(30, 41, seed=14).ravel(), 1e-3, 64, Ts=(1, 3, 8))),
     ("gtb_f32_bs7", lambda R: case_elem(R, "gtb_f32_bs7", fields.ties2d(21, 19, seed=15).ravel(), 3e-3, 7, Ts=(1, 4))),

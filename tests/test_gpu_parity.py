@@ -83,7 +83,13 @@ def test_dropin_plain_matches_reference_golden(path):
     for key in g:
         if key.startswith("plain_T"):
             dec = tz.decompress_plain(g[key][4:], data.size, eb, bs)
-            _eq(dec, g["plain_dec_T1"], f"{key} decode")
+            if "misdetect" in os.path.basename(path):
+                # the reference's chunk-count detection misreads this one-chunk stream
+                # (its output is 185 eb off); the drop-in walks it as one chunk
+                _eq(dec, oracle.plain_decompress(g[key][4:], data.size, eb, bs, T=1), f"{key} decode")
+                assert float(np.abs(dec - data).max()) <= eb * 1.0001
+            else:
+                _eq(dec, g["plain_dec_T1"], f"{key} decode")
 
 
 # ------------------------------------------------------------------------------------------

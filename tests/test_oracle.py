@@ -99,6 +99,23 @@ def test_plain_matches_reference(path):
                                           g["plain_dec_T" + key[7:]].view(np.uint32))
 
 
+@pytest.mark.parametrize("path", PLAIN, ids=os.path.basename)
+def test_plain_decoder_with_known_chunk_count(path):
+    """The decoder given the chunk count agrees with the reference's detecting decoder
+    wherever the detection is right, and stays within the error bound on the one-chunk
+    stream that the reference's detection misreads (gplain_misdetect, where the
+    reference's own output is 185 eb off)."""
+    g = _load(path)
+    data, eb, bs = g["data"], float(g["eb"]), int(g["bs"])
+    dec = oracle.plain_decompress(g["plain_T1"][4:], data.size, eb, bs, T=1)
+    assert float(np.abs(dec - data).max()) <= eb * 1.0001
+    ref = g["plain_dec_T1"]
+    if "misdetect" in os.path.basename(path):
+        assert float(np.abs(ref - data).max()) > 100 * eb
+    else:
+        np.testing.assert_array_equal(dec.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("path", ELEM, ids=os.path.basename)
 def test_elem_streams_match_reference(path):
     """Threadblock (float, double) and double random-access writers, every chunk count

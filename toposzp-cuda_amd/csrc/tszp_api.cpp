@@ -3,6 +3,7 @@
 // (include/szp_cuda_*.h), both driving the HIP kernels of this directory.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -49,7 +50,8 @@ enum Slot {
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
-  kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsCount
+  kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
+  kWsCount
 };
 
 struct Ctx {
@@ -59,6 +61,25 @@ struct Ctx {
   void* ptr[kWsCount] = {};
   size_t cap[kWsCount] = {};
   CompressResult* host_res = nullptr;  // pinned
+  void* host_batch = nullptr;          // pinned: batch tables and results
+  size_t host_batch_cap = 0;
+
+  int host_batch_get(size_t bytes, void** out) {
+    if (host_batch_cap < bytes) {
+      if (host_batch) {
+        hipError_t e = hipHostFree(host_batch);
+        host_batch = nullptr;
+        host_batch_cap = 0;
+        if (e != hipSuccess) return fail(TSZP_EHIP, "hipHostFree: %s", hipGetErrorString(e));
+      }
+      const size_t want = bytes + bytes / 4 + 4096;
+      hipError_t e = hipHostMalloc(&host_batch, want, hipHostMallocDefault);
+      if (e != hipSuccess) return fail(TSZP_EHIP, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
+      host_batch_cap = want;
+    }
+    *out = host_batch;
+    return TSZP_OK;
+  }
 
   int get(int slot, size_t bytes, void** out) {
     if (bytes == 0) bytes = 4;
@@ -85,7 +106,7 @@ std::vector<Ctx*> g_ctx;
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
 enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
-                kProfScatter, kProfSolve, kProfQuery, kProfElem, kProfElemDecode, kProfSaddle, kProfCount };
+                kProfScatter, kProfSolve, kProfQuery, kProfElem, kProfElemDecode, kProfSaddle, kProfBatch, kProfBatchDecode, kProfCount };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -719,6 +740,234 @@ int tszp_plain_decompress(const uint8_t* d_stream, const uint32_t* d_blk, unsign
   std::lock_guard<std::mutex> lk(c->mu);
   return decode(c, kModePlain, d_stream + 4 + 8 * (size_t)T, d_blk, n, bs, eb, d_out, nullptr, nullptr,
                 (hipStream_t)stream);
+}
+
+// ---- batches of plain float fields -----------------------------------------------------
+namespace {
+
+struct BatchClass {
+  uint32_t bs;      // a block size of the class (selects the kernel geometry)
+  uint32_t base;    // first tile in the batch's numbering
+  uint32_t ntiles;
+};
+
+// Validates the fields, orders them by block-size class, numbers their tiles, uploads
+// the table and the tile -> field map.  order[i] is the caller's index of table entry i.
+int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress, const char* what,
+               hipStream_t st, BatchField** d_tab, uint32_t** d_tmap, std::vector<BatchClass>* classes,
+               std::vector<uint32_t>* order, uint32_t* total_tiles) {
+  if (!fl || count == 0) return fail(TSZP_EINVAL, "%s: no fields", what);
+  if (count > (1u << 20)) return fail(TSZP_EINVAL, "%s: %u fields, at most 2^20 per batch", what, count);
+  std::vector<std::pair<uint64_t, uint32_t>> key(count);  // (geometry class, caller index)
+  for (uint32_t i = 0; i < count; i++) {
+    const tszp_batch_field& f = fl[i];
+    if (!f.d_field || !f.d_stream || !f.d_blk) return fail(TSZP_EINVAL, "%s: field %u: null pointer", what, i);
+    if (f.n == 0 || f.n >= 0x7fffffffull)
+      return fail(TSZP_EINVAL, "%s: field %u: element count %llu out of range [1, 2^31)", what, i,
+                  (unsigned long long)f.n);
+    if (!(f.eb > 0) || !std::isfinite(f.eb))
+      return fail(TSZP_EINVAL, "%s: field %u: error bound must be positive and finite", what, i);
+    if (f.block_size < 2 || f.block_size > 1025)
+      return fail(TSZP_EINVAL, "%s: field %u: blockSize %d out of range [2, 1025] (0: run tszp_batch_tune first)",
+                  what, i, f.block_size);
+    if ((uintptr_t)f.d_stream & 3u) return fail(TSZP_EINVAL, "%s: field %u: stream must be 4-byte aligned", what, i);
+    if (compress && f.stream_cap < kBatchHdr + 4)
+      return fail(TSZP_ECAP, "%s: field %u: stream capacity %llu too small", what, i,
+                  (unsigned long long)f.stream_cap);
+    int S, EPL;
+    uint32_t bpt;
+    block_geometry((uint32_t)f.block_size, &S, &EPL, &bpt);
+    key[i] = {((uint64_t)S << 32) | (uint32_t)EPL, i};
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
+                     return a.first < b.first;
+                   });
+  void* hp;
+  // room for the results too, so that reading them back never reallocates this buffer
+  int r = c->host_batch_get((size_t)count * (sizeof(BatchField) + sizeof(CompressResult)), &hp);
+  if (r) return r;
+  BatchField* tab = static_cast<BatchField*>(hp);
+  order->resize(count);
+  classes->clear();
+  uint64_t tiles = 0;
+  for (uint32_t k = 0; k < count; k++) {
+    const uint32_t i = key[k].second;
+    const tszp_batch_field& f = fl[i];
+    int S, EPL;
+    uint32_t bpt;
+    block_geometry((uint32_t)f.block_size, &S, &EPL, &bpt);
+    if (k == 0 || key[k].first != key[k - 1].first) classes->push_back({(uint32_t)f.block_size, (uint32_t)tiles, 0});
+    BatchField b{};
+    b.n = (uint32_t)f.n;
+    b.bs = (uint32_t)f.block_size;
+    b.nb = cdiv(f.n, f.block_size);
+    b.ntiles = cdiv(b.nb, bpt);
+    b.tile0 = (uint32_t)tiles;
+    b.eb = f.eb;
+    b.blk = f.d_blk;
+    if (compress) {
+      b.in = f.d_field;
+      b.out = f.d_stream;
+      const uint64_t pc = f.stream_cap - kBatchHdr - 4;  // keep 4 readable bytes past the end
+      b.cap = pc > 0xffffffffull ? 0xffffffffull : pc;
+      volatile float finv = 1 / f.eb;  // the plain codec's reciprocal (SZp/src/szp_float.cc:2125)
+      b.inv = (double)finv;
+    } else {
+      b.in = f.d_stream;
+      b.out = f.d_field;
+    }
+    tab[k] = b;
+    (*order)[k] = i;
+    tiles += b.ntiles;
+    classes->back().ntiles += b.ntiles;
+    if (tiles >= 0x7fffffffull) return fail(TSZP_EINVAL, "%s: more than 2^31 tiles in one batch", what);
+  }
+  *total_tiles = (uint32_t)tiles;
+  WS(kWsBatchTab, (size_t)count * sizeof(BatchField), *d_tab);
+  WS(kWsBatchMap, (size_t)tiles * 4, *d_tmap);
+  HIPCHK(hipMemcpyAsync(*d_tab, tab, (size_t)count * sizeof(BatchField), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_batch_tilemap(*d_tab, count, *d_tmap, st));
+  return TSZP_OK;
+}
+
+// Results of a batch launch back to the host (the pinned table is free again after it)
+int batch_results(Ctx* c, CompressResult* d_res, uint32_t count, hipStream_t st, CompressResult** out) {
+  void* hp;
+  int r = c->host_batch_get((size_t)count * sizeof(BatchField) + (size_t)count * sizeof(CompressResult), &hp);
+  if (r) return r;
+  CompressResult* h = reinterpret_cast<CompressResult*>(static_cast<char*>(hp) + (size_t)count * sizeof(BatchField));
+  HIPCHK(hipMemcpyAsync(h, d_res, (size_t)count * sizeof(CompressResult), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = h;
+  return TSZP_OK;
+}
+
+int batch_error(const char* what, uint32_t field, uint32_t bits) {
+  return fail(TSZP_EDEVICE, "%s: field %u: device error bits 0x%x (%s%s%s)", what, field, bits,
+              (bits & TSZP_ERR_LOOKBACK) ? "look-back timeout " : "", (bits & TSZP_ERR_STREAM) ? "corrupt stream " : "",
+              (bits & TSZP_ERR_OVERFLOW) ? "capacity " : "");
+}
+
+}  // namespace
+
+int tszp_plain_compress_batch(tszp_batch_field* fields, uint32_t count, void* stream) {
+  g_err.clear();
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  BatchField* tab;
+  uint32_t *tmap, tiles;
+  std::vector<BatchClass> classes;
+  std::vector<uint32_t> order;
+  r = batch_plan(c, fields, count, true, "plain_compress_batch", st, &tab, &tmap, &classes, &order, &tiles);
+  if (r) {
+    (void)hipStreamSynchronize(st);  // the pinned table may still be in a copy
+    return r;
+  }
+  uint64_t* status;
+  CompressResult* res;
+  WS(kWsBatchSt, (size_t)tiles * 8 + 8, status);
+  WS(kWsBatchRes, (size_t)count * sizeof(CompressResult), res);
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, (size_t)tiles * 2, st));
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4, st));
+  ProfScope prof(kProfBatch, st);
+  for (const BatchClass& k : classes)
+    HIPCHK(launch_compress_batch(k.bs, tab, tmap, k.base, k.ntiles, status, res, st));
+  prof.stop();
+  CompressResult* h;
+  if ((r = batch_results(c, res, count, st, &h))) return r;
+  prof.collect();
+  for (uint32_t k = 0; k < count; k++) {
+    const uint32_t i = order[k];
+    if (h[k].error) return batch_error("plain_compress_batch", i, h[k].error);
+    fields[i].stream_bytes = kBatchHdr + h[k].payload_bytes;
+  }
+  return TSZP_OK;
+}
+
+int tszp_plain_decompress_batch(const tszp_batch_field* fields, uint32_t count, void* stream) {
+  g_err.clear();
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  BatchField* tab;
+  uint32_t *tmap, tiles;
+  std::vector<BatchClass> classes;
+  std::vector<uint32_t> order;
+  r = batch_plan(c, fields, count, false, "plain_decompress_batch", st, &tab, &tmap, &classes, &order, &tiles);
+  if (r) {
+    (void)hipStreamSynchronize(st);
+    return r;
+  }
+  CompressResult* res;
+  WS(kWsBatchRes, (size_t)count * sizeof(CompressResult), res);
+  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4, st));
+  ProfScope prof(kProfBatchDecode, st);
+  for (const BatchClass& k : classes) HIPCHK(launch_decode_batch(k.bs, tab, tmap, k.base, k.ntiles, res, st));
+  prof.stop();
+  CompressResult* h;
+  if ((r = batch_results(c, res, count, st, &h))) return r;
+  prof.collect();
+  for (uint32_t k = 0; k < count; k++)
+    if (h[k].error) return batch_error("plain_decompress_batch", order[k], h[k].error);
+  return TSZP_OK;
+}
+
+int tszp_batch_tune(tszp_batch_field* fields, uint32_t count, void* stream) {
+  g_err.clear();
+  if (!fields || count == 0) return fail(TSZP_EINVAL, "batch_tune: no fields");
+  if (count > (1u << 20)) return fail(TSZP_EINVAL, "batch_tune: %u fields, at most 2^20 per batch", count);
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<uint32_t> pick;  // the fields to tune: block_size 0
+  for (uint32_t i = 0; i < count; i++) {
+    const tszp_batch_field& f = fields[i];
+    if (f.block_size != 0) continue;
+    if (!f.d_field || f.n == 0 || f.n >= 0x7fffffffull || !(f.eb > 0) || !std::isfinite(f.eb))
+      return fail(TSZP_EINVAL, "batch_tune: field %u: bad field, element count or error bound", i);
+    pick.push_back(i);
+  }
+  if (pick.empty()) return TSZP_OK;
+  const uint32_t m = (uint32_t)pick.size();
+  void* hp;
+  if ((r = c->host_batch_get((size_t)m * sizeof(BatchField) + (size_t)m * kTuneCands * 4, &hp))) return r;
+  BatchField* tab = static_cast<BatchField*>(hp);
+  uint32_t* hest = reinterpret_cast<uint32_t*>(tab + m);
+  for (uint32_t k = 0; k < m; k++) {
+    const tszp_batch_field& f = fields[pick[k]];
+    BatchField b{};
+    b.in = f.d_field;
+    b.n = (uint32_t)f.n;
+    b.eb = f.eb;
+    volatile float finv = 1 / f.eb;
+    b.inv = (double)finv;
+    tab[k] = b;
+  }
+  BatchField* dtab;
+  uint32_t* dest;
+  WS(kWsBatchTab, (size_t)m * sizeof(BatchField), dtab);
+  WS(kWsBatchRes, (size_t)m * kTuneCands * 4, dest);
+  HIPCHK(hipMemcpyAsync(dtab, tab, (size_t)m * sizeof(BatchField), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_batch_tune(dtab, m, dest, st));
+  HIPCHK(hipMemcpyAsync(hest, dest, (size_t)m * kTuneCands * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (uint32_t k = 0; k < m; k++) {
+    // the smallest estimated stream; on a tie 64, then the larger block size (fewer
+    // blocks to walk); 64 when the field is too short to sample (every estimate 0)
+    int best = 2;
+    for (int j = kTuneCands - 1; j >= 0; j--)
+      if (hest[(size_t)k * kTuneCands + j] < hest[(size_t)k * kTuneCands + best]) best = j;
+    fields[pick[k]].block_size = (int)kTuneSizes[best];
+  }
+  return TSZP_OK;
 }
 
 // ---- element-partitioned streams: threadblock codec (float, double), double RA ---------

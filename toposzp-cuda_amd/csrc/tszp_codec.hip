@@ -80,8 +80,9 @@ __device__ __forceinline__ int type_of(int ndim, float c, const float* nb) {
 // ---------------------------------------------------------------------------
 // compression
 // ---------------------------------------------------------------------------
+// One tile of BPT blocks; `tile_in` is the tile unless a.tile_counter hands tiles out.
 template <int MODE, int S, int EPL>
-__global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
+__device__ __forceinline__ void compress_tile(const CompressArgs& a, uint32_t tile_in) {
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
   constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   const int g = lane / S, ls = lane & (S - 1);
   const uint32_t bs = a.bs;
 
-  if (tid == 0) sh_tile = tile_id(a.tile_counter);
+  if (tid == 0) sh_tile = a.tile_counter ? atomicAdd(a.tile_counter, 1u) : tile_in;
   for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
@@ -385,11 +386,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
   }
 }
 
+template <int MODE, int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_compress(CompressArgs a) {
+  compress_tile<MODE, S, EPL>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // decompression (block offsets from the side-car)
 // ---------------------------------------------------------------------------
 template <int MODE, int S, int EPL>
-__global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
+__device__ __forceinline__ void decode_tile(const DecodeArgs& a, uint32_t tile) {
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
   constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
@@ -400,7 +406,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
   const int w = tid / kWave, lane = tid & (kWave - 1);
   const int g = lane / S, ls = lane & (S - 1);
   const uint32_t bs = a.bs;
-  const uint32_t tile = blockIdx.x;
   const uint32_t b0 = tile * BPT;
   const uint32_t b1 = min(a.nb, b0 + (uint32_t)BPT);
 
@@ -498,6 +503,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
       }
     }
   }
+}
+
+template <int MODE, int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_decode(DecodeArgs a) {
+  decode_tile<MODE, S, EPL>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -903,6 +913,121 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
 }
 
 // ---------------------------------------------------------------------------
+// batches of plain float fields: every field of one block-size class in one launch.
+// Workgroup b takes tile base + b of the batch's tile numbering; tmap names the tile's
+// field, whose tiles are contiguous, and the field's look-back status words start at
+// its first tile, so a tile only waits on earlier tiles of its own field.  Streams and
+// side-cars are those of the one-field codec with one chunk.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlockThreads) void k_batch_tilemap(const BatchField* __restrict__ fields,
+                                                                uint32_t* __restrict__ tmap) {
+  const uint32_t fi = blockIdx.x;  // one workgroup per field
+  const uint32_t t0 = fields[fi].tile0, nt = fields[fi].ntiles;
+  for (uint32_t t = threadIdx.x; t < nt; t += kBlockThreads) tmap[t0 + t] = fi;
+}
+
+template <int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_compress_batch(const BatchField* __restrict__ fields,
+                                                                 const uint32_t* __restrict__ tmap, uint32_t base,
+                                                                 uint64_t* __restrict__ status,
+                                                                 CompressResult* __restrict__ res) {
+  const uint32_t gt = base + blockIdx.x;
+  const uint32_t fi = tmap[gt];
+  const BatchField& f = fields[fi];
+  CompressArgs a{};
+  a.in = f.in;
+  a.n = f.n;
+  a.bs = f.bs;
+  a.nb = f.nb;
+  a.ntiles = f.ntiles;
+  a.eb = f.eb;
+  a.inv = f.inv;
+  a.payload = reinterpret_cast<uint8_t*>(f.out) + kBatchHdr;
+  a.cap = f.cap;
+  a.blk = f.blk;
+  a.st_bytes = status + f.tile0;
+  a.res = res + fi;
+  const uint32_t tile = gt - f.tile0;
+  if (tile == 0 && threadIdx.x == 0) {  // header: eb big-endian, then chunk offset 0
+    uint32_t* h = reinterpret_cast<uint32_t*>(f.out);
+    h[0] = __builtin_bswap32(__float_as_uint(f.eb));
+    h[1] = 0u;
+    h[2] = 0u;
+  }
+  compress_tile<kModePlain, S, EPL>(a, tile);
+}
+
+template <int S, int EPL>
+__global__ __launch_bounds__(kBlockThreads) void k_decode_batch(const BatchField* __restrict__ fields,
+                                                               const uint32_t* __restrict__ tmap, uint32_t base,
+                                                               CompressResult* __restrict__ res) {
+  const uint32_t gt = base + blockIdx.x;
+  const uint32_t fi = tmap[gt];
+  const BatchField& f = fields[fi];
+  DecodeArgs a{};
+  a.payload = reinterpret_cast<const uint8_t*>(f.in) + kBatchHdr;
+  a.blk = f.blk;
+  a.n = f.n;
+  a.bs = f.bs;
+  a.nb = f.nb;
+  a.eb = f.eb;
+  a.out = f.out;
+  a.err = &res[fi].error;
+  decode_tile<kModePlain, S, EPL>(a, gt - f.tile0);
+}
+
+// Payload bytes of sampled windows at each candidate block size: one workgroup per
+// field, each wave one window of 128 elements at a time (two per lane), windows spread
+// evenly over the field and aligned to 128 elements, so that every candidate's blocks
+// inside a window are blocks of the real stream.
+template <int L>
+__device__ __forceinline__ unsigned tune_block_bytes(unsigned m0, unsigned m1, int lane) {
+  // lane holds elements 2 lane, 2 lane + 1; m1 is the delta into 2 lane + 2, which
+  // belongs to the block only when 2 lane + 2 does not start the next one
+  const unsigned m = ((2 * lane + 2) % (2 * L)) ? (m1 > m0 ? m1 : m0) : m0;
+  const unsigned bm = seg_max<L>(m);
+  return (lane % L) == 0 ? block_bytes(2 * L, bitcount(bm), false) : 0u;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_batch_tune(const BatchField* __restrict__ fields,
+                                                             uint32_t* __restrict__ est) {
+  static_assert(kTuneWindow == 2 * kWave, "two elements per lane");
+  static_assert(kTuneSizes[0] == 16 && kTuneSizes[1] == 32 && kTuneSizes[2] == 64 && kTuneSizes[3] == 128,
+                "candidate block sizes");
+  __shared__ uint32_t red[kWavesPerBlock][kTuneCands];
+  const BatchField& f = fields[blockIdx.x];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const uint32_t nwin = f.n / kTuneWindow;
+  const uint32_t W = nwin < kTuneMaxWindows ? nwin : kTuneMaxWindows;
+  const float* x = reinterpret_cast<const float*>(f.in);
+  unsigned acc[kTuneCands] = {0u, 0u, 0u, 0u};
+  for (uint32_t k = w; k < W; k += kWavesPerBlock) {
+    const uint64_t e0 = ((uint64_t)k * nwin / W) * kTuneWindow + 2u * (uint32_t)lane;
+    const int q0 = cvt_i32((double)x[e0] * f.inv);
+    const int q1 = cvt_i32((double)x[e0 + 1] * f.inv);
+    const int q2 = __shfl_down(q0, 1, kWave);
+    const unsigned d0 = (unsigned)q1 - (unsigned)q0, d1 = (unsigned)q2 - (unsigned)q1;
+    const unsigned m0 = (int)d0 < 0 ? 0u - d0 : d0;
+    const unsigned m1 = lane + 1 < kWave ? ((int)d1 < 0 ? 0u - d1 : d1) : 0u;
+    acc[0] += tune_block_bytes<8>(m0, m1, lane);
+    acc[1] += tune_block_bytes<16>(m0, m1, lane);
+    acc[2] += tune_block_bytes<32>(m0, m1, lane);
+    acc[3] += tune_block_bytes<64>(m0, m1, lane);
+  }
+#pragma unroll
+  for (int c = 0; c < kTuneCands; c++) {
+    const unsigned v = seg_sum<kWave>(acc[c]);
+    if (lane == 0) red[w][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kTuneCands) {
+    uint32_t t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += red[k][threadIdx.x];
+    est[(size_t)blockIdx.x * kTuneCands + threadIdx.x] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side dispatch on block size
 // ---------------------------------------------------------------------------
 template <int MODE>
@@ -1021,6 +1146,55 @@ hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st) {
     case kModeInt: return launch_decode_mode<kModeInt>(a, S, EPL, ntiles, st);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32_t* tmap, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_tilemap, dim3(count), dim3(kBlockThreads), 0, st, fields, tmap);
+  return hipGetLastError();
+}
+
+hipError_t launch_compress_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                                 uint32_t ntiles, uint64_t* status, CompressResult* res, hipStream_t st) {
+  if (!ntiles) return hipSuccess;
+  int S, EPL;
+  uint32_t bpt;
+  block_geometry(bs_class, &S, &EPL, &bpt);
+#define TSZP_CASE(SS, EE)                                                                            \
+  if (S == SS && EPL == EE) {                                                                        \
+    hipLaunchKernelGGL((k_compress_batch<SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, base, \
+                       status, res);                                                                 \
+    return hipGetLastError();                                                                        \
+  }
+  TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8) TSZP_CASE(64, 16)
+  TSZP_CASE(64, 32)
+#undef TSZP_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                               uint32_t ntiles, CompressResult* res, hipStream_t st) {
+  if (!ntiles) return hipSuccess;
+  int S, EPL;
+  uint32_t bpt;
+  block_geometry(bs_class, &S, &EPL, &bpt);
+#define TSZP_CASE(SS, EE)                                                                                 \
+  if (S == SS && EPL == EE) {                                                                             \
+    hipLaunchKernelGGL((k_decode_batch<SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, base, res); \
+    return hipGetLastError();                                                                             \
+  }
+  TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8) TSZP_CASE(64, 16)
+  TSZP_CASE(64, 32)
+#undef TSZP_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_tune(const BatchField* fields, uint32_t count, uint32_t* est, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_tune, dim3(count), dim3(kBlockThreads), 0, st, fields, est);
+  return hipGetLastError();
 }
 
 }  // namespace tszp

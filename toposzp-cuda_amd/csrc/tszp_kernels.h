@@ -52,6 +52,33 @@ void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* blocks_per_tile);
 hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st);
 hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st);
 
+// ---- batches of plain float fields (tszp_codec.hip) ----
+// One entry per field; the fields of one launch share a block-size class (block_geometry)
+// and number their tiles contiguously from tile0 in field order.
+constexpr uint32_t kBatchHdr = 12;  // [f32 eb big-endian][u64 off[1] = 0]
+struct BatchField {
+  const void* in;         // compress: float[n]; decode: the stream (header included)
+  void* out;              // compress: the stream (4-byte aligned); decode: float[n]
+  uint32_t* blk;          // nb + 1 block offsets relative to the payload
+  uint64_t cap;           // compress: payload capacity in bytes
+  uint32_t n, bs, nb, ntiles, tile0;
+  float eb;
+  double inv;             // compress: reciprocal used for quantization
+};
+hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32_t* tmap, hipStream_t st);
+// tiles [base, base + ntiles) of the batch's numbering, all of one block-size class
+hipError_t launch_compress_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                                 uint32_t ntiles, uint64_t* status, CompressResult* res, hipStream_t st);
+hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                               uint32_t ntiles, CompressResult* res, hipStream_t st);
+// Estimated payload bytes of every field at each block size of kTuneSizes, from
+// sampled windows of kTuneWindow elements (est: count * kTuneCands words).
+constexpr int kTuneCands = 4;
+constexpr uint32_t kTuneSizes[kTuneCands] = {16, 32, 64, 128};
+constexpr uint32_t kTuneWindow = 128;
+constexpr uint32_t kTuneMaxWindows = 512;
+hipError_t launch_batch_tune(const BatchField* fields, uint32_t count, uint32_t* est, hipStream_t st);
+
 // Tile counter to pass to a look-back kernel: null (tiles = workgroup ids) unless
 // TSZP_TILE_COUNTER is set, which hands tiles out from the counter instead.
 inline uint32_t* dyn_counter(uint32_t* counter) { return getenv("TSZP_TILE_COUNTER") ? counter : nullptr; }

@@ -39,6 +39,13 @@ class RestoreInfo(C.Structure):
         return self.missing + self.spurious + self.changed
 
 
+class BatchField(C.Structure):
+    """tszp_batch_field (include/toposzp_hip.h)."""
+    _fields_ = [("d_field", C.c_void_p), ("n", C.c_uint64), ("eb", C.c_float), ("block_size", C.c_int),
+                ("d_stream", C.c_void_p), ("stream_cap", C.c_uint64), ("d_blk", C.c_void_p),
+                ("stream_bytes", C.c_uint64)]
+
+
 _lib = None
 _libc = C.CDLL("libc.so.6")
 _libc.free.argtypes = [C.c_void_p]
@@ -84,6 +91,9 @@ def lib():
     L.tszp_sortpos_decompress.argtypes = [vp, vp, u32, sz, i32, vp, vp]
     L.tszp_plain_compress.argtypes = [vp, sz, f32, i32, u32, vp, sz, vp, u64p, vp]
     L.tszp_plain_decompress.argtypes = [vp, vp, u32, sz, f32, i32, vp, vp]
+    for f in ("tszp_plain_compress_batch", "tszp_plain_decompress_batch", "tszp_batch_tune"):
+        getattr(L, f).argtypes = [C.POINTER(BatchField), u32, vp]
+        getattr(L, f).restype = i32
     L.tszp_classify.argtypes = [vp, i32, vp, vp, u64p, vp]
     L.tszp_gen_blobs.argtypes = [vp, i32, vp, C.c_uint64, i32, f32, f32, f32, i32, vp]
     L.tszp_gen_fbm.argtypes = [vp, vp, C.c_uint64, i32, f32, vp]
@@ -158,7 +168,7 @@ def lib():
 
 PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5,
               "restore": 6, "scatter": 7, "solve": 8, "query": 9, "elem": 10, "elem_decode": 11,
-              "saddle": 12}
+              "saddle": 12, "batch": 13, "batch_decode": 14}
 
 
 def prof_enable(on=True):
@@ -378,6 +388,56 @@ def plain_decompress(stream, blk, n, eb, bs, T=1, out=None):
         out = torch.empty(n, dtype=torch.float32, device=stream.device)
     _check(lib().tszp_plain_decompress(_dp(stream), _dp(blk), T, n, eb, bs, _dp(out), _stream()))
     return out
+
+
+def _batch_table(fields, ebs, bss, streams=None, blks=None):
+    n = len(fields)
+    if not (len(ebs) == n and len(bss) == n):
+        raise ValueError("one error bound and one block size per field")
+    tab = (BatchField * n)()
+    for i, f in enumerate(fields):
+        t = tab[i]
+        t.d_field = f.data_ptr()
+        t.n = f.numel()
+        t.eb = float(ebs[i])
+        t.block_size = int(bss[i])
+        if streams is not None:
+            t.d_stream = streams[i].data_ptr()
+            t.stream_cap = streams[i].numel()
+            t.d_blk = blks[i].data_ptr()
+    return tab
+
+
+def batch_tune(fields, ebs):
+    """Block size per field chosen by the library's autotuner (tszp_batch_tune)."""
+    tab = _batch_table(fields, ebs, [0] * len(fields))
+    _check(lib().tszp_batch_tune(tab, len(fields), _stream()))
+    return [tab[i].block_size for i in range(len(fields))]
+
+
+def plain_compress_batch(fields, ebs, bss, streams=None, blks=None):
+    """Plain one-chunk streams of many float32 CUDA tensors in one launch per block-size
+    class: (bytes per field, streams, side-cars)."""
+    torch = _torch()
+    L = lib()
+    if streams is None:
+        streams = [torch.empty(L.tszp_plain_bound(f.numel(), b, 1), dtype=torch.uint8, device=f.device)
+                   for f, b in zip(fields, bss)]
+        blks = [torch.empty(L.tszp_blocks(f.numel(), b) + 1, dtype=torch.int32, device=f.device)
+                for f, b in zip(fields, bss)]
+    tab = _batch_table(fields, ebs, bss, streams, blks)
+    _check(L.tszp_plain_compress_batch(tab, len(fields), _stream()))
+    return [tab[i].stream_bytes for i in range(len(fields))], streams, blks
+
+
+def plain_decompress_batch(streams, blks, ns, ebs, bss, outs=None):
+    """Decodes many plain one-chunk streams in one launch per block-size class."""
+    torch = _torch()
+    if outs is None:
+        outs = [torch.empty(n, dtype=torch.float32, device=s.device) for s, n in zip(streams, ns)]
+    tab = _batch_table(outs, ebs, bss, streams, blks)
+    _check(lib().tszp_plain_decompress_batch(tab, len(outs), _stream()))
+    return outs
 
 
 def elem_buffers(n, bs, T, elem_bytes, ra=False, device="cuda"):
