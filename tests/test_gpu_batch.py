@@ -87,6 +87,25 @@ def test_batch_large_and_small_fields_together():
         assert float((outs[i] - f).abs().max()) <= ebs[i] * 1.0001
 
 
+def test_batch_aligned_and_unaligned_inputs():
+    """Block sizes 32 and 64 on 16-byte aligned inputs take the 16-byte path, the same
+    sizes on inputs 4 bytes off take the generic one; both give the oracle's bytes."""
+    torch = _torch()
+    base = [fields.blobs2d(37, 53 + k, seed=20 + k).ravel() for k in range(6)]
+    dev, cases = [], []
+    for k, data in enumerate(base):
+        t = torch.from_numpy(data).cuda()
+        if k % 2:
+            t = t[1:]  # 4 bytes past an aligned allocation
+            data = data[1:]
+        dev.append(t)
+        cases.append((data, 2e-3, 64 if k < 3 else 32))
+    sizes, streams, blks = tz.plain_compress_batch(dev, [c[1] for c in cases], [c[2] for c in cases])
+    for i, (data, eb, bs) in enumerate(cases):
+        assert (dev[i].data_ptr() % 16 == 0) == (i % 2 == 0)
+        _eq(streams[i][:sizes[i]].cpu().numpy(), oracle.plain_compress(data, eb, bs, T=1), f"field {i} stream")
+
+
 def _payload(data, eb, bs):
     return len(oracle.plain_compress(data, eb, bs, T=1)) - 12
 

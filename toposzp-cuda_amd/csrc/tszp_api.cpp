@@ -747,9 +747,17 @@ namespace {
 
 struct BatchClass {
   uint32_t bs;      // a block size of the class (selects the kernel geometry)
+  bool v4;          // compression by the 16-byte path
   uint32_t base;    // first tile in the batch's numbering
   uint32_t ntiles;
 };
+
+// Compression of the field by the 16-byte path (k_compress_v4): block size 32 or 64 and
+// a 16-byte aligned input, as for one field; its tiles hold the same blocks.
+bool batch_v4(const tszp_batch_field& f, bool compress) {
+  return compress && (f.block_size == 32 || f.block_size == 64) && !((uintptr_t)f.d_field & 15u) &&
+         !getenv("TSZP_CODEC_GENERIC");
+}
 
 // Validates the fields, orders them by block-size class, numbers their tiles, uploads
 // the table and the tile -> field map.  order[i] is the caller's index of table entry i.
@@ -777,7 +785,7 @@ int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress
     int S, EPL;
     uint32_t bpt;
     block_geometry((uint32_t)f.block_size, &S, &EPL, &bpt);
-    key[i] = {((uint64_t)S << 32) | (uint32_t)EPL, i};
+    key[i] = {((uint64_t)batch_v4(f, compress) << 40) | ((uint64_t)S << 16) | (uint32_t)EPL, i};
   }
   std::stable_sort(key.begin(), key.end(),
                    [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
@@ -797,7 +805,8 @@ int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress
     int S, EPL;
     uint32_t bpt;
     block_geometry((uint32_t)f.block_size, &S, &EPL, &bpt);
-    if (k == 0 || key[k].first != key[k - 1].first) classes->push_back({(uint32_t)f.block_size, (uint32_t)tiles, 0});
+    if (k == 0 || key[k].first != key[k - 1].first)
+      classes->push_back({(uint32_t)f.block_size, batch_v4(f, compress), (uint32_t)tiles, 0});
     BatchField b{};
     b.n = (uint32_t)f.n;
     b.bs = (uint32_t)f.block_size;
@@ -875,7 +884,7 @@ int tszp_plain_compress_batch(tszp_batch_field* fields, uint32_t count, void* st
   HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4, st));
   ProfScope prof(kProfBatch, st);
   for (const BatchClass& k : classes)
-    HIPCHK(launch_compress_batch(k.bs, tab, tmap, k.base, k.ntiles, status, res, st));
+    HIPCHK(launch_compress_batch(k.bs, k.v4, tab, tmap, k.base, k.ntiles, status, res, st));
   prof.stop();
   CompressResult* h;
   if ((r = batch_results(c, res, count, st, &h))) return r;

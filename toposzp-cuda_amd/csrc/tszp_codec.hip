@@ -556,7 +556,7 @@ __device__ __forceinline__ int type_minmax(float c, const float (&lo)[ND], const
 }
 
 template <int MODE, int S, int ND>
-__global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a) {
+__device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t tile_in) {
   constexpr int EPL = 4, BS = 4 * S;
   using Gm = Geom<S, EPL>;
   constexpr int G = Gm::G, SPW = Gm::SPW, BPT = Gm::BPT;
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   const int g = lane / S, ls = lane & (S - 1);
   const uint32_t j0 = 4u * (uint32_t)ls;  // first element of the lane within its block
 
-  if (tid == 0) sh_tile = tile_id(a.tile_counter);
+  if (tid == 0) sh_tile = a.tile_counter ? atomicAdd(a.tile_counter, 1u) : tile_in;
   for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
@@ -912,6 +912,11 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
   }
 }
 
+template <int MODE, int S, int ND>
+__global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a) {
+  compress_v4_tile<MODE, S, ND>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // batches of plain float fields: every field of one block-size class in one launch.
 // Workgroup b takes tile base + b of the batch's tile numbering; tmap names the tile's
@@ -926,11 +931,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_tilemap(const BatchFiel
   for (uint32_t t = threadIdx.x; t < nt; t += kBlockThreads) tmap[t0 + t] = fi;
 }
 
-template <int S, int EPL>
-__global__ __launch_bounds__(kBlockThreads) void k_compress_batch(const BatchField* __restrict__ fields,
-                                                                 const uint32_t* __restrict__ tmap, uint32_t base,
-                                                                 uint64_t* __restrict__ status,
-                                                                 CompressResult* __restrict__ res) {
+template <int S, int EPL, bool V4>
+__global__ __launch_bounds__(kBlockThreads, V4 ? 4 : 1) void k_compress_batch(const BatchField* __restrict__ fields,
+                                                                            const uint32_t* __restrict__ tmap,
+                                                                            uint32_t base,
+                                                                            uint64_t* __restrict__ status,
+                                                                            CompressResult* __restrict__ res) {
   const uint32_t gt = base + blockIdx.x;
   const uint32_t fi = tmap[gt];
   const BatchField& f = fields[fi];
@@ -954,7 +960,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_compress_batch(const BatchFie
     h[1] = 0u;
     h[2] = 0u;
   }
-  compress_tile<kModePlain, S, EPL>(a, tile);
+  if constexpr (V4) {
+    compress_v4_tile<kModePlain, S, 2>(a, tile);
+  } else {
+    compress_tile<kModePlain, S, EPL>(a, tile);
+  }
 }
 
 template <int S, int EPL>
@@ -1154,16 +1164,28 @@ hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32
   return hipGetLastError();
 }
 
-hipError_t launch_compress_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
-                                 uint32_t ntiles, uint64_t* status, CompressResult* res, hipStream_t st) {
+hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
+                                 uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
+                                 hipStream_t st) {
   if (!ntiles) return hipSuccess;
+  if (v4) {  // block size exactly 32 or 64, every input of the class 16-byte aligned
+    if (bs_class == 64)
+      hipLaunchKernelGGL((k_compress_batch<16, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
+                         base, status, res);
+    else if (bs_class == 32)
+      hipLaunchKernelGGL((k_compress_batch<8, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
+                         base, status, res);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   int S, EPL;
   uint32_t bpt;
   block_geometry(bs_class, &S, &EPL, &bpt);
 #define TSZP_CASE(SS, EE)                                                                            \
   if (S == SS && EPL == EE) {                                                                        \
-    hipLaunchKernelGGL((k_compress_batch<SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, base, \
-                       status, res);                                                                 \
+    hipLaunchKernelGGL((k_compress_batch<SS, EE, false>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, \
+                       base, status, res);                                                           \
     return hipGetLastError();                                                                        \
   }
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)

@@ -66,9 +66,11 @@ struct BatchField {
   double inv;             // compress: reciprocal used for quantization
 };
 hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32_t* tmap, hipStream_t st);
-// tiles [base, base + ntiles) of the batch's numbering, all of one block-size class
-hipError_t launch_compress_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
-                                 uint32_t ntiles, uint64_t* status, CompressResult* res, hipStream_t st);
+// tiles [base, base + ntiles) of the batch's numbering, all of one block-size class;
+// v4: the 16-byte path (block size exactly 32 or 64, every input 16-byte aligned)
+hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
+                                 uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
+                                 hipStream_t st);
 hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
                                uint32_t ntiles, CompressResult* res, hipStream_t st);
 // Estimated payload bytes of every field at each block size of kTuneSizes, from
