@@ -274,11 +274,123 @@ def batch_side(tz, torch, seed, reps):
            "decompress_hbm_frac": (4 * total + cbytes) / (dk * 1e-3) / 1e9 / HBM_PEAK_GBS if dk else None,
            "identical_to_loop": bool(same), "max_err_over_eb": err}
     res["speedup_vs_loop"] = res["batch_gbs"] / res["loop_gbs"]
+    res["_pool"] = {"fields": fields, "ebs": ebs, "bss": bss}
     log(f"batch of {len(fields)} fields ({total / 1e6:.1f} M elements, ratio {res['ratio']:.2f}, block sizes "
         f"{res['block_sizes']}): batched {res['batch_gbs']:.1f} GB/s (p50 {res['batch_p50_ms']:.2f} ms, p99 "
         f"{res['batch_p99_ms']:.2f} ms; kernels {ck:.3f} + {dk:.3f} ms), loop of one-field calls "
         f"{res['loop_gbs']:.1f} GB/s (p50 {res['loop_p50_ms']:.2f} ms) -> {res['speedup_vs_loop']:.1f}x, "
         f"identical {same}, max err {err:.4f} eb, tune {tune_ms:.2f} ms")
+    return res
+
+
+def serve_side(tz, torch, pool, big, seed, nreq, rate):
+    """Served stream: nreq requests with Poisson arrivals at `rate` per second, 70 %
+    compressions of a pool field (the 256 mixed-size fields, or with probability 2 % the
+    512^3 field) and 30 % decompressions of a field compressed earlier, submitted to
+    the request loop (admission by a 200 GB HBM budget, continuous batching) and,
+    separately, run one request at a time with the one-field calls at the same arrival
+    times.  Latency = submission (or arrival) to completion."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    fields = pool["fields"] + [big["field"]]
+    ebs = pool["ebs"] + [big["eb"]]
+    bss = pool["bss"] + [big["bs"]]
+    nbig = len(fields) - 1
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    kinds, which = [], []
+    for _ in range(nreq):
+        if rng.random() < 0.7 or not kinds:
+            kinds.append(0)
+            which.append(nbig if rng.random() < 0.02 else int(rng.integers(0, nbig)))
+        else:
+            kinds.append(1)
+            done = [w for k, w in zip(kinds, which) if k == 0]
+            which.append(done[int(rng.integers(0, len(done)))])
+    outs = [torch.empty_like(f) for f in fields]
+    nbytes = sum(4 * fields[w].numel() for w in which)
+    budget = 200 << 30
+    try:
+        srv = tz.Server(budget, max_batch=256)
+    except tz.TszpError:
+        budget = 64 << 30
+        srv = tz.Server(budget, max_batch=256)
+    lock = threading.Lock()
+    latest = {}                 # field -> its latest finished compress ticket
+    tickets = [None] * nreq
+    ready = threading.Semaphore(0)
+    lat = np.zeros(nreq)
+
+    def waiter():
+        for i in range(nreq):
+            ready.acquire()
+            info = srv.wait(tickets[i])
+            lat[i] = info.latency_us
+            with lock:
+                if kinds[i] == 0:
+                    old = latest.get(which[i])
+                    latest[which[i]] = tickets[i]
+                    if old is not None:
+                        srv.release(old)
+                else:
+                    srv.release(tickets[i])
+
+    torch.cuda.synchronize()
+    wt = threading.Thread(target=waiter)
+    wt.start()
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        with lock:
+            if kinds[i] == 0:
+                tickets[i] = srv.compress(fields[which[i]], ebs[which[i]], bss[which[i]])
+            else:
+                src = latest.get(which[i])
+                if src is None:  # its compression has not finished yet: compress again
+                    kinds[i] = 0
+                    tickets[i] = srv.compress(fields[which[i]], ebs[which[i]], bss[which[i]])
+                else:
+                    tickets[i] = srv.decompress(src, outs[which[i]])
+        ready.release()
+    wt.join()
+    served_s = time.perf_counter() - t0
+    st = srv.stats()
+    with lock:
+        for t in latest.values():
+            srv.release(t)
+    srv.close()
+
+    # the same requests one at a time with the one-field calls
+    streams = {}
+    lat1 = np.zeros(nreq)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    free_at = 0.0
+    for i in range(nreq):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        w = which[i]
+        if kinds[i] == 0 or w not in streams:
+            nb, sbuf, blk = tz.plain_compress(fields[w], ebs[w], bss[w], 1)
+            streams[w] = (sbuf, blk)
+        else:
+            sbuf, blk = streams[w]
+            tz.plain_decompress(sbuf, blk, fields[w].numel(), ebs[w], bss[w], 1, out=outs[w])
+        torch.cuda.synchronize()
+        lat1[i] = 1e6 * (time.perf_counter() - t0 - arrivals[i])
+    loop_s = time.perf_counter() - t0
+    res = {"requests": nreq, "rate_per_s": rate, "compress_share": float(np.mean(np.array(kinds) == 0)),
+           "big_fields": int(sum(1 for k, w in zip(kinds, which) if w == nbig)), "input_gb": nbytes / 1e9,
+           "hbm_budget_gb": budget / 2 ** 30, "hbm_peak_gb": st["hbm_peak"] / 2 ** 30, "batches": st["batches"],
+           "served_gbs": nbytes / served_s / 1e9, "served_p50_ms": float(np.percentile(lat, 50)) / 1e3,
+           "served_p99_ms": float(np.percentile(lat, 99)) / 1e3,
+           "loop_gbs": nbytes / loop_s / 1e9, "loop_p50_ms": float(np.percentile(lat1, 50)) / 1e3,
+           "loop_p99_ms": float(np.percentile(lat1, 99)) / 1e3}
+    log(f"served {nreq} requests at {rate:.0f}/s ({nbytes / 1e9:.1f} GB, {st['batches']} batches, HBM peak "
+        f"{res['hbm_peak_gb']:.2f} GiB): {res['served_gbs']:.1f} GB/s, p50 {res['served_p50_ms']:.2f} ms, "
+        f"p99 {res['served_p99_ms']:.2f} ms; one request at a time: {res['loop_gbs']:.1f} GB/s, "
+        f"p50 {res['loop_p50_ms']:.2f} ms, p99 {res['loop_p99_ms']:.2f} ms")
     return res
 
 
@@ -296,7 +408,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
     ap.add_argument("--no-elem", action="store_true", help="leave out the threadblock / double side figures")
-    ap.add_argument("--no-batch", action="store_true", help="leave out the mixed-size batch side figure")
+    ap.add_argument("--no-batch", action="store_true", help="leave out the mixed-size batch side figures")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the served-stream side figure")
+    ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--serve-rate", type=float, default=40000.0, help="Poisson arrival rate per second")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -413,6 +528,13 @@ def main():
 
     # side figure: 256 mixed-size fields, batched against a loop of one-field calls
     batch = None if args.no_batch else batch_side(tz, torch, args.seed, 12)
+    served = None
+    if batch is not None:
+        pool = batch.pop("_pool")
+        if not args.no_serve:
+            served = serve_side(tz, torch, pool, {"field": f.view(-1), "eb": eb, "bs": bs}, args.seed,
+                                args.serve_requests, args.serve_rate)
+        del pool
 
     # side figure: topology quality of the restored field against the original
     # (the CUDA driver's report), one GPU pass
@@ -478,6 +600,7 @@ def main():
         "topo_stream_parse_ms": parse_ms,
         "elem_codecs": elem,
         "batched_plain": batch,
+        "served_plain": served,
         "quality_ms": quality_ms,
         "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
                                             "over_1eb", "pairs", "violations", "saddles_above",

@@ -109,6 +109,39 @@ int tszp_plain_decompress_batch(const tszp_batch_field *fields, uint32_t count, 
  * estimate (64 on a tie or for fields shorter than 128). */
 int tszp_batch_tune(tszp_batch_field *fields, uint32_t count, void *stream);
 
+/* Request loop over the batch entry points (no counterpart in the reference).  Requests
+ * from any host thread are admitted in arrival order while their streams fit the HBM
+ * budget (one device arena of hbm_budget bytes, allocated at creation); every request
+ * waiting when a batch finishes joins the next one, up to max_batch.  Inputs must be
+ * complete on the device when submitted and stay valid until the ticket finishes;
+ * outputs are valid after tszp_server_wait.  A compress ticket's stream and side-car
+ * stay in the arena until tszp_server_release; a compression that does not fit waits
+ * for releases (one larger than the budget fails).  Release every ticket when done
+ * with it.  Errors: tszp_server_last_error(). */
+typedef struct tszp_server tszp_server;
+typedef struct {
+  uint64_t stream_bytes;  /* compress tickets: [f32 eb big-endian][u64 0][blocks] */
+  double queue_us;        /* submission to the start of its batch */
+  double latency_us;      /* submission to the end of its batch */
+} tszp_ticket_info;
+typedef struct {
+  uint64_t batches, requests, queued, hbm_in_use, hbm_peak, hbm_budget;
+} tszp_server_stats_t;
+const char *tszp_server_last_error(void);
+int tszp_server_create(uint64_t hbm_budget, uint32_t max_batch, tszp_server **out);
+int tszp_server_destroy(tszp_server *server);  /* serves what is queued, then stops */
+int tszp_server_compress(tszp_server *server, const float *d_field, uint64_t n, float eb, int block_size,
+                         uint64_t *ticket);
+int tszp_server_decompress(tszp_server *server, uint64_t compress_ticket, float *d_out, uint64_t *ticket);
+int tszp_server_wait(tszp_server *server, uint64_t ticket, tszp_ticket_info *info);
+int tszp_server_stream(tszp_server *server, uint64_t compress_ticket, const uint8_t **d_stream,
+                       const uint32_t **d_blk, uint64_t *stream_bytes, int *block_size);
+/* copies a finished compress ticket's stream into d_dst (cap bytes), ordered on `stream` */
+int tszp_server_copy_stream(tszp_server *server, uint64_t compress_ticket, uint8_t *d_dst, uint64_t cap,
+                            void *stream);
+int tszp_server_release(tszp_server *server, uint64_t ticket);
+int tszp_server_stats(tszp_server *server, tszp_server_stats_t *out);
+
 /* Streams partitioned by elements: chunk t of nchunks covers [t * (n / nchunks), ...),
  * the last chunk up to n; header [eb big-endian, elem_bytes wide][u64 off[nchunks]].
  * d_blk receives tszp_elem_units(n, block_size, nchunks, ra) + 1 unit offsets relative

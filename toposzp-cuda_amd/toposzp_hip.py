@@ -46,6 +46,16 @@ class BatchField(C.Structure):
                 ("stream_bytes", C.c_uint64)]
 
 
+class TicketInfo(C.Structure):
+    """tszp_ticket_info (include/toposzp_hip.h)."""
+    _fields_ = [("stream_bytes", C.c_uint64), ("queue_us", C.c_double), ("latency_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    """tszp_server_stats_t (include/toposzp_hip.h)."""
+    _fields_ = [(k, C.c_uint64) for k in ("batches", "requests", "queued", "hbm_in_use", "hbm_peak", "hbm_budget")]
+
+
 _lib = None
 _libc = C.CDLL("libc.so.6")
 _libc.free.argtypes = [C.c_void_p]
@@ -91,6 +101,20 @@ def lib():
     L.tszp_sortpos_decompress.argtypes = [vp, vp, u32, sz, i32, vp, vp]
     L.tszp_plain_compress.argtypes = [vp, sz, f32, i32, u32, vp, sz, vp, u64p, vp]
     L.tszp_plain_decompress.argtypes = [vp, vp, u32, sz, f32, i32, vp, vp]
+    L.tszp_server_last_error.restype = C.c_char_p
+    L.tszp_server_create.argtypes = [C.c_uint64, u32, C.POINTER(vp)]
+    L.tszp_server_destroy.argtypes = [vp]
+    L.tszp_server_compress.argtypes = [vp, vp, C.c_uint64, f32, i32, u64p]
+    L.tszp_server_decompress.argtypes = [vp, C.c_uint64, vp, u64p]
+    L.tszp_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(TicketInfo)]
+    L.tszp_server_stream.argtypes = [vp, C.c_uint64, C.POINTER(vp), C.POINTER(vp), u64p, C.POINTER(i32)]
+    L.tszp_server_release.argtypes = [vp, C.c_uint64]
+    L.tszp_server_copy_stream.argtypes = [vp, C.c_uint64, vp, C.c_uint64, vp]
+    L.tszp_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    for f in ("tszp_server_create", "tszp_server_destroy", "tszp_server_compress", "tszp_server_decompress",
+              "tszp_server_wait", "tszp_server_stream", "tszp_server_release", "tszp_server_stats",
+              "tszp_server_copy_stream"):
+        getattr(L, f).restype = i32
     for f in ("tszp_plain_compress_batch", "tszp_plain_decompress_batch", "tszp_batch_tune"):
         getattr(L, f).argtypes = [C.POINTER(BatchField), u32, vp]
         getattr(L, f).restype = i32
@@ -438,6 +462,64 @@ def plain_decompress_batch(streams, blks, ns, ebs, bss, outs=None):
     tab = _batch_table(outs, ebs, bss, streams, blks)
     _check(lib().tszp_plain_decompress_batch(tab, len(outs), _stream()))
     return outs
+
+
+class Server:
+    """Request loop over the batch codec (tszp_server_*): submit from any thread, wait
+    for tickets, release them when done.  Inputs must be complete on the device when
+    submitted (the caller synchronises its stream first)."""
+
+    def __init__(self, hbm_budget, max_batch=256):
+        self._h = C.c_void_p()
+        self._chk(lib().tszp_server_create(int(hbm_budget), int(max_batch), C.byref(self._h)))
+
+    @staticmethod
+    def _chk(rc):
+        if rc != 0:
+            raise TszpError(lib().tszp_server_last_error().decode())
+
+    def compress(self, field, eb, bs):
+        t = C.c_uint64()
+        self._chk(lib().tszp_server_compress(self._h, _dp(field), field.numel(), float(eb), int(bs), C.byref(t)))
+        return t.value
+
+    def decompress(self, ticket, out):
+        t = C.c_uint64()
+        self._chk(lib().tszp_server_decompress(self._h, int(ticket), _dp(out), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        info = TicketInfo()
+        self._chk(lib().tszp_server_wait(self._h, int(ticket), C.byref(info)))
+        return info
+
+    def stream(self, ticket):
+        """(device address of the stream, of its side-car, stream bytes, block size)."""
+        ds, db, nb, bs = C.c_void_p(), C.c_void_p(), C.c_uint64(), C.c_int()
+        self._chk(lib().tszp_server_stream(self._h, int(ticket), C.byref(ds), C.byref(db), C.byref(nb),
+                                           C.byref(bs)))
+        return ds.value, db.value, nb.value, bs.value
+
+    def stream_copy(self, ticket):
+        """The stream of a finished compress ticket copied into a new uint8 CUDA tensor."""
+        torch = _torch()
+        _, _, nb, _ = self.stream(ticket)
+        out = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        self._chk(lib().tszp_server_copy_stream(self._h, int(ticket), _dp(out), nb, _stream()))
+        return out
+
+    def release(self, ticket):
+        self._chk(lib().tszp_server_release(self._h, int(ticket)))
+
+    def stats(self):
+        st = ServerStats()
+        self._chk(lib().tszp_server_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            self._chk(lib().tszp_server_destroy(self._h))
+            self._h = C.c_void_p()
 
 
 def elem_buffers(n, bs, T, elem_bytes, ra=False, device="cuda"):
