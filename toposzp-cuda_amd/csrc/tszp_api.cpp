@@ -242,8 +242,10 @@ int codec_prepare(Ctx* c, uint32_t ntiles, uint64_t** st_bytes, uint64_t** st_cn
   WS(kWsStatus, words * 8, s);
   CompressResult* r;
   WS(kWsResult, sizeof(CompressResult), r);
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(s), 0u, words * 2, st));
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(r), 0u, sizeof(CompressResult) / 4, st));
+  FillRanges fr{};
+  fr.add(reinterpret_cast<uint32_t*>(s), 0u, words * 2);
+  fr.add(reinterpret_cast<uint32_t*>(r), 0u, sizeof(CompressResult) / 4);
+  HIPCHK(launch_fill_multi(fr, st));
   *st_bytes = s;
   *st_cnt = s + ntiles;
   *counter = reinterpret_cast<uint32_t*>(s + 2 * ntiles);
@@ -880,8 +882,10 @@ int tszp_plain_compress_batch(tszp_batch_field* fields, uint32_t count, void* st
   CompressResult* res;
   WS(kWsBatchSt, (size_t)tiles * 8 + 8, status);
   WS(kWsBatchRes, (size_t)count * sizeof(CompressResult), res);
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, (size_t)tiles * 2, st));
-  HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4, st));
+  FillRanges fr{};
+  fr.add(reinterpret_cast<uint32_t*>(status), 0u, (size_t)tiles * 2);
+  fr.add(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4);
+  HIPCHK(launch_fill_multi(fr, st));
   ProfScope prof(kProfBatch, st);
   for (const BatchClass& k : classes)
     HIPCHK(launch_compress_batch(k.bs, k.v4, tab, tmap, k.base, k.ntiles, status, res, st));
@@ -1450,8 +1454,10 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   const bool vec_mark = (ndim == 2 ? dims[1] : dims[2]) % 4 == 0;
   uint8_t* badmap = nullptr;
   auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint8_t* bmap = nullptr) -> int {
-    HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
-    HIPCHK(launch_fill_u32(misc + kPpVerify, 0u, 8, st));
+    FillRanges fr{};
+    fr.add(misc + kPpBad, 0u, 1);
+    fr.add(misc + kPpVerify, 0u, 8);
+    HIPCHK(launch_fill_multi(fr, st));
     HIPCHK(pp_mark(g, d_field, d_types, final_pass ? d : nullptr, N, bad, cap, part, bmap, misc, st));
     int rr = read_misc();
     if (rr) return rr;
@@ -1476,7 +1482,6 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   uint32_t tiny = 0;
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
   if (nb && nb <= cap) {
-    HIPCHK(launch_fill_u32(misc + kPpTiny, 0u, 1, st));
     uint4* plist;
     uint16_t* rel;
     WS(kWsPpPlist, (size_t)nb * 16, plist);
@@ -1485,10 +1490,15 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(badmap), 0u, (n + 3) / 4, st));
       HIPCHK(pp_badmap(bad, nb, badmap, st));
     }
+    // the tiny pass touches neither the claim bits nor the mismatch count, so both are
+    // cleared for the re-marking that follows it in the same launch
+    FillRanges fr{};
+    fr.add(misc + kPpTiny, 0u, 1);
+    fr.add(misc + kPpBad, 0u, 1);
+    fr.add(bits, 0u, bit_words);
+    HIPCHK(launch_fill_multi(fr, st));
     HIPCHK(pp_rel(g, d, N, rel, st));
     HIPCHK(pp_tiny(g, bad, nb, d, rel, badmap, d_types, d_field, eb, use_sp, plist, part, misc, st));
-    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
-    HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
     HIPCHK(pp_remark(g, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
     if ((r = read_misc())) return r;
     std::swap(bad, bad2);
@@ -1510,9 +1520,11 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       // a closed neighbourhood holds at most one point of each colour class, so every
       // colour list has room for one entry per listed mismatch
       WS(kWsPpAct, (size_t)nl * colours * 4 + 16, act);
-      HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
-      HIPCHK(launch_fill_u32(misc + kPpMoves, 0u, 1, st));
-      HIPCHK(launch_fill_u32(misc + kPpActC, 0u, colours, st));
+      FillRanges fr{};
+      fr.add(bits, 0u, bit_words);
+      fr.add(misc + kPpMoves, 0u, 1);
+      fr.add(misc + kPpActC, 0u, colours);
+      HIPCHK(launch_fill_multi(fr, st));
       HIPCHK(pp_active(g, bad, nl, d, d_types, bits, act, nl, use_sp, misc, st));
       for (int col = 0; col < colours; col++)
         HIPCHK(pp_move(g, act, nl, nl, col, d, d_types, d_field, eb, misc, st));
@@ -1521,8 +1533,10 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       // re-check around the listed mismatches, or the whole field when that is cheaper
       const uint64_t ring = ndim == 2 ? 13 : 25;
       if (complete && (uint64_t)nl * ring < n / 4) {
-        HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
-        HIPCHK(launch_fill_u32(misc + kPpBad, 0u, 1, st));
+        FillRanges fr{};
+        fr.add(bits, 0u, bit_words);
+        fr.add(misc + kPpBad, 0u, 1);
+        HIPCHK(launch_fill_multi(fr, st));
         HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, 2, misc, st));
         if ((r = read_misc())) return r;
         std::swap(bad, bad2);
@@ -1553,12 +1567,16 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     const int phases = 1 << ndim;
     WS(kWsPpTBits, (ntiles / 32 + 4) * 4, tbits);
     WS(kWsPpTiles, (size_t)tcap * phases * 4 + 16, tiles);
-    HIPCHK(launch_fill_u32(bits, 0u, bit_words, st));
-    HIPCHK(launch_fill_u32(tbits, 0u, ntiles / 32 + 4, st));
-    HIPCHK(launch_fill_u32(misc + kPpSolved, 0u, 2, st));
-    HIPCHK(launch_fill_u32(misc + kPpFreed, 0u, 1, st));
-    HIPCHK(launch_fill_u32(misc + kPpTilesP, 0u, 8, st));
-    HIPCHK(launch_fill_u32(misc + kPpStatNodes, 0u, 12, st));
+    {
+      FillRanges fr{};
+      fr.add(bits, 0u, bit_words);
+      fr.add(tbits, 0u, ntiles / 32 + 4);
+      fr.add(misc + kPpSolved, 0u, 2);
+      fr.add(misc + kPpFreed, 0u, 1);
+      fr.add(misc + kPpTilesP, 0u, 8);
+      fr.add(misc + kPpStatNodes, 0u, 12);
+      HIPCHK(launch_fill_multi(fr, st));
+    }
     HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
@@ -1575,8 +1593,10 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     } else if (total) {  // every phase in one launch, ordered per neighbouring tiles
       uint32_t* done;
       WS(kWsPpDone, ntiles * 4 + 16, done);
-      HIPCHK(launch_fill_u32(done, 0u, ntiles, st));
-      HIPCHK(launch_fill_u32(misc + kPpFlowCounter, 0u, 1, st));
+      FillRanges fr{};
+      fr.add(done, 0u, ntiles);
+      fr.add(misc + kPpFlowCounter, 0u, 1);
+      HIPCHK(launch_fill_multi(fr, st));
       solve_timer.begin(st);
       HIPCHK(pp_solve_flow(g, tiles, total, tcap, tx, ty, tz, tbits, done, bits, d, d_types, d_field, eb, use_sp,
                            misc, st));

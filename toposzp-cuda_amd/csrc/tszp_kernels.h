@@ -164,6 +164,21 @@ hipError_t launch_cp_keys(const int32_t* cps, uint32_t count, int rec, const flo
 
 // ---- helpers (tszp_util.hip) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t st);
+// up to kFillRanges (pointer, value, count) ranges in one launch
+constexpr int kFillRanges = 8;
+struct FillRanges {
+  uint32_t* p[kFillRanges];
+  uint32_t v[kFillRanges];
+  size_t count[kFillRanges];
+  int n;
+  void add(uint32_t* ptr, uint32_t val, size_t cnt) {
+    p[n] = ptr;
+    v[n] = val;
+    count[n] = cnt;
+    n++;
+  }
+};
+hipError_t launch_fill_multi(const FillRanges& r, hipStream_t st);
 hipError_t launch_minmax(const float* f, size_t n, uint32_t* out2 /* okey min, okey max */,
                          hipStream_t st);
 hipError_t launch_write_header(uint8_t* stream, int kind, unsigned T, int adaptive, uint32_t nb,

@@ -19,6 +19,28 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t st
   return hipGetLastError();
 }
 
+// several ranges in one launch: blockIdx.y picks the range
+__global__ void k_fill_multi(FillRanges r) {
+  const uint32_t k = blockIdx.y;
+  uint32_t* p = r.p[k];
+  const uint32_t v = r.v[k];
+  const size_t count = r.count[k];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_multi(const FillRanges& r, hipStream_t st) {
+  if (r.n < 1 || r.n > kFillRanges) return hipErrorInvalidValue;
+  size_t most = 0;
+  for (int k = 0; k < r.n; k++) most = r.count[k] > most ? r.count[k] : most;
+  if (!most) return hipSuccess;
+  size_t blocks = (most + kBlockThreads - 1) / kBlockThreads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)blocks, (unsigned)r.n), dim3(kBlockThreads), 0, st, r);
+  return hipGetLastError();
+}
+
 // min / max as order-preserving keys (out2 must hold {0xffffffff, 0} on entry)
 __global__ __launch_bounds__(kBlockThreads) void k_minmax(const float* __restrict__ f, size_t n, uint32_t* out2) {
   uint32_t lo = 0xffffffffu, hi = 0;
