@@ -31,6 +31,7 @@ namespace tszp {
 constexpr int kChunkShift = 17;  // key offsets per chunk: kBinChunkWords * 32
 constexpr int kUnroll = 4;       // keys per thread and loop step (independent loads in flight)
 constexpr int kVec = 2;          // uint4 key vectors per thread and loop step
+constexpr int kQueryUnroll = 8;  // critical points per thread of the rank query (random reads in flight)
 
 __device__ __forceinline__ uint32_t comp(const uint4& v, int c) {
   return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
@@ -418,30 +419,31 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_query(const uint32_t* __r
                                                             const uint2* __restrict__ gw, int32_t* __restrict__ out,
                                                             uint32_t out_cap) {
   // consecutive lanes take consecutive critical points: their bitmap words are close
-  const uint32_t base = blockIdx.x * (kBlockThreads * kUnroll) + threadIdx.x;
-  uint32_t pv[kUnroll], kv[kUnroll];
+  constexpr int QU = kQueryUnroll;
+  const uint32_t base = blockIdx.x * (kBlockThreads * QU) + threadIdx.x;
+  uint32_t pv[QU], kv[QU];
 #pragma unroll
-  for (int u = 0; u < kUnroll; u++) {
+  for (int u = 0; u < QU; u++) {
     const uint32_t i = base + u * kBlockThreads;
     pv[u] = i < P ? pay[i] : 0xffffffffu;
     kv[u] = i < P ? key[i] : 0u;
   }
-  uint4 m[kUnroll];
-  uint32_t off[kUnroll];
-  bool live[kUnroll];
+  uint4 m[QU];
+  uint32_t off[QU];
+  bool live[QU];
 #pragma unroll
-  for (int u = 0; u < kUnroll; u++) {
+  for (int u = 0; u < QU; u++) {
     const uint32_t b = (uint32_t)(key_bin(kv[u], eb, inv) - bmin);
     live[u] = pv[u] < out_cap && b < B;  // saddles carry 0xffffffff
     m[u] = live[u] ? bt[b] : make_uint4(0, 0, 0, 1);
     live[u] = live[u] && !m[u].w;  // wide bins: ranked by the radix path
     off[u] = kv[u] - m[u].x;
   }
-  uint2 v[kUnroll];
+  uint2 v[QU];
 #pragma unroll
-  for (int u = 0; u < kUnroll; u++) v[u] = live[u] ? gw[m[u].y + (off[u] >> 5)] : make_uint2(0, 0);
+  for (int u = 0; u < QU; u++) v[u] = live[u] ? gw[m[u].y + (off[u] >> 5)] : make_uint2(0, 0);
 #pragma unroll
-  for (int u = 0; u < kUnroll; u++) {
+  for (int u = 0; u < QU; u++) {
     if (!live[u]) continue;
     uint32_t r = v[u].y + (uint32_t)__popc(v[u].x & ((1u << (off[u] & 31u)) - 1u));
     const uint32_t ci = off[u] >> kChunkShift;
@@ -498,7 +500,7 @@ hipError_t bin_bitmap(uint32_t nchunks, const BinWork& w, uint2* gw, hipStream_t
 hipError_t bin_query(const uint32_t* key, const uint32_t* pay, uint32_t P, float eb, double inv, int bmin, uint32_t B,
                      const BinWork& w, const uint2* gw, int32_t* out, uint32_t out_cap, hipStream_t st) {
   if (!P) return hipSuccess;
-  const uint32_t per = kBlockThreads * kUnroll;
+  const uint32_t per = kBlockThreads * kQueryUnroll;
   hipLaunchKernelGGL(k_bin_query, dim3((P + per - 1) / per), dim3(kBlockThreads), 0, st, key, pay, P, eb, inv, bmin,
                      B, w.bt, w.ctot, gw, out, out_cap);
   return hipGetLastError();
