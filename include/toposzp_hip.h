@@ -130,6 +130,8 @@ typedef struct {
 const char *tszp_server_last_error(void);
 int tszp_server_create(uint64_t hbm_budget, uint32_t max_batch, tszp_server **out);
 int tszp_server_destroy(tszp_server *server);  /* serves what is queued, then stops */
+/* block_size 0: chosen by the tuner of tszp_batch_tune when the request runs; the block
+ * size used is returned by tszp_server_stream */
 int tszp_server_compress(tszp_server *server, const float *d_field, uint64_t n, float eb, int block_size,
                          uint64_t *ticket);
 int tszp_server_decompress(tszp_server *server, uint64_t compress_ticket, float *d_out, uint64_t *ticket);

@@ -123,12 +123,32 @@ def test_request_larger_than_the_budget_fails_alone():
     srv.close()
 
 
+def test_tuned_block_size_matches_the_tuner():
+    """block_size 0: the worker tunes the batch; the stream is the one-field call's at the
+    block size tszp_batch_tune picks for the field."""
+    torch = _torch()
+    cases = _cases(3, 6)
+    dev = [torch.from_numpy(d).cuda() for d, _, _ in cases]
+    ebs = [eb for _, eb, _ in cases]
+    picks = tz.batch_tune(dev, ebs)
+    srv = tz.Server(1 << 28, max_batch=16)
+    tickets = [srv.compress(f, eb, 0) for f, eb in zip(dev, ebs)]
+    for i, t in enumerate(tickets):
+        srv.wait(t)
+        _, _, nb, bs = srv.stream(t)
+        assert bs == picks[i]
+        n1, s, _ = tz.plain_compress(dev[i], ebs[i], bs, T=1)
+        assert nb == n1 and torch.equal(srv.stream_copy(t), s[:n1]), f"request {i}"
+        srv.release(t)
+    srv.close()
+
+
 def test_bad_requests_are_rejected_at_submission():
     torch = _torch()
     srv = tz.Server(1 << 20, max_batch=4)
     f = torch.ones(1000, device="cuda")
     with pytest.raises(tz.TszpError, match="blockSize"):
-        srv.compress(f, 1e-3, 0)
+        srv.compress(f, 1e-3, 1)
     with pytest.raises(tz.TszpError, match="error bound"):
         srv.compress(f, 0.0, 64)
     with pytest.raises(tz.TszpError, match="not a live compress ticket"):

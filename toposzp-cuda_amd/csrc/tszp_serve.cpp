@@ -143,7 +143,7 @@ void tszp_server::run() {
         const uint64_t id = queue.front();
         Request& r = req[id];
         if (r.compress) {
-          const uint64_t need = align_up(r.cap) + align_up(4 * (tszp_blocks(r.n, r.bs) + 1));
+          const uint64_t need = align_up(r.cap) + align_up(4 * (tszp_blocks(r.n, r.bs ? r.bs : 16) + 1));
           if (!arena.alloc(need, &r.off)) {
             if (need > arena.size) {  // can never fit
               r.state = kFailed;
@@ -220,7 +220,10 @@ void tszp_server::run() {
     }
     // the GPU work of the batch, outside the lock: new requests queue meanwhile
     std::string ec, ed;
-    if (!fc.empty() && tszp_plain_compress_batch(fc.data(), (uint32_t)fc.size(), st) != TSZP_OK)
+    bool tune = false;
+    for (const tszp_batch_field& f : fc) tune = tune || f.block_size == 0;
+    if (tune && tszp_batch_tune(fc.data(), (uint32_t)fc.size(), st) != TSZP_OK) ec = tszp_last_error();
+    if (!fc.empty() && ec.empty() && tszp_plain_compress_batch(fc.data(), (uint32_t)fc.size(), st) != TSZP_OK)
       ec = tszp_last_error();
     if (!fd.empty() && tszp_plain_decompress_batch(fd.data(), (uint32_t)fd.size(), st) != TSZP_OK)
       ed = tszp_last_error();
@@ -233,6 +236,7 @@ void tszp_server::run() {
         if (ec.empty()) {
           r.state = kDone;
           r.bytes = fc[k].stream_bytes;
+          r.bs = fc[k].block_size;  // the tuner's choice for block_size 0
         } else {
           r.state = kFailed;
           r.err = ec;
@@ -316,15 +320,19 @@ int tszp_server_compress(tszp_server* s, const float* d_field, uint64_t n, float
   if (!s || !d_field || !ticket) return serve_fail(TSZP_EINVAL, "server_compress: bad arguments");
   if (n == 0 || n >= 0x7fffffffull) return serve_fail(TSZP_EINVAL, "server_compress: element count out of range");
   if (!(eb > 0) || eb != eb || eb > 3.4e38f) return serve_fail(TSZP_EINVAL, "server_compress: bad error bound");
-  if (block_size < 2 || block_size > 1025)
-    return serve_fail(TSZP_EINVAL, "server_compress: blockSize %d out of range [2, 1025]", block_size);
+  if (block_size != 0 && (block_size < 2 || block_size > 1025))
+    return serve_fail(TSZP_EINVAL, "server_compress: blockSize %d out of range [2, 1025] (0: tuned)", block_size);
   Request r;
   r.compress = true;
   r.d_field = const_cast<float*>(d_field);
   r.n = n;
   r.eb = eb;
   r.bs = block_size;
-  r.cap = tszp_plain_bound(n, block_size, 1);
+  // block size 0 is chosen by the tuner among 16, 32, 64 and 128: room for the largest
+  // of their streams (and, at admission, for the side-car of block size 16)
+  r.cap = block_size ? tszp_plain_bound(n, block_size, 1) : 0;
+  for (int c : {16, 32, 64, 128})
+    if (!block_size && tszp_plain_bound(n, c, 1) > r.cap) r.cap = tszp_plain_bound(n, c, 1);
   r.t_submit = now_us();
   {
     std::lock_guard<std::mutex> lk(s->mu);
