@@ -259,8 +259,33 @@ def batch_side(tz, torch, seed, reps):
         loop()
         torch.cuda.synchronize()
         lt.append(time.perf_counter() - t0)
+    # the same batch as a device-resident plan, compress + decompress captured in a graph
+    pstreams = [torch.empty_like(x) for x in streams]
+    pblks = [torch.empty_like(x) for x in blks]
+    pouts = [torch.empty_like(x) for x in outs]
+    plan = tz.BatchPlan(fields, ebs, bss, pstreams, pblks, pouts)
+    plan.compress()
+    plan.decompress()
+    plan.results()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        plan.compress()
+        plan.decompress()
+    graph.replay()
+    torch.cuda.synchronize()
+    gt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        graph.replay()
+        torch.cuda.synchronize()
+        gt.append(time.perf_counter() - t0)
+    psizes = plan.results()
+    gsame = psizes == sizes and all(torch.equal(pstreams[i][:sizes[i]], streams[i][:sizes[i]]) and
+                                    torch.equal(pouts[i], outs[i]) for i in range(len(fields)))
+    plan.close()
+    del graph
     cbytes = sum(sizes)
-    bt, lt = np.array(bt), np.array(lt)
+    bt, lt, gt = np.array(bt), np.array(lt), np.array(gt)
     ck = pr["batch"][0] / max(1, reps)
     dk = pr["batch_decode"][0] / max(1, reps)
     res = {"fields": len(fields), "elements": total, "ratio": 4 * total / cbytes,
@@ -272,14 +297,17 @@ def batch_side(tz, torch, seed, reps):
            "compress_kernels_ms": ck, "decompress_kernels_ms": dk,
            "compress_hbm_frac": (4 * total + cbytes) / (ck * 1e-3) / 1e9 / HBM_PEAK_GBS if ck else None,
            "decompress_hbm_frac": (4 * total + cbytes) / (dk * 1e-3) / 1e9 / HBM_PEAK_GBS if dk else None,
-           "identical_to_loop": bool(same), "max_err_over_eb": err}
+           "identical_to_loop": bool(same), "max_err_over_eb": err,
+           "graph_gbs": 4 * total / float(np.median(gt)) / 1e9, "graph_p50_ms": 1e3 * float(np.percentile(gt, 50)),
+           "graph_p99_ms": 1e3 * float(np.percentile(gt, 99)), "graph_identical": bool(gsame)}
     res["speedup_vs_loop"] = res["batch_gbs"] / res["loop_gbs"]
     res["_pool"] = {"fields": fields, "ebs": ebs, "bss": bss}
     log(f"batch of {len(fields)} fields ({total / 1e6:.1f} M elements, ratio {res['ratio']:.2f}, block sizes "
         f"{res['block_sizes']}): batched {res['batch_gbs']:.1f} GB/s (p50 {res['batch_p50_ms']:.2f} ms, p99 "
         f"{res['batch_p99_ms']:.2f} ms; kernels {ck:.3f} + {dk:.3f} ms), loop of one-field calls "
         f"{res['loop_gbs']:.1f} GB/s (p50 {res['loop_p50_ms']:.2f} ms) -> {res['speedup_vs_loop']:.1f}x, "
-        f"identical {same}, max err {err:.4f} eb, tune {tune_ms:.2f} ms")
+        f"identical {same}, max err {err:.4f} eb, tune {tune_ms:.2f} ms; plan replayed as a graph "
+        f"{res['graph_gbs']:.1f} GB/s (p50 {res['graph_p50_ms']:.2f} ms), identical {gsame}")
     return res
 
 

@@ -109,6 +109,24 @@ int tszp_plain_decompress_batch(const tszp_batch_field *fields, uint32_t count, 
  * estimate (64 on a tie or for fields shorter than 128). */
 int tszp_batch_tune(tszp_batch_field *fields, uint32_t count, void *stream);
 
+/* Batch plans: the layout of a batch (field table, tile map, look-back words, results)
+ * kept on the device for repeated runs over the same fields (pointers, sizes, error
+ * bounds and block sizes fixed; contents may change between runs).  The compress and
+ * decompress calls only launch kernels on `stream` -- no allocation, no host copy, no
+ * synchronisation -- so a caller may capture them in a hipGraph and replay it;
+ * tszp_batch_plan_results waits for the stream and returns the per-field stream sizes
+ * (caller's order) and the first device error of the last runs.  A plan is used by one
+ * thread at a time.  No counterpart in the reference. */
+typedef struct tszp_batch_plan tszp_batch_plan;
+/* d_outs (nullable): decompression output of each field (n floats); NULL decompresses
+ * into d_field */
+int tszp_batch_plan_create(const tszp_batch_field *fields, uint32_t count, float *const *d_outs,
+                           tszp_batch_plan **plan, void *stream);
+int tszp_batch_plan_destroy(tszp_batch_plan *plan);
+int tszp_batch_plan_compress(tszp_batch_plan *plan, void *stream);
+int tszp_batch_plan_decompress(tszp_batch_plan *plan, void *stream);
+int tszp_batch_plan_results(tszp_batch_plan *plan, uint64_t *stream_bytes, void *stream);
+
 /* Request loop over the batch entry points (no counterpart in the reference).  Requests
  * from any host thread are admitted in arrival order while their streams fit the HBM
  * budget (one device arena of hbm_budget bytes, allocated at creation); every request

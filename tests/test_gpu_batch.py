@@ -106,6 +106,51 @@ def test_batch_aligned_and_unaligned_inputs():
         _eq(streams[i][:sizes[i]].cpu().numpy(), oracle.plain_compress(data, eb, bs, T=1), f"field {i} stream")
 
 
+def test_plan_matches_batch_calls_and_replays_in_a_graph():
+    """A plan gives the batch calls' streams and reconstructions; its compress and
+    decompress launches captured in a graph give them again on every replay, with new
+    field contents written eagerly between replays."""
+    torch = _torch()
+    cases = _mixed(7, 20)
+    dev = [torch.from_numpy(d).cuda() for d, _, _ in cases]
+    ebs = [eb for _, eb, _ in cases]
+    bss = [bs for _, _, bs in cases]
+    ns = [d.numel() for d in dev]
+
+    def reference():
+        sizes, streams, blks = tz.plain_compress_batch(dev, ebs, bss)
+        return sizes, streams, blks, tz.plain_decompress_batch(streams, blks, ns, ebs, bss)
+
+    def check(got_sizes, ref):
+        sizes, streams, blks, outs = ref
+        assert got_sizes == sizes
+        for i in range(len(dev)):
+            assert torch.equal(pstreams[i][:sizes[i]], streams[i][:sizes[i]]), f"field {i} stream"
+            assert torch.equal(pblks[i], blks[i]), f"field {i} side-car"
+            assert torch.equal(pouts[i].view(torch.int32), outs[i].view(torch.int32)), f"field {i} decode"
+
+    ref = reference()
+    pstreams = [torch.empty_like(x) for x in ref[1]]
+    pblks = [torch.empty_like(x) for x in ref[2]]
+    pouts = [torch.empty_like(d) for d in dev]
+    plan = tz.BatchPlan(dev, ebs, bss, pstreams, pblks, pouts)
+    plan.compress()
+    plan.decompress()
+    check(plan.results(), ref)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.compress()
+        plan.decompress()
+    for rep in range(3):
+        for d in dev:
+            d.mul_(1.0 + 0.25 * (rep + 1))
+        torch.cuda.synchronize()
+        g.replay()
+        got = plan.results()
+        check(got, reference())
+    plan.close()
+
+
 def _payload(data, eb, bs):
     return len(oracle.plain_compress(data, eb, bs, T=1)) - 12
 

@@ -761,11 +761,10 @@ bool batch_v4(const tszp_batch_field& f, bool compress) {
          !getenv("TSZP_CODEC_GENERIC");
 }
 
-// Validates the fields, orders them by block-size class, numbers their tiles, uploads
-// the table and the tile -> field map.  order[i] is the caller's index of table entry i.
-int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress, const char* what,
-               hipStream_t st, BatchField** d_tab, uint32_t** d_tmap, std::vector<BatchClass>* classes,
-               std::vector<uint32_t>* order, uint32_t* total_tiles) {
+// Validates the fields, orders them by block-size class and numbers their tiles: the
+// table goes to tab (count entries), order[i] is the caller's index of table entry i.
+int batch_layout(const tszp_batch_field* fl, uint32_t count, bool compress, const char* what, BatchField* tab,
+                 std::vector<BatchClass>* classes, std::vector<uint32_t>* order, uint32_t* total_tiles) {
   if (!fl || count == 0) return fail(TSZP_EINVAL, "%s: no fields", what);
   if (count > (1u << 20)) return fail(TSZP_EINVAL, "%s: %u fields, at most 2^20 per batch", what, count);
   std::vector<std::pair<uint64_t, uint32_t>> key(count);  // (geometry class, caller index)
@@ -793,11 +792,6 @@ int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress
                    [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
                      return a.first < b.first;
                    });
-  void* hp;
-  // room for the results too, so that reading them back never reallocates this buffer
-  int r = c->host_batch_get((size_t)count * (sizeof(BatchField) + sizeof(CompressResult)), &hp);
-  if (r) return r;
-  BatchField* tab = static_cast<BatchField*>(hp);
   order->resize(count);
   classes->clear();
   uint64_t tiles = 0;
@@ -835,6 +829,22 @@ int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress
     if (tiles >= 0x7fffffffull) return fail(TSZP_EINVAL, "%s: more than 2^31 tiles in one batch", what);
   }
   *total_tiles = (uint32_t)tiles;
+  return TSZP_OK;
+}
+
+// The layout in the pinned buffer, uploaded with its tile -> field map.
+int batch_plan(Ctx* c, const tszp_batch_field* fl, uint32_t count, bool compress, const char* what,
+               hipStream_t st, BatchField** d_tab, uint32_t** d_tmap, std::vector<BatchClass>* classes,
+               std::vector<uint32_t>* order, uint32_t* total_tiles) {
+  if (!fl || count == 0) return fail(TSZP_EINVAL, "%s: no fields", what);
+  if (count > (1u << 20)) return fail(TSZP_EINVAL, "%s: %u fields, at most 2^20 per batch", what, count);
+  void* hp;
+  // room for the results too, so that reading them back never reallocates this buffer
+  int r = c->host_batch_get((size_t)count * (sizeof(BatchField) + sizeof(CompressResult)), &hp);
+  if (r) return r;
+  BatchField* tab = static_cast<BatchField*>(hp);
+  if ((r = batch_layout(fl, count, compress, what, tab, classes, order, total_tiles))) return r;
+  const uint32_t tiles = *total_tiles;
   WS(kWsBatchTab, (size_t)count * sizeof(BatchField), *d_tab);
   WS(kWsBatchMap, (size_t)tiles * 4, *d_tmap);
   HIPCHK(hipMemcpyAsync(*d_tab, tab, (size_t)count * sizeof(BatchField), hipMemcpyHostToDevice, st));
@@ -979,6 +989,137 @@ int tszp_batch_tune(tszp_batch_field* fields, uint32_t count, void* stream) {
     for (int j = kTuneCands - 1; j >= 0; j--)
       if (hest[(size_t)k * kTuneCands + j] < hest[(size_t)k * kTuneCands + best]) best = j;
     fields[pick[k]].block_size = (int)kTuneSizes[best];
+  }
+  return TSZP_OK;
+}
+
+// ---- batch plans: the layout kept on the device, launches without host work -----------
+struct tszp_batch_plan {
+  int device = 0;
+  uint32_t count = 0, tiles = 0;
+  std::vector<BatchClass> classes;
+  std::vector<uint32_t> order;
+  void* mem = nullptr;  // one device allocation for everything below
+  BatchField *ctab = nullptr, *dtab = nullptr;
+  uint32_t* tmap = nullptr;
+  uint64_t* status = nullptr;
+  CompressResult *cres = nullptr, *dres = nullptr;
+  CompressResult* host = nullptr;  // pinned: 2 * count results
+};
+
+int tszp_batch_plan_create(const tszp_batch_field* fields, uint32_t count, float* const* d_outs,
+                           tszp_batch_plan** out, void* stream) {
+  g_err.clear();
+  if (!out) return fail(TSZP_EINVAL, "batch_plan_create: null output");
+  if (!fields || count == 0) return fail(TSZP_EINVAL, "batch_plan_create: no fields");
+  if (count > (1u << 20)) return fail(TSZP_EINVAL, "batch_plan_create: %u fields, at most 2^20", count);
+  hipStream_t st = (hipStream_t)stream;
+  auto* p = new tszp_batch_plan();
+  std::vector<BatchField> tab(count);
+  int r = batch_layout(fields, count, true, "batch_plan_create", tab.data(), &p->classes, &p->order, &p->tiles);
+  if (r) {
+    delete p;
+    return r;
+  }
+  for (uint32_t i = 0; d_outs && i < count; i++)
+    if (!d_outs[i]) {
+      delete p;
+      return fail(TSZP_EINVAL, "batch_plan_create: field %u: null output", i);
+    }
+  p->count = count;
+  (void)hipGetDevice(&p->device);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t tb = up((size_t)count * sizeof(BatchField)), mb = up((size_t)p->tiles * 4),
+               sb = up((size_t)p->tiles * 8 + 8), rb = up((size_t)count * sizeof(CompressResult));
+  hipError_t e = hipMalloc(&p->mem, 2 * tb + mb + sb + 2 * rb);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&p->host, 2 * (size_t)count * sizeof(CompressResult), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    if (p->mem) (void)hipFree(p->mem);
+    delete p;
+    return fail(TSZP_EHIP, "batch_plan_create: %s", hipGetErrorString(e));
+  }
+  char* m = static_cast<char*>(p->mem);
+  p->ctab = reinterpret_cast<BatchField*>(m);
+  p->dtab = reinterpret_cast<BatchField*>(m + tb);
+  p->tmap = reinterpret_cast<uint32_t*>(m + 2 * tb);
+  p->status = reinterpret_cast<uint64_t*>(m + 2 * tb + mb);
+  p->cres = reinterpret_cast<CompressResult*>(m + 2 * tb + mb + sb);
+  p->dres = reinterpret_cast<CompressResult*>(m + 2 * tb + mb + sb + rb);
+  std::vector<BatchField> dtab(tab);
+  for (uint32_t k = 0; k < count; k++) {  // decode: stream in, field (or its own output) out
+    BatchField& b = dtab[k];
+    b.in = tab[k].out;
+    b.out = d_outs ? static_cast<void*>(d_outs[p->order[k]]) : const_cast<void*>(tab[k].in);
+  }
+  auto fin = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    (void)hipHostFree(p->host);
+    (void)hipFree(p->mem);
+    delete p;
+    return code;
+  };
+  if ((e = hipMemcpyAsync(p->ctab, tab.data(), (size_t)count * sizeof(BatchField), hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(p->dtab, dtab.data(), (size_t)count * sizeof(BatchField), hipMemcpyHostToDevice, st)) ||
+      (e = launch_batch_tilemap(p->ctab, count, p->tmap, st)))
+    return fin(fail(TSZP_EHIP, "batch_plan_create: %s", hipGetErrorString(e)));
+  FillRanges fr{};
+  fr.add(reinterpret_cast<uint32_t*>(p->cres), 0u, (size_t)count * sizeof(CompressResult) / 4);
+  fr.add(reinterpret_cast<uint32_t*>(p->dres), 0u, (size_t)count * sizeof(CompressResult) / 4);
+  if ((e = launch_fill_multi(fr, st)) || (e = hipStreamSynchronize(st)))  // the host vectors die at return
+    return fin(fail(TSZP_EHIP, "batch_plan_create: %s", hipGetErrorString(e)));
+  *out = p;
+  return TSZP_OK;
+}
+
+int tszp_batch_plan_destroy(tszp_batch_plan* plan) {
+  g_err.clear();
+  if (!plan) return fail(TSZP_EINVAL, "batch_plan_destroy: null plan");
+  (void)hipHostFree(plan->host);
+  hipError_t e = hipFree(plan->mem);
+  delete plan;
+  return e == hipSuccess ? TSZP_OK : fail(TSZP_EHIP, "batch_plan_destroy: %s", hipGetErrorString(e));
+}
+
+int tszp_batch_plan_compress(tszp_batch_plan* plan, void* stream) {
+  g_err.clear();
+  if (!plan) return fail(TSZP_EINVAL, "batch_plan_compress: null plan");
+  hipStream_t st = (hipStream_t)stream;
+  FillRanges fr{};
+  fr.add(reinterpret_cast<uint32_t*>(plan->status), 0u, (size_t)plan->tiles * 2);
+  fr.add(reinterpret_cast<uint32_t*>(plan->cres), 0u, (size_t)plan->count * sizeof(CompressResult) / 4);
+  HIPCHK(launch_fill_multi(fr, st));  // launches only: safe under stream capture
+  for (const BatchClass& k : plan->classes)
+    HIPCHK(launch_compress_batch(k.bs, k.v4, plan->ctab, plan->tmap, k.base, k.ntiles, plan->status, plan->cres, st));
+  return TSZP_OK;
+}
+
+int tszp_batch_plan_decompress(tszp_batch_plan* plan, void* stream) {
+  g_err.clear();
+  if (!plan) return fail(TSZP_EINVAL, "batch_plan_decompress: null plan");
+  hipStream_t st = (hipStream_t)stream;
+  FillRanges fr{};
+  fr.add(reinterpret_cast<uint32_t*>(plan->dres), 0u, (size_t)plan->count * sizeof(CompressResult) / 4);
+  HIPCHK(launch_fill_multi(fr, st));
+  for (const BatchClass& k : plan->classes)
+    HIPCHK(launch_decode_batch(k.bs, plan->dtab, plan->tmap, k.base, k.ntiles, plan->dres, st));
+  return TSZP_OK;
+}
+
+int tszp_batch_plan_results(tszp_batch_plan* plan, uint64_t* stream_bytes, void* stream) {
+  g_err.clear();
+  if (!plan) return fail(TSZP_EINVAL, "batch_plan_results: null plan");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(plan->host, plan->cres, (size_t)plan->count * sizeof(CompressResult), hipMemcpyDeviceToHost,
+                        st));
+  HIPCHK(hipMemcpyAsync(plan->host + plan->count, plan->dres, (size_t)plan->count * sizeof(CompressResult),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const CompressResult* hc = plan->host;
+  const CompressResult* hd = plan->host + plan->count;
+  for (uint32_t k = 0; k < plan->count; k++) {
+    if (hc[k].error) return batch_error("batch_plan (compression)", plan->order[k], hc[k].error);
+    if (hd[k].error) return batch_error("batch_plan (decompression)", plan->order[k], hd[k].error);
+    if (stream_bytes) stream_bytes[plan->order[k]] = kBatchHdr + hc[k].payload_bytes;
   }
   return TSZP_OK;
 }

@@ -115,6 +115,14 @@ def lib():
               "tszp_server_wait", "tszp_server_stream", "tszp_server_release", "tszp_server_stats",
               "tszp_server_copy_stream"):
         getattr(L, f).restype = i32
+    L.tszp_batch_plan_create.argtypes = [C.POINTER(BatchField), u32, vp, C.POINTER(vp), vp]
+    L.tszp_batch_plan_destroy.argtypes = [vp]
+    L.tszp_batch_plan_compress.argtypes = [vp, vp]
+    L.tszp_batch_plan_decompress.argtypes = [vp, vp]
+    L.tszp_batch_plan_results.argtypes = [vp, u64p, vp]
+    for f in ("tszp_batch_plan_create", "tszp_batch_plan_destroy", "tszp_batch_plan_compress",
+              "tszp_batch_plan_decompress", "tszp_batch_plan_results"):
+        getattr(L, f).restype = i32
     for f in ("tszp_plain_compress_batch", "tszp_plain_decompress_batch", "tszp_batch_tune"):
         getattr(L, f).argtypes = [C.POINTER(BatchField), u32, vp]
         getattr(L, f).restype = i32
@@ -462,6 +470,39 @@ def plain_decompress_batch(streams, blks, ns, ebs, bss, outs=None):
     tab = _batch_table(outs, ebs, bss, streams, blks)
     _check(lib().tszp_plain_decompress_batch(tab, len(outs), _stream()))
     return outs
+
+
+class BatchPlan:
+    """tszp_batch_plan_*: the layout of a batch kept on the device.  compress() and
+    decompress() only launch kernels on the current stream (capturable in a CUDA/HIP
+    graph); results() synchronises and returns the stream sizes.  fields, streams, side-
+    cars and outputs must stay allocated while the plan lives."""
+
+    def __init__(self, fields, ebs, bss, streams, blks, outs):
+        self._keep = (fields, streams, blks, outs)
+        self.n = len(fields)
+        self._ctab = _batch_table(fields, ebs, bss, streams, blks)
+        if len(outs) != self.n or any(o.numel() != f.numel() for o, f in zip(outs, fields)):
+            raise ValueError("one output per field, of the field's size")
+        self._outs = (C.c_void_p * self.n)(*[o.data_ptr() for o in outs])
+        self._h = C.c_void_p()
+        _check(lib().tszp_batch_plan_create(self._ctab, self.n, self._outs, C.byref(self._h), _stream()))
+
+    def compress(self):
+        _check(lib().tszp_batch_plan_compress(self._h, _stream()))
+
+    def decompress(self):
+        _check(lib().tszp_batch_plan_decompress(self._h, _stream()))
+
+    def results(self):
+        out = (C.c_uint64 * self.n)()
+        _check(lib().tszp_batch_plan_results(self._h, out, _stream()))
+        return list(out)
+
+    def close(self):
+        if self._h:
+            _check(lib().tszp_batch_plan_destroy(self._h))
+            self._h = C.c_void_p()
 
 
 class Server:
