@@ -1,8 +1,9 @@
-"""GPU: the two volume workloads at their full sizes.
+"""GPU: the field workloads at their full sizes.
 
-100 x 500 x 500 (blobs + sine waves + 1e-3 noise, rel 1e-4, bs 64) is compared with the
-oracle byte for byte.  512^3 (fBm, rel 1e-3, bs 64 and 128) is beyond what the oracle
-finishes in seconds, so it is checked by properties that do not depend on the size:
+1800 x 3600 (2D blobs + 0.01 noise, abs 1e-3, bs 64) and 100 x 500 x 500 (blobs + sine
+waves + 1e-3 noise, rel 1e-4, bs 64) are compared with the oracle byte for byte.  512^3
+(fBm, rel 1e-3, bs 64 and 128) is beyond what the oracle finishes in seconds, so it is
+checked by properties that do not depend on the size:
 the type map that comes out of the topology stream is the field's own classification
 (SZp/src/szp_topology_3d.cc:79-98), every decoded value is within the error bound,
 the sort-position stream decodes to the positions the compressor computed, the
@@ -23,6 +24,31 @@ def _torch():
     import torch
     assert torch.cuda.is_available(), "GPU tests need a GPU"
     return torch
+
+
+def test_blob_plane_1800x3600_matches_oracle():
+    torch = _torch()
+    shape = (1800, 3600)
+    f = tz.gen_blobs(shape, 100, 64, 20.0, 200.0, 0.01)
+    eb, bs = 1e-3, 64
+    info, buf = tz.topo_compress(f, eb, bs)
+    data = f.cpu().numpy()
+    cps = oracle.find_cps_2d(data, eb)
+    oracle.sort_positions(cps, data)
+    types = oracle.mark_types(cps, data.shape)
+    sp = oracle.extrema_sort_positions(cps)
+    topo = oracle.topo_compress(data, eb, bs, types, T=1, adaptive=True)
+    sps = oracle.sortpos_compress(sp, bs, T=1, adaptive=True)
+    assert info.cp_count == cps.size and info.ext_count == sp.size
+    assert np.array_equal(buf.topo[:info.topo_bytes].cpu().numpy(), topo), "topology stream"
+    assert np.array_equal(buf.sp[:info.sortpos_bytes].cpu().numpy(), sps), "sort-position stream"
+    out, ty = tz.topo_decompress(buf, data.size, eb, bs)
+    dec, _ = oracle.topo_decompress(topo, data.size, eb, bs)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), dec.view(np.uint32))
+    assert np.array_equal(ty.cpu().numpy(), types.ravel())
+    pos = tz.sortpos_decompress(buf, info.ext_count, bs)
+    r = tz.topo_restore(out, ty, shape, eb, sp=pos, n_ext=info.ext_count, out=torch.empty_like(out))
+    assert r.mismatches() == 0 and r.max_dev_over_eb < 1.0
 
 
 def test_blob_volume_100x500x500_matches_oracle():
