@@ -551,6 +551,33 @@ def main():
     plain_gbs = 4.0 * n / ((time.perf_counter() - t0) / args.steps) / 1e9
     log(f"plain SZp compress+decompress {plain_gbs:.1f} GB/s (ratio {4 * n / nbytes:.2f})")
 
+    # side figure: the same pipeline step at the workload's other block size
+    other_bs = {}
+    for obs in (64, 128):
+        if obs == bs:
+            continue
+        obuf = tz.TopoBuffers(n, obs, 1, dev)
+
+        def ostep():
+            oinfo, _ = tz.topo_compress(f, eb, obs, T=1, buf=obuf)
+            oout, otypes = tz.topo_decompress(obuf, n, eb, obs, T=1)
+            osp = tz.sortpos_decompress(obuf, oinfo.ext_count, obs, T=1) if oinfo.ext_count else None
+            r = tz.topo_restore(oout, otypes, shape, eb, sp=osp, n_ext=oinfo.ext_count, out=restored)
+            return oinfo, r
+        ostep()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            oinfo, r = ostep()
+        torch.cuda.synchronize()
+        oms = 1e3 * (time.perf_counter() - t0) / 3
+        other_bs[str(obs)] = {"ms_per_step": oms, "gbs": 4.0 * n / (oms * 1e-3) / 1e9,
+                              "compression_ratio": 4 * n / (oinfo.topo_bytes + oinfo.sortpos_bytes),
+                              "restore_mismatches": r.mismatches()}
+        log(f"block size {obs}: step {oms:.3f} ms -> {other_bs[str(obs)]['gbs']:.1f} GB/s, "
+            f"ratio {other_bs[str(obs)]['compression_ratio']:.3f}, restore mismatches {r.mismatches()}")
+        del obuf
+
     # side figures: threadblock / double random-access codecs, double 3D topology
     elem = None if args.no_elem else elem_side(tz, torch, f, shape, eb, bs, 3)
 
@@ -625,6 +652,7 @@ def main():
         "end_to_end_hbm_frac": e2e_frac,
         "stage_ms_per_step": {k: v[0] for k, v in stages.items()},
         "plain_gbs": plain_gbs,
+        "other_block_sizes": other_bs,
         "topo_stream_parse_ms": parse_ms,
         "elem_codecs": elem,
         "batched_plain": batch,
