@@ -121,13 +121,16 @@ CASES = [
     ("blobs3d_bs256", lambda: fields.blobs3d(30, 31, 37, seed=31, noise=1e-3), 1e-4, 256, 1),
     ("tiny3d", lambda: fields.blobs3d(3, 3, 3, seed=32, noise=0.1), 1e-3, 64, 1),
     ("const2d", lambda: np.full((40, 50), 1.25, np.float32), 1e-3, 64, 1),
-    # fastest axis a multiple of 4 and block size 32 or 64: 16-byte stencil path
+    # fastest axis a multiple of 4 and block size 32, 64 or 128: 16-byte stencil path
     ("vec2d_bs64", lambda: fields.blobs2d(96, 260, nblobs=24, seed=33), 1e-3, 64, 1),
     ("vec2d_bs32_T5", lambda: fields.blobs2d(77, 132, seed=34, noise=0.03), 1e-3, 32, 5),
     ("vec3d_bs64", lambda: fields.fbm3d(24, 36, 40, seed=35), 3e-3, 64, 1),
     ("vec3d_bs32_T3", lambda: fields.fbm3d(17, 20, 28, seed=36), 2e-3, 32, 3),
     ("vec3d_ragged_tail", lambda: fields.fbm3d(5, 7, 12, seed=37), 2e-3, 64, 1),
     ("vec2d_ties", lambda: fields.ties2d(64, 128, seed=38), 1e-3, 64, 2),
+    ("vec3d_bs128", lambda: fields.fbm3d(20, 24, 64, seed=39), 3e-3, 128, 1),
+    ("vec2d_bs128_T3", lambda: fields.blobs2d(90, 256, seed=40, noise=0.02), 1e-3, 128, 3),
+    ("vec3d_bs128_tail", lambda: fields.fbm3d(7, 9, 20, seed=41), 2e-3, 128, 1),
 ]
 
 
@@ -292,7 +295,8 @@ def test_sort_positions_bin_path_large():
     assert torch.equal(a, buf2.sp_values[:info.ext_count])
 
 
-@pytest.mark.parametrize("shape,bs", [((128, 256, 256), 64), ((40, 96, 100), 32), ((1024, 1028), 64)])
+@pytest.mark.parametrize("shape,bs", [((128, 256, 256), 64), ((40, 96, 100), 32), ((1024, 1028), 64),
+                                      ((128, 256, 256), 128), ((33, 70, 92), 128), ((515, 1004), 128)])
 def test_codec_fast_path_matches_generic(shape, bs):
     """The 16-byte-stencil codec against the generic kernel on the same field: identical
     topology, sort-position and plain streams."""

@@ -702,7 +702,13 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
       nmask |= (ok && neg ? 1u : 0u) << (s * 4 + r);
       m = mag > m ? mag : m;
     }
-    m = seg_max_dpp<S>(m);
+    if constexpr (S == 32) {  // two rows: DPP within each, then across them
+      m = seg_max_dpp<16>(m);
+      const unsigned o = (unsigned)__shfl_xor((int)m, 16, kWave);
+      m = o > m ? o : m;
+    } else {
+      m = seg_max_dpp<S>(m);
+    }
     const unsigned bc = bitcount(m);
     bcs[s] = bc;
     q0s[s] = q[0];
@@ -1091,7 +1097,8 @@ void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* bpt) {
   *bpt = (uint32_t)(kWavesPerBlock * spw * (kWave / s));
 }
 
-// The fast path (k_compress_v4) takes block sizes 32 and 64 with a 16-byte aligned
+// The fast path (k_compress_v4) takes block sizes 32, 64 and 128 (8, 16 or 32 lanes per
+// block; its tile holds as many blocks as the generic kernel's) with a 16-byte aligned
 // input; classification also needs the fastest axis a multiple of 4 long, the type map
 // 4-byte alignment.
 template <int MODE, int S>
@@ -1108,14 +1115,14 @@ static hipError_t launch_v4(const CompressArgs& a, hipStream_t st) {
 
 template <int MODE>
 static bool try_v4(const CompressArgs& a, hipStream_t st, hipError_t* e) {
-  if ((a.bs != 32 && a.bs != 64) || (reinterpret_cast<uintptr_t>(a.in) & 15u)) return false;
+  if ((a.bs != 32 && a.bs != 64 && a.bs != 128) || (reinterpret_cast<uintptr_t>(a.in) & 15u)) return false;
   if (MODE == kModeTopoClassify) {
     const Grid& g = a.grid;
     if (g.ndim != 2 && g.ndim != 3) return false;
     if ((g.ndim == 3 ? g.d2 : g.d1) % 4u) return false;
   }
   if ((MODE == kModeTopoMap || MODE == kModeTopoInt) && (reinterpret_cast<uintptr_t>(a.tmap) & 3u)) return false;
-  *e = a.bs == 64 ? launch_v4<MODE, 16>(a, st) : launch_v4<MODE, 8>(a, st);
+  *e = a.bs == 128 ? launch_v4<MODE, 32>(a, st) : a.bs == 64 ? launch_v4<MODE, 16>(a, st) : launch_v4<MODE, 8>(a, st);
   return true;
 }
 
@@ -1168,8 +1175,11 @@ hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* f
                                  uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
                                  hipStream_t st) {
   if (!ntiles) return hipSuccess;
-  if (v4) {  // block size exactly 32 or 64, every input of the class 16-byte aligned
-    if (bs_class == 64)
+  if (v4) {  // block size exactly 32, 64 or 128, every input of the class 16-byte aligned
+    if (bs_class == 128)
+      hipLaunchKernelGGL((k_compress_batch<32, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
+                         base, status, res);
+    else if (bs_class == 64)
       hipLaunchKernelGGL((k_compress_batch<16, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
                          base, status, res);
     else if (bs_class == 32)
