@@ -60,7 +60,9 @@ int tszp_rel_to_abs(const float *d_field, size_t n, float rel, float *eb, float 
  * critical points, per-bin sort positions, the sort-position stream and the
  * topology stream, byte-identical to the reference CPU path with nchunks threads
  * (SZp/src/szp_float.cc:28-1318; SZp/src/szp_topology_3d.cc:30-632).
- * d_sp_values (nullable, >= ext_count ints) receives the extrema sort positions. */
+ * d_sp_values (nullable, >= ext_count ints) receives the extrema sort positions.
+ * info holds the sizes on return; the streams are complete once `stream` reaches the
+ * point of the call (the last header bytes are written in stream order). */
 int tszp_topo_compress(const float *d_field, int ndim, const uint32_t *dims, float eb, int block_size,
                        unsigned nchunks, uint8_t *d_topo, size_t topo_cap, uint32_t *d_topo_blk,
                        uint8_t *d_sortpos, size_t sortpos_cap, uint32_t *d_sortpos_blk,

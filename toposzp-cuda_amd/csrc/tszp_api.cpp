@@ -61,6 +61,7 @@ struct Ctx {
   void* ptr[kWsCount] = {};
   size_t cap[kWsCount] = {};
   CompressResult* host_res = nullptr;  // pinned
+  uint32_t* host_err = nullptr;        // pinned: a device error word read at a later synchronisation
   void* host_batch = nullptr;          // pinned: batch tables and results
   size_t host_batch_cap = 0;
 
@@ -184,6 +185,7 @@ int get_ctx(Ctx** out) {
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(TSZP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     e = hipHostMalloc((void**)&c->host_res, sizeof(CompressResult), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_err, 64, hipHostMallocDefault);
     if (e != hipSuccess) { delete c; return fail(TSZP_EHIP, "hipHostMalloc: %s", hipGetErrorString(e)); }
     g_ctx[dev] = c;
   }
@@ -368,8 +370,10 @@ bool force_radix_sortpos() {
 // bins, or bins that are not monotone in the value); the caller then uses the radix
 // path.  The few bins whose key span exceeds kBinWideBits are ranked by the radix
 // sort on their buckets.
+// With defer_err the radix pass's error word goes to c->host_err without a
+// synchronisation: the caller checks it after its next one.
 int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32_t count, float eb, double inv,
-                       int32_t* out, uint32_t out_cap, hipStream_t st, bool* done) {
+                       int32_t* out, uint32_t out_cap, hipStream_t st, bool* done, bool defer_err = false) {
   *done = false;
   if (!count || force_radix_sortpos()) return TSZP_OK;
   BinWork w{};
@@ -426,10 +430,14 @@ int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32
     HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(sw.st), 0u, stw * 2, st));
     HIPCHK(radix_sort_pairs(w.bkey + wide_s, nullptr, w.bpay + wide_s, wn, sw, st));
     HIPCHK(launch_rank(w.bkey + wide_s, nullptr, w.bpay + wide_s, wn, eb, inv, out, out_cap, sw, st));
-    uint32_t herr = 0;
-    HIPCHK(hipMemcpyAsync(&herr, sw.err, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (herr) return fail(TSZP_EDEVICE, "sort: device error bits 0x%x", herr);
+    if (defer_err) {
+      HIPCHK(hipMemcpyAsync(c->host_err, sw.err, 4, hipMemcpyDeviceToHost, st));
+    } else {
+      uint32_t herr = 0;
+      HIPCHK(hipMemcpyAsync(&herr, sw.err, 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (herr) return fail(TSZP_EDEVICE, "sort: device error bits 0x%x", herr);
+    }
   }
   pr.stop();
   return TSZP_OK;
@@ -668,15 +676,17 @@ int tszp_topo_compress(const float* d_field, int ndim, const uint32_t* dims, flo
   int32_t* sp = d_sp_values;
   if (!sp) WS(kWsSp, (size_t)E * 4, sp);
   bool by_bins;
-  r = bin_sort_positions(c, cpk, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st, &by_bins);
+  c->host_err[0] = 0;
+  r = bin_sort_positions(c, cpk, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st, &by_bins, true);
   if (!r && !by_bins) r = sort_and_rank(c, cpk, nullptr, cpp, (uint32_t)res.cp_count, eb, inv, sp, E, st);
   if (r) return r;
   CompressResult r2;
   r = run_codec(c, kModeInt, sp, E, bs, 0.f, 0.0, nullptr, nullptr, d_sp, sp_cap, 8 + 8 * (size_t)T, d_sp_blk,
-                nullptr, nullptr, 0, st, &r2);
+                nullptr, nullptr, 0, st, &r2);  // synchronises: the deferred sort error is in
+  if (!r && c->host_err[0]) r = fail(TSZP_EDEVICE, "sort: device error bits 0x%x", c->host_err[0]);
   if (r) return r;
+  // the header kernel is ordered on the stream before anything the caller does next on it
   HIPCHK(launch_write_header(d_sp, 1, T, ndim == 2, cdiv(E, bs), d_sp_blk, E, 0.f, st));
-  HIPCHK(hipStreamSynchronize(st));
   info->sortpos_bytes = 8 + 8 * (uint64_t)T + r2.payload_bytes;
   return TSZP_OK;
 }
