@@ -121,7 +121,7 @@ CASES = [
     ("blobs3d_bs256", lambda: fields.blobs3d(30, 31, 37, seed=31, noise=1e-3), 1e-4, 256, 1),
     ("tiny3d", lambda: fields.blobs3d(3, 3, 3, seed=32, noise=0.1), 1e-3, 64, 1),
     ("const2d", lambda: np.full((40, 50), 1.25, np.float32), 1e-3, 64, 1),
-    # fastest axis a multiple of 4 and block size 32, 64 or 128: 16-byte stencil path
+    # fastest axis a multiple of 4 and block size 16 to 256 (powers of 2): 16-byte path
     ("vec2d_bs64", lambda: fields.blobs2d(96, 260, nblobs=24, seed=33), 1e-3, 64, 1),
     ("vec2d_bs32_T5", lambda: fields.blobs2d(77, 132, seed=34, noise=0.03), 1e-3, 32, 5),
     ("vec3d_bs64", lambda: fields.fbm3d(24, 36, 40, seed=35), 3e-3, 64, 1),
@@ -131,6 +131,10 @@ CASES = [
     ("vec3d_bs128", lambda: fields.fbm3d(20, 24, 64, seed=39), 3e-3, 128, 1),
     ("vec2d_bs128_T3", lambda: fields.blobs2d(90, 256, seed=40, noise=0.02), 1e-3, 128, 3),
     ("vec3d_bs128_tail", lambda: fields.fbm3d(7, 9, 20, seed=41), 2e-3, 128, 1),
+    ("vec3d_bs16", lambda: fields.fbm3d(18, 22, 36, seed=42), 3e-3, 16, 1),
+    ("vec2d_bs16_T7", lambda: fields.blobs2d(70, 96, seed=43, noise=0.02), 1e-3, 16, 7),
+    ("vec3d_bs256", lambda: fields.fbm3d(16, 20, 64, seed=44), 3e-3, 256, 1),
+    ("vec2d_bs256_T2", lambda: fields.blobs2d(50, 512, seed=45, noise=0.02), 1e-3, 256, 2),
 ]
 
 
@@ -166,7 +170,7 @@ def test_native_topology_matches_oracle(name, make, eb, bs, T):
     _eq(t2.cpu().numpy(), oracle.classify(data).ravel(), "classify")
 
 
-@pytest.mark.parametrize("bs,T", [(64, 1), (128, 1), (32, 5), (1025, 1), (2, 2), (100, 16)])
+@pytest.mark.parametrize("bs,T", [(64, 1), (128, 1), (32, 5), (1025, 1), (2, 2), (100, 16), (16, 3), (256, 1)])
 def test_native_plain_matches_oracle(bs, T):
     torch = _torch()
     data = fields.blobs2d(123, 457, seed=bs).ravel()
@@ -296,7 +300,9 @@ def test_sort_positions_bin_path_large():
 
 
 @pytest.mark.parametrize("shape,bs", [((128, 256, 256), 64), ((40, 96, 100), 32), ((1024, 1028), 64),
-                                      ((128, 256, 256), 128), ((33, 70, 92), 128), ((515, 1004), 128)])
+                                      ((128, 256, 256), 128), ((33, 70, 92), 128), ((515, 1004), 128),
+                                      ((64, 128, 128), 16), ((31, 40, 52), 16), ((64, 128, 256), 256),
+                                      ((300, 1024), 256)])
 def test_codec_fast_path_matches_generic(shape, bs):
     """The 16-byte-stencil codec against the generic kernel on the same field: identical
     topology, sort-position and plain streams."""

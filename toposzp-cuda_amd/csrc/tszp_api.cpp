@@ -764,10 +764,12 @@ struct BatchClass {
   uint32_t ntiles;
 };
 
-// Compression of the field by the 16-byte path (k_compress_v4): block size 32, 64 or 128
-// and a 16-byte aligned input, as for one field; its tiles hold the same blocks.
+// Compression of the field by the 16-byte path (k_compress_v4): block size 16, 32, 64, 128
+// or 256 and a 16-byte aligned input, as for one field; its tiles hold the same blocks.
 bool batch_v4(const tszp_batch_field& f, bool compress) {
-  return compress && (f.block_size == 32 || f.block_size == 64 || f.block_size == 128) &&
+  return compress &&
+         (f.block_size == 16 || f.block_size == 32 || f.block_size == 64 || f.block_size == 128 ||
+          f.block_size == 256) &&
          !((uintptr_t)f.d_field & 15u) &&
          !getenv("TSZP_CODEC_GENERIC");
 }

@@ -702,10 +702,13 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
       nmask |= (ok && neg ? 1u : 0u) << (s * 4 + r);
       m = mag > m ? mag : m;
     }
-    if constexpr (S == 32) {  // two rows: DPP within each, then across them
+    if constexpr (S > 16) {  // several rows: DPP within each, then across them
       m = seg_max_dpp<16>(m);
-      const unsigned o = (unsigned)__shfl_xor((int)m, 16, kWave);
-      m = o > m ? o : m;
+#pragma unroll
+      for (int o = 16; o < S; o <<= 1) {
+        const unsigned v = (unsigned)__shfl_xor((int)m, o, kWave);
+        m = v > m ? v : m;
+      }
     } else {
       m = seg_max_dpp<S>(m);
     }
@@ -1097,9 +1100,9 @@ void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* bpt) {
   *bpt = (uint32_t)(kWavesPerBlock * spw * (kWave / s));
 }
 
-// The fast path (k_compress_v4) takes block sizes 32, 64 and 128 (8, 16 or 32 lanes per
-// block; its tile holds as many blocks as the generic kernel's) with a 16-byte aligned
-// input; classification also needs the fastest axis a multiple of 4 long, the type map
+// The fast path (k_compress_v4) takes block sizes 16, 32, 64, 128 and 256 (4 to 64 lanes
+// per block; its tile holds as many blocks as the generic kernel's) with a 16-byte
+// aligned input; classification also needs the fastest axis a multiple of 4 long, the type map
 // 4-byte alignment.
 template <int MODE, int S>
 static hipError_t launch_v4(const CompressArgs& a, hipStream_t st) {
@@ -1115,14 +1118,22 @@ static hipError_t launch_v4(const CompressArgs& a, hipStream_t st) {
 
 template <int MODE>
 static bool try_v4(const CompressArgs& a, hipStream_t st, hipError_t* e) {
-  if ((a.bs != 32 && a.bs != 64 && a.bs != 128) || (reinterpret_cast<uintptr_t>(a.in) & 15u)) return false;
+  if ((a.bs != 16 && a.bs != 32 && a.bs != 64 && a.bs != 128 && a.bs != 256) ||
+      (reinterpret_cast<uintptr_t>(a.in) & 15u))
+    return false;
   if (MODE == kModeTopoClassify) {
     const Grid& g = a.grid;
     if (g.ndim != 2 && g.ndim != 3) return false;
     if ((g.ndim == 3 ? g.d2 : g.d1) % 4u) return false;
   }
   if ((MODE == kModeTopoMap || MODE == kModeTopoInt) && (reinterpret_cast<uintptr_t>(a.tmap) & 3u)) return false;
-  *e = a.bs == 128 ? launch_v4<MODE, 32>(a, st) : a.bs == 64 ? launch_v4<MODE, 16>(a, st) : launch_v4<MODE, 8>(a, st);
+  switch (a.bs) {
+    case 16: *e = launch_v4<MODE, 4>(a, st); break;
+    case 32: *e = launch_v4<MODE, 8>(a, st); break;
+    case 64: *e = launch_v4<MODE, 16>(a, st); break;
+    case 128: *e = launch_v4<MODE, 32>(a, st); break;
+    default: *e = launch_v4<MODE, 64>(a, st); break;
+  }
   return true;
 }
 
@@ -1175,8 +1186,14 @@ hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* f
                                  uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
                                  hipStream_t st) {
   if (!ntiles) return hipSuccess;
-  if (v4) {  // block size exactly 32, 64 or 128, every input of the class 16-byte aligned
-    if (bs_class == 128)
+  if (v4) {  // block size 16, 32, 64, 128 or 256, every input of the class 16-byte aligned
+    if (bs_class == 256)
+      hipLaunchKernelGGL((k_compress_batch<64, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
+                         base, status, res);
+    else if (bs_class == 16)
+      hipLaunchKernelGGL((k_compress_batch<4, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
+                         base, status, res);
+    else if (bs_class == 128)
       hipLaunchKernelGGL((k_compress_batch<32, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
                          base, status, res);
     else if (bs_class == 64)
