@@ -1702,9 +1702,11 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       complete = nb <= cap;
       moves += h[kPpMoves];
       if (stats) fprintf(stderr, "[restore] round %u: listed %u moved %u -> %u mismatches\n", rounds, nl, h[kPpMoves], nb);
-      // stop once a round removes less than a quarter of the mismatches: what is left
-      // is mostly plateaus that only the window solver can order
-      if (!h[kPpMoves] || (uint64_t)nb * 4 > (uint64_t)nb_before * 3) break;
+      // stop once a round removes less than a quarter of the mismatches (what is left is
+      // mostly plateaus that only the window solver can order), or once few are left:
+      // the one-launch solve takes about as long for a few thousand windows as for one
+      // (its time is the longest chain of dependent tiles), less than another round
+      if (!h[kPpMoves] || (uint64_t)nb * 4 > (uint64_t)nb_before * 3 || nb <= kPpSolveFew) break;
     }
     if (!nb) break;
     if (!exact) {

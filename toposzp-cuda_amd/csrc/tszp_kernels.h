@@ -305,6 +305,8 @@ enum : int {
   kPpSolveCounter = 43,
   kPpFlowCounter = 60  // tile counter of the one-launch solve
 };
+// mismatches left after a local round below which the window solve runs at once
+constexpr uint32_t kPpSolveFew = 16384;
 uint32_t pp_init_tiles(uint32_t n);
 hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
 // start values; work: pp_init_work_words(n) u32 (8-byte aligned), no initialisation needed
