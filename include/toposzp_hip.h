@@ -118,7 +118,8 @@ int tszp_batch_tune(tszp_batch_field *fields, uint32_t count, void *stream);
  * synchronisation -- so a caller may capture them in a hipGraph and replay it;
  * tszp_batch_plan_results waits for the stream and returns the per-field stream sizes
  * (caller's order) and the first device error of the last runs.  A plan is used by one
- * thread at a time.  No counterpart in the reference. */
+ * thread at a time, on the device that was current when it was created.  No
+ * counterpart in the reference. */
 typedef struct tszp_batch_plan tszp_batch_plan;
 /* d_outs (nullable): decompression output of each field (n floats); NULL decompresses
  * into d_field */
@@ -149,7 +150,8 @@ typedef struct {
 } tszp_server_stats_t;
 const char *tszp_server_last_error(void);
 int tszp_server_create(uint64_t hbm_budget, uint32_t max_batch, tszp_server **out);
-int tszp_server_destroy(tszp_server *server);  /* serves what is queued, then stops */
+/* serves what is queued, then stops; call it once no other thread uses the server */
+int tszp_server_destroy(tszp_server *server);
 /* block_size 0: chosen by the tuner of tszp_batch_tune when the request runs; the block
  * size used is returned by tszp_server_stream */
 int tszp_server_compress(tszp_server *server, const float *d_field, uint64_t n, float eb, int block_size,
