@@ -631,7 +631,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     line = {
-        "metric": "topo compress+decompress GB/s & %HBM peak (512^3 f32)",
+        "metric": "topo compress+decompress GB/s & %HBM peak (512^3 f32); batched p50/p99 vs plain",
         "value": gbs,
         "unit": "GB/s",
         "n_gpus": 1,
