@@ -444,8 +444,9 @@ def main():
                     help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the sharded design is "
-              f"described in DESIGN.md and is not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): sharding a field over GPUs "
+              f"(plane slabs with one-plane halos, a scan of per-shard byte counts, a bin-partitioned "
+              f"exchange for the sort positions) is designed but not built or measured.", file=sys.stderr)
         sys.exit(2)
 
     import torch
