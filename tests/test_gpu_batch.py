@@ -231,3 +231,23 @@ def test_batch_decode_rejects_corrupt_side_car():
     bad[128] = bad[64] - 4  # the second tile ends before it starts
     with pytest.raises(tz.TszpError, match="field 1: device error bits"):
         tz.plain_decompress_batch(streams, [blks[0], bad], [a.numel()] * 2, [1e-3] * 2, [64, 64])
+
+
+def test_plan_rejects_bad_fields():
+    torch = _torch()
+    f = torch.from_numpy(fields.blobs2d(20, 30, seed=5).ravel()).cuda()
+    s = torch.empty(tz.lib().tszp_plain_bound(f.numel(), 64, 1), dtype=torch.uint8, device="cuda")
+    blk = torch.empty(tz.lib().tszp_blocks(f.numel(), 64) + 1, dtype=torch.int32, device="cuda")
+    out = torch.empty_like(f)
+    with pytest.raises(tz.TszpError, match="blockSize 0"):
+        tz.BatchPlan([f], [1e-3], [0], [s], [blk], [out])
+    with pytest.raises(tz.TszpError, match="stream capacity"):
+        tz.BatchPlan([f], [1e-3], [64], [s[:8]], [blk], [out])
+    with pytest.raises(ValueError, match="one output per field"):
+        tz.BatchPlan([f], [1e-3], [64], [s], [blk], [out[:10]])
+    plan = tz.BatchPlan([f], [1e-3], [64], [s], [blk], [out])
+    plan.compress()
+    plan.decompress()
+    nb, s1, _ = tz.plain_compress(f, 1e-3, 64, T=1)
+    assert plan.results() == [nb] and torch.equal(s[:nb], s1[:nb])
+    plan.close()

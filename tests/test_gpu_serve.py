@@ -159,3 +159,26 @@ def test_bad_requests_are_rejected_at_submission():
     with pytest.raises(tz.TszpError, match="not a live compress ticket"):
         srv.decompress(t, f)
     srv.close()
+
+
+def test_release_while_its_decompression_is_queued():
+    """Releasing a compress ticket whose stream a queued decompression still reads defers
+    the free until that decompression has run; the result is the one-field call's."""
+    torch = _torch()
+    data = fields.blobs2d(300, 400, seed=9).ravel()
+    f = torch.from_numpy(data).cuda()
+    srv = tz.Server(1 << 26, max_batch=4)
+    c = srv.compress(f, 1e-3, 64)
+    srv.wait(c)
+    out = torch.empty_like(f)
+    d = srv.decompress(c, out)
+    srv.release(c)  # the stream stays until d has read it
+    with pytest.raises(tz.TszpError, match="not a live compress ticket"):
+        srv.decompress(c, out)
+    srv.wait(d)
+    nb, s1, blk = tz.plain_compress(f, 1e-3, 64, T=1)
+    ref = tz.plain_decompress(s1, blk, f.numel(), 1e-3, 64, T=1)
+    assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+    srv.release(d)
+    assert srv.stats()["hbm_in_use"] == 0
+    srv.close()
