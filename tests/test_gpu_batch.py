@@ -251,3 +251,23 @@ def test_plan_rejects_bad_fields():
     nb, s1, _ = tz.plain_compress(f, 1e-3, 64, T=1)
     assert plan.results() == [nb] and torch.equal(s[:nb], s1[:nb])
     plan.close()
+
+
+def test_batch_of_tiny_fields():
+    """Fields of 1 to 17 elements (one-element last blocks, fields shorter than a block)
+    next to a larger one: the oracle's streams and reconstructions."""
+    torch = _torch()
+    rng = np.random.default_rng(12)
+    sizes = [1, 2, 3, 15, 16, 17, 5, 1, 9]
+    bss = [2, 64, 16, 16, 16, 16, 3, 128, 8]
+    cases = [(rng.standard_normal(n).astype(np.float32), 1e-2, bs) for n, bs in zip(sizes, bss)]
+    cases.append((fields.blobs2d(40, 41, seed=13).ravel(), 1e-3, 64))
+    dev = [torch.from_numpy(d).cuda() for d, _, _ in cases]
+    ebs = [eb for _, eb, _ in cases]
+    bsl = [bs for _, _, bs in cases]
+    got, streams, blks = tz.plain_compress_batch(dev, ebs, bsl)
+    outs = tz.plain_decompress_batch(streams, blks, [d.numel() for d in dev], ebs, bsl)
+    for i, (data, eb, bs) in enumerate(cases):
+        ref = oracle.plain_compress(data, eb, bs, T=1)
+        _eq(streams[i][:got[i]].cpu().numpy(), ref, f"field {i} (n {data.size}, bs {bs})")
+        _eq(outs[i].cpu().numpy(), oracle.plain_decompress(ref[4:], data.size, eb, bs, T=1), f"field {i} decode")

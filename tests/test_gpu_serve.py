@@ -182,3 +182,45 @@ def test_release_while_its_decompression_is_queued():
     srv.release(d)
     assert srv.stats()["hbm_in_use"] == 0
     srv.close()
+
+
+def test_many_clients_with_releases():
+    """Eight threads each submit compressions and decompressions, wait and release as they
+    go; every result equals the one-field call's and the arena ends empty."""
+    torch = _torch()
+    cases = _cases(4, 24)
+    dev = [torch.from_numpy(d).cuda() for d, _, _ in cases]
+    ref = []
+    for f, (_, eb, bs) in zip(dev, cases):
+        nb, s, blk = tz.plain_compress(f, eb, bs, T=1)
+        ref.append((s[:nb].clone(), tz.plain_decompress(s, blk, f.numel(), eb, bs, T=1)))
+    torch.cuda.synchronize()
+    srv = tz.Server(1 << 28, max_batch=32)
+    errors = []
+
+    def client(k):
+        try:
+            rng = np.random.default_rng(100 + k)
+            for _ in range(12):
+                i = int(rng.integers(0, len(cases)))
+                out = torch.empty_like(dev[i])
+                c = srv.compress(dev[i], cases[i][1], cases[i][2])
+                d = srv.decompress(c, out)
+                srv.wait(c)
+                assert torch.equal(srv.stream_copy(c), ref[i][0]), f"client {k}: stream of field {i}"
+                srv.release(c)  # deferred while d may still read it
+                srv.wait(d)
+                assert torch.equal(out.view(torch.int32), ref[i][1].view(torch.int32)), f"client {k}: field {i}"
+                srv.release(d)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    st = srv.stats()
+    assert st["requests"] == 8 * 12 * 2 and st["hbm_in_use"] == 0 and st["queued"] == 0
+    srv.close()
