@@ -105,13 +105,38 @@ __global__ __launch_bounds__(kBlockThreads) void k_elem_compress(ElemArgs a) {
   // ---- loads: the lane's deltas of every step, and the element before each unit ----
   TF raw[SPW][EPL];
   TF before[SPW];
+  // float data, four deltas per lane: the lane's elements come from two aligned 16-byte
+  // loads when both lie inside the array (fewer, wider requests than four 4-byte ones)
+  constexpr bool kVecF = sizeof(TF) == 4 && EPL == 4;
+  const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15u) == 0;
 #pragma unroll
   for (int s = 0; s < SPW; s++) {
     const Unit un = elem_unit<RA>(a, tile * BPT + (uint32_t)((w * SPW + s) * G + g));
+    bool vec = false;
+    if constexpr (kVecF) {
+      const uint32_t e0 = un.s + (uint32_t)(ls * EPL);
+      const uint32_t b4 = e0 & ~3u;
+      vec = in16 && (uint32_t)(ls * EPL + EPL) <= un.m && b4 + 8u <= a.n;
+      if (vec) {
+        const uint4* v4 = reinterpret_cast<const uint4*>(a.in) + (b4 >> 2);
+        const uint4 lo = v4[0], hi = v4[1];
+        const uint32_t w8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const uint32_t m = e0 & 3u;
 #pragma unroll
-    for (int r = 0; r < EPL; r++) {
-      const uint32_t j = (uint32_t)(ls * EPL + r);
-      raw[s][r] = j < un.m ? in[un.s + j] : (TF)0;
+        for (int r = 0; r < EPL; r++) {
+          const uint32_t k = m + (uint32_t)r;  // 0..6
+          const uint32_t v = k == 0 ? w8[0] : k == 1 ? w8[1] : k == 2 ? w8[2] : k == 3 ? w8[3]
+                           : k == 4 ? w8[4] : k == 5 ? w8[5] : w8[6];
+          raw[s][r] = __uint_as_float(v);
+        }
+      }
+    }
+    if (!vec) {
+#pragma unroll
+      for (int r = 0; r < EPL; r++) {
+        const uint32_t j = (uint32_t)(ls * EPL + r);
+        raw[s][r] = j < un.m ? in[un.s + j] : (TF)0;
+      }
     }
     before[s] = (ls == 0 && (un.m || un.anchor)) ? in[un.s - 1u] : (TF)0;
   }
