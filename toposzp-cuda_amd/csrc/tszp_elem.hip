@@ -106,7 +106,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_elem_compress(ElemArgs a) {
   TF raw[SPW][EPL];
   TF before[SPW];
   // float data, four deltas per lane: the lane's elements come from two aligned 16-byte
-  // loads when both lie inside the array (fewer, wider requests than four 4-byte ones)
+  // loads when both lie inside the array (fewer, wider requests than four 4-byte ones;
+  // for double data three 16-byte loads per lane were slower than four 8-byte ones)
   constexpr bool kVecF = sizeof(TF) == 4 && EPL == 4;
   const bool in16 = (reinterpret_cast<uintptr_t>(a.in) & 15u) == 0;
 #pragma unroll
