@@ -223,7 +223,7 @@ int tszp_classify(const float *d_field, int ndim, const uint32_t *dims, uint8_t 
  * d_decoded holds the decoded (float)q * eb; the result goes to d_field, which may be
  * the same buffer (then a copy of the decoded values is kept internally).  Every point
  * stays in the interval of its cell -- [d, d + 0.9 eb] for d > 0, [d - 0.9 eb, d] for
- * d < 0 (error < eb), [-eb/1024, eb/1024] for d == 0 (error < eb (1 + 1/1024)) -- while
+ * d < 0 (error < eb), [-eb 2^-16, eb 2^-16] for d == 0 (error < eb (1 + 2^-16)) -- while
  * a colour-class local search and an exact solve of small plateau windows make the
  * critical-point type of every interior point equal to d_types.  d_sp (nullable):
  * extrema sort positions in row-major extremum order; with them the extrema of one

@@ -83,6 +83,12 @@ def test_unpack_without_valid_footer_is_one_topology_stream():
     # sizes larger than the buffer fall back the same way
     bad = np.concatenate([np.zeros(40, np.uint8), np.frombuffer(struct.pack("<QQ", 1000, 8), np.uint8)])
     assert tz.szp_unpack(bad) == (bad.size, bad.size, 0, 0)
+    # a sort-position size whose sum with the topology size wraps around 2^64 must not
+    # pass the check: sp_offset + sp_bytes stays within the buffer
+    wrap = np.concatenate([np.zeros(40, np.uint8), np.frombuffer(struct.pack("<QQ", 20, 2**64 - 30), np.uint8)])
+    assert tz.szp_unpack(wrap) == (wrap.size, wrap.size, 0, 0)
+    ok = np.concatenate([np.zeros(40, np.uint8), np.frombuffer(struct.pack("<QQ", 20, 20), np.uint8)])
+    assert tz.szp_unpack(ok)[:3] == (20, 20, 20)
 
 
 def test_pack_rejects_small_output():

@@ -100,5 +100,35 @@ def test_fbm_512_cube_properties(bs):
     info2, buf2 = tz.topo_compress(f, eb, bs)
     assert info2.topo_bytes == info.topo_bytes and torch.equal(buf2.topo[:info.topo_bytes], first)
     del buf2
-    r = tz.topo_restore(out, ty, shape, eb, sp=pos, n_ext=info.ext_count, out=torch.empty_like(out))
+    y = torch.empty_like(out)
+    r = tz.topo_restore(out, ty, shape, eb, sp=pos, n_ext=info.ext_count, out=y)
     assert r.mismatches() == 0 and r.max_dev_over_eb < 1.0
+    # pointwise bound of the restored field: eb, and eb (1 + 2^-16) in the zero cell
+    yerr = float((y.double() - f.view(-1).double()).abs().max())
+    assert yerr <= eb * (1 + 2.0 ** -16), f"restored error {yerr / eb} eb"
+
+
+@pytest.mark.parametrize("bs", [64, 128])
+def test_plain_512_cube_properties(bs):
+    """Plain SZp on the 512^3 fBm field (rel 1e-3): every decoded value within the error
+    bound (the reference's plain decoder rounds (float)q * eb the same way), the device
+    parse of the stream without its side-car finds the compressor's side-car, and a
+    second compression gives the same bytes."""
+    torch = _torch()
+    shape = (512, 512, 512)
+    n = 512 ** 3
+    f = tz.gen_fbm(shape, seed=20251203)
+    eb, _ = tz.rel_to_abs(f, 1e-3)
+    print(f"plain 512^3 bs={bs} eb={eb}", flush=True)
+    nbytes, out, blk = tz.plain_compress(f, eb, bs)
+    first = out[:nbytes].clone()
+    dec = tz.plain_decompress(out, blk, n, eb, bs)
+    err = float((dec - f.view(-1)).abs().max())
+    assert err < eb * (1 + 1e-6), f"error {err / eb} eb"
+    pblk, T = tz.parse_blocks(first, n, bs, "plain")
+    assert T == 1 and torch.equal(pblk, blk), "parsed side-car"
+    nbytes2, out2, _ = tz.plain_compress(f, eb, bs)
+    assert nbytes2 == nbytes and torch.equal(out2[:nbytes], first)
+    # the header: eb big-endian, one chunk at offset 0
+    h = first[:12].cpu().numpy()
+    assert h[:4].tobytes() == np.float32(eb).byteswap().tobytes() and not h[4:].any()

@@ -26,17 +26,21 @@ def _pipeline(data, eb, bs=64):
 
 def _check_cells(x, dec, y, eb):
     """Every restored value stays in the interval of its cell: [d, d + 0.9 eb] above
-    zero, [d - 0.9 eb, d] below, [-eb/1024, eb/1024] in the zero cell."""
+    zero, [d - 0.9 eb, d] below, [-eb 2^-16, eb 2^-16] in the zero cell."""
     x, dec, y = x.ravel().astype(np.float64), dec.ravel().astype(np.float64), y.ravel().astype(np.float64)
     slack = np.abs(dec) * 2.0 ** -22
     pos, neg, zero = dec > 0, dec < 0, dec == 0
     assert np.all(y[pos] >= dec[pos]) and np.all(y[pos] <= dec[pos] + 0.9 * eb * (1 + 1e-5) + slack[pos])
     assert np.all(y[neg] <= dec[neg]) and np.all(y[neg] >= dec[neg] - 0.9 * eb * (1 + 1e-5) - slack[neg])
-    assert np.all(np.abs(y[zero]) <= eb / 1024 * (1 + 1e-5))
-    # error bound: below eb outside the zero cell, below eb (1 + 1/1024) inside it
+    assert np.all(np.abs(y[zero]) <= np.float32(eb) * np.float32(2.0 ** -16))
+    # pointwise error bound: |y - x| <= eb outside the zero cell, up to one float spacing
+    # of d (the decoder rounds (float)q * eb, as the reference's does), and
+    # |y - x| <= eb (1 + 2^-16) inside it, where |x| < eb
+    ebf = float(np.float32(eb))
     err = np.abs(y - x)
-    assert np.all(err[~zero] < eb * (1 + 1e-4) + np.abs(x[~zero]) * 2.0 ** -21)
-    assert np.all(err[zero] < eb * (1 + 1 / 1024) * (1 + 1e-4))
+    ulp = np.spacing(np.abs(dec[~zero]).astype(np.float32)).astype(np.float64)
+    assert np.all(err[~zero] <= ebf + ulp)
+    assert np.all(err[zero] <= ebf * (1 + 2.0 ** -16))
 
 
 def _check_extrema_order(x, dec, y, types, eb, info, slack=1):
@@ -98,8 +102,8 @@ SMOOTH = [
 
 @pytest.mark.parametrize("name,make,eb", SMOOTH, ids=[c[0] for c in SMOOTH])
 def test_restore_smooth_fields(name, make, eb):
-    """Smooth fields with coarse bounds have large flat cells; the repair must at least
-    remove almost every mismatch and never leave the cells."""
+    """Smooth fields with coarse bounds have large flat cells (plateaus of many points);
+    the repair still removes every mismatch and never leaves the cells."""
     data = make()
     dec, y, types, info = _pipeline(data, eb)
     want = oracle.classify(data).ravel()
@@ -108,7 +112,7 @@ def test_restore_smooth_fields(name, make, eb):
     after = np.count_nonzero(got != want)
     print(f"{name}: cps={np.count_nonzero(want)} raw={np.count_nonzero(raw != want)} after={after}")
     assert after == info.mismatches()
-    assert after <= max(2, np.count_nonzero(raw != want) // 50)
+    assert after == 0
     _check_cells(data, dec, y, eb)
     _check_extrema_order(data, dec, y, types, eb, info, slack=8)
 

@@ -903,15 +903,20 @@ int tszp_plain_compress_batch(tszp_batch_field* fields, uint32_t count, void* st
   }
   uint64_t* status;
   CompressResult* res;
-  WS(kWsBatchSt, (size_t)tiles * 8 + 8, status);
+  // look-back words of every tile, then one tile counter per class
+  WS(kWsBatchSt, (size_t)tiles * 8 + 4 * classes.size() + 8, status);
   WS(kWsBatchRes, (size_t)count * sizeof(CompressResult), res);
+  uint32_t* counters = reinterpret_cast<uint32_t*>(status + tiles);
   FillRanges fr{};
-  fr.add(reinterpret_cast<uint32_t*>(status), 0u, (size_t)tiles * 2);
+  fr.add(reinterpret_cast<uint32_t*>(status), 0u, (size_t)tiles * 2 + classes.size());
   fr.add(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4);
   HIPCHK(launch_fill_multi(fr, st));
   ProfScope prof(kProfBatch, st);
-  for (const BatchClass& k : classes)
-    HIPCHK(launch_compress_batch(k.bs, k.v4, tab, tmap, k.base, k.ntiles, status, res, st));
+  for (size_t k = 0; k < classes.size(); k++) {
+    const BatchClass& cl = classes[k];
+    HIPCHK(launch_compress_batch(cl.bs, cl.v4, tab, tmap, cl.base, cl.ntiles, status, res,
+                                 dyn_counter(counters + k), st));
+  }
   prof.stop();
   CompressResult* h;
   if ((r = batch_results(c, res, count, st, &h))) return r;
@@ -1043,7 +1048,7 @@ int tszp_batch_plan_create(const tszp_batch_field* fields, uint32_t count, float
   (void)hipGetDevice(&p->device);
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t tb = up((size_t)count * sizeof(BatchField)), mb = up((size_t)p->tiles * 4),
-               sb = up((size_t)p->tiles * 8 + 8), rb = up((size_t)count * sizeof(CompressResult));
+               sb = up((size_t)p->tiles * 8 + 4 * p->classes.size() + 8), rb = up((size_t)count * sizeof(CompressResult));
   hipError_t e = hipMalloc(&p->mem, 2 * tb + mb + sb + 2 * rb);
   if (e == hipSuccess) e = hipHostMalloc((void**)&p->host, 2 * (size_t)count * sizeof(CompressResult), hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -1098,11 +1103,16 @@ int tszp_batch_plan_compress(tszp_batch_plan* plan, void* stream) {
   if (!plan) return fail(TSZP_EINVAL, "batch_plan_compress: null plan");
   hipStream_t st = (hipStream_t)stream;
   FillRanges fr{};
-  fr.add(reinterpret_cast<uint32_t*>(plan->status), 0u, (size_t)plan->tiles * 2);
+  // look-back words, then one tile counter per class
+  fr.add(reinterpret_cast<uint32_t*>(plan->status), 0u, (size_t)plan->tiles * 2 + plan->classes.size());
   fr.add(reinterpret_cast<uint32_t*>(plan->cres), 0u, (size_t)plan->count * sizeof(CompressResult) / 4);
   HIPCHK(launch_fill_multi(fr, st));  // launches only: safe under stream capture
-  for (const BatchClass& k : plan->classes)
-    HIPCHK(launch_compress_batch(k.bs, k.v4, plan->ctab, plan->tmap, k.base, k.ntiles, plan->status, plan->cres, st));
+  uint32_t* counters = reinterpret_cast<uint32_t*>(plan->status + plan->tiles);
+  for (size_t k = 0; k < plan->classes.size(); k++) {
+    const BatchClass& cl = plan->classes[k];
+    HIPCHK(launch_compress_batch(cl.bs, cl.v4, plan->ctab, plan->tmap, cl.base, cl.ntiles, plan->status, plan->cres,
+                                 dyn_counter(counters + k), st));
+  }
   return TSZP_OK;
 }
 
@@ -1969,8 +1979,9 @@ int tszp_szp_unpack(const uint8_t* buf, size_t len, size_t* topo_bytes, size_t* 
   size_t tb = len, sb = 0;
   if (len >= sizeof(f)) {
     HIPCHK(hipMemcpy(f, buf + len - sizeof(f), sizeof(f), hipMemcpyDefault));
-    const uint64_t calc = f[0] + f[1] + sizeof(f);
-    if (f[0] > 0 && calc >= f[0] && calc <= (uint64_t)len + 16 && f[0] < len) {
+    // the reference's test is f[0] + f[1] + 16 <= len + 16 with f[0] in (0, len); written
+    // as f[1] <= len - f[0] it cannot wrap around, so sp_offset + sp_bytes <= len
+    if (f[0] > 0 && f[0] < len && f[1] <= (uint64_t)len - f[0]) {
       tb = (size_t)f[0];
       sb = (size_t)f[1];
     }

@@ -931,7 +931,9 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_compress_v4(CompressArgs a
 // Workgroup b takes tile base + b of the batch's tile numbering; tmap names the tile's
 // field, whose tiles are contiguous, and the field's look-back status words start at
 // its first tile, so a tile only waits on earlier tiles of its own field.  Streams and
-// side-cars are those of the one-field codec with one chunk.
+// side-cars are those of the one-field codec with one chunk.  With a tile counter
+// (nullable) workgroups take their batch tile from it instead of their id, as the
+// one-field kernels do under TSZP_TILE_COUNTER.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlockThreads) void k_batch_tilemap(const BatchField* __restrict__ fields,
                                                                 uint32_t* __restrict__ tmap) {
@@ -945,8 +947,12 @@ __global__ __launch_bounds__(kBlockThreads, V4 ? 4 : 1) void k_compress_batch(co
                                                                             const uint32_t* __restrict__ tmap,
                                                                             uint32_t base,
                                                                             uint64_t* __restrict__ status,
-                                                                            CompressResult* __restrict__ res) {
-  const uint32_t gt = base + blockIdx.x;
+                                                                            CompressResult* __restrict__ res,
+                                                                            uint32_t* counter) {
+  __shared__ uint32_t sh_gt;
+  if (threadIdx.x == 0) sh_gt = base + tile_id(counter);
+  __syncthreads();
+  const uint32_t gt = sh_gt;
   const uint32_t fi = tmap[gt];
   const BatchField& f = fields[fi];
   CompressArgs a{};
@@ -1184,24 +1190,24 @@ hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32
 
 hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
                                  uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
-                                 hipStream_t st) {
+                                 uint32_t* counter, hipStream_t st) {
   if (!ntiles) return hipSuccess;
   if (v4) {  // block size 16, 32, 64, 128 or 256, every input of the class 16-byte aligned
     if (bs_class == 256)
       hipLaunchKernelGGL((k_compress_batch<64, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res);
+                         base, status, res, counter);
     else if (bs_class == 16)
       hipLaunchKernelGGL((k_compress_batch<4, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res);
+                         base, status, res, counter);
     else if (bs_class == 128)
       hipLaunchKernelGGL((k_compress_batch<32, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res);
+                         base, status, res, counter);
     else if (bs_class == 64)
       hipLaunchKernelGGL((k_compress_batch<16, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res);
+                         base, status, res, counter);
     else if (bs_class == 32)
       hipLaunchKernelGGL((k_compress_batch<8, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res);
+                         base, status, res, counter);
     else
       return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1212,7 +1218,7 @@ hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* f
 #define TSZP_CASE(SS, EE)                                                                            \
   if (S == SS && EPL == EE) {                                                                        \
     hipLaunchKernelGGL((k_compress_batch<SS, EE, false>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, \
-                       base, status, res);                                                           \
+                       base, status, res, counter);                                                  \
     return hipGetLastError();                                                                        \
   }
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)

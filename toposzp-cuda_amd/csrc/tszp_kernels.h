@@ -67,10 +67,11 @@ struct BatchField {
 };
 hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32_t* tmap, hipStream_t st);
 // tiles [base, base + ntiles) of the batch's numbering, all of one block-size class;
-// v4: the 16-byte path (block size exactly 32 or 64, every input 16-byte aligned)
+// v4: the 16-byte path (block size 16, 32, 64, 128 or 256, every input 16-byte aligned);
+// counter (nullable, zeroed): tiles handed out from it instead of workgroup ids
 hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
                                  uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
-                                 hipStream_t st);
+                                 uint32_t* counter, hipStream_t st);
 hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
                                uint32_t ntiles, CompressResult* res, hipStream_t st);
 // Estimated payload bytes of every field at each block size of kTuneSizes, from

@@ -8,8 +8,8 @@
 //
 //   * the interval of a point: [d, d + 0.9 eb] when d > 0, [d - 0.9 eb, d] when d < 0
 //     (both on the side of d that holds the original value, so |y - x| < eb), and
-//     [-eb/1024, eb/1024] in the zero cell, whose sign truncation lost
-//     (|y - x| < eb (1 + 1/1024)).  Intervals of different cells never overlap, so a
+//     [-eb 2^-16, eb 2^-16] in the zero cell, whose sign truncation lost
+//     (|y - x| < eb (1 + 2^-16)).  Intervals of different cells never overlap, so a
 //     comparison between points of different cells is the original's: only the order
 //     inside a plateau (a connected set of points with one decoded value) is free,
 //     and the original field proves that a solution exists;
@@ -185,7 +185,11 @@ __device__ __forceinline__ int classify_at_y(const Grid& g, const float* y, uint
   return classify_vals<ND>(y_ld<COH>(y, e), v);
 }
 
-// interval of the restored value of a point with decoded value d
+// interval of the restored value of a point with decoded value d.  The zero cell is
+// the one cell whose points may leave the error bound (by at most eb kZeroCell); its
+// half-width only has to leave room for the orders inside a plateau, which float
+// spacing near zero gives many times over.
+constexpr float kZeroCell = 1.f / 65536.f;
 __device__ __forceinline__ void cell_of(float d, float eb, float* lo, float* hi) {
   if (d > 0.f) {
     *lo = d;
@@ -194,8 +198,8 @@ __device__ __forceinline__ void cell_of(float d, float eb, float* lo, float* hi)
     *lo = d - 0.9f * eb;
     *hi = d;
   } else {
-    *lo = -eb * (1.f / 1024.f);
-    *hi = eb * (1.f / 1024.f);
+    *lo = -eb * kZeroCell;
+    *hi = eb * kZeroCell;
   }
 }
 
@@ -2064,10 +2068,13 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t*
   }
   __syncthreads();
   solve_tile<ND, true>(sh, g, id, ty, tz, pbits, d, t, y, eb, anchor, misc);
-  // every coherent store of the tile has completed before the flag is set
-  __builtin_amdgcn_s_waitcnt(0);
+  // the flag is a release store at agent scope: every field store of the tile is
+  // performed before a neighbouring tile can see it set.  The waiting side reads the
+  // field only with agent-coherent loads issued after its spin has seen the flag (the
+  // loop waits for the flag's value and the workgroup barrier orders the loads after
+  // it), so it needs no acquire, whose L2 invalidation would slow every window build.
   __syncthreads();
-  if (lane == 0) __hip_atomic_store(&done[id], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0) __hip_atomic_store(&done[id], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------
