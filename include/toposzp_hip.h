@@ -83,6 +83,17 @@ int tszp_plain_compress(const float *d_in, size_t n, float eb, int block_size, u
 int tszp_plain_decompress(const uint8_t *d_stream, const uint32_t *d_blk, unsigned nchunks, size_t n,
                           float eb, int block_size, float *d_out, void *stream);
 
+/* tszp_plain_compress without any allocation, for callers that cannot allocate device
+ * memory (code under stream capture, tight in-situ loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same arguments, bytes and side-car as
+ * tszp_plain_compress; its scratch (look-back words, tile counter, results) lives in a
+ * static device array of about 2 MiB per device, held until the process ends.  Fields
+ * that need more than 2^17 tiles (more than 2^28 elements at the smallest tiles) are
+ * rejected with TSZP_EINVAL.  Every call on a device shares that array, so calls that
+ * overlap on one device race; the call returns after its stream has finished it. */
+int tszp_plain_compress_noalloc(const float *d_in, size_t n, float eb, int block_size, unsigned nchunks,
+                                uint8_t *d_out, size_t cap, uint32_t *d_blk, uint64_t *out_bytes, void *stream);
+
 /* Batches of plain float fields: every field of a batch is compressed (decompressed) by
  * one launch per block-size class instead of one call per field; each field's stream
  * and side-car are byte-identical to tszp_plain_compress / tszp_plain_decompress with

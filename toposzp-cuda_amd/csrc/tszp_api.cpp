@@ -570,6 +570,10 @@ int check_n(size_t n) {
 
 }  // namespace
 
+// error reporting for entry points defined in other source files (tszp_last_error)
+int tszp::set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+void tszp::clear_error() { g_err.clear(); }
+
 // =========================================================================================
 // native API
 // =========================================================================================

@@ -82,6 +82,10 @@ constexpr uint32_t kTuneWindow = 128;
 constexpr uint32_t kTuneMaxWindows = 512;
 hipError_t launch_batch_tune(const BatchField* fields, uint32_t count, uint32_t* est, hipStream_t st);
 
+// tszp_last_error() of the calling thread, for entry points outside tszp_api.cpp
+int set_error(int code, const char* msg);
+void clear_error();
+
 // Tile counter to pass to a look-back kernel: null (tiles = workgroup ids) unless
 // TSZP_TILE_COUNTER is set, which hands tiles out from the counter instead.
 inline uint32_t* dyn_counter(uint32_t* counter) { return getenv("TSZP_TILE_COUNTER") ? counter : nullptr; }
