@@ -262,6 +262,61 @@ int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_t
 int tszp_topo_verify(const float *d_field, const uint8_t *d_types, int ndim, const uint32_t *dims,
                      uint64_t *counts, void *stream);
 
+/* Batched topology pipeline: many 2D / 3D fields of any sizes, error bounds and block
+ * sizes through one launch per stage (per block-size class) instead of one call chain
+ * per field.  No counterpart in the reference, whose callers run the per-field chain
+ * szp_cuda_find_critical_points -> szp_cuda_sort_critical_points_by_original_data ->
+ * szp_cuda_compress_sort_positions -> szp_cuda_float_compress_randomaccess_topology_
+ * preserved (SZp/src/szp_cuda_topology.cu:77-772, example/test_cuda_topology.cu:374-530)
+ * once per field.  Every field's streams and side-cars are byte-identical to
+ * tszp_topo_compress with nchunks 1: topology [u64 0][blocks], sort positions
+ * [u64 E][u64 0][blocks] (the header is written also when E == 0, when sortpos_bytes
+ * is 0).  A batch object keeps the layout and the workspace on the device: compress and
+ * decompress are kernel launches only (no allocation, host copy or synchronisation), so
+ * a caller may capture them in a hipGraph; restore synchronises a few times per batch
+ * (not per field); results reads the sizes and counts back.  Decoded values, types,
+ * sort positions and restored values live in the batch's own device arrays
+ * (tszp_topo_batch_output).  A batch is used by one thread at a time, on the device
+ * that was current when it was created.  Fields need |x / eb| < 2^31 (the
+ * one-field call also takes larger values).  Errors name the first failing field by its
+ * index in the caller's array. */
+typedef struct {
+  const float *d_field;      /* compression input (may be NULL for decompress-only batches) */
+  int ndim;                  /* 2 or 3 */
+  uint32_t dims[3];          /* dims[0] slowest */
+  float eb;                  /* absolute error bound */
+  int block_size;            /* 1 .. 257; 0: chosen as tszp_batch_tune chooses */
+  uint8_t *d_topo;           /* topology stream, topo_cap >= tszp_topo_bound(n, bs, 1), 4-byte aligned */
+  uint64_t topo_cap;
+  uint32_t *d_topo_blk;      /* tszp_blocks(n, bs) + 1 block offsets */
+  uint8_t *d_sortpos;        /* sort-position stream, sortpos_cap >= tszp_sortpos_bound(n, bs, 1) */
+  uint64_t sortpos_cap;
+  uint32_t *d_sortpos_blk;   /* tszp_blocks(n, bs) + 1 block offsets */
+  tszp_topo_info info;       /* set by tszp_topo_batch_results after a compression */
+  tszp_restore_info restore; /* set by tszp_topo_batch_results after a restoration (per field:
+                                missing / spurious / changed, max_dev_over_eb; the batch's
+                                moves, windows and rounds in every entry) */
+} tszp_topo_batch_field;
+typedef struct tszp_topo_batch tszp_topo_batch;
+/* fields[i].block_size 0 is replaced by the chosen block size */
+int tszp_topo_batch_create(tszp_topo_batch_field *fields, uint32_t count, tszp_topo_batch **batch, void *stream);
+int tszp_topo_batch_destroy(tszp_topo_batch *batch);
+int tszp_topo_batch_compress(tszp_topo_batch *batch, void *stream);
+/* topology + sort-position streams -> decoded values (float)q * eb, types, sort positions */
+int tszp_topo_batch_decompress(tszp_topo_batch *batch, void *stream);
+/* tszp_topo_restore of every decoded field into the batch's restored values */
+int tszp_topo_batch_restore(tszp_topo_batch *batch, int max_rounds, void *stream);
+/* waits for `stream`; fills info (after compress) and restore (after restore) of every field */
+int tszp_topo_batch_results(tszp_topo_batch *batch, tszp_topo_batch_field *fields, void *stream);
+/* device arrays of field i (n elements each): decoded values, restored values, types,
+ * sort positions (info.ext_count of them); any may be NULL */
+int tszp_topo_batch_output(tszp_topo_batch *batch, uint32_t i, float **d_decoded, float **d_restored,
+                           uint8_t **d_types, int32_t **d_sortpos);
+/* copies one of those arrays of field i into d_dst, ordered on `stream`: which 0 decoded
+ * values (n floats), 1 restored values (n floats), 2 types (n bytes), 3 sort positions
+ * (ext_count ints; `count` elements are copied, at most n) */
+int tszp_topo_batch_copy(tszp_topo_batch *batch, uint32_t i, int which, void *d_dst, uint64_t count, void *stream);
+
 /* Topology quality of a decompressed field y against the original x, as the
  * reference's CUDA driver reports it (example/test_cuda_topology.cu:540-868), for 2D
  * and 3D fields, in one GPU pass.  Critical points follow the codec's max / min /

@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsCount
+  kWsPpFields, kWsCount
 };
 
 struct Ctx {
@@ -107,7 +107,8 @@ std::vector<Ctx*> g_ctx;
 // Optional per-kernel timing: HIP events recorded on the launch stream around the
 // kernels of each stage, accumulated until tszp_prof_reset().
 enum ProfKind { kProfCodecTopo, kProfCodecInt, kProfCodecPlain, kProfDecode, kProfSort, kProfRank, kProfRestore,
-                kProfScatter, kProfSolve, kProfQuery, kProfElem, kProfElemDecode, kProfSaddle, kProfBatch, kProfBatchDecode, kProfCount };
+                kProfScatter, kProfSolve, kProfQuery, kProfElem, kProfElemDecode, kProfSaddle, kProfBatch, kProfBatchDecode,
+                kProfCount = kProfTopoBatchSort + 1 };
 int g_prof = 0;
 double g_prof_ms[kProfCount];
 uint64_t g_prof_n[kProfCount];
@@ -570,9 +571,16 @@ int check_n(size_t n) {
 
 }  // namespace
 
-// error reporting for entry points defined in other source files (tszp_last_error)
+// error reporting and timers for entry points defined in other source files
 int tszp::set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 void tszp::clear_error() { g_err.clear(); }
+Grid tszp::grid_of(int ndim, const uint32_t* dims) { return make_grid(ndim, dims); }
+bool tszp::prof_on() { return g_prof != 0; }
+void tszp::prof_add(int kind, double ms) {
+  if (kind < 0 || kind >= kProfCount) return;
+  g_prof_ms[kind] += ms;
+  g_prof_n[kind]++;
+}
 
 // =========================================================================================
 // native API
@@ -828,6 +836,7 @@ int batch_layout(const tszp_batch_field* fl, uint32_t count, bool compress, cons
     b.tile0 = (uint32_t)tiles;
     b.eb = f.eb;
     b.blk = f.d_blk;
+    b.hdr = kBatchHdr;
     if (compress) {
       b.in = f.d_field;
       b.out = f.d_stream;
@@ -918,7 +927,7 @@ int tszp_plain_compress_batch(tszp_batch_field* fields, uint32_t count, void* st
   ProfScope prof(kProfBatch, st);
   for (size_t k = 0; k < classes.size(); k++) {
     const BatchClass& cl = classes[k];
-    HIPCHK(launch_compress_batch(cl.bs, cl.v4, tab, tmap, cl.base, cl.ntiles, status, res,
+    HIPCHK(launch_compress_batch(kModePlain, cl.bs, cl.v4, 2, tab, tmap, cl.base, cl.ntiles, status, nullptr, res,
                                  dyn_counter(counters + k), st));
   }
   prof.stop();
@@ -953,7 +962,8 @@ int tszp_plain_decompress_batch(const tszp_batch_field* fields, uint32_t count, 
   WS(kWsBatchRes, (size_t)count * sizeof(CompressResult), res);
   HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(res), 0u, (size_t)count * sizeof(CompressResult) / 4, st));
   ProfScope prof(kProfBatchDecode, st);
-  for (const BatchClass& k : classes) HIPCHK(launch_decode_batch(k.bs, tab, tmap, k.base, k.ntiles, res, st));
+  for (const BatchClass& k : classes)
+    HIPCHK(launch_decode_batch(kModePlain, k.bs, tab, tmap, k.base, k.ntiles, res, st));
   prof.stop();
   CompressResult* h;
   if ((r = batch_results(c, res, count, st, &h))) return r;
@@ -1114,8 +1124,8 @@ int tszp_batch_plan_compress(tszp_batch_plan* plan, void* stream) {
   uint32_t* counters = reinterpret_cast<uint32_t*>(plan->status + plan->tiles);
   for (size_t k = 0; k < plan->classes.size(); k++) {
     const BatchClass& cl = plan->classes[k];
-    HIPCHK(launch_compress_batch(cl.bs, cl.v4, plan->ctab, plan->tmap, cl.base, cl.ntiles, plan->status, plan->cres,
-                                 dyn_counter(counters + k), st));
+    HIPCHK(launch_compress_batch(kModePlain, cl.bs, cl.v4, 2, plan->ctab, plan->tmap, cl.base, cl.ntiles, plan->status,
+                                 nullptr, plan->cres, dyn_counter(counters + k), st));
   }
   return TSZP_OK;
 }
@@ -1128,7 +1138,7 @@ int tszp_batch_plan_decompress(tszp_batch_plan* plan, void* stream) {
   fr.add(reinterpret_cast<uint32_t*>(plan->dres), 0u, (size_t)plan->count * sizeof(CompressResult) / 4);
   HIPCHK(launch_fill_multi(fr, st));
   for (const BatchClass& k : plan->classes)
-    HIPCHK(launch_decode_batch(k.bs, plan->dtab, plan->tmap, k.base, k.ntiles, plan->dres, st));
+    HIPCHK(launch_decode_batch(kModePlain, k.bs, plan->dtab, plan->tmap, k.base, k.ntiles, plan->dres, st));
   return TSZP_OK;
 }
 
@@ -1565,30 +1575,28 @@ int tszp_classify(const float* d_field, int ndim, const uint32_t* dims, uint8_t*
   return TSZP_OK;
 }
 
-int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_types, const int32_t* d_sp,
-                      uint64_t n_ext, int ndim, const uint32_t* dims, float eb, int max_rounds,
-                      tszp_restore_info* info, void* stream) {
-  g_err.clear();
-  if (!d_decoded || !d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0) ||
-      max_rounds < 0)
-    return fail(TSZP_EINVAL, "topo_restore: bad arguments");
-  if (reinterpret_cast<uintptr_t>(d_types) & 3u) return fail(TSZP_EINVAL, "topo_restore: types must be 4-byte aligned");
-  size_t n = 1;
-  for (int k = 0; k < ndim; k++) n *= dims[k];
-  int r = check_n(n);
-  if (r) return r;
-  if (n_ext > n) return fail(TSZP_EINVAL, "topo_restore: %llu extrema for %zu points", (unsigned long long)n_ext, n);
-  Ctx* c;
-  if ((r = get_ctx(&c))) return r;
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipStream_t st = (hipStream_t)stream;
-  const Grid g = make_grid(ndim, dims);
-  const uint32_t N = (uint32_t)n;
-  const float* d = d_decoded;  // the decoded values define every point's interval
+}  // extern "C"
+
+namespace {
+
+// Counts of one restoration run
+struct RestoreTotals {
+  uint64_t verify[4];  // missing, spurious, changed, max |y - d| bits (float)
+  uint64_t moves, windows, freed;
+  uint32_t rounds;
+};
+
+// The restoration's rounds over one field or a batch of fields in flat arrays (geo):
+// start values, small plateaus, local-search rounds re-checked around the previous
+// mismatches, exact window solves, a final verifying pass.  d: decoded values (not y),
+// y: output; n_ext_one: extrema of one field (checked against the types).
+int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types, const int32_t* d_sp,
+                uint64_t n_ext_one, bool use_sp, float* d_field, int max_rounds, RestoreTotals* out, hipStream_t st) {
+  const uint32_t N = geo.n;
+  const size_t n = N;
+  const int ndim = geo.ndim;
   uint32_t *bad, *bad2, *act, *bits, *misc;
   uint64_t* status;
-  float* dcopy = nullptr;
-  if (d_decoded == d_field) WS(kWsPpD, n * 4, dcopy);  // in place: keep a copy
   // mismatch lists: a quarter of the points (more are listed on the next full pass)
   const uint32_t cap = N / 4 + 4096;
   WS(kWsPpBad, (size_t)cap * 4, bad);
@@ -1603,15 +1611,9 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     HIPCHK(hipStreamSynchronize(st));
     return TSZP_OK;
   };
-  ProfScope prof(kProfRestore, st);
-  if (dcopy) {
-    HIPCHK(hipMemcpyAsync(dcopy, d_decoded, n * 4, hipMemcpyDeviceToDevice, st));
-    d = dcopy;
-  }
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
-  const bool use_sp = d_sp != nullptr && n_ext > 0;
-  if (use_sp) HIPCHK(pp_spmax(d_sp, (uint32_t)n_ext, misc, st));
-  HIPCHK(pp_init(d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext, N, eb, d_field,
+  if (use_sp) HIPCHK(pp_spmax(geo, d_sp, (uint32_t)n_ext_one, misc, st));
+  HIPCHK(pp_init(geo, d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext_one, d_field,
                  reinterpret_cast<uint32_t*>(status), misc, st));
   // full marking pass: fresh list of mismatches and the verification counts
   // The list `bad` is exact when it holds every current mismatch; the verification
@@ -1619,14 +1621,14 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   bool exact = false, verified = false;
   uint32_t* part;
   WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + pp_tiny_parts(cap) + 64) * 4, part);
-  const bool vec_mark = (ndim == 2 ? dims[1] : dims[2]) % 4 == 0;
+  const bool vec_mark = pp_mark_maps(geo);
   uint8_t* badmap = nullptr;
   auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint8_t* bmap = nullptr) -> int {
     FillRanges fr{};
     fr.add(misc + kPpBad, 0u, 1);
     fr.add(misc + kPpVerify, 0u, 8);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_mark(g, d_field, d_types, final_pass ? d : nullptr, N, bad, cap, part, bmap, misc, st));
+    HIPCHK(pp_mark(geo, d_field, d_types, final_pass ? d : nullptr, bad, cap, part, bmap, final_pass, misc, st));
     int rr = read_misc();
     if (rr) return rr;
     *nb_out = h[kPpBad];
@@ -1635,6 +1637,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     return TSZP_OK;
   };
   uint32_t nb = 0;
+  int r;
   WS(kWsPpBadmap, n + 16, badmap);
   if (vec_mark) {
     if ((r = full_mark(&nb, false, badmap))) return r;
@@ -1643,9 +1646,9 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   }
   const bool stats = getenv("TSZP_RESTORE_STATS") != nullptr;
   if (stats) fprintf(stderr, "[restore] after init: %u mismatches\n", nb);
-  if (use_sp && h[kPpExt] != n_ext)
+  if (!geo.fields && use_sp && h[kPpExt] != n_ext_one)
     return fail(TSZP_EINVAL, "topo_restore: %u extrema in the types, %llu sort positions given", h[kPpExt],
-                (unsigned long long)n_ext);
+                (unsigned long long)n_ext_one);
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
   uint32_t tiny = 0;
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
@@ -1665,9 +1668,9 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     fr.add(misc + kPpBad, 0u, 1);
     fr.add(bits, 0u, bit_words);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_rel(g, d, N, rel, st));
-    HIPCHK(pp_tiny(g, bad, nb, d, rel, badmap, d_types, d_field, eb, use_sp, plist, part, misc, st));
-    HIPCHK(pp_remark(g, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
+    HIPCHK(pp_rel(geo, d, rel, st));
+    HIPCHK(pp_tiny(geo, bad, nb, d, rel, badmap, d_types, d_field, use_sp, plist, part, misc, st));
+    HIPCHK(pp_remark(geo, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
     if ((r = read_misc())) return r;
     std::swap(bad, bad2);
     if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);
@@ -1693,9 +1696,8 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       fr.add(misc + kPpMoves, 0u, 1);
       fr.add(misc + kPpActC, 0u, colours);
       HIPCHK(launch_fill_multi(fr, st));
-      HIPCHK(pp_active(g, bad, nl, d, d_types, bits, act, nl, use_sp, misc, st));
-      for (int col = 0; col < colours; col++)
-        HIPCHK(pp_move(g, act, nl, nl, col, d, d_types, d_field, eb, misc, st));
+      HIPCHK(pp_active(geo, bad, nl, d, d_types, bits, act, nl, use_sp, misc, st));
+      for (int col = 0; col < colours; col++) HIPCHK(pp_move(geo, act, nl, nl, col, d, d_types, d_field, misc, st));
       rounds++;
       verified = false;
       // re-check around the listed mismatches, or the whole field when that is cheaper
@@ -1705,7 +1707,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
         fr.add(bits, 0u, bit_words);
         fr.add(misc + kPpBad, 0u, 1);
         HIPCHK(launch_fill_multi(fr, st));
-        HIPCHK(pp_remark(g, bad, nl, d_field, d_types, bits, bad2, cap, 2, misc, st));
+        HIPCHK(pp_remark(geo, bad, nl, d_field, d_types, bits, bad2, cap, 2, misc, st));
         if ((r = read_misc())) return r;
         std::swap(bad, bad2);
         nb = h[kPpBad];
@@ -1728,9 +1730,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       if (!nb) break;
     }
     // exact window solve around what is left
-    uint32_t tx, ty, tz;
-    pp_tile_counts(g, &tx, &ty, &tz);
-    const size_t ntiles = (size_t)tx * ty * tz;
+    const size_t ntiles = geo.ntiles;
     const uint32_t nl = nb < cap ? nb : cap;
     const uint32_t tcap = (uint32_t)(nl < ntiles ? nl : ntiles);
     uint32_t *tbits, *tiles;
@@ -1747,7 +1747,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       fr.add(misc + kPpStatNodes, 0u, 12);
       HIPCHK(launch_fill_multi(fr, st));
     }
-    HIPCHK(pp_tiles(g, bad, nl, ty, tz, bits, tbits, tiles, tcap, misc, st));
+    HIPCHK(pp_tiles(geo, bad, nl, bits, tbits, tiles, tcap, misc, st));
     if ((r = read_misc())) return r;
     uint32_t total = 0;
     for (int ph = 0; ph < phases; ph++) total += h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
@@ -1757,7 +1757,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
         const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
         if (!cnt) continue;
         solve_timer.begin(st);
-        HIPCHK(pp_solve(g, tiles, cnt, tcap, ty, tz, ph, bits, d, d_types, d_field, eb, use_sp, misc, st));
+        HIPCHK(pp_solve(geo, tiles, cnt, tcap, ph, bits, d, d_types, d_field, use_sp, misc, st));
         solve_timer.end(st);
       }
     } else if (total) {  // every phase in one launch, ordered per neighbouring tiles
@@ -1768,8 +1768,7 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
       fr.add(misc + kPpFlowCounter, 0u, 1);
       HIPCHK(launch_fill_multi(fr, st));
       solve_timer.begin(st);
-      HIPCHK(pp_solve_flow(g, tiles, total, tcap, tx, ty, tz, tbits, done, bits, d, d_types, d_field, eb, use_sp,
-                           misc, st));
+      HIPCHK(pp_solve_flow(geo, tiles, total, tcap, tbits, done, bits, d, d_types, d_field, use_sp, misc, st));
       solve_timer.end(st);
     }
     if ((r = full_mark(&nb, true))) return r;
@@ -1785,17 +1784,162 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
     if (!h[kPpSolved] && outer > 0) break;
   }
   if (!verified && (r = full_mark(&nb, true))) return r;
-  prof.stop();
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
   const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);
-  info->missing = hv[0];
-  info->spurious = hv[1];
-  info->changed = hv[2];
-  info->moves = moves;
-  info->windows = solved + tiny;
-  info->windows_reordered = freed;
-  info->rounds = rounds;
-  const uint32_t dv = (uint32_t)hv[3];
+  for (int k = 0; k < 4; k++) out->verify[k] = hv[k];
+  out->moves = moves;
+  out->windows = solved + tiny;
+  out->freed = freed;
+  out->rounds = rounds;
+  return TSZP_OK;
+}
+
+}  // namespace
+
+int tszp::restore_batch(const RestoreBatchField* fields, uint32_t count, uint64_t flat, const float* d,
+                        const uint8_t* t, const int32_t* sp, float* y, int max_rounds, tszp_restore_info* info,
+                        hipStream_t st) {
+  if (!fields || count == 0 || !d || !t || !sp || !y || !info) return fail(TSZP_EINVAL, "restore_batch: bad arguments");
+  if (flat == 0 || flat % kFlatAlign || flat >= 0x80000000ull)
+    return fail(TSZP_EINVAL, "restore_batch: flat size must be a positive multiple of %u below 2^31", kFlatAlign);
+  const int nd = fields[0].ndim;
+  Ctx* c;
+  int r = get_ctx(&c);
+  if (r) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  // the fields of each dimensionality form one restoration (its kernels are per dimension)
+  std::vector<PpField> pf;
+  std::vector<uint32_t> fmap(flat >> kFlatShift, 0u), tmap;
+  for (int dim = 2; dim <= 3; dim++) {
+    pf.clear();
+    std::vector<uint32_t> which;
+    uint32_t tiles = 0;
+    for (uint32_t i = 0; i < count; i++) {
+      const RestoreBatchField& f = fields[i];
+      if (f.ndim != dim) continue;
+      if (f.base % kFlatAlign) return fail(TSZP_EINVAL, "restore_batch: field %u: offset not aligned", i);
+      PpField p{};
+      p.g = make_grid(f.ndim, f.dims);
+      p.base = (uint32_t)f.base;
+      uint64_t n = 1;
+      for (int a = 0; a < f.ndim; a++) n *= f.dims[a];
+      p.n = (uint32_t)n;
+      if (f.base + n > flat) return fail(TSZP_EINVAL, "restore_batch: field %u beyond the flat arrays", i);
+      p.eb = f.eb;
+      pp_tile_counts(p.g, &p.tx, &p.ty, &p.tz);
+      p.tile0 = tiles;
+      tiles += p.tx * p.ty * p.tz;
+      p.n_ext = f.n_ext_dev;
+      which.push_back(i);
+      pf.push_back(p);
+    }
+    if (pf.empty()) continue;
+    const uint32_t m = (uint32_t)pf.size();
+    // entry m: an empty field for the chunks of the other dimensionality (every pass
+    // skips points at or past base + n of their field)
+    pf.push_back(PpField{});
+    tmap.assign(tiles, 0u);
+    std::fill(fmap.begin(), fmap.end(), m);
+    for (uint32_t k = 0; k < m; k++) {
+      const uint32_t c0 = pf[k].base >> kFlatShift, c1 = (uint32_t)((pf[k].base + (uint64_t)pf[k].n + kFlatAlign - 1) >> kFlatShift);
+      for (uint32_t q = c0; q < c1; q++) fmap[q] = k;
+      for (uint32_t q = 0; q < pf[k].tx * pf[k].ty * pf[k].tz; q++) tmap[pf[k].tile0 + q] = k;
+    }
+    // device tables: fields, flat map, tile map, per-field sort maxima and counts
+    const size_t tb = ((m + 1) * sizeof(PpField) + 255) & ~(size_t)255, fb = (fmap.size() * 4 + 255) & ~(size_t)255,
+                 mb = (tmap.size() * 4 + 255) & ~(size_t)255, sb = ((m + 1) * 4 + 255) & ~(size_t)255,
+                 cb = ((m + 1) * 32 + 255) & ~(size_t)255;
+    char* tabs;
+    WS(kWsPpFields, tb + fb + mb + sb + cb + 256, tabs);
+    // the empty field's extremum count: a zero word after the per-field counts
+    uint64_t* zero_word = reinterpret_cast<uint64_t*>(tabs + tb + fb + mb + sb + cb);
+    pf[m].n_ext = zero_word;
+    pf[m].g = pf[0].g;
+    PpGeo geo{};
+    geo.ndim = dim;
+    geo.n = (uint32_t)flat;
+    geo.fields = reinterpret_cast<const PpField*>(tabs);
+    geo.fmap = reinterpret_cast<const uint32_t*>(tabs + tb);
+    geo.tmap = reinterpret_cast<const uint32_t*>(tabs + tb + fb);
+    geo.spmax = reinterpret_cast<uint32_t*>(tabs + tb + fb + mb);
+    geo.fcount = reinterpret_cast<unsigned long long*>(tabs + tb + fb + mb + sb);
+    geo.nfields = m + 1;
+    geo.ntiles = tiles;
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(zero_word), 0u, 2, st));
+    HIPCHK(hipMemcpyAsync(tabs, pf.data(), (m + 1) * sizeof(PpField), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tabs + tb, fmap.data(), fmap.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tabs + tb + fb, tmap.data(), tmap.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // the host tables are reused for the next dimensionality
+    RestoreTotals tot{};
+    if ((r = restore_run(c, geo, d, t, sp, 0, true, y, max_rounds, &tot, st))) return r;
+    std::vector<unsigned long long> fc((size_t)m * 4);
+    HIPCHK(hipMemcpyAsync(fc.data(), geo.fcount, fc.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (uint32_t k = 0; k < m; k++) {
+      tszp_restore_info& ri = info[which[k]];
+      ri.missing = fc[4 * k];
+      ri.spurious = fc[4 * k + 1];
+      ri.changed = fc[4 * k + 2];
+      const uint32_t dv = (uint32_t)fc[4 * k + 3];
+      float dev;
+      memcpy(&dev, &dv, 4);
+      ri.max_dev_over_eb = dev / pf[k].eb;
+      ri.moves = tot.moves;
+      ri.windows = tot.windows;
+      ri.windows_reordered = tot.freed;
+      ri.rounds = tot.rounds;
+    }
+  }
+  (void)nd;
+  return TSZP_OK;
+}
+
+extern "C" {
+
+int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_types, const int32_t* d_sp,
+                      uint64_t n_ext, int ndim, const uint32_t* dims, float eb, int max_rounds,
+                      tszp_restore_info* info, void* stream) {
+  g_err.clear();
+  if (!d_decoded || !d_field || !d_types || !dims || !info || (ndim != 2 && ndim != 3) || !(eb > 0) ||
+      max_rounds < 0)
+    return fail(TSZP_EINVAL, "topo_restore: bad arguments");
+  if (reinterpret_cast<uintptr_t>(d_types) & 3u) return fail(TSZP_EINVAL, "topo_restore: types must be 4-byte aligned");
+  size_t n = 1;
+  for (int k = 0; k < ndim; k++) n *= dims[k];
+  int r = check_n(n);
+  if (r) return r;
+  if (n_ext > n) return fail(TSZP_EINVAL, "topo_restore: %llu extrema for %zu points", (unsigned long long)n_ext, n);
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  PpGeo geo{};
+  geo.ndim = ndim;
+  geo.n = (uint32_t)n;
+  geo.g = make_grid(ndim, dims);
+  geo.eb = eb;
+  pp_tile_counts(geo.g, &geo.tx, &geo.ty, &geo.tz);
+  geo.ntiles = geo.tx * geo.ty * geo.tz;
+  const float* d = d_decoded;  // the decoded values define every point's interval
+  ProfScope prof(kProfRestore, st);
+  if (d_decoded == d_field) {  // in place: keep a copy
+    float* dcopy;
+    WS(kWsPpD, n * 4, dcopy);
+    HIPCHK(hipMemcpyAsync(dcopy, d_decoded, n * 4, hipMemcpyDeviceToDevice, st));
+    d = dcopy;
+  }
+  const bool use_sp = d_sp != nullptr && n_ext > 0;
+  RestoreTotals tot{};
+  if ((r = restore_run(c, geo, d, d_types, d_sp, n_ext, use_sp, d_field, max_rounds, &tot, st))) return r;
+  prof.stop();
+  info->missing = tot.verify[0];
+  info->spurious = tot.verify[1];
+  info->changed = tot.verify[2];
+  info->moves = tot.moves;
+  info->windows = tot.windows;
+  info->windows_reordered = tot.freed;
+  info->rounds = tot.rounds;
+  const uint32_t dv = (uint32_t)tot.verify[3];
   float dev;
   memcpy(&dev, &dv, 4);
   info->max_dev_over_eb = dev / eb;
@@ -1820,7 +1964,11 @@ int tszp_topo_verify(const float* d_field, const uint8_t* d_types, int ndim, con
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
   uint32_t* part;
   WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + 64) * 4, part);
-  HIPCHK(pp_mark(make_grid(ndim, dims), d_field, d_types, nullptr, (uint32_t)n, nullptr, 0, part, nullptr, misc, st));
+  PpGeo geo{};
+  geo.ndim = ndim;
+  geo.n = (uint32_t)n;
+  geo.g = make_grid(ndim, dims);
+  HIPCHK(pp_mark(geo, d_field, d_types, nullptr, nullptr, 0, part, nullptr, false, misc, st));
   uint64_t hv[4];
   HIPCHK(hipMemcpyAsync(hv, misc + kPpVerify, 32, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -942,11 +942,28 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_tilemap(const BatchFiel
   for (uint32_t t = threadIdx.x; t < nt; t += kBlockThreads) tmap[t0 + t] = fi;
 }
 
-template <int S, int EPL, bool V4>
+// Element count, blocks and tiles of a field in a batch launch: n_dev (when set) holds
+// the count on the device, the host numbered tiles for the largest.
+template <int S, int EPL>
+__device__ __forceinline__ void batch_counts(const BatchField& f, uint32_t* n, uint32_t* nb, uint32_t* ntiles) {
+  if (f.n_dev) {
+    const uint64_t m = *f.n_dev;
+    *n = (uint32_t)(m < f.n ? m : f.n);
+    *nb = (*n + f.bs - 1) / f.bs;
+    *ntiles = (*nb + Geom<S, EPL>::BPT - 1) / Geom<S, EPL>::BPT;
+  } else {
+    *n = f.n;
+    *nb = f.nb;
+    *ntiles = f.ntiles;
+  }
+}
+
+template <int MODE, int S, int EPL, bool V4, int ND>
 __global__ __launch_bounds__(kBlockThreads, V4 ? 4 : 1) void k_compress_batch(const BatchField* __restrict__ fields,
                                                                             const uint32_t* __restrict__ tmap,
                                                                             uint32_t base,
                                                                             uint64_t* __restrict__ status,
+                                                                            uint64_t* __restrict__ status_cnt,
                                                                             CompressResult* __restrict__ res,
                                                                             uint32_t* counter) {
   __shared__ uint32_t sh_gt;
@@ -956,33 +973,48 @@ __global__ __launch_bounds__(kBlockThreads, V4 ? 4 : 1) void k_compress_batch(co
   const uint32_t fi = tmap[gt];
   const BatchField& f = fields[fi];
   CompressArgs a{};
+  // (the 16-byte path runs S lanes of 4 elements per block: Geom<S, 4> is its tile)
+  batch_counts<S, EPL>(f, &a.n, &a.nb, &a.ntiles);
+  const uint32_t tile = gt - f.tile0;
+  if (tile >= a.ntiles) return;  // past the blocks of a count read on the device
   a.in = f.in;
-  a.n = f.n;
   a.bs = f.bs;
-  a.nb = f.nb;
-  a.ntiles = f.ntiles;
   a.eb = f.eb;
   a.inv = f.inv;
-  a.payload = reinterpret_cast<uint8_t*>(f.out) + kBatchHdr;
+  a.grid = f.grid;
+  a.payload = reinterpret_cast<uint8_t*>(f.out) + f.hdr;
   a.cap = f.cap;
   a.blk = f.blk;
   a.st_bytes = status + f.tile0;
+  a.st_cnt = status_cnt ? status_cnt + f.tile0 : nullptr;
+  a.cp_key = f.cp_key;
+  a.cp_pay = f.cp_pay;
+  a.cp_cap = f.cp_cap;
   a.res = res + fi;
-  const uint32_t tile = gt - f.tile0;
-  if (tile == 0 && threadIdx.x == 0) {  // header: eb big-endian, then chunk offset 0
+  if (tile == 0 && threadIdx.x == 0) {
     uint32_t* h = reinterpret_cast<uint32_t*>(f.out);
-    h[0] = __builtin_bswap32(__float_as_uint(f.eb));
-    h[1] = 0u;
-    h[2] = 0u;
+    if constexpr (MODE == kModePlain) {  // eb big-endian, then chunk offset 0
+      h[0] = __builtin_bswap32(__float_as_uint(f.eb));
+      h[1] = 0u;
+      h[2] = 0u;
+    } else if constexpr (MODE == kModeInt) {  // extremum count, then chunk offset 0
+      h[0] = a.n;
+      h[1] = 0u;
+      h[2] = 0u;
+      h[3] = 0u;
+    } else {  // chunk offset 0
+      h[0] = 0u;
+      h[1] = 0u;
+    }
   }
   if constexpr (V4) {
-    compress_v4_tile<kModePlain, S, 2>(a, tile);
+    compress_v4_tile<MODE, S, ND>(a, tile);
   } else {
-    compress_tile<kModePlain, S, EPL>(a, tile);
+    compress_tile<MODE, S, EPL>(a, tile);
   }
 }
 
-template <int S, int EPL>
+template <int MODE, int S, int EPL>
 __global__ __launch_bounds__(kBlockThreads) void k_decode_batch(const BatchField* __restrict__ fields,
                                                                const uint32_t* __restrict__ tmap, uint32_t base,
                                                                CompressResult* __restrict__ res) {
@@ -990,15 +1022,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_decode_batch(const BatchField
   const uint32_t fi = tmap[gt];
   const BatchField& f = fields[fi];
   DecodeArgs a{};
-  a.payload = reinterpret_cast<const uint8_t*>(f.in) + kBatchHdr;
+  uint32_t ntiles;
+  batch_counts<S, EPL>(f, &a.n, &a.nb, &ntiles);
+  const uint32_t tile = gt - f.tile0;
+  if (tile >= ntiles) return;
+  a.payload = reinterpret_cast<const uint8_t*>(f.in) + f.hdr;
   a.blk = f.blk;
-  a.n = f.n;
   a.bs = f.bs;
-  a.nb = f.nb;
   a.eb = f.eb;
   a.out = f.out;
+  a.types8 = f.types8;
   a.err = &res[fi].error;
-  decode_tile<kModePlain, S, EPL>(a, gt - f.tile0);
+  decode_tile<MODE, S, EPL>(a, tile);
 }
 
 // Payload bytes of sampled windows at each candidate block size: one workgroup per
@@ -1188,61 +1223,102 @@ hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32
   return hipGetLastError();
 }
 
-hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
-                                 uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
-                                 uint32_t* counter, hipStream_t st) {
-  if (!ntiles) return hipSuccess;
-  if (v4) {  // block size 16, 32, 64, 128 or 256, every input of the class 16-byte aligned
-    if (bs_class == 256)
-      hipLaunchKernelGGL((k_compress_batch<64, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res, counter);
-    else if (bs_class == 16)
-      hipLaunchKernelGGL((k_compress_batch<4, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res, counter);
-    else if (bs_class == 128)
-      hipLaunchKernelGGL((k_compress_batch<32, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res, counter);
-    else if (bs_class == 64)
-      hipLaunchKernelGGL((k_compress_batch<16, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res, counter);
-    else if (bs_class == 32)
-      hipLaunchKernelGGL((k_compress_batch<8, 4, true>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap,
-                         base, status, res, counter);
-    else
-      return hipErrorInvalidValue;
-    return hipGetLastError();
+template <int MODE, int S, int EPL, bool V4, int ND>
+static hipError_t launch_cb(const BatchField* fields, const uint32_t* tmap, uint32_t base, uint32_t ntiles,
+                            uint64_t* status, uint64_t* status_cnt, CompressResult* res, uint32_t* counter,
+                            hipStream_t st) {
+  hipLaunchKernelGGL((k_compress_batch<MODE, S, EPL, V4, ND>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields,
+                     tmap, base, status, status_cnt, res, counter);
+  return hipGetLastError();
+}
+
+template <int MODE, int ND>
+static hipError_t launch_cb_v4(uint32_t bs, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                               uint32_t ntiles, uint64_t* status, uint64_t* status_cnt, CompressResult* res,
+                               uint32_t* counter, hipStream_t st) {
+  switch (bs) {
+    case 16: return launch_cb<MODE, 4, 4, true, ND>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
+    case 32: return launch_cb<MODE, 8, 4, true, ND>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
+    case 64: return launch_cb<MODE, 16, 4, true, ND>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
+    case 128: return launch_cb<MODE, 32, 4, true, ND>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
+    case 256: return launch_cb<MODE, 64, 4, true, ND>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
   }
+  return hipErrorInvalidValue;
+}
+
+// generic geometries: every block size up to 257 (topology and sort positions), up to
+// 1025 (plain)
+template <int MODE>
+static hipError_t launch_cb_generic(uint32_t bs, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                                    uint32_t ntiles, uint64_t* status, uint64_t* status_cnt, CompressResult* res,
+                                    uint32_t* counter, hipStream_t st) {
   int S, EPL;
   uint32_t bpt;
-  block_geometry(bs_class, &S, &EPL, &bpt);
-#define TSZP_CASE(SS, EE)                                                                            \
-  if (S == SS && EPL == EE) {                                                                        \
-    hipLaunchKernelGGL((k_compress_batch<SS, EE, false>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, \
-                       base, status, res, counter);                                                  \
-    return hipGetLastError();                                                                        \
-  }
+  block_geometry(bs, &S, &EPL, &bpt);
+#define TSZP_CASE(SS, EE)                                                                                      \
+  if (S == SS && EPL == EE)                                                                                    \
+    return launch_cb<MODE, SS, EE, false, 2>(fields, tmap, base, ntiles, status, status_cnt, res, counter, st);
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
-  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8) TSZP_CASE(64, 16)
-  TSZP_CASE(64, 32)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  if constexpr (MODE == kModePlain) {
+    TSZP_CASE(64, 16) TSZP_CASE(64, 32)
+  }
 #undef TSZP_CASE
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
-                               uint32_t ntiles, CompressResult* res, hipStream_t st) {
+hipError_t launch_compress_batch(int mode, uint32_t bs_class, bool v4, int nd, const BatchField* fields,
+                                 const uint32_t* tmap, uint32_t base, uint32_t ntiles, uint64_t* status,
+                                 uint64_t* status_cnt, CompressResult* res, uint32_t* counter, hipStream_t st) {
   if (!ntiles) return hipSuccess;
+  switch (mode) {
+    case kModePlain:
+      return v4 ? launch_cb_v4<kModePlain, 2>(bs_class, fields, tmap, base, ntiles, status, nullptr, res, counter, st)
+                : launch_cb_generic<kModePlain>(bs_class, fields, tmap, base, ntiles, status, nullptr, res, counter, st);
+    case kModeInt:
+      return v4 ? launch_cb_v4<kModeInt, 2>(bs_class, fields, tmap, base, ntiles, status, nullptr, res, counter, st)
+                : launch_cb_generic<kModeInt>(bs_class, fields, tmap, base, ntiles, status, nullptr, res, counter, st);
+    case kModeTopoClassify:
+      if (!v4)
+        return launch_cb_generic<kModeTopoClassify>(bs_class, fields, tmap, base, ntiles, status, status_cnt, res,
+                                                    counter, st);
+      return nd == 3 ? launch_cb_v4<kModeTopoClassify, 3>(bs_class, fields, tmap, base, ntiles, status, status_cnt,
+                                                          res, counter, st)
+                     : launch_cb_v4<kModeTopoClassify, 2>(bs_class, fields, tmap, base, ntiles, status, status_cnt,
+                                                          res, counter, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int MODE>
+static hipError_t launch_db(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
+                            uint32_t ntiles, CompressResult* res, hipStream_t st) {
   int S, EPL;
   uint32_t bpt;
   block_geometry(bs_class, &S, &EPL, &bpt);
 #define TSZP_CASE(SS, EE)                                                                                 \
   if (S == SS && EPL == EE) {                                                                             \
-    hipLaunchKernelGGL((k_decode_batch<SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, base, res); \
+    hipLaunchKernelGGL((k_decode_batch<MODE, SS, EE>), dim3(ntiles), dim3(kBlockThreads), 0, st, fields, tmap, \
+                       base, res);                                                                        \
     return hipGetLastError();                                                                             \
   }
   TSZP_CASE(1, 1) TSZP_CASE(2, 1) TSZP_CASE(4, 1) TSZP_CASE(8, 1) TSZP_CASE(16, 1)
-  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8) TSZP_CASE(64, 16)
-  TSZP_CASE(64, 32)
+  TSZP_CASE(8, 4) TSZP_CASE(16, 4) TSZP_CASE(64, 2) TSZP_CASE(64, 4) TSZP_CASE(64, 8)
+  if constexpr (MODE == kModePlain) {
+    TSZP_CASE(64, 16) TSZP_CASE(64, 32)
+  }
 #undef TSZP_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_decode_batch(int mode, uint32_t bs_class, const BatchField* fields, const uint32_t* tmap,
+                               uint32_t base, uint32_t ntiles, CompressResult* res, hipStream_t st) {
+  if (!ntiles) return hipSuccess;
+  switch (mode) {
+    case kModePlain: return launch_db<kModePlain>(bs_class, fields, tmap, base, ntiles, res, st);
+    case kModeTopoMap: return launch_db<kModeTopoMap>(bs_class, fields, tmap, base, ntiles, res, st);
+    case kModeInt: return launch_db<kModeInt>(bs_class, fields, tmap, base, ntiles, res, st);
+  }
   return hipErrorInvalidValue;
 }
 

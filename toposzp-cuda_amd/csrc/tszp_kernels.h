@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "tszp_types.h"
+#include "../../include/toposzp_hip.h"
 
 namespace tszp {
 
@@ -52,28 +53,44 @@ void block_geometry(uint32_t bs, int* S, int* EPL, uint32_t* blocks_per_tile);
 hipError_t launch_compress(int mode, const CompressArgs& a, hipStream_t st);
 hipError_t launch_decode(int mode, const DecodeArgs& a, hipStream_t st);
 
-// ---- batches of plain float fields (tszp_codec.hip) ----
+// ---- batches of fields (tszp_codec.hip) ----
 // One entry per field; the fields of one launch share a block-size class (block_geometry)
-// and number their tiles contiguously from tile0 in field order.
-constexpr uint32_t kBatchHdr = 12;  // [f32 eb big-endian][u64 off[1] = 0]
+// and number their tiles contiguously from tile0 in field order.  Streams have one
+// chunk: plain [f32 eb big-endian][u64 0], topology [u64 0], sort positions
+// [u64 E][u64 0] (the header bytes before the payload are `hdr`).
+constexpr uint32_t kBatchHdr = 12;      // plain: [f32 eb big-endian][u64 off[1] = 0]
+constexpr uint32_t kBatchTopoHdr = 8;   // topology: [u64 off[1] = 0]
+constexpr uint32_t kBatchSpHdr = 16;    // sort positions: [u64 E][u64 off[1] = 0]
 struct BatchField {
-  const void* in;         // compress: float[n]; decode: the stream (header included)
-  void* out;              // compress: the stream (4-byte aligned); decode: float[n]
+  const void* in;         // compress: float[n] (int[n] for sort positions); decode: the stream
+  void* out;              // compress: the stream (4-byte aligned); decode: float[n] (int[n])
   uint32_t* blk;          // nb + 1 block offsets relative to the payload
   uint64_t cap;           // compress: payload capacity in bytes
-  uint32_t n, bs, nb, ntiles, tile0;
+  uint32_t n, bs, nb, ntiles, tile0;  // n, nb, ntiles: the largest when n_dev is set
   float eb;
   double inv;             // compress: reciprocal used for quantization
+  uint32_t hdr;           // header bytes before the payload
+  const uint64_t* n_dev;  // nullable: element count read on the device (<= n); the
+                          // tiles past its blocks exit at once
+  Grid grid;              // topology compression: the field's geometry
+  uint32_t* cp_key;       // topology compression: critical-point keys and payloads, row-major
+  uint32_t* cp_pay;
+  uint32_t cp_cap;
+  uint8_t* types8;        // topology decode: uint8 types (nullable)
 };
 hipError_t launch_batch_tilemap(const BatchField* fields, uint32_t count, uint32_t* tmap, hipStream_t st);
 // tiles [base, base + ntiles) of the batch's numbering, all of one block-size class;
-// v4: the 16-byte path (block size 16, 32, 64, 128 or 256, every input 16-byte aligned);
-// counter (nullable, zeroed): tiles handed out from it instead of workgroup ids
-hipError_t launch_compress_batch(uint32_t bs_class, bool v4, const BatchField* fields, const uint32_t* tmap,
-                                 uint32_t base, uint32_t ntiles, uint64_t* status, CompressResult* res,
-                                 uint32_t* counter, hipStream_t st);
-hipError_t launch_decode_batch(uint32_t bs_class, const BatchField* fields, const uint32_t* tmap, uint32_t base,
-                               uint32_t ntiles, CompressResult* res, hipStream_t st);
+// mode kModePlain, kModeTopoClassify (status_cnt: second look-back words, for the
+// critical-point counts) or kModeInt; v4: the 16-byte path (block size 16, 32, 64, 128
+// or 256, every input 16-byte aligned, classification: every fastest axis a multiple
+// of 4 and every field of the class `nd`-dimensional); counter (nullable, zeroed):
+// tiles handed out from it instead of workgroup ids
+hipError_t launch_compress_batch(int mode, uint32_t bs_class, bool v4, int nd, const BatchField* fields,
+                                 const uint32_t* tmap, uint32_t base, uint32_t ntiles, uint64_t* status,
+                                 uint64_t* status_cnt, CompressResult* res, uint32_t* counter, hipStream_t st);
+// mode kModePlain, kModeTopoMap (values and types) or kModeInt
+hipError_t launch_decode_batch(int mode, uint32_t bs_class, const BatchField* fields, const uint32_t* tmap,
+                               uint32_t base, uint32_t ntiles, CompressResult* res, hipStream_t st);
 // Estimated payload bytes of every field at each block size of kTuneSizes, from
 // sampled windows of kTuneWindow elements (est: count * kTuneCands words).
 constexpr int kTuneCands = 4;
@@ -85,6 +102,75 @@ hipError_t launch_batch_tune(const BatchField* fields, uint32_t count, uint32_t*
 // tszp_last_error() of the calling thread, for entry points outside tszp_api.cpp
 int set_error(int code, const char* msg);
 void clear_error();
+// the library's per-stage timers (tszp_prof_*), for entry points outside tszp_api.cpp
+enum : int { kProfTopoBatch = 15, kProfTopoBatchDecode = 16, kProfTopoBatchRestore = 17, kProfTopoBatchSort = 18 };
+bool prof_on();
+void prof_add(int kind, double ms);
+Grid grid_of(int ndim, const uint32_t* dims);
+
+// ---- batched topology pipeline (tszp_topo_batch.hip) ----
+// Fields of a batch live at aligned offsets of flat arrays (critical-point keys, sort
+// positions, decoded values, types, restored values): field f covers
+// [base, base + n) of every one of them.
+constexpr uint32_t kFlatAlign = 16384;
+struct TopoBatchDev {
+  uint64_t base;        // flat element offset (a multiple of kFlatAlign)
+  uint32_t n;           // elements
+  uint32_t chunk0;      // first gather chunk (kSortTile key slots each)
+  uint32_t nchunks;
+  uint8_t* sp_stream;   // the field's sort-position stream
+};
+// table order = the batch's field order; res[i] the topology compression result of field i
+hipError_t launch_tb_offsets(const TopoBatchDev* tab, uint32_t count, const CompressResult* res, uint32_t* cbase,
+                             uint32_t* total, uint32_t* err, hipStream_t st);
+hipError_t launch_tb_chunkmap(const TopoBatchDev* tab, uint32_t count, uint32_t* chunk_field, hipStream_t st);
+hipError_t launch_tb_gather(const TopoBatchDev* tab, const uint32_t* chunk_field, uint32_t nchunks,
+                            const CompressResult* res, const uint32_t* cbase, const uint32_t* cp_key,
+                            const uint32_t* cp_pay, uint32_t* keys, uint32_t* hi, uint32_t* pay, hipStream_t st);
+// ---- geometry of a restoration (tszp_post.hip) ----
+// One field, or a batch of fields in flat arrays: field f covers [base, base + n) of the
+// decoded values, types, sort positions (its k-th extremum at base + k) and restored
+// values, with base a multiple of kFlatAlign, so that the field of flat index e is
+// fmap[e / kFlatAlign].  Indices stay flat (a field's points are contiguous, so e +-
+// stride is a neighbour); coordinates are the field's own.
+constexpr int kFlatShift = 14;
+static_assert((1u << kFlatShift) == kFlatAlign, "flat alignment");
+struct PpField {
+  Grid g;
+  uint32_t base, n;
+  float eb;
+  uint32_t tile0, tx, ty, tz;  // window-solve tiles of the field: [tile0, tile0 + tx * ty * tz)
+  const uint64_t* n_ext;       // extremum count of the field (device)
+};
+struct PpGeo {
+  int ndim;
+  uint32_t n;                  // points (one field) or flat size (batch)
+  Grid g;                      // one field
+  float eb;                    // one field
+  const PpField* fields;       // batch: device table, null for one field
+  const uint32_t* fmap;        // batch: field of each kFlatAlign points
+  const uint32_t* tmap;        // batch: field of each window-solve tile
+  uint32_t* spmax;             // batch: largest sort position of each field
+  unsigned long long* fcount;  // batch: missing / spurious / changed / max |y - d| bits per field
+  uint32_t nfields;
+  uint32_t ntiles;             // window-solve tiles (all fields)
+  uint32_t tx, ty, tz;         // one field's solve tiles
+};
+
+// Restoration of a batch of fields laid out in flat arrays (tszp_api.cpp): decoded
+// values, types, sort positions (the k-th extremum of field f at base + k), restored
+// values; one entry per field in flat order.  The workspace is the library's per-device
+// one; info receives each field's missing / spurious / changed counts and largest
+// |y - d| (moves, windows and rounds are the batch's).
+struct RestoreBatchField {
+  int ndim;
+  uint32_t dims[3];
+  float eb;
+  uint64_t base;
+  const uint64_t* n_ext_dev;  // device: the field's extremum count
+};
+int restore_batch(const RestoreBatchField* fields, uint32_t count, uint64_t flat, const float* d, const uint8_t* t,
+                  const int32_t* sp, float* y, int max_rounds, tszp_restore_info* info, hipStream_t st);
 
 // Tile counter to pass to a look-back kernel: null (tiles = workgroup ids) unless
 // TSZP_TILE_COUNTER is set, which hands tiles out from the counter instead.
@@ -217,15 +303,22 @@ struct LaunchTimer {
 // Stable LSD radix sort of (key, payload) pairs, 8-bit digits; key_hi (nullable)
 // is a more significant key word.  Results land back in keys / key_hi / pay.
 // scatter_timer (nullable) times every scatter launch.
+// dcount (nullable): the item count on the device, at most `count` (the launches cover
+// `count`); hi_bits: significant low bits of key_hi (passes of 8 bits over them).
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
-                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer = nullptr);
+                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer = nullptr,
+                            const uint32_t* dcount = nullptr, int hi_bits = 32);
 // Dense rank of distinct values inside quantized bins over the sorted keys
 // (SZp/src/szp_float.cc:344-358): rank resets at each new bin and grows at each new
 // value.  Bins come from the float value (by_value: bin = (int)((v + eb) * inv)) or
 // from key_hi (biased bin).  Writes out[pay] = rank for every pay < out_cap
 // (saddles carry pay = 0xffffffff).
+// With field_eb / field_inv (device, per field), key_hi is the field of each key and
+// its bins come from the value with that field's error bound (sorted by (field, value)).
 hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint32_t* pay, uint32_t count,
-                       float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st);
+                       float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st,
+                       const uint32_t* dcount = nullptr, const float* field_eb = nullptr,
+                       const double* field_inv = nullptr);
 
 // ---- sort positions by per-bin bitmaps (tszp_binrank.hip) ----
 // Ranks by value (keys are order-preserving float keys, bins come from the values)
@@ -313,43 +406,46 @@ enum : int {
 // mismatches left after a local round below which the window solve runs at once
 constexpr uint32_t kPpSolveFew = 16384;
 uint32_t pp_init_tiles(uint32_t n);
-hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
-// start values; work: pp_init_work_words(n) u32 (8-byte aligned), no initialisation needed
+// largest sort position: one field into misc[kPpSpMax], a batch into geo.spmax
+hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
+// start values; work: pp_init_work_words(geo.n) u32 (8-byte aligned), no initialisation
+// needed; n_ext: extrema of one field (a batch reads each field's count on the device)
 size_t pp_init_work_words(uint32_t n);
-hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
-                   float* y, uint32_t* work, uint32_t* misc, hipStream_t st);
+hipError_t pp_init(const PpGeo& geo, const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, float* y,
+                   uint32_t* work, uint32_t* misc, hipStream_t st);
 // full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
-// badmap (nullable) receives one byte per point, 1 where mismatched (fastest axis a
-// multiple of 4 only)
-hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
-                   uint32_t cap, uint32_t* part, uint8_t* badmap, uint32_t* misc, hipStream_t st);
+// badmap (nullable) receives one byte per point, 1 where mismatched (where pp_mark_maps);
+// per_field: a batch's per-field counts into geo.fcount
+hipError_t pp_mark(const PpGeo& geo, const float* y, const uint8_t* t, const float* d, uint32_t* bad, uint32_t cap,
+                   uint32_t* part, uint8_t* badmap, bool per_field, uint32_t* misc, hipStream_t st);
 uint32_t pp_mark_parts();
-hipError_t pp_active(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
+bool pp_mark_maps(const PpGeo& geo);
+hipError_t pp_active(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
                      uint32_t* bits, uint32_t* act, uint32_t cap, int anchor, uint32_t* misc, hipStream_t st);
-hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
-                   const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st);
-hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
+hipError_t pp_move(const PpGeo& geo, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
+                   const uint8_t* t, float* y, uint32_t* misc, hipStream_t st);
+hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st);
 // relation masks of the decoded values (bit k: same value at neighbour k, bit 8+k:
 // smaller value at neighbour k; neighbours inside the field only)
-hipError_t pp_rel(const Grid& g, const float* d, uint32_t n, uint16_t* rel, hipStream_t st);
-hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
-                   const uint8_t* badmap, const uint8_t* t, float* y, float eb, int anchor, uint4* plist,
-                   uint32_t* part, uint32_t* misc, hipStream_t st);
+hipError_t pp_rel(const PpGeo& geo, const float* d, uint16_t* rel, hipStream_t st);
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
+                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint4* plist, uint32_t* part,
+                   uint32_t* misc, hipStream_t st);
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
 // mismatch map (one byte per point) from a list; the map must be zeroed first
 hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
-hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t ty, uint32_t tz, uint32_t* pbits,
-                    uint32_t* tbits, uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);
-hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, uint32_t ty, uint32_t tz,
-                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
-                    int anchor, uint32_t* misc, hipStream_t st);
+hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* pbits, uint32_t* tbits,
+                    uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);
+hipError_t pp_solve(const PpGeo& geo, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, int phase,
+                    const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor, uint32_t* misc,
+                    hipStream_t st);
 // every phase in one launch (dataflow over neighbouring tiles): `total` listed tiles,
 // tbits the listed-tile bitmap, done one zeroed word per tile, misc[kPpFlowCounter] zeroed
-hipError_t pp_solve_flow(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
-                         uint32_t tz, const uint32_t* tbits, uint32_t* done, const uint32_t* pbits, const float* d,
-                         const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st);
+hipError_t pp_solve_flow(const PpGeo& geo, const uint32_t* tiles, uint32_t total, uint32_t cap, const uint32_t* tbits,
+                         uint32_t* done, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor,
+                         uint32_t* misc, hipStream_t st);
 
 // ---- synthetic fields (tszp_gen.hip) ----
 hipError_t launch_gen_blobs(float* f, int ndim, const uint32_t* dims, uint64_t seed, int nblobs,

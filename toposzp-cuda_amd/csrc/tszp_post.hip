@@ -185,6 +185,63 @@ __device__ __forceinline__ int classify_at_y(const Grid& g, const float* y, uint
   return classify_vals<ND>(y_ld<COH>(y, e), v);
 }
 
+// ---------------------------------------------------------------------------------
+// geometry source: the field of a flat index (coordinates, interval width, tiles)
+// ---------------------------------------------------------------------------------
+struct FieldView {
+  Grid g;
+  uint32_t base, n, fid;
+  float eb;
+  uint32_t tile0, tx, ty, tz;
+};
+// one field at flat offset 0
+struct Geo1 {
+  Grid g;
+  float eb;
+  uint32_t n, tx, ty, tz;
+  static constexpr bool kBatch = false;
+  __device__ __forceinline__ FieldView at(uint32_t) const {
+    FieldView v;
+    v.g = g;
+    v.base = 0;
+    v.n = n;
+    v.fid = 0;
+    v.eb = eb;
+    v.tile0 = 0;
+    v.tx = tx;
+    v.ty = ty;
+    v.tz = tz;
+    return v;
+  }
+  __device__ __forceinline__ FieldView of_tile(uint32_t) const { return at(0); }
+  __device__ __forceinline__ uint32_t spmax(const FieldView&, const uint32_t* misc) const { return misc[kPpSpMax]; }
+};
+// a batch of fields in flat arrays (PpField)
+struct GeoB {
+  const PpField* f;
+  const uint32_t* fmap;
+  const uint32_t* tmap;
+  const uint32_t* spm;
+  static constexpr bool kBatch = true;
+  __device__ __forceinline__ FieldView view(uint32_t i) const {
+    const PpField& p = f[i];
+    FieldView v;
+    v.g = p.g;
+    v.base = p.base;
+    v.n = p.n;
+    v.fid = i;
+    v.eb = p.eb;
+    v.tile0 = p.tile0;
+    v.tx = p.tx;
+    v.ty = p.ty;
+    v.tz = p.tz;
+    return v;
+  }
+  __device__ __forceinline__ FieldView at(uint32_t e) const { return view(fmap[e >> kFlatShift]); }
+  __device__ __forceinline__ FieldView of_tile(uint32_t tile) const { return view(tmap[tile]); }
+  __device__ __forceinline__ uint32_t spmax(const FieldView& v, const uint32_t*) const { return spm[v.fid]; }
+};
+
 // interval of the restored value of a point with decoded value d.  The zero cell is
 // the one cell whose points may leave the error bound (by at most eb kZeroCell); its
 // half-width only has to leave room for the orders inside a plateau, which float
@@ -285,6 +342,32 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_spmax(const int32_t* __res
   }
 }
 
+// largest sort position of every field of a batch: one workgroup per kFlatAlign chunk of
+// the flat sort-position array (a chunk lies in one field), one atomic per workgroup;
+// spm zeroed
+__global__ __launch_bounds__(kBlockThreads) void k_pp_spmax_b(GeoB geo, const int32_t* __restrict__ sp,
+                                                             uint32_t* __restrict__ spm) {
+  __shared__ uint32_t red[kWavesPerBlock];
+  const uint32_t c0 = blockIdx.x << kFlatShift;
+  const FieldView F = geo.at(c0);
+  const uint32_t ne = (uint32_t)*geo.f[F.fid].n_ext;
+  const uint32_t k0 = c0 - F.base, k1 = min(ne, k0 + kFlatAlign);
+  uint32_t v = 0;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlockThreads) {
+    const int32_t x = sp[F.base + k];
+    v = x > (int32_t)v ? (uint32_t)x : v;
+  }
+  v = seg_max<kWave>(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = red[0];
+#pragma unroll
+    for (int k = 1; k < kWavesPerBlock; k++) r = red[k] > r ? red[k] : r;
+    if (r) atomicMax(&spm[F.fid], r);
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // initial values.  A tile of kInitTile points ranks its extrema in row-major order
 // (look-back over tiles) so that the k-th extremum takes sort position sp[k].
@@ -294,10 +377,15 @@ constexpr uint32_t kInitTile = (uint32_t)kBlockThreads * kInitPer;
 
 // extrema per tile of kInitTile points (pass 1; an exclusive scan of these counts gives
 // every tile the rank of its first extremum, so the start pass needs no look-back)
-__global__ __launch_bounds__(kBlockThreads) void k_pp_ext_count(const uint8_t* __restrict__ t, uint32_t n,
+// (batch: kInitTile == kFlatAlign, so a tile lies in one field)
+static_assert(kInitTile == kFlatAlign, "an initial-value tile lies in one field");
+template <class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_ext_count(GEO geo, const uint8_t* __restrict__ t,
                                                                uint32_t* __restrict__ cnt) {
   __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t t0 = blockIdx.x * kInitTile;
+  const FieldView F = geo.at(t0);
+  const uint32_t n = F.base + F.n;  // end of the tile's field
   const uint32_t* tg = reinterpret_cast<const uint32_t*>(t);
   uint32_t c = 0;
 #pragma unroll 4
@@ -323,10 +411,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_ext_count(const uint8_t* _
   if (threadIdx.x == 0) cnt[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// tile_base[k]: extrema in the tiles before tile k (the scanned counts)
-__global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restrict__ d, const uint8_t* __restrict__ t,
-                                                          const int32_t* __restrict__ sp, uint32_t n_ext, uint32_t n,
-                                                          float eb, float* __restrict__ y,
+// tile_base[k]: extrema in the tiles before tile k (the scanned counts); the k-th
+// extremum of a field takes sort position sp[base + k]
+template <class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_init(GEO geo, const float* __restrict__ d,
+                                                          const uint8_t* __restrict__ t,
+                                                          const int32_t* __restrict__ sp, uint32_t n_ext_one,
+                                                          float* __restrict__ y,
                                                           const uint32_t* __restrict__ tile_base,
                                                           uint32_t* misc) {
   __shared__ uint32_t tw[kInitTile / 4];
@@ -336,6 +427,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restri
   const int tid = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const uint32_t t0 = tile * kInitTile;
+  const FieldView F = geo.at(t0);
+  const uint32_t n = F.base + F.n;  // end of the tile's field
+  const float eb = F.eb;
+  uint32_t n_ext = n_ext_one;
+  if constexpr (GEO::kBatch) n_ext = (uint32_t)*geo.f[F.fid].n_ext;
   // stage the tile's types (4-byte aligned words, coalesced)
   const uint32_t* tg = reinterpret_cast<const uint32_t*>(t);
 #pragma unroll 4
@@ -367,10 +463,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restri
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), red, &tot);
   exl[tid] = ex;
-  const uint32_t base = tile_base[tile];
-  if (tid == 0 && tile == gridDim.x - 1) misc[kPpExt] = base + tot;
+  uint32_t base = tile_base[tile];
+  if constexpr (GEO::kBatch) {
+    base -= tile_base[F.base / kInitTile];  // extrema of the field before this tile
+  } else {
+    if (tid == 0 && tile == gridDim.x - 1) misc[kPpExt] = base + tot;
+  }
   __syncthreads();
-  const float sden = (float)misc[kPpSpMax] + 2.f;
+  const float sden = (float)geo.spmax(F, misc) + 2.f;
   // float4 groups of 4 points (the tile start is a multiple of 4); all loads of a batch
   // are issued before the values are computed
   constexpr int kGroups = kInitTile / 4 / kBlockThreads;  // 16
@@ -405,7 +505,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(const float* __restri
         if ((tb == 1u || tb == 2u) && sp != nullptr) {
           const uint32_t o = i >> 6, b = i & 63u;
           const uint32_t k = base + exl[o] + (uint32_t)__popcll(msk[o] & ((1ull << b) - 1ull));
-          sv[u][c] = k < n_ext ? sp[k] : 0;
+          sv[u][c] = k < n_ext ? sp[F.base + k] : 0;
         }
       }
     }
@@ -747,11 +847,118 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_sum(const uint32_t*
   }
 }
 
+// Full marking pass over a batch of fields (any fastest-axis length): workgroups take
+// chunks of kMarkChunk points (a chunk lies in one field), list the mismatches through a
+// workgroup-staged LDS list (one global atomic per flush), write the mismatch map when
+// badmap is set, and leave per-workgroup verification partials (k_pp_reduce_verify).
+// With fcount, every field's own counts (missing, spurious, changed, max |y - d| bits)
+// are added up too, one atomic per chunk and count.
+constexpr uint32_t kMarkChunk = 4 * kBlockThreads;
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_mark_b(GeoB geo, const float* __restrict__ y,
+                                                            const uint8_t* __restrict__ t,
+                                                            const float* __restrict__ d, uint32_t nchunks,
+                                                            uint32_t* __restrict__ bad, uint32_t cap,
+                                                            uint32_t* __restrict__ part, uint8_t* __restrict__ badmap,
+                                                            unsigned long long* __restrict__ fcount,
+                                                            uint32_t* __restrict__ misc) {
+  constexpr uint32_t kStage = 4096;
+  __shared__ uint32_t red[4][kWavesPerBlock];
+  __shared__ uint32_t lbuf[kStage];
+  __shared__ uint32_t lcnt, lbase;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x == 0) lcnt = 0;
+  __syncthreads();
+  uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
+  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint32_t c0 = ch * kMarkChunk;
+    const FieldView F = geo.at(c0);
+    const uint32_t end = F.base + F.n;
+    if (c0 >= end) continue;  // padding after the field (uniform over the workgroup)
+    uint32_t fm = 0, fs = 0, fc = 0, fd = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t e = c0 + j * kBlockThreads + threadIdx.x;
+      bool isbad = false;
+      if (e < end) {
+        int c[ND];
+        coords_of<ND>(F.g, e - F.base, c);
+        const int k = classify_at<ND>(F.g, y, e, c);
+        const int tt = t[e];
+        fm += (tt != 0 && k == 0);
+        fs += (tt == 0 && k != 0);
+        fc += (tt != 0 && k != 0 && k != tt);
+        isbad = k != tt;
+        if (d != nullptr) {
+          const uint32_t dv = __float_as_uint(fabsf(y[e] - d[e]));
+          fd = dv > fd ? dv : fd;
+        }
+        if (badmap != nullptr) badmap[e] = isbad ? 1 : 0;
+      }
+      if (bad != nullptr) lds_append(isbad, e, lbuf, &lcnt);
+    }
+    if (bad != nullptr) {
+      __syncthreads();
+      if (lcnt > kStage - kMarkChunk) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+    }
+    if (fcount != nullptr) {  // the field's counts (whole workgroup)
+      const uint32_t v[4] = {seg_sum<kWave>(fm), seg_sum<kWave>(fs), seg_sum<kWave>(fc), seg_max<kWave>(fd)};
+      __syncthreads();
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) red[k][w] = v[k];
+      }
+      __syncthreads();
+      if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        uint32_t a = 0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock; q++) a = k == 3 ? (red[k][q] > a ? red[k][q] : a) : a + red[k][q];
+        if (a) {
+          if (k == 3) atomicMax(&fcount[4 * F.fid + 3], (unsigned long long)a);
+          else atomicAdd(&fcount[4 * F.fid + k], (unsigned long long)a);
+        }
+      }
+    }
+    miss += fm;
+    spur += fs;
+    chg += fc;
+    dev = fd > dev ? fd : dev;
+  }
+  if (bad != nullptr) lds_flush(lbuf, &lcnt, &lbase, bad, cap, &misc[kPpBad]);
+  miss = seg_sum<kWave>(miss);
+  spur = seg_sum<kWave>(spur);
+  chg = seg_sum<kWave>(chg);
+  dev = seg_max<kWave>(dev);
+  __syncthreads();
+  if (lane == 0) {
+    red[0][w] = miss;
+    red[1][w] = spur;
+    red[2][w] = chg;
+    red[3][w] = dev;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+      s2 += red[2][k];
+      s3 = red[3][k] > s3 ? red[3][k] : s3;
+    }
+    part[4 * blockIdx.x] = s0;
+    part[4 * blockIdx.x + 1] = s1;
+    part[4 * blockIdx.x + 2] = s2;
+    part[4 * blockIdx.x + 3] = s3;
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // active set: the non-anchored points of the closed neighbourhoods of the mismatches
 // ---------------------------------------------------------------------------------
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_active(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_active(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
                                                             const float* __restrict__ d,
                                                             const uint8_t* __restrict__ t, uint32_t* bits,
                                                             uint32_t* __restrict__ act, uint32_t cap, int anchor,
@@ -765,8 +972,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_active(Grid g, const uint3
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   const bool valid = i < nbad;
   const uint32_t e = valid ? bad[i] : 0u;
+  const FieldView F = geo.at(e);
+  const Grid& g = F.g;
   int c[ND];
-  coords_of<ND>(g, e, c);
+  coords_of<ND>(g, e - F.base, c);
   uint32_t pk[2 * ND + 1], off[2 * ND + 1];
   int col[2 * ND + 1];
 #pragma unroll
@@ -910,11 +1119,11 @@ __device__ bool move_point(const Grid& g, uint32_t p, const int (&c)[ND], const 
   return false;
 }
 
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_t* __restrict__ act, uint32_t cap,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_move(GEO geo, const uint32_t* __restrict__ act, uint32_t cap,
                                                           int colour, const float* __restrict__ d,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
-                                                          float eb, uint32_t* misc) {
+                                                          uint32_t* misc) {
   __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   uint32_t cnt = misc[kPpActC + colour];
@@ -922,9 +1131,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_move(Grid g, const uint32_
   bool moved = false;
   if (i < cnt) {
     const uint32_t p = act[(size_t)colour * cap + i];
+    const FieldView F = geo.at(p);
     int c[ND];
-    coords_of<ND>(g, p, c);
-    moved = move_point<ND>(g, p, c, d, t, y, eb);
+    coords_of<ND>(F.g, p - F.base, c);
+    moved = move_point<ND>(F.g, p, c, d, t, y, F.eb);
   }
   // one atomic per workgroup (same-address atomics serialise)
   const uint32_t nm = (uint32_t)__popcll(__ballot(moved));
@@ -1040,11 +1250,11 @@ __device__ __forceinline__ bool move_point_seg(const Grid& g, uint32_t p, const 
   return false;
 }
 
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_move_seg(Grid g, const uint32_t* __restrict__ act,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_move_seg(GEO geo, const uint32_t* __restrict__ act,
                                                               uint32_t cap, int colour, const float* __restrict__ d,
                                                               const uint8_t* __restrict__ t, float* __restrict__ y,
-                                                              float eb, uint32_t* misc) {
+                                                              uint32_t* misc) {
   __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t gi = blockIdx.x * kBlockThreads + threadIdx.x;
   const uint32_t i = gi >> 4;
@@ -1056,11 +1266,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_move_seg(Grid g, const uin
   int c[ND];
 #pragma unroll
   for (int b = 0; b < ND; b++) c[b] = 0;
-  if (valid) {
-    p = act[(size_t)colour * cap + i];
-    coords_of<ND>(g, p, c);
-  }
-  const bool moved = move_point_seg<ND>(g, p, c, d, t, y, eb, j, valid);
+  if (valid) p = act[(size_t)colour * cap + i];
+  const FieldView F = geo.at(p);
+  if (valid) coords_of<ND>(F.g, p - F.base, c);
+  const bool moved = move_point_seg<ND>(F.g, p, c, d, t, y, F.eb, j, valid);
   const uint32_t nm = (uint32_t)__popcll(__ballot(moved));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nm;
   __syncthreads();
@@ -1083,8 +1292,8 @@ __constant__ const int8_t kRing3D[25][3] = {
 
 constexpr int kRemarkChunks = 8;
 
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint32_t* __restrict__ old,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_remark(GEO geo, const uint32_t* __restrict__ old,
                                                             uint32_t nold, const float* __restrict__ y,
                                                             const uint8_t* __restrict__ t, uint32_t* bits,
                                                             uint32_t* __restrict__ bad, uint32_t cap, int rings,
@@ -1107,8 +1316,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(Grid g, const uint3
     uint32_t p = 0;
     if (i < nold) {
       const uint32_t e = old[i];
+      const FieldView F = geo.at(e);
+      const Grid& g = F.g;
       int c[ND], q[ND];
-      coords_of<ND>(g, e, c);
+      coords_of<ND>(g, e - F.base, c);
       int64_t off = 0;
 #pragma unroll
       for (int a = 0; a < ND; a++) {
@@ -1163,6 +1374,39 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_rel(Grid g, const float* _
   }
 }
 
+// the same over a batch of fields, in chunks of kMarkChunk points (a chunk lies in one
+// field)
+template <int ND>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_rel_b(GeoB geo, const float* __restrict__ d, uint32_t nchunks,
+                                                           uint16_t* __restrict__ rel) {
+  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint32_t c0 = ch * kMarkChunk;
+    const FieldView F = geo.at(c0);
+    const uint32_t end = F.base + F.n;
+    const Grid& g = F.g;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t e = c0 + j * kBlockThreads + threadIdx.x;
+      if (e >= end) continue;
+      int c[ND];
+      coords_of<ND>(g, e - F.base, c);
+      const float v = d[e];
+      uint32_t r = 0;
+#pragma unroll
+      for (int k = 0; k < 2 * ND; k++) {
+        const int a = k >> 1;
+        const bool ok = (k & 1) ? c[a] + 1 < dim_of<ND>(g, a) : c[a] >= 1;
+        if (!ok) continue;
+        const uint32_t s = stride_of<ND>(g, a);
+        const float q = d[(k & 1) ? e + s : e - s];
+        r |= (q == v ? 1u : 0u) << k;
+        r |= (v > q ? 1u : 0u) << (8 + k);
+      }
+      rel[e] = (uint16_t)r;
+    }
+  }
+}
+
 // the same with 4 consecutive points per thread along the fastest axis (its length a
 // multiple of 4, buffers aligned): one float4 per stencil row
 template <int ND>
@@ -1213,8 +1457,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_rel4(Grid g, const float* 
 
 // Pass 1: the plateau of every listed point, given up beyond three points; one thread
 // per plateau (the first to claim its smallest point) lists it as {P0, P1, P2, np}.
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(Grid g, const uint32_t* __restrict__ bad,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(GEO geo, const uint32_t* __restrict__ bad,
                                                                uint32_t nbad, const uint16_t* __restrict__ rel,
                                                                const uint8_t* __restrict__ badmap,
                                                                uint4* __restrict__ plist) {
@@ -1222,6 +1466,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(Grid g, const ui
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   if (i >= nbad) return;
   const uint32_t b = bad[i];
+  const Grid g = geo.at(b).g;  // a plateau lies in one field
   uint32_t P[3] = {b, b, b};
   int np = 1;
   bool big = false;
@@ -1264,12 +1509,12 @@ __constant__ const int8_t kWeak3[13][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 
                                            {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 __constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
 
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(Grid g, const uint4* __restrict__ plist,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(GEO geo, const uint4* __restrict__ plist,
                                                                 const float* __restrict__ d,
                                                                 const uint16_t* __restrict__ rel,
                                                                 const uint8_t* __restrict__ t, float* __restrict__ y,
-                                                                float eb, int anchor, uint32_t count,
+                                                                int anchor, uint32_t count,
                                                                 uint32_t* __restrict__ part) {
   constexpr int K = 2 * ND;
   __shared__ uint32_t red[kWavesPerBlock];
@@ -1280,6 +1525,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(Grid g, const u
     const uint32_t P[3] = {ent.x, ent.y, ent.z};
     const int np = (int)ent.w;
     const float db = d[P[0]];
+    const FieldView F = geo.at(P[0]);
+    const Grid& g = F.g;
+    const float eb = F.eb;
     int tp[3];
     bool intr[3], anc[3];
     float yv[3];
@@ -1294,7 +1542,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(Grid g, const u
       if (j >= np) continue;
       const uint32_t e = P[j];
       int c[ND];
-      coords_of<ND>(g, e, c);
+      coords_of<ND>(g, e - F.base, c);
       tp[j] = t[e];
       intr[j] = interior<ND>(g, c);
       anc[j] = anchor && anchored(tp[j], db);
@@ -1430,9 +1678,9 @@ __device__ __forceinline__ uint32_t tile_of(const int (&c)[ND], uint32_t ty, uin
   return id;
 }
 
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(Grid g, const uint32_t* __restrict__ bad, uint32_t nbad,
-                                                           uint32_t ty, uint32_t tz, uint32_t* pbits,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                           uint32_t* pbits,
                                                            uint32_t* tbits, uint32_t* __restrict__ tiles,
                                                            uint32_t cap, uint32_t* misc) {
   // one tile list per parity phase: tiles + phase * cap
@@ -1442,10 +1690,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiles(Grid g, const uint32
   int ph = 0;
   if (i < nbad) {
     const uint32_t e = bad[i];
+    const FieldView F = geo.at(e);
     int c[ND];
-    coords_of<ND>(g, e, c);
+    coords_of<ND>(F.g, e - F.base, c);
     atomicOr(&pbits[e >> 5], 1u << (e & 31u));
-    id = tile_of<ND>(c, ty, tz);
+    id = F.tile0 + tile_of<ND>(c, F.ty, F.tz);
 #pragma unroll
     for (int a = 0; a < ND; a++) ph |= (int)(((uint32_t)c[a] >> kTileShift) & 1u) << a;
     take = claim_bit(tbits, id);
@@ -1474,6 +1723,7 @@ struct Window {
   int8_t canchor[EM];
   uint32_t cand[kWave];
   int ne, nw, npick;
+  uint32_t base;  // flat offset of the window's field
   float lo, hi, db;
 };
 
@@ -1486,10 +1736,10 @@ __device__ __forceinline__ int find_el(const Window<ND>& w, uint32_t e) {
 
 // neighbour k of element point e, if it lies in the plateau of value db
 template <int ND>
-__device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, const float* __restrict__ d, uint32_t e, int k,
-                                                float db) {
+__device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, uint32_t base, const float* __restrict__ d, uint32_t e,
+                                                int k, float db) {
   int c[ND];
-  coords_of<ND>(g, e, c);
+  coords_of<ND>(g, e - base, c);
   const int a = k >> 1;
   c[a] += (k & 1) ? 1 : -1;
   if (!inside<ND>(g, c)) return 0xffffffffu;
@@ -1501,13 +1751,14 @@ __device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, const float* __re
 // Builds the window of mismatch b (whole wavefront: the lanes load neighbours in
 // parallel, lane 0 keeps the lists in order).
 template <int ND, bool COH>
-__device__ void build_window(Window<ND>& w, const Grid& g, const float* __restrict__ d,
+__device__ void build_window(Window<ND>& w, const Grid& g, uint32_t base, const float* __restrict__ d,
                              const uint8_t* __restrict__ t, const float* y, float eb, uint32_t b,
                              bool anchor, int lane) {
   constexpr int K = 2 * ND;
   if (lane == 0) {
     w.ne = 1;
     w.el[0] = b;
+    w.base = base;
     w.db = d[b];
     cell_of(w.db, eb, &w.lo, &w.hi);
   }
@@ -1519,7 +1770,7 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
     const int ne0 = w.ne;
     if (head >= ne0 || ne0 >= kWinMax) break;
     const int ncand = (ne0 - head) * K;
-    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, d, w.el[head + lane / K], lane % K, db) : 0xffffffffu;
+    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[head + lane / K], lane % K, db) : 0xffffffffu;
     __syncthreads();
     if (lane == 0)
       for (int j = 0; j < ncand && w.ne < kWinMax; j++) {
@@ -1534,7 +1785,7 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
   // the plateau points next to the window keep their values
   {
     const int ncand = w.nw * K;
-    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, d, w.el[lane / K], lane % K, db) : 0xffffffffu;
+    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[lane / K], lane % K, db) : 0xffffffffu;
     __syncthreads();
     if (lane == 0)
       for (int j = 0; j < ncand; j++) {
@@ -1550,7 +1801,7 @@ __device__ void build_window(Window<ND>& w, const Grid& g, const float* __restri
     const int tt = t[e];
     w.anch[lane] = anchor && anchored(tt, db);
     int c[ND];
-    coords_of<ND>(g, e, c);
+    coords_of<ND>(g, e - base, c);
     w.typ[lane] = interior<ND>(g, c) ? (int8_t)tt : (int8_t)-1;
   }
   __syncthreads();
@@ -1806,7 +2057,7 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
     if (my_dep >= 0) {
       const uint32_t e = w.el[lane];
       int c[ND];
-      coords_of<ND>(g, e, c);
+      coords_of<ND>(g, e - w.base, c);
       good = classify_at_y<ND, COH>(g, y, e, c) == (int)t[e];
     }
     if (!__any(!good)) {
@@ -1833,11 +2084,12 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
 // double), which makes the anchored values of its window contradictory; modes 1-3
 // free extrema, the most extreme anchored value first.
 template <int ND, bool COH>
-__device__ int solve_window(Window<ND>& w, const Grid& g, const float* __restrict__ d, const uint8_t* __restrict__ t,
+__device__ int solve_window(Window<ND>& w, const Grid& g, uint32_t base, const float* __restrict__ d,
+                            const uint8_t* __restrict__ t,
                             float* y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
                             uint32_t* nodes, uint32_t* stats) {
   const uint64_t t0 = wall_clock64();
-  build_window<ND, COH>(w, g, d, t, y, eb, b, anchor, lane);
+  build_window<ND, COH>(w, g, base, d, t, y, eb, b, anchor, lane);
   const uint64_t t1 = wall_clock64();
   stats[4] += (uint32_t)(t1 - t0);
   const bool ok0 = run_attempt<ND, COH>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats);
@@ -1902,19 +2154,21 @@ struct SolveShared {
   uint32_t tile;
 };
 
-// One wavefront solves the listed mismatches of tile `id` in flat order.
+// One wavefront solves the listed mismatches of tile `id` (of field F, numbered within
+// the field) in flat order; spmax is the field's largest sort position.
 template <int ND, bool COH>
-__device__ void solve_tile(SolveShared<ND>& sh, const Grid& g, uint32_t id, uint32_t ty, uint32_t tz,
+__device__ void solve_tile(SolveShared<ND>& sh, const FieldView& F, uint32_t id, uint32_t spmax,
                            const uint32_t* __restrict__ pbits, const float* __restrict__ d,
-                           const uint8_t* __restrict__ t, float* y, float eb, int anchor,
-                           uint32_t* misc) {
+                           const uint8_t* __restrict__ t, float* y, int anchor, uint32_t* misc) {
   constexpr int kT = kTileSide;
+  const Grid& g = F.g;
+  const float eb = F.eb;
   Window<ND>& w = sh.w;
   uint32_t* blist = sh.blist;
   const int lane = threadIdx.x;
   const uint64_t t_work = wall_clock64();
   int tc[ND];
-  tile_coords<ND>(id, ty, tz, tc);
+  tile_coords<ND>(id, F.ty, F.tz, tc);
   const int x0 = tc[0] * kT, x1 = min(x0 + kT, (int)g.d0);
   const int y0 = tc[1] * kT, y1 = min(y0 + kT, (int)g.d1);
   int z0 = 0, z1 = 1;
@@ -1932,12 +2186,12 @@ __device__ void solve_tile(SolveShared<ND>& sh, const Grid& g, uint32_t id, uint
       int x, len;
       if constexpr (ND == 2) {
         x = x0 + sgi;
-        e0 = (uint32_t)x * g.d1 + (uint32_t)y0;
+        e0 = F.base + (uint32_t)x * g.d1 + (uint32_t)y0;
         len = y1 - y0;
       } else {
         x = x0 + sgi / kT;
         const int yy = y0 + sgi % kT;
-        e0 = (uint32_t)x * g.plane + (uint32_t)yy * g.d2 + (uint32_t)z0;
+        e0 = F.base + (uint32_t)x * g.plane + (uint32_t)yy * g.d2 + (uint32_t)z0;
         len = z1 - z0;
         if (yy >= y1) x = x1;
       }
@@ -1962,14 +2216,14 @@ __device__ void solve_tile(SolveShared<ND>& sh, const Grid& g, uint32_t id, uint
   uint32_t solved = 0, failed = 0, freed = 0, nodes = 0;
   uint32_t tst[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // [0] leaves [1] verify fails [4] build ticks [5] attempt ticks
   // extrema of the sort positions s - 1 and s + 1 sit 0.9 (hi - lo) / (S + 2) away
-  const float delta_frac = 0.99f * 0.9f / ((float)misc[kPpSpMax] + 2.f);
+  const float delta_frac = 0.99f * 0.9f / ((float)spmax + 2.f);
   for (uint32_t j = 0; j < nb; j++) {
     const uint32_t e = blist[j];
     int c[ND];
-    coords_of<ND>(g, e, c);
+    coords_of<ND>(g, e - F.base, c);
     if (classify_at_y<ND, COH>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
     const int mode =
-        solve_window<ND, COH>(w, g, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
+        solve_window<ND, COH>(w, g, F.base, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
     solved += mode >= 0;
     failed += mode < 0;
     freed += mode >= 2;
@@ -1992,14 +2246,15 @@ __device__ void solve_tile(SolveShared<ND>& sh, const Grid& g, uint32_t id, uint
 
 // One wavefront per listed tile of one parity phase (a launch per phase: the launch
 // boundary makes one phase's values visible to the next).
-template <int ND>
-__global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                    uint32_t ty, uint32_t tz, int phase,
-                                                    const uint32_t* __restrict__ pbits, const float* __restrict__ d,
-                                                    const uint8_t* __restrict__ t, float* __restrict__ y, float eb,
-                                                    int anchor, uint32_t* misc) {
+template <int ND, class GEO>
+__global__ __launch_bounds__(kWave) void k_pp_solve(GEO geo, const uint32_t* __restrict__ tiles, uint32_t cap,
+                                                    int phase, const uint32_t* __restrict__ pbits,
+                                                    const float* __restrict__ d, const uint8_t* __restrict__ t,
+                                                    float* __restrict__ y, int anchor, uint32_t* misc) {
   __shared__ SolveShared<ND> sh;
-  solve_tile<ND, false>(sh, g, tiles[(size_t)phase * cap + blockIdx.x], ty, tz, pbits, d, t, y, eb, anchor, misc);
+  const uint32_t id = tiles[(size_t)phase * cap + blockIdx.x];
+  const FieldView F = geo.of_tile(id);
+  solve_tile<ND, false>(sh, F, id - F.tile0, geo.spmax(F, misc), pbits, d, t, y, anchor, misc);
 }
 
 // Every phase in one launch, as a dataflow: workgroup i takes the i-th listed tile in
@@ -2013,13 +2268,12 @@ __global__ __launch_bounds__(kWave) void k_pp_solve(Grid g, const uint32_t* __re
 // instead of the sum of the phases' slowest tiles.  A tile only waits for tiles of lower
 // ids, which have started before it, so the waits end; a bounded spin reports an error
 // instead of hanging.
-template <int ND>
-__global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t* __restrict__ tiles, uint32_t cap,
-                                                         uint32_t tx, uint32_t ty, uint32_t tz,
+template <int ND, class GEO>
+__global__ __launch_bounds__(kWave) void k_pp_solve_flow(GEO geo, const uint32_t* __restrict__ tiles, uint32_t cap,
                                                          const uint32_t* __restrict__ tbits, uint32_t* done,
                                                          const uint32_t* __restrict__ pbits,
                                                          const float* __restrict__ d, const uint8_t* __restrict__ t,
-                                                         float* __restrict__ y, float eb, int anchor, uint32_t* misc,
+                                                         float* __restrict__ y, int anchor, uint32_t* misc,
                                                          uint32_t* counter) {
   __shared__ SolveShared<ND> sh;
   const int lane = threadIdx.x;
@@ -2033,9 +2287,11 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t*
     i -= c;
   }
   if (p == (1u << ND)) return;  // more workgroups than listed tiles
-  const uint32_t id = tiles[(size_t)p * cap + i];
+  const uint32_t id = tiles[(size_t)p * cap + i];  // over all fields
+  const FieldView F = geo.of_tile(id);
+  const uint32_t tx = F.tx, ty = F.ty, tz = F.tz, lid = id - F.tile0;
   int tc[ND];
-  tile_coords<ND>(id, ty, tz, tc);
+  tile_coords<ND>(lid, ty, tz, tc);
   // lane k waits for neighbour k of the 3^ND block (k != centre), if listed and of a
   // lower phase
   constexpr int kNb = ND == 2 ? 9 : 27;
@@ -2054,6 +2310,7 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t*
       uint32_t nid = (uint32_t)nc[0];
 #pragma unroll
       for (int a = 1; a < ND; a++) nid = nid * (a == 1 ? ty : tz) + (uint32_t)nc[a];
+      nid += F.tile0;
       if ((tbits[nid >> 5] >> (nid & 31u)) & 1u) {
         uint32_t spins = 0;
         while (__hip_atomic_load(&done[nid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
@@ -2067,7 +2324,7 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(Grid g, const uint32_t*
     }
   }
   __syncthreads();
-  solve_tile<ND, true>(sh, g, id, ty, tz, pbits, d, t, y, eb, anchor, misc);
+  solve_tile<ND, true>(sh, F, lid, geo.spmax(F, misc), pbits, d, t, y, anchor, misc);
   // the flag is a release store at agent scope: every field store of the tile is
   // performed before a neighbouring tile can see it set.  The waiting side reads the
   // field only with agent-coherent loads issued after its spin has seen the flag (the
@@ -2088,7 +2345,47 @@ static inline unsigned grid_stride_blocks(uint64_t n) {
 
 uint32_t pp_init_tiles(uint32_t n) { return (n + kInitTile - 1) / kInitTile; }
 
-hipError_t pp_spmax(const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st) {
+static Geo1 geo1(const PpGeo& G) {
+  Geo1 g;
+  g.g = G.g;
+  g.eb = G.eb;
+  g.n = G.n;
+  g.tx = G.tx;
+  g.ty = G.ty;
+  g.tz = G.tz;
+  return g;
+}
+static GeoB geob(const PpGeo& G) {
+  GeoB b;
+  b.f = G.fields;
+  b.fmap = G.fmap;
+  b.tmap = G.tmap;
+  b.spm = G.spmax;
+  return b;
+}
+
+// launches kernel template K<ND, GEO> with the geometry of `geo` as first argument
+#define PP_GEO_LAUNCH(K, grid, blk, st, ...)                                                    \
+  do {                                                                                          \
+    if (geo.fields) {                                                                           \
+      const GeoB gb_ = geob(geo);                                                               \
+      if (geo.ndim == 2) hipLaunchKernelGGL((K<2, GeoB>), grid, blk, 0, st, gb_, __VA_ARGS__);  \
+      else hipLaunchKernelGGL((K<3, GeoB>), grid, blk, 0, st, gb_, __VA_ARGS__);                \
+    } else {                                                                                    \
+      const Geo1 g1_ = geo1(geo);                                                               \
+      if (geo.ndim == 2) hipLaunchKernelGGL((K<2, Geo1>), grid, blk, 0, st, g1_, __VA_ARGS__);  \
+      else hipLaunchKernelGGL((K<3, Geo1>), grid, blk, 0, st, g1_, __VA_ARGS__);                \
+    }                                                                                           \
+  } while (0)
+
+hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st) {
+  if (geo.fields) {  // every field's largest sort position
+    hipError_t e = launch_fill_u32(geo.spmax, 0u, geo.nfields, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pp_spmax_b, dim3(geo.n >> kFlatShift), dim3(kBlockThreads), 0, st, geob(geo), sp,
+                       geo.spmax);
+    return hipGetLastError();
+  }
   if (!m) return hipSuccess;
   hipLaunchKernelGGL(k_pp_spmax, dim3(grid_stride_blocks(m)), dim3(kBlockThreads), 0, st, sp, m, misc);
   return hipGetLastError();
@@ -2101,8 +2398,9 @@ size_t pp_init_work_words(uint32_t n) {
   return nt + 1 + 2 * scan_status_words(nt) + 2;
 }
 
-hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, uint32_t n, float eb,
-                   float* y, uint32_t* work, uint32_t* misc, hipStream_t st) {
+hipError_t pp_init(const PpGeo& geo, const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, float* y,
+                   uint32_t* work, uint32_t* misc, hipStream_t st) {
+  const uint32_t n = geo.n;
   const uint32_t tiles = pp_init_tiles(n);
   uint32_t* cnt = work;
   uint64_t* status = reinterpret_cast<uint64_t*>(work + ((tiles + 2) & ~1u));  // 8-byte aligned
@@ -2112,25 +2410,52 @@ hipError_t pp_init(const float* d, const uint8_t* t, const int32_t* sp, uint32_t
   if (e != hipSuccess) return e;
   e = launch_fill_u32(cnt + tiles, 0u, 1, st);  // the scan turns it into the total
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pp_ext_count, dim3(tiles), dim3(kBlockThreads), 0, st, t, n, cnt);
+  if (geo.fields) hipLaunchKernelGGL(k_pp_ext_count<GeoB>, dim3(tiles), dim3(kBlockThreads), 0, st, geob(geo), t, cnt);
+  else hipLaunchKernelGGL(k_pp_ext_count<Geo1>, dim3(tiles), dim3(kBlockThreads), 0, st, geo1(geo), t, cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan_excl(cnt, tiles + 1, status, ctr, ctr + 1, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pp_init, dim3(tiles), dim3(kBlockThreads), 0, st, d, t, sp, n_ext, n, eb, y, cnt, misc);
+  if (geo.fields)
+    hipLaunchKernelGGL(k_pp_init<GeoB>, dim3(tiles), dim3(kBlockThreads), 0, st, geob(geo), d, t, sp, n_ext, y, cnt,
+                       misc);
+  else
+    hipLaunchKernelGGL(k_pp_init<Geo1>, dim3(tiles), dim3(kBlockThreads), 0, st, geo1(geo), d, t, sp, n_ext, y, cnt,
+                       misc);
   return hipGetLastError();
 }
 
 constexpr unsigned kMarkBlocks = 2048;
 uint32_t pp_mark_parts() { return kMarkBlocks; }
 
-hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float* d, uint32_t n, uint32_t* bad,
-                   uint32_t cap, uint32_t* part, uint8_t* badmap, uint32_t* misc, hipStream_t st) {
+bool pp_mark_maps(const PpGeo& geo) { return geo.fields || (geo.ndim == 2 ? geo.g.d1 : geo.g.d2) % 4 == 0; }
+
+hipError_t pp_mark(const PpGeo& geo, const float* y, const uint8_t* t, const float* d, uint32_t* bad, uint32_t cap,
+                   uint32_t* part, uint8_t* badmap, bool per_field, uint32_t* misc, hipStream_t st) {
+  const uint32_t n = geo.n;
+  const dim3 blk(kBlockThreads);
+  if (geo.fields) {
+    const uint32_t nchunks = (n + kMarkChunk - 1) / kMarkChunk;
+    unsigned long long* fc = per_field ? geo.fcount : nullptr;
+    if (fc) {
+      hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(fc), 0u, 8 * (size_t)geo.nfields, st);
+      if (e != hipSuccess) return e;
+    }
+    if (geo.ndim == 2)
+      hipLaunchKernelGGL(k_pp_mark_b<2>, dim3(kMarkBlocks), blk, 0, st, geob(geo), y, t, d, nchunks, bad, cap, part,
+                         badmap, fc, misc);
+    else
+      hipLaunchKernelGGL(k_pp_mark_b<3>, dim3(kMarkBlocks), blk, 0, st, geob(geo), y, t, d, nchunks, bad, cap, part,
+                         badmap, fc, misc);
+    hipLaunchKernelGGL(k_pp_reduce_verify, dim3(1), blk, 0, st, part, kMarkBlocks, misc);
+    return hipGetLastError();
+  }
+  const Grid& g = geo.g;
   const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
   const bool vec = fast % 4 == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
                    (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0 &&
                    (reinterpret_cast<uintptr_t>(badmap) & 3) == 0;
   if (vec) {
     unsigned nb = kMarkBlocks;  // a multiple of the XCD count
-    const dim3 grid(nb), blk(kBlockThreads);
+    const dim3 grid(nb);
     uint32_t* bm4 = reinterpret_cast<uint32_t*>(badmap);
     if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
     else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
@@ -2138,52 +2463,53 @@ hipError_t pp_mark(const Grid& g, const float* y, const uint8_t* t, const float*
     return hipGetLastError();
   }
   if (badmap) return hipErrorInvalidValue;  // the mismatch map needs the vector layout
-  const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
+  const dim3 grid(grid_stride_blocks(n));
   if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark<2>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   else hipLaunchKernelGGL(k_pp_mark<3>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_active(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
+hipError_t pp_active(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
                      uint32_t* bits, uint32_t* act, uint32_t cap, int anchor, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_active<2>, grid, blk, 0, st, g, bad, nbad, d, t, bits, act, cap, anchor, misc);
-  else hipLaunchKernelGGL(k_pp_active<3>, grid, blk, 0, st, g, bad, nbad, d, t, bits, act, cap, anchor, misc);
+  PP_GEO_LAUNCH(k_pp_active, dim3(blocks_for(nbad)), dim3(kBlockThreads), st, bad, nbad, d, t, bits, act, cap, anchor,
+                misc);
   return hipGetLastError();
 }
 
-hipError_t pp_move(const Grid& g, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
-                   const uint8_t* t, float* y, float eb, uint32_t* misc, hipStream_t st) {
+hipError_t pp_move(const PpGeo& geo, const uint32_t* act, uint32_t max_act, uint32_t cap, int colour, const float* d,
+                   const uint8_t* t, float* y, uint32_t* misc, hipStream_t st) {
   const uint32_t m = max_act < cap ? max_act : cap;
   if (!m) return hipSuccess;
   const dim3 blk(kBlockThreads);
   if (getenv("TSZP_MOVE_SERIAL")) {  // one lane per point
-    const dim3 grid(blocks_for(m));
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
-    else hipLaunchKernelGGL(k_pp_move<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+    PP_GEO_LAUNCH(k_pp_move, dim3(blocks_for(m)), blk, st, act, cap, colour, d, t, y, misc);
   } else {  // sixteen lanes per point
-    const dim3 grid(blocks_for((uint64_t)m * 16));
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_move_seg<2>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
-    else hipLaunchKernelGGL(k_pp_move_seg<3>, grid, blk, 0, st, g, act, cap, colour, d, t, y, eb, misc);
+    PP_GEO_LAUNCH(k_pp_move_seg, dim3(blocks_for((uint64_t)m * 16)), blk, st, act, cap, colour, d, t, y, misc);
   }
   return hipGetLastError();
 }
 
-hipError_t pp_remark(const Grid& g, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
+hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st) {
   if (!nold) return hipSuccess;
-  const uint64_t R = rings == 1 ? 1 : (g.ndim == 2 ? 13 : 25);
+  const uint64_t R = rings == 1 ? 1 : (geo.ndim == 2 ? 13 : 25);
   const uint64_t items = (uint64_t)nold * R, per = (uint64_t)kBlockThreads * kRemarkChunks;
-  const dim3 grid((unsigned)((items + per - 1) / per)), blk(kBlockThreads);
-  if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_remark<2>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
-  else hipLaunchKernelGGL(k_pp_remark<3>, grid, blk, 0, st, g, old, nold, y, t, bits, bad, cap, rings, misc);
+  PP_GEO_LAUNCH(k_pp_remark, dim3((unsigned)((items + per - 1) / per)), dim3(kBlockThreads), st, old, nold, y, t, bits,
+                bad, cap, rings, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_rel(const Grid& g, const float* d, uint32_t n, uint16_t* rel, hipStream_t st) {
+hipError_t pp_rel(const PpGeo& geo, const float* d, uint16_t* rel, hipStream_t st) {
+  const uint32_t n = geo.n;
+  if (geo.fields) {
+    const uint32_t nchunks = (n + kMarkChunk - 1) / kMarkChunk;
+    if (geo.ndim == 2)
+      hipLaunchKernelGGL(k_pp_rel_b<2>, dim3(kMarkBlocks), dim3(kBlockThreads), 0, st, geob(geo), d, nchunks, rel);
+    else hipLaunchKernelGGL(k_pp_rel_b<3>, dim3(kMarkBlocks), dim3(kBlockThreads), 0, st, geob(geo), d, nchunks, rel);
+    return hipGetLastError();
+  }
+  const Grid& g = geo.g;
   const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
   const bool vec = fast % 4 == 0 && ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(rel)) & 15) == 0;
   if (vec) {
@@ -2199,18 +2525,13 @@ hipError_t pp_rel(const Grid& g, const float* d, uint32_t n, uint16_t* rel, hipS
   return hipGetLastError();
 }
 
-hipError_t pp_tiny(const Grid& g, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
-                   const uint8_t* badmap, const uint8_t* t, float* y, float eb, int anchor, uint4* plist,
-                   uint32_t* part, uint32_t* misc, hipStream_t st) {
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
+                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint4* plist, uint32_t* part,
+                   uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  if (g.ndim == 2) {
-    hipLaunchKernelGGL(k_pp_tiny_find<2>, grid, blk, 0, st, g, bad, nbad, rel, badmap, plist);
-    hipLaunchKernelGGL(k_pp_tiny_solve<2>, grid, blk, 0, st, g, plist, d, rel, t, y, eb, anchor, nbad, part);
-  } else {
-    hipLaunchKernelGGL(k_pp_tiny_find<3>, grid, blk, 0, st, g, bad, nbad, rel, badmap, plist);
-    hipLaunchKernelGGL(k_pp_tiny_solve<3>, grid, blk, 0, st, g, plist, d, rel, t, y, eb, anchor, nbad, part);
-  }
+  PP_GEO_LAUNCH(k_pp_tiny_find, grid, blk, st, bad, nbad, rel, badmap, plist);
+  PP_GEO_LAUNCH(k_pp_tiny_solve, grid, blk, st, plist, d, rel, t, y, anchor, nbad, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
@@ -2237,39 +2558,27 @@ void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz) {
   *tz = g.ndim == 3 ? (g.d2 + s - 1) / s : 1u;
 }
 
-hipError_t pp_tiles(const Grid& g, const uint32_t* bad, uint32_t nbad, uint32_t ty, uint32_t tz, uint32_t* pbits,
-                    uint32_t* tbits, uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st) {
+hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* pbits, uint32_t* tbits,
+                    uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_tiles<2>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
-  else hipLaunchKernelGGL(k_pp_tiles<3>, grid, blk, 0, st, g, bad, nbad, ty, tz, pbits, tbits, tiles, cap, misc);
+  PP_GEO_LAUNCH(k_pp_tiles, dim3(blocks_for(nbad)), dim3(kBlockThreads), st, bad, nbad, pbits, tbits, tiles, cap, misc);
   return hipGetLastError();
 }
 
-hipError_t pp_solve_flow(const Grid& g, const uint32_t* tiles, uint32_t total, uint32_t cap, uint32_t tx, uint32_t ty,
-                         uint32_t tz, const uint32_t* tbits, uint32_t* done, const uint32_t* pbits, const float* d,
-                         const uint8_t* t, float* y, float eb, int anchor, uint32_t* misc, hipStream_t st) {
+hipError_t pp_solve_flow(const PpGeo& geo, const uint32_t* tiles, uint32_t total, uint32_t cap, const uint32_t* tbits,
+                         uint32_t* done, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor,
+                         uint32_t* misc, hipStream_t st) {
   if (!total) return hipSuccess;
-  const dim3 grid(total), blk(kWave);
-  if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_solve_flow<2>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t, y,
-                       eb, anchor, misc, dyn_counter(misc + kPpFlowCounter));
-  else hipLaunchKernelGGL(k_pp_solve_flow<3>, grid, blk, 0, st, g, tiles, cap, tx, ty, tz, tbits, done, pbits, d, t,
-                          y, eb, anchor, misc, dyn_counter(misc + kPpFlowCounter));
+  PP_GEO_LAUNCH(k_pp_solve_flow, dim3(total), dim3(kWave), st, tiles, cap, tbits, done, pbits, d, t, y, anchor, misc,
+                dyn_counter(misc + kPpFlowCounter));
   return hipGetLastError();
 }
 
-hipError_t pp_solve(const Grid& g, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, uint32_t ty, uint32_t tz,
-                    int phase, const uint32_t* pbits, const float* d, const uint8_t* t, float* y, float eb,
-                    int anchor, uint32_t* misc, hipStream_t st) {
+hipError_t pp_solve(const PpGeo& geo, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, int phase,
+                    const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor, uint32_t* misc,
+                    hipStream_t st) {
   if (!ntiles) return hipSuccess;
-  const dim3 grid(ntiles), blk(kWave);
-  if (g.ndim == 2)
-    hipLaunchKernelGGL(k_pp_solve<2>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
-                       misc);
-  else hipLaunchKernelGGL(k_pp_solve<3>, grid, blk, 0, st, g, tiles, cap, ty, tz, phase, pbits, d, t, y, eb, anchor,
-                          misc);
+  PP_GEO_LAUNCH(k_pp_solve, dim3(ntiles), dim3(kWave), st, tiles, cap, phase, pbits, d, t, y, anchor, misc);
   return hipGetLastError();
 }
 

@@ -26,10 +26,19 @@ size_t sort_status_words(uint32_t count) {
 }
 
 // ---- per-tile digit histogram, written digit-major: hist[d * ntiles + tile] ----------
+// dcount (nullable): the item count on the device (<= count); the grid covers `count`
+__device__ __forceinline__ uint32_t dev_count(uint32_t count, const uint32_t* dcount) {
+  if (!dcount) return count;
+  const uint32_t c = *dcount;
+  return c < count ? c : count;
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_hist(const uint32_t* __restrict__ words, uint32_t count,
-                                                       int shift, uint32_t* __restrict__ hist, uint32_t ntiles) {
+                                                       int shift, uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                       const uint32_t* __restrict__ dcount) {
   __shared__ uint32_t h[256];
   const int tid = threadIdx.x;
+  count = dev_count(count, dcount);
   h[tid] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kSortTile;
@@ -86,7 +95,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ hout,
                                                           uint32_t* __restrict__ pout, uint32_t count, int shift,
                                                           int digit_from_hi, const uint32_t* __restrict__ hist,
-                                                          uint32_t ntiles) {
+                                                          uint32_t ntiles, const uint32_t* __restrict__ dcount) {
   constexpr int NS = kWavesPerBlock * kItemsPerThread;  // 64 (wave, step) groups
   __shared__ uint16_t cnt[NS][256];
   __shared__ uint32_t gbase[256], lstart[256];
@@ -94,6 +103,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
   __shared__ uint32_t skey[kSortTile], spay[kSortTile];
   __shared__ uint32_t shi[HI ? kSortTile : 1];
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
+  count = dev_count(count, dcount);
+  if (blockIdx.x * kSortTile >= count) return;  // past the items counted on the device
   uint32_t* cz = reinterpret_cast<uint32_t*>(&cnt[0][0]);
   for (int i = tid; i < NS * 256 / 2; i += kBlockThreads) cz[i] = 0;
   gbase[tid] = hist[(size_t)tid * ntiles + blockIdx.x];
@@ -161,6 +172,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter(const uint32_t* __res
 
 size_t scan_status_words(uint32_t count) { return ceil_div(count, kSortTile); }
 
+// back from the alternate buffers after an odd number of passes
+__global__ __launch_bounds__(kBlockThreads) void k_copy3(const uint32_t* __restrict__ k, const uint32_t* __restrict__ h,
+                                                        const uint32_t* __restrict__ p, uint32_t* __restrict__ ko,
+                                                        uint32_t* __restrict__ ho, uint32_t* __restrict__ po,
+                                                        uint32_t count, const uint32_t* __restrict__ dcount) {
+  count = dev_count(count, dcount);
+  for (uint32_t i = blockIdx.x * kSortTile + threadIdx.x; i < count && i < (blockIdx.x + 1) * kSortTile;
+       i += kBlockThreads) {
+    ko[i] = k[i];
+    po[i] = p[i];
+    if (h) ho[i] = h[i];
+  }
+}
+
 hipError_t launch_scan_excl(uint32_t* a, uint32_t count, uint64_t* status, uint32_t* counter, uint32_t* err,
                             hipStream_t st) {
   if (!count) return hipSuccess;
@@ -170,8 +195,10 @@ hipError_t launch_scan_excl(uint32_t* a, uint32_t count, uint64_t* status, uint3
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uint32_t count,
-                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer) {
-  if (count <= 1) return hipSuccess;
+                            const SortWork& wk, hipStream_t st, LaunchTimer* scatter_timer, const uint32_t* dcount,
+                            int hi_bits) {
+  if (count <= 1 && !dcount) return hipSuccess;
+  if (count == 0) return hipSuccess;
   const uint32_t ntiles = ceil_div(count, kSortTile);
   const uint32_t hwords = 256u * ntiles;
   const uint32_t stiles = ceil_div(hwords, kSortTile);
@@ -180,47 +207,66 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* key_hi, uint32_t* pay, uin
   const int words = key_hi ? 2 : 1;
   int pass = 0;
   for (int wd = 0; wd < words; wd++) {
-    for (int shift = 0; shift < 32; shift += 8, pass++) {
+    const int bits = wd ? hi_bits : 32;  // passes over the significant bits of the word
+    for (int shift = 0; shift < bits; shift += 8, pass++) {
       const uint32_t* dw = wd ? h0 : k0;
-      hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlockThreads), 0, st, dw, count, shift, wk.hist, ntiles);
+      hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kBlockThreads), 0, st, dw, count, shift, wk.hist, ntiles,
+                         dcount);
       hipLaunchKernelGGL(k_scan_excl, dim3(stiles), dim3(kBlockThreads), 0, st, wk.hist, hwords,
                          wk.st + (size_t)pass * stiles, dyn_counter(wk.counters + pass), wk.err);
       if (scatter_timer) scatter_timer->begin(st);
       if (h0)
         hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1, count,
-                           shift, wd, wk.hist, ntiles);
+                           shift, wd, wk.hist, ntiles, dcount);
       else
         hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, k1, h1, p1,
-                           count, shift, wd, wk.hist, ntiles);
+                           count, shift, wd, wk.hist, ntiles, dcount);
       if (scatter_timer) scatter_timer->end(st);
       std::swap(k0, k1);
       std::swap(h0, h1);
       std::swap(p0, p1);
     }
   }
+  if (pass & 1) {  // an odd number of passes leaves the result in the alternate buffers
+    hipLaunchKernelGGL(k_copy3, dim3(ntiles), dim3(kBlockThreads), 0, st, k0, h0, p0, keys, key_hi, pay, count, dcount);
+  }
   return hipGetLastError();
 }
 
 // ---- dense rank within bins (segmented look-back scan over sorted keys) ---------------
-__device__ __forceinline__ int bin_of(const uint32_t* keys, const uint32_t* key_hi, uint32_t i, float eb,
-                                      double inv) {
-  if (key_hi) return (int)(key_hi[i] ^ 0x80000000u);
-  return cp_bin(okey_value(keys[i]), eb, inv);
+// Whether items i - 1 and i lie in different bins: by value (the bin of each value with
+// one eb), by key_hi (a biased bin), or, with field bounds, key_hi is the field and the
+// bin comes from the value with that field's error bound.
+__device__ __forceinline__ bool new_bin(const uint32_t* keys, const uint32_t* key_hi, uint32_t i, float eb, double inv,
+                                        const float* feb, const double* finv) {
+  if (feb) {
+    const uint32_t f = key_hi[i];
+    if (f != key_hi[i - 1]) return true;
+    const float e = feb[f];
+    const double v = finv[f];
+    return cp_bin(okey_value(keys[i]), e, v) != cp_bin(okey_value(keys[i - 1]), e, v);
+  }
+  if (key_hi) return key_hi[i] != key_hi[i - 1];
+  return cp_bin(okey_value(keys[i]), eb, inv) != cp_bin(okey_value(keys[i - 1]), eb, inv);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_rank(const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ key_hi,
                                                        const uint32_t* __restrict__ pay, uint32_t count, float eb,
                                                        double inv, int32_t* __restrict__ out, uint32_t out_cap,
-                                                       uint64_t* st, uint32_t* counter, uint32_t* err) {
+                                                       uint64_t* st, uint32_t* counter, uint32_t* err,
+                                                       const uint32_t* __restrict__ dcount,
+                                                       const float* __restrict__ feb, const double* __restrict__ finv) {
   constexpr int NS = kWavesPerBlock * kItemsPerThread;
   __shared__ uint32_t sf[NS], sv[NS];
   __shared__ uint32_t sh_tile;
   __shared__ uint64_t sh_pre;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
+  count = dev_count(count, dcount);
   if (tid == 0) sh_tile = tile_id(counter);
   __syncthreads();
   const uint32_t tile = sh_tile;
+  if (tile * kSortTile >= count) return;  // past the items counted on the device
   const uint32_t base = tile * kSortTile + w * (kWave * kItemsPerThread);
 
   uint32_t F[kItemsPerThread], V[kItemsPerThread];
@@ -232,8 +278,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rank(const uint32_t* __restri
       if (i == 0) {
         f = 1;
       } else {
-        const int b = bin_of(keys, key_hi, i, eb, inv), bp = bin_of(keys, key_hi, i - 1, eb, inv);
-        f = b != bp;
+        f = new_bin(keys, key_hi, i, eb, inv, feb, finv);
         v = f ? 0u : (keys[i] != keys[i - 1]);
       }
     }
@@ -291,12 +336,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_rank(const uint32_t* __restri
 }
 
 hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint32_t* pay, uint32_t count,
-                       float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st) {
+                       float eb, double inv, int32_t* out, uint32_t out_cap, const SortWork& wk, hipStream_t st,
+                       const uint32_t* dcount, const float* field_eb, const double* field_inv) {
   if (count == 0) return hipSuccess;
   const uint32_t ntiles = ceil_div(count, kSortTile);
   const uint32_t stiles = ceil_div((uint64_t)256 * ntiles, kSortTile);
   hipLaunchKernelGGL(k_rank, dim3(ntiles), dim3(kBlockThreads), 0, st, keys, key_hi, pay, count, eb, inv, out,
-                     out_cap, wk.st + (size_t)8 * stiles, dyn_counter(wk.counters + 8), wk.err);
+                     out_cap, wk.st + (size_t)8 * stiles, dyn_counter(wk.counters + 8), wk.err, dcount, field_eb,
+                     field_inv);
   return hipGetLastError();
 }
 

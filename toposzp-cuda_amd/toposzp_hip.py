@@ -46,6 +46,13 @@ class BatchField(C.Structure):
                 ("stream_bytes", C.c_uint64)]
 
 
+class TopoBatchField(C.Structure):
+    _fields_ = [("d_field", C.c_void_p), ("ndim", C.c_int), ("dims", C.c_uint32 * 3), ("eb", C.c_float),
+                ("block_size", C.c_int), ("d_topo", C.c_void_p), ("topo_cap", C.c_uint64),
+                ("d_topo_blk", C.c_void_p), ("d_sortpos", C.c_void_p), ("sortpos_cap", C.c_uint64),
+                ("d_sortpos_blk", C.c_void_p), ("info", TopoInfo), ("restore", RestoreInfo)]
+
+
 class TicketInfo(C.Structure):
     """tszp_ticket_info (include/toposzp_hip.h)."""
     _fields_ = [("stream_bytes", C.c_uint64), ("queue_us", C.c_double), ("latency_us", C.c_double)]
@@ -126,6 +133,17 @@ def lib():
     for f in ("tszp_plain_compress_batch", "tszp_plain_decompress_batch", "tszp_batch_tune"):
         getattr(L, f).argtypes = [C.POINTER(BatchField), u32, vp]
         getattr(L, f).restype = i32
+    L.tszp_topo_batch_create.argtypes = [C.POINTER(TopoBatchField), u32, C.POINTER(vp), vp]
+    L.tszp_topo_batch_destroy.argtypes = [vp]
+    L.tszp_topo_batch_compress.argtypes = [vp, vp]
+    L.tszp_topo_batch_decompress.argtypes = [vp, vp]
+    L.tszp_topo_batch_restore.argtypes = [vp, i32, vp]
+    L.tszp_topo_batch_results.argtypes = [vp, C.POINTER(TopoBatchField), vp]
+    L.tszp_topo_batch_copy.argtypes = [vp, u32, i32, vp, C.c_uint64, vp]
+    for f in ("tszp_topo_batch_create", "tszp_topo_batch_destroy", "tszp_topo_batch_compress",
+              "tszp_topo_batch_decompress", "tszp_topo_batch_restore", "tszp_topo_batch_results",
+              "tszp_topo_batch_copy"):
+        getattr(L, f).restype = i32
     L.tszp_classify.argtypes = [vp, i32, vp, vp, u64p, vp]
     L.tszp_gen_blobs.argtypes = [vp, i32, vp, C.c_uint64, i32, f32, f32, f32, i32, vp]
     L.tszp_gen_fbm.argtypes = [vp, vp, C.c_uint64, i32, f32, vp]
@@ -200,7 +218,8 @@ def lib():
 
 PROF_KINDS = {"codec_topo": 0, "codec_int": 1, "codec_plain": 2, "decode": 3, "sort": 4, "rank": 5,
               "restore": 6, "scatter": 7, "solve": 8, "query": 9, "elem": 10, "elem_decode": 11,
-              "saddle": 12, "batch": 13, "batch_decode": 14}
+              "saddle": 12, "batch": 13, "batch_decode": 14, "topo_batch": 15, "topo_batch_decode": 16,
+              "topo_batch_restore": 17, "topo_batch_sort": 18}
 
 
 def prof_enable(on=True):
@@ -503,6 +522,90 @@ class BatchPlan:
         if self._h:
             _check(lib().tszp_batch_plan_destroy(self._h))
             self._h = C.c_void_p()
+
+
+class TopoBatch:
+    """Batched topology pipeline over many 2D / 3D float32 CUDA tensors (tszp_topo_batch_*):
+    compress / decompress are launches only (capturable in a graph), restore runs the
+    restoration of every field, results() reads sizes and counts back.  bss: block size
+    per field, 0 for the library's choice."""
+
+    def __init__(self, fields, ebs, bss=None, streams=None):
+        torch = _torch()
+        L = lib()
+        n = len(fields)
+        bss = list(bss) if bss is not None else [0] * n
+        self.fields = fields
+        tab = (TopoBatchField * n)()
+        for i, f in enumerate(fields):
+            t = tab[i]
+            t.d_field = f.data_ptr() if f is not None else None
+            t.ndim = f.dim()
+            for a in range(f.dim()):
+                t.dims[a] = f.shape[a]
+            t.eb = float(ebs[i])
+            t.block_size = int(bss[i])
+        if any(b == 0 for b in bss):  # the tuner needs no buffers: ask it for the block sizes first
+            tmp = _batch_table([f.view(-1) for f in fields], ebs, [0] * n)
+            _check(L.tszp_batch_tune(tmp, n, _stream()))
+            for i in range(n):
+                if bss[i] == 0:
+                    bss[i] = tmp[i].block_size
+                    tab[i].block_size = bss[i]
+        self.bss = bss
+        if streams is None:
+            streams = []
+            for i, f in enumerate(fields):
+                m, b = f.numel(), bss[i]
+                streams.append((torch.empty(L.tszp_topo_bound(m, b, 1), dtype=torch.uint8, device=f.device),
+                                torch.empty(L.tszp_blocks(m, b) + 1, dtype=torch.int32, device=f.device),
+                                torch.empty(L.tszp_sortpos_bound(m, b, 1), dtype=torch.uint8, device=f.device),
+                                torch.empty(L.tszp_blocks(m, b) + 1, dtype=torch.int32, device=f.device)))
+        self.streams = streams
+        for i, (tp, tb, sp, sb) in enumerate(streams):
+            t = tab[i]
+            t.d_topo, t.topo_cap, t.d_topo_blk = tp.data_ptr(), tp.numel(), tb.data_ptr()
+            t.d_sortpos, t.sortpos_cap, t.d_sortpos_blk = sp.data_ptr(), sp.numel(), sb.data_ptr()
+        self.tab = tab
+        self._h = C.c_void_p()
+        _check(L.tszp_topo_batch_create(tab, n, C.byref(self._h), _stream()))
+
+    def compress(self):
+        _check(lib().tszp_topo_batch_compress(self._h, _stream()))
+
+    def decompress(self):
+        _check(lib().tszp_topo_batch_decompress(self._h, _stream()))
+
+    def restore(self, max_rounds=16):
+        _check(lib().tszp_topo_batch_restore(self._h, max_rounds, _stream()))
+
+    def results(self):
+        """[(TopoInfo, RestoreInfo)] per field, in the caller's order."""
+        _check(lib().tszp_topo_batch_results(self._h, self.tab, _stream()))
+        return [(self.tab[i].info, self.tab[i].restore) for i in range(len(self.fields))]
+
+    def copy(self, i, which, count=None):
+        """Field i's decoded values ('decoded'), restored values ('restored'), types
+        ('types') or sort positions ('sortpos', count = its extremum count) as a new tensor."""
+        torch = _torch()
+        f = self.fields[i]
+        k = {"decoded": 0, "restored": 1, "types": 2, "sortpos": 3}[which]
+        m = f.numel() if count is None else count
+        dt = {0: torch.float32, 1: torch.float32, 2: torch.uint8, 3: torch.int32}[k]
+        out = torch.empty(max(m, 1), dtype=dt, device=f.device)
+        _check(lib().tszp_topo_batch_copy(self._h, i, k, out.data_ptr(), m, _stream()))
+        return out[:m]
+
+    def close(self):
+        if self._h:
+            _check(lib().tszp_topo_batch_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:
