@@ -311,6 +311,114 @@ def batch_side(tz, torch, seed, reps):
     return res
 
 
+def topo_batch_side(tz, torch, seed, reps):
+    """Mixed-size workload through the batched topology pipeline (tszp_topo_batch_*):
+    compress + decompress + restoration of all 256 fields per step, one launch per stage
+    and block-size class, against a loop of the one-field calls (tszp_topo_compress,
+    tszp_topo_decompress, tszp_sortpos_decompress, tszp_topo_restore) on the same fields
+    and block sizes.  GB/s of input per step; latency percentiles over the repetitions;
+    the compress + decompress part also replayed from a captured graph."""
+    import numpy as np
+    import workloads
+    wl = workloads.mixed_fields(seed)
+    ts = [f for f, _ in wl]
+    ebs = [eb for _, eb in wl]
+    total = sum(t.numel() for t in ts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch = tz.TopoBatch(ts, ebs, None)
+    create_ms = 1e3 * (time.perf_counter() - t0)
+    bss = batch.bss
+
+    def step():
+        batch.compress()
+        batch.decompress()
+        batch.restore()
+        return batch.results()
+    res = step()
+    mism = sum(r.mismatches() for _, r in res)
+    maxdev = max(r.max_dev_over_eb for _, r in res)
+    tz.prof_reset()
+    tz.prof_enable(True)
+    bt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        step()
+        bt.append(time.perf_counter() - t0)
+    tz.prof_enable(False)
+    pr = tz.prof_read()
+    # compress + decompress from a graph
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            batch.compress()
+            batch.decompress()
+    torch.cuda.current_stream().wait_stream(s)
+    g.replay()
+    torch.cuda.synchronize()
+    gt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        gt.append(time.perf_counter() - t0)
+    gres = batch.results()
+    gsame = all(a[0].topo_bytes == b[0].topo_bytes and a[0].sortpos_bytes == b[0].sortpos_bytes
+                for a, b in zip(gres, res))
+    del g
+    # the loop of one-field calls
+    bufs = [tz.TopoBuffers(f.numel(), b, 1, f.device) for f, b in zip(ts, bss)]
+    outs = [torch.empty(f.numel(), dtype=torch.float32, device=f.device) for f in ts]
+
+    def loop():
+        infos = []
+        for i, f in enumerate(ts):
+            info, _ = tz.topo_compress(f, ebs[i], bss[i], T=1, buf=bufs[i])
+            out, types = tz.topo_decompress(bufs[i], f.numel(), ebs[i], bss[i], T=1)
+            sp = tz.sortpos_decompress(bufs[i], info.ext_count, bss[i], T=1) if info.ext_count else None
+            r = tz.topo_restore(out, types, tuple(f.shape), ebs[i], sp=sp, n_ext=info.ext_count, out=outs[i])
+            infos.append((info, r))
+        return infos
+    linfo = loop()
+    torch.cuda.synchronize()
+    same = all(a[0].topo_bytes == b[0].topo_bytes and a[0].sortpos_bytes == b[0].sortpos_bytes and
+               torch.equal(bufs[i].topo[:a[0].topo_bytes], batch.streams[i][0][:a[0].topo_bytes]) and
+               torch.equal(bufs[i].sp[:a[0].sortpos_bytes], batch.streams[i][2][:a[0].sortpos_bytes])
+               for i, (a, b) in enumerate(zip(res, linfo)))
+    lt = []
+    for _ in range(max(2, reps // 4)):
+        t0 = time.perf_counter()
+        loop()
+        torch.cuda.synchronize()
+        lt.append(time.perf_counter() - t0)
+    batch.close()
+    del bufs, outs
+    cbytes = sum(r[0].topo_bytes + r[0].sortpos_bytes for r in res)
+    bt, lt, gt = np.array(bt), np.array(lt), np.array(gt)
+    per = lambda k: pr[k][0] / max(1, reps)
+    out = {"fields": len(ts), "elements": total, "ratio": 4 * total / cbytes,
+           "block_sizes": {str(b): bss.count(b) for b in sorted(set(bss))}, "create_ms": create_ms,
+           "batch_gbs": 4 * total / float(np.median(bt)) / 1e9,
+           "batch_p50_ms": 1e3 * float(np.percentile(bt, 50)), "batch_p99_ms": 1e3 * float(np.percentile(bt, 99)),
+           "loop_gbs": 4 * total / float(np.median(lt)) / 1e9,
+           "loop_p50_ms": 1e3 * float(np.percentile(lt, 50)), "loop_p99_ms": 1e3 * float(np.percentile(lt, 99)),
+           "compress_ms": per("topo_batch"), "sort_positions_ms": per("topo_batch_sort"),
+           "decompress_ms": per("topo_batch_decode"), "restore_ms": per("topo_batch_restore"),
+           "graph_compress_decompress_p50_ms": 1e3 * float(np.percentile(gt, 50)),
+           "graph_compress_decompress_gbs": 4 * total / float(np.median(gt)) / 1e9, "graph_identical": bool(gsame),
+           "identical_to_loop": bool(same), "mismatches": int(mism), "max_dev_over_eb": float(maxdev)}
+    out["speedup_vs_loop"] = out["batch_gbs"] / out["loop_gbs"]
+    log(f"topology batch of {len(ts)} fields ({total / 1e6:.1f} M elements, ratio {out['ratio']:.2f}, block sizes "
+        f"{out['block_sizes']}): compress+decompress+restore {out['batch_gbs']:.1f} GB/s (p50 {out['batch_p50_ms']:.2f}"
+        f" ms, p99 {out['batch_p99_ms']:.2f} ms; stages {out['compress_ms']:.2f} (sort {out['sort_positions_ms']:.2f})"
+        f" + {out['decompress_ms']:.2f} + {out['restore_ms']:.2f} ms), loop of one-field calls {out['loop_gbs']:.1f} "
+        f"GB/s (p50 {out['loop_p50_ms']:.1f} ms) -> {out['speedup_vs_loop']:.1f}x, identical {same}, mismatches "
+        f"{mism}; compress+decompress graph {out['graph_compress_decompress_p50_ms']:.2f} ms")
+    return out
+
+
 def serve_side(tz, torch, pool, big, seed, nreq, rate):
     """Served stream: nreq requests with Poisson arrivals at `rate` per second, 70 %
     compressions of a pool field (the 256 mixed-size fields, or with probability 2 % the
@@ -582,6 +690,9 @@ def main():
     # side figures: threadblock / double random-access codecs, double 3D topology
     elem = None if args.no_elem else elem_side(tz, torch, f, shape, eb, bs, 3)
 
+    # side figure: 256 mixed-size fields through the batched topology pipeline
+    topo_batch = None if args.no_batch else topo_batch_side(tz, torch, args.seed, 8)
+
     # side figure: 256 mixed-size fields, batched against a loop of one-field calls
     batch = None if args.no_batch else batch_side(tz, torch, args.seed, 12)
     served = None
@@ -656,6 +767,7 @@ def main():
         "other_block_sizes": other_bs,
         "topo_stream_parse_ms": parse_ms,
         "elem_codecs": elem,
+        "batched_topo": topo_batch,
         "batched_plain": batch,
         "served_plain": served,
         "quality_ms": quality_ms,

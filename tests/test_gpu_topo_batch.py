@@ -190,3 +190,60 @@ def test_topo_batch_rejects_bad_fields():
         tz.TopoBatch([f], [0.0], [64])
     with pytest.raises(tz.TszpError):
         tz.TopoBatch([f], [1e-3], [300])
+
+
+def test_topo_batch_mixed_workload_256_fields():
+    """The mixed-size workload at its real shapes: 128 2D fields (rows, cols log-uniform
+    in [64, 2048]) and 128 3D fields (dims log-uniform in [32, 256]), blobs + noise, abs
+    eb log-uniform in [1e-4, 1e-2], block sizes from the tuner.  Every field's streams and
+    side-cars equal the one-field call's, the 16 smallest also the oracle's; after the
+    batched restoration every field's classification (by the one-field classifier, and
+    by the oracle on the 16 smallest) equals the original's and every value stays in
+    its cell."""
+    import workloads
+    torch = _torch()
+    wl = workloads.mixed_fields(20251203)
+    ts = [f for f, _ in wl]
+    ebs = [eb for _, eb in wl]
+    print(f"mixed workload: {len(ts)} fields, {sum(t.numel() for t in ts)} elements", flush=True)
+    batch = tz.TopoBatch(ts, ebs, None)
+    batch.compress()
+    batch.decompress()
+    batch.restore()
+    res = batch.results()
+    small = set(sorted(range(len(ts)), key=lambda i: ts[i].numel())[:16])
+    for i, f in enumerate(ts):
+        info, rinfo = res[i]
+        bs, eb = batch.bss[i], ebs[i]
+        ref, buf = tz.topo_compress(f, eb, bs, T=1)
+        assert (info.topo_bytes, info.sortpos_bytes, info.cp_count, info.ext_count) == \
+            (ref.topo_bytes, ref.sortpos_bytes, ref.cp_count, ref.ext_count), f"field {i} {tuple(f.shape)}"
+        tp, tb, sp, sb = batch.streams[i]
+        assert torch.equal(tp[:info.topo_bytes], buf.topo[:ref.topo_bytes]), f"field {i}: topology stream"
+        assert torch.equal(tb, buf.topo_blk), f"field {i}: topology side-car"
+        assert torch.equal(sp[:info.sortpos_bytes], buf.sp[:ref.sortpos_bytes]), f"field {i}: sort-position stream"
+        out, types = tz.topo_decompress(buf, f.numel(), eb, bs)
+        assert torch.equal(batch.copy(i, "decoded"), out) and torch.equal(batch.copy(i, "types"), types)
+        # restoration: zero mismatches by the one-field classifier, values in their cells
+        y = batch.copy(i, "restored")
+        ty, _ = tz.classify(y.view(f.shape))
+        assert torch.equal(ty, types), f"field {i} {tuple(f.shape)}: critical points differ after restoration"
+        ebf = float(np.float32(eb))
+        # |y - d| <= 0.9 eb up to the float spacing of the largest |d| (d + 0.9 eb rounds)
+        amax = float(f.abs().max())
+        assert rinfo.mismatches() == 0 and rinfo.max_dev_over_eb <= 0.9 * (1 + 1e-5) + amax * 2.0 ** -22 / ebf
+        err = float((y.double() - f.view(-1).double()).abs().max())
+        assert err <= ebf * (1 + 2.0 ** -16) + float(f.abs().max()) * 2.0 ** -23, f"field {i}: error {err / ebf} eb"
+        if i in small:
+            data = f.cpu().numpy()
+            cps = oracle.find_cps_2d(data, eb) if data.ndim == 2 else oracle.find_cps_3d(data, eb)
+            oracle.sort_positions(cps, data)
+            otypes = oracle.mark_types(cps, data.shape)
+            assert np.array_equal(tp[:info.topo_bytes].cpu().numpy(), oracle.topo_compress(data, eb, bs, otypes,
+                                                                                             adaptive=data.ndim == 2))
+            spv = oracle.extrema_sort_positions(cps)
+            if spv.size:
+                assert np.array_equal(sp[:info.sortpos_bytes].cpu().numpy(), oracle.sortpos_compress(spv, bs))
+            assert np.array_equal(oracle.classify(y.cpu().numpy().reshape(data.shape)), oracle.classify(data))
+        del buf
+    batch.close()

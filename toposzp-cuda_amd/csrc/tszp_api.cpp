@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsPpFields, kWsCount
+  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsCount
 };
 
 struct Ctx {
@@ -1781,9 +1781,49 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
               h[kPpStatLeaves + 4] * 0.01, nb);
     solved += h[kPpSolved];
     freed += h[kPpFreed];
-    if (!h[kPpSolved] && outer > 0) break;
+    // another outer round only while many mismatches are left and the windows still
+    // remove a quarter of what was listed: the rest lies on plateaus, for the
+    // whole-plateau stage below
+    if (nb <= kPpSolveFew || (uint64_t)nb * 4 > (uint64_t)nl * 3) break;
   }
   if (!verified && (r = full_mark(&nb, true))) return r;
+  // what the windows could not order: whole plateaus, guided by the smoothed field first
+  if (nb) {
+    if (!exact && (r = full_mark(&nb, true))) return r;
+    const uint32_t nl = nb < cap ? nb : cap;
+    float *s0, *s1;
+    WS(kWsPpSm0, n * 4 + 16, s0);
+    WS(kWsPpSm1, n * 4 + 16, s1);
+    const float* src = d;
+    for (int a = 0; a < ndim; a++) {
+      float* dst = a % 2 ? s1 : s0;
+      HIPCHK(pp_blur(geo, src, dst, a, st));
+      src = dst;
+    }
+    FillRanges fr{};
+    fr.add(bits, 0u, bit_words);
+    fr.add(misc + kPpPlatSolved, 0u, 3);
+    fr.add(misc + kPpFreed, 0u, 2);  // freed, annealed
+    fr.add(misc + kPpPlatMaxTicks, 0u, 3);
+    HIPCHK(launch_fill_multi(fr, st));
+    HIPCHK(pp_plateau(geo, bad, nl, d, d_types, d_field, src, bits, false, misc, st));
+    const uint32_t before = nb;
+    if ((r = read_misc())) return r;
+    if (h[kPpPlatBig]) {  // plateaus beyond the small variant: the large one over the same list
+      FillRanges fb{};
+      fb.add(bits, 0u, bit_words);
+      HIPCHK(launch_fill_multi(fb, st));
+      HIPCHK(pp_plateau(geo, bad, nl, d, d_types, d_field, src, bits, true, misc, st));
+    }
+    if ((r = full_mark(&nb, true))) return r;
+    freed += h[kPpFreed];
+    if (stats)
+      fprintf(stderr, "[restore] plateaus solved %u (annealed %u) failed %u too large %u: %u -> %u mismatches; "
+              "max plateau us %.1f, max anneal us %.1f, max sweeps %u\n", h[kPpPlatSolved], h[kPpPlatAnneal],
+              h[kPpPlatFailed], h[kPpPlatBig], before, nb, h[kPpPlatMaxTicks] * 0.01, h[kPpPlatMaxAnneal] * 0.01,
+              h[kPpPlatMaxSweeps]);
+    solved += h[kPpPlatSolved];
+  }
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
   const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);
   for (int k = 0; k < 4; k++) out->verify[k] = hv[k];

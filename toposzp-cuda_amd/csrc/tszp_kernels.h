@@ -396,12 +396,14 @@ hipError_t launch_quality(const Grid& g, const float* x, const float* y, uint32_
 constexpr int kPpMiscWords = 64;
 enum : int {
   kPpBad = 0, kPpMoves = 1, kPpAct = 2, kPpTiles = 3, kPpVerify = 4, kPpErr = 12, kPpExt = 13, kPpSpMax = 14,
-  kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18,
+  kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18, kPpPlatAnneal = 19,
   kPpActC = 20,      // 7 colour-list counts
   kPpTilesP = 32,    // 8 parity-phase tile-list counts
   kPpStatNodes = 40, kPpStatMaxNodes = 41, kPpStatMaxWin = 44, kPpStatMaxTicks = 45, kPpStatMaxWait = 46, kPpStatLeaves = 47, kPpTiny = 42,
   kPpSolveCounter = 43,
-  kPpFlowCounter = 60  // tile counter of the one-launch solve
+  kPpPlatMaxTicks = 52, kPpPlatMaxAnneal = 53, kPpPlatMaxSweeps = 54,  // whole-plateau solve statistics
+  kPpFlowCounter = 60,  // tile counter of the one-launch solve
+  kPpPlatSolved = 61, kPpPlatFailed = 62, kPpPlatBig = 63  // whole-plateau solves
 };
 // mismatches left after a local round below which the window solve runs at once
 constexpr uint32_t kPpSolveFew = 16384;
@@ -413,6 +415,10 @@ hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* m
 size_t pp_init_work_words(uint32_t n);
 hipError_t pp_init(const PpGeo& geo, const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, float* y,
                    uint32_t* work, uint32_t* misc, hipStream_t st);
+// whole-plateau solves around the listed mismatches (one wavefront each); guide: the
+// field whose order is tried first (nullable); claim: zeroed bits, one per point
+// binomial smoothing of src along one axis into dst (radius 4, edges clamped)
+hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, hipStream_t st);
 // full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
 // badmap (nullable) receives one byte per point, 1 where mismatched (where pp_mark_maps);
 // per_field: a batch's per-field counts into geo.fcount
@@ -438,6 +444,9 @@ hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStr
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
 hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* pbits, uint32_t* tbits,
                     uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);
+// (large: the variant for plateaus of up to 3072 points, 150 KB of LDS per wavefront)
+hipError_t pp_plateau(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
+                      const float* guide, uint32_t* claim, bool large, uint32_t* misc, hipStream_t st);
 hipError_t pp_solve(const PpGeo& geo, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, int phase,
                     const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor, uint32_t* misc,
                     hipStream_t st);

@@ -368,6 +368,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_spmax_b(GeoB geo, const in
   }
 }
 
+// Binomial smoothing along one axis (weights C(8, k) / 256, radius 4, edges clamped):
+// the smoothed decoded field is the first guide of the whole-plateau solve.
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_blur(GEO geo, const float* __restrict__ src,
+                                                          float* __restrict__ dst, uint32_t n, int axis) {
+  constexpr float w[9] = {1.f / 256, 8.f / 256, 28.f / 256, 56.f / 256, 70.f / 256,
+                          56.f / 256, 28.f / 256, 8.f / 256, 1.f / 256};
+  for (uint32_t e = blockIdx.x * kBlockThreads + threadIdx.x; e < n; e += gridDim.x * kBlockThreads) {
+    const FieldView F = geo.at(e);
+    if (e >= F.base + F.n) continue;
+    int c[ND];
+    coords_of<ND>(F.g, e - F.base, c);
+    const int dim = dim_of<ND>(F.g, axis), ca = c[axis];
+    const int64_t s = (int64_t)stride_of<ND>(F.g, axis);
+    float acc = 0.f;
+#pragma unroll
+    for (int k = -4; k <= 4; k++) {
+      const int q = min(max(ca + k, 0), dim - 1);
+      acc += w[k + 4] * src[(int64_t)e + (int64_t)(q - ca) * s];
+    }
+    dst[e] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // initial values.  A tile of kInitTile points ranks its extrema in row-major order
 // (look-back over tiles) so that the k-th extremum takes sort position sp[k].
@@ -496,6 +520,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(GEO geo, const float*
 #pragma unroll
         for (int c = 0; c < 4; c++) dv[u][c] = e0 + c < n ? d[e0 + c] : 0.f;
       }
+
       const uint32_t tv = tw[gidx];
 #pragma unroll
       for (int c = 0; c < 4; c++) {
@@ -1666,6 +1691,10 @@ __device__ __forceinline__ float rdlf(float v, int src) {
 constexpr int kTileShift = 4;  // 16-wide tiles: windows of one parity class never interact
 constexpr int kWinMax = 8;
 constexpr int kSolveBudget = 1 << 14;
+// search nodes of one tile's windows beyond which its remaining windows are not tried: the
+// one-launch solve lasts as long as its longest chain of dependent tiles, and the windows
+// that exhaust their budgets lie on plateaus the whole-plateau stage orders
+constexpr uint32_t kTileNodeCap = 1u << 16;
 
 template <int ND>
 __device__ __forceinline__ uint32_t tile_of(const int (&c)[ND], uint32_t ty, uint32_t tz) {
@@ -2222,6 +2251,11 @@ __device__ void solve_tile(SolveShared<ND>& sh, const FieldView& F, uint32_t id,
     int c[ND];
     coords_of<ND>(g, e - F.base, c);
     if (classify_at_y<ND, COH>(g, y, e, c) == (int)t[e]) continue;  // fixed by an earlier window
+    // (lane 0 keeps the count; the branch must be uniform over the wave)
+    if ((uint32_t)__shfl((int)nodes, 0, kWave) >= kTileNodeCap) {  // left to the whole-plateau stage
+      failed++;
+      continue;
+    }
     const int mode =
         solve_window<ND, COH>(w, g, F.base, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
     solved += mode >= 0;
@@ -2335,6 +2369,591 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(GEO geo, const uint32_t
 }
 
 // ---------------------------------------------------------------------------------
+// Whole-plateau solve for what the window solve leaves.  A plateau (connected points of
+// one decoded value) orders its points freely inside the cell, and no other point's type
+// depends on that order (every other neighbour is in another cell), so a plateau is
+// solved on its own: its points are inserted one at a time in increasing value, and a
+// point is insertable when its type with the inserted neighbours below it and the others
+// above equals its target.  The search tries points in the order of a guide (a smoothed
+// field, then the current values), and after each insertion rejects it when a
+// neighbour could no longer reach its type whatever is inserted before it (forward
+// checking); it backtracks within a node budget.  Anchored extrema keep their values and
+// are inserted in the order of those values (ties back to back), and the other points
+// take values spaced between them, so the sort-position order of a cell's extrema holds.
+// One wavefront per listed mismatch; the plateau is claimed by its smallest point.
+// ---------------------------------------------------------------------------------
+// capacities: the small variant first, the large one (most of a CU's 160 KB of LDS) for
+// the plateaus the small one could not hold
+constexpr int kPlatSmall = 512, kPlatLarge = 3072;
+constexpr int kPlatAnchored = 4, kPlatPasses = 4 + 4;  // insertion passes with anchors, in all
+template <int ND, int PM>
+struct PlatShared {
+  static constexpr int kCap = PM;
+  static constexpr int kHash = PM <= 512 ? 1024 : 8192;  // slots, a power of two >= 2 PM
+  uint32_t el[PM];
+  int16_t nb[PM][2 * ND];  // >= 0: plateau index; -1 lower cell; -2 higher cell or outside
+  uint16_t sorted[PM];
+  uint16_t order[PM], choice[PM];
+  uint16_t alow[PM], aupto[PM];
+  float av[PM];
+  int8_t tgt[PM];  // target type, -1 off the interior
+  int8_t anc[PM];
+  uint32_t assigned[PM / 32];
+  uint32_t hkey[kHash];  // membership while the plateau is gathered, then the guide keys
+  uint16_t hval[kHash];
+  uint32_t cand[kWave];
+  int ne;
+  int abort;
+  float lo, hi, db;
+};
+
+template <int H>
+__device__ __forceinline__ uint32_t plat_hash(uint32_t e) {
+  constexpr int bits = H == 1024 ? 10 : 13;
+  return (e * 2654435761u) >> (32 - bits);
+}
+
+template <class SH>
+__device__ __forceinline__ int plat_find(const SH& sh, uint32_t e) {
+  constexpr int H = SH::kHash;
+  uint32_t h = plat_hash<H>(e);
+  for (int k = 0; k < H; k++) {
+    const uint32_t key = sh.hkey[h];
+    if (key == e) return sh.hval[h];
+    if (key == 0xffffffffu) return -1;
+    h = (h + 1) & (H - 1);
+  }
+  return -1;
+}
+
+template <class SH>
+__device__ __forceinline__ bool plat_assigned(const SH& sh, int i) {
+  return (sh.assigned[i >> 5] >> (i & 31)) & 1u;
+}
+
+// type from "lower than the centre" flags of the 2*ND neighbours (axis a: 2a, 2a + 1)
+template <int ND>
+__device__ __forceinline__ int type_from_low(uint32_t low) {
+  const uint32_t all = (1u << (2 * ND)) - 1u;
+  if (low == all) return 1;
+  if (low == 0) return 2;
+  bool hi = false, lo = false;
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    const uint32_t m = (low >> (2 * a)) & 3u;
+    hi = hi || m == 3u;
+    lo = lo || m == 0u;
+  }
+  return (hi && lo) ? 3 : 0;
+}
+
+template <int ND, class SH>
+__device__ __forceinline__ uint32_t low_now(const SH& sh, int i) {
+  uint32_t low = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * ND; k++) {
+    const int q = sh.nb[i][k];
+    if (q == -1 || (q >= 0 && plat_assigned(sh, q))) low |= 1u << k;
+  }
+  return low;
+}
+
+// whether element j can still take its type with some of its free neighbours inserted
+// before it (whole wave: lane l tries subset l of the free neighbours)
+template <int ND, class SH>
+__device__ __forceinline__ bool plat_possible(const SH& sh, int j, int lane) {
+  if (sh.tgt[j] < 0) return true;
+  const uint32_t base = low_now<ND>(sh, j);
+  int fk[2 * ND];
+  int nf = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * ND; k++) {
+    const int q = sh.nb[j][k];
+    fk[k] = -1;
+    if (q >= 0 && !plat_assigned(sh, q)) fk[nf++] = k;
+  }
+  bool ok = false;
+  if (lane < (1 << nf)) {
+    uint32_t low = base;
+    for (int b = 0; b < nf; b++)
+      if ((lane >> b) & 1) low |= 1u << fk[b];
+    ok = type_from_low<ND>(low) == sh.tgt[j];
+  }
+  return __ballot(ok) != 0;
+}
+
+// Annealing state over the order of a plateau: gk[i] is point i's key (order only; ties
+// broken by index), and a point's type is read from which of its neighbours have the
+// smaller (key, index).
+template <int ND, class SH>
+__device__ __forceinline__ bool plat_key_lower(const SH& sh, const float* gk, int q, int i) {
+  return gk[q] < gk[i] || (gk[q] == gk[i] && q < i);
+}
+template <int ND, class SH>
+__device__ __forceinline__ int plat_viol(const SH& sh, const float* gk, int i) {
+  if (sh.tgt[i] < 0) return 0;
+  uint32_t low = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * ND; k++) {
+    const int q = sh.nb[i][k];
+    if (q == -1 || (q >= 0 && plat_key_lower<ND>(sh, gk, q, i))) low |= 1u << k;
+  }
+  return type_from_low<ND>(low) != sh.tgt[i];
+}
+// violations of point i and of its plateau neighbours: everything a move of i can change
+template <int ND, class SH>
+__device__ __forceinline__ int plat_local_viol(const SH& sh, const float* gk, int i) {
+  int c = plat_viol<ND>(sh, gk, i);
+#pragma unroll
+  for (int k = 0; k < 2 * ND; k++) {
+    const int q = sh.nb[i][k];
+    if (q >= 0) c += plat_viol<ND>(sh, gk, q);
+  }
+  return c;
+}
+__device__ __forceinline__ uint64_t plat_mix(uint64_t h) {
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+constexpr int kAnnealCycle = 1000, kAnnealSweeps = 6 * kAnnealCycle;
+
+// Annealing over the order of a plateau, when no anchored insertion pass got through.
+// A point's type depends only on its order against its 2*ND neighbours, so the search
+// moves one point at a time to another gap between its neighbours' keys (Metropolis
+// acceptance on the number of violated types of it and its neighbours, temperature
+// falling linearly over each cycle of kAnnealCycle sweeps, then reheated). Points of one
+// colour, (x + 2y + 3z) mod 7 in 3D and (x + 2y) mod 5 in 2D, are at least 3 apart, so
+// none reads what another of its colour writes and a colour moves in parallel. Anchored
+// extrema keep their values (keys), so their order holds by construction.
+// Returns whether every type holds; then sh.order holds the points in increasing key.
+template <int ND, class SH>
+__device__ bool plat_anneal(SH& sh, float* gk, int ne, const FieldView& F, const float* __restrict__ guide,
+                            float db, int lane, uint32_t* sweeps_out) {
+  constexpr int K = 2 * ND;
+  const Grid& g = F.g;
+  constexpr int NC = ND == 3 ? 7 : 5;
+  // per point: the lower-neighbour flags and whether its type is violated, kept in step
+  // with the keys (in the hash table's value slots, free once the plateau is gathered)
+  uint8_t* msk = reinterpret_cast<uint8_t*>(sh.hval);
+  uint8_t* vio = msk + SH::kCap;
+  const float inv_eb = 1.f / F.eb;
+  for (int i = lane; i < ne; i += kWave) {
+    const uint32_t e = sh.el[i];
+    const uint64_t hsh = plat_mix(((uint64_t)e << 8) ^ 0x5bd1e995ull);
+    const float jit = sh.anc[i] ? 0.f : 1e-4f * ((float)(uint32_t)hsh * 2.3283064e-10f);
+    const float v0 = sh.anc[i] || guide == nullptr ? sh.av[i] : guide[e];
+    gk[i] = (v0 - db) * inv_eb + jit;
+    int c[ND];
+    coords_of<ND>(g, e - F.base, c);
+    const int col = ND == 3 ? (c[0] + 2 * c[1] + 3 * c[2]) % 7 : (c[0] + 2 * c[1]) % 5;
+    sh.choice[i] = (uint16_t)col;
+  }
+  __syncthreads();
+  if (lane == 0) {  // points by colour: sh.sorted, colour starts in sh.alow[0..NC]
+    for (int c = 0; c <= NC; c++) sh.alow[c] = 0;
+    for (int i = 0; i < ne; i++) sh.alow[sh.choice[i] + 1]++;
+    for (int c = 0; c < NC; c++) sh.alow[c + 1] += sh.alow[c];
+    for (int c = 0; c < NC; c++) sh.aupto[c] = sh.alow[c];
+    for (int i = 0; i < ne; i++) sh.sorted[sh.aupto[sh.choice[i]]++] = (uint16_t)i;
+  }
+  int cur = 0;
+  for (int i = lane; i < ne; i += kWave) {
+    uint32_t low = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = sh.nb[i][k];
+      if (q == -1 || (q >= 0 && plat_key_lower<ND>(sh, gk, q, i))) low |= 1u << k;
+    }
+    const int v = sh.tgt[i] >= 0 && type_from_low<ND>(low) != sh.tgt[i];
+    msk[i] = (uint8_t)low;
+    vio[i] = (uint8_t)v;
+    cur += v;
+  }
+  cur = wave_sum_i(cur);
+  __syncthreads();
+  int sw = 0;
+  for (; sw < kAnnealSweeps && cur > 0; sw++) {
+    const float temp = 0.7f * (1.f - (float)(sw % kAnnealCycle) / (float)kAnnealCycle) + 0.02f;
+    int dsum = 0;
+    for (int c = 0; c < NC; c++) {
+      const int s1 = sh.alow[c + 1];
+      for (int p = sh.alow[c] + lane; p < s1; p += kWave) {
+        const int i = sh.sorted[p];
+        if (sh.anc[i]) continue;
+        int qk[K];
+        float kq[K];
+        float nk[K];
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          qk[k] = sh.nb[i][k];
+          kq[k] = qk[k] >= 0 ? gk[qk[k]] : 0.f;
+          if (qk[k] >= 0) nk[n++] = kq[k];
+        }
+        if (!n) continue;
+        for (int a = 1; a < n; a++) {  // insertion sort of at most 6 keys
+          const float v = nk[a];
+          int b2 = a - 1;
+          while (b2 >= 0 && nk[b2] > v) {
+            nk[b2 + 1] = nk[b2];
+            b2--;
+          }
+          nk[b2 + 1] = v;
+        }
+        int gaps = 0;
+        for (int a = 0; a + 1 < n; a++) gaps += nk[a + 1] > nk[a];
+        const uint64_t r =
+            plat_mix(((uint64_t)sh.el[i] << 20) ^ ((uint64_t)sw << 3) ^ (uint64_t)blockIdx.x * 0x9e3779b97f4a7c15ull);
+        const float u1 = (float)(uint32_t)(r >> 40) * (1.f / 16777216.f);
+        const float u2 = (float)(uint32_t)((r >> 16) & 0xffffffu) * (1.f / 16777216.f);
+        const int pick = (int)((uint32_t)r & 0xffffu) % (gaps + 2);
+        float cand;
+        if (pick == 0) cand = nk[0] - 1e-3f * (0.5f + u1);
+        else if (pick == gaps + 1) cand = nk[n - 1] + 1e-3f * (0.5f + u1);
+        else {
+          int a = 0, seen = 0;
+          for (; a + 1 < n; a++)
+            if (nk[a + 1] > nk[a] && ++seen == pick) break;
+          cand = nk[a] + (nk[a + 1] - nk[a]) * (0.25f + 0.5f * u1);
+        }
+        // the move's effect on i and on each plateau neighbour q (which sees i in
+        // direction k ^ 1)
+        uint32_t low = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int q = qk[k];
+          if (q == -1 || (q >= 0 && (kq[k] < cand || (kq[k] == cand && q < i)))) low |= 1u << k;
+        }
+        const int vi = sh.tgt[i] >= 0 && type_from_low<ND>(low) != sh.tgt[i];
+        int dl = vi - (int)vio[i];
+        uint8_t qm[K], qv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int q = qk[k];
+          qm[k] = 0;
+          qv[k] = 0;
+          if (q < 0) continue;
+          const uint32_t bit = 1u << (k ^ 1);
+          const uint32_t mq = ((low >> k) & 1u) ? (msk[q] & ~bit) : (msk[q] | bit);
+          const int vq = sh.tgt[q] >= 0 && type_from_low<ND>(mq) != sh.tgt[q];
+          qm[k] = (uint8_t)mq;
+          qv[k] = (uint8_t)vq;
+          dl += vq - (int)vio[q];
+        }
+        if (dl <= 0 || u2 < __expf(-(float)dl / temp)) {
+          dsum += dl;
+          gk[i] = cand;
+          msk[i] = (uint8_t)low;
+          vio[i] = (uint8_t)vi;
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            if (qk[k] >= 0) {
+              msk[qk[k]] = qm[k];
+              vio[qk[k]] = qv[k];
+            }
+        }
+      }
+      __syncthreads();
+    }
+    cur += wave_sum_i(dsum);
+  }
+  *sweeps_out = (uint32_t)sw;
+  if (cur == 0) {
+    int v = 0;  // recount from the keys: the kept flags are exact, this is their check
+    for (int i = lane; i < ne; i += kWave) v += plat_viol<ND>(sh, gk, i);
+    if (wave_sum_i(v) == 0) {
+      for (int i = lane; i < ne; i += kWave) {
+        const float gi = gk[i];
+        int rk = 0;
+        for (int j = 0; j < ne; j++) rk += (gk[j] < gi) || (gk[j] == gi && j < i);
+        sh.order[rk] = (uint16_t)i;
+      }
+      __syncthreads();
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int ND, class GEO, int PM>
+__global__ __launch_bounds__(kWave) void k_pp_plateau(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                      const float* __restrict__ d, const uint8_t* __restrict__ t,
+                                                      float* __restrict__ y, const float* __restrict__ guide,
+                                                      uint32_t* __restrict__ claim, uint32_t* misc) {
+  constexpr int K = 2 * ND;
+  using SH = PlatShared<ND, PM>;
+  constexpr int H = SH::kHash;
+  __shared__ SH sh;
+  const int lane = threadIdx.x;
+  if (blockIdx.x >= nbad) return;
+  const uint32_t b = bad[blockIdx.x];
+  const FieldView F = geo.at(b);
+  const Grid& g = F.g;
+  {
+    int c[ND];
+    coords_of<ND>(g, b - F.base, c);
+    if (classify_at<ND>(g, y, b, c) == (int)t[b]) return;  // solved with another plateau
+  }
+  // ---- the plateau, breadth first (membership in an LDS hash table) ----------------
+  for (int i = lane; i < H; i += kWave) sh.hkey[i] = 0xffffffffu;
+  if (lane == 0) {
+    sh.ne = 1;
+    sh.abort = 0;
+    sh.el[0] = b;
+    sh.db = d[b];
+    cell_of(sh.db, F.eb, &sh.lo, &sh.hi);
+    const uint32_t h = plat_hash<H>(b);
+    sh.hkey[h] = b;
+    sh.hval[h] = 0;
+  }
+  __syncthreads();
+  const float db = sh.db;
+  int head = 0;
+  while (true) {
+    const int ne0 = sh.ne;
+    if (head >= ne0 || sh.abort) break;
+    for (int c0 = head * K; c0 < ne0 * K; c0 += kWave) {
+      const int ci = c0 + lane;
+      uint32_t q = 0xffffffffu;
+      if (ci < ne0 * K) {
+        q = plateau_nbr<ND>(g, F.base, d, sh.el[ci / K], ci % K, db);
+        if (q != 0xffffffffu && plat_find(sh, q) >= 0) q = 0xffffffffu;
+      }
+      sh.cand[lane] = q;
+      __syncthreads();
+      if (lane == 0) {
+        for (int j = 0; j < kWave && !sh.abort; j++) {
+          const uint32_t e = sh.cand[j];
+          if (e == 0xffffffffu || plat_find(sh, e) >= 0) continue;
+          if (sh.ne >= PM) {
+            sh.abort = 1;
+            break;
+          }
+          uint32_t h = plat_hash<H>(e);
+          while (sh.hkey[h] != 0xffffffffu) h = (h + 1) & (H - 1);
+          sh.hkey[h] = e;
+          sh.hval[h] = (uint16_t)sh.ne;
+          sh.el[sh.ne++] = e;
+        }
+      }
+      __syncthreads();
+    }
+    head = ne0;
+  }
+  if (sh.abort) {
+    if (lane == 0) atomicAdd(&misc[kPpPlatBig], 1u);
+    return;
+  }
+  const int ne = sh.ne;
+  // claimed by its smallest point: one wavefront per plateau
+  uint32_t mn = 0xffffffffu;
+  for (int i = lane; i < ne; i += kWave) mn = min(mn, sh.el[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, kWave));
+  if (lane == 0) sh.cand[0] = claim_bit(claim, mn) ? 1u : 0u;
+  __syncthreads();
+  if (!sh.cand[0]) return;
+  // ---- neighbour table, targets, anchors ---------------------------------------------
+  for (int i = lane; i < ne; i += kWave) {
+    const uint32_t e = sh.el[i];
+    int c[ND];
+    coords_of<ND>(g, e - F.base, c);
+    const bool in = interior<ND>(g, c);
+    const int tt = t[e];
+    sh.tgt[i] = in ? (int8_t)tt : (int8_t)-1;
+    sh.anc[i] = (tt == 1 || tt == 2) && db != 0.f;
+    sh.av[i] = y[e];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int a = k >> 1;
+      int q[ND];
+#pragma unroll
+      for (int x = 0; x < ND; x++) q[x] = c[x];
+      q[a] += (k & 1) ? 1 : -1;
+      int16_t code = -2;
+      if (inside<ND>(g, q)) {
+        const uint32_t s = stride_of<ND>(g, a);
+        const uint32_t qe = (k & 1) ? e + s : e - s;
+        const float dq = d[qe];
+        code = dq == db ? (int16_t)plat_find(sh, qe) : (dq < db ? (int16_t)-1 : (int16_t)-2);
+      }
+      sh.nb[i][k] = code;
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < ne; i += kWave) {
+    uint16_t lo = 0, up = 0;
+    if (sh.anc[i])
+      for (int j = 0; j < ne; j++) {
+        if (!sh.anc[j]) continue;
+        lo += sh.av[j] < sh.av[i];
+        up += sh.av[j] <= sh.av[i];
+      }
+    sh.alow[i] = lo;
+    sh.aupto[i] = up;
+  }
+  __syncthreads();
+  // Passes: the smoothed field and the current values as guides, then restarts on the
+  // smoothed field with deterministic noise (a backtracking search's run time is heavy
+  // tailed in its guide, so a few short searches beat one long one); first with the
+  // anchored extrema in place, then annealing with them in place (what large plateaus
+  // need: a backtracking search fails there on a few conflicts deep in its order), then
+  // (an extremum on a cell boundary can carry the sort position of the neighbouring bin)
+  // insertion passes with every point free.
+  float* gk = reinterpret_cast<float*>(sh.hkey);  // the hash table is no longer needed
+  const uint64_t t_solve = wall_clock64();
+  bool solved = false, annealed = false;
+  int pass = 0;
+  for (; pass < kPlatPasses && !solved; pass++) {
+    // a plateau beyond the small capacity goes to annealing after the two guided passes
+    if (pass == 2 && ne > kPlatSmall && pass < kPlatAnchored) pass = kPlatAnchored;
+    const int rp = pass < kPlatAnchored ? pass : pass - kPlatAnchored;  // pass within its group
+    if (pass == kPlatAnchored) {
+      uint32_t sweeps = 0;
+      const uint64_t ta = wall_clock64();
+      const bool ok = plat_anneal<ND>(sh, gk, ne, F, guide, db, lane, &sweeps);
+      if (lane == 0) {
+        atomicMax(&misc[kPpPlatMaxAnneal], (uint32_t)(wall_clock64() - ta));
+        atomicMax(&misc[kPpPlatMaxSweeps], sweeps);
+      }
+      if (ok) {
+        solved = annealed = true;
+        break;
+      }
+      for (int i = lane; i < ne; i += kWave) sh.anc[i] = 0;
+      __syncthreads();
+    }
+    // ---- guide ranks ------------------------------------------------------------------
+    for (int i = lane; i < ne; i += kWave) {
+      float gv = rp == 1 || guide == nullptr ? sh.av[i] : guide[sh.el[i]];
+      if (rp >= 2) {  // two hashed uniforms: about 0.3 eb of spread
+        uint64_t hsh = ((uint64_t)sh.el[i] << 8) ^ (uint64_t)(pass * 0x9e3779b9u);
+        hsh ^= hsh >> 33;
+        hsh *= 0xff51afd7ed558ccdull;
+        hsh ^= hsh >> 33;
+        hsh *= 0xc4ceb9fe1a85ec53ull;
+        hsh ^= hsh >> 33;
+        const float u0 = (float)(uint32_t)hsh * 2.3283064e-10f, u1 = (float)(uint32_t)(hsh >> 32) * 2.3283064e-10f;
+        gv += F.eb * 1.2f * (u0 + u1 - 1.f);
+      }
+      gk[i] = gv;
+    }
+    __syncthreads();
+    for (int i = lane; i < ne; i += kWave) {
+      const float gi = gk[i];
+      int r = 0;
+      for (int j = 0; j < ne; j++) r += (gk[j] < gi) || (gk[j] == gi && j < i);
+      sh.sorted[r] = (uint16_t)i;
+    }
+    for (int i = lane; i < PM / 32; i += kWave) sh.assigned[i] = 0u;
+    __syncthreads();
+    // ---- insertion search (uniform control flow over the wave) ------------------------
+    int depth = 0, pos = -1, nanc = 0;
+    uint32_t nodes = 0;
+    const uint32_t budget = rp < 2 && ne <= kPlatSmall ? 8u * (uint32_t)ne + 2048u : 4u * (uint32_t)ne + 1024u;
+    while (true) {
+      if (depth == ne) {
+        solved = true;
+        break;
+      }
+      if (++nodes > budget) break;
+      // tie pending: the last insertion was an anchor whose value other anchors share
+      int tie = -1;
+      if (depth > 0) {
+        const int last = sh.order[depth - 1];
+        if (sh.anc[last] && nanc < sh.aupto[last]) tie = sh.alow[last];
+      }
+      int found = -1, fpos = -1;
+      for (int p0 = pos + 1; p0 < ne && found < 0; p0 += kWave) {
+        const int p = p0 + lane;
+        bool ok = false;
+        int i = 0;
+        if (p < ne) {
+          i = sh.sorted[p];
+          ok = !plat_assigned(sh, i);
+          if (ok && tie >= 0) ok = sh.anc[i] && sh.alow[i] == tie;
+          if (ok && sh.anc[i]) ok = nanc >= sh.alow[i];
+          if (ok && sh.tgt[i] >= 0) ok = type_from_low<ND>(low_now<ND>(sh, i)) == sh.tgt[i];
+        }
+        const uint64_t m = __ballot(ok);
+        if (m) {
+          const int l = __ffsll((unsigned long long)m) - 1;
+          fpos = p0 + l;
+          found = sh.sorted[fpos];
+        }
+      }
+      if (found >= 0) {
+        if (lane == 0) sh.assigned[found >> 5] |= 1u << (found & 31);
+        __syncthreads();
+        bool alive = true;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int j = sh.nb[found][k];
+          if (j >= 0 && !plat_assigned(sh, j) && !plat_possible<ND>(sh, j, lane)) alive = false;
+        }
+        if (alive) {
+          if (lane == 0) {
+            sh.order[depth] = (uint16_t)found;
+            sh.choice[depth] = (uint16_t)fpos;
+          }
+          nanc += sh.anc[found] ? 1 : 0;
+          depth++;
+          pos = -1;
+        } else {
+          if (lane == 0) sh.assigned[found >> 5] &= ~(1u << (found & 31));
+          pos = fpos;
+        }
+        __syncthreads();
+      } else {
+        if (depth == 0) break;  // nothing insertable at all
+        depth--;
+        const int i = sh.order[depth];
+        if (lane == 0) sh.assigned[i >> 5] &= ~(1u << (i & 31));
+        nanc -= sh.anc[i] ? 1 : 0;
+        pos = sh.choice[depth];
+        __syncthreads();
+      }
+    }
+  }
+  if (lane == 0) atomicMax(&misc[kPpPlatMaxTicks], (uint32_t)(wall_clock64() - t_solve));
+  if (!solved) {
+    if (lane == 0) atomicAdd(&misc[kPpPlatFailed], 1u);
+    return;
+  }
+  // ---- values: anchors keep theirs, the points between take spaced values -----------
+  if (lane == 0) {
+    int s0 = 0;
+    float prev = sh.lo;
+    for (int r = 0; r <= ne; r++) {
+      const bool end = r == ne;
+      const int i = end ? -1 : sh.order[r];
+      if (end || sh.anc[i]) {
+        const float nxt = end ? sh.hi : sh.av[i];
+        const int cnt = r - s0;
+        for (int k = 0; k < cnt; k++) {
+          const int j = sh.order[s0 + k];
+          y[sh.el[j]] = prev + (nxt - prev) * ((float)(k + 1) / (float)(cnt + 1));
+        }
+        if (!end) prev = sh.av[i];
+        s0 = r + 1;
+      }
+    }
+    atomicAdd(&misc[kPpPlatSolved], 1u);
+    if (annealed) atomicAdd(&misc[kPpPlatAnneal], 1u);
+    else if (pass > kPlatAnchored) atomicAdd(&misc[kPpFreed], 1u);  // solved with the anchors freed
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
 static inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kBlockThreads - 1) / kBlockThreads); }
@@ -2396,6 +3015,11 @@ hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* m
 size_t pp_init_work_words(uint32_t n) {
   const uint32_t nt = pp_init_tiles(n) + 1;
   return nt + 1 + 2 * scan_status_words(nt) + 2;
+}
+
+hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, hipStream_t st) {
+  PP_GEO_LAUNCH(k_pp_blur, dim3(grid_stride_blocks(geo.n)), dim3(kBlockThreads), st, src, dst, geo.n, axis);
+  return hipGetLastError();
 }
 
 hipError_t pp_init(const PpGeo& geo, const float* d, const uint8_t* t, const int32_t* sp, uint32_t n_ext, float* y,
@@ -2571,6 +3195,35 @@ hipError_t pp_solve_flow(const PpGeo& geo, const uint32_t* tiles, uint32_t total
   if (!total) return hipSuccess;
   PP_GEO_LAUNCH(k_pp_solve_flow, dim3(total), dim3(kWave), st, tiles, cap, tbits, done, pbits, d, t, y, anchor, misc,
                 dyn_counter(misc + kPpFlowCounter));
+  return hipGetLastError();
+}
+
+#define PP_PLAT_LAUNCH(PMV)                                                                                   \
+  do {                                                                                                         \
+    if (geo.fields) {                                                                                          \
+      const GeoB gb_ = geob(geo);                                                                              \
+      if (geo.ndim == 2)                                                                                       \
+        hipLaunchKernelGGL((k_pp_plateau<2, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y, \
+                           guide, claim, misc);                                                                \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_pp_plateau<3, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y, \
+                           guide, claim, misc);                                                                \
+    } else {                                                                                                   \
+      const Geo1 g1_ = geo1(geo);                                                                              \
+      if (geo.ndim == 2)                                                                                       \
+        hipLaunchKernelGGL((k_pp_plateau<2, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y, \
+                           guide, claim, misc);                                                                \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_pp_plateau<3, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y, \
+                           guide, claim, misc);                                                                \
+    }                                                                                                          \
+  } while (0)
+
+hipError_t pp_plateau(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
+                      const float* guide, uint32_t* claim, bool large, uint32_t* misc, hipStream_t st) {
+  if (!nbad) return hipSuccess;
+  if (large) PP_PLAT_LAUNCH(kPlatLarge);
+  else PP_PLAT_LAUNCH(kPlatSmall);
   return hipGetLastError();
 }
 
