@@ -1745,6 +1745,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
       fr.add(misc + kPpFreed, 0u, 1);
       fr.add(misc + kPpTilesP, 0u, 8);
       fr.add(misc + kPpStatNodes, 0u, 12);
+      fr.add(misc + kPpStatSumBuild, 0u, 3);
       HIPCHK(launch_fill_multi(fr, st));
     }
     HIPCHK(pp_tiles(geo, bad, nl, bits, tbits, tiles, tcap, misc, st));
@@ -1779,6 +1780,12 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
               h[kPpStatMaxWin], h[kPpStatMaxTicks] * 0.01, h[kPpStatLeaves],
               h[kPpStatLeaves + 1], h[kPpStatLeaves + 2] * 0.01, h[kPpStatLeaves + 3] * 0.01,
               h[kPpStatLeaves + 4] * 0.01, nb);
+    if (stats) {
+      const double nwin = (double)(h[kPpSolved] + h[kPpFailed]);
+      fprintf(stderr, "[restore]   mean per window: build us %.2f attempts us %.2f; mean tile us %.2f\n",
+              nwin ? h[kPpStatSumBuild] * 0.01 / nwin : 0.0, nwin ? h[kPpStatSumAttempt] * 0.01 / nwin : 0.0,
+              total ? h[kPpStatSumTile] * 0.01 / total : 0.0);
+    }
     solved += h[kPpSolved];
     freed += h[kPpFreed];
     // another outer round only while many mismatches are left and the windows still

@@ -402,6 +402,7 @@ enum : int {
   kPpStatNodes = 40, kPpStatMaxNodes = 41, kPpStatMaxWin = 44, kPpStatMaxTicks = 45, kPpStatMaxWait = 46, kPpStatLeaves = 47, kPpTiny = 42,
   kPpSolveCounter = 43,
   kPpPlatMaxTicks = 52, kPpPlatMaxAnneal = 53, kPpPlatMaxSweeps = 54,  // whole-plateau solve statistics
+  kPpStatSumBuild = 55, kPpStatSumAttempt = 56, kPpStatSumTile = 57,    // window solve: summed ticks
   kPpFlowCounter = 60,  // tile counter of the one-launch solve
   kPpPlatSolved = 61, kPpPlatFailed = 62, kPpPlatBig = 63  // whole-plateau solves
 };

@@ -1756,13 +1756,6 @@ struct Window {
   float lo, hi, db;
 };
 
-template <int ND>
-__device__ __forceinline__ int find_el(const Window<ND>& w, uint32_t e) {
-  for (int i = 0; i < w.ne; i++)
-    if (w.el[i] == e) return i;
-  return -1;
-}
-
 // neighbour k of element point e, if it lies in the plateau of value db
 template <int ND>
 __device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, uint32_t base, const float* __restrict__ d, uint32_t e,
@@ -1777,55 +1770,70 @@ __device__ __forceinline__ uint32_t plateau_nbr(const Grid& g, uint32_t base, co
   return d[qe] == db ? qe : 0xffffffffu;
 }
 
+// Appends the candidates q of lanes [0, ncand) that are neither elements of the window
+// (lane i holds element i in el_reg, ne elements) nor repeats of a lower lane's
+// candidate, in lane order, up to cap elements; returns the new element count.  The
+// same list as appending one candidate at a time with a linear search.
+template <int ND>
+__device__ __forceinline__ int window_append(Window<ND>& w, uint32_t& el_reg, int ne, uint32_t q, int ncand, int cap,
+                                             int lane) {
+  bool fresh = lane < ncand && q != 0xffffffffu;
+  for (int i = 0; i < ne; i++) fresh = fresh && (uint32_t)rdl((int)el_reg, i) != q;
+  for (int j = 0; j < ncand; j++) {
+    const uint32_t v = (uint32_t)rdl((int)q, j);
+    fresh = fresh && !(j < lane && v == q);
+  }
+  const uint64_t m = __ballot(fresh);
+  const int room = cap - ne;
+  const int rank = __popcll(m & lanemask_lt(lane));
+  if (fresh && rank < room) w.el[ne + rank] = q;
+  __syncthreads();
+  const int add = min(__popcll(m), room);
+  if (lane >= ne && lane < ne + add) el_reg = w.el[lane];
+  return ne + add;
+}
+
 // Builds the window of mismatch b (whole wavefront: the lanes load neighbours in
-// parallel, lane 0 keeps the lists in order).
+// parallel; lane i keeps element i in a register, so membership is a wave-wide
+// compare instead of a list search).
 template <int ND, bool COH>
 __device__ void build_window(Window<ND>& w, const Grid& g, uint32_t base, const float* __restrict__ d,
                              const uint8_t* __restrict__ t, const float* y, float eb, uint32_t b,
                              bool anchor, int lane) {
   constexpr int K = 2 * ND;
+  static_assert(Window<ND>::EM <= kWave, "one window element per lane");
+  const float db = d[b];
+  uint32_t el_reg = lane == 0 ? b : 0xffffffffu;
   if (lane == 0) {
-    w.ne = 1;
     w.el[0] = b;
     w.base = base;
-    w.db = d[b];
-    cell_of(w.db, eb, &w.lo, &w.hi);
+    w.db = db;
+    cell_of(db, eb, &w.lo, &w.hi);
   }
   __syncthreads();
-  const float db = w.db;
   // breadth-first levels over the plateau, up to kWinMax points
-  int head = 0;
-  while (true) {
-    const int ne0 = w.ne;
-    if (head >= ne0 || ne0 >= kWinMax) break;
+  int ne = 1, head = 0;
+  while (head < ne && ne < kWinMax) {
+    const int ne0 = ne;
     const int ncand = (ne0 - head) * K;
-    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[head + lane / K], lane % K, db) : 0xffffffffu;
-    __syncthreads();
-    if (lane == 0)
-      for (int j = 0; j < ncand && w.ne < kWinMax; j++) {
-        const uint32_t q = w.cand[j];
-        if (q != 0xffffffffu && find_el<ND>(w, q) < 0) w.el[w.ne++] = q;
-      }
-    __syncthreads();
+    const uint32_t q =
+        lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[head + lane / K], lane % K, db) : 0xffffffffu;
+    ne = window_append<ND>(w, el_reg, ne, q, ncand, kWinMax, lane);
     head = ne0;
   }
-  if (lane == 0) w.nw = w.ne;
-  __syncthreads();
+  const int nw = ne;
   // the plateau points next to the window keep their values
   {
-    const int ncand = w.nw * K;
-    w.cand[lane] = lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[lane / K], lane % K, db) : 0xffffffffu;
-    __syncthreads();
-    if (lane == 0)
-      for (int j = 0; j < ncand; j++) {
-        const uint32_t q = w.cand[j];
-        if (q != 0xffffffffu && find_el<ND>(w, q) < 0 && w.ne < Window<ND>::EM) w.el[w.ne++] = q;
-      }
-    __syncthreads();
+    const int ncand = nw * K;
+    const uint32_t q = lane < ncand ? plateau_nbr<ND>(g, base, d, w.el[lane / K], lane % K, db) : 0xffffffffu;
+    ne = window_append<ND>(w, el_reg, ne, q, ncand, Window<ND>::EM, lane);
   }
-  const int ne = w.ne;
+  if (lane == 0) {
+    w.nw = nw;
+    w.ne = ne;
+  }
   if (lane < ne) {
-    const uint32_t e = w.el[lane];
+    const uint32_t e = el_reg;
     w.val[lane] = y_ld<COH>(y, e);
     const int tt = t[e];
     w.anch[lane] = anchor && anchored(tt, db);
@@ -1834,21 +1842,46 @@ __device__ void build_window(Window<ND>& w, const Grid& g, uint32_t base, const 
     w.typ[lane] = interior<ND>(g, c) ? (int8_t)tt : (int8_t)-1;
   }
   __syncthreads();
-  for (int j = lane; j < ne * K; j += kWave) {
-    const int i = j / K, k = j % K;
-    if (w.typ[i] < 0) continue;
+  // neighbour codes: (element, direction) pairs j = lane + kWave * it; every load of
+  // every pair is issued before the first is used (one memory round trip, not one per
+  // pair and load)
+  constexpr int kIt = (Window<ND>::EM * K + kWave - 1) / kWave;
+  int jj_r[kIt];
+  uint32_t qe_r[kIt];
+  float dq_r[kIt], yq_r[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int j = it * kWave + lane;
+    const bool on = j < ne * K;
+    const int i = on ? j / K : 0, k = on ? j % K : 0;
+    const bool con = on && w.typ[i] >= 0;
     const uint32_t e = w.el[i];
     const uint32_t s = stride_of<ND>(g, k >> 1);
-    const uint32_t qe = (k & 1) ? e + s : e - s;
-    const int jj = find_el<ND>(w, qe);
-    if (jj >= 0) {
-      w.cnb[i][k] = (int8_t)jj;
-    } else {
+    qe_r[it] = (k & 1) ? e + s : e - s;
+    int jj = -1;
+    for (int x = 0; x < ne; x++)
+      if ((uint32_t)rdl((int)el_reg, x) == qe_r[it] && jj < 0) jj = x;
+    jj_r[it] = con ? jj : -2;  // -2: no code to write
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const bool ld = jj_r[it] == -1;
+    dq_r[it] = ld ? d[qe_r[it]] : 0.f;
+    yq_r[it] = ld ? y_ld<COH>(y, qe_r[it]) : 0.f;
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int j = it * kWave + lane;
+    const int i = j / K, k = j % K;
+    if (jj_r[it] >= 0) {
+      w.cnb[i][k] = (int8_t)jj_r[it];
+    } else if (jj_r[it] == -1) {
       // not an element: another cell (the order of the cells) or a point that keeps its
       // value next to an element that keeps its value
-      const float dq = d[qe];
-      const float r = dq != db ? db - dq : w.val[i] - y_ld<COH>(y, qe);
-      w.cnb[i][k] = r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
+      const float dq = dq_r[it];
+      const float r = dq != db ? db - dq : w.val[i] - yq_r[it];
+      w.cnb[i][k] =
+          r > 0.f ? (int8_t)Window<ND>::kGt : (r < 0.f ? (int8_t)Window<ND>::kLt : (int8_t)Window<ND>::kEq);
     }
   }
   __syncthreads();
@@ -2077,7 +2110,9 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
     if (!leaf) break;
     // write the free points and check every constraint on the field
     if (my_free) {
-      my_old = y_ld<COH>(y, w.el[lane]);
+      // the window's points hold their values of the build: no other wavefront writes
+      // them, and every failed attempt puts them back
+      my_old = w.val[lane];
       y_st<COH>(y, w.el[lane], my_fval);
     }
     if constexpr (COH) __builtin_amdgcn_s_waitcnt(0);  // the stores land before other lanes read
@@ -2181,6 +2216,7 @@ struct SolveShared {
   uint32_t blist[ND == 2 ? kTileSide * kTileSide : kTileSide * kTileSide * kTileSide];
   uint32_t nb;
   uint32_t tile;
+  uint32_t wait[ND == 2 ? 9 : 27];  // the one-launch solve: neighbouring tiles to wait for
 };
 
 // One wavefront solves the listed mismatches of tile `id` (of field F, numbered within
@@ -2275,6 +2311,9 @@ __device__ void solve_tile(SolveShared<ND>& sh, const FieldView& F, uint32_t id,
     atomicMax(&misc[kPpStatLeaves + 2], tst[4]);
     atomicMax(&misc[kPpStatLeaves + 3], tst[5]);
     atomicMax(&misc[kPpStatLeaves + 4], tst[6]);
+    atomicAdd(&misc[kPpStatSumBuild], tst[4]);
+    atomicAdd(&misc[kPpStatSumAttempt], tst[5]);
+    atomicAdd(&misc[kPpStatSumTile], (uint32_t)(wall_clock64() - t_work));
   }
 }
 
@@ -2326,9 +2365,12 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(GEO geo, const uint32_t
   const uint32_t tx = F.tx, ty = F.ty, tz = F.tz, lid = id - F.tile0;
   int tc[ND];
   tile_coords<ND>(lid, ty, tz, tc);
-  // lane k waits for neighbour k of the 3^ND block (k != centre), if listed and of a
-  // lower phase
+  // lane k finds neighbour k of the 3^ND block (k != centre), if listed and of a lower
+  // phase; lane 0 then waits for those one at a time (one poll in flight per waiting
+  // tile: thousands of waiting tiles polling every neighbour flag at once flood the
+  // memory system that the working tiles need)
   constexpr int kNb = ND == 2 ? 9 : 27;
+  uint32_t nid = 0xffffffffu;
   if (lane < kNb && lane != kNb / 2) {
     int nc[ND];
     int k = lane;
@@ -2341,19 +2383,28 @@ __global__ __launch_bounds__(kWave) void k_pp_solve_flow(GEO geo, const uint32_t
       in = in && nc[a] >= 0 && (uint32_t)nc[a] < lim;
     }
     if (in && (uint32_t)tile_phase<ND>(nc) < p) {
-      uint32_t nid = (uint32_t)nc[0];
+      uint32_t q = (uint32_t)nc[0];
 #pragma unroll
-      for (int a = 1; a < ND; a++) nid = nid * (a == 1 ? ty : tz) + (uint32_t)nc[a];
-      nid += F.tile0;
-      if ((tbits[nid >> 5] >> (nid & 31u)) & 1u) {
-        uint32_t spins = 0;
-        while (__hip_atomic_load(&done[nid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-          if (++spins > kSpinLimit) {
-            atomicOr(&misc[kPpErr], TSZP_ERR_LOOKBACK);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
+      for (int a = 1; a < ND; a++) q = q * (a == 1 ? ty : tz) + (uint32_t)nc[a];
+      q += F.tile0;
+      if ((tbits[q >> 5] >> (q & 31u)) & 1u) nid = q;
+    }
+  }
+  const uint64_t wm = __ballot(nid != 0xffffffffu);
+  if (nid != 0xffffffffu) sh.wait[__popcll(wm & lanemask_lt(lane))] = nid;
+  __syncthreads();
+  if (lane == 0) {
+    const int nw = __popcll(wm);
+    for (int q = 0; q < nw; q++) {
+      const uint32_t id_q = sh.wait[q];
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&done[id_q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        if (++spins > kSpinLimit) {
+          atomicOr(&misc[kPpErr], TSZP_ERR_LOOKBACK);
+          break;
         }
+        if (spins < 8) __builtin_amdgcn_s_sleep(2);
+        else __builtin_amdgcn_s_sleep(64);  // about 4000 cycles
       }
     }
   }
