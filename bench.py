@@ -546,6 +546,7 @@ def main():
     ap.add_argument("--no-elem", action="store_true", help="leave out the threadblock / double side figures")
     ap.add_argument("--no-batch", action="store_true", help="leave out the mixed-size batch side figures")
     ap.add_argument("--no-serve", action="store_true", help="leave out the served-stream side figure")
+    ap.add_argument("--no-other-bs", action="store_true", help="leave out the step at the other block size")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-rate", type=float, default=40000.0, help="Poisson arrival rate per second")
     ap.add_argument("--profile-pass", action="store_true",
@@ -606,6 +607,7 @@ def main():
     elapsed = time.perf_counter() - t0
     tz.prof_enable(False)
     prof = tz.prof_read()
+    prof_restore = last.get("restore")  # with the solve's tile chain (computed while profiling)
     # a second, unprofiled timing of the same steps (event records add a little work)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -624,6 +626,10 @@ def main():
     # Single kernels timed per launch by the library, with their algorithmic bytes per
     # launch (the minimum HBM traffic of what each launch must read and write).
     ndim = len(shape)
+    # The kernels of the step that the library times one by one, with their algorithmic
+    # bytes per launch (the others, each under 0.35 ms per step at 512^3, are in the
+    # kernel-trace summary); None: no byte model (the window solve is a chain of
+    # dependent tiles, bound by latency).
     kernels = {
         # read the field, write the topology payload (the 8 B/CP sort keys it also emits
         # are an intermediate of this design and not counted)
@@ -634,16 +640,32 @@ def main():
         "query": ("k_bin_query", 8 * info.cp_count + 4 * info.ext_count),
         # read the sort positions, write their stream
         "codec_int": (codec_kernel(3, bs, 2, 4), 4 * info.ext_count + sp_payload),
+        # the exact window solve of the restoration
+        "solve": (f"k_pp_solve_flow<{ndim}, tszp::Geo1>", None),
     }
+
+    def hbm_roofline(key):
+        per_ms = prof[key][0] / max(1, prof[key][1])
+        nbytes = kernels[key][1]
+        ach = nbytes / (per_ms * 1e-3) / 1e9 if per_ms > 0 else None
+        return {"bound": "hbm", "kernel": kernels[key][0], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": pmc_traffic(kernels[key][0]),
+                "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": per_ms,
+                "launches_per_step": prof[key][1] / args.steps}
     dom = max(kernels, key=lambda k: stages.get(k, (0, 0))[0])
-    per_launch_ms = prof[dom][0] / max(1, prof[dom][1])
-    per_launch_bytes = kernels[dom][1]
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else None
-    traffic = pmc_traffic(kernels[dom][0])
-    roofline = {"bound": "hbm", "kernel": kernels[dom][0], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms,
-                "launches_per_step": prof[dom][1] / args.steps}
+    hbm_keys = [k for k in kernels if kernels[k][1] is not None]
+    dom_hbm = max(hbm_keys, key=lambda k: stages.get(k, (0, 0))[0])
+    if kernels[dom][1] is None:
+        r = prof_restore
+        per_ms = prof[dom][0] / max(1, prof[dom][1])
+        chain = r.solve_chain if r is not None else 0
+        roofline = {"bound": "latency", "kernel": kernels[dom][0], "achieved": None, "peak": None, "unit": None,
+                    "frac": None, "traffic": None, "avg_launch_ms": per_ms, "launches_per_step": prof[dom][1] / args.steps,
+                    "tiles": r.solve_tiles if r is not None else None, "dependent_chain_tiles": chain,
+                    "ms_per_chain_tile": per_ms / chain if chain else None,
+                    "hbm_kernel": hbm_roofline(dom_hbm)}
+    else:
+        roofline = hbm_roofline(dom)
     e2e_bytes = (4 * n + info.topo_bytes + info.sortpos_bytes) + (info.topo_bytes + info.sortpos_bytes + 5 * n
                                                                    + 4 * info.ext_count)
     e2e_frac = e2e_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS
@@ -663,7 +685,7 @@ def main():
     # side figure: the same pipeline step at the workload's other block size
     other_bs = {}
     for obs in (64, 128):
-        if obs == bs:
+        if obs == bs or args.no_other_bs:
             continue
         obuf = tz.TopoBuffers(n, obs, 1, dev)
 

@@ -251,6 +251,9 @@ typedef struct {
                                  sort position belongs to the neighbouring bin) */
   uint32_t rounds;         /* local-search rounds */
   float max_dev_over_eb;   /* max |y - d| / eb */
+  uint32_t solve_tiles;    /* tiles of the last exact window solve */
+  uint32_t solve_chain;    /* its longest chain of dependent tiles (one field, set while
+                              the stage timers run: tszp_prof_enable) */
 } tszp_restore_info;
 
 int tszp_topo_restore(const float *d_decoded, float *d_field, const uint8_t *d_types, const int32_t *d_sp,

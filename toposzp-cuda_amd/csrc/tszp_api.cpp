@@ -1584,7 +1584,47 @@ struct RestoreTotals {
   uint64_t verify[4];  // missing, spurious, changed, max |y - d| bits (float)
   uint64_t moves, windows, freed;
   uint32_t rounds;
+  uint32_t solve_tiles, solve_chain;  // last window solve: tiles, longest dependent chain
 };
+
+// Longest chain of dependent tiles of a one-launch window solve over one field: tiles in
+// phase order, each waiting for its listed neighbours (3^ndim block) of lower phases.
+// ids: the listed tiles, phase by phase.
+uint32_t solve_chain_length(const std::vector<uint32_t>& ids, int ndim, uint32_t tx, uint32_t ty, uint32_t tz) {
+  const size_t nt = (size_t)tx * ty * (ndim == 3 ? tz : 1);
+  std::vector<uint16_t> len(nt, 0);
+  uint32_t best = 0;
+  for (uint32_t id : ids) {
+    int tc[3] = {0, 0, 0};
+    uint32_t r = id;
+    if (ndim == 3) {
+      tc[2] = (int)(r % tz);
+      r /= tz;
+    }
+    tc[1] = (int)(r % ty);
+    tc[0] = (int)(r / ty);
+    int ph = 0;
+    for (int a = 0; a < ndim; a++) ph |= (tc[a] & 1) << a;
+    uint32_t m = 0;
+    const int kz = ndim == 3 ? 1 : 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -kz; dz <= kz; dz++) {
+          const int q[3] = {tc[0] + dx, tc[1] + dy, tc[2] + dz};
+          if (q[0] < 0 || q[1] < 0 || q[2] < 0 || (uint32_t)q[0] >= tx || (uint32_t)q[1] >= ty ||
+              (ndim == 3 && (uint32_t)q[2] >= tz))
+            continue;
+          int qph = 0;
+          for (int a = 0; a < ndim; a++) qph |= (q[a] & 1) << a;
+          if (qph >= ph) continue;
+          const size_t qid = ndim == 3 ? ((size_t)q[0] * ty + q[1]) * tz + q[2] : (size_t)q[0] * ty + q[1];
+          m = std::max<uint32_t>(m, len[qid]);
+        }
+    len[id] = (uint16_t)(m + 1);
+    best = std::max(best, m + 1);
+  }
+  return best;
+}
 
 // The restoration's rounds over one field or a batch of fields in flat arrays (geo):
 // start values, small plateaus, local-search rounds re-checked around the previous
@@ -1752,6 +1792,19 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     if ((r = read_misc())) return r;
     uint32_t total = 0;
     for (int ph = 0; ph < phases; ph++) total += h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+    out->solve_tiles = total;
+    if ((prof_on() || stats) && !geo.fields && total) {  // the solve's dependent-tile chain, for the profile
+      std::vector<uint32_t> ids(total);
+      size_t off = 0;
+      for (int ph = 0; ph < phases; ph++) {
+        const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
+        if (cnt) HIPCHK(hipMemcpyAsync(ids.data() + off, tiles + (size_t)ph * tcap, cnt * 4, hipMemcpyDeviceToHost, st));
+        off += cnt;
+      }
+      HIPCHK(hipStreamSynchronize(st));
+      out->solve_chain = solve_chain_length(ids, ndim, geo.tx, geo.ty, geo.tz);
+      if (stats) fprintf(stderr, "[restore] solve tiles %u, longest dependent chain %u tiles\n", total, out->solve_chain);
+    }
     EventTimer solve_timer(kProfSolve);
     if (getenv("TSZP_SOLVE_PHASES")) {  // one launch per parity phase
       for (int ph = 0; ph < phases; ph++) {
@@ -1986,6 +2039,8 @@ int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_t
   info->windows = tot.windows;
   info->windows_reordered = tot.freed;
   info->rounds = tot.rounds;
+  info->solve_tiles = tot.solve_tiles;
+  info->solve_chain = tot.solve_chain;
   const uint32_t dv = (uint32_t)tot.verify[3];
   float dev;
   memcpy(&dev, &dv, 4);

@@ -33,7 +33,7 @@ class TopoInfo(C.Structure):
 class RestoreInfo(C.Structure):
     _fields_ = [("missing", C.c_uint64), ("spurious", C.c_uint64), ("changed", C.c_uint64),
                 ("moves", C.c_uint64), ("windows", C.c_uint64), ("windows_reordered", C.c_uint64), ("rounds", C.c_uint32),
-                ("max_dev_over_eb", C.c_float)]
+                ("max_dev_over_eb", C.c_float), ("solve_tiles", C.c_uint32), ("solve_chain", C.c_uint32)]
 
     def mismatches(self):
         return self.missing + self.spurious + self.changed
