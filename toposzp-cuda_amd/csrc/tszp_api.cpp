@@ -1719,10 +1719,64 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     exact = nb <= cap;  // only the plateau points changed, all of them listed or fixed
     verified = false;
   }
+  // whole plateaus by min-conflicts (tszp_platmc.hip): a short pass over every plateau
+  // left after the small ones, then the local rounds and window solves below for what it
+  // leaves, then a long pass, guided by the smoothed field, for what they leave
+  uint64_t mc_solved = 0;
+  const bool mc_on = !getenv("TSZP_RESTORE_NO_MC");
+  auto env_u32 = [](const char* name, uint32_t dflt) {
+    const char* v = getenv(name);
+    return v ? (uint32_t)atoi(v) : dflt;
+  };
+  const float* blur = nullptr;
+  auto mc_pass = [&](uint32_t budget, bool guided, const char* tag) -> int {
+    if (!exact && (r = full_mark(&nb, false))) return r;
+    const uint32_t nl = nb < cap ? nb : cap;
+    if (guided && !blur) {  // the smoothed decoded field as the start order
+      float *s0, *s1;
+      WS(kWsPpSm0, n * 4 + 16, s0);
+      WS(kWsPpSm1, n * 4 + 16, s1);
+      const float* src = d;
+      for (int a = 0; a < ndim; a++) {
+        float* dst = a % 2 ? s1 : s0;
+        HIPCHK(pp_blur(geo, src, dst, a, st));
+        src = dst;
+      }
+      blur = src;
+    }
+    const float* guide = guided ? blur : nullptr;
+    FillRanges fr{};
+    fr.add(bits, 0u, bit_words);
+    fr.add(misc + kPpMcSolved, 0u, 5);
+    HIPCHK(launch_fill_multi(fr, st));
+    HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, false, budget, 1, misc, st));
+    int rr = read_misc();
+    if (rr) return rr;
+    const uint32_t solved_small = h[kPpMcSolved];
+    if (h[kPpMcBig]) {  // plateaus beyond the small variant: the large one over the same list
+      FillRanges fb{};
+      fb.add(bits, 0u, bit_words);
+      HIPCHK(launch_fill_multi(fb, st));
+      HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, true, budget, 1, misc, st));
+    }
+    const uint32_t before = nb;
+    if ((rr = full_mark(&nb, true))) return rr;
+    mc_solved += h[kPpMcSolved];
+    if (stats)
+      fprintf(stderr, "[restore] min-conflicts (%s, %u steps) plateaus solved %u (%u small) failed %u beyond the small "
+              "variant %u: %u -> %u mismatches; max steps %u, max us %.1f\n", tag, budget, h[kPpMcSolved],
+              solved_small, h[kPpMcFailed], h[kPpMcBig], before, nb, h[kPpMcMaxSteps], h[kPpMcMaxTicks] * 0.01);
+    return TSZP_OK;
+  };
+  if (nb && mc_on && (r = mc_pass(env_u32("TSZP_MC_BUDGET1", 192u), false, "short"))) return r;
+  // window solves: anchors (bit 0) and, behind the plateau solves, only windows that hold
+  // their whole plateau (bit 1)
   const int colours = ndim == 2 ? 5 : 7;
   uint64_t moves = 0, solved = 0, freed = 0;
   uint32_t rounds = 0;
-  for (int outer = 0; outer < 4 && nb; outer++) {
+  const bool stop_mc = getenv("TSZP_RESTORE_STOP_MC") != nullptr;  // diagnostics: nothing after the plateau solves
+  auto window_rounds = [&](int win_mode) -> int {
+  for (int outer = 0; outer < 4 && nb && !stop_mc; outer++) {
     // local search, re-checking only around the previous mismatches
     bool complete = nb <= cap;
     for (int round = 0; round < max_rounds && nb; round++) {
@@ -1811,7 +1865,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
         const uint32_t cnt = h[kPpTilesP + ph] < tcap ? h[kPpTilesP + ph] : tcap;
         if (!cnt) continue;
         solve_timer.begin(st);
-        HIPCHK(pp_solve(geo, tiles, cnt, tcap, ph, bits, d, d_types, d_field, use_sp, misc, st));
+        HIPCHK(pp_solve(geo, tiles, cnt, tcap, ph, bits, d, d_types, d_field, win_mode, misc, st));
         solve_timer.end(st);
       }
     } else if (total) {  // every phase in one launch, ordered per neighbouring tiles
@@ -1822,7 +1876,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
       fr.add(misc + kPpFlowCounter, 0u, 1);
       HIPCHK(launch_fill_multi(fr, st));
       solve_timer.begin(st);
-      HIPCHK(pp_solve_flow(geo, tiles, total, tcap, tbits, done, bits, d, d_types, d_field, use_sp, misc, st));
+      HIPCHK(pp_solve_flow(geo, tiles, total, tcap, tbits, done, bits, d, d_types, d_field, win_mode, misc, st));
       solve_timer.end(st);
     }
     if ((r = full_mark(&nb, true))) return r;
@@ -1846,9 +1900,20 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     // whole-plateau stage below
     if (nb <= kPpSolveFew || (uint64_t)nb * 4 > (uint64_t)nl * 3) break;
   }
+  return TSZP_OK;
+  };
+  // behind the plateau solves, windows only order plateaus they hold whole (bit 1 of the
+  // window mode; bit 0: anchored extrema)
+  if ((r = window_rounds((use_sp ? 1 : 0) | (mc_on ? 2 : 0)))) return r;
   if (!verified && (r = full_mark(&nb, true))) return r;
-  // what the windows could not order: whole plateaus, guided by the smoothed field first
-  if (nb) {
+  if (nb && mc_on && !stop_mc && (r = mc_pass(env_u32("TSZP_MC_BUDGET2", 30000u), true, "long"))) return r;
+  // plateaus beyond the plateau solves' capacity: windows over their parts
+  if (nb && mc_on && !stop_mc) {
+    if ((r = window_rounds(use_sp ? 1 : 0))) return r;
+    if (!verified && (r = full_mark(&nb, true))) return r;
+  }
+  // what is still left: whole plateaus by insertion searches and annealing
+  if (nb && !stop_mc) {
     if (!exact && (r = full_mark(&nb, true))) return r;
     const uint32_t nl = nb < cap ? nb : cap;
     float *s0, *s1;
@@ -1888,7 +1953,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);
   for (int k = 0; k < 4; k++) out->verify[k] = hv[k];
   out->moves = moves;
-  out->windows = solved + tiny;
+  out->windows = solved + tiny + mc_solved;
   out->freed = freed;
   out->rounds = rounds;
   return TSZP_OK;

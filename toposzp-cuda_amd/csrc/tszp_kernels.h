@@ -398,6 +398,7 @@ enum : int {
   kPpBad = 0, kPpMoves = 1, kPpAct = 2, kPpTiles = 3, kPpVerify = 4, kPpErr = 12, kPpExt = 13, kPpSpMax = 14,
   kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18, kPpPlatAnneal = 19,
   kPpActC = 20,      // 7 colour-list counts
+  kPpMcSolved = 27, kPpMcFailed = 28, kPpMcBig = 29, kPpMcMaxTicks = 30, kPpMcMaxSteps = 31,  // min-conflicts plateau solves
   kPpTilesP = 32,    // 8 parity-phase tile-list counts
   kPpStatNodes = 40, kPpStatMaxNodes = 41, kPpStatMaxWin = 44, kPpStatMaxTicks = 45, kPpStatMaxWait = 46, kPpStatLeaves = 47, kPpTiny = 42,
   kPpSolveCounter = 43,
@@ -448,6 +449,17 @@ hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32
 // (large: the variant for plateaus of up to 3072 points, 150 KB of LDS per wavefront)
 hipError_t pp_plateau(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
                       const float* guide, uint32_t* claim, bool large, uint32_t* misc, hipStream_t st);
+// whole-plateau min-conflicts solves around the listed mismatches (tszp_platmc.hip), one
+// wavefront each: guide (nullable) the start order, else breadth-first distances inside
+// the plateau; claim: zeroed bits, one per point; large: plateaus of up to 3072 points
+// (150 KB of LDS per wavefront) instead of 512; budget: steps per plateau; plateaus of
+// fewer than min_size points are left alone
+hipError_t pp_platmc(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
+                     const float* guide, uint32_t* claim, bool large, uint32_t budget, uint32_t min_size,
+                     uint32_t* misc, hipStream_t st);
+// window solves (one launch per parity phase, or every phase in one launch below);
+// anchor: bit 0 keeps the anchored extrema in place, bit 1 solves only windows that hold
+// their whole plateau (the larger ones are left to the plateau solves)
 hipError_t pp_solve(const PpGeo& geo, const uint32_t* tiles, uint32_t ntiles, uint32_t cap, int phase,
                     const uint32_t* pbits, const float* d, const uint8_t* t, float* y, int anchor, uint32_t* misc,
                     hipStream_t st);

@@ -30,12 +30,17 @@
 //     becomes values spaced evenly between the fixed ones.  Windows of one 16-wide tile
 //     run in order in one thread, tiles of one parity class in parallel, so the result
 //     does not depend on scheduling.
+// Between the small plateaus and the local rounds, whole plateaus are re-ordered by a
+// min-conflicts search (tszp_platmc.hip); behind it the window solve only takes windows
+// that hold their whole plateau, and a longer plateau search, then windows over parts of
+// plateaus beyond its capacity, handle what is left.
 // The full marking pass also yields the verification counts (missing / spurious /
 // changed critical points) and the largest |y - d|.
 #include <cstdlib>
 
 #include "tszp_device.hpp"
 #include "tszp_kernels.h"
+#include "tszp_post_geo.hpp"
 
 namespace tszp {
 
@@ -74,198 +79,6 @@ __device__ __forceinline__ XcdSweep xcd_sweep(uint32_t n, uint32_t chunk) {
   return r;
 }
 
-// ---------------------------------------------------------------------------------
-// geometry
-// ---------------------------------------------------------------------------------
-template <int ND>
-__device__ __forceinline__ void coords_of(const Grid& g, uint32_t e, int (&c)[ND]) {
-  if constexpr (ND == 2) {
-    const uint32_t x = fdiv(e, g.div_row);
-    c[0] = (int)x;
-    c[1] = (int)(e - x * g.d1);
-  } else {
-    const uint32_t x = fdiv(e, g.div_plane), r = e - x * g.plane;
-    const uint32_t y = fdiv(r, g.div_row);
-    c[0] = (int)x;
-    c[1] = (int)y;
-    c[2] = (int)(r - y * g.d2);
-  }
-}
-
-template <int ND>
-__device__ __forceinline__ int dim_of(const Grid& g, int a) {
-  return a == 0 ? (int)g.d0 : (a == 1 ? (int)g.d1 : (int)g.d2);
-}
-
-template <int ND>
-__device__ __forceinline__ uint32_t stride_of(const Grid& g, int a) {
-  if constexpr (ND == 2) return a == 0 ? g.d1 : 1u;
-  else return a == 0 ? g.plane : (a == 1 ? g.d2 : 1u);
-}
-
-template <int ND>
-__device__ __forceinline__ bool interior(const Grid& g, const int (&c)[ND]) {
-  bool r = true;
-#pragma unroll
-  for (int a = 0; a < ND; a++) r = r && c[a] >= 1 && c[a] + 2 <= dim_of<ND>(g, a);
-  return r;
-}
-
-template <int ND>
-__device__ __forceinline__ bool inside(const Grid& g, const int (&c)[ND]) {
-  bool r = true;
-#pragma unroll
-  for (int a = 0; a < ND; a++) r = r && c[a] >= 0 && c[a] < dim_of<ND>(g, a);
-  return r;
-}
-
-template <int ND>
-__device__ __forceinline__ int colour_of(const int (&c)[ND]) {
-  if constexpr (ND == 2) return (c[0] + 2 * c[1]) % 5;
-  else return (c[0] + 2 * c[1] + 3 * c[2]) % 7;
-}
-
-// classification from the centre value and its 2*ND neighbours in axis order
-// (axis a: v[2a] at -1, v[2a+1] at +1).  2D: max / min over the 4 neighbours, saddle
-// when one axis is higher on both sides and the other lower (example/
-// test_cuda_topology.cu:56-68); 3D: saddle when at least one axis is higher and one
-// lower (SZp/src/szp_topology_3d.cc:79-98) -- the same rule for two axes.
-template <int ND>
-__device__ __forceinline__ int classify_vals(float c, const float (&v)[2 * ND]) {
-  bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
-#pragma unroll
-  for (int a = 0; a < ND; a++) {
-    const bool hi = c > v[2 * a] && c > v[2 * a + 1];
-    const bool lo = c < v[2 * a] && c < v[2 * a + 1];
-    all_gt = all_gt && hi;
-    all_lt = all_lt && lo;
-    any_hi = any_hi || hi;
-    any_lo = any_lo || lo;
-  }
-  return all_gt ? 1 : (all_lt ? 2 : ((any_hi && any_lo) ? 3 : 0));
-}
-
-template <int ND>
-__device__ __forceinline__ int classify_at(const Grid& g, const float* __restrict__ y, uint32_t e,
-                                           const int (&c)[ND]) {
-  if (!interior<ND>(g, c)) return 0;
-  float v[2 * ND];
-#pragma unroll
-  for (int a = 0; a < ND; a++) {
-    const uint32_t s = stride_of<ND>(g, a);
-    v[2 * a] = y[e - s];
-    v[2 * a + 1] = y[e + s];
-  }
-  return classify_vals<ND>(y[e], v);
-}
-
-// Field accesses of the window solver.  COH: agent-coherent loads and stores (they
-// bypass the XCD-local caches), for the one-launch solve where neighbouring tiles run
-// on other XCDs of the same launch.
-template <bool COH>
-__device__ __forceinline__ float y_ld(const float* y, uint32_t e) {
-  if constexpr (COH) return __hip_atomic_load(const_cast<float*>(y) + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else return y[e];
-}
-template <bool COH>
-__device__ __forceinline__ void y_st(float* y, uint32_t e, float v) {
-  if constexpr (COH) __hip_atomic_store(y + e, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else y[e] = v;
-}
-template <int ND, bool COH>
-__device__ __forceinline__ int classify_at_y(const Grid& g, const float* y, uint32_t e, const int (&c)[ND]) {
-  if (!interior<ND>(g, c)) return 0;
-  float v[2 * ND];
-#pragma unroll
-  for (int a = 0; a < ND; a++) {
-    const uint32_t s = stride_of<ND>(g, a);
-    v[2 * a] = y_ld<COH>(y, e - s);
-    v[2 * a + 1] = y_ld<COH>(y, e + s);
-  }
-  return classify_vals<ND>(y_ld<COH>(y, e), v);
-}
-
-// ---------------------------------------------------------------------------------
-// geometry source: the field of a flat index (coordinates, interval width, tiles)
-// ---------------------------------------------------------------------------------
-struct FieldView {
-  Grid g;
-  uint32_t base, n, fid;
-  float eb;
-  uint32_t tile0, tx, ty, tz;
-};
-// one field at flat offset 0
-struct Geo1 {
-  Grid g;
-  float eb;
-  uint32_t n, tx, ty, tz;
-  static constexpr bool kBatch = false;
-  __device__ __forceinline__ FieldView at(uint32_t) const {
-    FieldView v;
-    v.g = g;
-    v.base = 0;
-    v.n = n;
-    v.fid = 0;
-    v.eb = eb;
-    v.tile0 = 0;
-    v.tx = tx;
-    v.ty = ty;
-    v.tz = tz;
-    return v;
-  }
-  __device__ __forceinline__ FieldView of_tile(uint32_t) const { return at(0); }
-  __device__ __forceinline__ uint32_t spmax(const FieldView&, const uint32_t* misc) const { return misc[kPpSpMax]; }
-};
-// a batch of fields in flat arrays (PpField)
-struct GeoB {
-  const PpField* f;
-  const uint32_t* fmap;
-  const uint32_t* tmap;
-  const uint32_t* spm;
-  static constexpr bool kBatch = true;
-  __device__ __forceinline__ FieldView view(uint32_t i) const {
-    const PpField& p = f[i];
-    FieldView v;
-    v.g = p.g;
-    v.base = p.base;
-    v.n = p.n;
-    v.fid = i;
-    v.eb = p.eb;
-    v.tile0 = p.tile0;
-    v.tx = p.tx;
-    v.ty = p.ty;
-    v.tz = p.tz;
-    return v;
-  }
-  __device__ __forceinline__ FieldView at(uint32_t e) const { return view(fmap[e >> kFlatShift]); }
-  __device__ __forceinline__ FieldView of_tile(uint32_t tile) const { return view(tmap[tile]); }
-  __device__ __forceinline__ uint32_t spmax(const FieldView& v, const uint32_t*) const { return spm[v.fid]; }
-};
-
-// interval of the restored value of a point with decoded value d.  The zero cell is
-// the one cell whose points may leave the error bound (by at most eb kZeroCell); its
-// half-width only has to leave room for the orders inside a plateau, which float
-// spacing near zero gives many times over.
-constexpr float kZeroCell = 1.f / 65536.f;
-__device__ __forceinline__ void cell_of(float d, float eb, float* lo, float* hi) {
-  if (d > 0.f) {
-    *lo = d;
-    *hi = d + 0.9f * eb;
-  } else if (d < 0.f) {
-    *lo = d - 0.9f * eb;
-    *hi = d;
-  } else {
-    *lo = -eb * kZeroCell;
-    *hi = eb * kZeroCell;
-  }
-}
-
-__device__ __forceinline__ bool anchored(int t, float d) { return (t == 1 || t == 2) && d != 0.f; }
-
-__device__ __forceinline__ bool claim_bit(uint32_t* bits, uint32_t e) {
-  const uint32_t b = 1u << (e & 31u);
-  return (atomicOr(&bits[e >> 5], b) & b) == 0u;
-}
 
 // append val to list for every lane whose flag is set (one atomic per wave)
 __device__ __forceinline__ void wave_append(bool flag, uint32_t val, uint32_t* __restrict__ list, uint32_t cap,
@@ -1690,11 +1503,11 @@ __device__ __forceinline__ float rdlf(float v, int src) {
 }
 constexpr int kTileShift = 4;  // 16-wide tiles: windows of one parity class never interact
 constexpr int kWinMax = 8;
-constexpr int kSolveBudget = 1 << 14;
+constexpr int kSolveBudget = 1 << 12;
 // search nodes of one tile's windows beyond which its remaining windows are not tried: the
 // one-launch solve lasts as long as its longest chain of dependent tiles, and the windows
 // that exhaust their budgets lie on plateaus the whole-plateau stage orders
-constexpr uint32_t kTileNodeCap = 1u << 16;
+constexpr uint32_t kTileNodeCap = 1u << 14;
 
 template <int ND>
 __device__ __forceinline__ uint32_t tile_of(const int (&c)[ND], uint32_t ty, uint32_t tz) {
@@ -2150,12 +1963,13 @@ __device__ bool run_attempt(Window<ND>& w, const Grid& g, const uint8_t* __restr
 template <int ND, bool COH>
 __device__ int solve_window(Window<ND>& w, const Grid& g, uint32_t base, const float* __restrict__ d,
                             const uint8_t* __restrict__ t,
-                            float* y, float eb, uint32_t b, bool anchor, float delta_frac, int lane,
-                            uint32_t* nodes, uint32_t* stats) {
+                            float* y, float eb, uint32_t b, bool anchor, bool whole_only, float delta_frac,
+                            int lane, uint32_t* nodes, uint32_t* stats) {
   const uint64_t t0 = wall_clock64();
   build_window<ND, COH>(w, g, base, d, t, y, eb, b, anchor, lane);
   const uint64_t t1 = wall_clock64();
   stats[4] += (uint32_t)(t1 - t0);
+  if (whole_only && w.ne > w.nw) return -1;  // part of a larger plateau: left to the plateau solves
   const bool ok0 = run_attempt<ND, COH>(w, g, t, y, 0, -1, 0.f, lane, nodes, stats);
   stats[5] += (uint32_t)(wall_clock64() - t1);
   if (ok0) return 0;
@@ -2293,7 +2107,8 @@ __device__ void solve_tile(SolveShared<ND>& sh, const FieldView& F, uint32_t id,
       continue;
     }
     const int mode =
-        solve_window<ND, COH>(w, g, F.base, d, t, y, eb, e, anchor != 0, delta_frac, lane, &nodes, tst);
+        solve_window<ND, COH>(w, g, F.base, d, t, y, eb, e, (anchor & 1) != 0, (anchor & 2) != 0, delta_frac, lane,
+                              &nodes, tst);
     solved += mode >= 0;
     failed += mode < 0;
     freed += mode >= 2;
@@ -3015,38 +2830,6 @@ static inline unsigned grid_stride_blocks(uint64_t n) {
 
 uint32_t pp_init_tiles(uint32_t n) { return (n + kInitTile - 1) / kInitTile; }
 
-static Geo1 geo1(const PpGeo& G) {
-  Geo1 g;
-  g.g = G.g;
-  g.eb = G.eb;
-  g.n = G.n;
-  g.tx = G.tx;
-  g.ty = G.ty;
-  g.tz = G.tz;
-  return g;
-}
-static GeoB geob(const PpGeo& G) {
-  GeoB b;
-  b.f = G.fields;
-  b.fmap = G.fmap;
-  b.tmap = G.tmap;
-  b.spm = G.spmax;
-  return b;
-}
-
-// launches kernel template K<ND, GEO> with the geometry of `geo` as first argument
-#define PP_GEO_LAUNCH(K, grid, blk, st, ...)                                                    \
-  do {                                                                                          \
-    if (geo.fields) {                                                                           \
-      const GeoB gb_ = geob(geo);                                                               \
-      if (geo.ndim == 2) hipLaunchKernelGGL((K<2, GeoB>), grid, blk, 0, st, gb_, __VA_ARGS__);  \
-      else hipLaunchKernelGGL((K<3, GeoB>), grid, blk, 0, st, gb_, __VA_ARGS__);                \
-    } else {                                                                                    \
-      const Geo1 g1_ = geo1(geo);                                                               \
-      if (geo.ndim == 2) hipLaunchKernelGGL((K<2, Geo1>), grid, blk, 0, st, g1_, __VA_ARGS__);  \
-      else hipLaunchKernelGGL((K<3, Geo1>), grid, blk, 0, st, g1_, __VA_ARGS__);                \
-    }                                                                                           \
-  } while (0)
 
 hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st) {
   if (geo.fields) {  // every field's largest sort position
