@@ -1748,23 +1748,42 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     FillRanges fr{};
     fr.add(bits, 0u, bit_words);
     fr.add(misc + kPpMcSolved, 0u, 5);
+    fr.add(misc + kPpBad, 0u, 1);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, false, budget, 1, misc, st));
-    int rr = read_misc();
-    if (rr) return rr;
-    const uint32_t solved_small = h[kPpMcSolved];
-    if (h[kPpMcBig]) {  // plateaus beyond the small variant: the large one over the same list
+    // the variants by capacity, each over the same list while plateaus exceeded the last;
+    // the claim bits stay, so a plateau is searched by one variant only; the searches
+    // list what they leave mismatched into bad2 (exact: the list they read was)
+    int rr = 0;
+    uint32_t solved_small = 0;
+    for (int variant = 0; variant < 3; variant++) {
       FillRanges fb{};
-      fb.add(bits, 0u, bit_words);
+      fb.add(misc + kPpMcBig, 0u, 1);
       HIPCHK(launch_fill_multi(fb, st));
-      HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, true, budget, 1, misc, st));
+      HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, variant, budget, bad2, cap, stats, misc, st));
+      if ((rr = read_misc())) return rr;
+      if (variant == 0) solved_small = h[kPpMcSolved];
+      if (!h[kPpMcBig]) break;
     }
     const uint32_t before = nb;
-    if ((rr = full_mark(&nb, true))) return rr;
+    std::swap(bad, bad2);
+    nb = h[kPpBad];
+    exact = nb <= cap;
+    verified = false;
+    if (getenv("TSZP_MC_CHECK")) {  // diagnostics: the relisted count against a full pass
+      const uint32_t listed = nb;
+      uint32_t sv[5];
+      for (int k = 0; k < 5; k++) sv[k] = h[kPpMcSolved + k];
+      if ((rr = full_mark(&nb, true))) return rr;
+      fprintf(stderr, "[restore] check: relisted %u, full pass %u (missing %llu spurious %llu changed %llu)\n", listed,
+              nb, (unsigned long long)reinterpret_cast<const uint64_t*>(h + kPpVerify)[0],
+              (unsigned long long)reinterpret_cast<const uint64_t*>(h + kPpVerify)[1],
+              (unsigned long long)reinterpret_cast<const uint64_t*>(h + kPpVerify)[2]);
+      for (int k = 0; k < 5; k++) h[kPpMcSolved + k] = sv[k];
+    }
     mc_solved += h[kPpMcSolved];
     if (stats)
-      fprintf(stderr, "[restore] min-conflicts (%s, %u steps) plateaus solved %u (%u small) failed %u beyond the small "
-              "variant %u: %u -> %u mismatches; max steps %u, max us %.1f\n", tag, budget, h[kPpMcSolved],
+      fprintf(stderr, "[restore] min-conflicts (%s, %u steps) plateaus solved %u (%u of up to 64 points) failed %u "
+              "beyond the largest variant %u: %u -> %u mismatches; max steps %u, max us %.1f\n", tag, budget, h[kPpMcSolved],
               solved_small, h[kPpMcFailed], h[kPpMcBig], before, nb, h[kPpMcMaxSteps], h[kPpMcMaxTicks] * 0.01);
     return TSZP_OK;
   };
@@ -1949,6 +1968,8 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
               h[kPpPlatMaxSweeps]);
     solved += h[kPpPlatSolved];
   }
+  // the verification counts come from a full pass over the final values
+  if (!verified && (r = full_mark(&nb, true))) return r;
   if (h[kPpErr]) return fail(TSZP_EHIP, "topo_restore: kernel error bits 0x%x", h[kPpErr]);
   const uint64_t* hv = reinterpret_cast<const uint64_t*>(h + kPpVerify);
   for (int k = 0; k < 4; k++) out->verify[k] = hv[k];

@@ -451,12 +451,17 @@ hipError_t pp_plateau(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, cons
                       const float* guide, uint32_t* claim, bool large, uint32_t* misc, hipStream_t st);
 // whole-plateau min-conflicts solves around the listed mismatches (tszp_platmc.hip), one
 // wavefront each: guide (nullable) the start order, else breadth-first distances inside
-// the plateau; claim: zeroed bits, one per point; large: plateaus of up to 3072 points
-// (150 KB of LDS per wavefront) instead of 512; budget: steps per plateau; plateaus of
-// fewer than min_size points are left alone
+// the plateau; claim: one bit per point, zeroed before the first variant and kept across
+// the variants, so that a plateau one variant took is not searched again; variant 0, 1,
+// 2: plateaus of up to 64, 512, 3072 points (4, 23, 150 KB of LDS per wavefront; a
+// larger plateau counts in misc[kPpMcBig] for the next variant); budget: steps per
+// plateau; relist (cap entries, counter misc[kPpBad]): every point of a searched
+// plateau that is mismatched afterwards, and with variant 2 the listed mismatches of
+// plateaus beyond it -- with all variants run over an exact list, relist is exact;
+// stats: solved / failed counts and the largest step count and time in misc
 hipError_t pp_platmc(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
-                     const float* guide, uint32_t* claim, bool large, uint32_t budget, uint32_t min_size,
-                     uint32_t* misc, hipStream_t st);
+                     const float* guide, uint32_t* claim, int variant, uint32_t budget, uint32_t* relist,
+                     uint32_t cap, bool stats, uint32_t* misc, hipStream_t st);
 // window solves (one launch per parity phase, or every phase in one launch below);
 // anchor: bit 0 keeps the anchored extrema in place, bit 1 solves only windows that hold
 // their whole plateau (the larger ones are left to the plateau solves)

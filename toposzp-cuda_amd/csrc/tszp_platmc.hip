@@ -43,11 +43,11 @@ constexpr int kMcSlots = 7 * kMcPos;    // the mismatched point and up to 6 plat
 constexpr uint32_t kMcWalkPct = 15;     // share of random moves, per cent
 constexpr uint32_t kMcTabu = 10;        // steps during which a moved point loses ties
 constexpr int16_t kNbLow = -1, kNbHigh = -2, kNbOut = -3;
-constexpr int kMcSmall = 512, kMcLarge = 3072;
+constexpr int kMcTiny = 64, kMcSmall = 512, kMcLarge = 3072;
 
 template <int ND, int PM>
 struct McShared {
-  static constexpr int kHash = PM <= 512 ? 1024 : 8192;  // slots, a power of two >= 2 PM
+  static constexpr int kHash = PM <= 64 ? 128 : (PM <= 512 ? 1024 : 8192);  // a power of two >= 2 PM
   uint32_t el[PM];
   int16_t nb[PM][2 * ND];  // >= 0: plateau index; kNbLow / kNbHigh: other cell; kNbOut: off the grid
   float key[PM];
@@ -80,10 +80,10 @@ __device__ __forceinline__ uint64_t mc_mix(uint64_t h) {
   return h;
 }
 
+constexpr int mc_log2(int v) { return v <= 1 ? 0 : 1 + mc_log2(v >> 1); }
 template <int H>
 __device__ __forceinline__ uint32_t mc_hash(uint32_t e) {
-  constexpr int bits = H == 1024 ? 10 : 13;
-  return (e * 2654435761u) >> (32 - bits);
+  return (e * 2654435761u) >> (32 - mc_log2(H));
 }
 
 template <class SH>
@@ -248,11 +248,41 @@ __device__ __forceinline__ bool mc_position(const float (&nk)[K], int n, int pos
   return true;
 }
 
+// lists the plateau's points whose classification from y is not their type (one
+// atomic per wave chunk that has any); outside the plateau only other cells' values are
+// read, and those stay in their cells whatever other wavefronts write
+template <int ND, class SH>
+__device__ void mc_relist(const SH& sh, const FieldView& F, const float* y, const uint8_t* __restrict__ t,
+                          uint32_t* __restrict__ relist, uint32_t cap, uint32_t* misc, int lane) {
+  for (int i0 = 0; i0 < sh.ne; i0 += kWave) {
+    const int i = i0 + lane;
+    bool bad = false;
+    uint32_t e = 0;
+    if (i < sh.ne) {
+      e = sh.el[i];
+      int c[ND];
+      coords_of<ND>(F.g, e - F.base, c);
+      bad = classify_at<ND>(F.g, y, e, c) != (int)t[e];
+    }
+    const uint64_t m = __ballot(bad);
+    if (!m) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&misc[kPpBad], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader, kWave);
+    if (bad) {
+      const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt(lane));
+      if (pos < cap) relist[pos] = e;
+    }
+  }
+}
+
 template <int ND, class GEO, int PM>
 __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
                                                      const float* __restrict__ d, const uint8_t* __restrict__ t,
                                                      float* __restrict__ y, const float* __restrict__ guide,
-                                                     uint32_t* __restrict__ claim, uint32_t budget, uint32_t min_size,
+                                                     uint32_t* __restrict__ claim, uint32_t budget, bool last,
+                                                     uint32_t* __restrict__ relist, uint32_t cap, bool stats,
                                                      uint32_t* misc) {
   constexpr int K = 2 * ND;
   using SH = McShared<ND, PM>;
@@ -338,11 +368,16 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     head = ne0;
   }
   if (sh.abort) {
-    if (lane == 0) atomicAdd(&misc[kPpMcBig], 1u);
+    if (lane == 0) {
+      atomicAdd(&misc[kPpMcBig], 1u);
+      if (last) {  // beyond every variant: the mismatch stays listed
+        const uint32_t pos = atomicAdd(&misc[kPpBad], 1u);
+        if (pos < cap) relist[pos] = b;
+      }
+    }
     return;
   }
   const int ne = sh.ne;
-  if ((uint32_t)ne < min_size) return;  // left to the solvers for small plateaus
   // claimed by its smallest point: one wavefront per plateau
   uint32_t mn = 0xffffffffu;
   for (int i = lane; i < ne; i += kWave) mn = min(mn, sh.el[i]);
@@ -675,19 +710,21 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     __syncthreads();
     nv = sh.nv;
   }
-  if (lane == 0) {
+  // (statistics only on request: one same-address atomic per plateau serialises)
+  if (lane == 0 && stats) {
     atomicMax(&misc[kPpMcMaxSteps], step);
     atomicMax(&misc[kPpMcMaxTicks], (uint32_t)min<uint64_t>(wall_clock64() - t_start, 0xffffffffull));
+    if (nv > 0) atomicAdd(&misc[kPpMcFailed], 1u);
   }
-  if (nv > 0) {
-    if (lane == 0) atomicAdd(&misc[kPpMcFailed], 1u);
+  if (nv > 0) {  // not finished: the plateau keeps its values; its mismatches are listed again
+    mc_relist<ND>(sh, F, y, t, relist, cap, misc, lane);
     return;
   }
   // ---- order by key, then values ----------------------------------------------------------
   for (int i = lane; i < p2; i += kWave) sk[i] = i < ne ? mc_sortkey(sh.key[i], i) : ~0ull;
   __syncthreads();
   wave_bitonic(sk, p2, lane);
-  if (lane == 0) {
+  if (lane == 0) {  // values into the key array, in key order
     int s0 = 0;
     float prev = sh.lo;
     for (int r = 0; r <= ne; r++) {
@@ -698,14 +735,23 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
         const int cnt = r - s0;
         for (int k = 0; k < cnt; k++) {
           const int j = (int)(uint32_t)sk[s0 + k];
-          y[sh.el[j]] = prev + (nxt - prev) * ((float)(k + 1) / (float)(cnt + 1));
+          sh.key[j] = prev + (nxt - prev) * ((float)(k + 1) / (float)(cnt + 1));
         }
-        if (!end) prev = nxt;
+        if (!end) {
+          sh.key[i] = nxt;
+          prev = nxt;
+        }
         s0 = r + 1;
       }
     }
-    atomicAdd(&misc[kPpMcSolved], 1u);
+    if (stats) atomicAdd(&misc[kPpMcSolved], 1u);
   }
+  __syncthreads();
+  for (int i = lane; i < ne; i += kWave) y[sh.el[i]] = sh.key[i];
+  __syncthreads();
+  // what the written values leave mismatched (equal anchor values with points between
+  // them, float spacing) is listed again
+  mc_relist<ND>(sh, F, y, t, relist, cap, misc, lane);
 }
 
 }  // namespace
@@ -716,27 +762,28 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
       const GeoB gb_ = geob(geo);                                                                                \
       if (geo.ndim == 2)                                                                                         \
         hipLaunchKernelGGL((k_pp_platmc<2, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, min_size, misc);                                                          \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
       else                                                                                                       \
         hipLaunchKernelGGL((k_pp_platmc<3, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, min_size, misc);                                                          \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
     } else {                                                                                                     \
       const Geo1 g1_ = geo1(geo);                                                                                \
       if (geo.ndim == 2)                                                                                         \
         hipLaunchKernelGGL((k_pp_platmc<2, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, min_size, misc);                                                          \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
       else                                                                                                       \
         hipLaunchKernelGGL((k_pp_platmc<3, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, min_size, misc);                                                          \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
     }                                                                                                            \
   } while (0)
 
 hipError_t pp_platmc(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
-                     const float* guide, uint32_t* claim, bool large, uint32_t budget, uint32_t min_size, uint32_t* misc,
-                     hipStream_t st) {
+                     const float* guide, uint32_t* claim, int variant, uint32_t budget, uint32_t* relist,
+                     uint32_t cap, bool stats, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  if (large) PP_MC_LAUNCH(kMcLarge);
-  else PP_MC_LAUNCH(kMcSmall);
+  if (variant == 2) PP_MC_LAUNCH(kMcLarge);
+  else if (variant == 1) PP_MC_LAUNCH(kMcSmall);
+  else PP_MC_LAUNCH(kMcTiny);
   return hipGetLastError();
 }
 
