@@ -530,6 +530,144 @@ def serve_side(tz, torch, pool, big, seed, nreq, rate):
     return res
 
 
+def serve_topo_side(tz, torch, big, seed, nreq):
+    """Served topology stream: nreq requests, 70 % compressions of a field of the
+    mixed-size workload (or with probability 2 % the 512^3 field) and 30 % decompressions
+    with the restoration of a field compressed earlier, through the topology request loop
+    (two lanes, a 200 GB budget).  Capacity: all requests submitted at once; latency:
+    Poisson arrivals at 70 % of that capacity; beside them the same requests one at a
+    time with the one-field calls (tszp_topo_compress; tszp_topo_decompress +
+    tszp_sortpos_decompress + tszp_topo_restore) at the same arrival times.  GB/s of
+    field bytes (input of a compression, output of a decompression)."""
+    import threading
+    import numpy as np
+    import workloads
+    rng = np.random.default_rng(seed + 7)
+    wl = workloads.mixed_fields(seed)
+    fields = [f for f, _ in wl] + [big["field"]]
+    ebs = [eb for _, eb in wl] + [big["eb"]]
+    bss = [0] * len(wl) + [big["bs"]]
+    nbig = len(fields) - 1
+    kinds, which = [], []
+    for _ in range(nreq):
+        if rng.random() < 0.7 or not kinds:
+            kinds.append(0)
+            which.append(nbig if rng.random() < 0.02 else int(rng.integers(0, nbig)))
+        else:
+            kinds.append(1)
+            comp = [w for k, w in zip(kinds, which) if k == 0]
+            which.append(comp[int(rng.integers(0, len(comp)))])
+    nbytes = sum(4 * fields[w].numel() for w in which)
+    outs = [torch.empty(f.numel(), dtype=torch.float32, device=f.device) for f in fields]
+    budget = 200 << 30
+    maxe = max(f.numel() for f in fields) + (1 << 24)
+    try:
+        srv = tz.TopoServer(budget, max_batch=256, max_batch_elements=maxe, lanes=2)
+    except tz.TszpError:
+        budget = 96 << 30
+        srv = tz.TopoServer(budget, max_batch=256, max_batch_elements=maxe, lanes=2)
+
+    def run(arrivals):
+        lock = threading.Lock()
+        latest = {}
+        tickets = [None] * nreq
+        kinds_run = list(kinds)
+        ready = threading.Semaphore(0)
+        lat = np.zeros(nreq)
+
+        def waiter():
+            for i in range(nreq):
+                ready.acquire()
+                info = srv.wait(tickets[i])
+                lat[i] = info.latency_us
+                with lock:
+                    if kinds_run[i] == 0:
+                        old = latest.get(which[i])
+                        latest[which[i]] = tickets[i]
+                        if old is not None:
+                            srv.release(old)
+                    else:
+                        srv.release(tickets[i])
+
+        torch.cuda.synchronize()
+        wt = threading.Thread(target=waiter)
+        wt.start()
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            if arrivals is not None:
+                while time.perf_counter() - t0 < arrivals[i]:
+                    pass
+            w = which[i]
+            with lock:
+                src = latest.get(w) if kinds_run[i] == 1 else None
+                if kinds_run[i] == 1 and src is None:
+                    kinds_run[i] = 0  # its compression has not finished yet: compress again
+                if kinds_run[i] == 0:
+                    tickets[i] = srv.compress(fields[w], ebs[w], bss[w])
+                else:
+                    tickets[i] = srv.decompress(src, outs[w])
+            ready.release()
+        wt.join()
+        el = time.perf_counter() - t0
+        with lock:
+            for t in latest.values():
+                srv.release(t)
+        return el, lat
+
+    run(None)  # warm-up: the lanes' workspaces grow to their working size
+    cap_s, cap_lat = run(None)
+    cap_gbs = nbytes / cap_s / 1e9
+    rate = 0.7 * nreq / cap_s
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    srv_s, lat = run(arrivals)
+    st = srv.stats()
+    srv.close()
+    # the same requests one at a time with the one-field calls
+    # the block sizes the request loop's tuner picks, for the one-field calls
+    tab = tz._batch_table([x.view(-1) for x in fields[:nbig]], ebs[:nbig], [0] * nbig)
+    tz._check(tz.lib().tszp_batch_tune(tab, nbig, tz._stream()))
+    tuned = [tab[i].block_size for i in range(nbig)] + [big["bs"]]
+    bufs, infos, bsel = {}, {}, {}
+    lat1 = np.zeros(nreq)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        w = which[i]
+        f = fields[w]
+        if kinds[i] == 0 or w not in infos:
+            bs = tuned[w]
+            if w not in bufs:
+                bufs[w] = tz.TopoBuffers(f.numel(), bs, 1, f.device)
+            infos[w], _ = tz.topo_compress(f, ebs[w], bs, T=1, buf=bufs[w])
+            bsel[w] = bs
+        else:
+            bs, info = bsel[w], infos[w]
+            out, types = tz.topo_decompress(bufs[w], f.numel(), ebs[w], bs, T=1)
+            sp = tz.sortpos_decompress(bufs[w], info.ext_count, bs, T=1) if info.ext_count else None
+            tz.topo_restore(out, types, tuple(f.shape), ebs[w], sp=sp, n_ext=info.ext_count, out=outs[w])
+        torch.cuda.synchronize()
+        lat1[i] = 1e6 * (time.perf_counter() - t0 - arrivals[i])
+    loop_s = time.perf_counter() - t0
+    del bufs
+    res = {"requests": nreq, "compress_share": float(np.mean(np.array(kinds) == 0)),
+           "big_fields": int(sum(1 for w in which if w == nbig)), "input_gb": nbytes / 1e9,
+           "hbm_budget_gb": budget / 2 ** 30, "hbm_peak_gb": st["hbm_peak"] / 2 ** 30, "lanes": 2,
+           "capacity_gbs": cap_gbs, "capacity_requests_per_s": nreq / cap_s,
+           "rate_per_s": rate, "served_gbs": nbytes / srv_s / 1e9,
+           "served_p50_ms": float(np.percentile(lat, 50)) / 1e3, "served_p99_ms": float(np.percentile(lat, 99)) / 1e3,
+           "loop_gbs": nbytes / loop_s / 1e9, "loop_p50_ms": float(np.percentile(lat1, 50)) / 1e3,
+           "loop_p99_ms": float(np.percentile(lat1, 99)) / 1e3}
+    res["capacity_vs_loop"] = res["capacity_gbs"] / res["loop_gbs"]
+    log(f"served topology stream of {nreq} requests ({nbytes / 1e9:.1f} GB, {res['big_fields']} 512^3 fields): "
+        f"capacity {cap_gbs:.1f} GB/s ({nreq / cap_s:.0f} requests/s); at {rate:.0f}/s: {res['served_gbs']:.1f} GB/s, "
+        f"p50 {res['served_p50_ms']:.2f} ms, p99 {res['served_p99_ms']:.2f} ms; one request at a time with the "
+        f"one-field calls: {res['loop_gbs']:.1f} GB/s, p50 {res['loop_p50_ms']:.1f} ms, p99 {res['loop_p99_ms']:.1f} ms "
+        f"-> capacity {res['capacity_vs_loop']:.1f}x")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -549,6 +687,7 @@ def main():
     ap.add_argument("--no-other-bs", action="store_true", help="leave out the step at the other block size")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-rate", type=float, default=40000.0, help="Poisson arrival rate per second")
+    ap.add_argument("--serve-topo-requests", type=int, default=2000)
     ap.add_argument("--profile-pass", action="store_true",
                     help="one step, then a 512 MiB elementwise copy that calibrates HBM counters; for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -725,6 +864,12 @@ def main():
                                 args.serve_requests, args.serve_rate)
         del pool
 
+    # side figure: the served topology stream (compress / decompress + restore requests)
+    served_topo = None
+    if not (args.no_batch or args.no_serve):
+        served_topo = serve_topo_side(tz, torch, {"field": f, "eb": eb, "bs": bs}, args.seed,
+                                      args.serve_topo_requests)
+
     # side figure: topology quality of the restored field against the original
     # (the CUDA driver's report), one GPU pass
     tz.quality_metrics(f, restored, shape, eb)
@@ -792,6 +937,7 @@ def main():
         "batched_topo": topo_batch,
         "batched_plain": batch,
         "served_plain": served,
+        "served_topo": served_topo,
         "quality_ms": quality_ms,
         "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
                                             "over_1eb", "pairs", "violations", "saddles_above",

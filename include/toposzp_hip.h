@@ -319,6 +319,70 @@ int tszp_topo_batch_output(tszp_topo_batch *batch, uint32_t i, float **d_decoded
  * values (n floats), 1 restored values (n floats), 2 types (n bytes), 3 sort positions
  * (ext_count ints; `count` elements are copied, at most n) */
 int tszp_topo_batch_copy(tszp_topo_batch *batch, uint32_t i, int which, void *d_dst, uint64_t count, void *stream);
+/* The same batch in the caller's device memory (256-byte aligned, mem_bytes of it): no
+ * allocation on the device, so nothing synchronises the device when the batch comes and
+ * goes (what a request loop that keeps several batches in flight needs).  Fails with
+ * TSZP_ECAP when the layout needs more than mem_bytes; tszp_topo_batch_mem_bound gives
+ * enough for any batch of up to `count` fields and `elements` elements in all.  The
+ * memory must outlive the batch. */
+int tszp_topo_batch_create_mem(tszp_topo_batch_field *fields, uint32_t count, void *d_mem, uint64_t mem_bytes,
+                               tszp_topo_batch **batch, void *stream);
+uint64_t tszp_topo_batch_mem_bound(uint64_t elements, uint32_t count);
+
+/* A restoration workspace of the caller's own (device buffers that grow on demand and
+ * stay until destroyed).  The library's calls otherwise share one workspace per device
+ * and run one at a time; restorations with different workspaces, on different streams,
+ * run concurrently.  A workspace serves one call at a time, on the device that was
+ * current when it was created. */
+typedef struct tszp_workspace tszp_workspace;
+int tszp_workspace_create(tszp_workspace **ws);
+int tszp_workspace_destroy(tszp_workspace *ws);
+/* tszp_topo_batch_restore with the given workspace (NULL: the device's own) */
+int tszp_topo_batch_restore_ws(tszp_topo_batch *batch, int max_rounds, tszp_workspace *ws, void *stream);
+
+/* Request loop of the topology pipeline (no counterpart in the reference, whose callers
+ * run the per-field chain once per field: example/test_cuda_topology.cu:374-530).
+ * Compress requests (2D / 3D fields) and decompress requests (decode, then the topology
+ * restoration, of a finished compress ticket's streams) from any host thread are
+ * admitted in arrival order while their streams fit the HBM budget; `lanes` worker
+ * threads, each with its own stream, batch memory for up to max_batch requests and
+ * max_batch_elements elements, and restoration workspace, take turns at the queue, so
+ * that the batch formed from what arrived during the previous one runs on the device
+ * beside it.  The budget covers the lanes' working memory (reserved at creation) and one
+ * device arena for the compress tickets' streams and side-cars (tszp_topo_bound and
+ * tszp_sortpos_bound of their block size, or of the largest candidate with block size
+ * 0).  A request larger than a batch, or than the arena, fails; one that does not fit
+ * yet waits for releases.  Streams are byte-identical to tszp_topo_compress with
+ * nchunks 1 and decoded values to tszp_topo_decompress; restored values meet the same
+ * invariants as tszp_topo_restore (the restoration's rounds depend on the batch).
+ * Inputs must be complete on the device when submitted and stay valid until the ticket
+ * finishes; outputs are valid after tszp_topo_server_wait.  Release every ticket when
+ * done with it.  Errors: tszp_topo_server_last_error(). */
+typedef struct tszp_topo_server tszp_topo_server;
+const char *tszp_topo_server_last_error(void);
+int tszp_topo_server_create(uint64_t hbm_budget, uint32_t max_batch, uint64_t max_batch_elements, uint32_t lanes,
+                            tszp_topo_server **out);
+/* serves what is queued, then stops; call it once no other thread uses the server */
+int tszp_topo_server_destroy(tszp_topo_server *server);
+int tszp_topo_server_compress(tszp_topo_server *server, const float *d_field, int ndim, const uint32_t *dims,
+                              float eb, int block_size, uint64_t *ticket);
+/* d_out: n floats, restored values (restore != 0) or decoded values; d_types: n bytes
+ * of types, or NULL */
+int tszp_topo_server_decompress(tszp_topo_server *server, uint64_t compress_ticket, float *d_out, uint8_t *d_types,
+                                int restore, uint64_t *ticket);
+/* stream_bytes of a compress ticket: topology + sort-position stream bytes */
+int tszp_topo_server_wait(tszp_topo_server *server, uint64_t ticket, tszp_ticket_info *info);
+/* a finished ticket's compression results (of its source, for a decompress ticket), its
+ * restoration counts (decompress tickets) and block size; any may be NULL */
+int tszp_topo_server_result(tszp_topo_server *server, uint64_t ticket, tszp_topo_info *info,
+                            tszp_restore_info *restore, int *block_size);
+/* device pointers into the arena of a finished compress ticket's streams and side-cars */
+int tszp_topo_server_streams(tszp_topo_server *server, uint64_t compress_ticket, const uint8_t **d_topo,
+                             const uint32_t **d_topo_blk, const uint8_t **d_sortpos,
+                             const uint32_t **d_sortpos_blk);
+int tszp_topo_server_release(tszp_topo_server *server, uint64_t ticket);
+/* hbm figures include the lanes' reserved working memory */
+int tszp_topo_server_stats(tszp_topo_server *server, tszp_server_stats_t *out);
 
 /* Topology quality of a decompressed field y against the original x, as the
  * reference's CUDA driver reports it (example/test_cuda_topology.cu:540-868), for 2D

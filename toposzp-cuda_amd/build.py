@@ -20,7 +20,7 @@ ARCH = os.environ.get("TSZP_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["tszp_codec.hip", "tszp_sort.hip", "tszp_topo.hip", "tszp_util.hip", "tszp_gen.hip",
            "tszp_post.hip", "tszp_binrank.hip", "tszp_parse.hip", "tszp_quality.hip", "tszp_elem.hip",
            "tszp_saddle.hip", "tszp_noalloc.hip", "tszp_topo_batch.hip", "tszp_platmc.hip", "tszp_api.cpp", "tszp_serve.cpp",
-           "tszp_batch_api.cpp"]
+           "tszp_batch_api.cpp", "tszp_topo_serve.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -1982,16 +1982,30 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
 
 }  // namespace
 
+// A workspace of its own: device buffers that grow on demand and stay until destroyed,
+// for callers that restore on several streams at once (the device's own workspace
+// serves one call at a time).
+struct tszp_workspace {
+  Ctx c;
+};
+
 int tszp::restore_batch(const RestoreBatchField* fields, uint32_t count, uint64_t flat, const float* d,
                         const uint8_t* t, const int32_t* sp, float* y, int max_rounds, tszp_restore_info* info,
-                        hipStream_t st) {
+                        tszp_workspace* ws, hipStream_t st) {
   if (!fields || count == 0 || !d || !t || !sp || !y || !info) return fail(TSZP_EINVAL, "restore_batch: bad arguments");
   if (flat == 0 || flat % kFlatAlign || flat >= 0x80000000ull)
     return fail(TSZP_EINVAL, "restore_batch: flat size must be a positive multiple of %u below 2^31", kFlatAlign);
   const int nd = fields[0].ndim;
   Ctx* c;
-  int r = get_ctx(&c);
-  if (r) return r;
+  int r = 0;
+  if (ws) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev != ws->c.device)
+      return fail(TSZP_EINVAL, "restore_batch: workspace of device %d used on device %d", ws->c.device, dev);
+    c = &ws->c;
+  } else if ((r = get_ctx(&c))) {
+    return r;
+  }
   std::lock_guard<std::mutex> lk(c->mu);
   // the fields of each dimensionality form one restoration (its kernels are per dimension)
   std::vector<PpField> pf;
@@ -2081,6 +2095,27 @@ int tszp::restore_batch(const RestoreBatchField* fields, uint32_t count, uint64_
 }
 
 extern "C" {
+
+int tszp_workspace_create(tszp_workspace** out) {
+  if (!out) return fail(TSZP_EINVAL, "workspace_create: null output");
+  *out = nullptr;
+  auto* w = new tszp_workspace();
+  if (hipGetDevice(&w->c.device) != hipSuccess) {
+    delete w;
+    return fail(TSZP_EDEVICE, "workspace_create: no device");
+  }
+  *out = w;
+  return TSZP_OK;
+}
+
+int tszp_workspace_destroy(tszp_workspace* ws) {
+  if (!ws) return fail(TSZP_EINVAL, "workspace_destroy: null workspace");
+  int r = TSZP_OK;
+  for (int k = 0; k < kWsCount; k++)
+    if (ws->c.ptr[k] && hipFree(ws->c.ptr[k]) != hipSuccess) r = fail(TSZP_EHIP, "workspace_destroy: hipFree failed");
+  delete ws;
+  return r;
+}
 
 int tszp_topo_restore(const float* d_decoded, float* d_field, const uint8_t* d_types, const int32_t* d_sp,
                       uint64_t n_ext, int ndim, const uint32_t* dims, float eb, int max_rounds,

@@ -107,6 +107,7 @@ struct tszp_topo_batch {
   int hi_bits = 8;
   uint32_t nchunks = 0;
   void* mem = nullptr;
+  bool own_mem = true;  // false: mem is the caller's (tszp_topo_batch_create_mem), host results pageable
   // device arrays (carved from mem)
   TopoBatchDev* dev = nullptr;
   float* feb = nullptr;
@@ -203,8 +204,11 @@ int batch_device_error(const tszp_topo_batch* B, const char* what, uint32_t bi, 
 
 void destroy(tszp_topo_batch* B) {
   if (!B) return;
-  if (B->host) (void)hipHostFree(B->host);
-  if (B->mem) (void)hipFree(B->mem);
+  if (B->host) {
+    if (B->own_mem) (void)hipHostFree(B->host);
+    else free(B->host);
+  }
+  if (B->mem && B->own_mem) (void)hipFree(B->mem);
   delete B;
 }
 
@@ -212,7 +216,36 @@ void destroy(tszp_topo_batch* B) {
 
 extern "C" {
 
+namespace {
+int topo_batch_create(tszp_topo_batch_field* fields, uint32_t count, void* ext_mem, uint64_t ext_bytes,
+                      tszp_topo_batch** out, void* stream);
+}
+
 int tszp_topo_batch_create(tszp_topo_batch_field* fields, uint32_t count, tszp_topo_batch** out, void* stream) {
+  return topo_batch_create(fields, count, nullptr, 0, out, stream);
+}
+
+int tszp_topo_batch_create_mem(tszp_topo_batch_field* fields, uint32_t count, void* d_mem, uint64_t mem_bytes,
+                               tszp_topo_batch** out, void* stream) {
+  if (!d_mem || ((uintptr_t)d_mem & 255u)) {
+    clear_error();
+    return bfail(TSZP_EINVAL, "topo_batch_create_mem: memory must be non-null and 256-byte aligned");
+  }
+  return topo_batch_create(fields, count, d_mem, mem_bytes, out, stream);
+}
+
+uint64_t tszp_topo_batch_mem_bound(uint64_t elements, uint32_t count) {
+  // per element: critical-point keys and payloads, sort keys and their alternates,
+  // sort positions, decoded and restored values, types, tile maps and look-back words;
+  // per field: tables, results and the padding of its flat range
+  return 80ull * (elements + (uint64_t)count * kFlatAlign) + 4096ull * count + (4ull << 20);
+}
+
+}  // extern "C"
+
+namespace {
+int topo_batch_create(tszp_topo_batch_field* fields, uint32_t count, void* ext_mem, uint64_t ext_bytes,
+                      tszp_topo_batch** out, void* stream) {
   clear_error();
   if (!out) return bfail(TSZP_EINVAL, "topo_batch_create: null output");
   *out = nullptr;
@@ -369,12 +402,28 @@ int tszp_topo_batch_create(tszp_topo_batch_field* fields, uint32_t count, tszp_t
   B->dec = cv.take<float>(flat);
   B->rest = cv.take<float>(flat);
   B->types = cv.take<uint8_t>(flat + 16);
-  hipError_t e = hipMalloc(&B->mem, cv.off);
-  if (e == hipSuccess)
-    e = hipHostMalloc((void**)&B->host, (4 * (size_t)count + 8) * sizeof(CompressResult), hipHostMallocDefault);
-  if (e != hipSuccess) {
-    destroy(B);
-    return bfail(TSZP_EHIP, "topo_batch_create: allocation of %zu bytes: %s", cv.off, hipGetErrorString(e));
+  hipError_t e = hipSuccess;
+  if (ext_mem) {  // the caller's memory; pageable host results
+    B->own_mem = false;
+    if (cv.off > ext_bytes) {
+      destroy(B);
+      return bfail(TSZP_ECAP, "topo_batch_create_mem: the batch needs %zu bytes, %llu given", cv.off,
+                   (unsigned long long)ext_bytes);
+    }
+    B->mem = ext_mem;
+    B->host = static_cast<CompressResult*>(malloc((4 * (size_t)count + 8) * sizeof(CompressResult)));
+    if (!B->host) {
+      destroy(B);
+      return bfail(TSZP_EHIP, "topo_batch_create_mem: host allocation failed");
+    }
+  } else {
+    e = hipMalloc(&B->mem, cv.off);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&B->host, (4 * (size_t)count + 8) * sizeof(CompressResult), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      destroy(B);
+      return bfail(TSZP_EHIP, "topo_batch_create: allocation of %zu bytes: %s", cv.off, hipGetErrorString(e));
+    }
   }
   void* m = B->mem;
   rebase(B->dev, m); rebase(B->feb, m); rebase(B->finv, m);
@@ -482,6 +531,9 @@ int tszp_topo_batch_create(tszp_topo_batch_field* fields, uint32_t count, tszp_t
   *out = B;
   return TSZP_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int tszp_topo_batch_destroy(tszp_topo_batch* B) {
   clear_error();
@@ -546,6 +598,10 @@ int tszp_topo_batch_decompress(tszp_topo_batch* B, void* stream) {
 }
 
 int tszp_topo_batch_restore(tszp_topo_batch* B, int max_rounds, void* stream) {
+  return tszp_topo_batch_restore_ws(B, max_rounds, nullptr, stream);
+}
+
+int tszp_topo_batch_restore_ws(tszp_topo_batch* B, int max_rounds, tszp_workspace* ws, void* stream) {
   clear_error();
   if (!B) return bfail(TSZP_EINVAL, "topo_batch_restore: null batch");
   if (max_rounds < 0) return bfail(TSZP_EINVAL, "topo_batch_restore: max_rounds must be >= 0");
@@ -561,7 +617,7 @@ int tszp_topo_batch_restore(tszp_topo_batch* B, int max_rounds, void* stream) {
   }
   B->rinfo.assign(B->count, tszp_restore_info{});
   int r = restore_batch(rf.data(), B->count, B->flat, B->dec, B->types, B->spv, B->rest, max_rounds,
-                        B->rinfo.data(), st);
+                        B->rinfo.data(), ws, st);
   timer.stop(st);
   if (r) return r;
   B->restored = true;

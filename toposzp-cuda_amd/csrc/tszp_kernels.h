@@ -169,8 +169,11 @@ struct RestoreBatchField {
   uint64_t base;
   const uint64_t* n_ext_dev;  // device: the field's extremum count
 };
+// ws: the restoration's workspace (tszp_workspace_create), or null for the device's
+// own one (shared by the library's calls on that device, one at a time)
 int restore_batch(const RestoreBatchField* fields, uint32_t count, uint64_t flat, const float* d, const uint8_t* t,
-                  const int32_t* sp, float* y, int max_rounds, tszp_restore_info* info, hipStream_t st);
+                  const int32_t* sp, float* y, int max_rounds, tszp_restore_info* info, tszp_workspace* ws,
+                  hipStream_t st);
 
 // Tile counter to pass to a look-back kernel: null (tiles = workgroup ids) unless
 // TSZP_TILE_COUNTER is set, which hands tiles out from the counter instead.

@@ -122,6 +122,24 @@ def lib():
               "tszp_server_wait", "tszp_server_stream", "tszp_server_release", "tszp_server_stats",
               "tszp_server_copy_stream"):
         getattr(L, f).restype = i32
+    L.tszp_topo_server_last_error.restype = C.c_char_p
+    L.tszp_topo_server_create.argtypes = [C.c_uint64, u32, C.c_uint64, u32, C.POINTER(vp)]
+    L.tszp_topo_server_destroy.argtypes = [vp]
+    L.tszp_topo_server_compress.argtypes = [vp, vp, i32, vp, f32, i32, u64p]
+    L.tszp_topo_server_decompress.argtypes = [vp, C.c_uint64, vp, vp, i32, u64p]
+    L.tszp_topo_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(TicketInfo)]
+    L.tszp_topo_server_result.argtypes = [vp, C.c_uint64, C.POINTER(TopoInfo), C.POINTER(RestoreInfo),
+                                          C.POINTER(i32)]
+    L.tszp_topo_server_streams.argtypes = [vp, C.c_uint64, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                           C.POINTER(vp)]
+    L.tszp_topo_server_release.argtypes = [vp, C.c_uint64]
+    L.tszp_topo_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    for f in ("tszp_topo_server_create", "tszp_topo_server_destroy", "tszp_topo_server_compress",
+              "tszp_topo_server_decompress", "tszp_topo_server_wait", "tszp_topo_server_result",
+              "tszp_topo_server_streams", "tszp_topo_server_release", "tszp_topo_server_stats"):
+        getattr(L, f).restype = i32
+    L.tszp_topo_batch_mem_bound.argtypes = [C.c_uint64, u32]
+    L.tszp_topo_batch_mem_bound.restype = C.c_uint64
     L.tszp_batch_plan_create.argtypes = [C.POINTER(BatchField), u32, vp, C.POINTER(vp), vp]
     L.tszp_batch_plan_destroy.argtypes = [vp]
     L.tszp_batch_plan_compress.argtypes = [vp, vp]
@@ -663,6 +681,93 @@ class Server:
     def close(self):
         if self._h:
             self._chk(lib().tszp_server_destroy(self._h))
+            self._h = C.c_void_p()
+
+
+_hip = None
+
+
+def _memcpy_d2d(dst, src, nbytes):
+    """hipMemcpy device to device between raw addresses (the HIP runtime's own call)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    rc = _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), int(nbytes), 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise TszpError(f"hipMemcpy failed ({rc})")
+
+
+class TopoServer:
+    """Request loop of the topology pipeline (tszp_topo_server_*): compress requests of
+    2D / 3D CUDA tensors and decompress (+ restore) requests of finished compress
+    tickets, from any thread; `lanes` batches run on the device at a time.  Inputs must
+    be complete on the device when submitted (the caller synchronises its stream
+    first)."""
+
+    def __init__(self, hbm_budget, max_batch=256, max_batch_elements=1 << 28, lanes=2):
+        self._h = C.c_void_p()
+        self._chk(lib().tszp_topo_server_create(int(hbm_budget), int(max_batch), int(max_batch_elements),
+                                                int(lanes), C.byref(self._h)))
+
+    @staticmethod
+    def _chk(rc):
+        if rc != 0:
+            raise TszpError(lib().tszp_topo_server_last_error().decode())
+
+    def compress(self, field, eb, bs=0):
+        t = C.c_uint64()
+        dims = (C.c_uint32 * 3)(*(list(field.shape) + [1] * (3 - field.dim())))
+        self._chk(lib().tszp_topo_server_compress(self._h, _dp(field), field.dim(), dims, float(eb), int(bs),
+                                                  C.byref(t)))
+        return t.value
+
+    def decompress(self, ticket, out, types=None, restore=True):
+        t = C.c_uint64()
+        self._chk(lib().tszp_topo_server_decompress(self._h, int(ticket), _dp(out),
+                                                    _dp(types) if types is not None else None, int(bool(restore)),
+                                                    C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        info = TicketInfo()
+        self._chk(lib().tszp_topo_server_wait(self._h, int(ticket), C.byref(info)))
+        return info
+
+    def result(self, ticket):
+        """(TopoInfo, RestoreInfo, block size) of a finished ticket."""
+        info, rinfo, bs = TopoInfo(), RestoreInfo(), C.c_int()
+        self._chk(lib().tszp_topo_server_result(self._h, int(ticket), C.byref(info), C.byref(rinfo), C.byref(bs)))
+        return info, rinfo, bs.value
+
+    def streams(self, ticket):
+        """(topology stream, sort-position stream) of a finished compress ticket, copied
+        into new uint8 CUDA tensors."""
+        torch = _torch()
+        info, _, _ = self.result(ticket)
+        dt, dtb, dsp, dspb = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._chk(lib().tszp_topo_server_streams(self._h, int(ticket), C.byref(dt), C.byref(dtb), C.byref(dsp),
+                                                 C.byref(dspb)))
+        out = []
+        for addr, nb in ((dt.value, info.topo_bytes), (dsp.value, info.sortpos_bytes)):
+            o = torch.empty(nb, dtype=torch.uint8, device="cuda")
+            if nb:
+                _memcpy_d2d(o.data_ptr(), addr, nb)
+            out.append(o)
+        return out[0], out[1]
+
+    def release(self, ticket):
+        self._chk(lib().tszp_topo_server_release(self._h, int(ticket)))
+
+    def stats(self):
+        st = ServerStats()
+        self._chk(lib().tszp_topo_server_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            self._chk(lib().tszp_topo_server_destroy(self._h))
             self._h = C.c_void_p()
 
 
