@@ -64,14 +64,20 @@ R.szp_float_decompress_openmp_threadblock_randomaccess_topology_preserved(C.byre
 ext = C.c_uint64.from_address(sps).value
 spd = R.szp_decompress_sort_positions(sps + 8, ext, bs)
 t2 = time.perf_counter()
+if len(sys.argv) > 8:  # the streams and the decoded values, for the checks against the GPU
+    pre = sys.argv[8]
+    open(pre + ".topo", "wb").write(C.string_at(topo, tn.value))
+    open(pre + ".sp", "wb").write(C.string_at(sps, spn.value))
+    np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_float)), (n,)).tofile(pre + ".dec")
 print(json.dumps({"compress_s": t1 - t0, "decompress_s": t2 - t1, "n": n, "cps": cnt.value,
                   "topo_bytes": tn.value, "sp_bytes": spn.value}))
 """
 
 
-def cpu_baseline(field_np, eb, bs, threads, slab):
+def cpu_baseline(field_np, eb, bs, threads, slab, dump=None):
     """The reference's OpenMP CPU path (oracle/_ref/libszp_ref.so, built from the
-    reference sources) on a bounded slab, in a child process."""
+    reference sources) on a bounded slab, in a child process; with dump, a file prefix
+    for its streams and decoded values."""
     import numpy as np
 
     lib = os.path.join(ROOT, "oracle", "_ref", "libszp_ref.so")
@@ -83,7 +89,8 @@ def cpu_baseline(field_np, eb, bs, threads, slab):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
     try:
         r = subprocess.run([sys.executable, "-c", CPU_SCRIPT, lib, path, *map(str, sub.shape), repr(eb),
-                            str(bs)], capture_output=True, text=True, timeout=300, env=env)
+                            str(bs)] + ([dump] if dump else []), capture_output=True, text=True, timeout=300,
+                           env=env)
     finally:
         os.unlink(path)
     if r.returncode != 0:
@@ -91,6 +98,67 @@ def cpu_baseline(field_np, eb, bs, threads, slab):
     res = json.loads(r.stdout.strip().splitlines()[-1])
     res["shape"] = list(sub.shape)
     return res, None
+
+
+def physical_cores():
+    """(physical cores of the host from /proc/cpuinfo, CPUs this process may run on)."""
+    pairs = set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":")[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":")[1].strip()
+            elif not line.strip() and core is not None:
+                pairs.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return (len(pairs) or None), allowed
+
+
+def driver_2d(field_np, eb, bs, threads):
+    """The reference's own 2D CPU drivers (oracle/_ref/ref_topo_{compress,decompress},
+    built from example/testfloat_{compress,decompress}_fastmode1_topology.cc) on a 2D
+    field: their compression window (CP find, sort, sort-position and topology
+    compression) and decompression window (decoders and the driver's post-process), as
+    they print them."""
+    exe_c = os.path.join(ROOT, "oracle", "_ref", "ref_topo_compress")
+    exe_d = os.path.join(ROOT, "oracle", "_ref", "ref_topo_decompress")
+    if not (os.path.exists(exe_c) and os.path.exists(exe_d)):
+        return None
+    rows, cols = field_np.shape
+    path = os.path.join("/tmp", f"tszp_cpu_2d_{os.getpid()}.f32")
+    field_np.astype("<f4").tofile(path)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = {}
+    try:
+        rc = subprocess.run([exe_c, path, str(bs), repr(eb), str(rows), str(cols)], capture_output=True, text=True,
+                            timeout=300, env=env, cwd="/tmp")
+        rd = subprocess.run([exe_d, path + ".SZp", str(rows * cols), str(bs), repr(eb), str(rows), str(cols)],
+                            capture_output=True, text=True, timeout=300, env=env, cwd="/tmp")
+        for line in (rc.stdout + rd.stdout).splitlines():
+            if line.startswith("compression time="):
+                out["compress_s"] = float(line.split("=")[1].split(",")[0])
+            elif line.startswith("decompression time="):
+                out["decompress_s"] = float(line.split("=")[1])
+            elif line.startswith("Max absolute error"):
+                out["max_err_over_eb"] = float(line.split("=")[1]) / eb
+            elif line.startswith("compressionRatio"):
+                out["ratio"] = float(line.split("=")[1])
+    finally:
+        for suf in ("", ".SZp", ".SZp.out"):
+            if os.path.exists(path + suf):
+                os.unlink(path + suf)
+    if "compress_s" not in out or "decompress_s" not in out:
+        return None
+    out["gbs"] = 4.0 * rows * cols / (out["compress_s"] + out["decompress_s"]) / 1e9
+    return out
 
 
 def codec_kernel(mode, bs, ndim, fast):
@@ -679,6 +747,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20251203)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
     ap.add_argument("--cpu-slab", type=int, default=64, help="planes of the field for the CPU baseline")
+    ap.add_argument("--cpu-slab-t1", type=int, default=8, help="planes for the one-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-restore", action="store_true", help="leave out the topology restoration")
     ap.add_argument("--no-elem", action="store_true", help="leave out the threadblock / double side figures")
@@ -895,18 +964,66 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        log(f"reference CPU baseline on a {args.cpu_slab}-plane slab with {threads} threads")
+        import numpy as np
+        phys, allowed = physical_cores()
+        threads = args.cpu_threads or min(allowed, 16)
+        log(f"reference CPU baseline on a {args.cpu_slab}-plane slab with {threads} threads "
+            f"(host: {phys} physical cores, {allowed} CPUs allowed)")
         field_np = f.cpu().numpy()
-        res, err = cpu_baseline(field_np, eb, bs, threads, args.cpu_slab)
+        pre = os.path.join("/tmp", f"tszp_cpu_dump_{os.getpid()}")
+        res, err = cpu_baseline(field_np, eb, bs, threads, args.cpu_slab, dump=pre)
+        checks = {}
         if res:
             t = res["compress_s"] + res["decompress_s"]
             cpu = {"value": 4.0 * res["n"] / t / 1e9, "unit": "GB/s", "cores": threads, "kind": "reference",
                    "sample": f"{res['shape']} slab of the same field, reference OpenMP compress "
                              f"({res['compress_s']:.3f} s) + decompress ({res['decompress_s']:.3f} s) "
-                             f"library calls, no post-process"}
+                             f"library calls (no 3D CPU post-process exists in the reference)",
+                   "host_physical_cores": phys, "host_cpus_allowed": allowed}
+            # the reference's own stream, decoded by the GPU drop-in, against the reference's decode
+            try:
+                topo = np.fromfile(pre + ".topo", dtype=np.uint8)
+                want = np.fromfile(pre + ".dec", dtype=np.float32)
+                t0 = time.perf_counter()
+                got, _ = tz.decompress_topology(topo, res["n"], eb, bs)  # host pointers: the drop-in
+                checks["dropin_decode_s"] = time.perf_counter() - t0
+                checks["decode_matches_reference"] = bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
+            finally:
+                for suf in (".topo", ".sp", ".dec"):
+                    if os.path.exists(pre + suf):
+                        os.unlink(pre + suf)
         else:
             cpu = {"value": None, "unit": "GB/s", "cores": threads, "kind": "reference", "sample": err}
+        # one thread on a smaller slab, and its bytes against the native compressor's
+        log(f"reference CPU path with 1 thread on a {args.cpu_slab_t1}-plane slab")
+        res1, err1 = cpu_baseline(field_np, eb, bs, 1, args.cpu_slab_t1, dump=pre)
+        if res1:
+            t = res1["compress_s"] + res1["decompress_s"]
+            cpu["t1"] = {"value": 4.0 * res1["n"] / t / 1e9, "unit": "GB/s", "cores": 1, "shape": res1["shape"],
+                         "compress_s": res1["compress_s"], "decompress_s": res1["decompress_s"]}
+            try:
+                slab = f[:args.cpu_slab_t1].contiguous()
+                info, buf = tz.topo_compress(slab, eb, bs, T=1)
+                ref_topo = np.fromfile(pre + ".topo", dtype=np.uint8)
+                ref_sp = np.fromfile(pre + ".sp", dtype=np.uint8)
+                checks["t1_streams_match_reference"] = bool(
+                    np.array_equal(buf.topo[:info.topo_bytes].cpu().numpy(), ref_topo) and
+                    np.array_equal(buf.sp[:info.sortpos_bytes].cpu().numpy(), ref_sp))
+                del buf
+            finally:
+                for suf in (".topo", ".sp", ".dec"):
+                    if os.path.exists(pre + suf):
+                        os.unlink(pre + suf)
+        else:
+            cpu["t1"] = {"value": None, "sample": err1}
+        # the reference's 2D drivers, post-process included, on the 1800 x 3600 blob plane
+        plane = tz.gen_blobs([1800, 3600], args.seed, 64, 20.0, 200.0, 0.01, device=dev).cpu().numpy()
+        drv = {}
+        for thr in sorted({threads, 1}, reverse=True):
+            log(f"reference 2D drivers on the 1800x3600 plane with {thr} thread(s)")
+            drv[str(thr)] = driver_2d(plane, 1e-3, 64, thr)
+        cpu["drivers_2d"] = {"field": "1800x3600 blobs + noise, abs eb 1e-3, bs 64", "by_threads": drv}
+        cpu["checks"] = checks
         log(f"cpu baseline: {cpu}")
 
     line = {
