@@ -598,6 +598,55 @@ def serve_side(tz, torch, pool, big, seed, nreq, rate):
     return res
 
 
+def dropin_side(tz, f, eb, bs):
+    """The host-pointer drop-in chain on the 512^3 field, timed like the reference's
+    drivers time their library calls (host buffers in and out, so PCIe transfers and the
+    host-side walks are inside): szp_cuda_find_critical_points_3d ->
+    szp_cuda_sort_critical_points_3d -> szp_cuda_compress_sort_positions_3d ->
+    szp_cuda_float_compress_topology_3d, then
+    szp_cuda_float_decompress_randomaccess_topology_preserved (values and int types) and
+    szp_cuda_decompress_sort_positions.  Library-allocated results are freed untouched."""
+    import ctypes as C
+    import numpy as np
+    L = tz.lib()
+    libc = C.CDLL("libc.so.6")
+    libc.free.argtypes = [C.c_void_p]
+    x = np.ascontiguousarray(f.cpu().numpy())
+    n = x.size
+    d1, d2, d3 = x.shape
+    P = x.ctypes.data_as(C.c_void_p)
+    res = {}
+    for rep in range(2):  # the first run warms the library's device workspace
+        t0 = time.perf_counter()
+        cnt = C.c_size_t()
+        cps = L.szp_cuda_find_critical_points_3d(P, C.byref(cnt), d1, d2, d3, eb)
+        L.szp_cuda_sort_critical_points_3d(C.c_void_p(cps), cnt.value, P, d2, d3)
+        spn = C.c_size_t()
+        sps = L.szp_cuda_compress_sort_positions_3d(C.c_void_p(cps), cnt.value, C.byref(spn), bs)
+        tn = C.c_size_t()
+        topo = L.szp_cuda_float_compress_topology_3d(P, C.byref(tn), eb, n, bs, C.c_void_p(cps), cnt.value, d1, d2,
+                                                      d3)
+        t1 = time.perf_counter()
+        ext = int(np.count_nonzero(np.isin(np.ctypeslib.as_array(
+            C.cast(cps, C.POINTER(C.c_int32)), (cnt.value * 6,))[3::6], (1, 2))))
+        t2 = time.perf_counter()
+        dp, fp = C.c_void_p(), C.c_void_p()
+        L.szp_cuda_float_decompress_randomaccess_topology_preserved(C.byref(dp), n, eb, bs, C.c_void_p(topo),
+                                                                    C.byref(fp))
+        spd = L.szp_cuda_decompress_sort_positions(C.c_void_p(sps + 8), ext, bs)
+        t3 = time.perf_counter()
+        ok = bool(dp.value and fp.value and (spd or ext == 0))
+        for q in (cps, sps, topo, dp.value, fp.value, spd):
+            if q:
+                libc.free(C.c_void_p(q))
+        res = {"compress_s": t1 - t0, "decompress_s": t3 - t2, "ok": ok, "cp_count": cnt.value,
+               "stream_bytes": tn.value + spn.value}
+    res["gbs"] = 4.0 * n / (res["compress_s"] + res["decompress_s"]) / 1e9
+    log(f"host-pointer drop-in chain on {tuple(x.shape)}: compress {res['compress_s'] * 1e3:.1f} ms, decompress "
+        f"{res['decompress_s'] * 1e3:.1f} ms -> {res['gbs']:.2f} GB/s (PCIe and host walks included)")
+    return res
+
+
 def serve_topo_side(tz, torch, big, seed, nreq):
     """Served topology stream: nreq requests, 70 % compressions of a field of the
     mixed-size workload (or with probability 2 % the 512^3 field) and 30 % decompressions
@@ -933,6 +982,9 @@ def main():
                                 args.serve_requests, args.serve_rate)
         del pool
 
+    # side figure: the host-pointer drop-in chain on the same field (PCIe included)
+    dropin = None if args.no_elem else dropin_side(tz, f, eb, bs)
+
     # side figure: the served topology stream (compress / decompress + restore requests)
     served_topo = None
     if not (args.no_batch or args.no_serve):
@@ -1055,6 +1107,7 @@ def main():
         "batched_plain": batch,
         "served_plain": served,
         "served_topo": served_topo,
+        "dropin_host_pointers": dropin,
         "quality_ms": quality_ms,
         "quality": {k: quality[k] for k in ("orig_cp", "preserved", "lost", "changed", "new_cps", "max_err",
                                             "over_1eb", "pairs", "violations", "saddles_above",

@@ -486,11 +486,17 @@ bool walk_blocks(const unsigned char* p, size_t n, int bs, bool topo, size_t lim
 // Chunk count of a stream whose offset table starts at cmp: the first T in
 // 1..128 whose offsets are non-decreasing and bounded and whose blocks walk within
 // `lim` bytes (SZp/include/szp_detect_threads.h:161-309; the offsets of T > 1 must
-// moreover fall on block boundaries).
-unsigned detect_chunks(const unsigned char* cmp, size_t n, int bs, bool topo, size_t lim) {
-  if (rd64(cmp) != 0) return 1;
+// moreover fall on block boundaries).  blk / total receive the accepted walk (block
+// offsets from the payload start, payload bytes), or stay empty when none walks.
+unsigned detect_chunks(const unsigned char* cmp, size_t n, int bs, bool topo, size_t lim, std::vector<uint32_t>& blk,
+                       size_t* total_out) {
   const size_t nb = (n + bs - 1) / bs;
-  std::vector<uint32_t> blk(nb + 1);
+  blk.assign(nb + 1, 0u);
+  *total_out = 0;
+  if (rd64(cmp) != 0) {
+    if (!walk_blocks(cmp + 8, n, bs, topo, lim, blk.data(), total_out)) blk.clear();
+    return 1;
+  }
   for (unsigned t = 1; t <= 128; t++) {
     bool ok = true;
     for (unsigned k = 1; k < t && ok; k++) {
@@ -511,20 +517,24 @@ unsigned detect_chunks(const unsigned char* cmp, size_t n, int bs, bool topo, si
       }
       if (blk[lo] != o) ok = false;
     }
-    if (ok) return t;
+    if (ok) {
+      *total_out = total;
+      return t;
+    }
   }
+  if (!walk_blocks(cmp + 8, n, bs, topo, lim, blk.data(), total_out)) blk.clear();
   return 1;
 }
 
 // Upload a host stream (payload after a T-chunk table) and its block offsets.
 int upload_stream(Ctx* c, const unsigned char* cmp, size_t n, int bs, bool topo, size_t lim, unsigned* T_out,
                   uint8_t** d_payload, uint32_t** d_blk, hipStream_t st) {
-  const unsigned T = detect_chunks(cmp, n, bs, topo, lim);
-  const size_t nb = (n + bs - 1) / bs;
-  std::vector<uint32_t> blk(nb + 1);
+  // one walk: the detection's accepted walk gives the block offsets and the length
+  std::vector<uint32_t> blk;
   size_t total = 0;
-  if (!walk_blocks(cmp + 8 * (size_t)T, n, bs, topo, lim, blk.data(), &total))
-    return fail(TSZP_EINVAL, "corrupt stream (block walk failed)");
+  const unsigned T = detect_chunks(cmp, n, bs, topo, lim, blk, &total);
+  const size_t nb = (n + bs - 1) / bs;
+  if (blk.empty()) return fail(TSZP_EINVAL, "corrupt stream (block walk failed)");
   uint8_t* dp;
   uint32_t* db;
   WS(kWsStream, total + 8, dp);
