@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsCount
+  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsCount
 };
 
 struct Ctx {
@@ -1739,7 +1739,10 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     return v ? (uint32_t)atoi(v) : dflt;
   };
   const float* blur = nullptr;
-  auto mc_pass = [&](uint32_t budget, bool guided, const char* tag) -> int {
+  // plateaus a first search (first steps) does not finish are searched again by replicas
+  // with the whole budget; their staging memory bounds the entries per launch
+  const size_t stage_floats = (size_t)env_u32("TSZP_MC_STAGE_MB", 64u) << 18;
+  auto mc_pass = [&](uint32_t budget, uint32_t first, uint32_t reps, bool guided, const char* tag) -> int {
     if (!exact && (r = full_mark(&nb, false))) return r;
     const uint32_t nl = nb < cap ? nb : cap;
     if (guided && !blur) {  // the smoothed decoded field as the start order
@@ -1758,21 +1761,48 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     FillRanges fr{};
     fr.add(bits, 0u, bit_words);
     fr.add(misc + kPpMcSolved, 0u, 5);
+    fr.add(misc + kPpMcTicks, 0u, 5);
+    fr.add(misc + kPpMcEsc, 0u, 1);
     fr.add(misc + kPpBad, 0u, 1);
     HIPCHK(launch_fill_multi(fr, st));
     // the variants by capacity, each over the same list while plateaus exceeded the last;
     // the claim bits stay, so a plateau is searched by one variant only; the searches
     // list what they leave mismatched into bad2 (exact: the list they read was)
     int rr = 0;
-    uint32_t solved_small = 0;
+    uint32_t solved_small = 0, escalated = 0;
+    const bool use_reps = reps > 1 && first < budget;
+    uint32_t* esc = nullptr;
+    if (use_reps) WS(kWsPpEsc, (size_t)nl * 4 + 16, esc);
     for (int variant = 0; variant < 3; variant++) {
       FillRanges fb{};
       fb.add(misc + kPpMcBig, 0u, 1);
+      fb.add(misc + kPpMcEsc, 0u, 1);
       HIPCHK(launch_fill_multi(fb, st));
-      HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, variant, budget, bad2, cap, stats, misc, st));
+      HIPCHK(pp_platmc(geo, bad, nl, d, d_types, d_field, guide, bits, variant, use_reps ? first : budget, bad2, cap,
+                       stats, misc, esc, st));
       if ((rr = read_misc())) return rr;
       if (variant == 0) solved_small = h[kPpMcSolved];
-      if (!h[kPpMcBig]) break;
+      const uint32_t big = h[kPpMcBig], nesc = use_reps ? h[kPpMcEsc] : 0u;
+      if (nesc) {  // the replicas, in launches that fit the staging memory
+        const size_t per = (size_t)reps * pp_platmc_capacity(variant);
+        const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(nesc, stage_floats / per));
+        float* stage;
+        uint32_t* ctl;
+        WS(kWsPpStage, (size_t)chunk * per * 4, stage);
+        WS(kWsPpCtl, (size_t)chunk * 8 + 16, ctl);
+        for (uint32_t c0 = 0; c0 < nesc; c0 += chunk) {
+          const uint32_t cnt = std::min(chunk, nesc - c0);
+          FillRanges fc{};
+          fc.add(ctl, ~0u, cnt);
+          fc.add(ctl + cnt, 0u, cnt);
+          HIPCHK(launch_fill_multi(fc, st));
+          HIPCHK(pp_platmc_replicas(geo, esc + c0, cnt, d, d_types, d_field, guide, variant, budget, reps, stage, ctl,
+                                    bad2, cap, stats, misc, st));
+        }
+        escalated += nesc;
+        if ((rr = read_misc())) return rr;
+      }
+      if (!big) break;
     }
     const uint32_t before = nb;
     std::swap(bad, bad2);
@@ -1793,11 +1823,19 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     mc_solved += h[kPpMcSolved];
     if (stats)
       fprintf(stderr, "[restore] min-conflicts (%s, %u steps) plateaus solved %u (%u of up to 64 points) failed %u "
-              "beyond the largest variant %u: %u -> %u mismatches; max steps %u, max us %.1f\n", tag, budget, h[kPpMcSolved],
-              solved_small, h[kPpMcFailed], h[kPpMcBig], before, nb, h[kPpMcMaxSteps], h[kPpMcMaxTicks] * 0.01);
+              "beyond the largest variant %u: %u -> %u mismatches; max steps %u, max us %.1f; to the replicas %u\n", tag,
+              budget, h[kPpMcSolved], solved_small, h[kPpMcFailed], h[kPpMcBig], before, nb, h[kPpMcMaxSteps],
+              h[kPpMcMaxTicks] * 0.01, escalated);
+    if (stats && h[kPpMcSteps])
+      fprintf(stderr, "[restore]   %u steps, us per step: draw + records %.2f, candidates %.2f, move %.2f, renorm %.2f\n",
+              h[kPpMcSteps], h[kPpMcTicks] * 0.01 / h[kPpMcSteps], h[kPpMcTicks + 1] * 0.01 / h[kPpMcSteps],
+              h[kPpMcTicks + 2] * 0.01 / h[kPpMcSteps], h[kPpMcTicks + 3] * 0.01 / h[kPpMcSteps]);
     return TSZP_OK;
   };
-  if (nb && mc_on && (r = mc_pass(env_u32("TSZP_MC_BUDGET1", 192u), false, "short"))) return r;
+  if (nb && mc_on &&
+      (r = mc_pass(env_u32("TSZP_MC_BUDGET1", 192u), env_u32("TSZP_MC_FIRST1", 32u), env_u32("TSZP_MC_REPS1", 4u),
+                   false, "short")))
+    return r;
   // window solves: anchors (bit 0) and, behind the plateau solves, only windows that hold
   // their whole plateau (bit 1)
   const int colours = ndim == 2 ? 5 : 7;
@@ -1935,7 +1973,10 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   // window mode; bit 0: anchored extrema)
   if ((r = window_rounds((use_sp ? 1 : 0) | (mc_on ? 2 : 0)))) return r;
   if (!verified && (r = full_mark(&nb, true))) return r;
-  if (nb && mc_on && !stop_mc && (r = mc_pass(env_u32("TSZP_MC_BUDGET2", 30000u), true, "long"))) return r;
+  if (nb && mc_on && !stop_mc &&
+      (r = mc_pass(env_u32("TSZP_MC_BUDGET2", 30000u), env_u32("TSZP_MC_FIRST2", 256u), env_u32("TSZP_MC_REPS2", 8u),
+                   true, "long")))
+    return r;
   // plateaus beyond the plateau solves' capacity: windows over their parts
   if (nb && mc_on && !stop_mc) {
     if ((r = window_rounds(use_sp ? 1 : 0))) return r;

@@ -396,7 +396,7 @@ hipError_t launch_quality(const Grid& g, const float* x, const float* y, uint32_
 
 // ---- topology restoration (tszp_post.hip) ----
 // Control block: uint32 words (kPpMiscWords) shared by the restoration kernels.
-constexpr int kPpMiscWords = 64;
+constexpr int kPpMiscWords = 80;
 enum : int {
   kPpBad = 0, kPpMoves = 1, kPpAct = 2, kPpTiles = 3, kPpVerify = 4, kPpErr = 12, kPpExt = 13, kPpSpMax = 14,
   kPpSolved = 15, kPpFailed = 16, kPpCounter = 17, kPpFreed = 18, kPpPlatAnneal = 19,
@@ -408,7 +408,10 @@ enum : int {
   kPpPlatMaxTicks = 52, kPpPlatMaxAnneal = 53, kPpPlatMaxSweeps = 54,  // whole-plateau solve statistics
   kPpStatSumBuild = 55, kPpStatSumAttempt = 56, kPpStatSumTile = 57,    // window solve: summed ticks
   kPpFlowCounter = 60,  // tile counter of the one-launch solve
-  kPpPlatSolved = 61, kPpPlatFailed = 62, kPpPlatBig = 63  // whole-plateau solves
+  kPpPlatSolved = 61, kPpPlatFailed = 62, kPpPlatBig = 63,  // whole-plateau solves
+  kPpMcTicks = 64,  // min-conflicts statistics: summed ticks of draw + records, candidates, move, renormalisation
+  kPpMcSteps = 68,  // ... and summed steps
+  kPpMcEsc = 72     // plateaus a first search left to the replicas
 };
 // mismatches left after a local round below which the window solve runs at once
 constexpr uint32_t kPpSolveFew = 16384;
@@ -461,10 +464,22 @@ hipError_t pp_plateau(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, cons
 // plateau; relist (cap entries, counter misc[kPpBad]): every point of a searched
 // plateau that is mismatched afterwards, and with variant 2 the listed mismatches of
 // plateaus beyond it -- with all variants run over an exact list, relist is exact;
-// stats: solved / failed counts and the largest step count and time in misc
+// stats: solved / failed counts and the largest step count and time in misc; esc
+// (nullable, nbad entries): a plateau the budget does not finish is not relisted but
+// its smallest point goes to esc (count misc[kPpMcEsc]), once per plateau
 hipError_t pp_platmc(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
                      const float* guide, uint32_t* claim, int variant, uint32_t budget, uint32_t* relist,
-                     uint32_t cap, bool stats, uint32_t* misc, hipStream_t st);
+                     uint32_t cap, bool stats, uint32_t* misc, uint32_t* esc, hipStream_t st);
+uint32_t pp_platmc_capacity(int variant);  // points of the variant's largest plateau
+// the plateaus of ent (smallest points from pp_platmc's esc, searched there by the same
+// variant) searched again by R (1..8) replicas each, budget steps, the first to finish
+// -- fewest steps, then lowest replica -- writing the values; stage: nent * R *
+// pp_platmc_capacity(variant) floats; ctl: 2 * nent words, the first nent ~0u and the
+// others 0; what is left mismatched goes to relist as with pp_platmc
+hipError_t pp_platmc_replicas(const PpGeo& geo, const uint32_t* ent, uint32_t nent, const float* d, const uint8_t* t,
+                              float* y, const float* guide, int variant, uint32_t budget, uint32_t R, float* stage,
+                              uint32_t* ctl, uint32_t* relist, uint32_t cap, bool stats, uint32_t* misc,
+                              hipStream_t st);
 // window solves (one launch per parity phase, or every phase in one launch below);
 // anchor: bit 0 keeps the anchored extrema in place, bit 1 solves only windows that hold
 // their whole plateau (the larger ones are left to the plateau solves)

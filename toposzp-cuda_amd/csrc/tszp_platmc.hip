@@ -38,8 +38,10 @@
 namespace tszp {
 namespace {
 
-constexpr int kMcPos = 19;              // candidate positions per mover
+constexpr int kMcGapPos = 3;            // positions inside each gap between neighbour keys
+constexpr int kMcPos = 4 + 5 * kMcGapPos;  // far / just below, the gaps, just / far above
 constexpr int kMcSlots = 7 * kMcPos;    // the mismatched point and up to 6 plateau neighbours
+constexpr int kMcSlotsPerLane = (kMcSlots + 63) / 64;
 constexpr uint32_t kMcWalkPct = 15;     // share of random moves, per cent
 constexpr uint32_t kMcTabu = 10;        // steps during which a moved point loses ties
 constexpr int16_t kNbLow = -1, kNbHigh = -2, kNbOut = -3;
@@ -61,14 +63,17 @@ struct McShared {
   uint16_t hval[kHash];
   uint32_t cand[kWave];
   // per step: each mover's neighbour records and its own (see mc_nbr_rec / mc_mover_rec)
+  uint64_t vbits[(PM + 63) / 64];  // violated types, one bit per point (drawn from uniformly)
   uint64_t tbl[2 * ND + 1][2 * ND];
   uint64_t mrec[2 * ND + 1][2];
+  float srt[2 * ND + 1][2 * ND + 1];  // each mover's neighbour keys in order, then the largest
   int ne, na, nv, abort;
   float lo, hi, db;
 };
 // renormalisation period of the keys (steps): repeated moves next to a neighbour's key
 // crowd the keys, and float positions between crowded keys degenerate into ties
 constexpr uint32_t kMcRenorm = 128;
+constexpr uint32_t kMcPoll = 16;  // steps between a replica's looks at its entry's best finish
 static_assert(sizeof(McShared<3, kMcLarge>) <= 160 * 1024, "LDS of the large variant");
 
 __device__ __forceinline__ uint64_t mc_mix(uint64_t h) {
@@ -194,63 +199,49 @@ __device__ __forceinline__ int mc_viol(const SH& sh, int i, int m, float km) {
 
 __device__ __forceinline__ float mc_near(float x) { return fmaxf(fabsf(x), 1.f) * 9.5367432e-7f; }  // 2^-20
 
-// value i of a register array of finite values, as a sum of products with one-hot
-// weights (a select chain is turned back into indexed private memory by the compiler);
-// exact: every other term is zero
-template <int N>
-__device__ __forceinline__ float mc_sel(const float (&a)[N], int i) {
-  float v = 0.f;
-#pragma unroll
-  for (int j = 0; j < N; j++) v += a[j] * (float)(i == j);
-  return v;
-}
 constexpr float kMcNone = 1e30f;  // key of a missing neighbour (sorts last, finite)
+
+// minimum over the wave, wave-uniform (DPP within rows of 16, then the four row results)
+__device__ __forceinline__ uint32_t mc_wave_min(uint32_t v) {
+  v = min(v, dpp_mov<kDppQuad1032>(v));
+  v = min(v, dpp_mov<kDppQuad2301>(v));
+  v = min(v, dpp_mov<kDppRowHalfMirror>(v));
+  v = min(v, dpp_mov<kDppRowMirror>(v));
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16),
+                 c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  return min(min(a, b), min(c, d));
+}
+
+// position of the k-th set bit of w (k < popcount(w)), by halving
+__device__ __forceinline__ int mc_kth_bit(uint64_t w, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int half = 32; half > 0; half >>= 1) {
+    const uint64_t lo = w & ((1ull << half) - 1ull);
+    const int c = __popcll(lo);
+    if (k >= c) {
+      k -= c;
+      w >>= half;
+      pos += half;
+    } else {
+      w = lo;
+    }
+  }
+  return pos;
+}
 
 // Per step, the movers are the drawn point (mover 0) and its plateau neighbours (mover
 // 1 + k, direction k).  Neighbour record of mover mi in direction k, 64 bits:
 //   bits 0-1 kind (0 plateau point, 1 lower cell, 2 higher cell, 3 off the grid),
-//   2-7 the point's lower-neighbour flags without the direction back to the mover,
-//   8-10 target type + 1 (0: unconstrained), 11 violated now, 16-31 plateau index,
-//   32-63 key.
+//   2 the point's type violated with the mover below it, 3 with the mover above it,
+//   4 violated now, 16-31 plateau index, 32-63 key.
 // Mover record: [0] bits 0-15 index, 16-18 target + 1, 19 violated, 20 valid, 21 has
-// an anchor below, 22 one above, 23 moved within the tabu window, 32-63 key;
-// [1] keys of the anchors below (low half) and above (high half).
+// an anchor below, 22 one above, 23 moved within the tabu window, 24-29 directions of
+// its neighbours in lower cells, 32-63 key; [1] keys of the anchors below (low half) and
+// above (high half).  A mover's candidate positions then cost one compare per
+// neighbour: the neighbours' own types are looked up, not recomputed.
 __device__ __forceinline__ float mc_hi_f(uint64_t v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
-// candidate position pos of a mover whose plateau neighbours' keys are nk (sorted, n
-// of them), clamped into its anchor gap; false when the position does not exist
-template <int K>
-__device__ __forceinline__ bool mc_position(const float (&nk)[K], int n, int pos, uint64_t mr0, uint64_t mr1,
-                                            float far, float* out) {
-  if (n == 0) return false;
-  const float first = nk[0], last = mc_sel(nk, n - 1);
-  float c;
-  if (pos == 0) c = first - far;
-  else if (pos == 1) c = first - mc_near(first);
-  else if (pos == kMcPos - 2) c = last + mc_near(last);
-  else if (pos == kMcPos - 1) c = last + far;
-  else {
-    const int a = (pos - 2) / 3, w = (pos - 2) % 3;
-    if (a + 1 >= n) return false;
-    const float lo = mc_sel(nk, a), hi = mc_sel(nk, a + 1);
-    if (!(hi > lo)) return false;
-    const float span = hi - lo;
-    c = w == 0 ? lo + span * (1.f / 1024.f) : (w == 1 ? lo + 0.5f * span : hi - span * (1.f / 1024.f));
-  }
-  const bool has_lo = (mr0 >> 21) & 1u, has_hi = (mr0 >> 22) & 1u;
-  if (has_lo || has_hi) {  // an anchored point stays strictly between its neighbours in anchor order
-    const float alo = __uint_as_float((uint32_t)mr1), ahi = mc_hi_f(mr1);
-    if (has_lo && !(c > alo)) c = has_hi ? alo + (ahi - alo) * (1.f / 1024.f) : alo + mc_near(alo);
-    if (has_hi && !(c < ahi)) c = has_lo ? ahi - (ahi - alo) * (1.f / 1024.f) : ahi - mc_near(ahi);
-    if ((has_lo && !(c > alo)) || (has_hi && !(c < ahi))) return false;
-  }
-  *out = c;
-  return true;
-}
-
-// lists the plateau's points whose classification from y is not their type (one
-// atomic per wave chunk that has any); outside the plateau only other cells' values are
-// read, and those stay in their cells whatever other wavefronts write
 template <int ND, class SH>
 __device__ void mc_relist(const SH& sh, const FieldView& F, const float* y, const uint8_t* __restrict__ t,
                           uint32_t* __restrict__ relist, uint32_t cap, uint32_t* misc, int lane) {
@@ -277,23 +268,60 @@ __device__ void mc_relist(const SH& sh, const FieldView& F, const float* y, cons
   }
 }
 
+// End of a replica (every replica of an entry calls it once, every lane): the last one to
+// finish writes the values of the winner -- the fewest steps, then the lowest replica --
+// from its staging row (ctl[ent]: (steps << 3) | replica of the best finish, ~0 if none;
+// ctl[nent + ent]: finished replicas) and lists what they leave mismatched, or, with no
+// winner, the plateau's mismatches as they are.
+template <int ND, int PM, class SH>
+__device__ void mc_replica_end(SH& sh, const FieldView& F, float* y, const uint8_t* __restrict__ t,
+                               const float* stage, uint32_t* ctl, uint32_t nent, uint32_t ent, uint32_t R,
+                               uint32_t* __restrict__ relist, uint32_t cap, bool stats, uint32_t* misc, int lane) {
+  __threadfence();  // this replica's staged values before its count
+  __syncthreads();
+  if (lane == 0) sh.cand[0] = atomicAdd(&ctl[nent + ent], 1u) == R - 1u ? 1u : 0u;
+  __syncthreads();
+  if (!sh.cand[0]) return;
+  __threadfence();  // the other replicas' staged values after their counts
+  const uint32_t best = __hip_atomic_load(&ctl[ent], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (best != ~0u) {
+    const float* src = stage + ((size_t)ent * R + (best & 7u)) * PM;
+    for (int i = lane; i < sh.ne; i += kWave) y[sh.el[i]] = src[i];
+    __syncthreads();
+  }
+  if (lane == 0 && stats) atomicAdd(&misc[best != ~0u ? kPpMcSolved : kPpMcFailed], 1u);
+  mc_relist<ND>(sh, F, y, t, relist, cap, misc, lane);
+}
+
+// R == 0: one search per plateau around each listed mismatch (claimed by the plateau's
+// smallest point); a plateau the budget does not finish goes to esc (its smallest point,
+// one entry per plateau) when esc is given, else its mismatches to relist.
+// R >= 1: entry bad[i] (a smallest point from esc) is searched by R replicas, blocks
+// i * R + r, each from the same start keys with a random stream of its own; a replica
+// stops once another has finished in fewer steps (polled every kMcPoll steps), a
+// finished one stages its values (stage: R * PM floats per entry), and the last replica
+// of an entry writes the winner's values (mc_replica_end): the result depends on the
+// plateau alone, not on the replicas' timing.
 template <int ND, class GEO, int PM>
 __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
                                                      const float* __restrict__ d, const uint8_t* __restrict__ t,
                                                      float* __restrict__ y, const float* __restrict__ guide,
                                                      uint32_t* __restrict__ claim, uint32_t budget, bool last,
                                                      uint32_t* __restrict__ relist, uint32_t cap, bool stats,
-                                                     uint32_t* misc) {
+                                                     uint32_t* misc, uint32_t R, uint32_t* __restrict__ esc,
+                                                     float* __restrict__ stage, uint32_t* __restrict__ ctl) {
   constexpr int K = 2 * ND;
   using SH = McShared<ND, PM>;
   constexpr int H = SH::kHash;
   __shared__ SH sh;
   const int lane = threadIdx.x;
-  if (blockIdx.x >= nbad) return;
-  const uint32_t b = bad[blockIdx.x];
+  const uint32_t ent = R ? blockIdx.x / R : blockIdx.x;
+  const uint32_t rep = R ? blockIdx.x % R : 0u;
+  if (ent >= nbad) return;
+  const uint32_t b = bad[ent];
   const FieldView F = geo.at(b);
   const Grid& g = F.g;
-  {
+  if (!R) {
     int c[ND];
     coords_of<ND>(g, b - F.base, c);
     if (classify_at<ND>(g, y, b, c) == (int)t[b]) return;  // solved with another plateau
@@ -368,6 +396,16 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     head = ne0;
   }
   if (sh.abort) {
+    // (a replica's entry was searched by the first launch of the same variant, so it
+    // never exceeds the variant; its replicas still count themselves out)
+    if (R) {
+      __syncthreads();
+      if (lane == 0 && atomicAdd(&ctl[nbad + ent], 1u) == R - 1u) {
+        const uint32_t pos = atomicAdd(&misc[kPpBad], 1u);
+        if (pos < cap) relist[pos] = b;
+      }
+      return;
+    }
     if (lane == 0) {
       atomicAdd(&misc[kPpMcBig], 1u);
       if (last) {  // beyond every variant: the mismatch stays listed
@@ -378,14 +416,15 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     return;
   }
   const int ne = sh.ne;
-  // claimed by its smallest point: one wavefront per plateau
-  uint32_t mn = 0xffffffffu;
-  for (int i = lane; i < ne; i += kWave) mn = min(mn, sh.el[i]);
+  if (!R) {  // claimed by its smallest point: one wavefront per plateau
+    uint32_t mn = 0xffffffffu;
+    for (int i = lane; i < ne; i += kWave) mn = min(mn, sh.el[i]);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, kWave));
-  if (lane == 0) sh.cand[0] = claim_bit(claim, mn) ? 1u : 0u;
-  __syncthreads();
-  if (!sh.cand[0]) return;
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, kWave));
+    if (lane == 0) sh.cand[0] = claim_bit(claim, mn) ? 1u : 0u;
+    __syncthreads();
+    if (!sh.cand[0]) return;
+  }
   const uint64_t t_start = wall_clock64();
   // ---- elements in index order (the hash table is no longer needed) -----------------
   int p2 = 1;
@@ -526,31 +565,32 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     }
     __syncthreads();
   }
-  // ---- violated types ---------------------------------------------------------------------
+  // ---- violated types: a byte per point for the records, a bit per point for the draws ----
+  static_assert((PM + 63) / 64 <= kWave, "one bitmap word per lane");
+  const int nwords = (ne + 63) / 64;
   int nv = 0;
   for (int i0 = 0; i0 < ne; i0 += kWave) {
     const int i = i0 + lane;
     const int v = i < ne ? mc_viol<ND>(sh, i, -1, 0.f) : 0;
-    if (i < ne) {
-      sh.vio[i] = (uint8_t)v;
-      sh.vpos[i] = 0xffff;
-    }
+    if (i < ne) sh.vio[i] = (uint8_t)v;
     const uint64_t m = __ballot(v != 0);
-    if (v) {
-      const int pos = nv + __popcll(m & lanemask_lt(lane));
-      sh.vlist[pos] = (uint16_t)i;
-      sh.vpos[i] = (uint16_t)pos;
-    }
+    if (lane == 0) sh.vbits[i0 / 64] = m;
     nv += __popcll(m);
   }
-  if (lane == 0) sh.nv = nv;
   __syncthreads();
   // ---- min-conflicts --------------------------------------------------------------------
-  const uint64_t seed = mc_mix((uint64_t)sh.el[0] ^ 0x5851f42d4c957f2dull);
+  const uint64_t seed = mc_mix((uint64_t)sh.el[0] ^ 0x5851f42d4c957f2dull ^ ((uint64_t)(R ? rep + 1u : 0u) << 40));
   uint32_t step = 0;
+  uint32_t polled = ~0u;  // the entry's best finish, loaded at the previous poll
   float far = 0.25f;  // the start keys span about one error bound
   uint64_t* sk = reinterpret_cast<uint64_t*>(sh.hkey);  // p2 <= kHash / 2 entries
+  uint64_t tk[4] = {0, 0, 0, 0};  // statistics: ticks per phase
   for (; step < budget && nv > 0; step++) {
+    if (R && step % kMcPoll == 0) {  // another replica finished in fewer steps: stop
+      if (step > (__builtin_amdgcn_readfirstlane(polled) >> 3)) break;
+      polled = __hip_atomic_load(&ctl[ent], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint64_t tp = stats ? wall_clock64() : 0;
     if (step > 0 && step % kMcRenorm == 0) {  // keys := ranks (the order is unchanged)
       for (int i = lane; i < p2; i += kWave) sk[i] = i < ne ? mc_sortkey(sh.key[i], i) : ~0ull;
       __syncthreads();
@@ -558,9 +598,28 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
       for (int rk = lane; rk < ne; rk += kWave) sh.key[(uint32_t)sk[rk]] = (float)rk;
       __syncthreads();
       far = 0.25f * (float)ne;
+      if (stats) {
+        const uint64_t t2 = wall_clock64();
+        tk[3] += t2 - tp;
+        tp = t2;
+      }
     }
     const uint64_t r = mc_mix(seed + (uint64_t)step * 0x9e3779b97f4a7c15ull);
-    const int p = sh.vlist[(uint32_t)(r >> 32) % (uint32_t)nv];
+    // a violated point drawn uniformly: the k-th set bit of the bitmap (word by a prefix
+    // sum of the words' counts, bit by halving)
+    int p;
+    {
+      const uint64_t w = lane < nwords ? sh.vbits[lane] : 0ull;
+      const uint32_t cnt = (uint32_t)__popcll(w);
+      const uint32_t incl = wave_incl_scan_dpp(cnt, lane);
+      const uint32_t k = (uint32_t)(r >> 32) % (uint32_t)nv;
+      const uint64_t hit = __ballot(incl > k);  // words whose prefix passes k
+      const int wi = __ffsll((unsigned long long)hit) - 1;
+      const uint64_t word = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), wi) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w, wi);
+      const uint32_t before = (uint32_t)__builtin_amdgcn_readlane((int)(incl - cnt), wi);
+      p = wi * 64 + mc_kth_bit(word, (int)(k - before));
+    }
     const bool walk = (uint32_t)(r & 0xffffu) % 100u < kMcWalkPct;
     // ---- records: lane mi * K + k, mover mi, direction k --------------------------------
     if (lane < (K + 1) * K) {
@@ -580,8 +639,12 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
             if (c2 >= 0) l = mc_before(sh.key[c2], c2, kq, c);
             base |= (uint32_t)l << k2;
           }
-          rec = ((uint64_t)__float_as_uint(kq) << 32) | ((uint32_t)c << 16) | ((uint32_t)sh.vio[c] << 11) |
-                ((uint32_t)(sh.tgt[c] + 1) << 8) | (base << 2);
+          // the neighbour's violation with the mover below it and above it
+          const int tq = sh.tgt[c];
+          const uint32_t vlo = tq >= 0 && mc_type<ND>(base | (1u << (k ^ 1))) != tq;
+          const uint32_t vhi = tq >= 0 && mc_type<ND>(base) != tq;
+          rec = ((uint64_t)__float_as_uint(kq) << 32) | ((uint32_t)c << 16) | ((uint32_t)sh.vio[c] << 4) |
+                (vhi << 3) | (vlo << 2);
         } else {
           rec = c == kNbLow ? 1u : (c == kNbHigh ? 2u : 3u);
         }
@@ -602,34 +665,20 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
       }
     }
     __syncthreads();
-    // ---- candidates: lane + 64 j is slot (mover, position) ----------------------------
-    uint64_t best = ~0ull;
-    float cands[3];
-    uint32_t newv[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      cands[j] = 0.f;
-      newv[j] = 0;
-      const int s = lane + j * kWave;
-      if (s >= kMcSlots) continue;
-      const int mi = s / kMcPos, pos = s % kMcPos;
-      if (mi > K) continue;  // 2D: four neighbours
-      const uint64_t mr0 = sh.mrec[mi][0];
-      if (!((mr0 >> 20) & 1u)) continue;
-      const uint64_t mr1 = sh.mrec[mi][1];
-      uint64_t rec[K];
+    // each mover's plateau-neighbour keys in order (lane mi), its last one, and the
+    // directions of its neighbours in lower cells
+    if (lane <= K && ((sh.mrec[lane][0] >> 20) & 1u)) {
       float nk[K];
       uint32_t fixed_low = 0;
+      float last = -kMcNone;
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        rec[k] = sh.tbl[mi][k];
-        const uint32_t kind = (uint32_t)rec[k] & 3u;
-        nk[k] = kind == 0 ? mc_hi_f(rec[k]) : kMcNone;
+        const uint64_t rec = sh.tbl[lane][k];
+        const uint32_t kind = (uint32_t)rec & 3u;
+        nk[k] = kind == 0 ? mc_hi_f(rec) : kMcNone;
+        last = kind == 0 ? fmaxf(last, nk[k]) : last;
         fixed_low |= (uint32_t)(kind == 1) << k;
       }
-      int n = 0;
-#pragma unroll
-      for (int k = 0; k < K; k++) n += ((uint32_t)rec[k] & 3u) == 0;
 #pragma unroll
       for (int pass = 0; pass < K; pass++)  // odd-even transposition sort
 #pragma unroll
@@ -638,86 +687,153 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
           nk[i] = lo;
           nk[i + 1] = hi;
         }
+#pragma unroll
+      for (int k = 0; k < K; k++) sh.srt[lane][k] = nk[k];
+      sh.srt[lane][K] = last;
+      sh.mrec[lane][0] |= (uint64_t)fixed_low << 24;
+    }
+    __syncthreads();
+    if (stats) {
+      const uint64_t t2 = wall_clock64();
+      tk[0] += t2 - tp;
+      tp = t2;
+    }
+    // ---- candidates: lane + 64 j is slot (mover, position) ----------------------------
+    static_assert(kMcSlots <= 256, "slot in the low byte of the key");
+    uint32_t best = ~0u;
+    float cands[kMcSlotsPerLane];
+    uint32_t newv[kMcSlotsPerLane];
+#pragma unroll
+    for (int j = 0; j < kMcSlotsPerLane; j++) {
+      cands[j] = 0.f;
+      newv[j] = 0;
+      const int s = lane + j * kWave;
+      const int mi = s / kMcPos, pos = s % kMcPos;
+      if (mi > K) continue;
+      const uint64_t mr0 = sh.mrec[mi][0];
+      if (!((mr0 >> 20) & 1u)) continue;
+      const float first = sh.srt[mi][0];
+      if (!(first < kMcNone)) continue;  // no plateau neighbour
       float c;
-      if (!mc_position<K>(nk, n, pos, mr0, mr1, far, &c)) continue;
+      if (pos == 0) c = first - far;
+      else if (pos == 1) c = first - mc_near(first);
+      else if (pos >= kMcPos - 2) {
+        const float last = sh.srt[mi][K];
+        c = pos == kMcPos - 2 ? last + mc_near(last) : last + far;
+      } else {
+        const int a = (pos - 2) / kMcGapPos, w = (pos - 2) % kMcGapPos;
+        if (a + 1 >= K) continue;  // 2D: three gaps
+        const float lo = sh.srt[mi][a], hi = sh.srt[mi][a + 1];
+        if (!(hi < kMcNone) || !(hi > lo)) continue;
+        const float span = hi - lo;
+        c = w == 0 ? lo + span * (1.f / 1024.f)
+                   : (w == kMcGapPos - 1 ? hi - span * (1.f / 1024.f) : lo + 0.5f * span);
+      }
+      if ((mr0 >> 21) & 3u) {  // an anchored point stays strictly between its neighbours in anchor order
+        const uint64_t mr1 = sh.mrec[mi][1];
+        const bool has_lo = (mr0 >> 21) & 1u, has_hi = (mr0 >> 22) & 1u;
+        const float alo = __uint_as_float((uint32_t)mr1), ahi = mc_hi_f(mr1);
+        if (has_lo && !(c > alo)) c = has_hi ? alo + (ahi - alo) * (1.f / 1024.f) : alo + mc_near(alo);
+        if (has_hi && !(c < ahi)) c = has_lo ? ahi - (ahi - alo) * (1.f / 1024.f) : ahi - mc_near(ahi);
+        if ((has_lo && !(c > alo)) || (has_hi && !(c < ahi))) continue;
+      }
       cands[j] = c;
       const int m = (int)(mr0 & 0xffffu);
       // the mover's own type, then each plateau neighbour's, with the mover at c
-      uint32_t low_m = fixed_low, nv_bits = 0;
+      uint32_t low_m = (uint32_t)(mr0 >> 24) & 63u, nv_bits = 0;
       int dl = 0;
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        if (((uint32_t)rec[k] & 3u) != 0) continue;
-        const int q = (int)(((uint32_t)rec[k] >> 16) & 0xffffu);
-        const float kq = mc_hi_f(rec[k]);
-        const bool q_low = mc_before(kq, q, c, m);
+        const uint64_t rec = sh.tbl[mi][k];
+        if (((uint32_t)rec & 3u) != 0) continue;
+        const int q = (int)(((uint32_t)rec >> 16) & 0xffffu);
+        const bool q_low = mc_before(mc_hi_f(rec), q, c, m);
         low_m |= (uint32_t)q_low << k;
-        const uint32_t low_q = (((uint32_t)rec[k] >> 2) & 63u) | ((uint32_t)!q_low << (k ^ 1));
-        const int tq = (int)(((uint32_t)rec[k] >> 8) & 7u) - 1;
-        const int vq = tq >= 0 && mc_type<ND>(low_q) != tq;
+        const int vq = (int)(((uint32_t)rec >> (q_low ? 3 : 2)) & 1u);
         nv_bits |= (uint32_t)vq << (1 + k);
-        dl += vq - (int)(((uint32_t)rec[k] >> 11) & 1u);
+        dl += vq - (int)(((uint32_t)rec >> 4) & 1u);
       }
       const int tm = (int)((mr0 >> 16) & 7u) - 1;
       const int vm = tm >= 0 && mc_type<ND>(low_m) != tm;
       nv_bits |= (uint32_t)vm;
       dl += vm - (int)((mr0 >> 19) & 1u);
       newv[j] = nv_bits;
-      const uint32_t rs = (uint32_t)mc_mix(r ^ ((uint64_t)s << 40));
-      uint32_t score;
-      if (walk) score = rs >> 8;
-      else score = ((uint32_t)(dl + 16) << 24) | ((uint32_t)((mr0 >> 23) & 1u) << 23) | (rs >> 9);
-      const uint64_t pk = ((uint64_t)score << 32) | (uint32_t)s;
+      uint32_t rs = (uint32_t)r ^ ((uint32_t)s * 0x9e3779b1u);  // tie-break
+      rs ^= rs >> 16;
+      rs *= 0x7feb352du;
+      rs ^= rs >> 15;
+      // 32-bit key: (violation change, tabu, random) or random alone, then the slot
+      const uint32_t pk = walk ? ((rs >> 8) << 8) | (uint32_t)s
+                               : ((uint32_t)(dl + 16) << 27) | ((uint32_t)((mr0 >> 23) & 1u) << 26) |
+                                     ((rs >> 14) << 8) | (uint32_t)s;
       best = pk < best ? pk : best;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t other = (uint64_t)__shfl_xor((long long)best, o, kWave);
-      best = other < best ? other : best;
+    best = mc_wave_min(best);
+    if (stats) {
+      const uint64_t t2 = wall_clock64();
+      tk[1] += t2 - tp;
+      tp = t2;
     }
-    if (best == ~0ull) continue;  // nothing can move (every mover pinned between anchors)
-    const int s = (int)(uint32_t)best;
+    if (best == ~0u) continue;  // nothing can move (every mover pinned between anchors)
+    const int s = (int)(best & 0xffu);
     const int js = s / kWave, ms = s / kMcPos;
-    const float c = __shfl(js == 0 ? cands[0] : (js == 1 ? cands[1] : cands[2]), s % kWave, kWave);
-    const uint32_t vbits = (uint32_t)__shfl((int)(js == 0 ? newv[0] : (js == 1 ? newv[1] : newv[2])), s % kWave, kWave);
-    // ---- the move, then the violated list (lane 0) -------------------------------------
-    if (lane == 0) {
+    float cw = cands[0];
+    uint32_t vw = newv[0];
+#pragma unroll
+    for (int j = 1; j < kMcSlotsPerLane; j++) {
+      cw = js == j ? cands[j] : cw;
+      vw = js == j ? newv[j] : vw;
+    }
+    const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cw), s % kWave));
+    const uint32_t vbits = (uint32_t)__builtin_amdgcn_readlane((int)vw, s % kWave);
+    // ---- the move; lane 0 updates the mover, lane 1 + k its neighbour k ----------------
+    if (lane <= K) {
       const int m = (int)(sh.mrec[ms][0] & 0xffffu);
-      sh.key[m] = c;
-      sh.tabu[m] = (uint16_t)(step + 1u);
-      for (int k = -1; k < K; k++) {
-        int q = m;
-        if (k >= 0) {
-          const uint64_t rec = sh.tbl[ms][k];
-          if (((uint32_t)rec & 3u) != 0) continue;
-          q = (int)(((uint32_t)rec >> 16) & 0xffffu);
-        }
-        const int vq = (vbits >> (k + 1)) & 1u;
-        if (vq == (int)sh.vio[q]) continue;
-        sh.vio[q] = (uint8_t)vq;
-        if (vq) {
-          sh.vpos[q] = (uint16_t)sh.nv;
-          sh.vlist[sh.nv++] = (uint16_t)q;
-        } else {
-          const int last = sh.vlist[--sh.nv];
-          const int at = sh.vpos[q];
-          sh.vlist[at] = (uint16_t)last;
-          sh.vpos[last] = (uint16_t)at;
-          sh.vpos[q] = 0xffff;
+      int q = m;
+      if (lane > 0) {
+        const uint64_t rec = sh.tbl[ms][lane - 1];
+        q = ((uint32_t)rec & 3u) == 0 ? (int)(((uint32_t)rec >> 16) & 0xffffu) : -1;
+      }
+      if (lane == 0) {
+        sh.key[m] = c;
+        sh.tabu[m] = (uint16_t)(step + 1u);
+      }
+      if (q >= 0) {
+        const int vq = (vbits >> lane) & 1u;
+        if (vq != (int)sh.vio[q]) {
+          sh.vio[q] = (uint8_t)vq;
+          const unsigned long long bit = 1ull << (q & 63);
+          if (vq) atomicOr(reinterpret_cast<unsigned long long*>(&sh.vbits[q >> 6]), bit);
+          else atomicAnd(reinterpret_cast<unsigned long long*>(&sh.vbits[q >> 6]), ~bit);
         }
       }
     }
     __syncthreads();
-    nv = sh.nv;
+    {
+      const uint64_t w = lane < nwords ? sh.vbits[lane] : 0ull;
+      nv = (int)wave_sum_dpp((uint32_t)__popcll(w));
+    }
+    if (stats) tk[2] += wall_clock64() - tp;
   }
   // (statistics only on request: one same-address atomic per plateau serialises)
   if (lane == 0 && stats) {
+    for (int k = 0; k < 4; k++) atomicAdd(&misc[kPpMcTicks + k], (uint32_t)tk[k]);
+    atomicAdd(&misc[kPpMcSteps], step);
     atomicMax(&misc[kPpMcMaxSteps], step);
     atomicMax(&misc[kPpMcMaxTicks], (uint32_t)min<uint64_t>(wall_clock64() - t_start, 0xffffffffull));
-    if (nv > 0) atomicAdd(&misc[kPpMcFailed], 1u);
+    if (nv > 0 && !R && !esc) atomicAdd(&misc[kPpMcFailed], 1u);
   }
-  if (nv > 0) {  // not finished: the plateau keeps its values; its mismatches are listed again
-    mc_relist<ND>(sh, F, y, t, relist, cap, misc, lane);
+  if (nv > 0) {  // not finished: the plateau keeps its values
+    if (R) {
+      mc_replica_end<ND, PM>(sh, F, y, t, stage, ctl, nbad, ent, R, relist, cap, stats, misc, lane);
+    } else if (esc) {  // to the replicas
+      if (lane == 0) {
+        const uint32_t pos = atomicAdd(&misc[kPpMcEsc], 1u);
+        if (pos < nbad) esc[pos] = sh.el[0];
+      }
+    } else {  // its mismatches are listed again
+      mc_relist<ND>(sh, F, y, t, relist, cap, misc, lane);
+    }
     return;
   }
   // ---- order by key, then values ----------------------------------------------------------
@@ -744,9 +860,16 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
         s0 = r + 1;
       }
     }
-    if (stats) atomicAdd(&misc[kPpMcSolved], 1u);
+    if (stats && !R) atomicAdd(&misc[kPpMcSolved], 1u);
   }
   __syncthreads();
+  if (R) {  // staged; the entry's last replica writes the winner's
+    float* dst = stage + ((size_t)ent * R + rep) * PM;
+    for (int i = lane; i < ne; i += kWave) dst[i] = sh.key[i];
+    if (lane == 0) atomicMin(&ctl[ent], (step << 3) | rep);
+    mc_replica_end<ND, PM>(sh, F, y, t, stage, ctl, nbad, ent, R, relist, cap, stats, misc, lane);
+    return;
+  }
   for (int i = lane; i < ne; i += kWave) y[sh.el[i]] = sh.key[i];
   __syncthreads();
   // what the written values leave mismatched (equal anchor values with points between
@@ -761,26 +884,47 @@ __global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __
     if (geo.fields) {                                                                                            \
       const GeoB gb_ = geob(geo);                                                                                \
       if (geo.ndim == 2)                                                                                         \
-        hipLaunchKernelGGL((k_pp_platmc<2, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
+        hipLaunchKernelGGL((k_pp_platmc<2, GeoB, PMV>), dim3(grid), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc, R, esc, stage, ctl);                                                          \
       else                                                                                                       \
-        hipLaunchKernelGGL((k_pp_platmc<3, GeoB, PMV>), dim3(nbad), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
+        hipLaunchKernelGGL((k_pp_platmc<3, GeoB, PMV>), dim3(grid), dim3(kWave), 0, st, gb_, bad, nbad, d, t, y,   \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc, R, esc, stage, ctl);                                                          \
     } else {                                                                                                     \
       const Geo1 g1_ = geo1(geo);                                                                                \
       if (geo.ndim == 2)                                                                                         \
-        hipLaunchKernelGGL((k_pp_platmc<2, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
+        hipLaunchKernelGGL((k_pp_platmc<2, Geo1, PMV>), dim3(grid), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc, R, esc, stage, ctl);                                                          \
       else                                                                                                       \
-        hipLaunchKernelGGL((k_pp_platmc<3, Geo1, PMV>), dim3(nbad), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
-                           guide, claim, budget, variant == 2, relist, cap, stats, misc);                                                          \
+        hipLaunchKernelGGL((k_pp_platmc<3, Geo1, PMV>), dim3(grid), dim3(kWave), 0, st, g1_, bad, nbad, d, t, y,   \
+                           guide, claim, budget, variant == 2, relist, cap, stats, misc, R, esc, stage, ctl);                                                          \
     }                                                                                                            \
   } while (0)
 
 hipError_t pp_platmc(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t, float* y,
                      const float* guide, uint32_t* claim, int variant, uint32_t budget, uint32_t* relist,
-                     uint32_t cap, bool stats, uint32_t* misc, hipStream_t st) {
+                     uint32_t cap, bool stats, uint32_t* misc, uint32_t* esc, hipStream_t st) {
   if (!nbad) return hipSuccess;
+  const uint32_t R = 0, grid = nbad;
+  float* stage = nullptr;
+  uint32_t* ctl = nullptr;
+  if (variant == 2) PP_MC_LAUNCH(kMcLarge);
+  else if (variant == 1) PP_MC_LAUNCH(kMcSmall);
+  else PP_MC_LAUNCH(kMcTiny);
+  return hipGetLastError();
+}
+
+uint32_t pp_platmc_capacity(int variant) { return variant == 2 ? kMcLarge : (variant == 1 ? kMcSmall : kMcTiny); }
+
+hipError_t pp_platmc_replicas(const PpGeo& geo, const uint32_t* ent, uint32_t nent, const float* d, const uint8_t* t,
+                              float* y, const float* guide, int variant, uint32_t budget, uint32_t R, float* stage,
+                              uint32_t* ctl, uint32_t* relist, uint32_t cap, bool stats, uint32_t* misc,
+                              hipStream_t st) {
+  if (!nent) return hipSuccess;
+  if (R < 1 || R > 8) return hipErrorInvalidValue;
+  const uint32_t* bad = ent;
+  const uint32_t nbad = nent, grid = nent * R;
+  uint32_t* claim = nullptr;
+  uint32_t* esc = nullptr;
   if (variant == 2) PP_MC_LAUNCH(kMcLarge);
   else if (variant == 1) PP_MC_LAUNCH(kMcSmall);
   else PP_MC_LAUNCH(kMcTiny);
