@@ -588,19 +588,28 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   uint32_t raw[SPW][4];
   uint32_t tbits = 0;
   const uint32_t* in32 = reinterpret_cast<const uint32_t*>(a.in);
+  // the full 16-byte loads of every step first (lanes past the end read vector 0), then
+  // the element loads of a partial last block: no wait between the steps' loads
+  uint4 v4[SPW];
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    const uint32_t b = block_of(s), cur = cur_of(b);
+    v4[s] = reinterpret_cast<const uint4*>(a.in)[j0 + 3 < cur ? (b * BS + j0) >> 2 : 0u];
+  }
 #pragma unroll
   for (int s = 0; s < SPW; s++) {
     const uint32_t b = block_of(s), cur = cur_of(b);
     const uint32_t e0 = b * BS + j0;
-    if (j0 + 3 < cur) {
-      const uint4 v = reinterpret_cast<const uint4*>(a.in)[e0 >> 2];
-      raw[s][0] = v.x;
-      raw[s][1] = v.y;
-      raw[s][2] = v.z;
-      raw[s][3] = v.w;
-    } else {
+    raw[s][0] = v4[s].x;
+    raw[s][1] = v4[s].y;
+    raw[s][2] = v4[s].z;
+    raw[s][3] = v4[s].w;
+    if (!(j0 + 3 < cur)) {
+      uint32_t p[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++) raw[s][r] = j0 + r < cur ? in32[e0 + r] : 0u;
+      for (int r = 0; r < 4; r++) p[r] = j0 + r < cur ? in32[e0 + r] : 0u;
+#pragma unroll
+      for (int r = 0; r < 4; r++) raw[s][r] = p[r];
     }
     if constexpr (MODE == kModeTopoMap || MODE == kModeTopoInt) {
       uint32_t tw = 0;
@@ -621,49 +630,62 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
     const uint32_t dz = ND == 3 ? gr.d2 : gr.d1;          // fastest axis length (multiple of 4)
     const uint32_t s1 = ND == 3 ? gr.plane : gr.d1;      // stride of the slowest axis
     const uint4* f4 = reinterpret_cast<const uint4*>(a.in);
+    // every stencil load of every step is issued before the first compare: one memory
+    // round trip for the tile's classification instead of one per step
+    uint4 nlo[SPW][ND - 1], nhi[SPW][ND - 1];
+    uint32_t zm[SPW], zp[SPW], zs[SPW];
+    bool rowok[SPW];
 #pragma unroll
     for (int s = 0; s < SPW; s++) {
       const uint32_t b = block_of(s), cur = cur_of(b);
       const uint32_t e0 = b * BS + j0;
       const bool act = j0 < cur;  // n is a multiple of 4: all four elements or none
       uint32_t z;
-      bool rowok;
+      bool ok;
       if constexpr (ND == 3) {
         const uint32_t x = fdiv(e0, gr.div_plane), rr = e0 - x * gr.plane;
         const uint32_t y = fdiv(rr, gr.div_row);
         z = rr - y * gr.d2;
-        rowok = act && x >= 1 && x + 1 < gr.d0 && y >= 1 && y + 1 < gr.d1;
+        ok = act && x >= 1 && x + 1 < gr.d0 && y >= 1 && y + 1 < gr.d1;
       } else {
         const uint32_t i = fdiv(e0, gr.div_row);
         z = e0 - i * gr.d1;
-        rowok = act && i >= 1 && i + 1 < gr.d0;
+        ok = act && i >= 1 && i + 1 < gr.d0;
+      }
+      zs[s] = z;
+      rowok[s] = ok;
+      // unconditional 16-byte loads; lanes without a stencil read vector 0 (unused)
+      nlo[s][0] = f4[ok ? (e0 - s1) >> 2 : 0u];
+      nhi[s][0] = f4[ok ? (e0 + s1) >> 2 : 0u];
+      if constexpr (ND == 3) {
+        nlo[s][1] = f4[ok ? (e0 - gr.d2) >> 2 : 0u];
+        nhi[s][1] = f4[ok ? (e0 + gr.d2) >> 2 : 0u];
       }
       // neighbours along the fastest axis: the adjacent lanes hold elements e0 - 1 and
       // e0 + 4 (blocks of a step are consecutive); the wave's end lanes load them
-      uint32_t zm = dpp_mov<kDppWaveShr1>(raw[s][3]), zp = dpp_mov<kDppWaveShl1>(raw[s][0]);
-      if (lane == 0 && rowok && z >= 1) zm = in32[e0 - 1];
-      if (lane == kWave - 1 && rowok && z + 4 < dz) zp = in32[e0 + 4];
-      // unconditional 16-byte loads; lanes without a stencil read vector 0 (unused)
-      uint4 nlo[ND - 1], nhi[ND - 1];
-      nlo[0] = f4[rowok ? (e0 - s1) >> 2 : 0u];
-      nhi[0] = f4[rowok ? (e0 + s1) >> 2 : 0u];
-      if constexpr (ND == 3) {
-        nlo[1] = f4[rowok ? (e0 - gr.d2) >> 2 : 0u];
-        nhi[1] = f4[rowok ? (e0 + gr.d2) >> 2 : 0u];
-      }
-      if (rowok) {
+      zm[s] = 0u;
+      zp[s] = 0u;
+      if (lane == 0 && ok && z >= 1) zm[s] = in32[e0 - 1];
+      if (lane == kWave - 1 && ok && z + 4 < dz) zp[s] = in32[e0 + 4];
+    }
+#pragma unroll
+    for (int s = 0; s < SPW; s++) {
+      const uint32_t z = zs[s];
+      const uint32_t zmv = dpp_mov<kDppWaveShr1>(raw[s][3]), zpv = dpp_mov<kDppWaveShl1>(raw[s][0]);
+      const uint32_t zm_s = lane == 0 ? zm[s] : zmv, zp_s = lane == kWave - 1 ? zp[s] : zpv;
+      if (rowok[s]) {
         const float c[4] = {__uint_as_float(raw[s][0]), __uint_as_float(raw[s][1]), __uint_as_float(raw[s][2]),
                             __uint_as_float(raw[s][3])};
-        const float zl[4] = {__uint_as_float(zm), c[0], c[1], c[2]};
-        const float zh[4] = {c[1], c[2], c[3], __uint_as_float(zp)};
+        const float zl[4] = {__uint_as_float(zm_s), c[0], c[1], c[2]};
+        const float zh[4] = {c[1], c[2], c[3], __uint_as_float(zp_s)};
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           // every element is classified; the ends of a row are masked afterwards
           float lo[ND], hi[ND];
 #pragma unroll
           for (int k = 0; k < ND - 1; k++) {
-            const uint32_t l4[4] = {nlo[k].x, nlo[k].y, nlo[k].z, nlo[k].w};
-            const uint32_t h4[4] = {nhi[k].x, nhi[k].y, nhi[k].z, nhi[k].w};
+            const uint32_t l4[4] = {nlo[s][k].x, nlo[s][k].y, nlo[s][k].z, nlo[s][k].w};
+            const uint32_t h4[4] = {nhi[s][k].x, nhi[s][k].y, nhi[s][k].z, nhi[s][k].w};
             lo[k] = __uint_as_float(l4[r]);
             hi[k] = __uint_as_float(h4[r]);
           }

@@ -42,7 +42,10 @@ __constant__ const uint32_t kBitcountThreshold[32] = {
 
 __device__ __forceinline__ unsigned bitcount(unsigned m) {
   if (m == 0) return 0;
-  unsigned k = 31u - (unsigned)__clz((int)m);
+  const unsigned k = 31u - (unsigned)__clz((int)m);
+  // below 2^20 the threshold is 2^(k+1), which m never reaches: no table read (a
+  // memory round trip) on the common path
+  if (__builtin_expect(m < (1u << 20), 1)) return k + 1u;
   return k + 1u + (m >= kBitcountThreshold[k] ? 1u : 0u);
 }
 
