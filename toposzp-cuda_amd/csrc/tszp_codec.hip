@@ -401,6 +401,7 @@ __device__ __forceinline__ void decode_tile(const DecodeArgs& a, uint32_t tile) 
   constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
   constexpr int TILE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 2;
   __shared__ uint32_t tw[TILE_WORDS];
+  __shared__ uint32_t sh_blk[BPT];  // the tile's block offsets
 
   const int tid = threadIdx.x;
   const int w = tid / kWave, lane = tid & (kWave - 1);
@@ -409,6 +410,7 @@ __device__ __forceinline__ void decode_tile(const DecodeArgs& a, uint32_t tile) 
   const uint32_t b0 = tile * BPT;
   const uint32_t b1 = min(a.nb, b0 + (uint32_t)BPT);
 
+  for (uint32_t i = tid; i < (uint32_t)BPT; i += kBlockThreads) sh_blk[i] = b0 + i < b1 ? a.blk[b0 + i] : 0u;
   const uint32_t t0 = a.blk[b0], t1 = a.blk[b1];
   const uint32_t wbase = t0 & ~3u;
   const uint32_t nwords = ((t1 + 3u) & ~3u) / 4u - wbase / 4u;
@@ -432,13 +434,16 @@ __device__ __forceinline__ void decode_tile(const DecodeArgs& a, uint32_t tile) 
     }
   }
   __syncthreads();
+  // 16-byte stores of four elements need block starts at multiples of 4 and aligned outputs
+  const bool vec = (bs & 3u) == 0 && ((uintptr_t)a.out & 15u) == 0 && ((uintptr_t)a.types8 & 3u) == 0 &&
+                   ((uintptr_t)a.types32 & 15u) == 0;
 
   for (int s = 0; s < SPW; s++) {
     const uint32_t lb = (uint32_t)((w * SPW + s) * G + g);
     const uint32_t b = b0 + lb;
     const uint32_t base = b * bs;
     const uint32_t cur = b < a.nb ? min(bs, a.n - base) : 0u;
-    const uint32_t o = cur ? a.blk[b] - wbase : 0u;
+    const uint32_t o = cur ? sh_blk[lb] - wbase : 0u;
     int anchor = 0;
     unsigned bc = 0;
     if (cur) {
@@ -485,6 +490,30 @@ __device__ __forceinline__ void decode_tile(const DecodeArgs& a, uint32_t tile) 
     if (cur == 0) continue;
     unsigned tbase = 0;
     if constexpr (TOPO) tbase = o + 5u + (bc ? sB + mag_bytes(n, bc) : 0u);
+    if constexpr (EPL == 4) {  // a lane's four whole elements: one 16-byte store (and one of types)
+      const uint32_t j0 = (uint32_t)(ls * 4);
+      if (vec && j0 + 3 < cur) {
+        int qv[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) qv[r] = (int)((unsigned)anchor + (unsigned)excl + (unsigned)dl[r]);
+        const uint32_t e = base + j0;
+        if constexpr (MODE == kModeInt) {
+          reinterpret_cast<int4*>(a.out)[e >> 2] = make_int4(qv[0], qv[1], qv[2], qv[3]);
+        } else {
+          reinterpret_cast<float4*>(a.out)[e >> 2] =
+              make_float4(__fmul_rn(__int2float_rn(qv[0]), a.eb), __fmul_rn(__int2float_rn(qv[1]), a.eb),
+                          __fmul_rn(__int2float_rn(qv[2]), a.eb), __fmul_rn(__int2float_rn(qv[3]), a.eb));
+        }
+        if constexpr (TOPO) {
+          const unsigned tb = lds_byte(tw, tbase + (j0 >> 2));  // element j0 in the top two bits
+          const unsigned t0v = (tb >> 6) & 3u, t1v = (tb >> 4) & 3u, t2v = (tb >> 2) & 3u, t3v = tb & 3u;
+          if (a.types8) reinterpret_cast<uint32_t*>(a.types8)[e >> 2] = t0v | t1v << 8 | t2v << 16 | t3v << 24;
+          if (a.types32)
+            reinterpret_cast<int4*>(a.types32)[e >> 2] = make_int4((int)t0v, (int)t1v, (int)t2v, (int)t3v);
+        }
+        continue;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < EPL; r++) {
       const uint32_t j = (uint32_t)(ls * EPL + r);
