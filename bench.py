@@ -884,7 +884,7 @@ def main():
     # launch (the minimum HBM traffic of what each launch must read and write).
     ndim = len(shape)
     # The kernels of the step that the library times one by one, with their algorithmic
-    # bytes per launch (the others, each under 0.35 ms per step at 512^3, are in the
+    # bytes per launch (the others, each under 0.35 ms per launch at 512^3, are in the
     # kernel-trace summary); None: no byte model (the window solve is a chain of
     # dependent tiles, bound by latency).
     kernels = {
