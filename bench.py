@@ -633,7 +633,7 @@ def dropin_side(tz, f, eb, bs):
         dp, fp = C.c_void_p(), C.c_void_p()
         L.szp_cuda_float_decompress_randomaccess_topology_preserved(C.byref(dp), n, eb, bs, C.c_void_p(topo),
                                                                     C.byref(fp))
-        spd = L.szp_cuda_decompress_sort_positions(C.c_void_p(sps + 8), ext, bs)
+        spd = L.szp_cuda_decompress_sort_positions(C.c_void_p(sps), ext, bs)  # [u64 0][blocks]
         t3 = time.perf_counter()
         ok = bool(dp.value and fp.value and (spd or ext == 0))
         for q in (cps, sps, topo, dp.value, fp.value, spd):
