@@ -1833,7 +1833,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     return TSZP_OK;
   };
   if (nb && mc_on &&
-      (r = mc_pass(env_u32("TSZP_MC_BUDGET1", 192u), env_u32("TSZP_MC_FIRST1", 32u), env_u32("TSZP_MC_REPS1", 4u),
+      (r = mc_pass(env_u32("TSZP_MC_BUDGET1", 192u), env_u32("TSZP_MC_FIRST1", 16u), env_u32("TSZP_MC_REPS1", 8u),
                    false, "short")))
     return r;
   // window solves: anchors (bit 0) and, behind the plateau solves, only windows that hold
@@ -1974,7 +1974,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   if ((r = window_rounds((use_sp ? 1 : 0) | (mc_on ? 2 : 0)))) return r;
   if (!verified && (r = full_mark(&nb, true))) return r;
   if (nb && mc_on && !stop_mc &&
-      (r = mc_pass(env_u32("TSZP_MC_BUDGET2", 30000u), env_u32("TSZP_MC_FIRST2", 256u), env_u32("TSZP_MC_REPS2", 8u),
+      (r = mc_pass(env_u32("TSZP_MC_BUDGET2", 30000u), env_u32("TSZP_MC_FIRST2", 128u), env_u32("TSZP_MC_REPS2", 8u),
                    true, "long")))
     return r;
   // plateaus beyond the plateau solves' capacity: windows over their parts
