@@ -414,6 +414,7 @@ int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32
     t.end(st);
   }
   HIPCHK(bin_bitmap(nchunks, w, gw, st));
+  HIPCHK(bin_chunk_scan(B, w, st));
   ps.stop();
   ProfScope pr(kProfRank, st);
   {

@@ -411,6 +411,24 @@ __global__ __launch_bounds__(kBitmapThreads) void k_bin_bitmap(const uint32_t* _
 }
 
 // ---- 7. ranks in the original order ---------------------------------------------------
+// chunk totals -> exclusive prefix over each bin's chunks, in place (one thread per
+// bin; a bin has at most kBinMaxChunks chunks), so that a rank query adds one word
+// instead of walking the chunks before its own
+__global__ __launch_bounds__(kBlockThreads) void k_bin_chunk_scan(const uint4* __restrict__ bt, uint32_t B,
+                                                                 uint32_t* __restrict__ ctot) {
+  const uint32_t b = blockIdx.x * kBlockThreads + threadIdx.x;
+  if (b >= B) return;
+  const uint4 m = bt[b];
+  if (m.w) return;  // wide bins: ranked by the radix path
+  const uint32_t nch = (bt[b + 1].y - m.y + kBinChunkWords - 1) / kBinChunkWords;
+  uint32_t run = 0;
+  for (uint32_t j = 0; j < nch; j++) {
+    const uint32_t v = ctot[m.z + j];
+    ctot[m.z + j] = run;
+    run += v;
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_bin_query(const uint32_t* __restrict__ key,
                                                             const uint32_t* __restrict__ pay, uint32_t P, float eb,
                                                             double inv, int bmin, uint32_t B,
@@ -445,9 +463,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_query(const uint32_t* __r
 #pragma unroll
   for (int u = 0; u < QU; u++) {
     if (!live[u]) continue;
-    uint32_t r = v[u].y + (uint32_t)__popc(v[u].x & ((1u << (off[u] & 31u)) - 1u));
+    // ctot holds, after k_bin_chunk_scan, the distinct keys of the bin's earlier chunks
     const uint32_t ci = off[u] >> kChunkShift;
-    for (uint32_t j = 0; j < ci; j++) r += ctot[m[u].z + j];
+    const uint32_t r = v[u].y + (uint32_t)__popc(v[u].x & ((1u << (off[u] & 31u)) - 1u)) + ctot[m[u].z + ci];
     out[pv[u]] = (int32_t)r;
   }
 }
@@ -494,6 +512,13 @@ hipError_t bin_bitmap(uint32_t nchunks, const BinWork& w, uint2* gw, hipStream_t
   if (!nchunks) return hipSuccess;
   hipLaunchKernelGGL(k_bin_bitmap, dim3(nchunks), dim3(kBitmapThreads), 0, st, w.bkey, w.boff, w.bt, w.cbin, w.ctot,
                      gw);
+  return hipGetLastError();
+}
+
+hipError_t bin_chunk_scan(uint32_t B, const BinWork& w, hipStream_t st) {
+  if (!B) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_chunk_scan, dim3((B + kBlockThreads - 1) / kBlockThreads), dim3(kBlockThreads), 0, st,
+                     w.bt, B, w.ctot);
   return hipGetLastError();
 }
 

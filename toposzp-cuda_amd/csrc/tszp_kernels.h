@@ -361,6 +361,8 @@ hipError_t bin_plan(uint32_t B, uint32_t nslice, const BinWork& w, hipStream_t s
 hipError_t bin_scatter(const uint32_t* key, const uint32_t* pay, uint32_t count, float eb, double inv, int bmin,
                        uint32_t B, const BinWork& w, hipStream_t st);
 hipError_t bin_bitmap(uint32_t nchunks, const BinWork& w, uint2* gw, hipStream_t st);
+// chunk totals -> per-bin exclusive prefix over the bin's chunks (before bin_query)
+hipError_t bin_chunk_scan(uint32_t B, const BinWork& w, hipStream_t st);
 hipError_t bin_query(const uint32_t* key, const uint32_t* pay, uint32_t count, float eb, double inv, int bmin,
                      uint32_t B, const BinWork& w, const uint2* gw, int32_t* out, uint32_t out_cap, hipStream_t st);
 

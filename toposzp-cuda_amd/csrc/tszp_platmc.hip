@@ -303,7 +303,7 @@ __device__ void mc_replica_end(SH& sh, const FieldView& F, float* y, const uint8
 // of an entry writes the winner's values (mc_replica_end): the result depends on the
 // plateau alone, not on the replicas' timing.
 template <int ND, class GEO, int PM>
-__global__ __launch_bounds__(kWave) void k_pp_platmc(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
+__global__ __launch_bounds__(kWave, PM <= kMcTiny ? 8 : 1) void k_pp_platmc(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
                                                      const float* __restrict__ d, const uint8_t* __restrict__ t,
                                                      float* __restrict__ y, const float* __restrict__ guide,
                                                      uint32_t* __restrict__ claim, uint32_t budget, bool last,
