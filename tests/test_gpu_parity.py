@@ -327,3 +327,32 @@ def test_codec_fast_path_matches_generic(shape, bs):
     _eq(a[2], b[2], "topology stream")
     _eq(a[3], b[3], "sort-position stream")
     _eq(a[4], b[4], "plain stream")
+
+
+@pytest.mark.parametrize("shape,bs", [((128, 256, 256), 64), ((40, 96, 100), 32), ((33, 70, 92), 128),
+                                      ((64, 128, 128), 16), ((515, 1004), 128), ((300, 1024), 256),
+                                      ((97, 1000), 64), ((31, 40, 52), 16)])
+def test_codec_lds_rows_match_global_rows(shape, bs):
+    """Classification with the tile's rows staged in LDS (the tile's points and one row on
+    either side, TSZP_CODEC_LDS_ROWS) against the default global-load stencil: identical
+    topology and sort-position streams, on grids whose rows start anywhere in a tile and
+    whose last tile is partial."""
+    torch = _torch()
+    f = tz.gen_fbm(shape, seed=93) if len(shape) == 3 else torch.from_numpy(
+        fields.blobs2d(*shape, nblobs=40, seed=94, noise=0.05)).cuda()
+    eb, _ = tz.rel_to_abs(f, 1e-3)
+    outs = []
+    for lds in (False, True):
+        if lds:
+            os.environ["TSZP_CODEC_LDS_ROWS"] = "1"
+        try:
+            info, buf = tz.topo_compress(f, eb, bs)
+            torch.cuda.synchronize()
+            outs.append((info.cp_count, info.ext_count, buf.topo[:info.topo_bytes].cpu().numpy().copy(),
+                         buf.sp[:info.sortpos_bytes].cpu().numpy().copy()))
+        finally:
+            os.environ.pop("TSZP_CODEC_LDS_ROWS", None)
+    (a, b) = outs
+    assert a[0] == b[0] and a[1] == b[1] and a[0] > 0
+    _eq(a[2], b[2], "topology stream")
+    _eq(a[3], b[3], "sort-position stream")

@@ -294,6 +294,7 @@ int run_codec(Ctx* c, int mode, const void* d_in, uint32_t n, int bs, float eb, 
   a.eb = eb;
   a.inv = inv;
   if (grid) a.grid = *grid;
+  a.lds_rows = getenv("TSZP_CODEC_LDS_ROWS") ? 1u : 0u;  // the same bytes either way
   a.tmap = d_tmap;
   a.payload = d_stream + hdr;
   uint64_t pc = cap - hdr - 4;  // keep 4 readable bytes past the end

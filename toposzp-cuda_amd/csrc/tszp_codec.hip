@@ -584,6 +584,9 @@ __device__ __forceinline__ int type_minmax(float c, const float (&lo)[ND], const
   return all_gt ? 1 : all_lt ? 2 : (any_hi && any_lo) ? 3 : 0;
 }
 
+constexpr int kV4TileElems = 4096;  // points per tile of the fast compressor (4 waves x 4 steps x 256)
+constexpr int kV4HaloMax = 1024;    // longest row its classification stages in LDS
+
 template <int MODE, int S, int ND>
 __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t tile_in) {
   constexpr int EPL = 4, BS = 4 * S;
@@ -592,8 +595,13 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   constexpr bool TOPO = (MODE == kModeTopoClassify || MODE == kModeTopoMap || MODE == kModeTopoInt);
   constexpr bool CLASSIFY = (MODE == kModeTopoClassify);
   constexpr int STAGE_WORDS = (BPT * Gm::worst_block(TOPO) + 3) / 4 + 8;  // runs may spill 16 zero bytes
+  // classification stages the tile's points and the rows next to it in the same LDS
+  // before the packing uses it (tile of kV4TileElems points, rows of up to kV4HaloMax)
+  static_assert(BPT * BS == kV4TileElems, "a wave step covers 256 points");
+  constexpr int LDS_WORDS = CLASSIFY && kV4TileElems + 2 * kV4HaloMax > STAGE_WORDS ? kV4TileElems + 2 * kV4HaloMax
+                                                                                      : STAGE_WORDS;
 
-  __shared__ uint32_t stage[STAGE_WORDS];
+  __shared__ __align__(16) uint32_t stage[LDS_WORDS];
   __shared__ uint32_t blk_off[BPT + 1];
   __shared__ uint32_t step_base[kWavesPerBlock * SPW];  // cp | ext << 16 (tile-local prefix)
   __shared__ uint64_t sh_prefix[2];
@@ -606,7 +614,8 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   const uint32_t j0 = 4u * (uint32_t)ls;  // first element of the lane within its block
 
   if (tid == 0) sh_tile = a.tile_counter ? atomicAdd(a.tile_counter, 1u) : tile_in;
-  for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
+  if constexpr (!CLASSIFY)  // (classification clears it after its staging)
+    for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   __syncthreads();
   const uint32_t tile = sh_tile;
 
@@ -654,13 +663,43 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   }
 
   // ---- A2: classification ------------------------------------------------------------
+  // Stencil along the fastest axis: the adjacent lanes (DPP).  Along the next axis (rows
+  // of rn points: d2 in 3D, d1 in 2D): 16-byte global loads, which the tile's own loads
+  // have mostly brought into L1 / L2; or, with a.lds_rows and rn <= kV4HaloMax, the
+  // tile's own points and one row before and after it staged in LDS (the same bytes;
+  // measured 2 % slower at 512^3, so not the default).  Along the slowest axis of a 3D
+  // grid (planes): 16-byte global loads (a plane apart; the tile 64 tiles back on the
+  // same XCD holds them in its L2).
   if constexpr (CLASSIFY) {
     const Grid& gr = a.grid;
     const uint32_t dz = ND == 3 ? gr.d2 : gr.d1;          // fastest axis length (multiple of 4)
     const uint32_t s1 = ND == 3 ? gr.plane : gr.d1;      // stride of the slowest axis
+    const uint32_t rn = ND == 3 ? gr.d2 : gr.d1;         // stride of the next axis (a row)
+    constexpr int KN = ND - 2;                           // its stencil index
     const uint4* f4 = reinterpret_cast<const uint4*>(a.in);
-    // every stencil load of every step is issued before the first compare: one memory
-    // round trip for the tile's classification instead of one per step
+    const bool near_lds = a.lds_rows && rn <= (uint32_t)kV4HaloMax;  // uniform
+    uint4* tl4 = reinterpret_cast<uint4*>(stage);        // point T0 - rn + i at word i
+    const uint32_t T0 = tile * (uint32_t)kV4TileElems;
+    if (near_lds) {
+#pragma unroll
+      for (int s = 0; s < SPW; s++) {
+        const uint32_t l = (uint32_t)(w * SPW + s) * 256u + 4u * (uint32_t)lane;  // tile-local point
+        tl4[(rn + l) >> 2] = make_uint4(raw[s][0], raw[s][1], raw[s][2], raw[s][3]);
+      }
+      // the row before the tile and the row after it, where they lie in the field
+      // (points outside it are never read: a point with a neighbour there is not classified)
+      const uint32_t q = rn >> 2;
+      for (uint32_t i = tid; i < 2 * q; i += kBlockThreads) {
+        const bool hi = i >= q;
+        const uint32_t k = hi ? i - q : i;
+        const uint64_t e = hi ? (uint64_t)T0 + kV4TileElems + 4u * k : (uint64_t)T0 + 4u * k - rn;
+        const bool ok = hi ? e + 3 < a.n : T0 >= rn;
+        if (ok) tl4[hi ? (rn + kV4TileElems) / 4 + k : k] = f4[e >> 2];
+      }
+      __syncthreads();
+    }
+    // every global stencil load of every step is issued before the first compare: one
+    // memory round trip for the tile's classification instead of one per step
     uint4 nlo[SPW][ND - 1], nhi[SPW][ND - 1];
     uint32_t zm[SPW], zp[SPW], zs[SPW];
     bool rowok[SPW];
@@ -684,11 +723,13 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
       zs[s] = z;
       rowok[s] = ok;
       // unconditional 16-byte loads; lanes without a stencil read vector 0 (unused)
-      nlo[s][0] = f4[ok ? (e0 - s1) >> 2 : 0u];
-      nhi[s][0] = f4[ok ? (e0 + s1) >> 2 : 0u];
       if constexpr (ND == 3) {
-        nlo[s][1] = f4[ok ? (e0 - gr.d2) >> 2 : 0u];
-        nhi[s][1] = f4[ok ? (e0 + gr.d2) >> 2 : 0u];
+        nlo[s][0] = f4[ok ? (e0 - s1) >> 2 : 0u];
+        nhi[s][0] = f4[ok ? (e0 + s1) >> 2 : 0u];
+      }
+      if (!near_lds) {
+        nlo[s][KN] = f4[ok ? (e0 - rn) >> 2 : 0u];
+        nhi[s][KN] = f4[ok ? (e0 + rn) >> 2 : 0u];
       }
       // neighbours along the fastest axis: the adjacent lanes hold elements e0 - 1 and
       // e0 + 4 (blocks of a step are consecutive); the wave's end lanes load them
@@ -702,6 +743,11 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
       const uint32_t z = zs[s];
       const uint32_t zmv = dpp_mov<kDppWaveShr1>(raw[s][3]), zpv = dpp_mov<kDppWaveShl1>(raw[s][0]);
       const uint32_t zm_s = lane == 0 ? zm[s] : zmv, zp_s = lane == kWave - 1 ? zp[s] : zpv;
+      if (near_lds) {  // rows from LDS: the point l - rn is at word l, l + rn at word l + 2 rn
+        const uint32_t l = (uint32_t)(w * SPW + s) * 256u + 4u * (uint32_t)lane;
+        nlo[s][KN] = tl4[l >> 2];
+        nhi[s][KN] = tl4[(l + 2 * rn) >> 2];
+      }
       if (rowok[s]) {
         const float c[4] = {__uint_as_float(raw[s][0]), __uint_as_float(raw[s][1]), __uint_as_float(raw[s][2]),
                             __uint_as_float(raw[s][3])};
@@ -726,6 +772,9 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
         }
       }
     }
+    // the staging area becomes the packing stage: cleared once every wave has read it
+    __syncthreads();
+    for (int i = tid; i < STAGE_WORDS; i += kBlockThreads) stage[i] = 0;
   }
 
   // ---- A3: quantize once; block bit counts, sizes and delta magnitudes ---------------

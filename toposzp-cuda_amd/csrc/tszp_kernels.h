@@ -35,6 +35,7 @@ struct CompressArgs {
   uint32_t* cp_key;       // classify mode: order-preserving value keys, row-major
   uint32_t* cp_pay;       // extremum ordinal, or 0xffffffff for saddles
   uint32_t cp_cap;
+  uint32_t lds_rows;      // classify mode, fast path: stage the rows next to each tile in LDS
   CompressResult* res;
 };
 
