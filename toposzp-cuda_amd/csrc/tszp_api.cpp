@@ -1713,16 +1713,15 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
       HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(badmap), 0u, (n + 3) / 4, st));
       HIPCHK(pp_badmap(bad, nb, badmap, st));
     }
-    // the tiny pass touches neither the claim bits nor the mismatch count, so both are
-    // cleared for the re-marking that follows it in the same launch
+    // a solved plateau's new values change no other point's type (its other neighbours
+    // lie in other cells), so re-classifying the listed points gives the exact list
     FillRanges fr{};
     fr.add(misc + kPpTiny, 0u, 1);
     fr.add(misc + kPpBad, 0u, 1);
-    fr.add(bits, 0u, bit_words);
     HIPCHK(launch_fill_multi(fr, st));
     HIPCHK(pp_rel(geo, d, rel, st));
     HIPCHK(pp_tiny(geo, bad, nb, d, rel, badmap, d_types, d_field, use_sp, plist, part, misc, st));
-    HIPCHK(pp_remark(geo, bad, nb, d_field, d_types, bits, bad2, cap, 1, misc, st));
+    HIPCHK(pp_relist(geo, bad, nb, d_field, d_types, bad2, cap, misc, st));
     if ((r = read_misc())) return r;
     std::swap(bad, bad2);
     if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);

@@ -452,6 +452,11 @@ hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const f
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
 // mismatch map (one byte per point) from a list; the map must be zeroed first
 hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
+// the listed points (no duplicates) whose types are violated in y, appended to out
+// (count misc[kPpBad]); after pp_tiny, whose solved plateaus change no other point's
+// type, the exact mismatch list
+hipError_t pp_relist(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* y, const uint8_t* t,
+                     uint32_t* out, uint32_t cap, uint32_t* misc, hipStream_t st);
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
 hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* pbits, uint32_t* tbits,
                     uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);

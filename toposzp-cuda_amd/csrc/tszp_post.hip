@@ -3009,6 +3009,64 @@ hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStr
   return hipGetLastError();
 }
 
+// The listed points whose types are still violated (a list without duplicates, so no
+// claim bits), compacted per workgroup: a workgroup takes kRelistChunk consecutive
+// entries, collects the kept ones in LDS in list order and appends them with one
+// atomic (same-address atomics serialise, so one per wave over millions of entries
+// would cost more than the classification).
+constexpr uint32_t kRelistChunk = 4096;
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_relist(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                            const float* __restrict__ y, const uint8_t* __restrict__ t,
+                                                            uint32_t* __restrict__ out, uint32_t cap,
+                                                            uint32_t* misc) {
+  __shared__ uint32_t kept[kRelistChunk];
+  __shared__ uint32_t wcnt[kWavesPerBlock], sh_base, sh_n;
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
+  const uint32_t c0 = blockIdx.x * kRelistChunk;
+  uint32_t n = 0;  // kept so far (uniform)
+  for (uint32_t r = 0; r < kRelistChunk / kBlockThreads; r++) {
+    const uint32_t i = c0 + r * kBlockThreads + tid;
+    uint32_t e = 0;
+    bool keep = false;
+    if (i < nbad) {
+      e = bad[i];
+      const FieldView F = geo.at(e);
+      int c[ND];
+      coords_of<ND>(F.g, e - F.base, c);
+      keep = classify_at<ND>(F.g, y, e, c) != (int)t[e];
+    }
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      before += k < w ? wcnt[k] : 0u;
+      tot += wcnt[k];
+    }
+    if (keep) kept[n + before + (uint32_t)__popcll(m & lanemask_lt(lane))] = e;
+    n += tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    sh_n = n;
+    sh_base = n ? atomicAdd(&misc[kPpBad], n) : 0u;
+  }
+  __syncthreads();
+  const uint32_t base = sh_base;
+  for (uint32_t k = tid; k < sh_n; k += kBlockThreads)
+    if (base + k < cap) out[base + k] = kept[k];
+}
+
+hipError_t pp_relist(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* y, const uint8_t* t,
+                     uint32_t* out, uint32_t cap, uint32_t* misc, hipStream_t st) {
+  if (!nbad) return hipSuccess;
+  PP_GEO_LAUNCH(k_pp_relist, dim3((nbad + kRelistChunk - 1) / kRelistChunk), dim3(kBlockThreads), st, bad, nbad, y, t,
+                out, cap, misc);
+  return hipGetLastError();
+}
+
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz) {
   const uint32_t s = 1u << kTileShift;
   *tx = (g.d0 + s - 1) / s;
