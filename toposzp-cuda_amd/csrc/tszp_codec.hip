@@ -598,8 +598,11 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   // classification stages the tile's points and the rows next to it in the same LDS
   // before the packing uses it (tile of kV4TileElems points, rows of up to kV4HaloMax)
   static_assert(BPT * BS == kV4TileElems, "a wave step covers 256 points");
-  constexpr int LDS_WORDS = CLASSIFY && kV4TileElems + 2 * kV4HaloMax > STAGE_WORDS ? kV4TileElems + 2 * kV4HaloMax
-                                                                                      : STAGE_WORDS;
+  // ... and, after the copy-out, the tile's critical-point records (a key and a payload
+  // word per point at most)
+  constexpr int CLS_WORDS = kV4TileElems + 2 * kV4HaloMax > 2 * kV4TileElems ? kV4TileElems + 2 * kV4HaloMax
+                                                                             : 2 * kV4TileElems;
+  constexpr int LDS_WORDS = CLASSIFY && CLS_WORDS > STAGE_WORDS ? CLS_WORDS : STAGE_WORDS;
 
   __shared__ __align__(16) uint32_t stage[LDS_WORDS];
   __shared__ uint32_t blk_off[BPT + 1];
@@ -983,10 +986,16 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
   if (tile == a.ntiles - 1 && tid == 0) a.blk[a.nb] = (uint32_t)(g0 + len);
 
   // ---- E: critical-point records in row-major order ---------------------------------------
+  // The tile's records are contiguous in the record arrays: gathered in LDS at their
+  // tile-local index, then written with consecutive stores (a lane's own records would be
+  // up to four scattered stores per array and step).
   if constexpr (CLASSIFY) {
     if (a.cp_key) {
       const uint64_t pc = sh_prefix[1];
       const uint32_t cp0 = (uint32_t)(pc & 0x7fffffffull), ext0 = (uint32_t)((pc >> 31) & 0x7fffffffull);
+      uint32_t* lk = stage;                  // tile-local keys
+      uint32_t* lp = stage + kV4TileElems;   // and payloads
+      __syncthreads();                       // every wave has copied the stream out of the stage
 #pragma unroll
       for (int s = 0; s < SPW; s++) {
         unsigned c = 0, x = 0;
@@ -1000,21 +1009,27 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
         const unsigned incl = wave_incl_scan_dpp(packed, lane);
         const unsigned excl = incl - packed;
         const unsigned sb = step_base[w * SPW + s];
-        uint32_t pcp = cp0 + (sb & 0xffffu) + (excl & 0xffffu);
+        uint32_t lc = (sb & 0xffffu) + (excl & 0xffffu);
         uint32_t pex = ext0 + (sb >> 16) + (excl >> 16);
         if (!c) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const unsigned t = (tbits >> (2 * (s * 4 + r))) & 3u;
           if (!t) continue;
-          if (pcp < a.cp_cap) {
-            a.cp_key[pcp] = okey(__uint_as_float(raw[s][r]));
-            a.cp_pay[pcp] = (t == 3) ? 0xffffffffu : pex;
-          } else {
-            atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
-          }
-          pcp++;
+          lk[lc] = okey(__uint_as_float(raw[s][r]));
+          lp[lc] = (t == 3) ? 0xffffffffu : pex;
+          lc++;
           if (t != 3) pex++;
+        }
+      }
+      __syncthreads();
+      const uint32_t ntile = (uint32_t)(sh_cnt_agg & 0x7fffffffull);
+      if (tid == 0 && (uint64_t)cp0 + ntile > a.cp_cap) atomicOr(&a.res->error, TSZP_ERR_OVERFLOW);
+      for (uint32_t i = tid; i < ntile; i += kBlockThreads) {
+        const uint32_t pcp = cp0 + i;
+        if (pcp < a.cp_cap) {
+          a.cp_key[pcp] = lk[i];
+          a.cp_pay[pcp] = lp[i];
         }
       }
     }

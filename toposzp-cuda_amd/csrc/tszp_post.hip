@@ -136,16 +136,28 @@ __device__ __forceinline__ void lds_flush(const uint32_t* lbuf, uint32_t* lcnt, 
 // ---------------------------------------------------------------------------------
 // largest sort position
 // ---------------------------------------------------------------------------------
+// (16-byte loads when sp is aligned; few workgroups, since their atomics on one word
+// serialise)
 __global__ __launch_bounds__(kBlockThreads) void k_pp_spmax(const int32_t* __restrict__ sp, uint32_t m,
                                                            uint32_t* __restrict__ misc) {
   __shared__ uint32_t red[kWavesPerBlock];
-  uint32_t v = 0;
-  for (uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x; i < m; i += gridDim.x * kBlockThreads) {
-    const int32_t s = sp[i];
-    v = s > (int32_t)v ? (uint32_t)s : v;
+  int32_t v = 0;
+  const uint32_t tid = blockIdx.x * kBlockThreads + threadIdx.x, stride = gridDim.x * kBlockThreads;
+  uint32_t head = m;
+  if ((reinterpret_cast<uintptr_t>(sp) & 15u) == 0) {
+    const int4* sp4 = reinterpret_cast<const int4*>(sp);
+    const uint32_t m4 = m / 4;
+    for (uint32_t i = tid; i < m4; i += stride) {
+      const int4 x = sp4[i];
+      v = max(v, max(max(x.x, x.y), max(x.z, x.w)));
+    }
+    head = 4 * m4;
+  } else {
+    head = 0;
   }
-  v = seg_max<kWave>(v);
-  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  for (uint32_t i = head + tid; i < m; i += stride) v = max(v, sp[i]);
+  const uint32_t vm = seg_max<kWave>((uint32_t)v);  // sort positions are non-negative
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = vm;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t r = red[0];
@@ -2840,7 +2852,8 @@ hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* m
     return hipGetLastError();
   }
   if (!m) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_spmax, dim3(grid_stride_blocks(m)), dim3(kBlockThreads), 0, st, sp, m, misc);
+  const unsigned g = blocks_for((m + 3) / 4);
+  hipLaunchKernelGGL(k_pp_spmax, dim3(g < 512u ? g : 512u), dim3(kBlockThreads), 0, st, sp, m, misc);
   return hipGetLastError();
 }
 
