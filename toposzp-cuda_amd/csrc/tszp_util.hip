@@ -41,12 +41,27 @@ hipError_t launch_fill_multi(const FillRanges& r, hipStream_t st) {
   return hipGetLastError();
 }
 
-// min / max as order-preserving keys (out2 must hold {0xffffffff, 0} on entry)
+// min / max as order-preserving keys (out2 must hold {0xffffffff, 0} on entry); 16-byte
+// loads when f is aligned, one atomic pair per workgroup (a few hundred workgroups:
+// atomics on one word serialise)
 __global__ __launch_bounds__(kBlockThreads) void k_minmax(const float* __restrict__ f, size_t n, uint32_t* out2) {
+  __shared__ uint32_t rlo[kWavesPerBlock], rhi[kWavesPerBlock];
   uint32_t lo = 0xffffffffu, hi = 0;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
+  size_t head = 0;
+  if ((reinterpret_cast<uintptr_t>(f) & 15u) == 0) {
+    const float4* f4 = reinterpret_cast<const float4*>(f);
+    const size_t n4 = n / 4;
+    for (size_t i = tid; i < n4; i += stride) {
+      const float4 v = f4[i];
+      const uint32_t k0 = okey(v.x), k1 = okey(v.y), k2 = okey(v.z), k3 = okey(v.w);
+      lo = min(lo, min(min(k0, k1), min(k2, k3)));
+      hi = max(hi, max(max(k0, k1), max(k2, k3)));
+    }
+    head = 4 * n4;
+  }
+  for (size_t i = head + tid; i < n; i += stride) {
     const uint32_t k = okey(f[i]);
     lo = k < lo ? k : lo;
     hi = k > hi ? k : hi;
@@ -57,6 +72,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_minmax(const float* __restric
     hi = b > hi ? b : hi;
   }
   if ((threadIdx.x & (kWave - 1)) == 0) {
+    rlo[threadIdx.x / kWave] = lo;
+    rhi[threadIdx.x / kWave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 1; k < kWavesPerBlock; k++) {
+      lo = rlo[k] < lo ? rlo[k] : lo;
+      hi = rhi[k] > hi ? rhi[k] : hi;
+    }
     atomicMin(&out2[0], lo);
     atomicMax(&out2[1], hi);
   }
@@ -66,7 +91,7 @@ hipError_t launch_minmax(const float* f, size_t n, uint32_t* out2, hipStream_t s
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, st, out2, 0xffffffffu, (size_t)1);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, st, out2 + 1, 0u, (size_t)1);
   size_t blocks = (n + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 512) blocks = 512;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_minmax, dim3((unsigned)blocks), dim3(kBlockThreads), 0, st, f, n, out2);
   return hipGetLastError();
