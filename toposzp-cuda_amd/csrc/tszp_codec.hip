@@ -864,26 +864,30 @@ __device__ __forceinline__ void compress_v4_tile(const CompressArgs& a, uint32_t
     }
   }
   __syncthreads();
-  // wave 0 finds the tile's stream and CP offsets while the other waves already pack
-  // (the packing uses tile-local offsets only)
+  // wave 0 finds the tile's stream offset and wave 1 its CP offsets (the two look-backs
+  // side by side) while the other waves already pack (the packing uses tile-local
+  // offsets only)
   if (w == 0) {
     const uint32_t tile_bytes = blk_off[BPT];
-    uint64_t cnt_agg = 0;
-    if constexpr (CLASSIFY) cnt_agg = sh_cnt_agg;
     const uint64_t pb = lookback(a.st_bytes, tile, tile_bytes, &a.res->error);
-    uint64_t pc = 0;
-    if constexpr (CLASSIFY) pc = lookback(a.st_cnt, tile, cnt_agg, &a.res->error);
     if (lane == 0) {
       sh_prefix[0] = pb;
-      sh_prefix[1] = pc;
-      if (tile == a.ntiles - 1) {
-        a.res->payload_bytes = pb + tile_bytes;
-        if constexpr (CLASSIFY) {
+      if (tile == a.ntiles - 1) a.res->payload_bytes = pb + tile_bytes;
+    }
+  } else if (w == 1) {
+    if constexpr (CLASSIFY) {
+      const uint64_t cnt_agg = sh_cnt_agg;
+      const uint64_t pc = lookback(a.st_cnt, tile, cnt_agg, &a.res->error);
+      if (lane == 0) {
+        sh_prefix[1] = pc;
+        if (tile == a.ntiles - 1) {
           const uint64_t tc = pc + cnt_agg;
           a.res->cp_count = tc & 0x7fffffffull;
           a.res->ext_count = (tc >> 31) & 0x7fffffffull;
         }
       }
+    } else if (lane == 0) {
+      sh_prefix[1] = 0;
     }
   }
 
