@@ -238,11 +238,14 @@ __device__ __forceinline__ uint32_t tile_id(uint32_t* counter) {
   return counter ? atomicAdd(counter, 1u) : blockIdx.x;
 }
 
-// Called by a full wave of tile `tile`.  Publishes `agg`, walks predecessors 64
-// at a time, publishes the inclusive prefix and returns the exclusive prefix
-// (wave-uniform).  Values add component-wise, so two 31-bit counters packed
+// Called by a full wave of tile `tile`.  Publishes `agg`, walks predecessors
+// kLookWindow at a time (lanes 0 .. kLookWindow - 1; the nearest inclusive prefix is
+// usually a few tiles back, and every examined word is an atomic load: 32 measured
+// faster than 8, 16, 64 or 256), publishes the inclusive prefix and returns the
+// exclusive prefix (wave-uniform).  Values add component-wise, so two 31-bit counters packed
 // into one value are scanned together.  A spin that exceeds kSpinLimit sets
 // *err and gives up (the result is then wrong but the kernel still ends).
+constexpr int kLookWindow = 32;
 __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg, uint32_t* err) {
   const int lane = threadIdx.x & (kWave - 1);
   if (tile == 0) {
@@ -255,7 +258,8 @@ __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg, uint
   uint32_t spins = 0;
   while (true) {
     int64_t idx = pos - lane;
-    uint64_t s = idx >= 0 ? ld_status(&status[idx]) : kFlagPre;  // virtual prefix 0 before tile 0
+    // lanes past the window hold an empty aggregate; a virtual prefix 0 before tile 0
+    uint64_t s = lane >= kLookWindow ? kFlagAgg : (idx >= 0 ? ld_status(&status[idx]) : kFlagPre);
     // wait until every examined predecessor has published something
     while (__any((s >> 62) == 0)) {
       if (++spins > kSpinLimit) {
@@ -276,7 +280,7 @@ __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg, uint
     uint64_t v = s & kValMask;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     excl += v;
-    pos -= kWave;
+    pos -= kLookWindow;
   }
   if (lane == 0) st_status(&status[tile], kFlagPre | ((excl + agg) & kValMask));
   return excl;
