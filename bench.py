@@ -240,6 +240,17 @@ def elem_side(tz, torch, f, shape, eb, bs, steps):
         log(f"{name}: compress+decompress {out[name]['gbs']:.1f} GB/s, ratio {out[name]['ratio']:.2f}, "
             f"kernels {cms:.3f} / {dms:.3f} ms ({out[name]['compress_hbm_frac']:.2f} / "
             f"{out[name]['decompress_hbm_frac']:.2f} of HBM peak), max err {err:.4f} eb")
+        # the same stream without its side-car: unit offsets parsed on the device
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pblk, pT = tz.elem_parse(sbuf, n, bs, ds, ra, length=nb)
+        torch.cuda.synchronize()
+        out[name]["parse_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+        out[name]["parse_matches_side_car"] = bool(pT == 1 and torch.equal(pblk, blk))
+        log(f"{name}: device parse of the stream without its side-car {out[name]['parse_ms']:.3f} ms, "
+            f"offsets equal the side-car: {out[name]['parse_matches_side_car']}")
         del sbuf, blk, dec
     # double 3D topology: critical-point records of the field, then the whole call
     types, _ = tz.classify(f.view(shape))

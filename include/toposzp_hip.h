@@ -422,6 +422,20 @@ int tszp_quality_metrics(const float *d_x, const float *d_y, int ndim, const uin
 int tszp_parse_blocks(const uint8_t *d_stream, size_t len, size_t n, int block_size, int kind, uint32_t *d_blk,
                       unsigned *nchunks, void *stream);
 
+/* Unit offsets of a device-resident element-partitioned stream (threadblock codec, or
+ * double random access with ra = 1) that arrives without its side-car, parsed on the
+ * device: replaces the host walk of the reference's threadblock decoders
+ * (SZp/src/szp_cuda_decompress.cu:480-525, :907-970) and detects the chunk count as the
+ * drop-in readers do (the largest T in 2..128 whose chunk offsets are non-decreasing
+ * and whose chunks walk, each ending where the next one starts; else 1).  d_stream: the
+ * whole stream [eb, elem_bytes wide][u64 off[T]][payload], len readable bytes of it;
+ * d_blk receives tszp_elem_units(n, block_size, T, ra) + 1 unit offsets relative to the
+ * payload, what tszp_threadblock_decompress / tszp_plain_decompress_f64 take (room for
+ * tszp_elem_units(n, block_size, 1, ra) + 130 serves every T up to 128); *T the chunk
+ * count.  Units whose largest size exceeds 2048 bytes are not supported. */
+int tszp_elem_parse(const uint8_t *d_stream, size_t len, size_t n, int block_size, int elem_bytes, int ra,
+                    uint32_t *d_blk, unsigned *T, void *stream);
+
 /* .SZp container written by the reference's CPU compress driver and read by its
  * decompress driver: [topology stream][sort-position stream][u64 topology bytes]
  * [u64 sort-position bytes] (example/testfloat_compress_fastmode1_topology.cc:105-139,

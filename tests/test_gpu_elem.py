@@ -331,3 +331,70 @@ def test_dropin_rejects_corrupt_threadblock_stream():
     p = L.szp_cuda_float_decompress_threadblock(x.size, 1e-3, 64, buf.ctypes.data_as(C.c_void_p))
     assert not p
     assert "corrupt" in tz.last_error()
+
+
+# ---- streams without their side-car: unit offsets and chunk count parsed on the device ----
+def _parse_and_decode(torch, s, data, eb, bs, ra):
+    """Parse a host stream (eb header included) uploaded without its side-car, then
+    decode it with the parsed offsets."""
+    d = torch.from_numpy(np.ascontiguousarray(s)).cuda()
+    blk, T = tz.elem_parse(d, data.size, bs, data.dtype.itemsize, ra)
+    dec = _decompress(torch, d, blk, data.size, eb, bs, T, ra, data.dtype)
+    return blk.cpu().numpy(), T, dec
+
+
+@pytest.mark.parametrize("path", ELEM, ids=os.path.basename)
+def test_elem_parse_reference_fixtures(path):
+    """Every chunk count of the reference's own streams: the parse finds it and decodes
+    the stream to the oracle's values."""
+    torch = _torch()
+    g = _load(path)
+    data, eb, bs, ra = g["data"], float(g["eb"]), int(g["bs"]), bool(g["ra"])
+    ds = data.dtype.itemsize
+    for key in g:
+        if not key.startswith("elem_T"):
+            continue
+        T = int(key[6:])
+        print(f"elem parse {os.path.basename(path)} T={T}", flush=True)
+        _, got_T, dec = _parse_and_decode(torch, g[key], data, eb, bs, ra)
+        assert got_T == T, (key, got_T)
+        want = oracle.elem_decompress(g[key][ds:], data.size, eb, bs, T, ra=ra, dtype=data.dtype)
+        assert np.array_equal(dec.view(np.uint8), want.view(np.uint8)), key
+
+
+@pytest.mark.parametrize("n,bs,T", [(1000, 64, 1), (1000, 64, 3), (4097, 400, 5), (20011, 32, 64), (77, 7, 16),
+                                    (300, 2, 128), (100003, 128, 7), (2, 8, 1), (65, 64, 2), (1 << 20, 64, 16)])
+@pytest.mark.parametrize("dt,ra", [(np.float32, False), (np.float64, False), (np.float64, True)],
+                         ids=["tb_f32", "tb_f64", "ra_f64"])
+def test_elem_parse_matches_side_car(n, bs, T, dt, ra):
+    """The parsed offsets equal the side-car the compressor wrote, the chunk count is
+    found (every chunk non-empty), and the decoded values are the side-car decode's."""
+    torch = _torch()
+    rng = np.random.default_rng(n + 3 * bs + T)
+    x = (np.cumsum(rng.normal(0, 1, n)) * 0.01 + np.sin(np.arange(n) * 0.01)).astype(dt)
+    print(f"elem parse n={n} bs={bs} T={T} {np.dtype(dt).name} ra={ra}", flush=True)
+    nbytes, out, blk = _compress(torch, x, 1e-3, bs, T, ra)
+    s = out[:nbytes].cpu().numpy()
+    pblk, got_T, dec = _parse_and_decode(torch, s, x, 1e-3, bs, ra)
+    assert got_T == T
+    assert np.array_equal(pblk, blk.cpu().numpy())
+    want = _decompress(torch, out, blk, n, 1e-3, bs, T, ra, dt)
+    assert np.array_equal(dec.view(np.uint8), want.view(np.uint8))
+
+
+def test_elem_parse_rejects_corrupt_streams():
+    torch = _torch()
+    x = (np.cumsum(np.random.default_rng(5).normal(0, 1, 5000)) * 0.01).astype(np.float32)
+    nbytes, out, _ = _compress(torch, x, 1e-3, 64, 4, False)
+    s = out[:nbytes].cpu().numpy().copy()
+    bad = s.copy()
+    bad[4:12] = 0xff  # first chunk offset not zero
+    with pytest.raises(tz.TszpError):
+        tz.elem_parse(torch.from_numpy(bad).cuda(), x.size, 64, 4, False)
+    cut = s[: nbytes // 2].copy()  # truncated: the chunks do not walk inside it
+    with pytest.raises(tz.TszpError):
+        tz.elem_parse(torch.from_numpy(cut).cuda(), x.size, 64, 4, False)
+    with pytest.raises(tz.TszpError):
+        tz.elem_parse(torch.from_numpy(s).cuda(), x.size, 64, 4, True)  # random access needs doubles
+    with pytest.raises(tz.TszpError, match="too large"):
+        tz.elem_parse(torch.from_numpy(s).cuda(), x.size, 1025, 4, False)  # units beyond 2048 bytes

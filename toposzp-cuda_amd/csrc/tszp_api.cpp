@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPlist, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsCount
+  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsCount
 };
 
 struct Ctx {
@@ -2298,6 +2298,10 @@ int tszp_parse_blocks(const uint8_t* d_stream, size_t len, size_t n, int bs, int
   ParseArgs a{};
   a.topo = topo;
   a.has_bc = topo || bs > 1;
+  a.bc_off = 4;  // SZp blocks: anchor, bit count, cur - 1 deltas
+  a.anchor = 1;
+  a.dm1 = 1;
+  a.bc_always = topo;
   uint32_t maxb = 0, minb = 0xffffffffu;
   for (unsigned bc = 0; bc <= 32; bc++) {
     const size_t m = (size_t)bs - 1;
@@ -2371,6 +2375,119 @@ int tszp_parse_blocks(const uint8_t* d_stream, size_t len, size_t n, int bs, int
     }
   }
   return fail(TSZP_EINVAL, "parse_blocks: no chunk count gives a consistent block sequence (corrupt stream?)");
+}
+
+// Unit offsets of an element-partitioned stream in device memory that arrives without
+// its side-car (the host walk of the drop-in decoders, walk_elem, on the device): the
+// chunk count is the host rule's (detect_elem_chunks: the largest T in 2..128 whose
+// offsets are non-decreasing and whose chunks walk, each ending where the next one
+// starts; else 1), and every chunk is cut into units by the segment-table parse of
+// tszp_parse.hip with the units' layout (threadblock: a bit-count byte and up to bs
+// deltas after the chunk's anchor; double random access: anchor, bit-count byte and
+// cur - 1 deltas per unit).
+int tszp_elem_parse(const uint8_t* d_full, size_t full_len, size_t n, int bs, int elem_bytes, int ra,
+                    uint32_t* d_blk, unsigned* T_out, void* stream) {
+  g_err.clear();
+  if (!d_full || !d_blk || !T_out || n == 0 || bs < 1 || bs > 1025 || (elem_bytes != 4 && elem_bytes != 8) ||
+      (ra && elem_bytes != 8))
+    return fail(TSZP_EINVAL, "elem_parse: bad arguments");
+  int r = check_n(n);
+  if (r) return r;
+  if (full_len <= (size_t)elem_bytes + 8) return fail(TSZP_EINVAL, "elem_parse: stream too short");
+  const uint8_t* d_stream = d_full + elem_bytes;  // the chunk table, after the error bound
+  const size_t len = full_len - elem_bytes;
+  // unit layout and the sizes of full units by bit count
+  ParseArgs a{};
+  a.topo = 0;
+  a.has_bc = 1;
+  a.bc_off = ra ? 4u : 0u;
+  a.anchor = ra ? 1 : 0;
+  a.dm1 = ra ? 1 : 0;
+  a.bc_always = ra ? 1 : 0;
+  const size_t m = ra ? (size_t)bs - 1 : (size_t)bs;  // deltas of a full unit
+  uint32_t maxb = 0, minb = 0xffffffffu;
+  for (unsigned bc = 0; bc <= 32; bc++) {
+    const size_t z = (ra ? 4 : 0) + 1 + (bc ? (m + 7) / 8 + fixed_len((unsigned)m, bc) : 0);
+    if (z > kParseMaxBlock) return fail(TSZP_EINVAL, "elem_parse: block size %d too large for the device parse", bs);
+    a.sz[bc] = (uint16_t)z;
+    maxb = z > maxb ? (uint32_t)z : maxb;
+    minb = z < minb ? (uint32_t)z : minb;
+  }
+  a.sz[33] = (uint16_t)minb;
+  a.maxb = maxb;
+  a.rounds = 1;
+  while (((uint64_t)1 << a.rounds) < kParseSeg / minb + 2) a.rounds++;
+  Ctx* c;
+  if ((r = get_ctx(&c))) return r;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  // the chunk table's head: the candidates
+  const size_t hb = std::min<size_t>(len, 8 * 129);
+  std::vector<unsigned char> head(hb);
+  HIPCHK(hipMemcpyAsync(head.data(), d_stream, hb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (rd64(head.data()) != 0) return fail(TSZP_EINVAL, "elem_parse: corrupt stream (first chunk offset not 0)");
+  std::vector<unsigned> cand;
+  {
+    unsigned tmax = 1;
+    for (unsigned t = 2; t <= 128 && 8 * (size_t)t <= hb; t++) {
+      const uint64_t o = rd64(head.data() + 8 * (size_t)(t - 1));
+      if (o < rd64(head.data() + 8 * (size_t)(t - 2)) || 8 * (size_t)t + o > len) break;
+      tmax = t;
+    }
+    for (unsigned t = tmax; t >= 2; t--) cand.push_back(t);
+    cand.push_back(1);
+  }
+  uint32_t *err, *ends;
+  WS(kWsMinMax, 16, err);
+  WS(kWsElemEnds, 128 * 4 + 16, ends);
+  void* work;
+  WS(kWsStream2, parse_work_bytes(len, a.maxb), work);
+  for (unsigned T : cand) {
+    if (len <= 8 * (size_t)T) continue;
+    const uint8_t* payload = d_stream + 8 * (size_t)T;
+    const uint64_t avail = len - 8 * (size_t)T;
+    if (avail > 0xffffffffull) return fail(TSZP_EINVAL, "elem_parse: payload beyond 4 GiB");
+    uint32_t ukc = 0;
+    elem_units(n, (uint32_t)bs, T, ra != 0, &ukc);
+    const uint64_t ce = n / T;
+    HIPCHK(launch_fill_u32(err, 0u, 2, st));
+    HIPCHK(launch_fill_u32(ends, 0xffffffffu, T, st));
+    bool bad = false;
+    for (unsigned t = 0; t < T && !bad; t++) {
+      const uint64_t lo = (uint64_t)t * ce, hi = t == T - 1 ? n : lo + ce, cl = hi - lo;
+      const uint64_t o = rd64(head.data() + 8 * (size_t)t);
+      const uint64_t e = t + 1 < T ? rd64(head.data() + 8 * (size_t)(t + 1)) : avail;
+      if (o > avail || e < o) {
+        bad = true;
+        break;
+      }
+      const uint64_t skip = ra ? 0 : 4;                 // the threadblock chunk's anchor
+      const uint64_t nd = ra ? cl : (cl >= 2 ? cl - 1 : 0);  // elements (RA) or deltas cut into units
+      if (!nd) continue;                                // (chunk starts and ends: parse_elem_finish)
+      if (o + skip >= e) {
+        bad = true;
+        break;
+      }
+      ParseArgs ac = a;
+      ac.p = payload + o + skip;
+      ac.len = e - o - skip;
+      ac.out_base = (uint32_t)(o + skip);
+      ac.end_out = ends + t;
+      const uint32_t nu = (uint32_t)((nd + bs - 1) / bs);
+      HIPCHK(parse_blocks(ac, nd, (uint32_t)bs, nu, d_blk + (size_t)t * ukc, work, err, st));
+    }
+    if (bad) continue;
+    HIPCHK(parse_elem_finish(d_stream, T, n, ukc, ra != 0, avail, d_blk, ends, err, st));
+    uint32_t h[2];
+    HIPCHK(hipMemcpyAsync(h, err, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!h[0]) {
+      *T_out = T;
+      return TSZP_OK;
+    }
+  }
+  return fail(TSZP_EINVAL, "elem_parse: no chunk count gives a consistent unit sequence (corrupt stream?)");
 }
 
 // .SZp container of the reference's CPU drivers: [topology stream][sort-position

@@ -379,11 +379,26 @@ struct ParseArgs {
   uint32_t rounds;   // pointer-doubling rounds: 2^rounds >= kParseSeg / smallest size + 1
   int has_bc;        // full blocks carry a bit-count byte
   int topo;          // blocks carry 2-bit types
+  // block layout (SZp blocks: anchor, bit count at +4, cur - 1 deltas); the units of the
+  // element-partitioned streams differ (tszp_elem.hip)
+  uint32_t bc_off;   // offset of the bit-count byte in a block
+  int anchor;        // a block starts with a 4-byte anchor
+  int dm1;           // a block of cur elements holds cur - 1 deltas (else cur)
+  int bc_always;     // the bit-count byte is there also without deltas
+  uint32_t out_base; // added to every offset written
+  uint32_t* end_out; // nullable: receives the end offset (+ out_base) of the last block
 };
 size_t parse_work_bytes(uint64_t len, uint32_t maxb);
 // err: 2 zeroed u32 (error bits, "last block reached")
 hipError_t parse_blocks(const ParseArgs& a, uint64_t n, uint32_t bs, uint32_t nb, uint32_t* blk, void* work,
                         uint32_t* err, hipStream_t st);
+// element-partitioned streams: blk[t * ukc] = off[t] for every threadblock chunk (whose
+// first unit holds the chunk's anchor) with its end in ends[t] when the chunk has one
+// element or none, then err |= TSZP_ERR_STREAM unless every chunk's walk ended where the
+// next chunk starts (ends[t] == off[t + 1]) and the last one within `avail` bytes; tab:
+// the stream's u64 chunk offsets (device)
+hipError_t parse_elem_finish(const uint8_t* tab, uint32_t T, uint64_t n, uint32_t ukc, bool ra, uint64_t avail,
+                             uint32_t* blk, uint32_t* ends, uint32_t* err, hipStream_t st);
 hipError_t parse_check_offsets(const uint8_t* head, uint32_t T, const uint32_t* blk, uint32_t nb, uint32_t* err,
                                hipStream_t st);
 

@@ -37,9 +37,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_seg(ParseArgs a, uint2*
   for (uint32_t i = threadIdx.x; i < kParseSeg + a.maxb; i += kBlockThreads)
     bytes[i] = s0 + i < a.len ? a.p[s0 + i] : (uint8_t)0;
   __syncthreads();
-  // one step from every position: bit count at +4, size of a full block
+  // one step from every position: bit count at +bc_off, size of a full block
   for (uint32_t q = threadIdx.x; q < kParseSeg; q += kBlockThreads) {
-    const uint32_t bc = a.has_bc ? bytes[q + 4] : 0u;
+    const uint32_t bc = a.has_bc ? bytes[q + a.bc_off] : 0u;
     J[0][q] = (uint16_t)(q + a.sz[bc <= 32 ? bc : 33]);  // >= kParseSeg: the walk has left
     Cn[0][q] = 1;
   }
@@ -82,8 +82,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_seg_walk(ParseArgs a, u
   __syncthreads();
   for (uint32_t e = threadIdx.x; e < a.maxb; e += kBlockThreads) {
     uint32_t q = e, c = 0;
-    while (q < kParseSeg) {  // every block is at least 4 bytes: at most kParseSeg / 4 steps
-      const uint32_t bc = a.has_bc ? bytes[q + 4] : 0u;
+    while (q < kParseSeg) {  // every block is at least one byte
+      const uint32_t bc = a.has_bc ? bytes[q + a.bc_off] : 0u;
       q += sz[bc <= 32 ? bc : 33];
       c++;
     }
@@ -143,11 +143,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_final(ParseArgs a, cons
   const uint64_t end = (uint64_t)(s + 1) * kParseSeg;
   uint32_t idx = en.y;
   while (pos < end && idx < nb) {
-    blk[idx] = (uint32_t)pos;
+    blk[idx] = (uint32_t)pos + a.out_base;
     const uint64_t i = (uint64_t)idx * bs;
     const uint32_t cur = i + bs > n ? (uint32_t)(n - i) : bs;
-    uint64_t q = pos + 4;
-    if (a.topo || cur > 1) {
+    const uint32_t dm = a.dm1 ? cur - 1u : cur;  // deltas of the block
+    uint64_t q = pos + (a.anchor ? 4u : 0u);
+    if (a.topo || a.bc_always || dm > 0) {
       if (q >= a.len) {
         atomicOr(err, TSZP_ERR_STREAM);
         return;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_final(ParseArgs a, cons
         atomicOr(err, TSZP_ERR_STREAM);
         return;
       }
-      if (bc) q += mag_bytes(cur - 1u, bc) + ((cur - 1u + 7u) >> 3);
+      if (bc) q += mag_bytes(dm, bc) + ((dm + 7u) >> 3);
     }
     if (a.topo) q += (2u * cur + 7u) >> 3;
     if (q > a.len) {
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_parse_final(ParseArgs a, cons
     pos = q;
     idx++;
     if (idx == nb) {
-      blk[nb] = (uint32_t)pos;
+      blk[nb] = (uint32_t)pos + a.out_base;
+      if (a.end_out) *a.end_out = (uint32_t)pos + a.out_base;
       err[1] = 1u;  // the last block was reached
     }
   }
@@ -227,6 +229,41 @@ hipError_t parse_blocks(const ParseArgs& a, uint64_t n, uint32_t bs, uint32_t nb
                      nseg, kParseGroup, a.maxb, e1, e0, err);
   hipLaunchKernelGGL(k_parse_final, dim3((nseg + kBlockThreads - 1) / kBlockThreads), dim3(kBlockThreads), 0, st, a,
                      e0, nseg, n, bs, nb, blk, err);
+  return hipGetLastError();
+}
+
+// ---- element-partitioned streams: chunk starts and the chunk-end check -----------------
+__device__ __forceinline__ uint64_t rd_u64(const uint8_t* p) {
+  uint64_t o = 0;
+  for (int b = 0; b < 8; b++) o |= (uint64_t)p[b] << (8 * b);
+  return o;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_parse_elem_finish(const uint8_t* __restrict__ tab, uint32_t T,
+                                                                    uint64_t n, uint32_t ukc, int ra, uint64_t avail,
+                                                                    uint32_t* __restrict__ blk,
+                                                                    const uint32_t* __restrict__ ends,
+                                                                    uint32_t* err) {
+  const uint32_t t = blockIdx.x * kBlockThreads + threadIdx.x;
+  if (t >= T) return;
+  const uint64_t ce = n / T, lo = (uint64_t)t * ce, hi = t == T - 1 ? n : lo + ce;
+  const uint64_t o = rd_u64(tab + 8 * (size_t)t);
+  uint64_t e = ends[t];
+  if (hi - lo == 0) e = o;                     // no units
+  else if (!ra && hi - lo == 1) e = o + 4;     // the anchor alone
+  if (!ra && hi > lo) blk[(size_t)t * ukc] = (uint32_t)o;  // the first unit holds the anchor
+  if (t + 1 < T) {
+    if (e != rd_u64(tab + 8 * (size_t)(t + 1))) atomicOr(err, TSZP_ERR_STREAM);
+  } else {
+    if (e == 0xffffffffull || e > avail) atomicOr(err, TSZP_ERR_STREAM);
+    if (hi - lo <= (ra ? 0u : 1u)) blk[(size_t)t * ukc + (hi > lo ? 1u : 0u)] = (uint32_t)e;  // the end entry
+  }
+}
+
+hipError_t parse_elem_finish(const uint8_t* tab, uint32_t T, uint64_t n, uint32_t ukc, bool ra, uint64_t avail,
+                             uint32_t* blk, uint32_t* ends, uint32_t* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_parse_elem_finish, dim3((T + kBlockThreads - 1) / kBlockThreads), dim3(kBlockThreads), 0, st,
+                     tab, T, n, ukc, ra ? 1 : 0, avail, blk, ends, err);
   return hipGetLastError();
 }
 

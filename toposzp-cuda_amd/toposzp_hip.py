@@ -211,6 +211,8 @@ def lib():
     L.tszp_quality_metrics.restype = i32
     L.tszp_parse_blocks.argtypes = [vp, sz, sz, i32, i32, vp, C.POINTER(u32), vp]
     L.tszp_parse_blocks.restype = i32
+    L.tszp_elem_parse.argtypes = [vp, sz, sz, i32, i32, i32, vp, C.POINTER(u32), vp]
+    L.tszp_elem_parse.restype = i32
     L.tszp_szp_pack.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz), vp]
     L.tszp_szp_pack.restype = i32
     L.tszp_szp_unpack.argtypes = [vp, sz, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), u64p]
@@ -927,6 +929,20 @@ def parse_blocks(stream, n, bs, kind, length=None):
     _check(lib().tszp_parse_blocks(_dp(stream), length if length is not None else stream.numel(), n, bs,
                                    PARSE_KINDS[kind], _dp(blk), C.byref(T), _stream()))
     return blk, T.value
+
+
+def elem_parse(stream, n, bs, elem_bytes, ra=False, length=None):
+    """Unit offsets (int32 tensor, tszp_elem_units(n, bs, T, ra) + 1 of them) and chunk
+    count of a device element-partitioned stream (eb header included) written without
+    its side-car."""
+    torch = _torch()
+    L = lib()
+    cap = L.tszp_elem_units(n, bs, 1, int(ra)) + 130  # enough for any chunk count up to 128
+    blk = torch.empty(cap, dtype=torch.int32, device=stream.device)
+    T = C.c_uint32()
+    _check(L.tszp_elem_parse(_dp(stream), length if length is not None else stream.numel(), n, bs, elem_bytes,
+                             int(ra), _dp(blk), C.byref(T), _stream()))
+    return blk[:L.tszp_elem_units(n, bs, T.value, int(ra)) + 1], T.value
 
 
 def quality_metrics(x, y, shape, eb):
