@@ -5,29 +5,40 @@
 
 namespace tszp {
 
+// one range: words up to 16-byte alignment, then 16-byte stores, then the tail
+__device__ __forceinline__ void fill_range(uint32_t* p, uint32_t v, size_t count, size_t tid, size_t stride) {
+  size_t head = ((16u - (reinterpret_cast<uintptr_t>(p) & 15u)) & 15u) / 4u;  // p is 4-byte aligned
+  if (head > count) head = count;
+  if (tid < head) p[tid] = v;
+  uint4* p4 = reinterpret_cast<uint4*>(p + head);
+  const size_t n4 = (count - head) / 4;
+  const uint4 v4 = make_uint4(v, v, v, v);
+  for (size_t i = tid; i < n4; i += stride) p4[i] = v4;
+  for (size_t i = head + 4 * n4 + tid; i < count; i += stride) p[i] = v;
+}
+
+// (a few hundred workgroups at most: a fill is store-bound, and empty workgroups of a
+// large grid still cost their dispatch)
+static inline unsigned fill_blocks(size_t count) {
+  const size_t b = (count + 4 * kBlockThreads - 1) / (4 * kBlockThreads);
+  return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < count; i += stride) p[i] = v;
+  fill_range(p, v, count, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t st) {
   if (!count) return hipSuccess;
-  size_t blocks = (count + kBlockThreads - 1) / kBlockThreads;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(kBlockThreads), 0, st, p, v, count);
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(count)), dim3(kBlockThreads), 0, st, p, v, count);
   return hipGetLastError();
 }
 
 // several ranges in one launch: blockIdx.y picks the range
 __global__ void k_fill_multi(FillRanges r) {
   const uint32_t k = blockIdx.y;
-  uint32_t* p = r.p[k];
-  const uint32_t v = r.v[k];
-  const size_t count = r.count[k];
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < count; i += stride) p[i] = v;
+  fill_range(r.p[k], r.v[k], r.count[k], (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+             (size_t)gridDim.x * blockDim.x);
 }
 
 hipError_t launch_fill_multi(const FillRanges& r, hipStream_t st) {
@@ -35,9 +46,7 @@ hipError_t launch_fill_multi(const FillRanges& r, hipStream_t st) {
   size_t most = 0;
   for (int k = 0; k < r.n; k++) most = r.count[k] > most ? r.count[k] : most;
   if (!most) return hipSuccess;
-  size_t blocks = (most + kBlockThreads - 1) / kBlockThreads;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)blocks, (unsigned)r.n), dim3(kBlockThreads), 0, st, r);
+  hipLaunchKernelGGL(k_fill_multi, dim3(fill_blocks(most), (unsigned)r.n), dim3(kBlockThreads), 0, st, r);
   return hipGetLastError();
 }
 
