@@ -1305,21 +1305,161 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_rel4(Grid g, const float* 
   }
 }
 
-// Pass 1: the plateau of every listed point, given up beyond three points; one thread
-// per plateau (the first to claim its smallest point) lists it as {P0, P1, P2, np}.
+// Solve of one plateau of two or three points.  Every other neighbour of such a plateau
+// lies in another cell, so only the order inside the plateau decides the
+// types of its points, and no other point's type depends on it.  Each point's type is
+// tabulated for every relation (<, =, >) to each of the other plateau points (a grid
+// has no triangles, so a point sees at most two of them); the 3 (two points) or 13
+// (three points) weak orders are then table look-ups, and the first that gives every
+// point its type is written.
+__constant__ const int8_t kWeak3[13][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
+                                           {1, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                           {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+__constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+
 template <int ND, class GEO>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(GEO geo, const uint32_t* __restrict__ bad,
-                                                               uint32_t nbad, const uint16_t* __restrict__ rel,
-                                                               const uint8_t* __restrict__ badmap,
-                                                               uint4* __restrict__ plist) {
+__device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&P)[3], int np,
+                                               const float* __restrict__ d, const uint16_t* __restrict__ rel,
+                                               const uint8_t* __restrict__ t, float* __restrict__ y, int anchor) {
   constexpr int K = 2 * ND;
+  bool done = false;
+  const float db = d[P[0]];
+  const FieldView F = geo.at(P[0]);
+  const Grid& g = F.g;
+  const float eb = F.eb;
+  int tp[3];
+  bool intr[3], anc[3];
+  float yv[3];
+  uint32_t ttab[3];  // type of point j for relations (ra, rb) to its two other points: 2 bits each
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    tp[j] = 0;
+    intr[j] = false;
+    anc[j] = false;
+    yv[j] = 0.f;
+    ttab[j] = 0;
+    if (j >= np) continue;
+    const uint32_t e = P[j];
+    int c[ND];
+    coords_of<ND>(g, e - F.base, c);
+    tp[j] = t[e];
+    intr[j] = interior<ND>(g, c);
+    anc[j] = anchor && anchored(tp[j], db);
+    yv[j] = y[e];
+    if (!intr[j]) continue;
+    // the other two plateau indices of j, in increasing order
+    const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
+    int code[K];  // +-1: fixed sign of (point - neighbour); 2: other point oa; 3: other point ob
+    const uint32_t rm = rel[e];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t s = stride_of<ND>(g, k >> 1);
+      const uint32_t qe = (k & 1) ? e + s : e - s;
+      // a same-value neighbour of a plateau of at most three points is one of them
+      code[k] = (oa < np && qe == P[oa]) ? 2 : (ob < np && qe == P[ob]) ? 3 : (((rm >> (8 + k)) & 1u) ? 1 : -1);
+    }
+#pragma unroll
+    for (int ra = -1; ra <= 1; ra++) {
+#pragma unroll
+      for (int rb = -1; rb <= 1; rb++) {
+        bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+          const int c0 = code[2 * a], c1 = code[2 * a + 1];
+          const int r0 = c0 == 2 ? ra : c0 == 3 ? rb : c0;
+          const int r1 = c1 == 2 ? ra : c1 == 3 ? rb : c1;
+          const bool hi2 = r0 > 0 && r1 > 0, lo2 = r0 < 0 && r1 < 0;
+          all_gt = all_gt && hi2;
+          all_lt = all_lt && lo2;
+          any_hi = any_hi || hi2;
+          any_lo = any_lo || lo2;
+        }
+        const uint32_t k = all_gt ? 1u : (all_lt ? 2u : ((any_hi && any_lo) ? 3u : 0u));
+        ttab[j] |= k << (2 * (3 * (ra + 1) + (rb + 1)));
+      }
+    }
+  }
+  float lo, hi;
+  cell_of(db, eb, &lo, &hi);
+  const int nord = np == 2 ? 3 : 13;
+  for (int o = 0; o < nord && !done; o++) {
+    int rk[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) rk[j] = j < np ? (np == 2 ? kWeak2[o][j] : kWeak3[o][j]) : 0;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j >= np || !intr[j]) continue;
+      const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
+      const int ra = rk[j] > rk[oa] ? 1 : (rk[j] < rk[oa] ? -1 : 0);
+      const int rb = ob < np ? (rk[j] > rk[ob] ? 1 : (rk[j] < rk[ob] ? -1 : 0)) : 0;
+      ok = ok && (int)((ttab[j] >> (2 * (3 * (ra + 1) + (rb + 1)))) & 3u) == tp[j];
+    }
+    if (!ok) continue;
+    // values: classes holding an anchor keep its value, the others are spaced
+    // evenly between their neighbours (or the interval ends)
+    int nr = 0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (j < np) nr = max(nr, rk[j] + 1);
+    float cv[3] = {0.f, 0.f, 0.f};
+    bool ca[3] = {false, false, false};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j >= np || !anc[j]) continue;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (k != rk[j]) continue;
+        if (ca[k] && cv[k] != yv[j]) ok = false;
+        ca[k] = true;
+        cv[k] = yv[j];
+      }
+    }
+    if (!ok) continue;
+    float fv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      fv[k] = cv[k];
+      if (k >= nr || ca[k]) continue;
+      int pa = -1, na = nr;  // nearest anchored classes below and above
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        if (m < nr && ca[m] && m < k) pa = m;
+        if (m < nr && ca[m] && m > k && na == nr) na = m;
+      }
+      const float L = pa >= 0 ? (pa == 0 ? cv[0] : (pa == 1 ? cv[1] : cv[2])) : lo;
+      const float U = na < nr ? (na == 0 ? cv[0] : (na == 1 ? cv[1] : cv[2])) : hi;
+      fv[k] = L + (U - L) * ((float)(k - pa) / (float)(na - pa));
+    }
+#pragma unroll
+    for (int k = 0; k + 1 < 3; k++) ok = ok && (k + 1 >= nr || fv[k] < fv[k + 1]);
+    if (!ok) continue;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
+    done = true;
+  }
+  return done;
+}
+
+// The plateau of every listed point, given up beyond three points; the thread of the
+// plateau's smallest listed point solves it.  Plateaus are disjoint and a solve reads
+// and writes only its own points' values, so no thread waits for another.
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
+                                                          uint32_t nbad, const float* __restrict__ d,
+                                                          const uint16_t* __restrict__ rel,
+                                                          const uint8_t* __restrict__ badmap,
+                                                          const uint8_t* __restrict__ t, float* __restrict__ y,
+                                                          int anchor, uint32_t* __restrict__ part) {
+  constexpr int K = 2 * ND;
+  __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  if (i >= nbad) return;
-  const uint32_t b = bad[i];
+  const uint32_t b = i < nbad ? bad[i] : 0u;
   const Grid g = geo.at(b).g;  // a plateau lies in one field
   uint32_t P[3] = {b, b, b};
   int np = 1;
-  bool big = false;
+  bool big = i >= nbad;
 #pragma unroll
   for (int u = 0; u < 3; u++) {
     if (u >= np || big) continue;
@@ -1340,162 +1480,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_find(GEO geo, const u
       }
     }
   }
-  // the plateau goes to the thread of its smallest mismatched point
+  // the plateau goes to the thread of its smallest mismatched point, which solves it
   bool take = !big && np >= 2;
   if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
   if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
-  plist[i] = take ? make_uint4(P[0], P[1], P[2], (uint32_t)np) : make_uint4(0, 0, 0, 0);
-}
-
-// Pass 2, one thread per listed plateau.  Every other neighbour of a plateau of two or
-// three points lies in another cell, so only the order inside the plateau decides the
-// types of its points, and no other point's type depends on it.  Each point's type is
-// tabulated for every relation (<, =, >) to each of the other plateau points (a grid
-// has no triangles, so a point sees at most two of them); the 3 (two points) or 13
-// (three points) weak orders are then table look-ups, and the first that gives every
-// point its type is written.
-__constant__ const int8_t kWeak3[13][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
-                                           {1, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 2, 1}, {1, 0, 2},
-                                           {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-__constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
-
-template <int ND, class GEO>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_tiny_solve(GEO geo, const uint4* __restrict__ plist,
-                                                                const float* __restrict__ d,
-                                                                const uint16_t* __restrict__ rel,
-                                                                const uint8_t* __restrict__ t, float* __restrict__ y,
-                                                                int anchor, uint32_t count,
-                                                                uint32_t* __restrict__ part) {
-  constexpr int K = 2 * ND;
-  __shared__ uint32_t red[kWavesPerBlock];
-  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  bool done = false;
-  const uint4 ent = i < count ? plist[i] : make_uint4(0, 0, 0, 0);
-  if (ent.w >= 2) {
-    const uint32_t P[3] = {ent.x, ent.y, ent.z};
-    const int np = (int)ent.w;
-    const float db = d[P[0]];
-    const FieldView F = geo.at(P[0]);
-    const Grid& g = F.g;
-    const float eb = F.eb;
-    int tp[3];
-    bool intr[3], anc[3];
-    float yv[3];
-    uint32_t ttab[3];  // type of point j for relations (ra, rb) to its two other points: 2 bits each
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      tp[j] = 0;
-      intr[j] = false;
-      anc[j] = false;
-      yv[j] = 0.f;
-      ttab[j] = 0;
-      if (j >= np) continue;
-      const uint32_t e = P[j];
-      int c[ND];
-      coords_of<ND>(g, e - F.base, c);
-      tp[j] = t[e];
-      intr[j] = interior<ND>(g, c);
-      anc[j] = anchor && anchored(tp[j], db);
-      yv[j] = y[e];
-      if (!intr[j]) continue;
-      // the other two plateau indices of j, in increasing order
-      const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
-      int code[K];  // +-1: fixed sign of (point - neighbour); 2: other point oa; 3: other point ob
-      const uint32_t rm = rel[e];
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        const uint32_t s = stride_of<ND>(g, k >> 1);
-        const uint32_t qe = (k & 1) ? e + s : e - s;
-        // a same-value neighbour of a plateau of at most three points is one of them
-        code[k] = (oa < np && qe == P[oa]) ? 2 : (ob < np && qe == P[ob]) ? 3 : (((rm >> (8 + k)) & 1u) ? 1 : -1);
-      }
-#pragma unroll
-      for (int ra = -1; ra <= 1; ra++) {
-#pragma unroll
-        for (int rb = -1; rb <= 1; rb++) {
-          bool all_gt = true, all_lt = true, any_hi = false, any_lo = false;
-#pragma unroll
-          for (int a = 0; a < ND; a++) {
-            const int c0 = code[2 * a], c1 = code[2 * a + 1];
-            const int r0 = c0 == 2 ? ra : c0 == 3 ? rb : c0;
-            const int r1 = c1 == 2 ? ra : c1 == 3 ? rb : c1;
-            const bool hi2 = r0 > 0 && r1 > 0, lo2 = r0 < 0 && r1 < 0;
-            all_gt = all_gt && hi2;
-            all_lt = all_lt && lo2;
-            any_hi = any_hi || hi2;
-            any_lo = any_lo || lo2;
-          }
-          const uint32_t k = all_gt ? 1u : (all_lt ? 2u : ((any_hi && any_lo) ? 3u : 0u));
-          ttab[j] |= k << (2 * (3 * (ra + 1) + (rb + 1)));
-        }
-      }
-    }
-    float lo, hi;
-    cell_of(db, eb, &lo, &hi);
-    const int nord = np == 2 ? 3 : 13;
-    for (int o = 0; o < nord && !done; o++) {
-      int rk[3];
-#pragma unroll
-      for (int j = 0; j < 3; j++) rk[j] = j < np ? (np == 2 ? kWeak2[o][j] : kWeak3[o][j]) : 0;
-      bool ok = true;
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        if (j >= np || !intr[j]) continue;
-        const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
-        const int ra = rk[j] > rk[oa] ? 1 : (rk[j] < rk[oa] ? -1 : 0);
-        const int rb = ob < np ? (rk[j] > rk[ob] ? 1 : (rk[j] < rk[ob] ? -1 : 0)) : 0;
-        ok = ok && (int)((ttab[j] >> (2 * (3 * (ra + 1) + (rb + 1)))) & 3u) == tp[j];
-      }
-      if (!ok) continue;
-      // values: classes holding an anchor keep its value, the others are spaced
-      // evenly between their neighbours (or the interval ends)
-      int nr = 0;
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-        if (j < np) nr = max(nr, rk[j] + 1);
-      float cv[3] = {0.f, 0.f, 0.f};
-      bool ca[3] = {false, false, false};
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        if (j >= np || !anc[j]) continue;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          if (k != rk[j]) continue;
-          if (ca[k] && cv[k] != yv[j]) ok = false;
-          ca[k] = true;
-          cv[k] = yv[j];
-        }
-      }
-      if (!ok) continue;
-      float fv[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        fv[k] = cv[k];
-        if (k >= nr || ca[k]) continue;
-        int pa = -1, na = nr;  // nearest anchored classes below and above
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-          if (m < nr && ca[m] && m < k) pa = m;
-          if (m < nr && ca[m] && m > k && na == nr) na = m;
-        }
-        const float L = pa >= 0 ? (pa == 0 ? cv[0] : (pa == 1 ? cv[1] : cv[2])) : lo;
-        const float U = na < nr ? (na == 0 ? cv[0] : (na == 1 ? cv[1] : cv[2])) : hi;
-        fv[k] = L + (U - L) * ((float)(k - pa) / (float)(na - pa));
-      }
-#pragma unroll
-      for (int k = 0; k + 1 < 3; k++) ok = ok && (k + 1 >= nr || fv[k] < fv[k + 1]);
-      if (!ok) continue;
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-        if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
-      done = true;
-    }
-  }
+  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rel, t, y, anchor);
   const uint32_t nd = (uint32_t)__popcll(__ballot(done));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nd;
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
+
 
 // ---------------------------------------------------------------------------------
 // exact window solve
@@ -2997,12 +2992,11 @@ hipError_t pp_rel(const PpGeo& geo, const float* d, uint16_t* rel, hipStream_t s
 }
 
 hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
-                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint4* plist, uint32_t* part,
-                   uint32_t* misc, hipStream_t st) {
+                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc,
+                   hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  PP_GEO_LAUNCH(k_pp_tiny_find, grid, blk, st, bad, nbad, rel, badmap, plist);
-  PP_GEO_LAUNCH(k_pp_tiny_solve, grid, blk, st, plist, d, rel, t, y, anchor, nbad, part);
+  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, rel, badmap, t, y, anchor, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
