@@ -1066,13 +1066,13 @@ def main():
                          "compress_s": res1["compress_s"], "decompress_s": res1["decompress_s"]}
             try:
                 slab = f[:args.cpu_slab_t1].contiguous()
-                info, buf = tz.topo_compress(slab, eb, bs, T=1)
+                sinfo, sbuf = tz.topo_compress(slab, eb, bs, T=1)
                 ref_topo = np.fromfile(pre + ".topo", dtype=np.uint8)
                 ref_sp = np.fromfile(pre + ".sp", dtype=np.uint8)
                 checks["t1_streams_match_reference"] = bool(
-                    np.array_equal(buf.topo[:info.topo_bytes].cpu().numpy(), ref_topo) and
-                    np.array_equal(buf.sp[:info.sortpos_bytes].cpu().numpy(), ref_sp))
-                del buf
+                    np.array_equal(sbuf.topo[:sinfo.topo_bytes].cpu().numpy(), ref_topo) and
+                    np.array_equal(sbuf.sp[:sinfo.sortpos_bytes].cpu().numpy(), ref_sp))
+                del sbuf
             finally:
                 for suf in (".topo", ".sp", ".dec"):
                     if os.path.exists(pre + suf):

@@ -49,7 +49,7 @@ enum Slot {
   kWsStatus, kWsResult, kWsCpKey, kWsCpPay, kWsKeyAlt, kWsPayAlt, kWsHiKey, kWsHiAlt, kWsHist, kWsSortSt,
   kWsSp, kWsSpBlk, kWsIn, kWsOut, kWsOut2, kWsBlk, kWsTypes, kWsStream, kWsStream2, kWsCps, kWsMinMax,
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
-  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpRel, kWsPpPart, kWsPpBadmap, kWsQuality,
+  kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
   kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsCount
 };
@@ -1705,8 +1705,6 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   uint32_t tiny = 0;
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
   if (nb && nb <= cap) {
-    uint16_t* rel;
-    WS(kWsPpRel, n * 2 + 16, rel);
     if (!vec_mark) {  // the mismatch map from the list
       HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(badmap), 0u, (n + 3) / 4, st));
       HIPCHK(pp_badmap(bad, nb, badmap, st));
@@ -1717,8 +1715,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     fr.add(misc + kPpTiny, 0u, 1);
     fr.add(misc + kPpBad, 0u, 1);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_rel(geo, d, rel, st));
-    HIPCHK(pp_tiny(geo, bad, nb, d, rel, badmap, d_types, d_field, use_sp, part, misc, st));
+    HIPCHK(pp_tiny(geo, bad, nb, d, badmap, d_types, d_field, use_sp, part, misc, st));
     HIPCHK(pp_relist(geo, bad, nb, d_field, d_types, bad2, cap, misc, st));
     if ((r = read_misc())) return r;
     std::swap(bad, bad2);

@@ -458,12 +458,10 @@ hipError_t pp_move(const PpGeo& geo, const uint32_t* act, uint32_t max_act, uint
                    const uint8_t* t, float* y, uint32_t* misc, hipStream_t st);
 hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st);
-// relation masks of the decoded values (bit k: same value at neighbour k, bit 8+k:
-// smaller value at neighbour k; neighbours inside the field only)
-hipError_t pp_rel(const PpGeo& geo, const float* d, uint16_t* rel, hipStream_t st);
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
-                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc,
-                   hipStream_t st);
+// plateaus of two or three listed points solved exactly (relations of the decoded
+// values d computed on the fly); solved counts go to misc[kPpTiny]
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+                   const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st);
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
 // mismatch map (one byte per point) from a list; the map must be zeroed first
 hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);

@@ -53,18 +53,11 @@ namespace tszp {
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kXcds = 8;
 
-// non-temporal 16- and 8-byte accesses for data streamed once (keeps L2 for reused rows)
+// non-temporal 16-byte loads for data streamed once (keeps L2 for reused rows)
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
-typedef unsigned nt_u2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float4 ld_nt(const float4* p) {
   const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
   return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void st_nt(uint2* p, uint2 v) {
-  nt_u2 w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<nt_u2*>(p));
 }
 struct XcdSweep {
   uint32_t begin, end, step;
@@ -1188,121 +1181,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_remark(GEO geo, const uint
 }
 
 // ---------------------------------------------------------------------------------
-// exact solve of plateaus of two or three points.  Every other neighbour of such a
-// plateau lies in another cell, so only the order inside the plateau decides the
-// types of its points, and no other point's type depends on it: the 3 (two points)
-// or 13 (three points) weak orders are checked in registers and the first that gives
-// every point its type is written.
+// exact solve of plateaus of two or three points
 // ---------------------------------------------------------------------------------
 
-
-// ---------------------------------------------------------------------------------
-// relation masks of the decoded values: for every point and each of its 2*ND
-// neighbours (axis a: bit 2a at -1, bit 2a+1 at +1), bit k says the neighbour is inside
-// the field with the same decoded value, bit 8+k that it is inside with a smaller one.
-// The plateau passes read one 16-bit mask per point instead of its neighbours' values.
-// ---------------------------------------------------------------------------------
+// relation mask of one point's decoded value: for each of its 2*ND neighbours (axis a:
+// bit 2a at -1, bit 2a+1 at +1), bit k says the neighbour is inside the field with the
+// same value, bit 8+k that it is inside with a smaller one
 template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_rel(Grid g, const float* __restrict__ d, uint32_t n,
-                                                         uint16_t* __restrict__ rel) {
-  for (uint32_t e = blockIdx.x * kBlockThreads + threadIdx.x; e < n; e += gridDim.x * kBlockThreads) {
-    int c[ND];
-    coords_of<ND>(g, e, c);
-    const float v = d[e];
-    uint32_t r = 0;
+__device__ __forceinline__ uint32_t rel_of(const Grid& g, uint32_t base, const float* __restrict__ d, uint32_t e) {
+  int c[ND];
+  coords_of<ND>(g, e - base, c);
+  const float v = d[e];
+  uint32_t r = 0;
 #pragma unroll
-    for (int k = 0; k < 2 * ND; k++) {
-      const int a = k >> 1;
-      const bool ok = (k & 1) ? c[a] + 1 < dim_of<ND>(g, a) : c[a] >= 1;
-      if (!ok) continue;
-      const uint32_t s = stride_of<ND>(g, a);
-      const float q = d[(k & 1) ? e + s : e - s];
-      r |= (q == v ? 1u : 0u) << k;
-      r |= (v > q ? 1u : 0u) << (8 + k);
-    }
-    rel[e] = (uint16_t)r;
+  for (int k = 0; k < 2 * ND; k++) {
+    const int a = k >> 1;
+    const bool ok = (k & 1) ? c[a] + 1 < dim_of<ND>(g, a) : c[a] >= 1;
+    if (!ok) continue;
+    const uint32_t s = stride_of<ND>(g, a);
+    const float q = d[(k & 1) ? e + s : e - s];
+    r |= (q == v ? 1u : 0u) << k;
+    r |= (v > q ? 1u : 0u) << (8 + k);
   }
-}
-
-// the same over a batch of fields, in chunks of kMarkChunk points (a chunk lies in one
-// field)
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_rel_b(GeoB geo, const float* __restrict__ d, uint32_t nchunks,
-                                                           uint16_t* __restrict__ rel) {
-  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const uint32_t c0 = ch * kMarkChunk;
-    const FieldView F = geo.at(c0);
-    const uint32_t end = F.base + F.n;
-    const Grid& g = F.g;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t e = c0 + j * kBlockThreads + threadIdx.x;
-      if (e >= end) continue;
-      int c[ND];
-      coords_of<ND>(g, e - F.base, c);
-      const float v = d[e];
-      uint32_t r = 0;
-#pragma unroll
-      for (int k = 0; k < 2 * ND; k++) {
-        const int a = k >> 1;
-        const bool ok = (k & 1) ? c[a] + 1 < dim_of<ND>(g, a) : c[a] >= 1;
-        if (!ok) continue;
-        const uint32_t s = stride_of<ND>(g, a);
-        const float q = d[(k & 1) ? e + s : e - s];
-        r |= (q == v ? 1u : 0u) << k;
-        r |= (v > q ? 1u : 0u) << (8 + k);
-      }
-      rel[e] = (uint16_t)r;
-    }
-  }
-}
-
-// the same with 4 consecutive points per thread along the fastest axis (its length a
-// multiple of 4, buffers aligned): one float4 per stencil row
-template <int ND>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_rel4(Grid g, const float* __restrict__ d, uint32_t n4,
-                                                          uint2* __restrict__ rel4) {
-  constexpr int FA = ND - 1;
-  const float4* d4 = reinterpret_cast<const float4*>(d);
-  const int dz = dim_of<ND>(g, FA);
-  const XcdSweep sw = xcd_sweep(n4, kBlockThreads);
-  for (uint32_t gi = sw.begin + threadIdx.x; gi < sw.end; gi += sw.step) {
-    const uint32_t e0 = 4 * gi;
-    int c[ND];
-    coords_of<ND>(g, e0, c);
-    const float4 v4 = d4[gi];
-    const float v[4] = {v4.x, v4.y, v4.z, v4.w};
-    float lo[FA][4], hi[FA][4];
-    bool hlo[FA], hhi[FA];
-#pragma unroll
-    for (int a = 0; a < FA; a++) {
-      const uint32_t s4 = stride_of<ND>(g, a) >> 2;
-      hlo[a] = c[a] >= 1;
-      hhi[a] = c[a] + 1 < dim_of<ND>(g, a);
-      // unconditional 16-byte loads (a missing neighbour reads the centre, unused)
-      const float4 l = d4[hlo[a] ? gi - s4 : gi], h = d4[hhi[a] ? gi + s4 : gi];
-      lo[a][0] = l.x; lo[a][1] = l.y; lo[a][2] = l.z; lo[a][3] = l.w;
-      hi[a][0] = h.x; hi[a][1] = h.y; hi[a][2] = h.z; hi[a][3] = h.w;
-    }
-    const int z = c[FA];
-    const float zm = d[z >= 1 ? e0 - 1 : e0], zp = d[z + 4 < dz ? e0 + 4 : e0];
-    uint32_t r[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      uint32_t m = 0;
-#pragma unroll
-      for (int a = 0; a < FA; a++) {
-        if (hlo[a]) m |= (lo[a][i] == v[i] ? 1u : 0u) << (2 * a) | (v[i] > lo[a][i] ? 1u : 0u) << (8 + 2 * a);
-        if (hhi[a]) m |= (hi[a][i] == v[i] ? 1u : 0u) << (2 * a + 1) | (v[i] > hi[a][i] ? 1u : 0u) << (9 + 2 * a);
-      }
-      const bool hzl = z + i >= 1, hzh = z + i + 1 < dz;
-      const float ql = i == 0 ? zm : v[i - 1], qh = i == 3 ? zp : v[i + 1];
-      if (hzl) m |= (ql == v[i] ? 1u : 0u) << (2 * FA) | (v[i] > ql ? 1u : 0u) << (8 + 2 * FA);
-      if (hzh) m |= (qh == v[i] ? 1u : 0u) << (2 * FA + 1) | (v[i] > qh ? 1u : 0u) << (9 + 2 * FA);
-      r[i] = m;
-    }
-    st_nt(&rel4[gi], make_uint2(r[0] | r[1] << 16, r[2] | r[3] << 16));
-  }
+  return r;
 }
 
 // Solve of one plateau of two or three points.  Every other neighbour of such a plateau
@@ -1319,7 +1220,7 @@ __constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
 
 template <int ND, class GEO>
 __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&P)[3], int np,
-                                               const float* __restrict__ d, const uint16_t* __restrict__ rel,
+                                               const float* __restrict__ d, const uint32_t (&rms)[3],
                                                const uint8_t* __restrict__ t, float* __restrict__ y, int anchor) {
   constexpr int K = 2 * ND;
   bool done = false;
@@ -1350,7 +1251,7 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
     // the other two plateau indices of j, in increasing order
     const int oa = j == 0 ? 1 : 0, ob = j == 2 ? 1 : 2;
     int code[K];  // +-1: fixed sign of (point - neighbour); 2: other point oa; 3: other point ob
-    const uint32_t rm = rel[e];
+    const uint32_t rm = rms[j];
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const uint32_t s = stride_of<ND>(g, k >> 1);
@@ -1448,7 +1349,6 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
 template <int ND, class GEO>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
                                                           uint32_t nbad, const float* __restrict__ d,
-                                                          const uint16_t* __restrict__ rel,
                                                           const uint8_t* __restrict__ badmap,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
                                                           int anchor, uint32_t* __restrict__ part) {
@@ -1456,14 +1356,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
   __shared__ uint32_t red[kWavesPerBlock];
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
   const uint32_t b = i < nbad ? bad[i] : 0u;
-  const Grid g = geo.at(b).g;  // a plateau lies in one field
+  const FieldView F = geo.at(b);  // a plateau lies in one field
+  const Grid& g = F.g;
   uint32_t P[3] = {b, b, b};
+  uint32_t rms[3] = {0u, 0u, 0u};
   int np = 1;
   bool big = i >= nbad;
 #pragma unroll
   for (int u = 0; u < 3; u++) {
     if (u >= np || big) continue;
-    const uint32_t eq = rel[P[u]] & ((1u << K) - 1u);  // same-value neighbours
+    rms[u] = rel_of<ND>(g, F.base, d, P[u]);
+    const uint32_t eq = rms[u] & ((1u << K) - 1u);  // same-value neighbours
 #pragma unroll
     for (int k = 0; k < K; k++) {
       if (!((eq >> k) & 1u)) continue;
@@ -1484,7 +1387,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
   bool take = !big && np >= 2;
   if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
   if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
-  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rel, t, y, anchor);
+  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor);
   const uint32_t nd = (uint32_t)__popcll(__ballot(done));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nd;
   __syncthreads();
@@ -2966,37 +2869,11 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
   return hipGetLastError();
 }
 
-hipError_t pp_rel(const PpGeo& geo, const float* d, uint16_t* rel, hipStream_t st) {
-  const uint32_t n = geo.n;
-  if (geo.fields) {
-    const uint32_t nchunks = (n + kMarkChunk - 1) / kMarkChunk;
-    if (geo.ndim == 2)
-      hipLaunchKernelGGL(k_pp_rel_b<2>, dim3(kMarkBlocks), dim3(kBlockThreads), 0, st, geob(geo), d, nchunks, rel);
-    else hipLaunchKernelGGL(k_pp_rel_b<3>, dim3(kMarkBlocks), dim3(kBlockThreads), 0, st, geob(geo), d, nchunks, rel);
-    return hipGetLastError();
-  }
-  const Grid& g = geo.g;
-  const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
-  const bool vec = fast % 4 == 0 && ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(rel)) & 15) == 0;
-  if (vec) {
-    const dim3 grid(2048), blk(kBlockThreads);  // a multiple of the XCD count
-    uint2* r4 = reinterpret_cast<uint2*>(rel);
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_rel4<2>, grid, blk, 0, st, g, d, n / 4, r4);
-    else hipLaunchKernelGGL(k_pp_rel4<3>, grid, blk, 0, st, g, d, n / 4, r4);
-  } else {
-    const dim3 grid(grid_stride_blocks(n)), blk(kBlockThreads);
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_rel<2>, grid, blk, 0, st, g, d, n, rel);
-    else hipLaunchKernelGGL(k_pp_rel<3>, grid, blk, 0, st, g, d, n, rel);
-  }
-  return hipGetLastError();
-}
-
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint16_t* rel,
-                   const uint8_t* badmap, const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc,
-                   hipStream_t st) {
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+                   const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
-  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, rel, badmap, t, y, anchor, part);
+  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, badmap, t, y, anchor, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
