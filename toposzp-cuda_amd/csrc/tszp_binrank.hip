@@ -298,6 +298,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_plan(uint32_t B, uint32_t
 }
 
 // ---- 5. scatter keys into buckets ---------------------------------------------------
+constexpr uint32_t kScatterChunk = 8192;  // keys per workgroup and step, ordered by bin in LDS
 __global__ __launch_bounds__(kBlockThreads) void k_bin_scatter(const uint32_t* __restrict__ key,
                                                               const uint32_t* __restrict__ pay, uint32_t P, float eb,
                                                               double inv, int bmin, uint32_t B, uint32_t slice,
@@ -305,33 +306,80 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_scatter(const uint32_t* _
                                                               const uint32_t* __restrict__ info,
                                                               uint32_t* __restrict__ bkey,
                                                               uint32_t* __restrict__ bpay) {
-  __shared__ uint32_t cur[kBinMax];  // next slot of this slice in each bin's bucket
+  __shared__ uint32_t cur[kBinMax];      // next slot of this slice in each bin's bucket
+  __shared__ uint32_t run[kBinMax + 1];  // per step: keys of each bin, then where its run starts
+  __shared__ uint32_t skey[kScatterChunk];
+  __shared__ uint32_t lds4[kWavesPerBlock];
+  const int tid = threadIdx.x;
   const uint32_t nslice = gridDim.x;
-  for (uint32_t b = threadIdx.x; b < B; b += kBlockThreads) cur[b] = hist[(size_t)b * nslice + blockIdx.x];
+  for (uint32_t b = tid; b < B; b += kBlockThreads) {
+    cur[b] = hist[(size_t)b * nslice + blockIdx.x];
+    run[b] = 0;
+  }
   const uint32_t wlo = info[2], whi = info[3];
+  // the bins this thread scans and advances
+  const uint32_t PB = (B + kBlockThreads - 1) / kBlockThreads;
+  const uint32_t b0 = min(tid * PB, B), b1 = min(b0 + PB, B);
   __syncthreads();
-  // consecutive lanes take consecutive keys: neighbouring critical points share bins,
-  // so each store instruction touches few bucket runs
   const uint32_t s = blockIdx.x * slice, e = min(s + slice, P);
-  constexpr int U = 2 * kUnroll;
-  for (uint32_t i0 = s + threadIdx.x; i0 < e; i0 += kBlockThreads * U) {
-    uint32_t kv[U];
+  constexpr int U = kScatterChunk / kBlockThreads;
+  constexpr uint32_t kSkip = 0xffffffffu;
+  for (uint32_t c0 = s; c0 < e; c0 += kScatterChunk) {
+    uint32_t kv[U], bv[U], rk[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t i = i0 + u * kBlockThreads;
+      const uint32_t i = c0 + u * kBlockThreads + tid;
       kv[u] = i < e ? key[i] : 0u;
     }
+    // 1. each key's bin and its rank among the step's keys of that bin; keys of the wide
+    //    bins (a few, for the radix path, which needs their payloads) go straight out
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t i = i0 + u * kBlockThreads;
-      const uint32_t b = (uint32_t)(key_bin(kv[u], eb, inv) - bmin);
-      if (i >= e || b >= B) continue;
-      const uint32_t pos = atomicAdd(&cur[b], 1u);
-      if (pos < P) {
-        bkey[pos] = kv[u];
-        if (b >= wlo && b <= whi) bpay[pos] = pay[i];  // the radix path needs these payloads
+      const uint32_t i = c0 + u * kBlockThreads + tid;
+      uint32_t b = (uint32_t)(key_bin(kv[u], eb, inv) - bmin);
+      if (i >= e || b >= B) b = kSkip;
+      if (b != kSkip && b >= wlo && b <= whi) {
+        const uint32_t pos = atomicAdd(&cur[b], 1u);
+        if (pos < P) {
+          bkey[pos] = kv[u];
+          bpay[pos] = pay[i];
+        }
+        b = kSkip;
       }
+      bv[u] = b;
+      rk[u] = b != kSkip ? atomicAdd(&run[b], 1u) : 0u;
     }
+    __syncthreads();
+    // 2. counts -> run starts (exclusive scan over the bins)
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += run[b];
+    uint32_t tot;
+    uint32_t r = block_excl_scan(sum, lds4, &tot);  // ends with a barrier
+    for (uint32_t b = b0; b < b1; b++) {
+      const uint32_t c = run[b];
+      run[b] = r;
+      r += c;
+    }
+    if (tid == 0) run[B] = tot;
+    __syncthreads();
+    // 3. the step's keys in bin order
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (bv[u] != kSkip) skey[run[bv[u]] + rk[u]] = kv[u];
+    __syncthreads();
+    // 4. consecutive lanes store consecutive slots of each bin's run
+    for (uint32_t j = tid; j < tot; j += kBlockThreads) {
+      const uint32_t k = skey[j];
+      const uint32_t b = (uint32_t)(key_bin(k, eb, inv) - bmin);
+      const uint32_t pos = cur[b] + (j - run[b]);
+      if (pos < P) bkey[pos] = k;
+    }
+    __syncthreads();
+    // 5. advance the cursors by the step's counts, then clear the counts
+    for (uint32_t b = b0; b < b1; b++) cur[b] += run[b + 1] - run[b];
+    __syncthreads();
+    for (uint32_t b = b0; b < b1; b++) run[b] = 0;
+    __syncthreads();
   }
 }
 
