@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsCount
+  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsPpOrd, kWsCount
 };
 
 struct Ctx {
@@ -1675,12 +1675,20 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + pp_tiny_parts(cap) + 64) * 4, part);
   const bool vec_mark = pp_mark_maps(geo);
   uint8_t* badmap = nullptr;
+  uint64_t* ordst = nullptr;
+  if (vec_mark && !geo.fields) WS(kWsPpOrd, (size_t)pp_compact_tiles(N) * 8 + 64, ordst);
   auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint8_t* bmap = nullptr) -> int {
     FillRanges fr{};
     fr.add(misc + kPpBad, 0u, 1);
     fr.add(misc + kPpVerify, 0u, 8);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_mark(geo, d_field, d_types, final_pass ? d : nullptr, bad, cap, part, bmap, final_pass, misc, st));
+    // one field with a mismatch map: the list is compacted from the map in index order
+    // (the plateau passes then visit neighbourhoods in the order of the field; a batch's
+    // map leaves the padding between fields unwritten)
+    const bool ordered = bmap && ordst;
+    HIPCHK(pp_mark(geo, d_field, d_types, final_pass ? d : nullptr, ordered ? nullptr : bad, cap, part, bmap,
+                   final_pass, misc, st));
+    if (ordered) HIPCHK(pp_compact_map(bmap, N, ordst, bad, cap, misc, st));
     int rr = read_misc();
     if (rr) return rr;
     *nb_out = h[kPpBad];

@@ -463,6 +463,12 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
 hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
                    const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st);
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
+// the mismatch list in index order from a byte map of n points (0 / 1 bytes, buffer
+// 16-byte aligned and readable up to n rounded up to 16); status: pp_compact_tiles(n)
+// words of 8 bytes; the count goes to misc[kPpBad]
+uint32_t pp_compact_tiles(uint32_t n);
+hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
+                          uint32_t* misc, hipStream_t st);
 // mismatch map (one byte per point) from a list; the map must be zeroed first
 hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
 // the listed points (no duplicates) whose types are violated in y, appended to out

@@ -71,6 +71,12 @@ __device__ __forceinline__ XcdSweep xcd_sweep(uint32_t n, uint32_t chunk) {
   r.step = per * chunk;
   return r;
 }
+// one chunk per workgroup, the chunks split into kXcds contiguous ranges (grid: a
+// multiple of kXcds from xcd_grid; chunks past the last are empty)
+__device__ __forceinline__ uint32_t xcd_chunk() {
+  return (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+}
+static inline uint32_t xcd_grid(uint32_t chunks) { return (chunks + kXcds - 1) / kXcds * kXcds; }
 
 
 // append val to list for every lane whose flag is set (one atomic per wave)
@@ -629,6 +635,66 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
     part[4 * blockIdx.x + 1] = s1;
     part[4 * blockIdx.x + 2] = s2;
     part[4 * blockIdx.x + 3] = s3;
+  }
+}
+
+// The mismatch list in index order from the byte map (bytes 0 / 1, length a multiple
+// of 4): one tile of kCompactTile points per workgroup, tiles ordered by a decoupled
+// look-back; status: one zeroed word per tile.  The last tile stores the total in
+// misc[kPpBad]; entries past cap are dropped.
+constexpr uint32_t kCompactTile = 65536;
+__device__ __forceinline__ uint32_t bytes_below(uint32_t avail) {  // mask of the first min(avail, 4) bytes
+  return avail >= 4u ? 0xffffffffu : (1u << (8u * avail)) - 1u;
+}
+__global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* __restrict__ bm16, uint32_t n,
+                                                                 uint64_t* __restrict__ status,
+                                                                 uint32_t* __restrict__ out, uint32_t cap,
+                                                                 uint32_t* __restrict__ misc) {
+  constexpr int V = kCompactTile / 16 / kBlockThreads;  // 16-byte map words per thread (consecutive)
+  __shared__ uint32_t lds4[kWavesPerBlock];
+  __shared__ uint32_t sh_excl;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t v0 = tile * (kCompactTile / 16) + threadIdx.x * V;
+  const uint32_t n16 = (n + 15) / 16;
+  uint4 m[V];
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    m[k] = v0 + k < n16 ? bm16[v0 + k] : make_uint4(0, 0, 0, 0);
+    const uint32_t first = 16u * (v0 + k);
+    if (first + 16u > n) {  // bytes at and past n are not part of the map
+      const uint32_t avail = first < n ? n - first : 0u;
+      m[k].x &= bytes_below(avail);
+      m[k].y &= bytes_below(avail > 4u ? avail - 4u : 0u);
+      m[k].z &= bytes_below(avail > 8u ? avail - 8u : 0u);
+      m[k].w &= bytes_below(avail > 12u ? avail - 12u : 0u);
+    }
+    c += __popc(m[k].x) + __popc(m[k].y) + __popc(m[k].z) + __popc(m[k].w);
+  }
+  uint32_t tot;
+  uint32_t pos = block_excl_scan(c, lds4, &tot);
+  if (threadIdx.x < kWave) {
+    const uint64_t ex = lookback(status, tile, tot, &misc[kPpErr]);
+    if (threadIdx.x == 0) {
+      sh_excl = (uint32_t)ex;
+      if (tile == gridDim.x - 1) misc[kPpBad] = (uint32_t)ex + tot;
+    }
+  }
+  __syncthreads();
+  pos += sh_excl;
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    const uint32_t w[4] = {m[k].x, m[k].y, m[k].z, m[k].w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      uint32_t b = w[q];
+      while (b) {
+        const uint32_t bit = (uint32_t)__builtin_ctz(b);
+        b &= b - 1u;
+        if (pos < cap) out[pos] = 16u * (v0 + k) + 4u * q + bit / 8u;
+        pos++;
+      }
+    }
   }
 }
 
@@ -1354,7 +1420,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
                                                           int anchor, uint32_t* __restrict__ part) {
   constexpr int K = 2 * ND;
   __shared__ uint32_t red[kWavesPerBlock];
-  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  // every XCD takes one contiguous eighth of the list, in order: a list in index order
+  // then keeps the plateaus' neighbourhoods in the L2 that loaded them
+  const uint32_t chunk = xcd_chunk();
+  const uint32_t i = chunk * kBlockThreads + threadIdx.x;
   const uint32_t b = i < nbad ? bad[i] : 0u;
   const FieldView F = geo.at(b);  // a plateau lies in one field
   const Grid& g = F.g;
@@ -2872,13 +2941,13 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
 hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
                    const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  const dim3 grid(blocks_for(nbad)), blk(kBlockThreads);
+  const dim3 grid(pp_tiny_parts(nbad)), blk(kBlockThreads);
   PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, badmap, t, y, anchor, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
 
-uint32_t pp_tiny_parts(uint32_t nbad) { return blocks_for(nbad); }
+uint32_t pp_tiny_parts(uint32_t nbad) { return xcd_grid(blocks_for(nbad)); }
 
 // mismatch map from a list: badmap must be zero; one byte store per listed point
 __global__ __launch_bounds__(kBlockThreads) void k_pp_badmap(const uint32_t* __restrict__ bad, uint32_t nbad,
@@ -2907,7 +2976,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(GEO geo, const uint
   __shared__ uint32_t kept[kRelistChunk];
   __shared__ uint32_t wcnt[kWavesPerBlock], sh_base, sh_n;
   const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
-  const uint32_t c0 = blockIdx.x * kRelistChunk;
+  const uint32_t c0 = xcd_chunk() * kRelistChunk;
   uint32_t n = 0;  // kept so far (uniform)
   for (uint32_t r = 0; r < kRelistChunk / kBlockThreads; r++) {
     const uint32_t i = c0 + r * kBlockThreads + tid;
@@ -2943,11 +3012,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(GEO geo, const uint
     if (base + k < cap) out[base + k] = kept[k];
 }
 
+uint32_t pp_compact_tiles(uint32_t n) { return (n + kCompactTile - 1) / kCompactTile; }
+
+hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
+                          uint32_t* misc, hipStream_t st) {
+  // the map is read in 16-byte words up to n rounded up to 16: its buffer must extend
+  // that far (the bytes past n are masked off)
+  if (reinterpret_cast<uintptr_t>(badmap) & 15u) return hipErrorInvalidValue;
+  const uint32_t tiles = pp_compact_tiles(n);
+  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 2 * (size_t)tiles, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pp_compact_map, dim3(tiles), dim3(kBlockThreads), 0, st,
+                     reinterpret_cast<const uint4*>(badmap), n, status, out, cap, misc);
+  return hipGetLastError();
+}
+
 hipError_t pp_relist(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* y, const uint8_t* t,
                      uint32_t* out, uint32_t cap, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  PP_GEO_LAUNCH(k_pp_relist, dim3((nbad + kRelistChunk - 1) / kRelistChunk), dim3(kBlockThreads), st, bad, nbad, y, t,
-                out, cap, misc);
+  PP_GEO_LAUNCH(k_pp_relist, dim3(xcd_grid((nbad + kRelistChunk - 1) / kRelistChunk)), dim3(kBlockThreads), st, bad,
+                nbad, y, t, out, cap, misc);
   return hipGetLastError();
 }
 
