@@ -28,7 +28,7 @@
 
 namespace tszp {
 
-constexpr int kChunkShift = 17;  // key offsets per chunk: kBinChunkWords * 32
+constexpr int kChunkShift = 19;  // key offsets per chunk: kBinChunkWords * 32
 constexpr int kUnroll = 4;       // keys per thread and loop step (independent loads in flight)
 constexpr int kVec = 2;          // uint4 key vectors per thread and loop step
 constexpr int kQueryUnroll = 8;  // critical points per thread of the rank query (random reads in flight)

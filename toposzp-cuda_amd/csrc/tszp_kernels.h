@@ -333,8 +333,8 @@ hipError_t launch_rank(const uint32_t* keys, const uint32_t* key_hi, const uint3
 // must be 16-byte aligned and readable up to the next multiple of 4 entries.
 constexpr uint32_t kBinMax = 4096;             // bins per call (more: radix path)
 constexpr uint32_t kBinSlices = 512;           // key slices (one workgroup each) for count and scatter
-constexpr uint32_t kBinChunkWords = 4096;      // bitmap words per workgroup
-constexpr uint32_t kBinMaxChunks = 256;        // chunks per bin (spans up to 2^25 keys: every bin but the zero cell at usual bounds)
+constexpr uint32_t kBinChunkWords = 16384;     // bitmap words per workgroup
+constexpr uint32_t kBinMaxChunks = 256;        // chunks per bin (spans up to 2^27 keys: every bin but the zero cell at usual bounds)
 constexpr uint64_t kBinWideBits = (uint64_t)kBinChunkWords * 32 * kBinMaxChunks;
 struct BinWork {
   uint32_t* mm;       // 4: min key, ~max key, min bin ^ 2^31, ~(max bin ^ 2^31)
