@@ -743,9 +743,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_verify(const uint32
 __global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_sum(const uint32_t* __restrict__ part, uint32_t n,
                                                                 uint32_t* __restrict__ misc, int idx) {
   __shared__ uint32_t red[kWavesPerBlock];
-  uint32_t v = 0;
-  for (uint32_t k = threadIdx.x; k < n; k += kBlockThreads) v += part[k];
-  v = seg_sum<kWave>(v);
+  // independent partial sums, so that several loads are in flight per thread
+  uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+  uint32_t k = threadIdx.x;
+  for (; k + 3 * kBlockThreads < n; k += 4 * kBlockThreads) {
+    v0 += part[k];
+    v1 += part[k + kBlockThreads];
+    v2 += part[k + 2 * kBlockThreads];
+    v3 += part[k + 3 * kBlockThreads];
+  }
+  for (; k < n; k += kBlockThreads) v0 += part[k];
+  uint32_t v = seg_sum<kWave>(v0 + v1 + v2 + v3);
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
