@@ -638,11 +638,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
   }
 }
 
-// The mismatch list in index order from the byte map (bytes 0 / 1, length a multiple
-// of 4): one tile of kCompactTile points per workgroup, tiles ordered by a decoupled
-// look-back; status: one zeroed word per tile.  The last tile stores the total in
+// The mismatch list in index order from the byte map (bytes 0 / 1): one tile of
+// kCompactTile points per workgroup, tiles ordered by a decoupled look-back; status:
+// one zeroed word per tile.  Lanes read consecutive 16-byte words (k-th word of the
+// tile: k * 256 + lane), so entries are ordered by (k, thread); each thread's four
+// counts are scanned two per 32-bit value.  The last tile stores the total in
 // misc[kPpBad]; entries past cap are dropped.
-constexpr uint32_t kCompactTile = 65536;
+constexpr uint32_t kCompactTile = 16384;
 __device__ __forceinline__ uint32_t bytes_below(uint32_t avail) {  // mask of the first min(avail, 4) bytes
   return avail >= 4u ? 0xffffffffu : (1u << (8u * avail)) - 1u;
 }
@@ -650,18 +652,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* _
                                                                  uint64_t* __restrict__ status,
                                                                  uint32_t* __restrict__ out, uint32_t cap,
                                                                  uint32_t* __restrict__ misc) {
-  constexpr int V = kCompactTile / 16 / kBlockThreads;  // 16-byte map words per thread (consecutive)
+  constexpr int V = kCompactTile / 16 / kBlockThreads;  // 16-byte map words per thread
+  static_assert(V == 4, "four counts, scanned two per 32-bit value");
   __shared__ uint32_t lds4[kWavesPerBlock];
   __shared__ uint32_t sh_excl;
   const uint32_t tile = blockIdx.x;
-  const uint32_t v0 = tile * (kCompactTile / 16) + threadIdx.x * V;
+  const uint32_t w0 = tile * (kCompactTile / 16) + threadIdx.x;
   const uint32_t n16 = (n + 15) / 16;
   uint4 m[V];
-  uint32_t c = 0;
+  uint32_t c[V];
 #pragma unroll
   for (int k = 0; k < V; k++) {
-    m[k] = v0 + k < n16 ? bm16[v0 + k] : make_uint4(0, 0, 0, 0);
-    const uint32_t first = 16u * (v0 + k);
+    const uint32_t wi = w0 + k * kBlockThreads;
+    m[k] = wi < n16 ? bm16[wi] : make_uint4(0, 0, 0, 0);
+    const uint32_t first = 16u * wi;
     if (first + 16u > n) {  // bytes at and past n are not part of the map
       const uint32_t avail = first < n ? n - first : 0u;
       m[k].x &= bytes_below(avail);
@@ -669,10 +673,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* _
       m[k].z &= bytes_below(avail > 8u ? avail - 8u : 0u);
       m[k].w &= bytes_below(avail > 12u ? avail - 12u : 0u);
     }
-    c += __popc(m[k].x) + __popc(m[k].y) + __popc(m[k].z) + __popc(m[k].w);
+    c[k] = __popc(m[k].x) + __popc(m[k].y) + __popc(m[k].z) + __popc(m[k].w);
   }
-  uint32_t tot;
-  uint32_t pos = block_excl_scan(c, lds4, &tot);
+  // counts of at most 16 per word and 4096 per tile: two 16-bit fields per scan value
+  uint32_t t01, t23;
+  const uint32_t e01 = block_excl_scan(c[0] | c[1] << 16, lds4, &t01);
+  const uint32_t e23 = block_excl_scan(c[2] | c[3] << 16, lds4, &t23);
+  const uint32_t T0 = t01 & 0xffffu, T1 = t01 >> 16, T2 = t23 & 0xffffu, T3 = t23 >> 16;
+  const uint32_t tot = T0 + T1 + T2 + T3;
   if (threadIdx.x < kWave) {
     const uint64_t ex = lookback(status, tile, tot, &misc[kPpErr]);
     if (threadIdx.x == 0) {
@@ -681,17 +689,21 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* _
     }
   }
   __syncthreads();
-  pos += sh_excl;
+  const uint32_t ex = sh_excl;
+  const uint32_t start[V] = {ex + (e01 & 0xffffu), ex + T0 + (e01 >> 16), ex + T0 + T1 + (e23 & 0xffffu),
+                             ex + T0 + T1 + T2 + (e23 >> 16)};
 #pragma unroll
   for (int k = 0; k < V; k++) {
+    uint32_t pos = start[k];
+    const uint32_t wi = w0 + k * kBlockThreads;
     const uint32_t w[4] = {m[k].x, m[k].y, m[k].z, m[k].w};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      uint32_t b = w[q];
-      while (b) {
-        const uint32_t bit = (uint32_t)__builtin_ctz(b);
-        b &= b - 1u;
-        if (pos < cap) out[pos] = 16u * (v0 + k) + 4u * q + bit / 8u;
+      uint32_t bits = w[q];
+      while (bits) {
+        const uint32_t bit = (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1u;
+        if (pos < cap) out[pos] = 16u * wi + 4u * q + bit / 8u;
         pos++;
       }
     }
