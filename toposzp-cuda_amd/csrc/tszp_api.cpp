@@ -51,7 +51,7 @@ enum Slot {
   kWsPpD, kWsPpBad, kWsPpBad2, kWsPpAct, kWsPpBits, kWsPpTBits, kWsPpTiles, kWsPpMisc, kWsPpSt,
   kWsBinMeta, kWsBinKey, kWsBinPay, kWsBinBits, kWsPpPart, kWsPpBadmap, kWsQuality,
   kWsQ, kWsMoves, kWsSaddleList, kWsSaddleMeta, kWsPpDone, kWsBatchTab, kWsBatchMap, kWsBatchRes, kWsBatchSt,
-  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsPpOrd, kWsCount
+  kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsPpOrd, kWsPpFflags, kWsCount
 };
 
 struct Ctx {
@@ -1749,14 +1749,19 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   auto mc_pass = [&](uint32_t budget, uint32_t first, uint32_t reps, bool guided, const char* tag) -> int {
     if (!exact && (r = full_mark(&nb, false))) return r;
     const uint32_t nl = nb < cap ? nb : cap;
-    if (guided && !blur) {  // the smoothed decoded field as the start order
+    if (guided) {  // the smoothed decoded field as the start order (a batch: the listed points' fields)
       float *s0, *s1;
       WS(kWsPpSm0, n * 4 + 16, s0);
       WS(kWsPpSm1, n * 4 + 16, s1);
+      uint32_t* fflags = nullptr;
+      if (geo.fields) {
+        WS(kWsPpFflags, (size_t)geo.nfields * 4 + 16, fflags);
+        HIPCHK(pp_field_flags(geo, bad, nl, fflags, st));
+      }
       const float* src = d;
       for (int a = 0; a < ndim; a++) {
         float* dst = a % 2 ? s1 : s0;
-        HIPCHK(pp_blur(geo, src, dst, a, st));
+        HIPCHK(pp_blur(geo, src, dst, a, fflags, st));
         src = dst;
       }
       blur = src;
@@ -1993,10 +1998,15 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     float *s0, *s1;
     WS(kWsPpSm0, n * 4 + 16, s0);
     WS(kWsPpSm1, n * 4 + 16, s1);
+    uint32_t* fflags = nullptr;
+    if (geo.fields) {  // a batch smooths only the fields of the listed points
+      WS(kWsPpFflags, (size_t)geo.nfields * 4 + 16, fflags);
+      HIPCHK(pp_field_flags(geo, bad, nl, fflags, st));
+    }
     const float* src = d;
     for (int a = 0; a < ndim; a++) {
       float* dst = a % 2 ? s1 : s0;
-      HIPCHK(pp_blur(geo, src, dst, a, st));
+      HIPCHK(pp_blur(geo, src, dst, a, fflags, st));
       src = dst;
     }
     FillRanges fr{};

@@ -443,8 +443,12 @@ hipError_t pp_init(const PpGeo& geo, const float* d, const uint8_t* t, const int
                    uint32_t* work, uint32_t* misc, hipStream_t st);
 // whole-plateau solves around the listed mismatches (one wavefront each); guide: the
 // field whose order is tried first (nullable); claim: zeroed bits, one per point
-// binomial smoothing of src along one axis into dst (radius 4, edges clamped)
-hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, hipStream_t st);
+// binomial smoothing of src along one axis into dst (radius 4, edges clamped); fflags
+// (batch, nullable): one word per field, only flagged fields are smoothed
+hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, const uint32_t* fflags,
+                   hipStream_t st);
+// a batch's field flags: 1 for the fields of the listed points, 0 for the others
+hipError_t pp_field_flags(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* flags, hipStream_t st);
 // full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
 // badmap (nullable) receives one byte per point, 1 where mismatched (where pp_mark_maps);
 // per_field: a batch's per-field counts into geo.fcount

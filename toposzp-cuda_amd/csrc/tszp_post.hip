@@ -193,26 +193,48 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_spmax_b(GeoB geo, const in
 }
 
 // Binomial smoothing along one axis (weights C(8, k) / 256, radius 4, edges clamped):
-// the smoothed decoded field is the first guide of the whole-plateau solve.
-template <int ND, class GEO>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_blur(GEO geo, const float* __restrict__ src,
-                                                          float* __restrict__ dst, uint32_t n, int axis) {
+// the smoothed decoded field is the first guide of the whole-plateau solve.  fflags
+// (batch, nullable): only fields whose flag is set are smoothed, chunk by chunk of the
+// field map (the others' dst is left as it was).
+template <int ND>
+__device__ __forceinline__ void blur_point(const FieldView& F, const float* __restrict__ src,
+                                           float* __restrict__ dst, uint32_t e, int axis) {
   constexpr float w[9] = {1.f / 256, 8.f / 256, 28.f / 256, 56.f / 256, 70.f / 256,
                           56.f / 256, 28.f / 256, 8.f / 256, 1.f / 256};
+  int c[ND];
+  coords_of<ND>(F.g, e - F.base, c);
+  const int dim = dim_of<ND>(F.g, axis), ca = c[axis];
+  const int64_t s = (int64_t)stride_of<ND>(F.g, axis);
+  float acc = 0.f;
+#pragma unroll
+  for (int k = -4; k <= 4; k++) {
+    const int q = min(max(ca + k, 0), dim - 1);
+    acc += w[k + 4] * src[(int64_t)e + (int64_t)(q - ca) * s];
+  }
+  dst[e] = acc;
+}
+
+template <int ND, class GEO>
+__global__ __launch_bounds__(kBlockThreads) void k_pp_blur(GEO geo, const float* __restrict__ src,
+                                                          float* __restrict__ dst, uint32_t n, int axis,
+                                                          const uint32_t* __restrict__ fflags) {
+  if constexpr (GEO::kBatch) {
+    if (fflags != nullptr) {
+      const uint32_t nch = (n + kFlatAlign - 1) / kFlatAlign;
+      for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const uint32_t e0 = ch * kFlatAlign;
+        const FieldView F = geo.at(e0);  // a chunk of the field map lies in one field
+        if (!fflags[F.fid]) continue;
+        const uint32_t end = min(e0 + kFlatAlign, F.base + F.n);
+        for (uint32_t e = e0 + threadIdx.x; e < end; e += kBlockThreads) blur_point<ND>(F, src, dst, e, axis);
+      }
+      return;
+    }
+  }
   for (uint32_t e = blockIdx.x * kBlockThreads + threadIdx.x; e < n; e += gridDim.x * kBlockThreads) {
     const FieldView F = geo.at(e);
     if (e >= F.base + F.n) continue;
-    int c[ND];
-    coords_of<ND>(F.g, e - F.base, c);
-    const int dim = dim_of<ND>(F.g, axis), ca = c[axis];
-    const int64_t s = (int64_t)stride_of<ND>(F.g, axis);
-    float acc = 0.f;
-#pragma unroll
-    for (int k = -4; k <= 4; k++) {
-      const int q = min(max(ca + k, 0), dim - 1);
-      acc += w[k + 4] * src[(int64_t)e + (int64_t)(q - ca) * s];
-    }
-    dst[e] = acc;
+    blur_point<ND>(F, src, dst, e, axis);
   }
 }
 
@@ -2851,8 +2873,26 @@ size_t pp_init_work_words(uint32_t n) {
   return nt + 1 + 2 * scan_status_words(nt) + 2;
 }
 
-hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, hipStream_t st) {
-  PP_GEO_LAUNCH(k_pp_blur, dim3(grid_stride_blocks(geo.n)), dim3(kBlockThreads), st, src, dst, geo.n, axis);
+hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, const uint32_t* fflags,
+                   hipStream_t st) {
+  PP_GEO_LAUNCH(k_pp_blur, dim3(grid_stride_blocks(geo.n)), dim3(kBlockThreads), st, src, dst, geo.n, axis,
+                geo.fields ? fflags : nullptr);
+  return hipGetLastError();
+}
+
+// flags[f] = 1 for every field f that holds a listed point (flags zeroed first)
+__global__ __launch_bounds__(kBlockThreads) void k_pp_field_flags(GeoB geo, const uint32_t* __restrict__ bad,
+                                                                 uint32_t nbad, uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i < nbad) flags[geo.at(bad[i]).fid] = 1u;
+}
+
+hipError_t pp_field_flags(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* flags, hipStream_t st) {
+  if (!geo.fields) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u32(flags, 0u, geo.nfields, st);
+  if (e != hipSuccess || !nbad) return e;
+  hipLaunchKernelGGL(k_pp_field_flags, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, geob(geo), bad, nbad,
+                     flags);
   return hipGetLastError();
 }
 
