@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_init(GEO geo, const float*
   // float4 groups of 4 points (the tile start is a multiple of 4); all loads of a batch
   // are issued before the values are computed
   constexpr int kGroups = kInitTile / 4 / kBlockThreads;  // 16
-  constexpr int kBatch = 4;
+  constexpr int kBatch = 2;
   const float4* d4 = reinterpret_cast<const float4*>(d);
   float4* y4 = reinterpret_cast<float4*>(y);
   const bool vec = ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(y)) & 15u) == 0;
