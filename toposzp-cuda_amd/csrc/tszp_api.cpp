@@ -1718,13 +1718,14 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
       HIPCHK(pp_badmap(bad, nb, badmap, st));
     }
     // a solved plateau's new values change no other point's type (its other neighbours
-    // lie in other cells), so re-classifying the listed points gives the exact list
+    // lie in other cells), so the listed points its solve left unmatched, and those of
+    // the plateaus not solved, are the exact list
     FillRanges fr{};
     fr.add(misc + kPpTiny, 0u, 1);
     fr.add(misc + kPpBad, 0u, 1);
     HIPCHK(launch_fill_multi(fr, st));
     HIPCHK(pp_tiny(geo, bad, nb, d, badmap, d_types, d_field, use_sp, part, misc, st));
-    HIPCHK(pp_relist(geo, bad, nb, d_field, d_types, bad2, cap, misc, st));
+    HIPCHK(pp_relist(bad, nb, badmap, bad2, cap, misc, st));
     if ((r = read_misc())) return r;
     std::swap(bad, bad2);
     if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);

@@ -1329,7 +1329,8 @@ __constant__ const int8_t kWeak2[3][2] = {{0, 0}, {0, 1}, {1, 0}};
 template <int ND, class GEO>
 __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&P)[3], int np,
                                                const float* __restrict__ d, const uint32_t (&rms)[3],
-                                               const uint8_t* __restrict__ t, float* __restrict__ y, int anchor) {
+                                               const uint8_t* __restrict__ t, float* __restrict__ y, int anchor,
+                                               uint32_t* matched) {
   constexpr int K = 2 * ND;
   bool done = false;
   const float db = d[P[0]];
@@ -1448,16 +1449,25 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
       if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
     done = true;
   }
+  // after a solve every point carries its type: the interior ones by the check above,
+  // the others (classified as regular) where that is their type
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < 3; j++) m |= (j < np && (intr[j] || tp[j] == 0)) ? 1u << j : 0u;
+  *matched = done ? m : 0u;
   return done;
 }
 
 // The plateau of every listed point, given up beyond three points; the thread of the
 // plateau's smallest listed point solves it.  Plateaus are disjoint and a solve reads
-// and writes only its own points' values, so no thread waits for another.
+// and writes only its own points' values, so no thread waits for another.  The listed
+// points a solve leaves matched get 2 in the mismatch map (still non-zero for the
+// ownership test of the other threads): cells of different decoded values do not
+// overlap, so no other listed point's type changes, and the re-listing keeps the 1s.
 template <int ND, class GEO>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
                                                           uint32_t nbad, const float* __restrict__ d,
-                                                          const uint8_t* __restrict__ badmap,
+                                                          uint8_t* __restrict__ badmap,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
                                                           int anchor, uint32_t* __restrict__ part) {
   constexpr int K = 2 * ND;
@@ -1498,7 +1508,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
   bool take = !big && np >= 2;
   if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
   if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
-  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor);
+  uint32_t matched = 0;
+  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor, &matched);
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if ((matched >> j) & 1u && badmap[P[j]]) badmap[P[j]] = 2;
   const uint32_t nd = (uint32_t)__popcll(__ballot(done));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nd;
   __syncthreads();
@@ -2998,7 +3012,7 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
   return hipGetLastError();
 }
 
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, uint8_t* badmap,
                    const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(pp_tiny_parts(nbad)), blk(kBlockThreads);
@@ -3022,15 +3036,14 @@ hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStr
   return hipGetLastError();
 }
 
-// The listed points whose types are still violated (a list without duplicates, so no
-// claim bits), compacted per workgroup: a workgroup takes kRelistChunk consecutive
-// entries, collects the kept ones in LDS in list order and appends them with one
-// atomic (same-address atomics serialise, so one per wave over millions of entries
-// would cost more than the classification).
+// The listed points whose map byte is still 1 (a list without duplicates, so no claim
+// bits), compacted per workgroup: a workgroup takes kRelistChunk consecutive entries,
+// collects the kept ones in LDS in list order and appends them with one atomic
+// (same-address atomics serialise, so one per wave over millions of entries would cost
+// more than the filter).
 constexpr uint32_t kRelistChunk = 4096;
-template <int ND, class GEO>
-__global__ __launch_bounds__(kBlockThreads) void k_pp_relist(GEO geo, const uint32_t* __restrict__ bad, uint32_t nbad,
-                                                            const float* __restrict__ y, const uint8_t* __restrict__ t,
+__global__ __launch_bounds__(kBlockThreads) void k_pp_relist(const uint32_t* __restrict__ bad, uint32_t nbad,
+                                                            const uint8_t* __restrict__ badmap,
                                                             uint32_t* __restrict__ out, uint32_t cap,
                                                             uint32_t* misc) {
   __shared__ uint32_t kept[kRelistChunk];
@@ -3044,10 +3057,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(GEO geo, const uint
     bool keep = false;
     if (i < nbad) {
       e = bad[i];
-      const FieldView F = geo.at(e);
-      int c[ND];
-      coords_of<ND>(F.g, e - F.base, c);
-      keep = classify_at<ND>(F.g, y, e, c) != (int)t[e];
+      keep = badmap[e] == 1;
     }
     const uint64_t m = __ballot(keep);
     if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
@@ -3087,11 +3097,11 @@ hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, u
   return hipGetLastError();
 }
 
-hipError_t pp_relist(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* y, const uint8_t* t,
-                     uint32_t* out, uint32_t cap, uint32_t* misc, hipStream_t st) {
+hipError_t pp_relist(const uint32_t* bad, uint32_t nbad, const uint8_t* badmap, uint32_t* out, uint32_t cap,
+                     uint32_t* misc, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  PP_GEO_LAUNCH(k_pp_relist, dim3(xcd_grid((nbad + kRelistChunk - 1) / kRelistChunk)), dim3(kBlockThreads), st, bad,
-                nbad, y, t, out, cap, misc);
+  hipLaunchKernelGGL(k_pp_relist, dim3(xcd_grid((nbad + kRelistChunk - 1) / kRelistChunk)), dim3(kBlockThreads), 0,
+                     st, bad, nbad, badmap, out, cap, misc);
   return hipGetLastError();
 }
 
