@@ -1719,15 +1719,14 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
     }
     // a solved plateau's new values change no other point's type (its other neighbours
     // lie in other cells), so the listed points its solve left unmatched, and those of
-    // the plateaus not solved, are the exact list
+    // the plateaus not solved, are the exact list (compacted from the sparse one)
     FillRanges fr{};
     fr.add(misc + kPpTiny, 0u, 1);
     fr.add(misc + kPpBad, 0u, 1);
     HIPCHK(launch_fill_multi(fr, st));
-    HIPCHK(pp_tiny(geo, bad, nb, d, badmap, d_types, d_field, use_sp, part, misc, st));
-    HIPCHK(pp_relist(bad, nb, badmap, bad2, cap, misc, st));
+    HIPCHK(pp_tiny(geo, bad, nb, d, badmap, d_types, d_field, use_sp, bad2, part, misc, st));
+    HIPCHK(pp_relist(bad2, nb, bad, cap, misc, st));
     if ((r = read_misc())) return r;
-    std::swap(bad, bad2);
     if (stats) fprintf(stderr, "[restore] small plateaus solved %u: %u -> %u mismatches\n", h[kPpTiny], nb, h[kPpBad]);
     nb = h[kPpBad];
     tiny = h[kPpTiny];

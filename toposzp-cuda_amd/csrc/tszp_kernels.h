@@ -463,10 +463,12 @@ hipError_t pp_move(const PpGeo& geo, const uint32_t* act, uint32_t max_act, uint
 hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st);
 // plateaus of two or three listed points solved exactly (relations of the decoded
-// values d computed on the fly); solved counts go to misc[kPpTiny]; badmap (1 at the
-// listed points) gets 2 at the listed points a solve left matched
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, uint8_t* badmap,
-                   const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st);
+// values d computed on the fly; badmap: non-zero at the listed points); solved counts
+// go to misc[kPpTiny]; keep[i] (nbad words): bad[i] if it stays mismatched, else
+// 0xffffffff -- after the solves, which change no other point's type, the exact list
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+                   const uint8_t* t, float* y, int anchor, uint32_t* keep, uint32_t* part, uint32_t* misc,
+                   hipStream_t st);
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
 // the mismatch list in index order from a byte map of n points (0 / 1 bytes, buffer
 // 16-byte aligned and readable up to n rounded up to 16); status: pp_compact_tiles(n)
@@ -476,11 +478,9 @@ hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, u
                           uint32_t* misc, hipStream_t st);
 // mismatch map (one byte per point) from a list; the map must be zeroed first
 hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
-// the listed points whose map byte is still 1, appended to out (count misc[kPpBad]);
-// after pp_tiny, whose solved plateaus change no other point's type, the exact
-// mismatch list
-hipError_t pp_relist(const uint32_t* bad, uint32_t nbad, const uint8_t* badmap, uint32_t* out, uint32_t cap,
-                     uint32_t* misc, hipStream_t st);
+// the entries of a sparse list (0xffffffff: empty) appended to out (count misc[kPpBad])
+hipError_t pp_relist(const uint32_t* sparse, uint32_t n, uint32_t* out, uint32_t cap, uint32_t* misc,
+                     hipStream_t st);
 void pp_tile_counts(const Grid& g, uint32_t* tx, uint32_t* ty, uint32_t* tz);
 hipError_t pp_tiles(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* pbits, uint32_t* tbits,
                     uint32_t* tiles, uint32_t cap, uint32_t* misc, hipStream_t st);

@@ -1330,7 +1330,7 @@ template <int ND, class GEO>
 __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&P)[3], int np,
                                                const float* __restrict__ d, const uint32_t (&rms)[3],
                                                const uint8_t* __restrict__ t, float* __restrict__ y, int anchor,
-                                               uint32_t* matched) {
+                                               bool write, uint32_t* matched) {
   constexpr int K = 2 * ND;
   bool done = false;
   const float db = d[P[0]];
@@ -1446,7 +1446,7 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
     if (!ok) continue;
 #pragma unroll
     for (int j = 0; j < 3; j++)
-      if (j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
+      if (write && j < np && !anc[j]) y[P[j]] = rk[j] == 0 ? fv[0] : (rk[j] == 1 ? fv[1] : fv[2]);
     done = true;
   }
   // after a solve every point carries its type: the interior ones by the check above,
@@ -1459,17 +1459,21 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
 }
 
 // The plateau of every listed point, given up beyond three points; the thread of the
-// plateau's smallest listed point solves it.  Plateaus are disjoint and a solve reads
-// and writes only its own points' values, so no thread waits for another.  The listed
-// points a solve leaves matched get 2 in the mismatch map (still non-zero for the
-// ownership test of the other threads): cells of different decoded values do not
-// overlap, so no other listed point's type changes, and the re-listing keeps the 1s.
+// plateau's smallest listed point solves it and writes the values.  Plateaus are
+// disjoint and a solve writes only its own points' values, so no thread waits for
+// another.  Every other listed point of the plateau runs the same (deterministic) solve
+// without writing, to learn whether its own point ends up matched: the values it reads
+// are the decoded ones, the types and the anchored points' values, which no solve
+// writes.  Cells of different decoded values do not overlap, so no other point's type
+// changes: keep[i] is the listed point if it stays mismatched, else kNoEntry.
+constexpr uint32_t kNoEntry = 0xffffffffu;  // an empty slot of a sparse list
 template <int ND, class GEO>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
                                                           uint32_t nbad, const float* __restrict__ d,
-                                                          uint8_t* __restrict__ badmap,
+                                                          const uint8_t* __restrict__ badmap,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
-                                                          int anchor, uint32_t* __restrict__ part) {
+                                                          int anchor, uint32_t* __restrict__ keep,
+                                                          uint32_t* __restrict__ part) {
   constexpr int K = 2 * ND;
   __shared__ uint32_t red[kWavesPerBlock];
   // every XCD takes one contiguous eighth of the list, in order: a list in index order
@@ -1505,15 +1509,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
     }
   }
   // the plateau goes to the thread of its smallest mismatched point, which solves it
-  bool take = !big && np >= 2;
+  const bool solvable = !big && np >= 2;
+  bool take = solvable;
   if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
   if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
   uint32_t matched = 0;
-  const bool done = take && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor, &matched);
-#pragma unroll
-  for (int j = 0; j < 3; j++)
-    if ((matched >> j) & 1u && badmap[P[j]]) badmap[P[j]] = 2;
-  const uint32_t nd = (uint32_t)__popcll(__ballot(done));
+  const bool done = solvable && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor, take, &matched);
+  if (i < nbad) keep[i] = done && (matched & 1u) ? kNoEntry : b;  // P[0] is the thread's point
+  const uint32_t nd = (uint32_t)__popcll(__ballot(done && take));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nd;
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
@@ -3012,11 +3015,12 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
   return hipGetLastError();
 }
 
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, uint8_t* badmap,
-                   const uint8_t* t, float* y, int anchor, uint32_t* part, uint32_t* misc, hipStream_t st) {
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+                   const uint8_t* t, float* y, int anchor, uint32_t* keep, uint32_t* part, uint32_t* misc,
+                   hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(pp_tiny_parts(nbad)), blk(kBlockThreads);
-  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, badmap, t, y, anchor, part);
+  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, badmap, t, y, anchor, keep, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
@@ -3036,14 +3040,12 @@ hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStr
   return hipGetLastError();
 }
 
-// The listed points whose map byte is still 1 (a list without duplicates, so no claim
-// bits), compacted per workgroup: a workgroup takes kRelistChunk consecutive entries,
-// collects the kept ones in LDS in list order and appends them with one atomic
-// (same-address atomics serialise, so one per wave over millions of entries would cost
-// more than the filter).
+// The entries of a sparse list (kNoEntry: empty), compacted per workgroup: a workgroup
+// takes kRelistChunk consecutive entries, collects the kept ones in LDS in list order
+// and appends them with one atomic (same-address atomics serialise, so one per wave
+// over millions of entries would cost more than the filter).
 constexpr uint32_t kRelistChunk = 4096;
 __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(const uint32_t* __restrict__ bad, uint32_t nbad,
-                                                            const uint8_t* __restrict__ badmap,
                                                             uint32_t* __restrict__ out, uint32_t cap,
                                                             uint32_t* misc) {
   __shared__ uint32_t kept[kRelistChunk];
@@ -3057,7 +3059,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(const uint32_t* __r
     bool keep = false;
     if (i < nbad) {
       e = bad[i];
-      keep = badmap[e] == 1;
+      keep = e != kNoEntry;
     }
     const uint64_t m = __ballot(keep);
     if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
@@ -3097,11 +3099,11 @@ hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, u
   return hipGetLastError();
 }
 
-hipError_t pp_relist(const uint32_t* bad, uint32_t nbad, const uint8_t* badmap, uint32_t* out, uint32_t cap,
-                     uint32_t* misc, hipStream_t st) {
-  if (!nbad) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_relist, dim3(xcd_grid((nbad + kRelistChunk - 1) / kRelistChunk)), dim3(kBlockThreads), 0,
-                     st, bad, nbad, badmap, out, cap, misc);
+hipError_t pp_relist(const uint32_t* sparse, uint32_t n, uint32_t* out, uint32_t cap, uint32_t* misc,
+                     hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_pp_relist, dim3(xcd_grid((n + kRelistChunk - 1) / kRelistChunk)), dim3(kBlockThreads), 0, st,
+                     sparse, n, out, cap, misc);
   return hipGetLastError();
 }
 
