@@ -484,15 +484,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_query(const uint32_t* __r
                                                             const uint32_t* __restrict__ ctot,
                                                             const uint2* __restrict__ gw, int32_t* __restrict__ out,
                                                             uint32_t out_cap) {
-  // consecutive lanes take consecutive critical points: their bitmap words are close
+  // consecutive lanes take consecutive critical points: their bitmap words are close.
+  // The records are read once (non-temporal: the L2 keeps bitmap words instead).
   constexpr int QU = kQueryUnroll;
   const uint32_t base = blockIdx.x * (kBlockThreads * QU) + threadIdx.x;
   uint32_t pv[QU], kv[QU];
 #pragma unroll
   for (int u = 0; u < QU; u++) {
     const uint32_t i = base + u * kBlockThreads;
-    pv[u] = i < P ? pay[i] : 0xffffffffu;
-    kv[u] = i < P ? key[i] : 0u;
+    pv[u] = i < P ? __builtin_nontemporal_load(&pay[i]) : 0xffffffffu;
+    kv[u] = i < P ? __builtin_nontemporal_load(&key[i]) : 0u;
   }
   uint4 m[QU];
   uint32_t off[QU];
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_bin_query(const uint32_t* __r
     // ctot holds, after k_bin_chunk_scan, the distinct keys of the bin's earlier chunks
     const uint32_t ci = off[u] >> kChunkShift;
     const uint32_t r = v[u].y + (uint32_t)__popc(v[u].x & ((1u << (off[u] & 31u)) - 1u)) + ctot[m[u].z + ci];
-    out[pv[u]] = (int32_t)r;
+    __builtin_nontemporal_store((int32_t)r, &out[pv[u]]);
   }
 }
 
