@@ -821,7 +821,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark_b(GeoB geo, const flo
   if (threadIdx.x == 0) lcnt = 0;
   __syncthreads();
   uint32_t miss = 0, spur = 0, chg = 0, dev = 0;
-  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  // 3D: every XCD sweeps one contiguous eighth of the chunks, so the plane neighbours
+  // stay in the L2 that loaded them (measured: 1.06 -> 0.78 ms for the 3D fields of the
+  // mixed 256-field batch; the same sweep made its 2D fields 2.5x slower, 0.25 -> 0.61 ms)
+  XcdSweep sw{blockIdx.x, nchunks, gridDim.x};
+  if constexpr (ND == 3) sw = xcd_sweep(nchunks, 1);
+  for (uint32_t ch = sw.begin; ch < sw.end; ch += sw.step) {
     const uint32_t c0 = ch * kMarkChunk;
     const FieldView F = geo.at(c0);
     const uint32_t end = F.base + F.n;
