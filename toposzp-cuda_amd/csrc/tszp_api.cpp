@@ -1674,10 +1674,10 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   uint32_t* part;
   WS(kWsPpPart, ((size_t)4 * pp_mark_parts() + pp_tiny_parts(cap) + 64) * 4, part);
   const bool vec_mark = pp_mark_maps(geo);
-  uint8_t* badmap = nullptr;
+  uint32_t* badmap = nullptr;  // mismatch bitmap of the first full pass
   uint64_t* ordst = nullptr;
   if (vec_mark && !geo.fields) WS(kWsPpOrd, (size_t)pp_compact_tiles(N) * 8 + 64, ordst);
-  auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint8_t* bmap = nullptr) -> int {
+  auto full_mark = [&](uint32_t* nb_out, bool final_pass, uint32_t* bmap = nullptr) -> int {
     FillRanges fr{};
     fr.add(misc + kPpBad, 0u, 1);
     fr.add(misc + kPpVerify, 0u, 8);
@@ -1698,7 +1698,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   };
   uint32_t nb = 0;
   int r;
-  WS(kWsPpBadmap, n + 16, badmap);
+  WS(kWsPpBadmap, (n / 32 + 64) * 4, badmap);  // (a batch pass writes whole 1024-point chunks)
   if (vec_mark) {
     if ((r = full_mark(&nb, false, badmap))) return r;
   } else {
@@ -1714,7 +1714,7 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   // plateaus of two or three points: exact in one pass; what stays listed is re-checked
   if (nb && nb <= cap) {
     if (!vec_mark) {  // the mismatch map from the list
-      HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(badmap), 0u, (n + 3) / 4, st));
+      HIPCHK(launch_fill_u32(badmap, 0u, (n + 31) / 32, st));
       HIPCHK(pp_badmap(bad, nb, badmap, st));
     }
     // a solved plateau's new values change no other point's type (its other neighbours

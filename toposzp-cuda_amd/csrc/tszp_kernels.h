@@ -450,10 +450,11 @@ hipError_t pp_blur(const PpGeo& geo, const float* src, float* dst, int axis, con
 // a batch's field flags: 1 for the fields of the listed points, 0 for the others
 hipError_t pp_field_flags(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, uint32_t* flags, hipStream_t st);
 // full marking pass; part: 4 * pp_mark_parts() words of per-workgroup partials;
-// badmap (nullable) receives one byte per point, 1 where mismatched (where pp_mark_maps);
+// bm (nullable) receives one bit per point (word e / 32, bit e % 32), 1 where mismatched
+// (where pp_mark_maps; a batch's padding between fields is left unwritten);
 // per_field: a batch's per-field counts into geo.fcount
 hipError_t pp_mark(const PpGeo& geo, const float* y, const uint8_t* t, const float* d, uint32_t* bad, uint32_t cap,
-                   uint32_t* part, uint8_t* badmap, bool per_field, uint32_t* misc, hipStream_t st);
+                   uint32_t* part, uint32_t* bm, bool per_field, uint32_t* misc, hipStream_t st);
 uint32_t pp_mark_parts();
 bool pp_mark_maps(const PpGeo& geo);
 hipError_t pp_active(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* t,
@@ -463,21 +464,21 @@ hipError_t pp_move(const PpGeo& geo, const uint32_t* act, uint32_t max_act, uint
 hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const float* y, const uint8_t* t,
                      uint32_t* bits, uint32_t* bad, uint32_t cap, int rings, uint32_t* misc, hipStream_t st);
 // plateaus of two or three listed points solved exactly (relations of the decoded
-// values d computed on the fly; badmap: non-zero at the listed points); solved counts
-// go to misc[kPpTiny]; keep[i] (nbad words): bad[i] if it stays mismatched, else
+// values d computed on the fly; bm: the mismatch bitmap, set at the listed points);
+// solved counts go to misc[kPpTiny]; keep[i] (nbad words): bad[i] if it stays mismatched, else
 // 0xffffffff -- after the solves, which change no other point's type, the exact list
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint32_t* bm,
                    const uint8_t* t, float* y, int anchor, uint32_t* keep, uint32_t* part, uint32_t* misc,
                    hipStream_t st);
 uint32_t pp_tiny_parts(uint32_t nbad);  // partial-count words pp_tiny needs
-// the mismatch list in index order from a byte map of n points (0 / 1 bytes, buffer
-// 16-byte aligned and readable up to n rounded up to 16); status: pp_compact_tiles(n)
+// the mismatch list in index order from the mismatch bitmap of n points (buffer 16-byte
+// aligned and readable up to n rounded up to 128 points); status: pp_compact_tiles(n)
 // words of 8 bytes; the count goes to misc[kPpBad]
 uint32_t pp_compact_tiles(uint32_t n);
-hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
+hipError_t pp_compact_map(const uint32_t* bm, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
                           uint32_t* misc, hipStream_t st);
-// mismatch map (one byte per point) from a list; the map must be zeroed first
-hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st);
+// mismatch bitmap (one bit per point) from a list; the bitmap must be zeroed first
+hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint32_t* bm, hipStream_t st);
 // the entries of a sparse list (0xffffffff: empty) appended to out (count misc[kPpBad])
 hipError_t pp_relist(const uint32_t* sparse, uint32_t n, uint32_t* out, uint32_t cap, uint32_t* misc,
                      hipStream_t st);

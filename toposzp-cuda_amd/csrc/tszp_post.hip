@@ -512,14 +512,15 @@ __device__ __forceinline__ void wave_stage_flush(const uint32_t* lw, uint32_t& w
 
 // Verification counts go to per-workgroup partials (part[4 * block + k]) that
 // k_pp_reduce_verify adds up: thousands of same-address atomics would serialise.
-// badmap4 (nullable) receives one byte per point, 1 where the point is mismatched.
+// bm (nullable) receives one bit per point (word e / 32, bit e % 32), 1 where the point
+// is mismatched.
 template <int ND>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float* __restrict__ y,
                                                            const uint8_t* __restrict__ t,
                                                            const float* __restrict__ d, uint32_t n4,
                                                            uint32_t* __restrict__ bad, uint32_t cap,
                                                            uint32_t* __restrict__ part,
-                                                           uint32_t* __restrict__ badmap4,
+                                                           uint32_t* __restrict__ bm,
                                                            uint32_t* __restrict__ misc) {
   constexpr int FA = ND - 1;  // fastest axis
   constexpr int U = kMarkGroups;
@@ -593,10 +594,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
         chg += (in[u] && tt != 0 && k != 0 && k != tt);
         flag[i] = in[u] && k != tt;
       }
-      if (badmap4 != nullptr && in[u])
-        __builtin_nontemporal_store(
-            (flag[0] ? 1u : 0u) | (flag[1] ? 1u : 0u) << 8 | (flag[2] ? 1u : 0u) << 16 | (flag[3] ? 1u : 0u) << 24,
-            &badmap4[b0 + u * kBlockThreads + threadIdx.x]);
+      if (bm != nullptr) {
+        // eight lanes' flags make one bit word (32 points; group b0 + ... is a multiple
+        // of 8 at lanes 0, 8, ...): lane l's four bits at 4 (l % 8)
+        uint32_t v = ((flag[0] ? 1u : 0u) | (flag[1] ? 2u : 0u) | (flag[2] ? 4u : 0u) | (flag[3] ? 8u : 0u))
+                     << (4 * (lane & 7));
+        v |= (uint32_t)__shfl_xor((int)v, 1, kWave);
+        v |= (uint32_t)__shfl_xor((int)v, 2, kWave);
+        v |= (uint32_t)__shfl_xor((int)v, 4, kWave);
+        const uint32_t gi = b0 + u * kBlockThreads + threadIdx.x;
+        if ((lane & 7) == 0 && in[u]) __builtin_nontemporal_store(v, &bm[gi >> 3]);
+      }
       if (d != nullptr && in[u]) {
         const float4 dd = ld_nt(&d4[b0 + u * kBlockThreads + threadIdx.x]);
         const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
@@ -660,49 +668,43 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark4(Grid g, const float*
   }
 }
 
-// The mismatch list in index order from the byte map (bytes 0 / 1): one tile of
-// kCompactTile points per workgroup, tiles ordered by a decoupled look-back; status:
-// one zeroed word per tile.  Lanes read consecutive 16-byte words (k-th word of the
-// tile: k * 256 + lane), so entries are ordered by (k, thread); each thread's four
-// counts are scanned two per 32-bit value.  The last tile stores the total in
+// The mismatch list in index order from the mismatch bitmap (bit e % 32 of word e / 32):
+// one tile of kCompactTile points per workgroup, each thread a contiguous run of
+// kCompactTile / 256 points (eight 16-byte words), tiles ordered by a decoupled
+// look-back; status: one zeroed word per tile.  The last tile stores the total in
 // misc[kPpBad]; entries past cap are dropped.
-constexpr uint32_t kCompactTile = 16384;
-__device__ __forceinline__ uint32_t bytes_below(uint32_t avail) {  // mask of the first min(avail, 4) bytes
-  return avail >= 4u ? 0xffffffffu : (1u << (8u * avail)) - 1u;
+constexpr uint32_t kCompactTile = 262144;
+__device__ __forceinline__ uint32_t bits_below(uint32_t avail) {  // mask of the first min(avail, 32) bits
+  return avail >= 32u ? 0xffffffffu : (1u << avail) - 1u;
 }
 __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* __restrict__ bm16, uint32_t n,
                                                                  uint64_t* __restrict__ status,
                                                                  uint32_t* __restrict__ out, uint32_t cap,
                                                                  uint32_t* __restrict__ misc) {
-  constexpr int V = kCompactTile / 16 / kBlockThreads;  // 16-byte map words per thread
-  static_assert(V == 4, "four counts, scanned two per 32-bit value");
+  constexpr int V = kCompactTile / 128 / kBlockThreads;  // 16-byte words (128 points each) per thread
   __shared__ uint32_t lds4[kWavesPerBlock];
   __shared__ uint32_t sh_excl;
   const uint32_t tile = blockIdx.x;
-  const uint32_t w0 = tile * (kCompactTile / 16) + threadIdx.x;
-  const uint32_t n16 = (n + 15) / 16;
+  const uint32_t v0 = (tile * kBlockThreads + threadIdx.x) * V;
+  const uint32_t n16 = (n + 127) / 128;
   uint4 m[V];
-  uint32_t c[V];
+  uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < V; k++) {
-    const uint32_t wi = w0 + k * kBlockThreads;
+    const uint32_t wi = v0 + k;
     m[k] = wi < n16 ? bm16[wi] : make_uint4(0, 0, 0, 0);
-    const uint32_t first = 16u * wi;
-    if (first + 16u > n) {  // bytes at and past n are not part of the map
+    const uint32_t first = 128u * wi;
+    if (first + 128u > n) {  // points at and past n are not part of the map
       const uint32_t avail = first < n ? n - first : 0u;
-      m[k].x &= bytes_below(avail);
-      m[k].y &= bytes_below(avail > 4u ? avail - 4u : 0u);
-      m[k].z &= bytes_below(avail > 8u ? avail - 8u : 0u);
-      m[k].w &= bytes_below(avail > 12u ? avail - 12u : 0u);
+      m[k].x &= bits_below(avail);
+      m[k].y &= bits_below(avail > 32u ? avail - 32u : 0u);
+      m[k].z &= bits_below(avail > 64u ? avail - 64u : 0u);
+      m[k].w &= bits_below(avail > 96u ? avail - 96u : 0u);
     }
-    c[k] = __popc(m[k].x) + __popc(m[k].y) + __popc(m[k].z) + __popc(m[k].w);
+    c += __popc(m[k].x) + __popc(m[k].y) + __popc(m[k].z) + __popc(m[k].w);
   }
-  // counts of at most 16 per word and 4096 per tile: two 16-bit fields per scan value
-  uint32_t t01, t23;
-  const uint32_t e01 = block_excl_scan(c[0] | c[1] << 16, lds4, &t01);
-  const uint32_t e23 = block_excl_scan(c[2] | c[3] << 16, lds4, &t23);
-  const uint32_t T0 = t01 & 0xffffu, T1 = t01 >> 16, T2 = t23 & 0xffffu, T3 = t23 >> 16;
-  const uint32_t tot = T0 + T1 + T2 + T3;
+  uint32_t tot;
+  uint32_t pos = block_excl_scan(c, lds4, &tot);
   if (threadIdx.x < kWave) {
     const uint64_t ex = lookback(status, tile, tot, &misc[kPpErr]);
     if (threadIdx.x == 0) {
@@ -711,13 +713,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* _
     }
   }
   __syncthreads();
-  const uint32_t ex = sh_excl;
-  const uint32_t start[V] = {ex + (e01 & 0xffffu), ex + T0 + (e01 >> 16), ex + T0 + T1 + (e23 & 0xffffu),
-                             ex + T0 + T1 + T2 + (e23 >> 16)};
+  pos += sh_excl;
 #pragma unroll
   for (int k = 0; k < V; k++) {
-    uint32_t pos = start[k];
-    const uint32_t wi = w0 + k * kBlockThreads;
     const uint32_t w[4] = {m[k].x, m[k].y, m[k].z, m[k].w};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -725,7 +723,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_compact_map(const uint4* _
       while (bits) {
         const uint32_t bit = (uint32_t)__builtin_ctz(bits);
         bits &= bits - 1u;
-        if (pos < cap) out[pos] = 16u * wi + 4u * q + bit / 8u;
+        if (pos < cap) out[pos] = 128u * (v0 + k) + 32u * q + bit;
         pos++;
       }
     }
@@ -801,7 +799,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_reduce_sum(const uint32_t*
 // Full marking pass over a batch of fields (any fastest-axis length): workgroups take
 // chunks of kMarkChunk points (a chunk lies in one field), list the mismatches through a
 // workgroup-staged LDS list (one global atomic per flush), write the mismatch map when
-// badmap is set, and leave per-workgroup verification partials (k_pp_reduce_verify).
+// bm (one bit per point) is set, and leave per-workgroup verification partials
+// (k_pp_reduce_verify).
 // With fcount, every field's own counts (missing, spurious, changed, max |y - d| bits)
 // are added up too, one atomic per chunk and count.
 constexpr uint32_t kMarkChunk = 4 * kBlockThreads;
@@ -810,7 +809,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark_b(GeoB geo, const flo
                                                             const uint8_t* __restrict__ t,
                                                             const float* __restrict__ d, uint32_t nchunks,
                                                             uint32_t* __restrict__ bad, uint32_t cap,
-                                                            uint32_t* __restrict__ part, uint8_t* __restrict__ badmap,
+                                                            uint32_t* __restrict__ part, uint32_t* __restrict__ bm,
                                                             unsigned long long* __restrict__ fcount,
                                                             uint32_t* __restrict__ misc) {
   constexpr uint32_t kStage = 4096;
@@ -832,6 +831,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark_b(GeoB geo, const flo
     const uint32_t end = F.base + F.n;
     if (c0 >= end) continue;  // padding after the field (uniform over the workgroup)
     uint32_t fm = 0, fs = 0, fc = 0, fd = 0;
+    bool flags[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint32_t e = c0 + j * kBlockThreads + threadIdx.x;
@@ -849,9 +849,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_mark_b(GeoB geo, const flo
           const uint32_t dv = __float_as_uint(fabsf(y[e] - d[e]));
           fd = dv > fd ? dv : fd;
         }
-        if (badmap != nullptr) badmap[e] = isbad ? 1 : 0;
       }
+      flags[j] = isbad;
       if (bad != nullptr) lds_append(isbad, e, lbuf, &lcnt);
+    }
+    if (bm != nullptr) {  // each wave's 64 points of a step are two bit words (c0: a multiple of 1024)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint64_t m = __ballot(flags[j]);
+        const uint32_t wd = (c0 + j * kBlockThreads + w * kWave) >> 5;
+        if (lane == 0) {
+          bm[wd] = (uint32_t)m;
+          bm[wd + 1] = (uint32_t)(m >> 32);
+        }
+      }
     }
     if (bad != nullptr) {
       __syncthreads();
@@ -1475,7 +1486,7 @@ constexpr uint32_t kNoEntry = 0xffffffffu;  // an empty slot of a sparse list
 template <int ND, class GEO>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
                                                           uint32_t nbad, const float* __restrict__ d,
-                                                          const uint8_t* __restrict__ badmap,
+                                                          const uint32_t* __restrict__ bm,
                                                           const uint8_t* __restrict__ t, float* __restrict__ y,
                                                           int anchor, uint32_t* __restrict__ keep,
                                                           uint32_t* __restrict__ part) {
@@ -1516,8 +1527,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32
   // the plateau goes to the thread of its smallest mismatched point, which solves it
   const bool solvable = !big && np >= 2;
   bool take = solvable;
-  if (take && np > 1 && P[1] < b && badmap[P[1]]) take = false;
-  if (take && np > 2 && P[2] < b && badmap[P[2]]) take = false;
+  if (take && np > 1 && P[1] < b && ((bm[P[1] >> 5] >> (P[1] & 31u)) & 1u)) take = false;
+  if (take && np > 2 && P[2] < b && ((bm[P[2] >> 5] >> (P[2] & 31u)) & 1u)) take = false;
   uint32_t matched = 0;
   const bool done = solvable && tiny_solve_one<ND>(geo, P, np, d, rms, t, y, anchor, take, &matched);
   if (i < nbad) keep[i] = done && (matched & 1u) ? kNoEntry : b;  // P[0] is the thread's point
@@ -2949,7 +2960,7 @@ uint32_t pp_mark_parts() { return kMarkBlocks; }
 bool pp_mark_maps(const PpGeo& geo) { return geo.fields || (geo.ndim == 2 ? geo.g.d1 : geo.g.d2) % 4 == 0; }
 
 hipError_t pp_mark(const PpGeo& geo, const float* y, const uint8_t* t, const float* d, uint32_t* bad, uint32_t cap,
-                   uint32_t* part, uint8_t* badmap, bool per_field, uint32_t* misc, hipStream_t st) {
+                   uint32_t* part, uint32_t* bm, bool per_field, uint32_t* misc, hipStream_t st) {
   const uint32_t n = geo.n;
   const dim3 blk(kBlockThreads);
   if (geo.fields) {
@@ -2961,28 +2972,26 @@ hipError_t pp_mark(const PpGeo& geo, const float* y, const uint8_t* t, const flo
     }
     if (geo.ndim == 2)
       hipLaunchKernelGGL(k_pp_mark_b<2>, dim3(kMarkBlocks), blk, 0, st, geob(geo), y, t, d, nchunks, bad, cap, part,
-                         badmap, fc, misc);
+                         bm, fc, misc);
     else
       hipLaunchKernelGGL(k_pp_mark_b<3>, dim3(kMarkBlocks), blk, 0, st, geob(geo), y, t, d, nchunks, bad, cap, part,
-                         badmap, fc, misc);
+                         bm, fc, misc);
     hipLaunchKernelGGL(k_pp_reduce_verify, dim3(1), blk, 0, st, part, kMarkBlocks, misc);
     return hipGetLastError();
   }
   const Grid& g = geo.g;
   const uint32_t fast = g.ndim == 2 ? g.d1 : g.d2;
   const bool vec = fast % 4 == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(badmap) & 3) == 0;
+                   (reinterpret_cast<uintptr_t>(t) & 3) == 0 && (reinterpret_cast<uintptr_t>(d) & 15) == 0;
   if (vec) {
     unsigned nb = kMarkBlocks;  // a multiple of the XCD count
     const dim3 grid(nb);
-    uint32_t* bm4 = reinterpret_cast<uint32_t*>(badmap);
-    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
-    else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm4, misc);
+    if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark4<2>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm, misc);
+    else hipLaunchKernelGGL(k_pp_mark4<3>, grid, blk, 0, st, g, y, t, d, n / 4, bad, cap, part, bm, misc);
     hipLaunchKernelGGL(k_pp_reduce_verify, dim3(1), blk, 0, st, part, nb, misc);
     return hipGetLastError();
   }
-  if (badmap) return hipErrorInvalidValue;  // the mismatch map needs the vector layout
+  if (bm) return hipErrorInvalidValue;  // the mismatch map needs the vector layout
   const dim3 grid(grid_stride_blocks(n));
   if (g.ndim == 2) hipLaunchKernelGGL(k_pp_mark<2>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
   else hipLaunchKernelGGL(k_pp_mark<3>, grid, blk, 0, st, g, y, t, d, n, bad, cap, misc);
@@ -3020,28 +3029,31 @@ hipError_t pp_remark(const PpGeo& geo, const uint32_t* old, uint32_t nold, const
   return hipGetLastError();
 }
 
-hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint8_t* badmap,
+hipError_t pp_tiny(const PpGeo& geo, const uint32_t* bad, uint32_t nbad, const float* d, const uint32_t* bm,
                    const uint8_t* t, float* y, int anchor, uint32_t* keep, uint32_t* part, uint32_t* misc,
                    hipStream_t st) {
   if (!nbad) return hipSuccess;
   const dim3 grid(pp_tiny_parts(nbad)), blk(kBlockThreads);
-  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, badmap, t, y, anchor, keep, part);
+  PP_GEO_LAUNCH(k_pp_tiny, grid, blk, st, bad, nbad, d, bm, t, y, anchor, keep, part);
   hipLaunchKernelGGL(k_pp_reduce_sum, dim3(1), blk, 0, st, part, grid.x, misc, (int)kPpTiny);
   return hipGetLastError();
 }
 
 uint32_t pp_tiny_parts(uint32_t nbad) { return xcd_grid(blocks_for(nbad)); }
 
-// mismatch map from a list: badmap must be zero; one byte store per listed point
+// mismatch bitmap from a list: bm must be zero; one bit per listed point
 __global__ __launch_bounds__(kBlockThreads) void k_pp_badmap(const uint32_t* __restrict__ bad, uint32_t nbad,
-                                                            uint8_t* __restrict__ badmap) {
+                                                            uint32_t* __restrict__ bm) {
   const uint32_t i = blockIdx.x * kBlockThreads + threadIdx.x;
-  if (i < nbad) badmap[bad[i]] = 1;
+  if (i < nbad) {
+    const uint32_t e = bad[i];
+    atomicOr(&bm[e >> 5], 1u << (e & 31u));
+  }
 }
 
-hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint8_t* badmap, hipStream_t st) {
+hipError_t pp_badmap(const uint32_t* bad, uint32_t nbad, uint32_t* bm, hipStream_t st) {
   if (!nbad) return hipSuccess;
-  hipLaunchKernelGGL(k_pp_badmap, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, bad, nbad, badmap);
+  hipLaunchKernelGGL(k_pp_badmap, dim3(blocks_for(nbad)), dim3(kBlockThreads), 0, st, bad, nbad, bm);
   return hipGetLastError();
 }
 
@@ -3091,16 +3103,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_pp_relist(const uint32_t* __r
 
 uint32_t pp_compact_tiles(uint32_t n) { return (n + kCompactTile - 1) / kCompactTile; }
 
-hipError_t pp_compact_map(const uint8_t* badmap, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
+hipError_t pp_compact_map(const uint32_t* bm, uint32_t n, uint64_t* status, uint32_t* out, uint32_t cap,
                           uint32_t* misc, hipStream_t st) {
-  // the map is read in 16-byte words up to n rounded up to 16: its buffer must extend
-  // that far (the bytes past n are masked off)
-  if (reinterpret_cast<uintptr_t>(badmap) & 15u) return hipErrorInvalidValue;
+  // the map is read in 16-byte words up to n rounded up to 128 points: its buffer must
+  // extend that far (the bits past n are masked off)
+  if (reinterpret_cast<uintptr_t>(bm) & 15u) return hipErrorInvalidValue;
   const uint32_t tiles = pp_compact_tiles(n);
   hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 2 * (size_t)tiles, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pp_compact_map, dim3(tiles), dim3(kBlockThreads), 0, st,
-                     reinterpret_cast<const uint4*>(badmap), n, status, out, cap, misc);
+  hipLaunchKernelGGL(k_pp_compact_map, dim3(tiles), dim3(kBlockThreads), 0, st, reinterpret_cast<const uint4*>(bm), n,
+                     status, out, cap, misc);
   return hipGetLastError();
 }
 
