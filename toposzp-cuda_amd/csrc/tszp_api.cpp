@@ -54,6 +54,8 @@ enum Slot {
   kWsPpFields, kWsPpSm0, kWsPpSm1, kWsPpEsc, kWsPpStage, kWsPpCtl, kWsElemEnds, kWsPpOrd, kWsPpFflags, kWsCount
 };
 
+constexpr size_t kHostSmall = 512;  // bytes of a context's pinned read buffer
+
 struct Ctx {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -62,6 +64,7 @@ struct Ctx {
   size_t cap[kWsCount] = {};
   CompressResult* host_res = nullptr;  // pinned
   uint32_t* host_err = nullptr;        // pinned: a device error word read at a later synchronisation
+  uint32_t* host_small = nullptr;      // pinned: small device reads (counters, error words)
   void* host_batch = nullptr;          // pinned: batch tables and results
   size_t host_batch_cap = 0;
 
@@ -191,6 +194,24 @@ int get_ctx(Ctx** out) {
     g_ctx[dev] = c;
   }
   *out = g_ctx[dev];
+  return TSZP_OK;
+}
+
+// A small device read (counters, error words) through the context's pinned buffer: a
+// pageable target makes the copy stage through a bounce buffer (measured: the
+// restoration's dozen counter reads per call took 0.1 ms longer that way).
+int read_small(Ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes > kHostSmall) return fail(TSZP_EINVAL, "read_small: %zu bytes", bytes);
+  if (!c->host_small) {
+    const hipError_t e = hipHostMalloc((void**)&c->host_small, kHostSmall, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->host_small = nullptr;
+      return fail(TSZP_EHIP, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->host_small, src, bytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  memcpy(dst, c->host_small, bytes);
   return TSZP_OK;
 }
 
@@ -353,8 +374,7 @@ int sort_and_rank(Ctx* c, uint32_t* keys, uint32_t* hi, uint32_t* pay, uint32_t 
   HIPCHK(launch_rank(keys, hi, pay, count, eb, inv, out, out_cap, w, st));
   pr.stop();
   uint32_t herr = 0;
-  HIPCHK(hipMemcpyAsync(&herr, w.err, 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rr = read_small(c, &herr, w.err, 4, st)) return rr;
   if (herr) return fail(TSZP_EDEVICE, "sort: device error bits 0x%x", herr);
   return TSZP_OK;
 }
@@ -385,8 +405,7 @@ int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32
   uint32_t h[16];
   ProfScope ps(kProfSort, st);
   HIPCHK(bin_range(keys, count, eb, inv, w, st));
-  HIPCHK(hipMemcpyAsync(h, w.mm, 16, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rr = read_small(c, h, w.mm, 16, st)) return rr;
   const int64_t bmin = (int64_t)(int32_t)(h[2] ^ 0x80000000u), bmax = (int64_t)(int32_t)(~h[3] ^ 0x80000000u);
   if (bmax < bmin || bmax - bmin + 1 > (int64_t)kBinMax) return TSZP_OK;
   const uint32_t B = (uint32_t)(bmax - bmin + 1);
@@ -394,8 +413,7 @@ int bin_sort_positions(Ctx* c, const uint32_t* keys, const uint32_t* pay, uint32
   bin_slices(count, &nslice, &slice);
   HIPCHK(bin_count(keys, count, eb, inv, (int)bmin, B, w, st));
   HIPCHK(bin_plan(B, nslice, w, st));
-  HIPCHK(hipMemcpyAsync(h, w.info, 64, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rr = read_small(c, h, w.info, 64, st)) return rr;
   if (h[8]) return fail(TSZP_EDEVICE, "sort positions: device error bits 0x%x", h[8]);
   if (h[6] != count) return TSZP_OK;  // some key outside [bmin, bmax]: bins not monotone
   *done = true;
@@ -570,8 +588,7 @@ int decode(Ctx* c, int mode, const uint8_t* payload, const uint32_t* blk, size_t
   HIPCHK(launch_decode(mode, a, st));
   prof.stop();
   uint32_t herr = 0;
-  HIPCHK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rr = read_small(c, &herr, err, 4, st)) return rr;
   if (herr) return fail(TSZP_EDEVICE, "decode: device error bits 0x%x", herr);
   return TSZP_OK;
 }
@@ -1658,11 +1675,8 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   WS(kWsPpMisc, kPpMiscWords * 4, misc);
   WS(kWsPpSt, pp_init_work_words(N) * 4 + 64, status);
   uint32_t h[kPpMiscWords];
-  auto read_misc = [&]() -> int {
-    HIPCHK(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return TSZP_OK;
-  };
+  static_assert(sizeof(h) <= kHostSmall, "the counters fit the pinned read buffer");
+  auto read_misc = [&]() -> int { return read_small(c, h, misc, sizeof(h), st); };
   HIPCHK(launch_fill_u32(misc, 0u, kPpMiscWords, st));
   if (use_sp) HIPCHK(pp_spmax(geo, d_sp, (uint32_t)n_ext_one, misc, st));
   HIPCHK(pp_init(geo, d, d_types, use_sp ? d_sp : nullptr, (uint32_t)n_ext_one, d_field,
@@ -2178,6 +2192,8 @@ int tszp_workspace_destroy(tszp_workspace* ws) {
   int r = TSZP_OK;
   for (int k = 0; k < kWsCount; k++)
     if (ws->c.ptr[k] && hipFree(ws->c.ptr[k]) != hipSuccess) r = fail(TSZP_EHIP, "workspace_destroy: hipFree failed");
+  if (ws->c.host_small && hipHostFree(ws->c.host_small) != hipSuccess)
+    r = fail(TSZP_EHIP, "workspace_destroy: hipHostFree failed");
   delete ws;
   return r;
 }
