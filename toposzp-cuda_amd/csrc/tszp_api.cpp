@@ -1530,8 +1530,7 @@ int topo64_impl(Ctx* c, const double* d_field, const uint32_t* dims, size_t n, d
   if (r) return r;
   HIPCHK(launch_write_header(d_out, 0, 1, 0, cdiv(n, bs), d_blk, 0, 0.f, st));
   uint32_t hm[2] = {0u, 0u};
-  HIPCHK(hipMemcpyAsync(hm, meta, 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rr = read_small(c, hm, meta, 8, st)) return rr;
   if (hm[1]) return fail(TSZP_EDEVICE, "topo_compress_f64_3d: saddle list scan failed");
   if (moved) *moved = hm[0];
   *out_bytes = 8 + res.payload_bytes;
