@@ -1866,9 +1866,12 @@ int restore_run(Ctx* c, const PpGeo& geo, const float* d, const uint8_t* d_types
   const bool stop_mc = getenv("TSZP_RESTORE_STOP_MC") != nullptr;  // diagnostics: nothing after the plateau solves
   auto window_rounds = [&](int win_mode) -> int {
   for (int outer = 0; outer < 4 && nb && !stop_mc; outer++) {
-    // local search, re-checking only around the previous mismatches
+    // local search, re-checking only around the previous mismatches; a handful of
+    // mismatches goes straight to the window solves (a local round costs about as much
+    // as solving them; what the windows leave gets the local rounds of the next pass)
     bool complete = nb <= cap;
-    for (int round = 0; round < max_rounds && nb; round++) {
+    const int local_rounds = outer == 0 && exact && nb <= kPpLocalFew ? 0 : max_rounds;
+    for (int round = 0; round < local_rounds && nb; round++) {
       const uint32_t nl = nb < cap ? nb : cap;
       const uint32_t nb_before = nb;
       // a closed neighbourhood holds at most one point of each colour class, so every

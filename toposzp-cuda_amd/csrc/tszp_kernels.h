@@ -433,6 +433,7 @@ enum : int {
 };
 // mismatches left after a local round below which the window solve runs at once
 constexpr uint32_t kPpSolveFew = 16384;
+constexpr uint32_t kPpLocalFew = 64;  // listed mismatches solved by windows without a local round first
 uint32_t pp_init_tiles(uint32_t n);
 // largest sort position: one field into misc[kPpSpMax], a batch into geo.spmax
 hipError_t pp_spmax(const PpGeo& geo, const int32_t* sp, uint32_t m, uint32_t* misc, hipStream_t st);
