@@ -1478,10 +1478,11 @@ __device__ __forceinline__ bool tiny_solve_one(const GEO& geo, const uint32_t (&
 // plateau's smallest listed point solves it and writes the values.  Plateaus are
 // disjoint and a solve writes only its own points' values, so no thread waits for
 // another.  Every other listed point of the plateau runs the same (deterministic) solve
-// without writing, to learn whether its own point ends up matched: the values it reads
-// are the decoded ones, the types and the anchored points' values, which no solve
-// writes.  Cells of different decoded values do not overlap, so no other point's type
-// changes: keep[i] is the listed point if it stays mismatched, else kNoEntry.
+// without writing, to learn whether its own point ends up matched: its result depends
+// on the decoded values, the types and the anchored points' values, none of which a
+// solve writes (the other points' values it loads go unused).  Cells of different
+// decoded values do not overlap, so no other point's type changes: keep[i] is the
+// listed point if it stays mismatched, else kNoEntry.
 constexpr uint32_t kNoEntry = 0xffffffffu;  // an empty slot of a sparse list
 template <int ND, class GEO>
 __global__ __launch_bounds__(kBlockThreads) void k_pp_tiny(GEO geo, const uint32_t* __restrict__ bad,
